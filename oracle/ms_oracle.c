@@ -1,0 +1,929 @@
+/*
+ * ms_oracle.c — FP64 CPU restatement of match-series' registration hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see ms_oracle.h). It restates the reference's algorithm operation by
+ * operation, in the reference's loop and summation order, so that it can serve as the checker of
+ * the GPU kernels and as the timed single-thread CPU baseline. References are to QuocMesh
+ * (quocmesh/modules/{aol,quoc}/ and projects/electronMicroscopy/).
+ *
+ * Build: make -C oracle lib   (gcc -std=c99 -O2 -ffp-contract=off)
+ */
+#include "ms_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define IDX(x, y, N) ((x) + (y) * (N))
+
+/* 2-point Gauss rule on [0,1], tensorised; quad point k = (p[k/2], p[k%2]), weight 1/4
+ * (baseFunctionSet.h:1535-1554 for the points, 1436-1467 for the ordering and weights). */
+static const double QP[2] = {0.2113248654051871, 0.7886751345948129};
+static const double QW = 0.25;
+
+static double gridH(int N) { return 1.0 / (double)(N - 1); } /* rectangularGrid.h:50-57 */
+
+/* Q1 basis on the unit square, local dofs (0,0),(1,0),(0,1),(1,1) (baseFunctionSet.h:127-193). */
+static double basis(int b, double x, double y) {
+  switch (b) {
+    case 0: return (1.0 - x) * (1.0 - y);
+    case 1: return x * (1.0 - y);
+    case 2: return (1.0 - x) * y;
+    default: return x * y;
+  }
+}
+static double dxBasis(int b, double x, double y) {
+  (void)x;
+  switch (b) {
+    case 0: return y - 1.0;
+    case 1: return 1.0 - y;
+    case 2: return -y;
+    default: return y;
+  }
+}
+static double dyBasis(int b, double x, double y) {
+  (void)y;
+  switch (b) {
+    case 0: return x - 1.0;
+    case 1: return -x;
+    case 2: return 1.0 - x;
+    default: return x;
+  }
+}
+/* global index of local dof b of element (ex, ey) (configurators.h:880-890) */
+static int l2g(int ex, int ey, int b, int N) { return (ex + ((b & 1) != 0)) + (ey + ((b & 2) != 0)) * N; }
+
+/* cached basis values at the quad points (BaseFunctionSetInterface quad cache) */
+static double BQ[4][4]; /* [b][q] */
+static int bqInit = 0;
+static void initBQ(void) {
+  if (bqInit) return;
+  for (int b = 0; b < 4; ++b)
+    for (int q = 0; q < 4; ++q) BQ[b][q] = basis(b, QP[q / 2], QP[q % 2]);
+  bqInit = 1;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Grid transfer
+ * ------------------------------------------------------------------------------------------- */
+
+/* CellCenteredRestrictOp::apply (cellCenteredGrid.h:165-180): 2x2 sum, x fastest, times 1/4. */
+void msor_restrict(const double *fine, int N, double *coarse) {
+  const int n = N / 2;
+  const double scaling = 1.0 / 4.0;
+  for (int cy = 0; cy < n; ++cy)
+    for (int cx = 0; cx < n; ++cx) {
+      double v = 0;
+      v += fine[IDX(2 * cx, 2 * cy, N)];
+      v += fine[IDX(2 * cx + 1, 2 * cy, N)];
+      v += fine[IDX(2 * cx, 2 * cy + 1, N)];
+      v += fine[IDX(2 * cx + 1, 2 * cy + 1, N)];
+      coarse[IDX(cx, cy, n)] = scaling * v;
+    }
+}
+
+/* CellCenteredProlongOp::apply (cellCenteredGrid.h:188-210): piecewise-constant injection. */
+void msor_prolong(const double *coarse, int n, double *fine) {
+  const int N = 2 * n;
+  for (int cy = 0; cy < n; ++cy)
+    for (int cx = 0; cx < n; ++cx) {
+      const double v = coarse[IDX(cx, cy, n)];
+      fine[IDX(2 * cx, 2 * cy, N)] = v;
+      fine[IDX(2 * cx + 1, 2 * cy, N)] = v;
+      fine[IDX(2 * cx, 2 * cy + 1, N)] = v;
+      fine[IDX(2 * cx + 1, 2 * cy + 1, N)] = v;
+    }
+}
+
+/* ScalarArray<QC_2D>::interpolate (scalarArray.h:499-529), including the Y == 0 branch. */
+double msor_interpolate(const double *img, int N, double X, double Y) {
+  int xL = (int)X, yL = (int)Y;
+  if (X >= (double)(N - 1)) xL = N - 2;
+  if (Y >= (double)(N - 1)) yL = N - 2;
+  if (X < 0.) xL = 0;
+  if (Y < 0.) yL = 0;
+  X -= xL;
+  Y -= yL;
+  const int xU = xL + 1, yU = yL + 1;
+  const double v0 = img[IDX(xL, yL, N)], v1 = img[IDX(xU, yL, N)];
+  const double v2 = img[IDX(xL, yU, N)], v3 = img[IDX(xU, yU, N)];
+  if (Y == 0) return (1 - X) * v0 + X * v1;
+  return (1 - X) * (1 - Y) * v0 + X * (1 - Y) * v1 + (1 - X) * Y * v2 + X * Y * v3;
+}
+
+/* ScalarArray<QC_2D>::resampleFrom via ResampleCopier (scalarArray.h:308-331,1052-1060): the step
+ * (n-1)/(N-1) is held in long double; the scaled coordinate is rounded to double. */
+void msor_resample(const double *src, int n, double *dst, int N) {
+  const long double hx = (long double)(n - 1) / (N - 1);
+  const long double hy = (long double)(n - 1) / (N - 1);
+  for (int y = 0; y < N; ++y)
+    for (int x = 0; x < N; ++x) {
+      const double sx = (double)(x * hx), sy = (double)(y * hy);
+      dst[IDX(x, y, N)] = msor_interpolate(src, n, sx, sy);
+    }
+}
+
+static double rintd(double v) { return floor(v + 0.5); } /* aol::Rint */
+
+/* DeformImage with ExtendWithConstant == true (deformations.h:62-101). */
+void msor_deform_image(const double *img, const double *phi, int N, double ext, int nearest, double *out) {
+  const double h = gridH(N);
+  const int NN = N * N;
+  for (int y = 0; y < N; ++y)
+    for (int x = 0; x < N; ++x) {
+      const int i = IDX(x, y, N);
+      double ds[2];
+      int inside = 1;
+      ds[0] = x + phi[i] / h;
+      ds[1] = y + phi[NN + i] / h;
+      for (int c = 0; c < 2; ++c) {
+        if (ds[c] < 0.0 || ds[c] > (double)(N - 1) || isnan(ds[c])) inside = 0;
+        if (nearest) ds[c] = rintd(ds[c]);
+      }
+      out[i] = inside ? msor_interpolate(img, N, ds[0], ds[1]) : ext;
+    }
+}
+
+/* doFastDeformImage<QC_2D> (deformations.h:140-162): clamp to the domain, then interpolate. */
+void msor_fast_deform_image(const double *img, const double *phi, int N, double *out) {
+  const double h = gridH(N);
+  const int NN = N * N;
+  for (int y = 0; y < N; ++y)
+    for (int x = 0; x < N; ++x) {
+      const int i = IDX(x, y, N);
+      double dx = x + phi[i] / h, dy = y + phi[NN + i] / h;
+      const double hi = (double)(N - 1);
+      dx = dx < 0 ? 0 : (dx > hi ? hi : dx);
+      dy = dy < 0 ? 0 : (dy > hi ? hi : dy);
+      out[i] = msor_interpolate(img, N, dx, dy);
+    }
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * SSD energy and its variation
+ * ------------------------------------------------------------------------------------------- */
+
+/* DiscreteFunctionDefault::evaluateAtQuadPoint (discreteFunction.h:239-246) */
+static double evalAtQP(const double *dofs, int ex, int ey, int q, int N) {
+  double w = 0.;
+  for (int b = 0; b < 4; ++b) w += dofs[l2g(ex, ey, b, N)] * BQ[b][q];
+  return w;
+}
+
+/* transformAndClipCoord with the per-axis flag (deformations.h:795-821) followed by
+ * getLocalCoordsRegularRectangularGrid (configurators.h:32-50). */
+static void transformAndClip(int ex, int ey, int q, double offx, double offy, int N, int *tel, double *tloc,
+                             int *within) {
+  const double h = gridH(N);
+  double coord[2];
+  coord[0] = ((double)ex + QP[q / 2]) * h;
+  coord[1] = ((double)ey + QP[q % 2]) * h;
+  coord[0] += offx;
+  coord[1] += offy;
+  for (int c = 0; c < 2; ++c) {
+    const int width = (int)((N - 1) * h); /* = 1 for every dyadic N */
+    within[c] = 1;
+    if (coord[c] < 0.) {
+      coord[c] = 0.;
+      within[c] = 0;
+    } else if (coord[c] >= width) {
+      coord[c] = width - DBL_EPSILON;
+      within[c] = 0;
+    }
+  }
+  for (int c = 0; c < 2; ++c) {
+    const double sc = coord[c] / h;
+    int el = (short)sc;
+    double loc = sc - el;
+    if (el == N - 1 && fabs(loc - 0.0) < 1e-15) {
+      el = N - 2;
+      loc = 1;
+    }
+    tel[c] = el;
+    tloc[c] = loc;
+  }
+}
+
+/* DiscreteFunctionDefault::evaluate (discreteFunction.h:229-236) */
+static double evalAt(const double *dofs, const int *el, const double *loc, int N) {
+  double w = 0.;
+  for (int b = 0; b < 4; ++b) w += dofs[l2g(el[0], el[1], b, N)] * basis(b, loc[0], loc[1]);
+  return w;
+}
+
+/* DiscreteFunctionDefault::evaluateGradient (discreteFunction.h:259-268) */
+static void evalGradAt(const double *dofs, const int *el, const double *loc, int N, double *g) {
+  const double h = gridH(N);
+  g[0] = 0;
+  g[1] = 0;
+  for (int b = 0; b < 4; ++b) {
+    const double d = dofs[l2g(el[0], el[1], b, N)];
+    double v0 = dxBasis(b, loc[0], loc[1]) / h, v1 = dyBasis(b, loc[0], loc[1]) / h;
+    v0 *= d;
+    v1 *= d;
+    g[0] += v0;
+    g[1] += v1;
+  }
+}
+
+/* E_SSD = sum_El h^2 sum_q 1/4 * 1/2 (T(x_q + phi(x_q)) - R(x_q))^2
+ * (SSDEnergy::evaluateIntegrand registration.h:1607-1615 in FENonlinIntegrationVectorInterface::applyAdd
+ * FEOpInterface.h:2586-2613; elements x-fastest, rectangularGrid.h:139-150). */
+double msor_ssd_energy(const double *R, const double *T, const double *phi, int N) {
+  initBQ();
+  const int NN = N * N;
+  const double h = gridH(N), vol = h * h;
+  double res = 0.;
+  for (int ey = 0; ey < N - 1; ++ey)
+    for (int ex = 0; ex < N - 1; ++ex) {
+      double a = 0.;
+      for (int q = 0; q < 4; ++q) {
+        const double ox = evalAtQP(phi, ex, ey, q, N), oy = evalAtQP(phi + NN, ex, ey, q, N);
+        int tel[2], within[2];
+        double tloc[2];
+        transformAndClip(ex, ey, q, ox, oy, N, tel, tloc, within);
+        const double t = evalAt(T, tel, tloc, N) - evalAtQP(R, ex, ey, q, N);
+        a += 0.5 * (t * t) * QW;
+      }
+      a *= vol;
+      res += a;
+    }
+  return 0. + res;
+}
+
+/* SSDForce::getNonlinearity (registration.h:1646-1666) assembled by
+ * FENonlinVectorOpInterface::applyAdd (FEOpInterface.h:1803-1853). */
+void msor_ssd_force(const double *R, const double *T, const double *phi, int N, double *out) {
+  initBQ();
+  const int NN = N * N;
+  const double h = gridH(N), vol = h * h;
+  memset(out, 0, sizeof(double) * 2 * NN);
+  for (int ey = 0; ey < N - 1; ++ey)
+    for (int ex = 0; ex < N - 1; ++ex) {
+      double nl[4][2];
+      for (int q = 0; q < 4; ++q) {
+        const double ox = evalAtQP(phi, ex, ey, q, N), oy = evalAtQP(phi + NN, ex, ey, q, N);
+        int tel[2], within[2];
+        double tloc[2];
+        transformAndClip(ex, ey, q, ox, oy, N, tel, tloc, within);
+        const double td = evalAt(T, tel, tloc, N);
+        const double r = evalAtQP(R, ex, ey, q, N);
+        double g[2];
+        evalGradAt(T, tel, tloc, N, g);
+        for (int c = 0; c < 2; ++c) nl[q][c] = within[c] ? g[c] * (td - r) : 0;
+      }
+      for (int dof = 0; dof < 4; ++dof) {
+        double a0 = 0, a1 = 0;
+        for (int q = 0; q < 4; ++q) {
+          const double f = BQ[dof][q] * QW;
+          a0 += f * nl[q][0];
+          a1 += f * nl[q][1];
+        }
+        const int gi = l2g(ex, ey, dof, N);
+        out[gi] += vol * a0;
+        out[NN + gi] += vol * a1;
+      }
+    }
+}
+
+/* Assembled Q1 stiffness matrix applied to one component (StiffOp, FEOpInterface.h:2333-2348 with
+ * the local matrix of FEOpInterface.h:1362-1398; MultilinFEBandMatrix<2D>). On the unit square
+ * element the local matrix is (1/6)[4 -1 -1 -2; -1 4 -2 -1; -1 -2 4 -1; -2 -1 -1 4] in the dof
+ * order (0,0),(1,0),(0,1),(1,1); the interior 9-point stencil is centre 8/3, neighbours -1/3.
+ * Implemented row by row as the assembled band matrix: each row sums the element contributions. */
+void msor_stiff_apply(const double *u, int N, double *out) {
+  static const double KL[4][4] = {{4. / 6, -1. / 6, -1. / 6, -2. / 6},
+                                  {-1. / 6, 4. / 6, -2. / 6, -1. / 6},
+                                  {-1. / 6, -2. / 6, 4. / 6, -1. / 6},
+                                  {-2. / 6, -1. / 6, -1. / 6, 4. / 6}};
+  for (int y = 0; y < N; ++y)
+    for (int x = 0; x < N; ++x) {
+      /* assemble the row of node (x,y): coefficient of each of the 9 neighbours */
+      double c[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int ey = y - 1; ey <= y; ++ey)
+        for (int ex = x - 1; ex <= x; ++ex) {
+          if (ex < 0 || ey < 0 || ex >= N - 1 || ey >= N - 1) continue;
+          const int bi = (x - ex) + 2 * (y - ey);
+          for (int bj = 0; bj < 4; ++bj) {
+            const int jx = ex + (bj & 1), jy = ey + ((bj & 2) >> 1);
+            c[jy - y + 1][jx - x + 1] += KL[bi][bj];
+          }
+        }
+      double s = 0;
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int jx = x + dx, jy = y + dy;
+          if (jx < 0 || jy < 0 || jx >= N || jy >= N) continue;
+          s += c[dy + 1][dx + 1] * u[IDX(jx, jy, N)];
+        }
+      out[IDX(x, y, N)] = s;
+    }
+}
+
+/* DisplacementLengthEnergy::apply (hyperelastic.h:178-183): 1/2 sum_c phi_c . K phi_c, the
+ * MultiVector dot product summed component by component (multiVector.h:570-578, vec.cpp:33-40). */
+double msor_reg_energy(const double *phi, int N) {
+  const int NN = N * N;
+  double *k = (double *)malloc(sizeof(double) * NN);
+  double dot = 0;
+  for (int c = 0; c < 2; ++c) {
+    msor_stiff_apply(phi + c * NN, N, k);
+    double s = 0;
+    for (int i = 0; i < NN; ++i) s += phi[c * NN + i] * k[i];
+    dot += s;
+  }
+  free(k);
+  return 0.5 * dot;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * H1 metric: FD heat-equation multigrid (linearSmoothOp.cpp)
+ * ------------------------------------------------------------------------------------------- */
+
+/* HeatEquation2DFD::apply (linearSmoothOp.cpp:59-146): u + tau (N-1)^2 (4u - sum of neighbours),
+ * with mirrored (doubled) neighbours on edges and corners. */
+void msor_heat_apply(const double *u, int N, double tau, double *out) {
+  if (N == 1) {
+    out[0] = u[0];
+    return;
+  }
+  const double hs = tau * (double)((N - 1) * (N - 1));
+  const int off[4] = {1, -1, N, -N};
+  for (int X = 1; X < N - 1; ++X)
+    for (int Y = 1; Y < N - 1; ++Y) {
+      const int i = IDX(X, Y, N);
+      const double a = u[i] * 4;
+      double b = 0.0;
+      for (int k = 0; k < 4; ++k) b -= u[i + off[k]];
+      out[i] = u[i] + hs * (a + b);
+    }
+  const int e = N - 1;
+  const int corners[4][2] = {{0, 0}, {e, 0}, {0, e}, {e, e}};
+  const int coffs[4][2] = {{1, 1}, {-1, 1}, {1, -1}, {-1, -1}};
+  for (int k = 0; k < 4; ++k) {
+    const int cx = corners[k][0], cy = corners[k][1];
+    const double a = u[IDX(cx, cy, N)] * 4;
+    const double b = -u[IDX(cx + coffs[k][0], cy, N)] - u[IDX(cx, cy + coffs[k][1], N)];
+    out[IDX(cx, cy, N)] = u[IDX(cx, cy, N)] + hs * (a + 2 * b);
+  }
+  for (int X = 1; X < N - 1; ++X) {
+    double a = u[IDX(X, e, N)] * 4;
+    double b = -2 * (u[IDX(X, e - 1, N)]);
+    b -= u[IDX(X + 1, e, N)];
+    b -= u[IDX(X - 1, e, N)];
+    out[IDX(X, e, N)] = u[IDX(X, e, N)] + hs * (a + b);
+    a = u[IDX(X, 0, N)] * 4;
+    b = -2 * (u[IDX(X, 1, N)]);
+    b -= u[IDX(X + 1, 0, N)];
+    b -= u[IDX(X - 1, 0, N)];
+    out[IDX(X, 0, N)] = u[IDX(X, 0, N)] + hs * (a + b);
+  }
+  for (int X = 1; X < N - 1; ++X) {
+    double a = u[IDX(e, X, N)] * 4;
+    double b = -2 * (u[IDX(e - 1, X, N)]);
+    b -= u[IDX(e, X + 1, N)];
+    b -= u[IDX(e, X - 1, N)];
+    out[IDX(e, X, N)] = u[IDX(e, X, N)] + hs * (a + b);
+    a = u[IDX(0, X, N)] * 4;
+    b = -2 * (u[IDX(1, X, N)]);
+    b -= u[IDX(0, X + 1, N)];
+    b -= u[IDX(0, X - 1, N)];
+    out[IDX(0, X, N)] = u[IDX(0, X, N)] + hs * (a + b);
+  }
+}
+
+/* HeatMultigrid::jacobi (linearSmoothOp.cpp:556-580): x += (rhs - A x) / (1 + 4 tau (N-1)^2). */
+static void jacobi(int N, double tau, double *x, const double *rhs, double *dummy) {
+  const double diag = 1 + 4 * tau * (double)((N - 1) * (N - 1));
+  const double relax = 1.0;
+  msor_heat_apply(x, N, tau, dummy);
+  for (int l = 0; l < N * N; ++l) {
+    const double v = rhs[l] - dummy[l];
+    x[l] += relax * v / diag;
+  }
+}
+
+/* MDMultigrid::solve (linearSmoothOp.cpp:451-494): 2 pre-smoothing Jacobi sweeps, residual,
+ * restriction, recursion, injection of the coarse correction, 2 post-smoothing sweeps;
+ * 100 sweeps on the 1x1 level. */
+static void mgSolve(int level, double tau, double **x, double **rhs, double **dummy, double **res) {
+  const int N = 1 << level;
+  if (level == 0) {
+    for (int i = 0; i < 100; ++i) jacobi(N, tau, x[0], rhs[0], dummy[0]);
+    return;
+  }
+  for (int i = 0; i < 2; ++i) jacobi(N, tau, x[level], rhs[level], dummy[level]);
+  msor_heat_apply(x[level], N, tau, dummy[level]);
+  for (int l = 0; l < N * N; ++l) res[level][l] = rhs[level][l] - dummy[level][l];
+  msor_restrict(res[level], N, rhs[level - 1]);
+  memset(x[level - 1], 0, sizeof(double) * (N / 2) * (N / 2));
+  mgSolve(level - 1, tau, x, rhs, dummy, res);
+  msor_prolong(x[level - 1], N / 2, dummy[level]);
+  for (int l = 0; l < N * N; ++l) x[level][l] += dummy[level][l];
+  for (int i = 0; i < 2; ++i) jacobi(N, tau, x[level], rhs[level], dummy[level]);
+}
+
+/* LinearSmoothOp::apply (linearSmoothOp.cpp:591-612) with tau = sigma^2/2 (linearSmoothOp.h:50-52). */
+void msor_vcycle(const double *d, int N, double sigma, double *out) {
+  int L = 0;
+  while ((1 << L) < N) ++L;
+  const double tau = 0.5 * (sigma * sigma);
+  double *x[32], *rhs[32], *dummy[32], *res[32];
+  for (int l = 0; l <= L; ++l) {
+    const size_t n = (size_t)1 << (2 * l);
+    x[l] = (double *)calloc(n, sizeof(double));
+    rhs[l] = (double *)calloc(n, sizeof(double));
+    dummy[l] = (double *)calloc(n, sizeof(double));
+    res[l] = (double *)calloc(n, sizeof(double));
+  }
+  memcpy(rhs[L], d, sizeof(double) * N * N);
+  mgSolve(L, tau, x, rhs, dummy, res);
+  memcpy(out, x[L], sizeof(double) * N * N);
+  for (int l = 0; l <= L; ++l) {
+    free(x[l]);
+    free(rhs[l]);
+    free(dummy[l]);
+    free(res[l]);
+  }
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * One level of the H1 gradient flow with nonlinear CG and Armijo step control
+ * ------------------------------------------------------------------------------------------- */
+
+typedef struct {
+  const double *R, *T;
+  int N;
+  double lambda;
+  double *tmp; /* 2*N*N scratch for positions */
+  int evals;
+} Energy;
+
+/* E = (0 + E_SSD) + lambda * E_reg (LinCombOp::applyAdd, op.h:255-272) */
+static double energyAt(Energy *e, const double *phi) {
+  double E = 0;
+  E += msor_ssd_energy(e->R, e->T, phi, e->N);
+  E += e->lambda * msor_reg_energy(phi, e->N);
+  return E;
+}
+
+/* DE = (0 + F_SSD) + lambda * K phi, componentwise */
+static void gradientAt(Energy *e, const double *phi, double *out) {
+  const int NN = e->N * e->N;
+  double *k = (double *)malloc(sizeof(double) * NN);
+  msor_ssd_force(e->R, e->T, phi, e->N, out);
+  for (int c = 0; c < 2; ++c) {
+    msor_stiff_apply(phi + c * NN, e->N, k);
+    for (int i = 0; i < NN; ++i) out[c * NN + i] += e->lambda * k[i];
+  }
+  free(k);
+}
+
+/* ArmijoLineSearchHelpFunction_evaluate (gradientDescent.h:154-174): E(x + tau d) */
+static double evalStep(Energy *e, const double *x, const double *d, double tau) {
+  const int n = 2 * e->N * e->N;
+  for (int i = 0; i < n; ++i) e->tmp[i] = x[i] + tau * d[i];
+  e->evals++;
+  return energyAt(e, e->tmp);
+}
+
+static double dot2(const double *a, const double *b, int NN) {
+  double dot = 0;
+  for (int c = 0; c < 2; ++c) {
+    double s = 0;
+    for (int i = 0; i < NN; ++i) s += a[c * NN + i] * b[c * NN + i];
+    dot += s;
+  }
+  return dot;
+}
+
+static const double TAU_MIN = 9.313225746154785e-10; /* 2^-30 (ArmijoSearch.h:64-73) */
+static const double TAU_MAX = 33554432.0;            /* 2^25 */
+static const double ARMIJO_SIGMA = 0.5;
+
+/* getTimestepWidthWithArmijoLineSearch (ArmijoSearch.h:118-200) */
+static double armijo(Energy *e, const double *x, const double *d, const double *rawDE, double oldTau, double f) {
+  const int NN = e->N * e->N;
+  double tau = oldTau < TAU_MIN ? TAU_MIN : (oldTau > TAU_MAX ? TAU_MAX : oldTau);
+  double fNew = evalStep(e, x, d, tau);
+  const double Df = dot2(rawDE, d, NN);
+  double G = (fNew - f) / (Df * tau);
+  if (Df >= 0) return 0;
+  if (!isfinite(f)) return 0;
+  if (G >= ARMIJO_SIGMA && f >= fNew) {
+    do {
+      tau *= 2.;
+      fNew = evalStep(e, x, d, tau);
+      G = (fNew - f) / (Df * tau);
+    } while (G >= ARMIJO_SIGMA && f >= fNew && tau <= TAU_MAX);
+    tau *= 0.5;
+  } else {
+    do {
+      if (tau > TAU_MIN) tau *= 0.5;
+      fNew = evalStep(e, x, d, tau);
+      G = (fNew - f) / (Df * tau);
+    } while ((!isfinite(fNew) || (G < ARMIJO_SIGMA || f < fNew)) && (tau > TAU_MIN));
+  }
+  return tau > TAU_MIN ? tau : 0.;
+}
+
+/* GradientDescentBase::applySingle (gradientDescent.h:295-464) as configured by
+ * StandardRegistration::updateTransformationWithGD (registration.h:1003-1021): H1 metric
+ * (H1GradientDescent::smoothDirection, gradientDescent.h:652-662), Fletcher-Reeves CG restarted every
+ * 10 steps (calcTaudAndDescentDir, L200-236), automatic filter width starting at 1 and halved when
+ * the energy decrease drops below 1e-5 (GradientDescentWithAutomaticFilterWidth L675-691; L416-439).
+ * An energy output stream is attached in the reference run, so the energy at the new position is
+ * re-evaluated after each update instead of being taken from the Armijo cache (L361-396). */
+int msor_level_solve(const double *R, const double *T, double *phi, int N, const msor_level_params *p,
+                     double *energy_trace, double *tau_trace, double *sigma_trace, int *evals_trace,
+                     int trace_cap, int *n_armijo_calls, double *final_energy) {
+  const int NN = N * N, n2 = 2 * NN;
+  Energy e = {R, T, N, p->lambda, (double *)malloc(sizeof(double) * n2), 0};
+  double *de = (double *)malloc(sizeof(double) * n2);
+  double *dir = (double *)malloc(sizeof(double) * n2);
+  double *oldDir = (double *)calloc(n2, sizeof(double));
+  double *oldGrad = (double *)calloc(n2, sizeof(double));
+  double tau = 1.0;
+  double filterWidth = 1.0;
+  const double stopFilterWidth = 0.0001, updateTol = 0.00001;
+  int calls = 0;
+
+  double energy = energyAt(&e, phi);
+  double energyNew = energy;
+  int iter = 0;
+  if (energy_trace && trace_cap > 0) energy_trace[0] = energyNew;
+  if (p->max_iter <= 0) {
+    if (final_energy) *final_energy = energy;
+    if (n_armijo_calls) *n_armijo_calls = 0;
+    free(e.tmp); free(de); free(dir); free(oldDir); free(oldGrad);
+    return 0;
+  }
+  int stopping;
+  do {
+    iter++;
+    energy = energyNew;
+    /* calcTaudAndDescentDir */
+    gradientAt(&e, phi, de);
+    for (int i = 0; i < n2; ++i) dir[i] = -de[i];
+    for (int c = 0; c < 2; ++c) msor_vcycle(dir + c * NN, N, filterWidth, dir + c * NN);
+    const double beta = ((iter - 1) % 10) ? (dot2(dir, dir, NN) / dot2(oldGrad, oldGrad, NN)) : 0.0;
+    memcpy(oldGrad, dir, sizeof(double) * n2);
+    for (int i = 0; i < n2; ++i) dir[i] += beta * oldDir[i];
+    memcpy(oldDir, dir, sizeof(double) * n2);
+    e.evals = 0;
+    tau = armijo(&e, phi, dir, de, tau, energy);
+    if (tau_trace && calls < trace_cap) {
+      tau_trace[calls] = tau;
+      if (sigma_trace) sigma_trace[calls] = filterWidth;
+      if (evals_trace) evals_trace[calls] = e.evals;
+    }
+    calls++;
+    for (int i = 0; i < n2; ++i) dir[i] *= tau;
+    if (tau == 0) {
+      memcpy(dir, oldGrad, sizeof(double) * n2);
+      memset(oldDir, 0, sizeof(double) * n2);
+      e.evals = 0;
+      tau = armijo(&e, phi, dir, de, tau, energy);
+      if (tau_trace && calls < trace_cap) {
+        tau_trace[calls] = tau;
+        if (sigma_trace) sigma_trace[calls] = filterWidth;
+        if (evals_trace) evals_trace[calls] = e.evals;
+      }
+      calls++;
+      for (int i = 0; i < n2; ++i) dir[i] *= tau;
+    }
+    if (tau != 0)
+      for (int i = 0; i < n2; ++i) phi[i] += dir[i];
+    energyNew = energyAt(&e, phi);
+    if (energy_trace && iter < trace_cap) energy_trace[iter] = energyNew;
+
+    stopping = ((energy - energyNew) > p->stop_eps) && (iter < p->max_iter);
+    if ((0.5 * filterWidth > stopFilterWidth) && ((energy - energyNew) < updateTol)) {
+      filterWidth *= 0.5;
+      tau = 1.;
+      stopping = (filterWidth > stopFilterWidth) && (iter < p->max_iter);
+    }
+  } while (stopping);
+
+  if (final_energy) *final_energy = energyNew;
+  if (n_armijo_calls) *n_armijo_calls = calls;
+  free(e.tmp); free(de); free(dir); free(oldDir); free(oldGrad);
+  return iter;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Input normalisation
+ * ------------------------------------------------------------------------------------------- */
+
+static int cmpd(const void *a, const void *b) {
+  const double x = *(const double *)a, y = *(const double *)b;
+  return (x > y) - (x < y);
+}
+
+/* prepareMultilevelArray (registration.h:331-347): saturated min/max (vec.h:864-870) + clamp +
+ * scaleValuesTo01 (vec.h:1084-1089), or shift-to-zero and divide by the maximum. */
+void msor_normalize(double *img, int count, double pct, int min_to_zero) {
+  if (pct >= 0) {
+    double *t = (double *)malloc(sizeof(double) * count);
+    memcpy(t, img, sizeof(double) * count);
+    qsort(t, count, sizeof(double), cmpd);
+    const int m1 = count - 1;
+    int i0 = (int)(pct / 200 * m1), i1 = (int)((1 - pct / 200) * m1);
+    i0 = i0 < 0 ? 0 : (i0 > m1 ? m1 : i0);
+    i1 = i1 < 0 ? 0 : (i1 > m1 ? m1 : i1);
+    const double lo = t[i0], hi = t[i1];
+    free(t);
+    for (int i = 0; i < count; ++i) img[i] = img[i] < lo ? lo : (img[i] > hi ? hi : img[i]);
+    double mn = img[0];
+    for (int i = 1; i < count; ++i) if (img[i] < mn) mn = img[i];
+    for (int i = 0; i < count; ++i) img[i] += -1 * mn;
+    double mx = img[0];
+    for (int i = 1; i < count; ++i) if (img[i] > mx) mx = img[i];
+    if (mx != 0)
+      for (int i = 0; i < count; ++i) img[i] /= mx;
+  } else {
+    double mn = img[0];
+    for (int i = 1; i < count; ++i) if (img[i] < mn) mn = img[i];
+    if (mn < 0. || min_to_zero)
+      for (int i = 0; i < count; ++i) img[i] += -mn;
+    double mx = img[0];
+    for (int i = 1; i < count; ++i) if (img[i] > mx) mx = img[i];
+    if (mx != 0.)
+      for (int i = 0; i < count; ++i) img[i] /= mx;
+  }
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Composition with harmonic extension
+ * ------------------------------------------------------------------------------------------- */
+
+/* DeformImage with an extension image of NaN (deformations.h:226-256) */
+static void deformWithNaN(const double *img, const double *phi, int N, double *out) {
+  const double h = gridH(N);
+  const int NN = N * N;
+  for (int y = 0; y < N; ++y)
+    for (int x = 0; x < N; ++x) {
+      const int i = IDX(x, y, N);
+      double ds[2];
+      int inside = 1;
+      ds[0] = x + phi[i] / h;
+      ds[1] = y + phi[NN + i] / h;
+      for (int c = 0; c < 2; ++c)
+        if (ds[c] < 0.0 || ds[c] > (double)(N - 1)) inside = 0;
+      out[i] = inside ? msor_interpolate(img, N, ds[0], ds[1]) : NAN;
+    }
+}
+
+/* Stiffness matrix with Dirichlet rows/columns removed for defined nodes and unit diagonal there
+ * (assembleAddMatrix with a Dirichlet mask and setDirichletNodes, FEOpInterface.h:624-639). */
+static void dirichletStiffApply(const double *u, const unsigned char *defined, int N, double *out, double *tmp) {
+  const int NN = N * N;
+  for (int i = 0; i < NN; ++i) tmp[i] = defined[i] ? 0.0 : u[i];
+  msor_stiff_apply(tmp, N, out);
+  for (int i = 0; i < NN; ++i) out[i] = defined[i] ? u[i] : out[i];
+}
+
+static double dirichletDiag(int i, const unsigned char *defined, int N) {
+  if (defined[i]) return 1.0;
+  const int x = i % N, y = i / N;
+  int ne = 0; /* number of adjacent elements */
+  for (int ey = y - 1; ey <= y; ++ey)
+    for (int ex = x - 1; ex <= x; ++ex)
+      if (ex >= 0 && ey >= 0 && ex < N - 1 && ey < N - 1) ++ne;
+  return ne * (4.0 / 6.0);
+}
+
+/* PCGInverse::apply (solver.h:264-359) with a diagonal preconditioner, absolute stopping at
+ * 1e-16 on |g|^2, at most 1000 iterations; dest holds the start value. */
+static void pcgDirichlet(const double *rhs, const unsigned char *defined, int N, double *dest) {
+  const int NN = N * N;
+  double *g = (double *)calloc(NN, sizeof(double));
+  double *d = (double *)calloc(NN, sizeof(double));
+  double *h = (double *)calloc(NN, sizeof(double));
+  double *tmp = (double *)calloc(NN, sizeof(double));
+  double *diag = (double *)malloc(sizeof(double) * NN);
+  for (int i = 0; i < NN; ++i) diag[i] = dirichletDiag(i, defined, N);
+  dirichletStiffApply(dest, defined, N, h, tmp);
+  for (int i = 0; i < NN; ++i) g[i] = h[i] - rhs[i];
+  for (int i = 0; i < NN; ++i) h[i] = g[i] / diag[i];
+  double spn = 0;
+  for (int i = 0; i < NN; ++i) spn += g[i] * g[i];
+  for (int i = 0; i < NN; ++i) d[i] -= h[i];
+  int it = 0;
+  while (!(spn < 1e-16) && it < 1000 && !isnan(spn)) {
+    double an = 0;
+    for (int i = 0; i < NN; ++i) an += g[i] * h[i];
+    const double bd = an;
+    dirichletStiffApply(d, defined, N, h, tmp);
+    double ad = 0;
+    for (int i = 0; i < NN; ++i) ad += d[i] * h[i];
+    const double alpha = an / ad;
+    for (int i = 0; i < NN; ++i) dest[i] += alpha * d[i];
+    for (int i = 0; i < NN; ++i) g[i] += alpha * h[i];
+    for (int i = 0; i < NN; ++i) h[i] = g[i] / diag[i];
+    double bn = 0;
+    for (int i = 0; i < NN; ++i) bn += g[i] * h[i];
+    const double bf = bn / bd;
+    for (int i = 0; i < NN; ++i) d[i] *= bf;
+    for (int i = 0; i < NN; ++i) d[i] -= h[i];
+    spn = 0;
+    for (int i = 0; i < NN; ++i) spn += g[i] * g[i];
+    ++it;
+  }
+  free(g); free(d); free(h); free(tmp); free(diag);
+}
+
+/* ConcatAndSmoothlyExtendDeformations (deformations.h:499-520) with SmoothlyExtendImage
+ * (deformations.h:364-404). */
+void msor_compose(const double *phi1, const double *phi2, int N, double *result) {
+  const int NN = N * N;
+  double *r = (double *)malloc(sizeof(double) * 2 * NN);
+  for (int c = 0; c < 2; ++c) deformWithNaN(phi1 + c * NN, phi2, N, r + c * NN);
+  for (int i = 0; i < 2 * NN; ++i) r[i] += phi2[i];
+  unsigned char *defined = (unsigned char *)malloc(NN);
+  for (int i = 0; i < NN; ++i) defined[i] = !isnan(r[i]);
+  double *bc = (double *)malloc(sizeof(double) * NN);
+  double *rhs = (double *)malloc(sizeof(double) * NN);
+  for (int c = 0; c < 2; ++c) {
+    for (int i = 0; i < NN; ++i) bc[i] = defined[i] ? r[c * NN + i] : 0.;
+    msor_stiff_apply(bc, N, rhs);
+    for (int i = 0; i < NN; ++i) rhs[i] *= -1.;
+    for (int i = 0; i < NN; ++i) if (defined[i]) rhs[i] = bc[i];
+    pcgDirichlet(rhs, defined, N, result + c * NN);
+  }
+  free(r); free(defined); free(bc); free(rhs);
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Reconstruction statistics
+ * ------------------------------------------------------------------------------------------- */
+
+void msor_series_stats(const double *stack, int F, int NN, double fill, double *mean, double *count,
+                       double *median) {
+  double *s = (double *)malloc(sizeof(double) * (F > 0 ? F : 1));
+  for (int j = 0; j < NN; ++j) {
+    double m = 0;
+    int c = 0;
+    for (int f = 0; f < F; ++f) {
+      const double v = stack[(size_t)f * NN + j];
+      if (v != INFINITY) {
+        m += v;
+        s[c++] = v;
+      }
+    }
+    if (c > 0) m /= c; else m = fill;
+    mean[j] = m;
+    count[j] = c;
+    if (c == 0) {
+      median[j] = fill;
+    } else {
+      qsort(s, c, sizeof(double), cmpd);
+      median[j] = (c % 2 == 1) ? s[c / 2] : (s[c / 2 - 1] + s[c / 2]) / 2;
+    }
+  }
+  free(s);
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Series matching (one stage)
+ * ------------------------------------------------------------------------------------------- */
+
+typedef struct {
+  int L;          /* finest level (precisionLevel) */
+  double *lev[32]; /* images per level */
+} Pyramid;
+
+static void pyrBuild(Pyramid *p, const double *fine, int L) {
+  p->L = L;
+  for (int l = 0; l <= L; ++l) p->lev[l] = (double *)malloc(sizeof(double) * ((size_t)1 << (2 * l)));
+  memcpy(p->lev[L], fine, sizeof(double) * ((size_t)1 << (2 * L)));
+  for (int l = L; l > 0; --l) msor_restrict(p->lev[l], 1 << l, p->lev[l - 1]); /* MultilevelArray::levRestrict */
+}
+static void pyrFree(Pyramid *p) {
+  for (int l = 0; l <= p->L; ++l) free(p->lev[l]);
+}
+
+/* deformation pyramid: 2 components per level */
+typedef struct {
+  int L;
+  double *lev[32];
+} DefPyramid;
+
+static void defAlloc(DefPyramid *d, int L) {
+  d->L = L;
+  for (int l = 0; l <= L; ++l) d->lev[l] = (double *)calloc((size_t)2 << (2 * l), sizeof(double));
+}
+static void defFree(DefPyramid *d) {
+  for (int l = 0; l <= d->L; ++l) free(d->lev[l]);
+}
+/* setTransformation + levRestrict (registration.h:549-553) */
+static void defSet(DefPyramid *d, const double *fine) {
+  const size_t NN = (size_t)1 << (2 * d->L);
+  memcpy(d->lev[d->L], fine, sizeof(double) * 2 * NN);
+  for (int l = d->L; l > 0; --l) {
+    const size_t n = (size_t)1 << (2 * (l - 1));
+    for (int c = 0; c < 2; ++c) msor_restrict(d->lev[l] + c * ((size_t)1 << (2 * l)), 1 << l, d->lev[l - 1] + c * n);
+  }
+}
+/* RegistrationMultilevelDescentInterface::prolongate (registration.h:510-521) */
+static void defProlong(DefPyramid *d, int l, int resample) {
+  const int n = 1 << l, N = 2 * n;
+  for (int c = 0; c < 2; ++c) {
+    if (resample) msor_resample(d->lev[l] + c * n * n, n, d->lev[l + 1] + c * N * N, N);
+    else msor_prolong(d->lev[l] + c * n * n, n, d->lev[l + 1] + c * N * N);
+  }
+}
+
+static double levelLambda(const msor_series_params *p, int level) {
+  const double f = p->lambda_factor > 0 ? pow(p->lambda_factor, p->stop_level - level) : 1; /* registration.h:523-526 */
+  return f * p->lambda;
+}
+
+/* solveAndProlongToMaxDepth (registration.h:462-467) over MultilevelDescentInterface::solve
+ * (quocDescent.h:83-105) with descentOnCurrentGrid (registration.h:1092-1112). */
+static double solveML(const Pyramid *R, const Pyramid *T, DefPyramid *d, int start, int stop,
+                      const msor_series_params *p) {
+  double energy = NAN;
+  for (int level = start; level <= stop; ++level) {
+    msor_level_params lp = {levelLambda(p, level), p->max_iter, p->stop_eps};
+    msor_level_solve(R->lev[level], T->lev[level], d->lev[level], 1 << level, &lp, NULL, NULL, NULL, NULL, 0,
+                     NULL, &energy);
+    if (level < stop) defProlong(d, level, p->resample_prolong);
+  }
+  for (int l = stop; l < d->L; ++l) defProlong(d, l, p->resample_prolong);
+  return energy;
+}
+
+static void prepared(const double *raw, int NN, const msor_series_params *p, double *out) {
+  memcpy(out, raw, sizeof(double) * NN);
+  if (!p->dont_normalize) msor_normalize(out, NN, p->saturation_pct, p->normalize_min_to_zero);
+}
+
+int msor_match_series(const double *templates, int n, const double *reference, int N,
+                      const msor_series_params *p, const double *pairwise_in,
+                      double *deformations, double *pairwise_out, double *energies,
+                      double *mean, double *count, double *median) {
+  int L = 0;
+  while ((1 << L) < N) ++L;
+  const size_t NN = (size_t)N * N;
+  double *img = (double *)malloc(sizeof(double) * NN);
+  double *acc = (double *)calloc(2 * NN, sizeof(double));
+  double *tmp = (double *)malloc(sizeof(double) * 2 * NN);
+  Pyramid ref0, ref, tpl;
+  prepared(reference, (int)NN, p, img);
+  pyrBuild(&ref0, img, L);
+  pyrBuild(&ref, img, L);
+  DefPyramid def;
+  defAlloc(&def, L);
+
+  for (int i = 0; i < n; ++i) {
+    double *zero = (double *)calloc(2 * NN, sizeof(double));
+    defSet(&def, zero); /* setTransformationToZero */
+    free(zero);
+    prepared(templates + i * NN, (int)NN, p, img);
+    pyrBuild(&tpl, img, L);
+    if (pairwise_in == NULL || i == 0) {
+      energies[i] = solveML(&ref, &tpl, &def, p->start_level, p->stop_level, p);
+    } else {
+      defSet(&def, pairwise_in + i * 2 * NN); /* loadTransformation of the previous stage */
+    }
+    if (pairwise_out) memcpy(pairwise_out + i * 2 * NN, def.lev[L], sizeof(double) * 2 * NN);
+    if (!p->dont_accumulate) {
+      memcpy(tmp, acc, sizeof(double) * 2 * NN);
+      msor_compose(def.lev[L], tmp, N, acc); /* composeDeformations(current, old acc) -> acc */
+      if (i > 0) {
+        defSet(&def, acc);
+        pyrFree(&ref);
+        pyrBuild(&ref, ref0.lev[L], L); /* setReference(reference) */
+        energies[i] = solveML(&ref, &tpl, &def, p->refine_start_level, p->refine_stop_level, p);
+        memcpy(acc, def.lev[L], sizeof(double) * 2 * NN);
+      }
+      pyrFree(&ref);
+      pyrBuild(&ref, tpl.lev[L], L); /* setReference(template) */
+    }
+    memcpy(deformations + i * 2 * NN, (!p->dont_accumulate && i > 0) ? acc : def.lev[L], sizeof(double) * 2 * NN);
+    pyrFree(&tpl);
+  }
+
+  /* averageSeries (matchSeries.h:353-451): raw templates, Inf outside, fill = last template mean */
+  double *stack = (double *)malloc(sizeof(double) * NN * (n > 0 ? n : 1));
+  double lastMean = 0;
+  for (int i = 0; i < n; ++i) {
+    msor_deform_image(templates + i * NN, deformations + i * 2 * NN, N, INFINITY, 0, stack + i * NN);
+    if (i == n - 1) {
+      double s = 0;
+      for (size_t j = 0; j < NN; ++j) s += templates[i * NN + j];
+      lastMean = s / NN;
+    }
+  }
+  msor_series_stats(stack, n, (int)NN, lastMean, mean, count, median);
+  free(stack);
+  pyrFree(&ref0);
+  pyrFree(&ref);
+  defFree(&def);
+  free(img);
+  free(acc);
+  free(tmp);
+  return 0;
+}

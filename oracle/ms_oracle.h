@@ -1,0 +1,99 @@
+/*
+ * ms_oracle.h — FP64 CPU restatement of match-series' registration hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load liboracle.so, and only as the checker (or the timed CPU baseline). The product never calls it.
+ *
+ * Conventions (shared with the reference, QuocMesh):
+ *   - a level-l grid has N = 2^l nodes per side, h = 1/(N-1) (rectangularGrid.h:50-57);
+ *   - node (x, y) is stored at x + y*N (configurators.h:889-900);
+ *   - a displacement field phi is two consecutive N*N arrays (x component, then y component), in
+ *     domain units: the pixel offset is phi/h (deformations.h:85).
+ * Every function cites the reference code it restates.
+ */
+#ifndef MS_ORACLE_H
+#define MS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* --- grid transfer (cellCenteredGrid.h:157-210, scalarArray.h:308-331,1052-1060) --- */
+void msor_restrict(const double *fine, int N, double *coarse);        /* N -> N/2, 2x2 mean      */
+void msor_prolong(const double *coarse, int n, double *fine);         /* n -> 2n, injection      */
+void msor_resample(const double *src, int n, double *dst, int N);     /* n -> N, bilinear        */
+
+/* --- images (scalarArray.h:499-529, deformations.h:62-101,140-162) --- */
+double msor_interpolate(const double *img, int N, double X, double Y);
+void msor_deform_image(const double *img, const double *phi, int N, double ext, int nearest, double *out);
+void msor_fast_deform_image(const double *img, const double *phi, int N, double *out);
+
+/* --- energy terms (registration.h:1607-1615,1646-1666; FEOpInterface.h:1803-1853,2586-2613) --- */
+double msor_ssd_energy(const double *R, const double *T, const double *phi, int N);
+void msor_ssd_force(const double *R, const double *T, const double *phi, int N, double *out);
+void msor_stiff_apply(const double *u, int N, double *out);           /* one component, K u      */
+double msor_reg_energy(const double *phi, int N);                     /* 1/2 sum_c phi_c K phi_c */
+
+/* --- H1 metric: one FD heat-equation multigrid V-cycle (linearSmoothOp.cpp:11-147,451-612) --- */
+void msor_heat_apply(const double *u, int N, double tau, double *out);
+void msor_vcycle(const double *d, int N, double sigma, double *out);
+
+/* --- one level of the gradient flow (gradientDescent.h:200-236,295-464; ArmijoSearch.h:118-200) ---
+ * phi (2*N*N) is updated in place. Traces (may be NULL) are written per Armijo call
+ * (tau, sigma at the call, energy evaluations of the call) and per iteration (energy after the
+ * step, energy_trace[0] = initial energy); trace_cap bounds every trace. Returns iterations. */
+typedef struct {
+  double lambda;      /* regularisation weight of this level            */
+  int max_iter;       /* maxGDIterations                                */
+  double stop_eps;    /* stopEpsilon                                    */
+} msor_level_params;
+
+int msor_level_solve(const double *R, const double *T, double *phi, int N, const msor_level_params *p,
+                     double *energy_trace, double *tau_trace, double *sigma_trace, int *evals_trace,
+                     int trace_cap, int *n_armijo_calls, double *final_energy);
+
+/* --- input normalisation and image pyramid (registration.h:331-367; vec.h:864-870,1084-1089) ---
+ * pct < 0 means "enhanceContrastSaturationPercentage" absent. */
+void msor_normalize(double *img, int count, double pct, int min_to_zero);
+
+/* --- deformation composition with harmonic extension (deformations.h:364-404,499-520) ---
+ * result = phi1(x + phi2(x)) + phi2(x) where defined, extended by a Dirichlet-Laplace PCG solve.
+ * result holds the PCG start value on entry (the reference passes its output array, which holds
+ * the previous accumulated deformation). */
+void msor_compose(const double *phi1, const double *phi2, int N, double *result);
+
+/* --- reconstruction statistics (matchSeries.h:406-444; multiVector.cpp:99-117; vec.cpp:933-949) ---
+ * stack: F arrays of NN values, +Inf marks a missing sample. */
+void msor_series_stats(const double *stack, int F, int NN, double fill, double *mean, double *count,
+                       double *median);
+
+/* --- one stage of series matching (matchSeries.h:87-269,353-451,1249-1320) ---
+ * templates: n raw images of N*N values; reference: raw N*N image (stage 1: the first frame,
+ * later stages: the previous stage's median). Images are normalised for registration (unless
+ * dont_normalize) and averaged raw, as the reference does.
+ * pairwise_in: NULL, or n fields of 2*N*N whose entry i (i >= 1) replaces the pairwise solve of
+ *   template i ("reuseStage1Results": the previous stage's pairwise deformation of template i-1).
+ * deformations (out): n fields, the deformation each template is averaged with.
+ * pairwise_out (out, may be NULL): n fields, each template's pairwise deformation.
+ * energies (out): n final energies; mean/count/median (out): N*N each. */
+typedef struct {
+  int start_level, stop_level, refine_start_level, refine_stop_level, precision_level;
+  double lambda, lambda_factor;       /* lambda_factor <= 0: "lambdaFactor" absent            */
+  int max_iter;
+  double stop_eps;
+  double saturation_pct;              /* < 0: absent                                          */
+  int normalize_min_to_zero;
+  int dont_normalize;                 /* dontNormalizeInputImages                             */
+  int resample_prolong;               /* resampleInsteadOfProlongateDeformation               */
+  int dont_accumulate;                /* dontAccumulateDeformation                            */
+} msor_series_params;
+
+int msor_match_series(const double *templates, int n, const double *reference, int N,
+                      const msor_series_params *p, const double *pairwise_in,
+                      double *deformations, double *pairwise_out, double *energies,
+                      double *mean, double *count, double *median);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
