@@ -1,0 +1,173 @@
+/*
+ * ms_gpu.h — C ABI of the MI355X (gfx950) implementation of match-series' registration hot path.
+ *
+ * Library: match-series_amd/lib/libmsgpu.so. Every function returns MSG_OK (0) or an error code;
+ * msg_last_error() describes the last failure of the calling thread. No function throws.
+ *
+ * Conventions (those of the reference, QuocMesh):
+ *   - a level-l grid has n = 2^l nodes per side, h = 1/(n-1); node (x, y) is stored at x + y*n;
+ *   - an image is n*n values; a displacement field is 2*n*n values (x component, then y), in
+ *     domain units (pixel offset = value / h);
+ *   - a batch of k images/fields is stored back to back.
+ * Host arrays are float64 (the reference's RealType). Device arrays carry a dtype: MSG_F32 fields
+ * are stored in FP32 and computed in FP64 registers; MSG_F64 stores FP64. Energies, slopes and
+ * Armijo decisions are FP64 in both.
+ *
+ * Each entry point names the reference interface it replaces (file:line, QuocMesh sources under
+ * quocmesh/modules/ and quocmesh/projects/electronMicroscopy/).
+ */
+#ifndef MS_GPU_H
+#define MS_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { MSG_OK = 0, MSG_ERR_ARG = 1, MSG_ERR_HIP = 2, MSG_ERR_STATE = 3, MSG_ERR_ALLOC = 4 };
+enum { MSG_F32 = 0, MSG_F64 = 1 };
+
+/* Description of the last error raised on the calling thread ("" if none). */
+const char *msg_last_error(void);
+/* Library version string. */
+const char *msg_version(void);
+
+/* ------------------------------------------------------------------------------------------------
+ * Registration context: a batch of `npairs` independent (reference, template, deformation) triples
+ * with image pyramids up to level `max_level` — the state of
+ * qc::RegistrationMultilevelDescentInterface (quoc/registration.h:422-672) and
+ * qc::StandardRegistrationMultilevelDescent (registration.h:1072-1132), one per pair.
+ * ---------------------------------------------------------------------------------------------- */
+typedef struct msg_ctx msg_ctx;
+
+/* RegistrationMultilevelDescentInterfaceBase(Parser) with precisionLevel = max_level
+ * (registration.h:263-274); allocates every device buffer the solve needs. */
+int msg_ctx_create(msg_ctx **ctx, int max_level, int npairs, int dtype);
+int msg_ctx_destroy(msg_ctx *ctx);
+
+/* setReference / setTemplate (registration.h:310-319): host float64 image at max_level; the pyramid
+ * is rebuilt by 2x2 restriction (MultilevelArray::levRestrict, multilevelArray.h:99-110). */
+int msg_set_reference(msg_ctx *ctx, int pair, const double *img);
+int msg_set_template(msg_ctx *ctx, int pair, const double *img);
+/* Same for all pairs at once from device memory (npairs images of dtype `dtype`). */
+int msg_set_images_device(msg_ctx *ctx, const void *d_ref, const void *d_tpl, int dtype);
+
+/* setTransformation / getTransformation / setTransformationToZero (registration.h:549-576):
+ * host float64 field at max_level; setting restricts it to every level. */
+int msg_set_deformation(msg_ctx *ctx, int pair, const double *phi);
+int msg_get_deformation(msg_ctx *ctx, int pair, double *phi);
+int msg_set_deformation_zero(msg_ctx *ctx, int pair);
+/* Device copy of all pairs' max_level fields (dtype of the context), in or out. */
+int msg_set_deformations_device(msg_ctx *ctx, const void *d_phi);
+int msg_get_deformations_device(msg_ctx *ctx, void *d_phi);
+
+/* Parameters of the gradient flow (StandardRegistration::setSettingsFromParser,
+ * registration.h:1059-1066; getRegularizationWeight, L523-526). */
+typedef struct {
+  double lambda;          /* "lambda"                                                         */
+  double lambda_factor;   /* "lambdaFactor"; <= 0: absent                                     */
+  int lambda_ref_level;   /* "stopLevel" the lambda factor is anchored to                     */
+  int max_iter;           /* "maxGDIterations" (reference default 1000)                      */
+  double stop_eps;        /* "stopEpsilon" (reference default 0)                             */
+  int resample_prolong;   /* "resampleInsteadOfProlongateDeformation"                         */
+  int spec;               /* Armijo candidates per energy pass, 1..4 (0: default 3)          */
+} msg_solve_params;
+
+/* Per-iteration record of a level solve (the content of energy_<LL>.txt plus step data). */
+typedef struct {
+  double energy, ereg, tau, tau_first, sigma, evals, passes, beta;
+} msg_iter_trace;
+
+/* One level of gradient flow for every pair (descentOnCurrentGrid, registration.h:1092-1112 ->
+ * StandardRegistration::findTransformation L967-1000 -> GradientDescentBase::applySingle,
+ * aol/gradientDescent.h:295-464). energies/iterations: npairs outputs (may be NULL).
+ * trace: NULL or npairs * trace_cap records. */
+int msg_solve_level(msg_ctx *ctx, int level, double lambda, int max_iter, double stop_eps, int spec,
+                    double *energies, int *iterations, msg_iter_trace *trace, int trace_cap);
+
+/* solveAndProlongToMaxDepth(start, stop) for every pair (registration.h:462-467 over
+ * MultilevelDescentInterface::solve, quoc/quocDescent.h:83-105); energies = getEnergyOfLastSolution. */
+int msg_solve(msg_ctx *ctx, int start_level, int stop_level, const msg_solve_params *p, double *energies);
+
+/* Totals of the last msg_solve / msg_solve_level: iterations, reference-equivalent energy
+ * evaluations and energy passes summed over pairs and levels. */
+int msg_last_counts(msg_ctx *ctx, long long *iterations, long long *evals, long long *passes);
+
+/* Device pointer of a pair's level-l image or field inside the context (for zero-copy callers). */
+int msg_ctx_buffer(msg_ctx *ctx, int which /* 0 ref, 1 tpl, 2 phi */, int level, void **ptr);
+
+/* ------------------------------------------------------------------------------------------------
+ * Operator surface on device arrays (batched). stream: hipStream_t or NULL (default stream).
+ * ---------------------------------------------------------------------------------------------- */
+
+/* SSDEnergy (registration.h:1584-1616) and DisplacementLengthEnergy (quoc/hyperelastic.h:168-190):
+ * per pair E_SSD(phi) and 1/2 sum_c phi_c^T K phi_c (host outputs, batch values each). */
+int msg_energy(int dtype, const void *R, const void *T, const void *phi, int n, int batch, double *e_ssd,
+               double *e_reg, void *stream);
+
+/* SSDForce (registration.h:1622-1667) + lambda * K phi (DirichletRegularizationConfigurator,
+ * registration.h:677-710): the variation DE of the registration energy. */
+int msg_gradient(int dtype, const void *R, const void *T, const void *phi, void *DE, int n, int batch,
+                 double lambda, void *stream);
+
+/* LinearSmoothOp::apply (quoc/linearSmoothOp.cpp:591-612): one FD heat multigrid V-cycle with
+ * tau = sigma^2/2 on each of nfields fields (in and out may alias). */
+int msg_h1_smooth(int dtype, const void *in, void *out, int n, int nfields, double sigma, void *stream);
+
+/* DeformImage (quoc/deformations.h:62-101, mode 0: constant `ext` outside, optional nearest
+ * neighbour), FastDeformImage (L140-162, mode 1: clamp) and the NaN-extended warp (L226-256,
+ * mode 2). phi_shared: one field for all nimg images. */
+int msg_warp(int dtype, const void *img, const void *phi, void *out, int n, int nimg, int phi_shared, int mode,
+             double ext, int nearest, void *stream);
+
+/* CellCenteredRestrictOp / CellCenteredProlongOp (quoc/cellCenteredGrid.h:157-210) and
+ * ScalarArray<QC_2D>::resampleFrom (quoc/scalarArray.h:308-331, 1052-1060). */
+int msg_restrict(int dtype, const void *fine, void *coarse, int n, int nfields, void *stream);
+int msg_prolong(int dtype, const void *coarse, void *fine, int nc, int nfields, void *stream);
+int msg_resample(int dtype, const void *src, void *dst, int nc, int n, int nfields, void *stream);
+
+/* ConcatAndSmoothlyExtendDeformations (quoc/deformations.h:499-520, with SmoothlyExtendImage
+ * L364-404): result = phi1(x + phi2(x)) + phi2(x), harmonically extended where undefined.
+ * `result` holds the PCG start value on entry, as the reference's output argument does. */
+int msg_compose(int dtype, const void *phi1, const void *phi2, void *result, int n, void *stream);
+
+/* Reconstruction statistics of SeriesMatching::averageSeries (matchSeries.h:404-444;
+ * MultiVector::getMedianVecOverComponents, aol/multiVector.cpp:99-117): per pixel mean over
+ * finite samples (+Inf marks a missing one), sample count and median; `fill` where no sample. */
+int msg_series_stats(int dtype, const void *stack, int nimg, int n, double fill, void *mean, void *count,
+                     void *median, void *stream);
+
+/* Input normalisation of prepareMultilevelArray (registration.h:331-347): saturation clamp at the
+ * (pct/200) quantiles and [0,1] scaling (pct >= 0), else shift-to-zero and divide by the maximum
+ * (aol/vec.h:864-870, 1084-1089). In place on nimg device images. */
+int msg_normalize(int dtype, void *img, int n, int nimg, double pct, int min_to_zero, void *stream);
+
+/* ------------------------------------------------------------------------------------------------
+ * Series matching: one stage of SeriesMatching::matchAndAverageSeries (matchSeries.h:87-269,
+ * 353-451, 565-570) on device-resident frames.
+ * ---------------------------------------------------------------------------------------------- */
+typedef struct {
+  int start_level, stop_level, refine_start_level, refine_stop_level, precision_level;
+  double lambda, lambda_factor;       /* lambda_factor <= 0: absent                          */
+  int max_iter;
+  double stop_eps;
+  double saturation_pct;              /* "enhanceContrastSaturationPercentage"; < 0: absent   */
+  int normalize_min_to_zero;          /* "normalizeMinToZero"                                 */
+  int dont_normalize;                 /* "dontNormalizeInputImages"                           */
+  int resample_prolong;               /* "resampleInsteadOfProlongateDeformation"             */
+  int dont_accumulate;                /* "dontAccumulateDeformation"                          */
+  int spec;                           /* Armijo candidates per energy pass (0: default)       */
+} msg_series_params;
+
+/* templates: n device images (dtype) at precision_level; reference: one device image.
+ * pairwise_in: NULL or n device fields; entry i >= 1 replaces template i's pairwise solve
+ * ("reuseStage1Results"). Outputs (device, dtype): deformations (n fields, those used for the
+ * average), pairwise_out (n fields or NULL), mean/count/median (one image each); energies: host,
+ * n values. */
+int msg_match_series(const msg_series_params *p, int dtype, const void *templates, int n, const void *reference,
+                     const void *pairwise_in, void *deformations, void *pairwise_out, double *energies,
+                     void *mean, void *count, void *median);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1122 @@
+// Host side of libmsgpu.so: contexts, launches and the C ABI declared in include/ms_gpu.h.
+#include "ms_gpu.h"
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "ms_kernels.h"
+#include "ms_compose.h"
+#include "ms_stats.h"
+
+using msg::PairState;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &m) {
+  g_err = m;
+  return code;
+}
+
+#define HIPCHK(x)                                                                                 \
+  do {                                                                                            \
+    hipError_t e_ = (x);                                                                          \
+    if (e_ != hipSuccess) return fail(MSG_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHK(x)                 \
+  do {                         \
+    int r_ = (x);              \
+    if (r_ != MSG_OK) return r_; \
+  } while (0)
+
+constexpr int MAXL = 15;
+
+inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+inline size_t esz(int dtype) { return dtype == MSG_F64 ? 8 : 4; }
+inline hipStream_t strm(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+int check_dtype(int dtype) {
+  if (dtype != MSG_F32 && dtype != MSG_F64) return fail(MSG_ERR_ARG, "dtype must be MSG_F32 or MSG_F64");
+  return MSG_OK;
+}
+int check_n(int n) {
+  if (n < 2 || (n & (n - 1)) != 0 || n > (1 << MAXL))
+    return fail(MSG_ERR_ARG, "grid size must be a power of two between 2 and 32768");
+  return MSG_OK;
+}
+int log2i(int n) {
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return l;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Grow-only device scratch for the stateless operator entry points (one set per device).
+// ------------------------------------------------------------------------------------------------
+struct Scratch {
+  std::mutex mu;
+  std::vector<std::pair<void *, size_t>> bufs;  // slot -> (ptr, bytes)
+};
+Scratch g_scratch[64];
+
+int scratch_get(int slot, size_t bytes, void **p) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  Scratch &s = g_scratch[dev & 63];
+  if ((int)s.bufs.size() <= slot) s.bufs.resize(slot + 1, {nullptr, 0});
+  auto &b = s.bufs[slot];
+  if (b.second < bytes) {
+    if (b.first) HIPCHK(hipFree(b.first));
+    b.first = nullptr;
+    b.second = 0;
+    HIPCHK(hipMalloc(&b.first, bytes));
+    b.second = bytes;
+  }
+  *p = b.first;
+  return MSG_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Small helper kernels
+// ------------------------------------------------------------------------------------------------
+__global__ void k_state_init(PairState *st, int npairs, double lambda) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  PairState s;
+  memset(&s, 0, sizeof(s));
+  s.ncand = 1;
+  s.cand[0] = 0.0;
+  s.dirmode = msg::DIR_NONE;
+  s.lambda = lambda;
+  s.active = 1;
+  s.sigma = 1.0;
+  st[p] = s;
+}
+
+template <typename D, typename S>
+__global__ void k_convert(const S *__restrict__ in, D *__restrict__ out, size_t cnt) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cnt) out[i] = static_cast<D>(in[i]);
+}
+
+int convert(const void *in, int din, void *out, int dout, size_t cnt, hipStream_t s) {
+  if (cnt == 0) return MSG_OK;
+  if (din == dout) {
+    HIPCHK(hipMemcpyAsync(out, in, cnt * esz(din), hipMemcpyDeviceToDevice, s));
+    return MSG_OK;
+  }
+  const int grid = (int)((cnt + 255) / 256);
+  if (din == MSG_F64)
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid), dim3(256), 0, s, (const double *)in, (float *)out, cnt);
+  else
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid), dim3(256), 0, s, (const float *)in, (double *)out, cnt);
+  HIPCHK(hipGetLastError());
+  return MSG_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Launch wrappers shared by the context path and the stateless entry points
+// ------------------------------------------------------------------------------------------------
+struct EnergyGeom {
+  int ntx, nty, nblk;
+};
+EnergyGeom energy_geom(int n) {
+  EnergyGeom g;
+  g.ntx = cdiv(n - 1, msg::EN_TX);
+  g.nty = cdiv(n - 1, msg::EN_TY);
+  g.nblk = g.ntx * g.nty;
+  return g;
+}
+
+template <typename S>
+int launch_energy(const S *R, const S *T, const S *phi, const S *dcg, const S *dgrad, int n, const PairState *st,
+                  double *part, int npairs, hipStream_t s) {
+  const EnergyGeom g = energy_geom(n);
+  const double h = 1.0 / (double)(n - 1);
+  const int width = (int)((n - 1) * h);
+  hipLaunchKernelGGL((msg::k_energy<S>), dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, dcg, dgrad, n, h, width,
+                     st, part, g.ntx, g.nty, npairs);
+  HIPCHK(hipGetLastError());
+  return MSG_OK;
+}
+
+template <typename S>
+int launch_grad(const S *R, const S *T, const S *phi, S *DE, int n, const PairState *st, double lambda, int npairs,
+                hipStream_t s) {
+  const int ntx = cdiv(n, msg::GR_TX), nty = cdiv(n, msg::GR_TY);
+  const double h = 1.0 / (double)(n - 1);
+  const int width = (int)((n - 1) * h);
+  hipLaunchKernelGGL((msg::k_grad<S>), dim3(ntx * nty * npairs), dim3(256), 0, s, R, T, phi, DE, n, h, width, st,
+                     lambda, ntx, nty, npairs);
+  HIPCHK(hipGetLastError());
+  return MSG_OK;
+}
+
+// Buffers of one V-cycle over fields of level l: per multigrid level m < l the coarse rhs, the
+// pre-smoothed iterate and the final iterate.
+template <typename S>
+struct MGBufs {
+  S *rhs[MAXL + 1];
+  S *x2[MAXL + 1];
+  S *x[MAXL + 1];
+  S *x2_top;  // pre-smoothed iterate of the top level
+};
+
+int mg_norm_blocks(int l) {
+  if (l <= msg::MGC_MAXL) return 1;
+  const int n = 1 << l;
+  return cdiv(n, msg::MG_TX) * cdiv(n, msg::MG_TY);
+}
+
+// out = V-cycle(sign * in) for nfields fields of level l (LinearSmoothOp::apply).
+template <typename S>
+int launch_vcycle(const S *in, int negate, S *out, int l, const MGBufs<S> &b, const PairState *st, double sigma,
+                  int nfields, double *nrm_part, hipStream_t s) {
+  if (l <= msg::MGC_MAXL) {
+    hipLaunchKernelGGL((msg::k_mg_coarse<S>), dim3(nfields), dim3(256), 0, s, in, negate, out, l, st, sigma, nfields,
+                       nrm_part);
+    HIPCHK(hipGetLastError());
+    return MSG_OK;
+  }
+  // down leg: levels l .. MGC_MAXL+1
+  for (int m = l; m > msg::MGC_MAXL; --m) {
+    const int n = 1 << m;
+    const int ntx = cdiv(n, msg::MG_TX), nty = cdiv(n, msg::MG_TY);
+    const S *src = (m == l) ? in : b.rhs[m];
+    const int neg = (m == l) ? negate : 0;
+    S *x2 = (m == l) ? b.x2_top : b.x2[m];
+    hipLaunchKernelGGL((msg::k_mg_down<S>), dim3(ntx * nty * nfields), dim3(256), 0, s, src, neg, x2, b.rhs[m - 1], n,
+                       st, sigma, ntx, nty, nfields);
+    HIPCHK(hipGetLastError());
+  }
+  hipLaunchKernelGGL((msg::k_mg_coarse<S>), dim3(nfields), dim3(256), 0, s, b.rhs[msg::MGC_MAXL], 0,
+                     b.x[msg::MGC_MAXL], msg::MGC_MAXL, st, sigma, nfields, (double *)nullptr);
+  HIPCHK(hipGetLastError());
+  for (int m = msg::MGC_MAXL + 1; m <= l; ++m) {
+    const int n = 1 << m;
+    const int ntx = cdiv(n, msg::MG_TX), nty = cdiv(n, msg::MG_TY);
+    const S *x2 = (m == l) ? b.x2_top : b.x2[m];
+    const S *rhs = (m == l) ? in : b.rhs[m];
+    const int neg = (m == l) ? negate : 0;
+    S *xo = (m == l) ? out : b.x[m];
+    hipLaunchKernelGGL((msg::k_mg_up<S>), dim3(ntx * nty * nfields), dim3(256), 0, s, x2, b.x[m - 1], rhs, neg, xo, n,
+                       st, sigma, ntx, nty, nfields, (m == l) ? nrm_part : (double *)nullptr);
+    HIPCHK(hipGetLastError());
+  }
+  return MSG_OK;
+}
+
+template <typename S>
+int launch_restrict(const S *fine, S *coarse, int n, int nfields, hipStream_t s) {
+  const size_t cnt = (size_t)(n / 2) * (n / 2) * nfields;
+  hipLaunchKernelGGL((msg::k_restrict<S>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, fine, coarse, n,
+                     nfields);
+  HIPCHK(hipGetLastError());
+  return MSG_OK;
+}
+
+template <typename S>
+int launch_prolong(const S *coarse, S *fine, int nc, int nfields, hipStream_t s) {
+  const size_t cnt = (size_t)(2 * nc) * (2 * nc) * nfields;
+  hipLaunchKernelGGL((msg::k_prolong<S>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, coarse, fine, nc,
+                     nfields);
+  HIPCHK(hipGetLastError());
+  return MSG_OK;
+}
+
+// Sampling positions k * (nc-1)/(n-1) of ResampleCopier, with the step held in long double.
+int resample_coords(int nc, int n, double **dcoord, hipStream_t s) {
+  std::vector<double> c(n);
+  const long double hx = (long double)(nc - 1) / (n - 1);
+  for (int k = 0; k < n; ++k) c[k] = (double)(k * hx);
+  void *p = nullptr;
+  CHK(scratch_get(15, sizeof(double) * n, &p));
+  HIPCHK(hipMemcpyAsync(p, c.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));  // c is a host temporary
+  *dcoord = (double *)p;
+  return MSG_OK;
+}
+
+template <typename S>
+int launch_resample(const S *src, S *dst, int nc, int n, int nfields, const double *coord, hipStream_t s) {
+  const size_t cnt = (size_t)n * n * nfields;
+  hipLaunchKernelGGL((msg::k_resample<S>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, src, dst, nc, n,
+                     coord, nfields);
+  HIPCHK(hipGetLastError());
+  return MSG_OK;
+}
+
+}  // namespace
+
+// =================================================================================================
+// Context
+// =================================================================================================
+struct msg_ctx {
+  int L = 0, B = 0, dtype = MSG_F32;
+  size_t es = 4;
+  void *R[MAXL + 1] = {}, *T[MAXL + 1] = {}, *phi[MAXL + 1] = {};
+  void *DE = nullptr, *z = nullptr, *d = nullptr, *x2 = nullptr;
+  void *mg_rhs[MAXL + 1] = {}, *mg_x2[MAXL + 1] = {}, *mg_x[MAXL + 1] = {};
+  double *part_e = nullptr, *part_n = nullptr, *part_c = nullptr;
+  double *coord = nullptr;
+  PairState *st = nullptr;
+  int *flags_d = nullptr, *flags_h = nullptr;
+  msg::IterTrace *trace_d = nullptr;
+  int trace_cap = 0;
+  hipStream_t stream = nullptr;
+  long long cnt_iter = 0, cnt_evals = 0, cnt_passes = 0;
+  std::vector<void *> allocs;
+};
+
+namespace {
+
+int ctx_alloc(msg_ctx *c, void **p, size_t bytes) {
+  HIPCHK(hipMalloc(p, bytes ? bytes : 16));
+  c->allocs.push_back(*p);
+  return MSG_OK;
+}
+
+template <typename S>
+MGBufs<S> ctx_mg(msg_ctx *c) {
+  MGBufs<S> b;
+  for (int m = 0; m <= MAXL; ++m) {
+    b.rhs[m] = (S *)c->mg_rhs[m];
+    b.x2[m] = (S *)c->mg_x2[m];
+    b.x[m] = (S *)c->mg_x[m];
+  }
+  b.x2_top = (S *)c->x2;
+  return b;
+}
+
+template <typename S>
+int rebuild_pyramid(msg_ctx *c, void **lev, int pair, int nfields_per_pair) {
+  for (int l = c->L; l > 0; --l) {
+    const int n = 1 << l;
+    const size_t fs = (size_t)n * n * nfields_per_pair, cs = (size_t)(n / 2) * (n / 2) * nfields_per_pair;
+    if (pair < 0)
+      CHK(launch_restrict<S>((const S *)lev[l], (S *)lev[l - 1], n, nfields_per_pair * c->B, c->stream));
+    else
+      CHK(launch_restrict<S>((const S *)lev[l] + pair * fs, (S *)lev[l - 1] + pair * cs, n, nfields_per_pair,
+                             c->stream));
+  }
+  return MSG_OK;
+}
+
+int upload_host(msg_ctx *c, void *dst, const double *src, size_t cnt) {
+  if (c->dtype == MSG_F64) {
+    HIPCHK(hipMemcpyAsync(dst, src, cnt * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> tmp(cnt);
+    for (size_t i = 0; i < cnt; ++i) tmp[i] = (float)src[i];
+    HIPCHK(hipMemcpyAsync(dst, tmp.data(), cnt * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return MSG_OK;
+}
+
+int download_host(msg_ctx *c, double *dst, const void *src, size_t cnt) {
+  if (c->dtype == MSG_F64) {
+    HIPCHK(hipMemcpyAsync(dst, src, cnt * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> tmp(cnt);
+    HIPCHK(hipMemcpyAsync(tmp.data(), src, cnt * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < cnt; ++i) dst[i] = tmp[i];
+  }
+  return MSG_OK;
+}
+
+template <typename S>
+int set_image(msg_ctx *c, void **lev, int pair, const double *img) {
+  const size_t nn = (size_t)1 << (2 * c->L);
+  CHK(upload_host(c, (char *)lev[c->L] + pair * nn * c->es, img, nn));
+  return rebuild_pyramid<S>(c, lev, pair, 1);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Batched level solve (GradientDescentBase::applySingle for every pair at once)
+// ------------------------------------------------------------------------------------------------
+template <typename S>
+int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop_eps, int spec, double *energies,
+                     int *iterations, msg_iter_trace *trace, int trace_cap) {
+  const int n = 1 << l, B = c->B;
+  hipStream_t s = c->stream;
+  const S *R = (const S *)c->R[l], *T = (const S *)c->T[l];
+  S *phi = (S *)c->phi[l];
+  S *DE = (S *)c->DE, *z = (S *)c->z, *d = (S *)c->d;
+  const MGBufs<S> mg = ctx_mg<S>(c);
+  const EnergyGeom eg = energy_geom(n);
+  const int nblk_norm = mg_norm_blocks(l);
+  const int nblk_ncg = cdiv(n * n, 256 * msg::NCG_PER_THREAD);
+  const int nblk_upd = cdiv(2 * n * n, 256 * msg::NCG_PER_THREAD);
+  if (spec <= 0) spec = 3;
+  if (spec > msg::MAX_CAND) spec = msg::MAX_CAND;
+
+  msg::IterTrace *tr = nullptr;
+  int tcap = 0;
+  if (trace && trace_cap > 0) {
+    if (c->trace_cap < trace_cap * B) {
+      if (c->trace_d) HIPCHK(hipFree(c->trace_d));
+      HIPCHK(hipMalloc(&c->trace_d, sizeof(msg::IterTrace) * trace_cap * B));
+      c->trace_cap = trace_cap * B;
+    }
+    tr = c->trace_d;
+    tcap = trace_cap;
+    HIPCHK(hipMemsetAsync(tr, 0, sizeof(msg::IterTrace) * trace_cap * B, s));
+  }
+
+  msg::CtrlArgs a;
+  a.nblk_energy = eg.nblk;
+  a.nblk_norm = nblk_norm;
+  a.nblk_ncg = nblk_ncg;
+  a.lambda = lambda;
+  a.stop_eps = stop_eps;
+  a.max_iter = max_iter;
+  a.spec = spec;
+  a.trace_cap = tcap;
+
+  hipLaunchKernelGGL(k_state_init, dim3(cdiv(B, 64)), dim3(64), 0, s, c->st, B, lambda);
+  HIPCHK(hipGetLastError());
+  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s));
+  a.phase = msg::PH_LEVEL_INIT;
+  hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
+  HIPCHK(hipGetLastError());
+
+  bool any_active = max_iter > 0;
+  while (any_active) {
+    CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s));
+    CHK(launch_vcycle<S>(DE, 1, z, l, mg, c->st, 1.0, 2 * B, c->part_n, s));
+    a.phase = msg::PH_BETA;
+    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL((msg::k_ncg<S>), dim3(nblk_ncg * 2 * B), dim3(256), 0, s, z, d, DE, n, c->st, c->part_c, nblk_ncg,
+                       2 * B);
+    HIPCHK(hipGetLastError());
+    a.phase = msg::PH_ARMIJO_INIT;
+    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
+    HIPCHK(hipGetLastError());
+    bool searching = true;
+    int rounds = 0;
+    while (searching) {
+      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s));
+      a.phase = msg::PH_ARMIJO_STEP;
+      hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d,
+                         tr);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(c->flags_h, c->flags_d, sizeof(int) * B, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      searching = false;
+      any_active = false;
+      for (int p = 0; p < B; ++p) {
+        searching |= (c->flags_h[p] & 1) != 0;
+        any_active |= (c->flags_h[p] & 2) != 0;
+      }
+      if (++rounds > 200) return fail(MSG_ERR_STATE, "Armijo search did not terminate");
+    }
+    hipLaunchKernelGGL((msg::k_update<S>), dim3(nblk_upd * B), dim3(256), 0, s, phi, d, z, n, c->st, nblk_upd, B);
+    HIPCHK(hipGetLastError());
+  }
+
+  std::vector<PairState> hs(B);
+  HIPCHK(hipMemcpyAsync(hs.data(), c->st, sizeof(PairState) * B, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int p = 0; p < B; ++p) {
+    if (energies) energies[p] = hs[p].energy;
+    if (iterations) iterations[p] = hs[p].iter;
+    c->cnt_iter += hs[p].iter;
+    c->cnt_evals += hs[p].total_evals;
+    c->cnt_passes += hs[p].total_passes;
+  }
+  if (tr) {
+    static_assert(sizeof(msg::IterTrace) == sizeof(msg_iter_trace), "trace layout");
+    HIPCHK(hipMemcpyAsync(trace, tr, sizeof(msg::IterTrace) * trace_cap * B, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  return MSG_OK;
+}
+
+template <typename S>
+int prolong_deformation(msg_ctx *c, int l, int resample) {
+  const int nc = 1 << l, n = 2 * nc;
+  if (resample) {
+    double *coord = nullptr;
+    CHK(resample_coords(nc, n, &coord, c->stream));
+    CHK(launch_resample<S>((const S *)c->phi[l], (S *)c->phi[l + 1], nc, n, 2 * c->B, coord, c->stream));
+  } else {
+    CHK(launch_prolong<S>((const S *)c->phi[l], (S *)c->phi[l + 1], nc, 2 * c->B, c->stream));
+  }
+  return MSG_OK;
+}
+
+template <typename S>
+int solve_impl(msg_ctx *c, int start, int stop, const msg_solve_params *p, double *energies) {
+  std::vector<double> e(c->B, NAN);
+  for (int level = start; level <= stop; ++level) {
+    const double f = p->lambda_factor > 0 ? (double)std::pow(p->lambda_factor, p->lambda_ref_level - level) : 1;
+    const double lam = f * p->lambda;
+    CHK(solve_level_impl<S>(c, level, lam, p->max_iter, p->stop_eps, p->spec, e.data(), nullptr, nullptr, 0));
+    if (level < stop) CHK(prolong_deformation<S>(c, level, p->resample_prolong));
+  }
+  for (int l = stop; l < c->L; ++l) CHK(prolong_deformation<S>(c, l, p->resample_prolong));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (energies)
+    for (int i = 0; i < c->B; ++i) energies[i] = e[i];
+  return MSG_OK;
+}
+
+}  // namespace
+
+// =================================================================================================
+// C ABI
+// =================================================================================================
+extern "C" {
+
+const char *msg_last_error(void) { return g_err.c_str(); }
+const char *msg_version(void) { return "msgpu 0.1 (gfx950)"; }
+
+int msg_ctx_create(msg_ctx **out, int max_level, int npairs, int dtype) {
+  if (!out) return fail(MSG_ERR_ARG, "null context pointer");
+  *out = nullptr;
+  CHK(check_dtype(dtype));
+  if (max_level < 1 || max_level > MAXL) return fail(MSG_ERR_ARG, "max_level out of range");
+  if (npairs < 1) return fail(MSG_ERR_ARG, "npairs must be positive");
+  msg_ctx *c = new msg_ctx();
+  c->L = max_level;
+  c->B = npairs;
+  c->dtype = dtype;
+  c->es = esz(dtype);
+  const size_t B = npairs;
+  int rc = MSG_OK;
+  auto A = [&](void **p, size_t bytes) {
+    if (rc == MSG_OK) rc = ctx_alloc(c, p, bytes);
+  };
+  for (int l = 0; l <= max_level; ++l) {
+    const size_t nn = (size_t)1 << (2 * l);
+    A(&c->R[l], B * nn * c->es);
+    A(&c->T[l], B * nn * c->es);
+    A(&c->phi[l], B * 2 * nn * c->es);
+    if (l < max_level && l <= MAXL) {
+      A(&c->mg_rhs[l], B * 2 * nn * c->es);
+      A(&c->mg_x2[l], B * 2 * nn * c->es);
+      A(&c->mg_x[l], B * 2 * nn * c->es);
+    }
+  }
+  const size_t NN = (size_t)1 << (2 * max_level);
+  A(&c->DE, B * 2 * NN * c->es);
+  A(&c->z, B * 2 * NN * c->es);
+  A(&c->d, B * 2 * NN * c->es);
+  A(&c->x2, B * 2 * NN * c->es);
+  const int N = 1 << max_level;
+  const EnergyGeom eg = energy_geom(N);
+  A((void **)&c->part_e, sizeof(double) * B * msg::MAX_CAND * 2 * eg.nblk);
+  A((void **)&c->part_n, sizeof(double) * B * 2 * std::max(1, mg_norm_blocks(max_level)));
+  A((void **)&c->part_c, sizeof(double) * B * 2 * 2 * cdiv(N * N, 256 * msg::NCG_PER_THREAD));
+  A((void **)&c->st, sizeof(PairState) * B);
+  A((void **)&c->flags_d, sizeof(int) * B);
+  if (rc != MSG_OK) {
+    msg_ctx_destroy(c);
+    return rc;
+  }
+  if (hipHostMalloc((void **)&c->flags_h, sizeof(int) * B, hipHostMallocDefault) != hipSuccess) {
+    msg_ctx_destroy(c);
+    return fail(MSG_ERR_ALLOC, "pinned allocation failed");
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    msg_ctx_destroy(c);
+    return fail(MSG_ERR_HIP, "stream creation failed");
+  }
+  // zero images and deformations so that a context is usable before it is filled
+  for (int l = 0; l <= max_level; ++l) {
+    const size_t nn = (size_t)1 << (2 * l);
+    (void)hipMemsetAsync(c->R[l], 0, B * nn * c->es, c->stream);
+    (void)hipMemsetAsync(c->T[l], 0, B * nn * c->es, c->stream);
+    (void)hipMemsetAsync(c->phi[l], 0, B * 2 * nn * c->es, c->stream);
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    msg_ctx_destroy(c);
+    return fail(MSG_ERR_HIP, "context initialisation failed");
+  }
+  *out = c;
+  return MSG_OK;
+}
+
+int msg_ctx_destroy(msg_ctx *c) {
+  if (!c) return MSG_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void *p : c->allocs) (void)hipFree(p);
+  if (c->trace_d) (void)hipFree(c->trace_d);
+  if (c->flags_h) (void)hipHostFree(c->flags_h);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return MSG_OK;
+}
+
+#define DISPATCH(c, fn, ...) ((c)->dtype == MSG_F64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+static int check_pair(msg_ctx *c, int pair) {
+  if (!c) return fail(MSG_ERR_ARG, "null context");
+  if (pair < 0 || pair >= c->B) return fail(MSG_ERR_ARG, "pair index out of range");
+  return MSG_OK;
+}
+
+int msg_set_reference(msg_ctx *c, int pair, const double *img) {
+  CHK(check_pair(c, pair));
+  if (!img) return fail(MSG_ERR_ARG, "null image");
+  return DISPATCH(c, set_image, c, c->R, pair, img);
+}
+
+int msg_set_template(msg_ctx *c, int pair, const double *img) {
+  CHK(check_pair(c, pair));
+  if (!img) return fail(MSG_ERR_ARG, "null image");
+  return DISPATCH(c, set_image, c, c->T, pair, img);
+}
+
+int msg_set_images_device(msg_ctx *c, const void *d_ref, const void *d_tpl, int dtype) {
+  if (!c) return fail(MSG_ERR_ARG, "null context");
+  CHK(check_dtype(dtype));
+  const size_t cnt = ((size_t)1 << (2 * c->L)) * c->B;
+  if (d_ref) {
+    CHK(convert(d_ref, dtype, c->R[c->L], c->dtype, cnt, c->stream));
+    CHK(DISPATCH(c, rebuild_pyramid, c, c->R, -1, 1));
+  }
+  if (d_tpl) {
+    CHK(convert(d_tpl, dtype, c->T[c->L], c->dtype, cnt, c->stream));
+    CHK(DISPATCH(c, rebuild_pyramid, c, c->T, -1, 1));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MSG_OK;
+}
+
+int msg_set_deformation(msg_ctx *c, int pair, const double *phi) {
+  CHK(check_pair(c, pair));
+  if (!phi) return fail(MSG_ERR_ARG, "null field");
+  const size_t nn = (size_t)1 << (2 * c->L);
+  CHK(upload_host(c, (char *)c->phi[c->L] + pair * 2 * nn * c->es, phi, 2 * nn));
+  CHK(DISPATCH(c, rebuild_pyramid, c, c->phi, pair, 2));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MSG_OK;
+}
+
+int msg_get_deformation(msg_ctx *c, int pair, double *phi) {
+  CHK(check_pair(c, pair));
+  if (!phi) return fail(MSG_ERR_ARG, "null field");
+  const size_t nn = (size_t)1 << (2 * c->L);
+  return download_host(c, phi, (char *)c->phi[c->L] + pair * 2 * nn * c->es, 2 * nn);
+}
+
+int msg_set_deformation_zero(msg_ctx *c, int pair) {
+  CHK(check_pair(c, pair));
+  for (int l = 0; l <= c->L; ++l) {
+    const size_t nn = (size_t)1 << (2 * l);
+    HIPCHK(hipMemsetAsync((char *)c->phi[l] + pair * 2 * nn * c->es, 0, 2 * nn * c->es, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MSG_OK;
+}
+
+int msg_set_deformations_device(msg_ctx *c, const void *d_phi) {
+  if (!c || !d_phi) return fail(MSG_ERR_ARG, "null argument");
+  const size_t cnt = ((size_t)2 << (2 * c->L)) * c->B;
+  HIPCHK(hipMemcpyAsync(c->phi[c->L], d_phi, cnt * c->es, hipMemcpyDeviceToDevice, c->stream));
+  CHK(DISPATCH(c, rebuild_pyramid, c, c->phi, -1, 2));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MSG_OK;
+}
+
+int msg_get_deformations_device(msg_ctx *c, void *d_phi) {
+  if (!c || !d_phi) return fail(MSG_ERR_ARG, "null argument");
+  const size_t cnt = ((size_t)2 << (2 * c->L)) * c->B;
+  HIPCHK(hipMemcpyAsync(d_phi, c->phi[c->L], cnt * c->es, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MSG_OK;
+}
+
+int msg_solve_level(msg_ctx *c, int level, double lambda, int max_iter, double stop_eps, int spec, double *energies,
+                    int *iterations, msg_iter_trace *trace, int trace_cap) {
+  if (!c) return fail(MSG_ERR_ARG, "null context");
+  if (level < 1 || level > c->L) return fail(MSG_ERR_ARG, "level out of range");
+  c->cnt_iter = c->cnt_evals = c->cnt_passes = 0;
+  return DISPATCH(c, solve_level_impl, c, level, lambda, max_iter, stop_eps, spec, energies, iterations, trace,
+                  trace_cap);
+}
+
+int msg_solve(msg_ctx *c, int start, int stop, const msg_solve_params *p, double *energies) {
+  if (!c || !p) return fail(MSG_ERR_ARG, "null argument");
+  if (start < 1 || stop > c->L || start > stop) return fail(MSG_ERR_ARG, "invalid level range");
+  c->cnt_iter = c->cnt_evals = c->cnt_passes = 0;
+  return DISPATCH(c, solve_impl, c, start, stop, p, energies);
+}
+
+int msg_last_counts(msg_ctx *c, long long *iterations, long long *evals, long long *passes) {
+  if (!c) return fail(MSG_ERR_ARG, "null context");
+  if (iterations) *iterations = c->cnt_iter;
+  if (evals) *evals = c->cnt_evals;
+  if (passes) *passes = c->cnt_passes;
+  return MSG_OK;
+}
+
+int msg_ctx_buffer(msg_ctx *c, int which, int level, void **ptr) {
+  if (!c || !ptr) return fail(MSG_ERR_ARG, "null argument");
+  if (level < 0 || level > c->L) return fail(MSG_ERR_ARG, "level out of range");
+  if (which == 0) *ptr = c->R[level];
+  else if (which == 1) *ptr = c->T[level];
+  else if (which == 2) *ptr = c->phi[level];
+  else return fail(MSG_ERR_ARG, "unknown buffer");
+  return MSG_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Stateless operator entry points
+// ---------------------------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+template <typename S>
+int energy_impl(const void *R, const void *T, const void *phi, int n, int batch, double *e_ssd,
+                       double *e_reg, hipStream_t s) {
+  const EnergyGeom g = energy_geom(n);
+  void *stp = nullptr, *part = nullptr;
+  CHK(scratch_get(0, sizeof(PairState) * batch, &stp));
+  CHK(scratch_get(1, sizeof(double) * batch * msg::MAX_CAND * 2 * g.nblk, &part));
+  hipLaunchKernelGGL(k_state_init, dim3(cdiv(batch, 64)), dim3(64), 0, s, (PairState *)stp, batch, 0.0);
+  HIPCHK(hipGetLastError());
+  CHK(launch_energy<S>((const S *)R, (const S *)T, (const S *)phi, (const S *)phi, (const S *)phi, n,
+                       (const PairState *)stp, (double *)part, batch, s));
+  std::vector<double> hp((size_t)batch * msg::MAX_CAND * 2 * g.nblk);
+  HIPCHK(hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int p = 0; p < batch; ++p) {
+    double a = 0, b = 0;
+    const double *pp = hp.data() + (size_t)p * msg::MAX_CAND * 2 * g.nblk;
+    for (int i = 0; i < g.nblk; ++i) a += pp[i];
+    for (int i = 0; i < g.nblk; ++i) b += pp[g.nblk + i];
+    if (e_ssd) e_ssd[p] = a;
+    if (e_reg) e_reg[p] = b;
+  }
+  return MSG_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int msg_energy(int dtype, const void *R, const void *T, const void *phi, int n, int batch, double *e_ssd,
+               double *e_reg, void *stream) {
+  CHK(check_dtype(dtype));
+  CHK(check_n(n));
+  if (!R || !T || !phi || batch < 1) return fail(MSG_ERR_ARG, "invalid argument");
+  return dtype == MSG_F64 ? energy_impl<double>(R, T, phi, n, batch, e_ssd, e_reg, strm(stream))
+                          : energy_impl<float>(R, T, phi, n, batch, e_ssd, e_reg, strm(stream));
+}
+
+int msg_gradient(int dtype, const void *R, const void *T, const void *phi, void *DE, int n, int batch, double lambda,
+                 void *stream) {
+  CHK(check_dtype(dtype));
+  CHK(check_n(n));
+  if (!R || !T || !phi || !DE || batch < 1) return fail(MSG_ERR_ARG, "invalid argument");
+  if (dtype == MSG_F64)
+    CHK(launch_grad<double>((const double *)R, (const double *)T, (const double *)phi, (double *)DE, n, nullptr,
+                            lambda, batch, strm(stream)));
+  else
+    CHK(launch_grad<float>((const float *)R, (const float *)T, (const float *)phi, (float *)DE, n, nullptr, lambda,
+                           batch, strm(stream)));
+  HIPCHK(hipStreamSynchronize(strm(stream)));
+  return MSG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+template <typename S>
+int h1_impl(const void *in, void *out, int n, int nfields, double sigma, hipStream_t s) {
+  const int l = log2i(n);
+  MGBufs<S> b;
+  for (int m = 0; m <= MAXL; ++m) b.rhs[m] = b.x2[m] = b.x[m] = nullptr;
+  for (int m = 0; m < l; ++m) {
+    const size_t cnt = ((size_t)1 << (2 * m)) * nfields;
+    void *p1, *p2, *p3;
+    CHK(scratch_get(20 + 3 * m, cnt * sizeof(S), &p1));
+    CHK(scratch_get(21 + 3 * m, cnt * sizeof(S), &p2));
+    CHK(scratch_get(22 + 3 * m, cnt * sizeof(S), &p3));
+    b.rhs[m] = (S *)p1;
+    b.x2[m] = (S *)p2;
+    b.x[m] = (S *)p3;
+  }
+  void *top, *tmpout;
+  CHK(scratch_get(2, (size_t)n * n * nfields * sizeof(S), &top));
+  CHK(scratch_get(3, (size_t)n * n * nfields * sizeof(S), &tmpout));
+  b.x2_top = (S *)top;
+  // write into a temporary so that in == out is allowed
+  CHK(launch_vcycle<S>((const S *)in, 0, (S *)tmpout, l, b, nullptr, sigma, nfields, nullptr, s));
+  HIPCHK(hipMemcpyAsync(out, tmpout, (size_t)n * n * nfields * sizeof(S), hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSG_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int msg_h1_smooth(int dtype, const void *in, void *out, int n, int nfields, double sigma, void *stream) {
+  CHK(check_dtype(dtype));
+  CHK(check_n(n));
+  if (!in || !out || nfields < 1) return fail(MSG_ERR_ARG, "invalid argument");
+  return dtype == MSG_F64 ? h1_impl<double>(in, out, n, nfields, sigma, strm(stream))
+                          : h1_impl<float>(in, out, n, nfields, sigma, strm(stream));
+}
+
+int msg_warp(int dtype, const void *img, const void *phi, void *out, int n, int nimg, int phi_shared, int mode,
+             double ext, int nearest, void *stream) {
+  CHK(check_dtype(dtype));
+  CHK(check_n(n));
+  if (!img || !phi || !out || nimg < 1 || mode < 0 || mode > 2) return fail(MSG_ERR_ARG, "invalid argument");
+  const size_t cnt = (size_t)n * n * nimg;
+  const dim3 grid((unsigned)((cnt + 255) / 256));
+  if (dtype == MSG_F64)
+    hipLaunchKernelGGL((msg::k_warp<double>), grid, dim3(256), 0, strm(stream), (const double *)img,
+                       (const double *)phi, (double *)out, n, nimg, phi_shared, mode, ext, nearest);
+  else
+    hipLaunchKernelGGL((msg::k_warp<float>), grid, dim3(256), 0, strm(stream), (const float *)img, (const float *)phi,
+                       (float *)out, n, nimg, phi_shared, mode, ext, nearest);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(strm(stream)));
+  return MSG_OK;
+}
+
+int msg_restrict(int dtype, const void *fine, void *coarse, int n, int nfields, void *stream) {
+  CHK(check_dtype(dtype));
+  CHK(check_n(n));
+  if (!fine || !coarse || nfields < 1) return fail(MSG_ERR_ARG, "invalid argument");
+  if (dtype == MSG_F64) CHK(launch_restrict<double>((const double *)fine, (double *)coarse, n, nfields, strm(stream)));
+  else CHK(launch_restrict<float>((const float *)fine, (float *)coarse, n, nfields, strm(stream)));
+  HIPCHK(hipStreamSynchronize(strm(stream)));
+  return MSG_OK;
+}
+
+int msg_prolong(int dtype, const void *coarse, void *fine, int nc, int nfields, void *stream) {
+  CHK(check_dtype(dtype));
+  CHK(check_n(2 * nc));
+  if (!fine || !coarse || nfields < 1) return fail(MSG_ERR_ARG, "invalid argument");
+  if (dtype == MSG_F64) CHK(launch_prolong<double>((const double *)coarse, (double *)fine, nc, nfields, strm(stream)));
+  else CHK(launch_prolong<float>((const float *)coarse, (float *)fine, nc, nfields, strm(stream)));
+  HIPCHK(hipStreamSynchronize(strm(stream)));
+  return MSG_OK;
+}
+
+int msg_resample(int dtype, const void *src, void *dst, int nc, int n, int nfields, void *stream) {
+  CHK(check_dtype(dtype));
+  CHK(check_n(n));
+  CHK(check_n(nc));
+  if (!src || !dst || nfields < 1) return fail(MSG_ERR_ARG, "invalid argument");
+  double *coord = nullptr;
+  CHK(resample_coords(nc, n, &coord, strm(stream)));
+  if (dtype == MSG_F64)
+    CHK(launch_resample<double>((const double *)src, (double *)dst, nc, n, nfields, coord, strm(stream)));
+  else
+    CHK(launch_resample<float>((const float *)src, (float *)dst, nc, n, nfields, coord, strm(stream)));
+  HIPCHK(hipStreamSynchronize(strm(stream)));
+  return MSG_OK;
+}
+
+}  // extern "C"
+
+// =================================================================================================
+// Composition, statistics, normalisation, series matching
+// =================================================================================================
+namespace {
+
+template <typename S>
+int compose_impl(const void *phi1, const void *phi2, void *result, int n, hipStream_t s) {
+  const int nn = n * n;
+  const int nblk = cdiv(nn, 256);
+  void *bc, *rhs, *invd, *x, *g, *hb, *d, *def, *part, *part2, *scp;
+  CHK(scratch_get(70, sizeof(double) * 2 * nn, &bc));
+  CHK(scratch_get(71, sizeof(double) * 2 * nn, &rhs));
+  CHK(scratch_get(72, sizeof(double) * nn, &invd));
+  CHK(scratch_get(73, sizeof(double) * 2 * nn, &x));
+  CHK(scratch_get(74, sizeof(double) * 2 * nn, &g));
+  CHK(scratch_get(75, sizeof(double) * 2 * nn, &hb));
+  CHK(scratch_get(76, sizeof(double) * 2 * nn, &d));
+  CHK(scratch_get(77, nn, &def));
+  CHK(scratch_get(78, sizeof(double) * 4 * nblk, &part));
+  CHK(scratch_get(79, sizeof(double) * 2 * nblk, &part2));
+  CHK(scratch_get(80, sizeof(msg::PcgScalars) * 2, &scp));
+  msg::PcgScalars *sc = (msg::PcgScalars *)scp;
+  const dim3 g1(nblk), g2(nblk, 2), b(256);
+  hipLaunchKernelGGL((msg::k_compose_init<S>), g1, b, 0, s, (const S *)phi1, (const S *)phi2, (double *)bc,
+                     (unsigned char *)def, n);
+  hipLaunchKernelGGL(msg::k_compose_rhs, dim3(cdiv(2 * nn, 256)), b, 0, s, (const double *)bc,
+                     (const unsigned char *)def, (double *)rhs, (double *)invd, n);
+  CHK(convert(result, sizeof(S) == 8 ? MSG_F64 : MSG_F32, x, MSG_F64, 2 * (size_t)nn, s));
+  hipLaunchKernelGGL((msg::k_pcg_start<S>), g2, b, 0, s, (const S *)nullptr, (double *)x, (const double *)rhs,
+                     (const unsigned char *)def, (const double *)invd, (double *)g, (double *)hb, (double *)d,
+                     (double *)part, n, nblk);
+  hipLaunchKernelGGL(msg::k_dot_blocks, g2, b, 0, s, (const double *)g, (const double *)hb, (double *)part2, n, nblk);
+  hipLaunchKernelGGL(msg::k_pcg_scalar, dim3(2), b, 0, s, sc, (const double *)part, nblk, 0, (const double *)part2);
+  HIPCHK(hipGetLastError());
+  msg::PcgScalars hsc[2];
+  for (int it = 0; it < 1000;) {
+    HIPCHK(hipMemcpyAsync(hsc, sc, sizeof(hsc), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hsc[0].done && hsc[1].done) break;
+    const int chunk = 8;
+    for (int k = 0; k < chunk; ++k, ++it) {
+      hipLaunchKernelGGL(msg::k_pcg_apply, g2, b, 0, s, (const double *)d, (const unsigned char *)def, (double *)hb,
+                         (double *)part, (const msg::PcgScalars *)sc, n, nblk);
+      hipLaunchKernelGGL(msg::k_pcg_scalar, dim3(2), b, 0, s, sc, (const double *)part, nblk, 1, (const double *)nullptr);
+      hipLaunchKernelGGL(msg::k_pcg_update, g2, b, 0, s, (double *)x, (const double *)d, (double *)g, (double *)hb,
+                         (const double *)invd, (double *)part, (const msg::PcgScalars *)sc, n, nblk);
+      hipLaunchKernelGGL(msg::k_pcg_scalar, dim3(2), b, 0, s, sc, (const double *)part, nblk, 2, (const double *)nullptr);
+      hipLaunchKernelGGL(msg::k_pcg_dir, g2, b, 0, s, (double *)d, (const double *)hb, (const msg::PcgScalars *)sc, n);
+    }
+    HIPCHK(hipGetLastError());
+  }
+  hipLaunchKernelGGL((msg::k_store<S>), dim3(cdiv(2 * nn, 256)), b, 0, s, (const double *)x, (S *)result, 2 * nn);
+  HIPCHK(hipGetLastError());
+  return MSG_OK;
+}
+
+template <typename S>
+int stats_impl(const void *stack, int nimg, int n, double fill, void *mean, void *count, void *median, hipStream_t s) {
+  const int nn = n * n;
+  const size_t lds = sizeof(double) * 64 * nimg;
+  hipLaunchKernelGGL((msg::k_series_stats<S>), dim3(cdiv(nn, 64)), dim3(64), lds, s, (const S *)stack, nimg, nn, fill,
+                     (S *)mean, (S *)count, (S *)median);
+  HIPCHK(hipGetLastError());
+  return MSG_OK;
+}
+
+template <typename S>
+int normalize_impl(void *img, int n, int nimg, double pct, int min_to_zero, hipStream_t s) {
+  const int nn = n * n;
+  void *sorted, *tmp;
+  CHK(scratch_get(81, sizeof(S) * nn, &sorted));
+  size_t tbytes = 0;
+  HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, tbytes, (const S *)img, (S *)sorted, nn, 0, sizeof(S) * 8, s));
+  CHK(scratch_get(82, tbytes, &tmp));
+  const int m1 = nn - 1;
+  int i0 = 0, i1 = m1;
+  if (pct >= 0) {
+    i0 = (int)(pct / 200 * m1);
+    i1 = (int)((1 - pct / 200) * m1);
+    i0 = std::min(std::max(i0, 0), m1);
+    i1 = std::min(std::max(i1, 0), m1);
+  }
+  for (int k = 0; k < nimg; ++k) {
+    S *im = (S *)img + (size_t)k * nn;
+    HIPCHK(hipcub::DeviceRadixSort::SortKeys(tmp, tbytes, (const S *)im, (S *)sorted, nn, 0, sizeof(S) * 8, s));
+    hipLaunchKernelGGL((msg::k_normalize<S>), dim3(cdiv(nn, 256)), dim3(256), 0, s, im, nn, (const S *)sorted, i0, i1,
+                       pct >= 0 ? 1 : 0, min_to_zero);
+    HIPCHK(hipGetLastError());
+  }
+  return MSG_OK;
+}
+
+template <typename S>
+__global__ void k_sum_blocks(const S *__restrict__ a, int nn, double *__restrict__ part) {
+  __shared__ double red[4];
+  double v = 0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < nn; i += gridDim.x * 256) v += msg::ld(a, i);
+  const double s = msg::block_sum<256>(v, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+template <typename S>
+int mean_value(const void *img, int nn, double *out, hipStream_t s) {
+  void *part;
+  const int nb = 256;
+  CHK(scratch_get(83, sizeof(double) * nb, &part));
+  hipLaunchKernelGGL((k_sum_blocks<S>), dim3(nb), dim3(256), 0, s, (const S *)img, nn, (double *)part);
+  HIPCHK(hipGetLastError());
+  double hp[256];
+  HIPCHK(hipMemcpyAsync(hp, part, sizeof(hp), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  double t = 0;
+  for (int i = 0; i < nb; ++i) t += hp[i];
+  *out = t / nn;
+  return MSG_OK;
+}
+
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+template <typename S>
+int match_series_impl(const msg_series_params *p, const void *templates, int n, const void *reference,
+                      const void *pairwise_in, void *deformations, void *pairwise_out, double *energies, void *mean,
+                      void *count, void *median) {
+  const int L = p->precision_level;
+  const int N = 1 << L;
+  const size_t nn = (size_t)N * N;
+  const int dt = sizeof(S) == 8 ? MSG_F64 : MSG_F32;
+  const S *tpl = (const S *)templates, *ref = (const S *)reference;
+  msg_solve_params sp;
+  sp.lambda = p->lambda;
+  sp.lambda_factor = p->lambda_factor;
+  sp.lambda_ref_level = p->stop_level;
+  sp.max_iter = p->max_iter;
+  sp.stop_eps = p->stop_eps;
+  sp.resample_prolong = p->resample_prolong;
+  sp.spec = p->spec;
+  const bool reuse = pairwise_in != nullptr;
+  const int npw = reuse ? 1 : n;  // pairwise solves to run (pairs 0..npw-1)
+
+  // --- batched pairwise solves: template i against its predecessor (or the reference) ---
+  msg_ctx *pc = nullptr;
+  CHK(msg_ctx_create(&pc, L, npw, dt));
+  struct Guard {
+    msg_ctx *c;
+    ~Guard() { msg_ctx_destroy(c); }
+  } gpc{pc};
+  hipStream_t s = pc->stream;
+  for (int i = 0; i < npw; ++i) {
+    const S *r = (i == 0 || p->dont_accumulate) ? ref : tpl + (size_t)(i - 1) * nn;
+    HIPCHK(hipMemcpyAsync((S *)pc->R[L] + i * nn, r, nn * sizeof(S), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync((S *)pc->T[L] + i * nn, tpl + (size_t)i * nn, nn * sizeof(S), hipMemcpyDeviceToDevice, s));
+  }
+  if (!p->dont_normalize) {
+    CHK(normalize_impl<S>(pc->R[L], N, npw, p->saturation_pct, p->normalize_min_to_zero, s));
+    CHK(normalize_impl<S>(pc->T[L], N, npw, p->saturation_pct, p->normalize_min_to_zero, s));
+  }
+  CHK(rebuild_pyramid<S>(pc, pc->R, -1, 1));
+  CHK(rebuild_pyramid<S>(pc, pc->T, -1, 1));
+  std::vector<double> epw(npw);
+  CHK(solve_impl<S>(pc, p->start_level, p->stop_level, &sp, epw.data()));
+
+  // pairwise deformation of every template
+  DevBuf pw;
+  HIPCHK(hipMalloc(&pw.p, sizeof(S) * 2 * nn * n));
+  S *pwp = (S *)pw.p;
+  HIPCHK(hipMemcpyAsync(pwp, pc->phi[L], sizeof(S) * 2 * nn * npw, hipMemcpyDeviceToDevice, s));
+  if (reuse && n > 1)
+    HIPCHK(hipMemcpyAsync(pwp + 2 * nn, (const S *)pairwise_in + 2 * nn, sizeof(S) * 2 * nn * (n - 1),
+                          hipMemcpyDeviceToDevice, s));
+  if (pairwise_out) HIPCHK(hipMemcpyAsync(pairwise_out, pwp, sizeof(S) * 2 * nn * n, hipMemcpyDeviceToDevice, s));
+  for (int i = 0; i < n; ++i) energies[i] = (i < npw) ? epw[i] : NAN;
+  S *defs = (S *)deformations;
+
+  if (p->dont_accumulate) {
+    HIPCHK(hipMemcpyAsync(defs, pwp, sizeof(S) * 2 * nn * n, hipMemcpyDeviceToDevice, s));
+  } else {
+    // --- sequential chain: accumulate by composition, refine against the reference ---
+    msg_ctx *rc = nullptr;
+    CHK(msg_ctx_create(&rc, L, 1, dt));
+    Guard grc{rc};
+    HIPCHK(hipStreamSynchronize(s));
+    hipStream_t rs = rc->stream;
+    HIPCHK(hipMemcpyAsync(rc->R[L], ref, nn * sizeof(S), hipMemcpyDeviceToDevice, rs));
+    if (!p->dont_normalize) CHK(normalize_impl<S>(rc->R[L], N, 1, p->saturation_pct, p->normalize_min_to_zero, rs));
+    CHK(rebuild_pyramid<S>(rc, rc->R, -1, 1));
+    DevBuf acc;
+    HIPCHK(hipMalloc(&acc.p, sizeof(S) * 2 * nn));
+    HIPCHK(hipMemsetAsync(acc.p, 0, sizeof(S) * 2 * nn, rs));
+    DevBuf prev;
+    HIPCHK(hipMalloc(&prev.p, sizeof(S) * 2 * nn));
+    for (int i = 0; i < n; ++i) {
+      HIPCHK(hipMemcpyAsync(prev.p, acc.p, sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, rs));
+      CHK(compose_impl<S>(pwp + (size_t)i * 2 * nn, prev.p, acc.p, N, rs));
+      if (i > 0) {
+        HIPCHK(hipMemcpyAsync(rc->T[L], tpl + (size_t)i * nn, nn * sizeof(S), hipMemcpyDeviceToDevice, rs));
+        if (!p->dont_normalize)
+          CHK(normalize_impl<S>(rc->T[L], N, 1, p->saturation_pct, p->normalize_min_to_zero, rs));
+        CHK(rebuild_pyramid<S>(rc, rc->T, -1, 1));
+        HIPCHK(hipMemcpyAsync(rc->phi[L], acc.p, sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, rs));
+        CHK(rebuild_pyramid<S>(rc, rc->phi, -1, 2));
+        double e = 0;
+        CHK(solve_impl<S>(rc, p->refine_start_level, p->refine_stop_level, &sp, &e));
+        energies[i] = e;
+        HIPCHK(hipMemcpyAsync(acc.p, rc->phi[L], sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, rs));
+      }
+      HIPCHK(hipMemcpyAsync(defs + (size_t)i * 2 * nn, (i > 0) ? acc.p : (const void *)pwp, sizeof(S) * 2 * nn,
+                            hipMemcpyDeviceToDevice, rs));
+    }
+    HIPCHK(hipStreamSynchronize(rs));
+  }
+
+  // --- reconstruction: warp the raw templates (Inf outside), then mean / count / median ---
+  DevBuf stack;
+  HIPCHK(hipMalloc(&stack.p, sizeof(S) * nn * std::max(n, 1)));
+  if (n > 0) {
+    const size_t cnt = nn * n;
+    hipLaunchKernelGGL((msg::k_warp<S>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, tpl, (const S *)defs,
+                       (S *)stack.p, N, n, 0, 0, (double)INFINITY, 0);
+    HIPCHK(hipGetLastError());
+  }
+  double fill = 0;
+  if (n > 0) CHK(mean_value<S>(tpl + (size_t)(n - 1) * nn, (int)nn, &fill, s));
+  if (n > msg::ST_MAXF) return fail(MSG_ERR_ARG, "series longer than 128 frames are not supported by the median kernel");
+  CHK(stats_impl<S>(stack.p, n, N, fill, mean, count, median, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int msg_compose(int dtype, const void *phi1, const void *phi2, void *result, int n, void *stream) {
+  CHK(check_dtype(dtype));
+  CHK(check_n(n));
+  if (!phi1 || !phi2 || !result) return fail(MSG_ERR_ARG, "invalid argument");
+  if (dtype == MSG_F64) CHK(compose_impl<double>(phi1, phi2, result, n, strm(stream)));
+  else CHK(compose_impl<float>(phi1, phi2, result, n, strm(stream)));
+  HIPCHK(hipStreamSynchronize(strm(stream)));
+  return MSG_OK;
+}
+
+int msg_series_stats(int dtype, const void *stack, int nimg, int n, double fill, void *mean, void *count,
+                     void *median, void *stream) {
+  CHK(check_dtype(dtype));
+  CHK(check_n(n));
+  if (!stack || !mean || !count || !median || nimg < 0 || nimg > msg::ST_MAXF)
+    return fail(MSG_ERR_ARG, "invalid argument (at most 128 images)");
+  if (dtype == MSG_F64) CHK(stats_impl<double>(stack, nimg, n, fill, mean, count, median, strm(stream)));
+  else CHK(stats_impl<float>(stack, nimg, n, fill, mean, count, median, strm(stream)));
+  HIPCHK(hipStreamSynchronize(strm(stream)));
+  return MSG_OK;
+}
+
+int msg_normalize(int dtype, void *img, int n, int nimg, double pct, int min_to_zero, void *stream) {
+  CHK(check_dtype(dtype));
+  CHK(check_n(n));
+  if (!img || nimg < 1) return fail(MSG_ERR_ARG, "invalid argument");
+  if (dtype == MSG_F64) CHK(normalize_impl<double>(img, n, nimg, pct, min_to_zero, strm(stream)));
+  else CHK(normalize_impl<float>(img, n, nimg, pct, min_to_zero, strm(stream)));
+  HIPCHK(hipStreamSynchronize(strm(stream)));
+  return MSG_OK;
+}
+
+int msg_match_series(const msg_series_params *p, int dtype, const void *templates, int n, const void *reference,
+                     const void *pairwise_in, void *deformations, void *pairwise_out, double *energies, void *mean,
+                     void *count, void *median) {
+  if (!p || !templates || !reference || !deformations || !energies || !mean || !count || !median || n < 1)
+    return fail(MSG_ERR_ARG, "invalid argument");
+  CHK(check_dtype(dtype));
+  const int L = p->precision_level;
+  if (L < 1 || L > MAXL) return fail(MSG_ERR_ARG, "precision_level out of range");
+  if (p->start_level < 1 || p->stop_level > L || p->start_level > p->stop_level)
+    return fail(MSG_ERR_ARG, "invalid startLevel/stopLevel");
+  if (!p->dont_accumulate && n > 1 &&
+      (p->refine_start_level < 1 || p->refine_stop_level > L || p->refine_start_level > p->refine_stop_level))
+    return fail(MSG_ERR_ARG, "invalid refineStartLevel/refineStopLevel");
+  if (dtype == MSG_F64)
+    return match_series_impl<double>(p, templates, n, reference, pairwise_in, deformations, pairwise_out, energies,
+                                     mean, count, median);
+  return match_series_impl<float>(p, templates, n, reference, pairwise_in, deformations, pairwise_out, energies, mean,
+                                  count, median);
+}
+
+}  // extern "C"

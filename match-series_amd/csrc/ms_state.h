@@ -1,0 +1,57 @@
+// Per-pair scalar state of a batched level solve. Lives in device memory; only the control kernel
+// writes it (one workgroup per pair), every other kernel reads it. All scalars are FP64.
+#pragma once
+
+namespace msg {
+
+enum ArmijoPhase { AP_IDLE = 0, AP_FIRST = 1, AP_GROW = 2, AP_SHRINK = 3 };
+enum DirMode { DIR_CG = 0, DIR_GRAD = 1, DIR_NONE = 2 };
+enum { MAX_CAND = 4 };
+
+struct PairState {
+  // gradient-flow state (GradientDescentBase::applySingle)
+  double energy;      // E at the current position ("energy", also the Armijo f)
+  double energy_new;  // E after the last step
+  double ereg_new;    // E_reg (unweighted) after the last step
+  double tau;         // step size carried to the next Armijo search
+  double sigma;       // filter width of the H1 metric
+  double beta;        // Fletcher-Reeves coefficient of this iteration
+  double zz_old;      // |old smoothed gradient|^2
+  double Df, Dfz;     // <DE, d>, <DE, z>
+  double lambda;      // regularisation weight of the level
+  double stop_eps;
+  // Armijo machine (getTimestepWidthWithArmijoLineSearch)
+  double a_tau;       // tau of the sequential algorithm
+  double a_Df;        // slope of the active direction
+  double cand[MAX_CAND];
+  double cand_E[MAX_CAND], cand_R[MAX_CAND];
+  double last_tau[2], last_E[2], last_R[2];  // the last two consumed evaluations
+  double step_tau;    // tau of the update (0: no update)
+  double tau_first;   // result of the CG-direction search this iteration (trace)
+  int ncand;          // candidates to evaluate in the next energy pass (0: none)
+  int a_phase;
+  int dirmode;        // direction the energy pass / update uses
+  int step_dir;
+  int iter, max_iter;
+  int active;         // still iterating at this level
+  int searching;      // Armijo search in progress
+  int olddir_zero;    // the previous search fell back: treat the old CG direction as zero
+  int evals;          // evaluations consumed by this iteration's searches (reference count)
+  int passes;         // energy passes launched for this iteration
+  int total_evals, total_passes;
+  int pad;
+};
+
+// Per-iteration trace record (optional).
+struct IterTrace {
+  double energy;    // energy after the step
+  double ereg;      // unweighted regularisation energy after the step
+  double tau;       // final step size
+  double tau_first; // step size of the CG-direction search (0 if it fell back)
+  double sigma;     // filter width used for the step
+  double evals;     // reference-equivalent energy evaluations
+  double passes;    // energy passes
+  double beta;
+};
+
+}  // namespace msg

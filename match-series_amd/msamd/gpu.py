@@ -1,0 +1,314 @@
+"""ctypes binding of libmsgpu.so (include/ms_gpu.h) for tests and benchmarks.
+
+Device arrays are torch tensors on a ROCm device (torch is only the allocator here); every call goes
+through the library's C ABI. Importing this module loads the library and fails loudly if it is
+missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libmsgpu.so")
+
+F32, F64 = 0, 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+
+class SolveParams(ctypes.Structure):
+    _fields_ = [("lambda_", ctypes.c_double), ("lambda_factor", ctypes.c_double),
+                ("lambda_ref_level", ctypes.c_int), ("max_iter", ctypes.c_int), ("stop_eps", ctypes.c_double),
+                ("resample_prolong", ctypes.c_int), ("spec", ctypes.c_int)]
+
+
+class IterTrace(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ("energy", "ereg", "tau", "tau_first", "sigma", "evals", "passes", "beta")]
+
+
+class SeriesParams(ctypes.Structure):
+    _fields_ = [
+        ("start_level", ctypes.c_int), ("stop_level", ctypes.c_int),
+        ("refine_start_level", ctypes.c_int), ("refine_stop_level", ctypes.c_int),
+        ("precision_level", ctypes.c_int),
+        ("lambda_", ctypes.c_double), ("lambda_factor", ctypes.c_double),
+        ("max_iter", ctypes.c_int), ("stop_eps", ctypes.c_double),
+        ("saturation_pct", ctypes.c_double), ("normalize_min_to_zero", ctypes.c_int),
+        ("dont_normalize", ctypes.c_int), ("resample_prolong", ctypes.c_int),
+        ("dont_accumulate", ctypes.c_int), ("spec", ctypes.c_int),
+    ]
+
+
+# exported symbols with their argument types (also the list the C-ABI test checks)
+SIGNATURES = {
+    "msg_last_error": ([], ctypes.c_char_p),
+    "msg_version": ([], ctypes.c_char_p),
+    "msg_ctx_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "msg_ctx_destroy": ([_vp], ctypes.c_int),
+    "msg_set_reference": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
+    "msg_set_template": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
+    "msg_set_images_device": ([_vp, _vp, _vp, ctypes.c_int], ctypes.c_int),
+    "msg_set_deformation": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
+    "msg_get_deformation": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
+    "msg_set_deformation_zero": ([_vp, ctypes.c_int], ctypes.c_int),
+    "msg_set_deformations_device": ([_vp, _vp], ctypes.c_int),
+    "msg_get_deformations_device": ([_vp, _vp], ctypes.c_int),
+    "msg_solve_level": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _ip,
+                         _vp, ctypes.c_int], ctypes.c_int),
+    "msg_solve": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SolveParams), _dp], ctypes.c_int),
+    "msg_last_counts": ([_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                         ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+    "msg_ctx_buffer": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    "msg_energy": ([ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _vp], ctypes.c_int),
+    "msg_gradient": ([ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp], ctypes.c_int),
+    "msg_h1_smooth": ([ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp], ctypes.c_int),
+    "msg_warp": ([ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                  ctypes.c_int, _vp], ctypes.c_int),
+    "msg_restrict": ([ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+    "msg_prolong": ([ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+    "msg_resample": ([ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+    "msg_compose": ([ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
+    "msg_series_stats": ([ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _vp, _vp, _vp],
+                         ctypes.c_int),
+    "msg_normalize": ([ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp], ctypes.c_int),
+    "msg_match_series": ([ctypes.POINTER(SeriesParams), ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _dp, _vp,
+                          _vp, _vp], ctypes.c_int),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libmsgpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise RuntimeError(f"libmsgpu error {rc}: {lib().msg_last_error().decode()}")
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def dtype_code(t):
+    torch = _torch()
+    if t.dtype == torch.float32:
+        return F32
+    if t.dtype == torch.float64:
+        return F64
+    raise TypeError("tensors must be float32 or float64")
+
+
+def _ptr(t):
+    assert t.is_contiguous() and t.is_cuda, "expected a contiguous device tensor"
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def torch_dtype(code):
+    torch = _torch()
+    return torch.float64 if code == F64 else torch.float32
+
+
+# ------------------------------------------------------------------------------------------------
+# Operator surface
+# ------------------------------------------------------------------------------------------------
+def energy(R, T, phi):
+    """Per-pair (E_SSD, 1/2 sum_c phi_c K phi_c). R, T: (B, n, n); phi: (B, 2, n, n)."""
+    B, n = R.shape[0], R.shape[-1]
+    es = np.zeros(B)
+    er = np.zeros(B)
+    check(lib().msg_energy(dtype_code(R), _ptr(R), _ptr(T), _ptr(phi), n, B, _dptr(es), _dptr(er), None))
+    return es, er
+
+
+def gradient(R, T, phi, lam):
+    B, n = R.shape[0], R.shape[-1]
+    out = _torch().empty_like(phi)
+    check(lib().msg_gradient(dtype_code(R), _ptr(R), _ptr(T), _ptr(phi), _ptr(out), n, B, lam, None))
+    return out
+
+
+def h1_smooth(d, sigma):
+    """d: (k, n, n) fields smoothed independently."""
+    n = d.shape[-1]
+    k = d.numel() // (n * n)
+    out = _torch().empty_like(d)
+    check(lib().msg_h1_smooth(dtype_code(d), _ptr(d), _ptr(out), n, k, sigma, None))
+    return out
+
+
+def warp(img, phi, mode=0, ext=float("inf"), nearest=False):
+    """img: (k, n, n); phi: (k, 2, n, n) or (2, n, n) shared. mode 0: constant outside, 1: clamp, 2: NaN."""
+    n = img.shape[-1]
+    k = img.numel() // (n * n)
+    shared = 1 if phi.dim() == 3 else 0
+    out = _torch().empty_like(img)
+    check(lib().msg_warp(dtype_code(img), _ptr(img), _ptr(phi), _ptr(out), n, k, shared, mode, ext, int(nearest), None))
+    return out
+
+
+def restrict(fine):
+    n = fine.shape[-1]
+    k = fine.numel() // (n * n)
+    out = _torch().empty(fine.shape[:-2] + (n // 2, n // 2), dtype=fine.dtype, device=fine.device)
+    check(lib().msg_restrict(dtype_code(fine), _ptr(fine), _ptr(out), n, k, None))
+    return out
+
+
+def prolong(coarse):
+    nc = coarse.shape[-1]
+    k = coarse.numel() // (nc * nc)
+    out = _torch().empty(coarse.shape[:-2] + (2 * nc, 2 * nc), dtype=coarse.dtype, device=coarse.device)
+    check(lib().msg_prolong(dtype_code(coarse), _ptr(coarse), _ptr(out), nc, k, None))
+    return out
+
+
+def resample(src, n):
+    nc = src.shape[-1]
+    k = src.numel() // (nc * nc)
+    out = _torch().empty(src.shape[:-2] + (n, n), dtype=src.dtype, device=src.device)
+    check(lib().msg_resample(dtype_code(src), _ptr(src), _ptr(out), nc, n, k, None))
+    return out
+
+
+def compose(phi1, phi2, start=None):
+    out = _torch().zeros_like(phi1) if start is None else start.clone()
+    check(lib().msg_compose(dtype_code(phi1), _ptr(phi1), _ptr(phi2), _ptr(out), phi1.shape[-1], None))
+    return out
+
+
+def series_stats(stack, fill):
+    k, n = stack.shape[0], stack.shape[-1]
+    torch = _torch()
+    mean = torch.empty((n, n), dtype=stack.dtype, device=stack.device)
+    cnt = torch.empty_like(mean)
+    med = torch.empty_like(mean)
+    check(lib().msg_series_stats(dtype_code(stack), _ptr(stack), k, n, fill, _ptr(mean), _ptr(cnt), _ptr(med), None))
+    return mean, cnt, med
+
+
+def normalize(img, pct=-1.0, min_to_zero=True):
+    """In place on (k, n, n) images."""
+    n = img.shape[-1]
+    k = img.numel() // (n * n)
+    check(lib().msg_normalize(dtype_code(img), _ptr(img), n, k, pct, int(min_to_zero), None))
+    return img
+
+
+# ------------------------------------------------------------------------------------------------
+# Registration context
+# ------------------------------------------------------------------------------------------------
+class Context:
+    """A batch of registration pairs with image pyramids up to `level` (msg_ctx)."""
+
+    def __init__(self, level, npairs, dtype=F32):
+        self.level, self.npairs, self.dtype = level, npairs, dtype
+        self.n = 1 << level
+        h = ctypes.c_void_p()
+        check(lib().msg_ctx_create(ctypes.byref(h), level, npairs, dtype))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().msg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_reference(self, pair, img):
+        a = np.ascontiguousarray(img, dtype=np.float64)
+        check(lib().msg_set_reference(self.h, pair, _dptr(a)))
+
+    def set_template(self, pair, img):
+        a = np.ascontiguousarray(img, dtype=np.float64)
+        check(lib().msg_set_template(self.h, pair, _dptr(a)))
+
+    def set_images_device(self, ref, tpl):
+        check(lib().msg_set_images_device(self.h, _ptr(ref) if ref is not None else None,
+                                          _ptr(tpl) if tpl is not None else None,
+                                          dtype_code(ref if ref is not None else tpl)))
+
+    def set_deformation(self, pair, phi):
+        a = np.ascontiguousarray(phi, dtype=np.float64)
+        check(lib().msg_set_deformation(self.h, pair, _dptr(a)))
+
+    def get_deformation(self, pair):
+        out = np.zeros((2, self.n, self.n))
+        check(lib().msg_get_deformation(self.h, pair, _dptr(out)))
+        return out
+
+    def set_deformation_zero(self, pair):
+        check(lib().msg_set_deformation_zero(self.h, pair))
+
+    def set_deformations_device(self, phi):
+        check(lib().msg_set_deformations_device(self.h, _ptr(phi)))
+
+    def get_deformations_device(self, phi):
+        check(lib().msg_get_deformations_device(self.h, _ptr(phi)))
+
+    def solve_level(self, level, lam, max_iter, stop_eps, spec=0, trace_cap=0):
+        e = np.zeros(self.npairs)
+        it = np.zeros(self.npairs, dtype=np.int32)
+        tr = None
+        if trace_cap:
+            tr = (IterTrace * (trace_cap * self.npairs))()
+        check(lib().msg_solve_level(self.h, level, lam, max_iter, stop_eps, spec, _dptr(e), it.ctypes.data_as(_ip),
+                                    ctypes.cast(tr, _vp) if tr is not None else None, trace_cap))
+        traces = None
+        if tr is not None:
+            raw = np.ctypeslib.as_array(ctypes.cast(tr, _dp), shape=(self.npairs, trace_cap, 8)).copy()
+            traces = [raw[p, : it[p]] for p in range(self.npairs)]
+        return e, it, traces
+
+    def solve(self, start, stop, lam, max_iter, stop_eps, lambda_factor=-1.0, lambda_ref_level=None,
+              resample_prolong=0, spec=0):
+        p = SolveParams(lam, lambda_factor, stop if lambda_ref_level is None else lambda_ref_level, max_iter, stop_eps,
+                        resample_prolong, spec)
+        e = np.zeros(self.npairs)
+        check(lib().msg_solve(self.h, start, stop, ctypes.byref(p), _dptr(e)))
+        return e
+
+    def counts(self):
+        a, b, c = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        check(lib().msg_last_counts(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+
+def match_series(templates, reference, params, pairwise_in=None, want_pairwise=False):
+    """One stage of series matching on device tensors. templates: (n, N, N); reference: (N, N)."""
+    torch = _torch()
+    n, N = templates.shape[0], templates.shape[-1]
+    defs = torch.zeros((n, 2, N, N), dtype=templates.dtype, device=templates.device)
+    pw = torch.zeros_like(defs) if want_pairwise else None
+    en = np.zeros(n)
+    mean = torch.zeros((N, N), dtype=templates.dtype, device=templates.device)
+    cnt = torch.zeros_like(mean)
+    med = torch.zeros_like(mean)
+    check(lib().msg_match_series(ctypes.byref(params), dtype_code(templates), _ptr(templates), n, _ptr(reference),
+                                 _ptr(pairwise_in) if pairwise_in is not None else None, _ptr(defs),
+                                 _ptr(pw) if pw is not None else None, _dptr(en), _ptr(mean), _ptr(cnt), _ptr(med)))
+    return dict(deformations=defs, pairwise=pw, energies=en, mean=mean, count=cnt, median=med)

@@ -703,10 +703,11 @@ static void pcgDirichlet(const double *rhs, const unsigned char *defined, int N,
   double *h = (double *)calloc(NN, sizeof(double));
   double *tmp = (double *)calloc(NN, sizeof(double));
   double *diag = (double *)malloc(sizeof(double) * NN);
-  for (int i = 0; i < NN; ++i) diag[i] = dirichletDiag(i, defined, N);
+  /* DiagonalPreconditioner stores the inverse |diagonal| (preconditioner.h:25-35) */
+  for (int i = 0; i < NN; ++i) diag[i] = 1.0 / fabs(dirichletDiag(i, defined, N));
   dirichletStiffApply(dest, defined, N, h, tmp);
   for (int i = 0; i < NN; ++i) g[i] = h[i] - rhs[i];
-  for (int i = 0; i < NN; ++i) h[i] = g[i] / diag[i];
+  for (int i = 0; i < NN; ++i) h[i] = diag[i] * g[i];
   double spn = 0;
   for (int i = 0; i < NN; ++i) spn += g[i] * g[i];
   for (int i = 0; i < NN; ++i) d[i] -= h[i];
@@ -721,7 +722,7 @@ static void pcgDirichlet(const double *rhs, const unsigned char *defined, int N,
     const double alpha = an / ad;
     for (int i = 0; i < NN; ++i) dest[i] += alpha * d[i];
     for (int i = 0; i < NN; ++i) g[i] += alpha * h[i];
-    for (int i = 0; i < NN; ++i) h[i] = g[i] / diag[i];
+    for (int i = 0; i < NN; ++i) h[i] = diag[i] * g[i];
     double bn = 0;
     for (int i = 0; i < NN; ++i) bn += g[i] * h[i];
     const double bf = bn / bd;

@@ -1,0 +1,209 @@
+"""Write the golden fixtures of tests/golden/ from the reference itself.
+
+Runs in the build container only (it needs the reference build of oracle/_ref/, made by
+`make -C oracle ref` from the sources under /root/reference). Two sources of vectors:
+  * oracle/_ref/ref_harness: the reference's operators on seeded inputs (SSD energy and variation,
+    stiffness, H1 V-cycle, warps, grid transfer, composition, normalisation, statistics, one
+    complete level solve with its step-size trace);
+  * oracle/_ref/matchSeries: the reference program itself on a seeded 4-frame 64x64 STEM series with
+    two stages, reading frames written here in its P9 format and writing its usual output tree.
+Usage: python tests/golden/make_golden.py
+"""
+import bz2
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "match-series_amd"))
+from msamd import synth  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref")
+HARNESS = os.path.join(REF, "ref_harness")
+MATCHSERIES = os.path.join(REF, "matchSeries")
+
+
+def harness(cmd, N, params, arrays, tmp):
+    fin, fout = os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")
+    np.concatenate([np.ravel(a) for a in arrays]).astype("<f8").tofile(fin)
+    subprocess.check_call([HARNESS, cmd, str(N)] + [repr(float(p)) if isinstance(p, float) else str(p) for p in params]
+                          + [fin, fout])
+    return np.fromfile(fout, dtype="<f8")
+
+
+def write_p9(path, img):
+    N = img.shape[0]
+    hdr = f"P9\n# synthetic frame\n{img.shape[1]} {N}\n{int(img.max())}\n".encode()
+    data = hdr + np.ascontiguousarray(img, dtype="<f8").tobytes()
+    with open(path, "wb") as f:
+        f.write(bz2.compress(data) if path.endswith((".bz2", ".q2bz")) else data)
+
+
+def read_p9(path):
+    b = open(path, "rb").read()
+    if path.endswith((".bz2", ".q2bz")):
+        b = bz2.decompress(b)
+    fields, pos = [], 0
+    while len(fields) < 3:
+        e = b.index(b"\n", pos)
+        line, pos = b[pos:e], e + 1
+        if not line.startswith(b"#"):
+            fields.append(line)
+    W, H = map(int, fields[1].split())
+    return np.frombuffer(b[pos:pos + 8 * W * H], dtype="<f8").reshape(H, W).copy()
+
+
+def rand_problem(N, seed, amp_px=3.0):
+    rng = np.random.default_rng(seed)
+    h = 1.0 / (N - 1)
+    R = rng.random((N, N))
+    T = np.roll(R, 1, axis=1) + 0.05 * rng.random((N, N))
+    phi = (rng.random((2, N, N)) - 0.5) * 2 * amp_px * h
+    return R, T, phi
+
+
+def stem_pair(N):
+    sc = N / 256.0
+    s = np.stack([synth.stem_frame(N, synth.frame_seed(0, f), spacing=max(12 * sc, 3.0), sigma=max(2.5 * sc, 0.8),
+                                   jitter_step=0.3 * sc) for f in range(2)])
+    return s
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        sys.exit("oracle/_ref is not built (make -C oracle ref)")
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for N in (8, 32, 64):
+            R, T, phi = rand_problem(N, N)
+            o = harness("ssd", N, [], [R, T, phi], tmp)
+            nn = N * N
+            out[f"ssd{N}"] = dict(R=R, T=T, phi=phi, e_ssd=o[0], e_reg=o[1], de_ssd=o[2:2 + 2 * nn].reshape(2, N, N),
+                                  kphi=o[2 + 2 * nn:].reshape(2, N, N))
+        for N in (8, 64):
+            d = np.random.default_rng(100 + N).standard_normal((N, N))
+            for k, s in enumerate((1.0, 2.0 ** -5)):
+                out[f"smooth{N}_{k}"] = dict(d=d, sigma=s, out=harness("smooth", N, [s], [d], tmp).reshape(N, N))
+        N = 32
+        R, _, phi = rand_problem(N, 7, amp_px=4.0)
+        o = harness("warp", N, ["inf", 0], [R, phi], tmp)
+        out["warp32"] = dict(img=R, phi=phi, deform_inf=o[:N * N].reshape(N, N), fast=o[N * N:].reshape(N, N))
+        o = harness("warp", N, [0.25, 1], [R, phi], tmp)
+        out["warp32_nn"] = dict(img=R, phi=phi, deform_nn=o[:N * N].reshape(N, N))
+        rng = np.random.default_rng(11)
+        fine, coarse = rng.random((N, N)), rng.random((N // 2, N // 2))
+        o = harness("transfer", N, [], [fine, coarse], tmp)
+        n = N // 2
+        out["transfer32"] = dict(fine=fine, coarse=coarse, restrict=o[:n * n].reshape(n, n),
+                                 prolong=o[n * n:n * n + N * N].reshape(N, N), resample=o[n * n + N * N:].reshape(N, N))
+        h = 1.0 / (N - 1)
+        y, x = np.mgrid[0:N, 0:N].astype(float)
+        p1 = np.stack([3 * h * np.sin(2 * np.pi * y / N + 1), 3 * h * np.cos(2 * np.pi * x / N)])
+        p2 = np.stack([3 * h * np.cos(2 * np.pi * y / N), 3 * h * np.sin(2 * np.pi * x / N + 2)])
+        out["compose32"] = dict(phi1=p1, phi2=p2, out=harness("compose", N, [], [p1, p2], tmp).reshape(2, N, N))
+        img = rng.random((N, N)) * 3 - 1
+        for tag, pct in (("sat", 0.15), ("nosat", -1.0)):
+            o = harness("normalize", N, [pct, 1], [img], tmp)
+            out[f"normalize32_{tag}"] = dict(img=img, pct=pct, out=o[:N * N].reshape(N, N),
+                                            level4=o[N * N:N * N + 256].reshape(16, 16))
+        st = rng.random((5, 16, 16))
+        st[rng.random((5, 16, 16)) < 0.3] = np.inf
+        st[:, 0, 0] = np.inf
+        o = harness("median", 16, [5, 0.375], [st], tmp)
+        out["stats16"] = dict(stack=st, fill=0.375, mean=o[:256].reshape(16, 16), count=o[256:512].reshape(16, 16),
+                              median=o[512:].reshape(16, 16))
+        # complete level solves with their traces
+        for N, lam, its, eps in ((32, 10.0, 50, -1.0), (64, 10.0, 50, -1.0), (64, 1.0, 200, 1e-6)):
+            s = stem_pair(N)
+            o = harness("normalize", N, [0.15, 1], [s[0]], tmp)
+            R = o[:N * N].reshape(N, N)
+            o = harness("normalize", N, [0.15, 1], [s[1]], tmp)
+            T = o[:N * N].reshape(N, N)
+            o = harness("solve", N, [lam, its, eps], [R, T, np.zeros((2, N, N))], tmp)
+            nn = N * N
+            k = 2 * nn
+            iters = int(o[k])
+            e_fin = o[k + 1]
+            k += 2
+            energies = o[k:k + iters + 1]
+            k += iters + 1
+            nc = int(o[k])
+            k += 1
+            out[f"solve{N}_lam{lam:g}"] = dict(R=R, T=T, lam=lam, max_iter=its, stop_eps=eps,
+                                             phi=o[:2 * nn].reshape(2, N, N), iterations=iters, energy=e_fin,
+                                             energies=energies, taus=o[k:k + nc], sigmas=o[k + nc:k + 2 * nc],
+                                             evals=o[k + 2 * nc:k + 3 * nc])
+        # the reference program end to end: 4 frames of 64x64, two stages
+        frames = synth.stem_series(64, 4, workload=0, spacing=6.0, sigma=1.5, jitter_step=0.15)
+        wd = os.path.join(tmp, "e2e")
+        os.makedirs(os.path.join(wd, "in"))
+        for i, f in enumerate(frames):
+            write_p9(os.path.join(wd, "in", f"frame_{i:02d}.dat.bz2"), f)
+        par = E2E_PAR
+        with open(os.path.join(wd, "run.par"), "w") as f:
+            f.write(par)
+        env = dict(os.environ, QUOC_NO_SYSTEM_PAUSE="ON")
+        subprocess.check_call([MATCHSERIES, "run.par"], cwd=wd, env=env, stdout=subprocess.DEVNULL,
+                              stderr=subprocess.DEVNULL)
+        od = os.path.join(wd, "out")
+        e2e = dict(frames=frames)
+        for st_ in (1, 2):
+            nt = 3 if st_ == 1 else 4
+            defs = []
+            for i in range(nt):
+                sub = f"{i}" if i == 0 else f"{i}-r"
+                defs.append(np.stack([read_p9(os.path.join(od, f"stage{st_}", sub, f"deformation_06_{c}.dat.bz2"))
+                                      for c in range(2)]))
+            e2e[f"s{st_}_deformations"] = np.stack(defs)
+            if st_ == 1:
+                e2e["s1_pairwise"] = np.stack([np.stack([read_p9(os.path.join(od, "stage1", f"{i}",
+                                                                                f"deformation_06_{c}.dat.bz2"))
+                                                         for c in range(2)]) for i in range(nt)])
+            for nm in ("average", "median", "numSamples"):
+                e2e[f"s{st_}_{nm}"] = read_p9(os.path.join(od, f"stage{st_}", f"{nm}.q2bz"))
+            en = np.loadtxt(os.path.join(od, f"stage{st_}", "energies.txt"))
+            e2e[f"s{st_}_energies"] = en[:, 1]
+        e2e["par"] = np.frombuffer(par.encode(), dtype=np.uint8)
+        out["e2e64"] = e2e
+    for key, d in out.items():
+        np.savez_compressed(os.path.join(HERE, f"{key}.npz"), **d)
+    print("wrote", len(out), "fixtures")
+
+
+E2E_PAR = """templateNamePattern in/frame_%02d.dat.bz2
+templateNumOffset 0
+templateNumStep 1
+numTemplates 4
+deformationModel 22
+saveDirectory out/
+lambda 10
+lambdaFactor 1
+maxGDIterations 15
+stopEpsilon -1
+startLevel 4
+stopLevel 6
+precisionLevel 6
+refineStartLevel 5
+refineStopLevel 6
+numExtraStages 1
+extraStagesLambdaFactor 0.1
+useMedianAsNewTarget 1
+reuseStage1Results 1
+skipStage1 0
+dontNormalizeInputImages 0
+enhanceContrastSaturationPercentage 0.15
+normalizeMinToZero 1
+resampleInsteadOfProlongateDeformation 1
+dontAccumulateDeformation 0
+reduceDeformations 0
+onlySaveDisplacement 1
+checkboxWidth 8
+"""
+
+if __name__ == "__main__":
+    main()

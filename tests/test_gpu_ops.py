@@ -1,0 +1,169 @@
+"""GPU operator parity: every kernel behind the C ABI against the FP64 oracle on seeded inputs.
+
+MSG_F64 (FP64 storage) must agree with the oracle to rounding (1e-12 relative); MSG_F32 (FP32
+storage, FP64 arithmetic) to the storage precision.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from msamd import gpu as G
+    G.lib()
+    return G
+
+
+def dev(a, dt):
+    return torch.tensor(np.ascontiguousarray(a), dtype=dt, device="cuda")
+
+
+def host(t):
+    return t.detach().cpu().double().numpy()
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def random_problem(N, seed, amp_px=3.0):
+    rng = np.random.default_rng(seed)
+    h = 1.0 / (N - 1)
+    R = rng.random((N, N))
+    T = np.roll(R, 1, axis=1) + 0.05 * rng.random((N, N))
+    phi = (rng.random((2, N, N)) - 0.5) * 2 * amp_px * h
+    return R, T, phi
+
+
+DT = [(1, 1e-12), (0, 2e-6)]  # (dtype code, tolerance)
+
+
+@pytest.mark.parametrize("N", [4, 8, 32, 64, 128])
+@pytest.mark.parametrize("code,tol", DT)
+def test_energy(gpu, N, code, tol):
+    probs = [random_problem(N, s) for s in range(3)]
+    dt = gpu.torch_dtype(code)
+    R = dev(np.stack([p[0] for p in probs]), dt)
+    T = dev(np.stack([p[1] for p in probs]), dt)
+    phi = dev(np.stack([p[2] for p in probs]), dt)
+    es, er = gpu.energy(R, T, phi)
+    for b, (r, t, ph) in enumerate(probs):
+        r32, t32, ph32 = host(R[b]), host(T[b]), host(phi[b])
+        assert abs(es[b] - O.ssd_energy(r32, t32, ph32)) <= tol * O.ssd_energy(r32, t32, ph32)
+        eo = O.reg_energy(ph32)
+        assert abs(er[b] - eo) <= max(tol, 1e-12) * eo
+
+
+@pytest.mark.parametrize("N", [4, 8, 32, 64, 128])
+@pytest.mark.parametrize("code,tol", DT)
+def test_gradient(gpu, N, code, tol):
+    probs = [random_problem(N, 10 + s) for s in range(2)]
+    dt = gpu.torch_dtype(code)
+    R = dev(np.stack([p[0] for p in probs]), dt)
+    T = dev(np.stack([p[1] for p in probs]), dt)
+    phi = dev(np.stack([p[2] for p in probs]), dt)
+    lam = 10.0
+    out = host(gpu.gradient(R, T, phi, lam))
+    for b in range(len(probs)):
+        r, t, ph = host(R[b]), host(T[b]), host(phi[b])
+        ref = O.ssd_force(r, t, ph) + lam * np.stack([O.stiff_apply(ph[0]), O.stiff_apply(ph[1])])
+        assert rel(out[b], ref) < max(tol, 1e-12) * 10
+
+
+@pytest.mark.parametrize("N", [2, 8, 32, 64, 128, 256])
+@pytest.mark.parametrize("sigma", [1.0, 2.0 ** -5])
+@pytest.mark.parametrize("code,tol", DT)
+def test_h1_smooth(gpu, N, sigma, code, tol):
+    rng = np.random.default_rng(N)
+    d = rng.standard_normal((3, N, N))
+    out = host(gpu.h1_smooth(dev(d, gpu.torch_dtype(code)), sigma))
+    for k in range(3):
+        dk = host(dev(d[k], gpu.torch_dtype(code)))
+        assert rel(out[k], O.vcycle(dk, sigma)) < max(tol, 1e-12) * 10
+
+
+@pytest.mark.parametrize("N", [8, 64])
+@pytest.mark.parametrize("code,tol", DT)
+def test_warp(gpu, N, code, tol):
+    R, _, phi = random_problem(N, 3, amp_px=4.0)
+    dt = gpu.torch_dtype(code)
+    img, ph = dev(R, dt), dev(phi, dt)
+    r, p = host(img), host(ph)
+    a = host(gpu.warp(img[None], ph, mode=0, ext=np.inf))[0]
+    o = O.deform_image(r, p, np.inf)
+    assert np.array_equal(np.isfinite(a), np.isfinite(o))
+    assert rel(a[np.isfinite(o)], o[np.isfinite(o)]) < tol
+    b = host(gpu.warp(img[None], ph, mode=1))[0]
+    assert rel(b, O.fast_deform_image(r, p)) < tol
+    c = host(gpu.warp(img[None], ph, mode=0, ext=0.25, nearest=True))[0]
+    assert rel(c, O.deform_image(r, p, 0.25, nearest=True)) < tol
+
+
+@pytest.mark.parametrize("N", [2, 16, 128])
+@pytest.mark.parametrize("code,tol", DT)
+def test_transfer(gpu, N, code, tol):
+    rng = np.random.default_rng(5)
+    dt = gpu.torch_dtype(code)
+    f = dev(rng.random((2, N, N)), dt)
+    r = host(gpu.restrict(f))
+    for k in range(2):
+        assert rel(r[k], O.restrict(host(f[k]))) < tol
+    if N >= 4:
+        c = dev(rng.random((2, N // 2, N // 2)), dt)
+        p = host(gpu.prolong(c))
+        s = host(gpu.resample(c, N))
+        for k in range(2):
+            assert np.array_equal(p[k], O.prolong(host(c[k])))
+            assert rel(s[k], O.resample(host(c[k]), N)) < max(tol, 1e-14)
+
+
+@pytest.mark.parametrize("N,amp", [(16, 1.0), (64, 3.0), (64, 6.0)])
+@pytest.mark.parametrize("code,tol", DT)
+def test_compose(gpu, N, amp, code, tol):
+    h = 1.0 / (N - 1)
+    y, x = np.mgrid[0:N, 0:N].astype(float)
+    p1 = np.stack([amp * h * np.sin(2 * np.pi * y / N + 1), amp * h * np.cos(2 * np.pi * x / N)])
+    p2 = np.stack([amp * h * np.cos(2 * np.pi * y / N), amp * h * np.sin(2 * np.pi * x / N + 2)])
+    start = 0.5 * p2
+    dt = gpu.torch_dtype(code)
+    g = host(gpu.compose(dev(p1, dt), dev(p2, dt), dev(start, dt)))
+    o = O.compose(host(dev(p1, dt)), host(dev(p2, dt)), host(dev(start, dt)))
+    assert rel(g, o) < max(tol, 1e-10) * 10
+
+
+@pytest.mark.parametrize("F", [1, 2, 5, 8])
+@pytest.mark.parametrize("code,tol", DT)
+def test_series_stats(gpu, F, code, tol):
+    rng = np.random.default_rng(F)
+    N = 32
+    st = rng.random((F, N, N))
+    st[rng.random((F, N, N)) < 0.3] = np.inf
+    st[:, 0, 0] = np.inf  # a pixel with no sample
+    dt = gpu.torch_dtype(code)
+    m, c, med = (host(t) for t in gpu.series_stats(dev(st, dt), 0.375))
+    mo, co, medo = O.series_stats(host(dev(st, dt)), 0.375)
+    assert np.array_equal(c, co)
+    assert rel(m, mo) < tol
+    assert rel(med, medo) < tol
+    assert med[0, 0] == 0.375 and m[0, 0] == 0.375
+
+
+@pytest.mark.parametrize("pct", [0.15, 5.0, -1.0])
+@pytest.mark.parametrize("code,tol", DT)
+def test_normalize(gpu, pct, code, tol):
+    rng = np.random.default_rng(7)
+    img = rng.random((2, 64, 64)) * 3 - 1
+    dt = gpu.torch_dtype(code)
+    t = dev(img, dt)
+    src = host(t)
+    out = host(gpu.normalize(t, pct, True))
+    for k in range(2):
+        assert rel(out[k], O.normalize(src[k], pct, True)) < tol

@@ -1,0 +1,118 @@
+"""GPU level solve and series matching against the oracle's restatement of the reference loop.
+
+MSG_F64 must reproduce the oracle's trajectory: the same step size at every Armijo call, the same
+number of iterations, energies to 1e-10 and the deformation to 1e-8 px. MSG_F32 (FP32 fields,
+FP64 energies, slopes and decisions) is held to the north-star bar: final energy within 1e-6
+relative and final deformation within 1e-3 px after the fixed iteration schedule.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from msamd import gpu as G
+    G.lib()
+    return G
+
+
+def stem_pair(N, seed_offset=0):
+    from msamd import synth
+    sc = N / 256.0
+    s = np.stack([synth.stem_frame(N, synth.frame_seed(0, f + seed_offset), spacing=max(12 * sc, 3.0),
+                                   sigma=max(2.5 * sc, 0.8), jitter_step=0.3 * sc) for f in range(2)])
+    return O.normalize(s[0], 0.15, True), O.normalize(s[1], 0.15, True)
+
+
+def gpu_calls(trace):
+    """Per Armijo call step sizes from the per-iteration trace (tau_first, then tau if it fell back)."""
+    out = []
+    for row in trace:
+        out.append(row[3])
+        if row[3] == 0:
+            out.append(row[2])
+    return np.array(out)
+
+
+@pytest.mark.parametrize("N,lam,its,eps", [(16, 10.0, 30, -1.0), (64, 10.0, 50, -1.0), (64, 1.0, 200, 1e-6),
+                                            (128, 10.0, 25, -1.0)])
+def test_level_solve_f64_trajectory(gpu, N, lam, its, eps):
+    R, T = stem_pair(N)
+    L = int(np.log2(N))
+    ctx = gpu.Context(L, 1, gpu.F64)
+    ctx.set_reference(0, R)
+    ctx.set_template(0, T)
+    e, it, tr = ctx.solve_level(L, lam, its, eps, trace_cap=its)
+    phi = ctx.get_deformation(0)
+    ctx.close()
+    phi_o, info = O.level_solve(R, T, np.zeros((2, N, N)), lam, its, eps)
+    assert it[0] == info["iterations"]
+    np.testing.assert_array_equal(gpu_calls(tr[0]), info["taus"])
+    np.testing.assert_allclose(tr[0][:, 0], info["energies"][1:], rtol=1e-10)
+    assert abs(e[0] - info["energy"]) <= 1e-10 * abs(info["energy"])
+    assert np.abs(phi - phi_o).max() * (N - 1) < 1e-8
+
+
+@pytest.mark.parametrize("N,lam,its", [(64, 10.0, 50), (256, 10.0, 30)])
+def test_level_solve_f32(gpu, N, lam, its):
+    R, T = stem_pair(N)
+    L = int(np.log2(N))
+    ctx = gpu.Context(L, 1, gpu.F32)
+    ctx.set_reference(0, R)
+    ctx.set_template(0, T)
+    e, it, _ = ctx.solve_level(L, lam, its, -1.0)
+    phi = ctx.get_deformation(0)
+    ctx.close()
+    # the oracle on the FP32-rounded images
+    R32, T32 = R.astype(np.float32).astype(np.float64), T.astype(np.float32).astype(np.float64)
+    phi_o, info = O.level_solve(R32, T32, np.zeros((2, N, N)), lam, its, -1.0)
+    assert it[0] == info["iterations"]
+    assert abs(e[0] - info["energy"]) <= 1e-6 * abs(info["energy"])
+    assert np.abs(phi - phi_o).max() * (N - 1) < 1e-3
+
+
+def test_batched_pairs_are_independent(gpu):
+    N, L = 32, 5
+    pairs = [stem_pair(N, 3 * k) for k in range(3)]
+    ctx = gpu.Context(L, 3, gpu.F64)
+    for k, (R, T) in enumerate(pairs):
+        ctx.set_reference(k, R)
+        ctx.set_template(k, T)
+    e, it, _ = ctx.solve_level(L, 5.0, 40, -1.0)
+    for k, (R, T) in enumerate(pairs):
+        phi_o, info = O.level_solve(R, T, np.zeros((2, N, N)), 5.0, 40, -1.0)
+        assert it[k] == info["iterations"]
+        assert abs(e[k] - info["energy"]) <= 1e-10 * abs(info["energy"])
+        assert np.abs(ctx.get_deformation(k) - phi_o).max() * (N - 1) < 1e-8
+    ctx.close()
+
+
+@pytest.mark.parametrize("resample", [0, 1])
+def test_match_series_f64(gpu, resample):
+    from msamd import synth
+    N, L = 64, 6
+    frames = synth.stem_series(N, 4, workload=0, spacing=6.0, sigma=1.5, jitter_step=0.15)
+    p = O.SeriesParams(start_level=4, stop_level=6, refine_start_level=5, refine_stop_level=6, precision_level=6,
+                       lambda_=10.0, lambda_factor=-1, max_iter=15, stop_eps=-1.0, saturation_pct=0.15,
+                       normalize_min_to_zero=1, dont_normalize=0, resample_prolong=resample, dont_accumulate=0)
+    ref = O.match_series(frames[1:], frames[0], p)
+    gp = gpu.SeriesParams(start_level=4, stop_level=6, refine_start_level=5, refine_stop_level=6, precision_level=6,
+                          lambda_=10.0, lambda_factor=-1, max_iter=15, stop_eps=-1.0, saturation_pct=0.15,
+                          normalize_min_to_zero=1, dont_normalize=0, resample_prolong=resample, dont_accumulate=0,
+                          spec=0)
+    t = torch.tensor(frames, dtype=torch.float64, device="cuda")
+    out = gpu.match_series(t[1:].contiguous(), t[0].contiguous(), gp)
+    d = out["deformations"].cpu().numpy()
+    assert np.abs(d - ref["deformations"]).max() * (N - 1) < 1e-6
+    np.testing.assert_allclose(out["energies"], ref["energies"], rtol=1e-9)
+    np.testing.assert_array_equal(out["count"].cpu().numpy(), ref["count"])
+    np.testing.assert_allclose(out["mean"].cpu().numpy(), ref["mean"], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(out["median"].cpu().numpy(), ref["median"], rtol=1e-9, atol=1e-12)
