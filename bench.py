@@ -1,0 +1,148 @@
+"""Benchmark: registration + reconstruction of a 32-frame 1024x1024 STEM series on one MI355X.
+
+One step = stage 1 of matchSeries (deformationModel 22, SSD + Dirichlet regulariser) on the whole
+series, inputs already resident in HBM: 31 pairwise multilevel solves (levels 7-10, batched), the
+accumulate-and-refine chain against the first frame (levels 9-10), and the series mean / median.
+100 gradient-flow iterations per level (stopEpsilon -1: fixed schedule), lambda 10. Fields FP32,
+energies / slopes / step decisions FP64.
+
+Prints one JSON line with frames/s, the roofline of the dominant kernel (the energy pass, timed
+with HIP events on its own stream during the timed steps) and a CPU baseline: the reference's own
+level solve (oracle/_ref/ref_harness, compiled from the reference sources) timed for a few
+iterations at 1024^2 on this host, extrapolated to the same schedule (cost proportional to N^2).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "match-series_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def series_params(gpu, levels=(7, 10), refine=(9, 10), its=100, lam=10.0, spec=0):
+    L = levels[1]
+    return gpu.SeriesParams(start_level=levels[0], stop_level=levels[1], refine_start_level=refine[0],
+                            refine_stop_level=refine[1], precision_level=L, lambda_=lam, lambda_factor=-1.0,
+                            max_iter=its, stop_eps=-1.0, saturation_pct=0.15, normalize_min_to_zero=1,
+                            dont_normalize=0, resample_prolong=1, dont_accumulate=0, spec=spec)
+
+
+def cpu_baseline(N, its_sample, schedule, frames):
+    """Reference level solve at N x N for its_sample iterations on one host core, extrapolated."""
+    from msamd import synth
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    kind = "reference"
+    s = synth.stem_series(N, 2, workload=1, spacing=16.0, sigma=3.0, jitter_step=0.4)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    R, T = O.normalize(s[0], 0.15, True), O.normalize(s[1], 0.15, True)
+    with tempfile.TemporaryDirectory() as tmp:
+        fin, fout = os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")
+        np.concatenate([R.ravel(), T.ravel(), np.zeros(2 * N * N)]).astype("<f8").tofile(fin)
+        if os.path.exists(harness):
+            t0 = time.perf_counter()
+            subprocess.check_call([harness, "solve", str(N), "10", str(its_sample), "-1", fin, fout],
+                                  stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+            t = time.perf_counter() - t0
+        else:  # the oracle's restatement of the same loop
+            kind = "port"
+            t0 = time.perf_counter()
+            O.level_solve(R, T, np.zeros((2, N, N)), 10.0, its_sample, -1.0)
+            t = time.perf_counter() - t0
+    t_iter = t / its_sample  # includes the initial energy and setup: slightly pessimistic for the CPU
+    L = int(np.log2(N))
+    (l0, l1), (r0, r1), its = schedule
+    per_pair = sum(its * t_iter * 4.0 ** (l - L) for l in range(l0, l1 + 1))
+    per_refine = sum(its * t_iter * 4.0 ** (l - L) for l in range(r0, r1 + 1))
+    total = (frames - 1) * per_pair + (frames - 2) * per_refine
+    return dict(value=frames / total, unit="frames/s", cores=1, kind=kind,
+                sample=f"{its_sample} gradient-flow iterations of one {N}x{N} pair (level {L}) in "
+                       f"{'oracle/_ref/ref_harness solve' if kind == 'reference' else 'the oracle'} "
+                       f"= {t_iter:.3f} s/iteration, extrapolated to the step's schedule with cost ~ N^2")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--frames", type=int, default=32)
+    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--iters", type=int, default=100)
+    ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
+    ap.add_argument("--spec", type=int, default=0)
+    ap.add_argument("--cpu-iters", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the series shards by frame across GPUs "
+                                   "in a multi-GPU deployment, which is not part of this build", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    from msamd import gpu, synth
+
+    torch.cuda.set_device(0)
+    N, F = args.size, args.frames
+    L = int(np.log2(N))
+    levels, refine = (L - 3, L), (L - 1, L)
+    dt = torch.float32 if args.precision == "f32" else torch.float64
+    frames = synth.stem_series(N, F, workload=1, spacing=16.0, sigma=3.0, jitter_step=0.4)
+    dev = torch.tensor(frames, dtype=dt, device="cuda")
+    tpl, ref = dev[1:].contiguous(), dev[0].contiguous()
+    p = series_params(gpu, levels, refine, args.iters, spec=args.spec)
+
+    out = None
+    for _ in range(args.warmup):
+        out = gpu.match_series(tpl, ref, p)
+    torch.cuda.synchronize()
+    gpu.profile_reset()
+    gpu.profile_enable(("energy",))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = gpu.match_series(tpl, ref, p)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    gpu.profile_enable(())
+    prof = gpu.profile_read("energy")
+    dt_s = t1 - t0
+    ms_step = 1000.0 * dt_s / args.steps
+    value = F * args.steps / dt_s
+
+    achieved = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9 if prof["ms"] > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": "msg::k_energy", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "launches": prof["launches"], "avg_launch_us": round(1000.0 * prof["ms"] / max(prof["launches"], 1), 2)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(N, args.cpu_iters, (levels, refine, args.iters), F)
+
+    finite = bool(torch.isfinite(out["median"]).all().item()) if out is not None else False
+    line = {
+        "metric": "frames/s at 1024^2 to fixed energy (plain vs NN warm-start); % HBM roofline",
+        "value": round(value, 3), "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.precision, "data": "synthetic (seeded STEM lattice series, Poisson noise, scan distortion)",
+        "config": {"workload": f"plain matchSeries stage 1: {F} frames of {N}x{N}, levels {levels[0]}-{levels[1]}, "
+                               f"refine {refine[0]}-{refine[1]}, {args.iters} it/level, lambda 10, SSD",
+                   "frames": F, "size": N, "iterations_per_level": args.iters, "method": "plain"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "energies_final_mean": float(np.mean(out["energies"])) if out is not None else None,
+        "median_finite": finite,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -156,6 +156,7 @@ typedef struct {
   int resample_prolong;               /* "resampleInsteadOfProlongateDeformation"             */
   int dont_accumulate;                /* "dontAccumulateDeformation"                          */
   int spec;                           /* Armijo candidates per energy pass (0: default)       */
+  double pre_smooth_sigma;            /* "preSmoothSigma" (pixels); <= 0: none                */
 } msg_series_params;
 
 /* templates: n device images (dtype) at precision_level; reference: one device image.
@@ -166,6 +167,16 @@ typedef struct {
 int msg_match_series(const msg_series_params *p, int dtype, const void *templates, int n, const void *reference,
                      const void *pairwise_in, void *deformations, void *pairwise_out, double *energies,
                      void *mean, void *count, void *median);
+
+/* ------------------------------------------------------------------------------------------------
+ * Kernel timing (no counterpart in the reference; used by the benchmark). With class bit k set in
+ * `mask`, every launch of kernel class k is bracketed by HIP events on its own stream. Classes:
+ * 0 energy, 1 variation, 2 H1 V-cycle, 3 CG direction, 4 update, 5 control, 6 composition.
+ * msg_profile_read returns the summed event time, launch count and algorithmic bytes of a class.
+ * ---------------------------------------------------------------------------------------------- */
+int msg_profile_enable(unsigned mask);
+int msg_profile_reset(void);
+int msg_profile_read(int cls, double *ms, long long *launches, double *bytes);
 
 #ifdef __cplusplus
 }

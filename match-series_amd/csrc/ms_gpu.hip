@@ -87,6 +87,76 @@ int scratch_get(int slot, size_t bytes, void **p) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// Kernel timing with HIP events on the stream each kernel is launched on (opt-in, per class mask).
+// ------------------------------------------------------------------------------------------------
+enum KernelClass { KC_ENERGY = 0, KC_GRAD = 1, KC_VCYCLE = 2, KC_NCG = 3, KC_UPDATE = 4, KC_CONTROL = 5,
+                   KC_COMPOSE = 6, KC_COUNT = 7 };
+struct ProfRec {
+  int cls;
+  hipEvent_t a, b;
+  double bytes;
+};
+struct Profiler {
+  std::mutex mu;
+  unsigned mask = 0;
+  std::vector<ProfRec> pending;
+  std::vector<hipEvent_t> pool;
+  double ms[KC_COUNT] = {};
+  long long launches[KC_COUNT] = {};
+  double bytes[KC_COUNT] = {};
+};
+Profiler g_prof;
+
+hipEvent_t prof_event() {
+  if (!g_prof.pool.empty()) {
+    hipEvent_t e = g_prof.pool.back();
+    g_prof.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// RAII bracket around one launch: records start/stop events when the class is enabled.
+struct ProfScope {
+  bool on;
+  int cls;
+  hipStream_t s;
+  double bytes;
+  hipEvent_t a = nullptr;
+  ProfScope(int c, hipStream_t st, double b) : on((g_prof.mask >> c) & 1u), cls(c), s(st), bytes(b) {
+    if (on) {
+      std::lock_guard<std::mutex> g(g_prof.mu);
+      a = prof_event();
+      (void)hipEventRecord(a, s);
+    }
+  }
+  ~ProfScope() {
+    if (on) {
+      std::lock_guard<std::mutex> g(g_prof.mu);
+      hipEvent_t b = prof_event();
+      (void)hipEventRecord(b, s);
+      g_prof.pending.push_back({cls, a, b, bytes});
+    }
+  }
+};
+
+void prof_resolve() {
+  for (auto &r : g_prof.pending) {
+    (void)hipEventSynchronize(r.b);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, r.a, r.b);
+    g_prof.ms[r.cls] += ms;
+    g_prof.launches[r.cls] += 1;
+    g_prof.bytes[r.cls] += r.bytes;
+    g_prof.pool.push_back(r.a);
+    g_prof.pool.push_back(r.b);
+  }
+  g_prof.pending.clear();
+}
+
+// ------------------------------------------------------------------------------------------------
 // Small helper kernels
 // ------------------------------------------------------------------------------------------------
 __global__ void k_state_init(PairState *st, int npairs, double lambda) {
@@ -140,7 +210,8 @@ EnergyGeom energy_geom(int n) {
 
 template <typename S>
 int launch_energy(const S *R, const S *T, const S *phi, const S *dcg, const S *dgrad, int n, const PairState *st,
-                  double *part, int npairs, hipStream_t s) {
+                  double *part, int npairs, hipStream_t s, double bytes = 0) {
+  ProfScope ps(KC_ENERGY, s, bytes);
   const EnergyGeom g = energy_geom(n);
   const double h = 1.0 / (double)(n - 1);
   const int width = (int)((n - 1) * h);
@@ -153,6 +224,7 @@ int launch_energy(const S *R, const S *T, const S *phi, const S *dcg, const S *d
 template <typename S>
 int launch_grad(const S *R, const S *T, const S *phi, S *DE, int n, const PairState *st, double lambda, int npairs,
                 hipStream_t s) {
+  ProfScope ps(KC_GRAD, s, 6.0 * n * n * sizeof(S) * npairs);
   const int ntx = cdiv(n, msg::GR_TX), nty = cdiv(n, msg::GR_TY);
   const double h = 1.0 / (double)(n - 1);
   const int width = (int)((n - 1) * h);
@@ -182,6 +254,7 @@ int mg_norm_blocks(int l) {
 template <typename S>
 int launch_vcycle(const S *in, int negate, S *out, int l, const MGBufs<S> &b, const PairState *st, double sigma,
                   int nfields, double *nrm_part, hipStream_t s) {
+  ProfScope ps(KC_VCYCLE, s, 0);
   if (l <= msg::MGC_MAXL) {
     hipLaunchKernelGGL((msg::k_mg_coarse<S>), dim3(nfields), dim3(256), 0, s, in, negate, out, l, st, sigma, nfields,
                        nrm_part);
@@ -389,12 +462,14 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
 
   hipLaunchKernelGGL(k_state_init, dim3(cdiv(B, 64)), dim3(64), 0, s, c->st, B, lambda);
   HIPCHK(hipGetLastError());
-  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s));
+  const double field_bytes = (double)n * n * sizeof(S);
+  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, 4.0 * field_bytes * B));
   a.phase = msg::PH_LEVEL_INIT;
   hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
   HIPCHK(hipGetLastError());
 
   bool any_active = max_iter > 0;
+  int nactive = max_iter > 0 ? B : 0;
   while (any_active) {
     CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s));
     CHK(launch_vcycle<S>(DE, 1, z, l, mg, c->st, 1.0, 2 * B, c->part_n, s));
@@ -409,8 +484,9 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
     HIPCHK(hipGetLastError());
     bool searching = true;
     int rounds = 0;
+    int nsearch = nactive;  // pairs evaluating in this round (upper bound for the first round)
     while (searching) {
-      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s));
+      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, 6.0 * field_bytes * nsearch));
       a.phase = msg::PH_ARMIJO_STEP;
       hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d,
                          tr);
@@ -419,10 +495,14 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
       HIPCHK(hipStreamSynchronize(s));
       searching = false;
       any_active = false;
+      nsearch = 0;
+      nactive = 0;
       for (int p = 0; p < B; ++p) {
-        searching |= (c->flags_h[p] & 1) != 0;
-        any_active |= (c->flags_h[p] & 2) != 0;
+        nsearch += (c->flags_h[p] & 1) != 0;
+        nactive += (c->flags_h[p] & 2) != 0;
       }
+      searching = nsearch > 0;
+      any_active = nactive > 0;
       if (++rounds > 200) return fail(MSG_ERR_STATE, "Armijo search did not terminate");
     }
     hipLaunchKernelGGL((msg::k_update<S>), dim3(nblk_upd * B), dim3(256), 0, s, phi, d, z, n, c->st, nblk_upd, B);
@@ -991,6 +1071,11 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
     CHK(normalize_impl<S>(pc->R[L], N, npw, p->saturation_pct, p->normalize_min_to_zero, s));
     CHK(normalize_impl<S>(pc->T[L], N, npw, p->saturation_pct, p->normalize_min_to_zero, s));
   }
+  if (p->pre_smooth_sigma > 0) {  // prepareMultilevelArray's optional pre-smoothing (registration.h:356-362)
+    const double sig = p->pre_smooth_sigma / (double)(N - 1);
+    CHK(h1_impl<S>(pc->R[L], pc->R[L], N, npw, sig, s));
+    CHK(h1_impl<S>(pc->T[L], pc->T[L], N, npw, sig, s));
+  }
   CHK(rebuild_pyramid<S>(pc, pc->R, -1, 1));
   CHK(rebuild_pyramid<S>(pc, pc->T, -1, 1));
   std::vector<double> epw(npw);
@@ -1019,6 +1104,7 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
     hipStream_t rs = rc->stream;
     HIPCHK(hipMemcpyAsync(rc->R[L], ref, nn * sizeof(S), hipMemcpyDeviceToDevice, rs));
     if (!p->dont_normalize) CHK(normalize_impl<S>(rc->R[L], N, 1, p->saturation_pct, p->normalize_min_to_zero, rs));
+    if (p->pre_smooth_sigma > 0) CHK(h1_impl<S>(rc->R[L], rc->R[L], N, 1, p->pre_smooth_sigma / (double)(N - 1), rs));
     CHK(rebuild_pyramid<S>(rc, rc->R, -1, 1));
     DevBuf acc;
     HIPCHK(hipMalloc(&acc.p, sizeof(S) * 2 * nn));
@@ -1032,6 +1118,8 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
         HIPCHK(hipMemcpyAsync(rc->T[L], tpl + (size_t)i * nn, nn * sizeof(S), hipMemcpyDeviceToDevice, rs));
         if (!p->dont_normalize)
           CHK(normalize_impl<S>(rc->T[L], N, 1, p->saturation_pct, p->normalize_min_to_zero, rs));
+        if (p->pre_smooth_sigma > 0)
+          CHK(h1_impl<S>(rc->T[L], rc->T[L], N, 1, p->pre_smooth_sigma / (double)(N - 1), rs));
         CHK(rebuild_pyramid<S>(rc, rc->T, -1, 1));
         HIPCHK(hipMemcpyAsync(rc->phi[L], acc.p, sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, rs));
         CHK(rebuild_pyramid<S>(rc, rc->phi, -1, 2));
@@ -1117,6 +1205,38 @@ int msg_match_series(const msg_series_params *p, int dtype, const void *template
                                      mean, count, median);
   return match_series_impl<float>(p, templates, n, reference, pairwise_in, deformations, pairwise_out, energies, mean,
                                   count, median);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int msg_profile_enable(unsigned mask) {
+  std::lock_guard<std::mutex> g(g_prof.mu);
+  prof_resolve();
+  g_prof.mask = mask;
+  return MSG_OK;
+}
+
+int msg_profile_reset(void) {
+  std::lock_guard<std::mutex> g(g_prof.mu);
+  prof_resolve();
+  for (int k = 0; k < KC_COUNT; ++k) {
+    g_prof.ms[k] = 0;
+    g_prof.launches[k] = 0;
+    g_prof.bytes[k] = 0;
+  }
+  return MSG_OK;
+}
+
+int msg_profile_read(int cls, double *ms, long long *launches, double *bytes) {
+  if (cls < 0 || cls >= KC_COUNT) return fail(MSG_ERR_ARG, "unknown kernel class");
+  std::lock_guard<std::mutex> g(g_prof.mu);
+  prof_resolve();
+  if (ms) *ms = g_prof.ms[cls];
+  if (launches) *launches = g_prof.launches[cls];
+  if (bytes) *bytes = g_prof.bytes[cls];
+  return MSG_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,347 @@
+// matchSeries <file.par> [ACTION] — the reference's command line (projects/electronMicroscopy/
+// matchSeries.cpp:11-209, matchSeries.h:1249-1320) over the gfx950 library's C ABI.
+//
+// deformationModel 22 (SSD data term, Dirichlet regulariser, H1 gradient flow) is computed on the
+// GPU; ACTION 0 (match and average) and 1 (only average) are supported. The output tree is the
+// reference's: <saveDirectory>/{log.txt, parameter-dump.txt, stage<k>/...}. Errors are reported as
+// the reference does (message on stderr, exit code 0); MSG_STRICT=1 makes them exit with 1.
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <memory>
+#include <sstream>
+#include <streambuf>
+#include <string>
+#include <vector>
+
+#include "array_io.h"
+#include "ms_gpu.h"
+#include "params.h"
+
+using namespace mshost;
+
+namespace {
+
+// Duplicates everything written to a stream into log.txt (aol::AdditionalOutputToFile,
+// multiStreambuf.h:49-63).
+class TeeBuf : public std::streambuf {
+ public:
+  TeeBuf(std::streambuf *a, std::streambuf *b) : a_(a), b_(b) {}
+  int overflow(int c) override {
+    if (c == EOF) return !EOF;
+    const int r1 = a_->sputc((char)c), r2 = b_->sputc((char)c);
+    return (r1 == EOF || r2 == EOF) ? EOF : c;
+  }
+  int sync() override { return (a_->pubsync() == 0 && b_->pubsync() == 0) ? 0 : -1; }
+
+ private:
+  std::streambuf *a_, *b_;
+};
+
+void hipCheck(hipError_t e, const char *what) {
+  if (e != hipSuccess) MSH_THROW(strprintf("%s: %s", what, hipGetErrorString(e)));
+}
+void msgCheck(int rc) {
+  if (rc != MSG_OK) MSH_THROW(strprintf("libmsgpu: %s", msg_last_error()));
+}
+
+struct DeviceBuffer {
+  void *p = nullptr;
+  explicit DeviceBuffer(size_t bytes) { hipCheck(hipMalloc(&p, bytes ? bytes : 16), "hipMalloc"); }
+  ~DeviceBuffer() {
+    if (p) (void)hipFree(p);
+  }
+  DeviceBuffer(const DeviceBuffer &) = delete;
+  DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+};
+
+// host float64 -> device field type
+void upload(void *dst, const double *src, size_t n, int dtype) {
+  if (dtype == MSG_F64) {
+    hipCheck(hipMemcpy(dst, src, n * 8, hipMemcpyHostToDevice), "upload");
+  } else {
+    std::vector<float> t(src, src + n);
+    hipCheck(hipMemcpy(dst, t.data(), n * 4, hipMemcpyHostToDevice), "upload");
+  }
+}
+void download(double *dst, const void *src, size_t n, int dtype) {
+  if (dtype == MSG_F64) {
+    hipCheck(hipMemcpy(dst, src, n * 8, hipMemcpyDeviceToHost), "download");
+  } else {
+    std::vector<float> t(n);
+    hipCheck(hipMemcpy(t.data(), src, n * 4, hipMemcpyDeviceToHost), "download");
+    for (size_t i = 0; i < n; ++i) dst[i] = t[i];
+  }
+}
+
+std::string dirOf(const ParameterParser &p) { return p.getStringExpandTilde("saveDirectory"); }
+
+// Options of the reference that change the computation but are not implemented here.
+void rejectUnsupported(const ParameterParser &p) {
+  const char *flags[] = {"reduceDeformations", "useCorrelationToInitTranslation", "reverseRolesInSeriesMatching",
+                         "calcInverseDeformation", "cropInput", "resizeInput", "calcIQ", "calcPrecision",
+                         "performNoiseAnalysis"};
+  for (const char *f : flags)
+    if (p.checkAndGetBool(f)) MSH_THROW(strprintf("\"%s 1\" is not supported by this build", f));
+  if (p.hasVariable("altStartLevel") && p.getInt("altStartLevel") != p.getInt("startLevel"))
+    MSH_THROW("\"altStartLevel\" is not supported by this build");
+  if (p.hasVariable("initial-deformation")) MSH_THROW("\"initial-deformation\" is not supported by this build");
+  if (p.hasVariable("minimizationAlgo") && p.getString("minimizationAlgo") != "GradientDescent")
+    MSH_THROW("only \"minimizationAlgo GradientDescent\" is supported by this build");
+}
+
+msg_series_params seriesParams(const ParameterParser &p) {
+  msg_series_params s;
+  memset(&s, 0, sizeof(s));
+  s.precision_level = p.getInt("precisionLevel");
+  s.start_level = p.getInt("startLevel");
+  s.stop_level = p.getInt("stopLevel");
+  s.refine_start_level = p.getIntOrDefault("refineStartLevel", s.precision_level);
+  s.refine_stop_level = p.getIntOrDefault("refineStopLevel", s.precision_level);
+  s.lambda = p.getDouble("lambda");
+  s.lambda_factor = p.hasVariable("lambdaFactor") ? p.getDouble("lambdaFactor") : -1.0;
+  s.max_iter = p.getIntOrDefault("maxGDIterations", 1000);  // StandardRegistration defaults (registration.h:940-950)
+  s.stop_eps = p.getDoubleOrDefault("stopEpsilon", 0.0);
+  s.saturation_pct = p.hasVariable("enhanceContrastSaturationPercentage")
+                         ? p.getDouble("enhanceContrastSaturationPercentage") : -1.0;
+  s.normalize_min_to_zero = p.checkAndGetBool("normalizeMinToZero");
+  s.dont_normalize = p.checkAndGetBool("dontNormalizeInputImages");
+  s.resample_prolong = p.checkAndGetBool("resampleInsteadOfProlongateDeformation");
+  s.dont_accumulate = p.checkAndGetBool("dontAccumulateDeformation");
+  s.spec = p.getIntOrDefault("gpuArmijoCandidates", 0);
+  s.pre_smooth_sigma = p.getDoubleOrDefault("preSmoothSigma", 0.0);
+  return s;
+}
+
+int fieldType(const ParameterParser &p) { return p.getIntOrDefault("gpuFieldPrecision", 32) == 64 ? MSG_F64 : MSG_F32; }
+
+Image loadChecked(const std::string &f, int N) {
+  Image im = loadImage(f);
+  if (im.nx != N || im.ny != N)
+    MSH_THROW(strprintf("Input data size (%dx%d of %s) doesn't match the size of the computational grid (%dx%d).",
+                        im.nx, im.ny, f.c_str(), N, N));
+  return im;
+}
+
+// <dir><i>[-r]/deformation_<LL> (SeriesMatching::createDeformationBaseFileName, matchSeries.h:58-67)
+std::string deformationBase(const ParameterParser &p, const std::string &dir, int i, int maxDepth) {
+  const bool refined = !p.checkAndGetBool("dontAccumulateDeformation") && i > 0;
+  return strprintf("%s%d%s/deformation_%02d", dir.c_str(), i, refined ? "-r" : "",
+                   refined ? maxDepth : p.getInt("stopLevel"));
+}
+
+void writeReconstruction(const std::string &dir, const std::vector<double> &mean, const std::vector<double> &median,
+                         const std::vector<double> &count, int N) {
+  saveP9(dir + "average.q2bz", mean.data(), N, N);
+  saveP9(dir + "median.q2bz", median.data(), N, N);
+  saveP9(dir + "numSamples.q2bz", count.data(), N, N);
+}
+
+// ACTION 1: SeriesMatching::averageSeries (matchSeries.h:353-451) from saved deformations.
+void averageOnly(const ParameterParser &p) {
+  const std::string dir = dirOf(p);
+  const int L = p.getInt("precisionLevel"), N = 1 << L;
+  const size_t nn = (size_t)N * N;
+  const int dt = fieldType(p), es = dt == MSG_F64 ? 8 : 4;
+  const std::vector<std::string> files = createTemplateFileNameList(p);
+  const int n = (int)files.size();
+  DeviceBuffer img(es * nn * n), defs(es * 2 * nn * n), stack(es * nn * n), mean(es * nn), cnt(es * nn), med(es * nn);
+  double lastMean = 0;
+  for (int i = 0; i < n; ++i) {
+    const Image im = loadChecked(files[i], N);
+    upload((char *)img.p + (size_t)i * es * nn, im.data.data(), nn, dt);
+    std::vector<double> phi;
+    loadField(deformationBase(p, dir, i, L) + "_%d.dat.bz2", N, phi);
+    upload((char *)defs.p + (size_t)i * es * 2 * nn, phi.data(), 2 * nn, dt);
+    if (i == n - 1) {
+      double s = 0;
+      for (double v : im.data) s += v;
+      lastMean = s / nn;
+    }
+  }
+  msgCheck(msg_warp(dt, img.p, defs.p, stack.p, N, n, 0, 0, INFINITY, 0, nullptr));
+  msgCheck(msg_series_stats(dt, stack.p, n, N, lastMean, mean.p, cnt.p, med.p, nullptr));
+  std::vector<double> hm(nn), hc(nn), hmed(nn);
+  download(hm.data(), mean.p, nn, dt);
+  download(hc.data(), cnt.p, nn, dt);
+  download(hmed.data(), med.p, nn, dt);
+  writeReconstruction(dir, hm, hmed, hc, N);
+}
+
+// ACTION 0 of one stage: SeriesMatching::matchAndAverageSeries (matchSeries.h:565-570).
+void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int stage) {
+  const std::string dir = dirOf(p);
+  const int L = p.getInt("precisionLevel"), N = 1 << L;
+  const size_t nn = (size_t)N * N;
+  const int dt = fieldType(p), es = dt == MSG_F64 ? 8 : 4;
+  const std::vector<std::string> files = createTemplateFileNameList(p);
+  const int n = (int)files.size();
+  if (n == 0) MSH_THROW("no template images");
+  const msg_series_params sp = seriesParams(p);
+
+  DeviceBuffer tpl(es * nn * n), ref(es * nn), defs(es * 2 * nn * n), pw(es * 2 * nn * n), mean(es * nn),
+      cnt(es * nn), med(es * nn);
+  for (int i = 0; i < n; ++i) {
+    const Image im = loadChecked(files[i], N);
+    upload((char *)tpl.p + (size_t)i * es * nn, im.data.data(), nn, dt);
+  }
+  {
+    const Image r = loadChecked(p.getStringExpandTilde("reference"), N);
+    upload(ref.p, r.data.data(), nn, dt);
+  }
+  std::unique_ptr<DeviceBuffer> pin;
+  if (loadStageOneResults && n > 1) {
+    // pairwise deformation of template i is stage 1's deformation of its template i-1
+    pin.reset(new DeviceBuffer(es * 2 * nn * n));
+    for (int i = 1; i < n; ++i) {
+      std::vector<double> phi;
+      loadField(strprintf("%s../stage1/%d/deformation_%02d_%%d.dat.bz2", dir.c_str(), i - 1, L), N, phi);
+      upload((char *)pin->p + (size_t)i * es * 2 * nn, phi.data(), 2 * nn, dt);
+    }
+  }
+  std::vector<double> energies(n);
+  const auto t0 = std::chrono::steady_clock::now();
+  msgCheck(msg_match_series(&sp, dt, tpl.p, n, ref.p, pin ? pin->p : nullptr, defs.p, pw.p, energies.data(), mean.p,
+                            cnt.p, med.p));
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::cerr << "Registered " << n << " templates on the GPU in " << secs << " s\n";
+
+  std::vector<double> hdefs(2 * nn * n), hpw(2 * nn * n);
+  download(hdefs.data(), defs.p, 2 * nn * n, dt);
+  download(hpw.data(), pw.p, 2 * nn * n, dt);
+  std::ofstream defNorms, energiesFile(dir + "/energies.txt");
+  if (!loadStageOneResults) defNorms.open(dir + "/defNorms.txt");
+  const bool accumulate = !p.checkAndGetBool("dontAccumulateDeformation");
+  const int refineStop = sp.refine_stop_level;
+  for (int i = 0; i < n; ++i) {
+    const double *phw = hpw.data() + (size_t)i * 2 * nn;
+    if (!loadStageOneResults || i == 0) {
+      const std::string sub = strprintf("%s%d/", dir.c_str(), i);
+      makeDirectory(sub);
+      saveField(sub + strprintf("deformation_%02d_%%d.dat.bz2", L), phw, N);
+      if (!loadStageOneResults) {
+        double norm = 0;
+        for (int c = 0; c < 2; ++c) {
+          double s = 0;
+          for (size_t k = 0; k < nn; ++k) s += phw[c * nn + k] * phw[c * nn + k];
+          norm += std::sqrt(s);
+        }
+        defNorms << i << " " << norm << " # " << files[i] << "\n";
+      }
+    }
+    if (accumulate && i > 0) {
+      const std::string sub = strprintf("%s%d-r/", dir.c_str(), i);
+      makeDirectory(sub);
+      const double *phd = hdefs.data() + (size_t)i * 2 * nn;
+      saveField(sub + strprintf("deformation_%02d_%%d.dat.bz2", L), phd, N);
+      if (refineStop < L) saveField(sub + strprintf("deformation_%02d_%%d.dat.bz2", refineStop), phd, N);
+    }
+    energiesFile << i << " " << energies[i] << "\n";
+  }
+  std::vector<double> hm(nn), hc(nn), hmed(nn);
+  download(hm.data(), mean.p, nn, dt);
+  download(hc.data(), cnt.p, nn, dt);
+  download(hmed.data(), med.p, nn, dt);
+  writeReconstruction(dir, hm, hmed, hc, N);
+  (void)stage;
+}
+
+void doAction(ParameterParser &p, int action, bool loadStageOneResults, int stage) {
+  // the registration object's constructor prepares the stage directory (registration.h:263-274)
+  makeDirectory(dirOf(p));
+  p.dumpToFile("parameter-dump.txt", dirOf(p));
+  switch (action) {
+    case 0: matchAndAverage(p, loadStageOneResults, stage); break;
+    case 1: averageOnly(p); break;
+    default: MSH_THROW(strprintf("ACTION %d is not supported by this build (0: match and average, 1: average)", action));
+  }
+}
+
+// matchSeries<ConfType, RegisType> (matchSeries.h:1249-1320)
+void runMatchSeries(ParameterParser &parser, int argc, char **argv) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int numExtraStages = parser.getInt("numExtraStages");
+  const double extraStagesLambdaFactor = parser.getDouble("extraStagesLambdaFactor");
+  const std::string saveDir = dirOf(parser);
+  makeDirectory(saveDir);
+  parser.dumpToFile("parameter-dump.txt", saveDir);
+  std::ofstream logFile(saveDir + "log.txt");
+  TeeBuf tee(std::cerr.rdbuf(), logFile.rdbuf());
+  std::streambuf *old = std::cerr.rdbuf(&tee);
+  struct Restore {
+    std::streambuf *o;
+    ~Restore() { std::cerr.rdbuf(o); }
+  } restore{old};
+  std::cerr << "matchSeries on " << msg_version() << "\n";
+
+  if (parser.hasVariable("templateNamePattern")) {
+    if (parser.hasVariable("reference"))
+      MSH_THROW("Paramater \"reference\" may not be specified if \"templateNamePattern\" is used.");
+    if (!parser.hasVariable("initialAverage")) {
+      parser.addVariable("reference", strprintf(parser.getString("templateNamePattern").c_str(),
+                                                parser.getInt("templateNumOffset")));
+      parser.changeVariableValue("templateNumOffset", parser.getInt("templateNumOffset") + parser.getInt("templateNumStep"));
+      parser.changeVariableValue("numTemplates", parser.getInt("numTemplates") - 1);
+    } else {
+      parser.addVariable("reference", parser.getStringExpandTilde("initialAverage"));
+    }
+  }
+  if (!parser.hasVariable("template")) parser.addVariable("template", parser.getString("reference"));
+  rejectUnsupported(parser);
+
+  ParameterParser stage1(parser);
+  stage1.changeVariableValue("saveDirectory", saveDir + "stage1/");
+  const int action = argc > 2 ? atoi(argv[2]) : 0;
+  if (!parser.checkAndGetBool("skipStage1")) doAction(stage1, action, false, 1);
+  const std::string averageName = std::string(parser.checkAndGetBool("useMedianAsNewTarget") ? "median" : "average") + ".q2bz";
+  std::string lastAverage = saveDir + "stage1/" + averageName;
+
+  if (numExtraStages > 0) {
+    ParameterParser stage2(parser);
+    if (!parser.hasVariable("initialAverage")) {
+      stage2.changeVariableValue("templateNumOffset", parser.getInt("templateNumOffset") - parser.getInt("templateNumStep"));
+      stage2.changeVariableValue("numTemplates", parser.getInt("numTemplates") + 1);
+    }
+    // the reference passes lambda through "%f"
+    stage2.changeVariableValue("lambda", strprintf("%f", parser.getDouble("lambda") * extraStagesLambdaFactor));
+    for (int i = 0; i < numExtraStages; ++i) {
+      ParameterParser stageN(stage2);
+      stageN.changeVariableValue("saveDirectory", strprintf("%sstage%d/", saveDir.c_str(), i + 2));
+      stageN.changeVariableValue("reference", lastAverage);
+      stageN.addVariable("stage", i + 2);
+      doAction(stageN, action, parser.checkAndGetBool("reuseStage1Results"), i + 2);
+      lastAverage = strprintf("%sstage%d/", saveDir.c_str(), i + 2) + averageName;
+    }
+  }
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::cerr << "--------------------------------------------------------------\n"
+            << "Wall clock time = " << secs << " s\n"
+            << "All output has been written to file " << saveDir << "log.txt\n";
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  const char *strictEnv = getenv("MSG_STRICT");
+  const bool strict = strictEnv && atoi(strictEnv) == 1;
+  try {
+    ParameterParser parser(argc > 1 ? argv[1] : "matchSeries.par");
+    const int model = parser.getIntOrDefault("deformationModel", 0);
+    if (model != 22)
+      MSH_THROW(strprintf("deformationModel %d is not supported by this build (22: SSD + Dirichlet regulariser)", model));
+    runMatchSeries(parser, argc, argv);
+  } catch (Exception &e) {
+    e.dump(std::cerr);
+    return strict ? 1 : 0;
+  } catch (std::exception &e) {
+    std::cerr << "error: " << e.what() << "\n";
+    return strict ? 1 : 0;
+  }
+  return 0;
+}

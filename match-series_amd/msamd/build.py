@@ -33,18 +33,25 @@ def build_lib():
 
 
 def build_cli():
+    """bin/matchSeries (the drop-in CLI) and bin/msgio (array file utility), sharing params/array_io."""
     host = os.path.join(PKG, "host")
-    srcs = sorted(os.path.join(host, f) for f in os.listdir(host) if f.endswith(".cpp")) if os.path.isdir(host) else []
-    if not srcs:
-        return None
+    common = [os.path.join(host, f) for f in ("params.cpp", "array_io.cpp")]
     os.makedirs(os.path.join(PKG, "bin"), exist_ok=True)
-    out = os.path.join(PKG, "bin", "matchSeries")
     libdir = os.path.join(PKG, "lib")
-    run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", host, "-I", "/opt/rocm/include",
-         "-D__HIP_PLATFORM_AMD__"] + srcs + ["-o", out, "-L", libdir, "-lmsgpu", f"-Wl,-rpath,$ORIGIN/../lib",
-                                              "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib",
-                                              "/usr/lib/x86_64-linux-gnu/libbz2.so.1.0"])
-    return out
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", host, "-I", "/opt/rocm/include",
+           "-idirafter", "/opt/conda/include",  # bzlib.h (the library itself is the system libbz2)
+           "-D__HIP_PLATFORM_AMD__"]
+    bz2 = "/usr/lib/x86_64-linux-gnu/libbz2.so.1.0"
+    outs = []
+    out = os.path.join(PKG, "bin", "matchSeries")
+    run(["g++", "-std=c++17", "-O2", "-Wall"] + inc + common + [os.path.join(host, "matchSeries.cpp"), "-o", out,
+         "-L", libdir, "-lmsgpu", "-Wl,-rpath,$ORIGIN/../lib", "-L/opt/rocm/lib", "-lamdhip64",
+         "-Wl,-rpath,/opt/rocm/lib", bz2])
+    outs.append(out)
+    out = os.path.join(PKG, "bin", "msgio")
+    run(["g++", "-std=c++17", "-O2", "-Wall"] + inc + common + [os.path.join(host, "msgio.cpp"), "-o", out, bz2])
+    outs.append(out)
+    return outs
 
 
 def build_oracle(with_reference=True):

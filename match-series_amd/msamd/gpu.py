@@ -38,7 +38,7 @@ class SeriesParams(ctypes.Structure):
         ("max_iter", ctypes.c_int), ("stop_eps", ctypes.c_double),
         ("saturation_pct", ctypes.c_double), ("normalize_min_to_zero", ctypes.c_int),
         ("dont_normalize", ctypes.c_int), ("resample_prolong", ctypes.c_int),
-        ("dont_accumulate", ctypes.c_int), ("spec", ctypes.c_int),
+        ("dont_accumulate", ctypes.c_int), ("spec", ctypes.c_int), ("pre_smooth_sigma", ctypes.c_double),
     ]
 
 
@@ -74,6 +74,9 @@ SIGNATURES = {
     "msg_series_stats": ([ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _vp, _vp, _vp],
                          ctypes.c_int),
     "msg_normalize": ([ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp], ctypes.c_int),
+    "msg_profile_enable": ([ctypes.c_uint], ctypes.c_int),
+    "msg_profile_reset": ([], ctypes.c_int),
+    "msg_profile_read": ([ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong), _dp], ctypes.c_int),
     "msg_match_series": ([ctypes.POINTER(SeriesParams), ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _dp, _vp,
                           _vp, _vp], ctypes.c_int),
 }
@@ -312,3 +315,26 @@ def match_series(templates, reference, params, pairwise_in=None, want_pairwise=F
                                  _ptr(pairwise_in) if pairwise_in is not None else None, _ptr(defs),
                                  _ptr(pw) if pw is not None else None, _dptr(en), _ptr(mean), _ptr(cnt), _ptr(med)))
     return dict(deformations=defs, pairwise=pw, energies=en, mean=mean, count=cnt, median=med)
+
+
+# ------------------------------------------------------------------------------------------------
+# Kernel timing
+# ------------------------------------------------------------------------------------------------
+KERNEL_CLASSES = ("energy", "variation", "h1_vcycle", "cg_direction", "update", "control", "composition")
+
+
+def profile_enable(classes=("energy",)):
+    mask = 0
+    for c in classes:
+        mask |= 1 << KERNEL_CLASSES.index(c)
+    check(lib().msg_profile_enable(mask))
+
+
+def profile_reset():
+    check(lib().msg_profile_reset())
+
+
+def profile_read(cls):
+    ms, n, b = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double()
+    check(lib().msg_profile_read(KERNEL_CLASSES.index(cls), ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b)))
+    return dict(ms=ms.value, launches=n.value, bytes=b.value)

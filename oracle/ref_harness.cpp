@@ -196,6 +196,12 @@ int main(int argc, char **argv) {
       }
       stack.getMedianVecOverComponents(med, fill);
       append(out, mean); append(out, cnt); append(out, med);
+    } else if (cmd == "load") {
+      // params: file ; in: ignored ; out: nx, ny, then the array loaded by qc::ScalarArray<QC_2D>(file)
+      qc::ScalarArray<RType, qc::QC_2D> a(argv[3]);
+      out.push_back(a.getNumX());
+      out.push_back(a.getNumY());
+      append(out, a);
     } else if (cmd == "solve") {
       // params: lambda maxIt stopEps ; in: R, T, phix0, phiy0
       // out: phix, phiy, iterations, E_final, then per iteration k: E_total(k), tau(k), sigma(k), evals(k)

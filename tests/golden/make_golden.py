@@ -117,6 +117,10 @@ def main():
         o = harness("median", 16, [5, 0.375], [st], tmp)
         out["stats16"] = dict(stack=st, fill=0.375, mean=o[:256].reshape(16, 16), count=o[256:512].reshape(16, 16),
                               median=o[512:].reshape(16, 16))
+        # the reference's own image fixture, as its array loader reads it
+        o = harness("load", 2, [os.path.join(HERE, "image_129.pgm.bz2")], [np.zeros(1)], tmp)
+        out["image129"] = dict(nx=int(o[0]), ny=int(o[1]), data=o[2:].reshape(int(o[1]), int(o[0])))
+        # a P9 file written by the reference (stage-1 median of the end-to-end run) is covered below
         # complete level solves with their traces
         for N, lam, its, eps in ((32, 10.0, 50, -1.0), (64, 10.0, 50, -1.0), (64, 1.0, 200, 1e-6)):
             s = stem_pair(N)
