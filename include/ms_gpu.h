@@ -167,6 +167,9 @@ typedef struct {
 int msg_match_series(const msg_series_params *p, int dtype, const void *templates, int n, const void *reference,
                      const void *pairwise_in, void *deformations, void *pairwise_out, double *energies,
                      void *mean, void *count, void *median);
+/* msg_match_series keeps its device contexts for reuse by later calls of the same shape (no
+ * counterpart in the reference); this frees them. */
+int msg_release_cache(void);
 
 /* ------------------------------------------------------------------------------------------------
  * Kernel timing (no counterpart in the reference; used by the benchmark). With class bit k set in

@@ -63,16 +63,16 @@ int log2i(int n) {
 // ------------------------------------------------------------------------------------------------
 // Grow-only device scratch for the stateless operator entry points (one set per device).
 // ------------------------------------------------------------------------------------------------
+// Per host thread and device, so that concurrent callers (one context per thread) never share it.
 struct Scratch {
-  std::mutex mu;
   std::vector<std::pair<void *, size_t>> bufs;  // slot -> (ptr, bytes)
 };
-Scratch g_scratch[64];
+thread_local Scratch g_scratch[16];
 
 int scratch_get(int slot, size_t bytes, void **p) {
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
-  Scratch &s = g_scratch[dev & 63];
+  Scratch &s = g_scratch[dev & 15];
   if ((int)s.bufs.size() <= slot) s.bufs.resize(slot + 1, {nullptr, 0});
   auto &b = s.bufs[slot];
   if (b.second < bytes) {
@@ -335,7 +335,7 @@ int launch_resample(const S *src, S *dst, int nc, int n, int nfields, const doub
 // Context
 // =================================================================================================
 struct msg_ctx {
-  int L = 0, B = 0, dtype = MSG_F32;
+  int L = 0, B = 0, dtype = MSG_F32, device = 0;
   size_t es = 4;
   void *R[MAXL + 1] = {}, *T[MAXL + 1] = {}, *phi[MAXL + 1] = {};
   void *DE = nullptr, *z = nullptr, *d = nullptr, *x2 = nullptr;
@@ -573,6 +573,10 @@ int msg_ctx_create(msg_ctx **out, int max_level, int npairs, int dtype) {
   if (max_level < 1 || max_level > MAXL) return fail(MSG_ERR_ARG, "max_level out of range");
   if (npairs < 1) return fail(MSG_ERR_ARG, "npairs must be positive");
   msg_ctx *c = new msg_ctx();
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    delete c;
+    return fail(MSG_ERR_HIP, "no HIP device");
+  }
   c->L = max_level;
   c->B = npairs;
   c->dtype = dtype;
@@ -1034,6 +1038,37 @@ struct DevBuf {
   }
 };
 
+// Contexts of msg_match_series are kept for reuse by later calls with the same shape (a stage is
+// typically run many times on series of one size); each is handed to one call at a time.
+struct CtxCache {
+  std::mutex mu;
+  std::vector<msg_ctx *> free;
+};
+CtxCache g_ctx_cache;
+
+int ctx_acquire(int L, int B, int dtype, msg_ctx **out) {
+  {
+    std::lock_guard<std::mutex> g(g_ctx_cache.mu);
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    for (size_t i = 0; i < g_ctx_cache.free.size(); ++i) {
+      msg_ctx *c = g_ctx_cache.free[i];
+      if (c->L == L && c->B == B && c->dtype == dtype && c->device == dev) {
+        g_ctx_cache.free.erase(g_ctx_cache.free.begin() + i);
+        *out = c;
+        return MSG_OK;
+      }
+    }
+  }
+  return msg_ctx_create(out, L, B, dtype);
+}
+
+void ctx_release(msg_ctx *c) {
+  if (!c) return;
+  std::lock_guard<std::mutex> g(g_ctx_cache.mu);
+  g_ctx_cache.free.push_back(c);
+}
+
 template <typename S>
 int match_series_impl(const msg_series_params *p, const void *templates, int n, const void *reference,
                       const void *pairwise_in, void *deformations, void *pairwise_out, double *energies, void *mean,
@@ -1056,10 +1091,10 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
 
   // --- batched pairwise solves: template i against its predecessor (or the reference) ---
   msg_ctx *pc = nullptr;
-  CHK(msg_ctx_create(&pc, L, npw, dt));
+  CHK(ctx_acquire(L, npw, dt, &pc));
   struct Guard {
     msg_ctx *c;
-    ~Guard() { msg_ctx_destroy(c); }
+    ~Guard() { ctx_release(c); }
   } gpc{pc};
   hipStream_t s = pc->stream;
   for (int i = 0; i < npw; ++i) {
@@ -1098,7 +1133,7 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
   } else {
     // --- sequential chain: accumulate by composition, refine against the reference ---
     msg_ctx *rc = nullptr;
-    CHK(msg_ctx_create(&rc, L, 1, dt));
+    CHK(ctx_acquire(L, 1, dt, &rc));
     Guard grc{rc};
     HIPCHK(hipStreamSynchronize(s));
     hipStream_t rs = rc->stream;
@@ -1236,6 +1271,17 @@ int msg_profile_read(int cls, double *ms, long long *launches, double *bytes) {
   if (ms) *ms = g_prof.ms[cls];
   if (launches) *launches = g_prof.launches[cls];
   if (bytes) *bytes = g_prof.bytes[cls];
+  return MSG_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int msg_release_cache(void) {
+  std::lock_guard<std::mutex> g(g_ctx_cache.mu);
+  for (msg_ctx *c : g_ctx_cache.free) msg_ctx_destroy(c);
+  g_ctx_cache.free.clear();
   return MSG_OK;
 }
 

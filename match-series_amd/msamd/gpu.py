@@ -74,6 +74,7 @@ SIGNATURES = {
     "msg_series_stats": ([ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _vp, _vp, _vp],
                          ctypes.c_int),
     "msg_normalize": ([ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp], ctypes.c_int),
+    "msg_release_cache": ([], ctypes.c_int),
     "msg_profile_enable": ([ctypes.c_uint], ctypes.c_int),
     "msg_profile_reset": ([], ctypes.c_int),
     "msg_profile_read": ([ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong), _dp], ctypes.c_int),

@@ -1,0 +1,67 @@
+"""The matchSeries drop-in end to end on the GPU, against the reference program's own output tree for
+the same parameter file and frames (tests/golden/e2e64.npz: 4 STEM frames of 64x64, two stages)."""
+import bz2
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "match-series_amd", "bin", "matchSeries")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def read_p9(path):
+    b = open(path, "rb").read()
+    if path.endswith((".bz2", ".q2bz")):
+        b = bz2.decompress(b)
+    fields, pos = [], 0
+    while len(fields) < 3:
+        e = b.index(b"\n", pos)
+        line, pos = b[pos:e], e + 1
+        if not line.startswith(b"#"):
+            fields.append(line)
+    W, H = map(int, fields[1].split())
+    return np.frombuffer(b[pos:pos + 8 * W * H], dtype="<f8").reshape(H, W)
+
+
+def write_p9(path, img):
+    hdr = f"P9\n# synthetic frame\n{img.shape[1]} {img.shape[0]}\n{int(img.max())}\n".encode()
+    with open(path, "wb") as f:
+        f.write(bz2.compress(hdr + np.ascontiguousarray(img, dtype="<f8").tobytes()))
+
+
+@pytest.mark.parametrize("precision,tol_s1,tol_s2", [(64, 1e-6, 1e-3), (32, 1e-3, 1e-2)])
+def test_matchseries_two_stages(tmp_path, precision, tol_s1, tol_s2):
+    if not os.path.exists(EXE):
+        pytest.skip("bin/matchSeries not built")
+    with np.load(os.path.join(GOLD, "e2e64.npz")) as z:
+        g = {k: z[k] for k in z.files}
+    (tmp_path / "in").mkdir()
+    for i, f in enumerate(g["frames"]):
+        write_p9(str(tmp_path / "in" / f"frame_{i:02d}.dat.bz2"), f)
+    par = bytes(g["par"]).decode() + f"gpuFieldPrecision {precision}\n"
+    (tmp_path / "run.par").write_text(par)
+    r = subprocess.run([EXE, "run.par"], cwd=tmp_path, capture_output=True, text=True,
+                       env=dict(os.environ, MSG_STRICT="1"), timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = tmp_path / "out"
+    N = 64
+    for st, nt in ((1, 3), (2, 4)):
+        tol = tol_s1 if st == 1 else tol_s2
+        for i in range(nt):
+            sub = f"{i}" if i == 0 else f"{i}-r"
+            d = np.stack([read_p9(str(out / f"stage{st}" / sub / f"deformation_06_{c}.dat.bz2")) for c in range(2)])
+            assert np.abs(d - g[f"s{st}_deformations"][i]).max() * (N - 1) < tol, (st, i)
+        cnt = read_p9(str(out / f"stage{st}" / "numSamples.q2bz"))
+        assert np.abs(cnt - g[f"s{st}_numSamples"]).sum() <= (0 if st == 1 and precision == 64 else 8)
+        med = read_p9(str(out / f"stage{st}" / "median.q2bz"))
+        ok = cnt == g[f"s{st}_numSamples"]
+        assert np.abs(med - g[f"s{st}_median"])[ok].max() < (1e-6 if st == 1 and precision == 64 else 1e-2)
+        en = np.loadtxt(str(out / f"stage{st}" / "energies.txt"))[:, 1]
+        np.testing.assert_allclose(en, g[f"s{st}_energies"], rtol=1e-4 if precision == 64 else 1e-3)
+    assert (out / "log.txt").exists() and (out / "stage1" / "parameter-dump.txt").exists()
+    assert (out / "stage1" / "defNorms.txt").exists()
