@@ -79,6 +79,7 @@ def main():
     ap.add_argument("--iters", type=int, default=100)
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
     ap.add_argument("--spec", type=int, default=0)
+    ap.add_argument("--mode", type=int, default=2, help="0 host-synchronised, 1 device-driven, 2 + hipGraph")
     ap.add_argument("--cpu-iters", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -92,6 +93,7 @@ def main():
     from msamd import gpu, synth
 
     torch.cuda.set_device(0)
+    gpu.set_default_mode(args.mode)
     N, F = args.size, args.frames
     L = int(np.log2(N))
     levels, refine = (L - 3, L), (L - 1, L)
@@ -135,7 +137,8 @@ def main():
         "dtype": args.precision, "data": "synthetic (seeded STEM lattice series, Poisson noise, scan distortion)",
         "config": {"workload": f"plain matchSeries stage 1: {F} frames of {N}x{N}, levels {levels[0]}-{levels[1]}, "
                                f"refine {refine[0]}-{refine[1]}, {args.iters} it/level, lambda 10, SSD",
-                   "frames": F, "size": N, "iterations_per_level": args.iters, "method": "plain"},
+                   "frames": F, "size": N, "iterations_per_level": args.iters, "method": "plain",
+                   "iteration_mode": args.mode, "armijo_candidates": args.spec or 3},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "energies_final_mean": float(np.mean(out["energies"])) if out is not None else None,

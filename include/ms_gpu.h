@@ -88,6 +88,14 @@ int msg_solve_level(msg_ctx *ctx, int level, double lambda, int max_iter, double
  * MultilevelDescentInterface::solve, quoc/quocDescent.h:83-105); energies = getEnergyOfLastSolution. */
 int msg_solve(msg_ctx *ctx, int start_level, int stop_level, const msg_solve_params *p, double *energies);
 
+/* How a level solve is driven (no counterpart in the reference; results are identical):
+ * 0: the host reads the per-pair flags after every Armijo energy pass;
+ * 1: device-driven — control runs in the last workgroup of each kernel, the host checks flags
+ *    every 8 iterations; 2: as 1 with the 8 iterations replayed as one hipGraph. Default 0.
+ * msg_set_default_mode applies to contexts created later and to msg_match_series. */
+int msg_ctx_set_mode(msg_ctx *ctx, int mode);
+int msg_set_default_mode(int mode);
+
 /* Totals of the last msg_solve / msg_solve_level: iterations, reference-equivalent energy
  * evaluations and energy passes summed over pairs and levels. */
 int msg_last_counts(msg_ctx *ctx, long long *iterations, long long *evals, long long *passes);

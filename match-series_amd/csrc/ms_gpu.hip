@@ -106,6 +106,8 @@ struct Profiler {
   double bytes[KC_COUNT] = {};
 };
 Profiler g_prof;
+thread_local bool g_capturing = false;  // no event timing inside a stream capture
+int g_default_mode = 0;
 
 hipEvent_t prof_event() {
   if (!g_prof.pool.empty()) {
@@ -125,7 +127,7 @@ struct ProfScope {
   hipStream_t s;
   double bytes;
   hipEvent_t a = nullptr;
-  ProfScope(int c, hipStream_t st, double b) : on((g_prof.mask >> c) & 1u), cls(c), s(st), bytes(b) {
+  ProfScope(int c, hipStream_t st, double b) : on(((g_prof.mask >> c) & 1u) && !g_capturing), cls(c), s(st), bytes(b) {
     if (on) {
       std::lock_guard<std::mutex> g(g_prof.mu);
       a = prof_event();
@@ -210,13 +212,13 @@ EnergyGeom energy_geom(int n) {
 
 template <typename S>
 int launch_energy(const S *R, const S *T, const S *phi, const S *dcg, const S *dgrad, int n, const PairState *st,
-                  double *part, int npairs, hipStream_t s, double bytes = 0) {
+                  double *part, int npairs, hipStream_t s, double bytes = 0, const msg::Tail &tail = msg::Tail{}) {
   ProfScope ps(KC_ENERGY, s, bytes);
   const EnergyGeom g = energy_geom(n);
   const double h = 1.0 / (double)(n - 1);
   const int width = (int)((n - 1) * h);
   hipLaunchKernelGGL((msg::k_energy<S>), dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, dcg, dgrad, n, h, width,
-                     st, part, g.ntx, g.nty, npairs);
+                     st, part, g.ntx, g.nty, npairs, tail);
   HIPCHK(hipGetLastError());
   return MSG_OK;
 }
@@ -253,11 +255,11 @@ int mg_norm_blocks(int l) {
 // out = V-cycle(sign * in) for nfields fields of level l (LinearSmoothOp::apply).
 template <typename S>
 int launch_vcycle(const S *in, int negate, S *out, int l, const MGBufs<S> &b, const PairState *st, double sigma,
-                  int nfields, double *nrm_part, hipStream_t s) {
+                  int nfields, double *nrm_part, hipStream_t s, const msg::Tail &tail = msg::Tail{}) {
   ProfScope ps(KC_VCYCLE, s, 0);
   if (l <= msg::MGC_MAXL) {
     hipLaunchKernelGGL((msg::k_mg_coarse<S>), dim3(nfields), dim3(256), 0, s, in, negate, out, l, st, sigma, nfields,
-                       nrm_part);
+                       nrm_part, tail);
     HIPCHK(hipGetLastError());
     return MSG_OK;
   }
@@ -273,7 +275,7 @@ int launch_vcycle(const S *in, int negate, S *out, int l, const MGBufs<S> &b, co
     HIPCHK(hipGetLastError());
   }
   hipLaunchKernelGGL((msg::k_mg_coarse<S>), dim3(nfields), dim3(256), 0, s, b.rhs[msg::MGC_MAXL], 0,
-                     b.x[msg::MGC_MAXL], msg::MGC_MAXL, st, sigma, nfields, (double *)nullptr);
+                     b.x[msg::MGC_MAXL], msg::MGC_MAXL, st, sigma, nfields, (double *)nullptr, msg::Tail{});
   HIPCHK(hipGetLastError());
   for (int m = msg::MGC_MAXL + 1; m <= l; ++m) {
     const int n = 1 << m;
@@ -283,7 +285,8 @@ int launch_vcycle(const S *in, int negate, S *out, int l, const MGBufs<S> &b, co
     const int neg = (m == l) ? negate : 0;
     S *xo = (m == l) ? out : b.x[m];
     hipLaunchKernelGGL((msg::k_mg_up<S>), dim3(ntx * nty * nfields), dim3(256), 0, s, x2, b.x[m - 1], rhs, neg, xo, n,
-                       st, sigma, ntx, nty, nfields, (m == l) ? nrm_part : (double *)nullptr);
+                       st, sigma, ntx, nty, nfields, (m == l) ? nrm_part : (double *)nullptr,
+                       (m == l) ? tail : msg::Tail{});
     HIPCHK(hipGetLastError());
   }
   return MSG_OK;
@@ -344,11 +347,21 @@ struct msg_ctx {
   double *coord = nullptr;
   PairState *st = nullptr;
   int *flags_d = nullptr, *flags_h = nullptr;
+  unsigned *counters = nullptr;  // last-workgroup counters: [4][B] (V-cycle, CG direction, energy, update)
   msg::IterTrace *trace_d = nullptr;
   int trace_cap = 0;
   hipStream_t stream = nullptr;
   long long cnt_iter = 0, cnt_evals = 0, cnt_passes = 0;
   std::vector<void *> allocs;
+  int mode = 0;                 // 0 host-synchronised rounds, 1 device-driven, 2 device-driven + hipGraph
+  struct GraphEntry {
+    int level, spec, K, R, trace_cap;
+    double lambda, stop_eps;
+    int max_iter;
+    void *trace;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
 };
 
 namespace {
@@ -477,7 +490,7 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
     hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL((msg::k_ncg<S>), dim3(nblk_ncg * 2 * B), dim3(256), 0, s, z, d, DE, n, c->st, c->part_c, nblk_ncg,
-                       2 * B);
+                       2 * B, msg::Tail{});
     HIPCHK(hipGetLastError());
     a.phase = msg::PH_ARMIJO_INIT;
     hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
@@ -505,7 +518,8 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
       any_active = nactive > 0;
       if (++rounds > 200) return fail(MSG_ERR_STATE, "Armijo search did not terminate");
     }
-    hipLaunchKernelGGL((msg::k_update<S>), dim3(nblk_upd * B), dim3(256), 0, s, phi, d, z, n, c->st, nblk_upd, B);
+    hipLaunchKernelGGL((msg::k_update<S>), dim3(nblk_upd * B), dim3(256), 0, s, phi, d, z, n, c->st, nblk_upd, B,
+                       c->counters + 3 * B);
     HIPCHK(hipGetLastError());
   }
 
@@ -527,6 +541,158 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
   return MSG_OK;
 }
 
+// ------------------------------------------------------------------------------------------------
+// Device-driven level solve: every control phase runs in the last workgroup of the kernel that
+// produced its partial sums, so one iteration is a fixed launch sequence (variation, V-cycle,
+// CG direction, R energy passes, update) with no host round trip. The host replays K iterations
+// (as one hipGraph in mode 2) and then reads the per-pair flags once. A pair whose step search is
+// still open after the R passes simply sits out the next iteration's gradient work and finishes the
+// search in its energy passes, so the result is that of the host-synchronised loop.
+// ------------------------------------------------------------------------------------------------
+template <typename S>
+int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_eps, int spec, double *energies,
+                   int *iterations, msg_iter_trace *trace, int trace_cap) {
+  const int n = 1 << l, B = c->B;
+  hipStream_t s = c->stream;
+  const S *R = (const S *)c->R[l], *T = (const S *)c->T[l];
+  S *phi = (S *)c->phi[l];
+  S *DE = (S *)c->DE, *z = (S *)c->z, *d = (S *)c->d;
+  const MGBufs<S> mg = ctx_mg<S>(c);
+  const EnergyGeom eg = energy_geom(n);
+  const int nblk_ncg = cdiv(n * n, 256 * msg::NCG_PER_THREAD);
+  const int nblk_upd = cdiv(2 * n * n, 256 * msg::NCG_PER_THREAD);
+  if (spec <= 0) spec = 3;
+  if (spec > msg::MAX_CAND) spec = msg::MAX_CAND;
+  const int Rr = 2, K = 8;
+  const bool use_graph = (c->mode == 2);
+
+  msg::IterTrace *tr = nullptr;
+  int tcap = 0;
+  if (trace && trace_cap > 0) {
+    if (c->trace_cap < trace_cap * B) {
+      if (c->trace_d) HIPCHK(hipFree(c->trace_d));
+      HIPCHK(hipMalloc(&c->trace_d, sizeof(msg::IterTrace) * trace_cap * B));
+      c->trace_cap = trace_cap * B;
+      for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+      c->graphs.clear();
+    }
+    tr = c->trace_d;
+    tcap = trace_cap;
+    HIPCHK(hipMemsetAsync(tr, 0, sizeof(msg::IterTrace) * trace_cap * B, s));
+  }
+  msg::CtrlArgs a;
+  a.nblk_energy = eg.nblk;
+  a.nblk_norm = mg_norm_blocks(l);
+  a.nblk_ncg = nblk_ncg;
+  a.lambda = lambda;
+  a.stop_eps = stop_eps;
+  a.max_iter = max_iter;
+  a.spec = spec;
+  a.trace_cap = tcap;
+  auto mk = [&](int phase, int slot) {
+    msg::Tail t;
+    t.st = c->st;
+    t.part_e = c->part_e;
+    t.part_n = c->part_n;
+    t.part_c = c->part_c;
+    t.a = a;
+    t.a.phase = phase;
+    t.flags = c->flags_d;
+    t.trace = tr;
+    t.counter = c->counters + slot * B;
+    return t;
+  };
+  const msg::Tail t_beta = mk(msg::PH_BETA, 0), t_init = mk(msg::PH_ARMIJO_INIT, 1);
+  const msg::Tail t_step = mk(msg::PH_ARMIJO_STEP, 2), t_level = mk(msg::PH_LEVEL_INIT, 2);
+
+  hipLaunchKernelGGL(k_state_init, dim3(cdiv(B, 64)), dim3(64), 0, s, c->st, B, lambda);
+  HIPCHK(hipGetLastError());
+  const double field_bytes = (double)n * n * sizeof(S);
+  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, 4.0 * field_bytes * B, t_level));
+
+  auto iteration = [&]() -> int {
+    CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s));
+    CHK(launch_vcycle<S>(DE, 1, z, l, mg, c->st, 1.0, 2 * B, c->part_n, s, t_beta));
+    hipLaunchKernelGGL((msg::k_ncg<S>), dim3(nblk_ncg * 2 * B), dim3(256), 0, s, z, d, DE, n, c->st, c->part_c, nblk_ncg,
+                       2 * B, t_init);
+    HIPCHK(hipGetLastError());
+    for (int r = 0; r < Rr; ++r) CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, 0.0, t_step));
+    hipLaunchKernelGGL((msg::k_update<S>), dim3(nblk_upd * B), dim3(256), 0, s, phi, d, z, n, c->st, nblk_upd, B,
+                       c->counters + 3 * B);
+    HIPCHK(hipGetLastError());
+    return MSG_OK;
+  };
+
+  hipGraphExec_t exec = nullptr;
+  if (use_graph) {
+    for (auto &g : c->graphs)
+      if (g.level == l && g.spec == spec && g.K == K && g.R == Rr && g.lambda == lambda && g.stop_eps == stop_eps &&
+          g.max_iter == max_iter && g.trace == (void *)tr && g.trace_cap == tcap)
+        exec = g.exec;
+    if (!exec) {
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      g_capturing = true;
+      int rc = MSG_OK;
+      for (int k = 0; k < K && rc == MSG_OK; ++k) rc = iteration();
+      g_capturing = false;
+      hipGraph_t graph = nullptr;
+      const hipError_t ec = hipStreamEndCapture(s, &graph);
+      if (rc != MSG_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      HIPCHK(ec);
+      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIPCHK(ei);
+      msg_ctx::GraphEntry ge{l, spec, K, Rr, tcap, lambda, stop_eps, max_iter, (void *)tr, exec};
+      c->graphs.push_back(ge);
+    }
+  }
+
+  // Every pair needs at most max_iter iterations; a search that spills over costs extra launches
+  // only. The bound below is far above any legal run and only guards against a logic error.
+  const long long max_batches = (long long)(max_iter + 4) * 8 / K + 64;
+  for (long long batch = 0;; ++batch) {
+    if (batch > max_batches) return fail(MSG_ERR_STATE, "device-driven level solve did not terminate");
+    if (max_iter <= 0) break;
+    if (use_graph) {
+      HIPCHK(hipGraphLaunch(exec, s));
+    } else {
+      for (int k = 0; k < K; ++k) CHK(iteration());
+    }
+    HIPCHK(hipMemcpyAsync(c->flags_h, c->flags_d, sizeof(int) * B, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    bool busy = false;
+    for (int p = 0; p < B; ++p) busy |= (c->flags_h[p] & 3) != 0;
+    if (!busy) break;
+  }
+
+  std::vector<PairState> hs(B);
+  HIPCHK(hipMemcpyAsync(hs.data(), c->st, sizeof(PairState) * B, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int p = 0; p < B; ++p) {
+    if (energies) energies[p] = hs[p].energy;
+    if (iterations) iterations[p] = hs[p].iter;
+    c->cnt_iter += hs[p].iter;
+    c->cnt_evals += hs[p].total_evals;
+    c->cnt_passes += hs[p].total_passes;
+  }
+  if (tr) {
+    HIPCHK(hipMemcpyAsync(trace, tr, sizeof(msg::IterTrace) * trace_cap * B, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  return MSG_OK;
+}
+
+template <typename S>
+int solve_level_any(msg_ctx *c, int l, double lambda, int max_iter, double stop_eps, int spec, double *energies,
+                    int *iterations, msg_iter_trace *trace, int trace_cap) {
+  if (c->mode == 0)
+    return solve_level_impl<S>(c, l, lambda, max_iter, stop_eps, spec, energies, iterations, trace, trace_cap);
+  return solve_level_dd<S>(c, l, lambda, max_iter, stop_eps, spec, energies, iterations, trace, trace_cap);
+}
+
 template <typename S>
 int prolong_deformation(msg_ctx *c, int l, int resample) {
   const int nc = 1 << l, n = 2 * nc;
@@ -546,7 +712,7 @@ int solve_impl(msg_ctx *c, int start, int stop, const msg_solve_params *p, doubl
   for (int level = start; level <= stop; ++level) {
     const double f = p->lambda_factor > 0 ? (double)std::pow(p->lambda_factor, p->lambda_ref_level - level) : 1;
     const double lam = f * p->lambda;
-    CHK(solve_level_impl<S>(c, level, lam, p->max_iter, p->stop_eps, p->spec, e.data(), nullptr, nullptr, 0));
+    CHK(solve_level_any<S>(c, level, lam, p->max_iter, p->stop_eps, p->spec, e.data(), nullptr, nullptr, 0));
     if (level < stop) CHK(prolong_deformation<S>(c, level, p->resample_prolong));
   }
   for (int l = stop; l < c->L; ++l) CHK(prolong_deformation<S>(c, l, p->resample_prolong));
@@ -573,6 +739,7 @@ int msg_ctx_create(msg_ctx **out, int max_level, int npairs, int dtype) {
   if (max_level < 1 || max_level > MAXL) return fail(MSG_ERR_ARG, "max_level out of range");
   if (npairs < 1) return fail(MSG_ERR_ARG, "npairs must be positive");
   msg_ctx *c = new msg_ctx();
+  c->mode = g_default_mode;
   if (hipGetDevice(&c->device) != hipSuccess) {
     delete c;
     return fail(MSG_ERR_HIP, "no HIP device");
@@ -609,6 +776,7 @@ int msg_ctx_create(msg_ctx **out, int max_level, int npairs, int dtype) {
   A((void **)&c->part_c, sizeof(double) * B * 2 * 2 * cdiv(N * N, 256 * msg::NCG_PER_THREAD));
   A((void **)&c->st, sizeof(PairState) * B);
   A((void **)&c->flags_d, sizeof(int) * B);
+  A((void **)&c->counters, sizeof(unsigned) * 4 * B);
   if (rc != MSG_OK) {
     msg_ctx_destroy(c);
     return rc;
@@ -621,6 +789,7 @@ int msg_ctx_create(msg_ctx **out, int max_level, int npairs, int dtype) {
     msg_ctx_destroy(c);
     return fail(MSG_ERR_HIP, "stream creation failed");
   }
+  (void)hipMemsetAsync(c->counters, 0, sizeof(unsigned) * 4 * B, c->stream);
   // zero images and deformations so that a context is usable before it is filled
   for (int l = 0; l <= max_level; ++l) {
     const size_t nn = (size_t)1 << (2 * l);
@@ -639,6 +808,7 @@ int msg_ctx_create(msg_ctx **out, int max_level, int npairs, int dtype) {
 int msg_ctx_destroy(msg_ctx *c) {
   if (!c) return MSG_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->trace_d) (void)hipFree(c->trace_d);
   if (c->flags_h) (void)hipHostFree(c->flags_h);
@@ -732,7 +902,7 @@ int msg_solve_level(msg_ctx *c, int level, double lambda, int max_iter, double s
   if (!c) return fail(MSG_ERR_ARG, "null context");
   if (level < 1 || level > c->L) return fail(MSG_ERR_ARG, "level out of range");
   c->cnt_iter = c->cnt_evals = c->cnt_passes = 0;
-  return DISPATCH(c, solve_level_impl, c, level, lambda, max_iter, stop_eps, spec, energies, iterations, trace,
+  return DISPATCH(c, solve_level_any, c, level, lambda, max_iter, stop_eps, spec, energies, iterations, trace,
                   trace_cap);
 }
 
@@ -1282,6 +1452,25 @@ int msg_release_cache(void) {
   std::lock_guard<std::mutex> g(g_ctx_cache.mu);
   for (msg_ctx *c : g_ctx_cache.free) msg_ctx_destroy(c);
   g_ctx_cache.free.clear();
+  return MSG_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int msg_ctx_set_mode(msg_ctx *c, int mode) {
+  if (!c) return fail(MSG_ERR_ARG, "null context");
+  if (mode < 0 || mode > 2) return fail(MSG_ERR_ARG, "mode must be 0, 1 or 2");
+  c->mode = mode;
+  return MSG_OK;
+}
+
+int msg_set_default_mode(int mode) {
+  if (mode < 0 || mode > 2) return fail(MSG_ERR_ARG, "mode must be 0, 1 or 2");
+  g_default_mode = mode;
+  std::lock_guard<std::mutex> g(g_ctx_cache.mu);
+  for (msg_ctx *c : g_ctx_cache.free) c->mode = mode;
   return MSG_OK;
 }
 

@@ -13,523 +13,6 @@ constexpr double TAU_MAX = 33554432.0;             // 2^25
 constexpr double ARMIJO_SIGMA = 0.5;
 
 // ================================================================================================
-// Energy E(phi + tau d) for up to MAX_CAND step sizes per pair, element-centric.
-// SSD part: SSDEnergy::evaluateIntegrand (registration.h:1607-1615) integrated as in
-// FENonlinIntegrationVectorInterface::applyAdd (FEOpInterface.h:2586-2613).
-// Regularisation part: DisplacementLengthEnergy (hyperelastic.h:178-183) = 1/2 sum_c psi_c^T K psi_c,
-// assembled element by element. Output: per block partial sums (ssd, reg) per candidate.
-// ================================================================================================
-constexpr int EN_TX = 64, EN_TY = 4;  // elements per workgroup tile (one per thread)
-
-template <typename S>
-__global__ void __launch_bounds__(256)
-k_energy(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__ phi, const S *__restrict__ dcg,
-         const S *__restrict__ dgrad, int n, double h, int width, const PairState *__restrict__ st,
-         double *__restrict__ part, int ntx, int nty, int npairs) {
-  const int tiles = ntx * nty;
-  const int wg = xcd_remap(blockIdx.x, tiles * npairs);
-  const int pair = wg / tiles, tile = wg - pair * tiles;
-  const int nc = st[pair].ncand;
-  if (nc == 0) return;  // uniform across the workgroup
-  const int dm = st[pair].dirmode;
-  __shared__ double red[4];
-
-  const size_t nn = (size_t)n * n;
-  const S *Rp = R + pair * nn, *Tp = T + pair * nn;
-  const S *px = phi + pair * 2 * nn, *py = px + nn;
-  const S *dsel = (dm == DIR_CG) ? dcg : dgrad;
-  const S *dx = dsel + pair * 2 * nn, *dy = dx + nn;
-  const bool use_dir = (dm != DIR_NONE);
-
-  const int ex = (tile % ntx) * EN_TX + (threadIdx.x % EN_TX);
-  const int ey = (tile / ntx) * EN_TY + (threadIdx.x / EN_TX);
-  const bool valid = (ex < n - 1) && (ey < n - 1);
-
-  double accS[MAX_CAND], accR[MAX_CAND];
-#pragma unroll
-  for (int k = 0; k < MAX_CAND; ++k) accS[k] = accR[k] = 0.0;
-
-  if (valid) {
-    double f0[4], f1[4], d0[4], d1[4], rq[4];
-    double rn[4];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int g = l2g(ex, ey, b, n);
-      f0[b] = ld(px, g);
-      f1[b] = ld(py, g);
-      d0[b] = use_dir ? ld(dx, g) : 0.0;
-      d1[b] = use_dir ? ld(dy, g) : 0.0;
-      rn[b] = ld(Rp, g);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      double w = 0.;
-      for (int b = 0; b < 4; ++b) w += rn[b] * bq(b, q);
-      rq[q] = w;
-    }
-    const double vol = h * h;
-#pragma unroll
-    for (int k = 0; k < MAX_CAND; ++k) {
-      if (k < nc) {
-        const double tau = st[pair].cand[k];
-        double p0[4], p1[4];
-        for (int b = 0; b < 4; ++b) {
-          p0[b] = use_dir ? rnd<S>(f0[b] + tau * d0[b]) : f0[b];
-          p1[b] = use_dir ? rnd<S>(f1[b] + tau * d1[b]) : f1[b];
-        }
-        double a = 0.;
-        for (int q = 0; q < 4; ++q) {
-          double ox = 0., oy = 0.;
-          for (int b = 0; b < 4; ++b) {
-            ox += p0[b] * bq(b, q);
-            oy += p1[b] * bq(b, q);
-          }
-          const Hit hp = deform_qp(ex, ey, q, ox, oy, n, h, width);
-          double t4[4];
-          const double t = eval_img(Tp, hp, n, t4) - rq[q];
-          a += 0.5 * (t * t) * 0.25;
-        }
-        a *= vol;
-        accS[k] = a;
-        accR[k] = 0.5 * (stiff_form(p0) + stiff_form(p1));
-      }
-    }
-  }
-  const int nblk = tiles;
-  for (int k = 0; k < nc; ++k) {
-    const double s = block_sum<256>(accS[k], red);
-    const double r = block_sum<256>(accR[k], red);
-    if (threadIdx.x == 0) {
-      part[((size_t)(pair * MAX_CAND + k) * 2 + 0) * nblk + tile] = s;
-      part[((size_t)(pair * MAX_CAND + k) * 2 + 1) * nblk + tile] = r;
-    }
-  }
-}
-
-// ================================================================================================
-// Variation DE = F_SSD + lambda K phi, node-centric with the element contributions staged in LDS.
-// F_SSD: SSDForce::getNonlinearity (registration.h:1646-1666) assembled as in
-// FENonlinVectorOpInterface::applyAdd (FEOpInterface.h:1803-1853); every node sums the
-// contributions of its (up to) four elements in the reference's element order.
-// ================================================================================================
-constexpr int GR_TX = 32, GR_TY = 8;  // nodes per workgroup tile
-constexpr int GR_EX = GR_TX + 1, GR_EY = GR_TY + 1;
-
-template <typename S>
-__global__ void __launch_bounds__(256)
-k_grad(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__ phi, S *__restrict__ DE, int n,
-       double h, int width, const PairState *__restrict__ st, double lambda_default, int ntx, int nty, int npairs) {
-  const int tiles = ntx * nty;
-  const int wg = xcd_remap(blockIdx.x, tiles * npairs);
-  const int pair = wg / tiles, tile = wg - pair * tiles;
-  if (st && !st[pair].active) return;
-  const double lambda = st ? st[pair].lambda : lambda_default;
-
-  __shared__ double cf[GR_EY * GR_EX][4][2];  // force contribution per element, dof, component
-  __shared__ double ck[GR_EY * GR_EX][4][2];  // stiffness contribution
-
-  const size_t nn = (size_t)n * n;
-  const S *Rp = R + pair * nn, *Tp = T + pair * nn;
-  const S *px = phi + pair * 2 * nn, *py = px + nn;
-  const int x0 = (tile % ntx) * GR_TX, y0 = (tile / ntx) * GR_TY;
-  const double vol = h * h;
-
-  for (int e = threadIdx.x; e < GR_EX * GR_EY; e += 256) {
-    const int ex = x0 - 1 + (e % GR_EX), ey = y0 - 1 + (e / GR_EX);
-    if (ex < 0 || ey < 0 || ex >= n - 1 || ey >= n - 1) continue;
-    double f0[4], f1[4], rn[4];
-    for (int b = 0; b < 4; ++b) {
-      const int g = l2g(ex, ey, b, n);
-      f0[b] = ld(px, g);
-      f1[b] = ld(py, g);
-      rn[b] = ld(Rp, g);
-    }
-    double nl[4][2];
-    for (int q = 0; q < 4; ++q) {
-      double ox = 0., oy = 0., r = 0.;
-      for (int b = 0; b < 4; ++b) {
-        ox += f0[b] * bq(b, q);
-        oy += f1[b] * bq(b, q);
-      }
-      for (int b = 0; b < 4; ++b) r += rn[b] * bq(b, q);
-      const Hit hp = deform_qp(ex, ey, q, ox, oy, n, h, width);
-      double t4[4];
-      const double td = eval_img(Tp, hp, n, t4);
-      double g0 = 0., g1 = 0.;
-      for (int b = 0; b < 4; ++b) {
-        double v0 = q1dx(b, hp.ly) / h, v1 = q1dy(b, hp.lx) / h;
-        v0 *= t4[b];
-        v1 *= t4[b];
-        g0 += v0;
-        g1 += v1;
-      }
-      nl[q][0] = hp.inx ? g0 * (td - r) : 0;
-      nl[q][1] = hp.iny ? g1 * (td - r) : 0;
-    }
-    for (int dof = 0; dof < 4; ++dof) {
-      double a0 = 0, a1 = 0;
-      for (int q = 0; q < 4; ++q) {
-        const double fct = bq(dof, q) * 0.25;
-        a0 += fct * nl[q][0];
-        a1 += fct * nl[q][1];
-      }
-      cf[e][dof][0] = vol * a0;
-      cf[e][dof][1] = vol * a1;
-    }
-    // K_loc (1/6)[4 -1 -1 -2; -1 4 -2 -1; -1 -2 4 -1; -2 -1 -1 4] applied to the element's dofs
-    const double kl[4][4] = {{4. / 6, -1. / 6, -1. / 6, -2. / 6},
-                             {-1. / 6, 4. / 6, -2. / 6, -1. / 6},
-                             {-1. / 6, -2. / 6, 4. / 6, -1. / 6},
-                             {-2. / 6, -1. / 6, -1. / 6, 4. / 6}};
-    for (int i = 0; i < 4; ++i) {
-      double s0 = 0, s1 = 0;
-      for (int j = 0; j < 4; ++j) {
-        s0 += kl[i][j] * f0[j];
-        s1 += kl[i][j] * f1[j];
-      }
-      ck[e][i][0] = s0;
-      ck[e][i][1] = s1;
-    }
-  }
-  __syncthreads();
-  const int x = x0 + (threadIdx.x % GR_TX), y = y0 + (threadIdx.x / GR_TX);
-  if (x >= n || y >= n) return;
-  double F0 = 0, F1 = 0, K0 = 0, K1 = 0;
-  for (int ey = y - 1; ey <= y; ++ey)
-    for (int ex = x - 1; ex <= x; ++ex) {
-      if (ex < 0 || ey < 0 || ex >= n - 1 || ey >= n - 1) continue;
-      const int e = (ex - (x0 - 1)) + (ey - (y0 - 1)) * GR_EX;
-      const int dof = (x - ex) + 2 * (y - ey);
-      F0 += cf[e][dof][0];
-      F1 += cf[e][dof][1];
-      K0 += ck[e][dof][0];
-      K1 += ck[e][dof][1];
-    }
-  S *o = DE + pair * 2 * nn;
-  const int i = x + y * n;
-  o[i] = static_cast<S>(F0 + lambda * K0);
-  o[nn + i] = static_cast<S>(F1 + lambda * K1);
-}
-
-// ================================================================================================
-// H1 metric: one V-cycle of the FD heat operator (LinearSmoothOp::apply, linearSmoothOp.cpp:591-612,
-// MDMultigrid::solve L451-494, HeatMultigrid::jacobi L556-580) per field, 2 fields per pair.
-// Down-leg kernel: x1 = rhs/diag (first Jacobi sweep from zero), x2 = x1 + (rhs - A x1)/diag,
-// residual r = rhs - A x2, coarse rhs = 2x2 mean of r. Up-leg kernel: x = x2 + injected coarse
-// correction, then two Jacobi sweeps. Tiles with a 2-node halo live in LDS.
-// ================================================================================================
-constexpr int MG_TX = 32, MG_TY = 16;
-constexpr int MG_HX = MG_TX + 4, MG_HY = MG_TY + 4;
-
-__device__ __forceinline__ void level_consts(const PairState *st, int pair, double sigma_default, int n, double &hs,
-                                             double &diag) {
-  const double sigma = st ? st[pair].sigma : sigma_default;
-  const double tau = 0.5 * (sigma * sigma);  // LinearSmoothOp::setSigma (linearSmoothOp.h:50-52)
-  const double s = (double)((n - 1) * (n - 1));
-  hs = tau * s;
-  diag = 1 + 4 * tau * s;
-}
-
-template <typename S>
-__global__ void __launch_bounds__(256)
-k_mg_down(const S *__restrict__ rhs, int negate, S *__restrict__ x2out, S *__restrict__ rhsc, int n,
-          const PairState *__restrict__ st, double sigma_default, int ntx, int nty, int nfields) {
-  const int tiles = ntx * nty;
-  const int wg = xcd_remap(blockIdx.x, tiles * nfields);
-  const int field = wg / tiles, tile = wg - field * tiles;
-  const int pair = field >> 1;
-  if (st && !st[pair].active) return;
-  double hs, diag;
-  level_consts(st, pair, sigma_default, n, hs, diag);
-
-  __shared__ double sr[MG_HY][MG_HX];   // rhs on tile+2
-  __shared__ double s1[MG_HY][MG_HX];   // x1 on tile+2
-  __shared__ double s2[MG_HY][MG_HX];   // x2 on tile+1
-  __shared__ double sres[MG_TY][MG_TX]; // residual on tile
-  const size_t nn = (size_t)n * n;
-  const S *rp = rhs + field * nn;
-  const int x0 = (tile % ntx) * MG_TX, y0 = (tile / ntx) * MG_TY;
-  const int bx = x0 - 2, by = y0 - 2;  // LDS origin
-
-  for (int i = threadIdx.x; i < MG_HX * MG_HY; i += 256) {
-    const int lx = i % MG_HX, ly = i / MG_HX;
-    const int gx = bx + lx, gy = by + ly;
-    if (gx >= 0 && gy >= 0 && gx < n && gy < n) {
-      double v = ld(rp, gx + gy * n);
-      if (negate) v = -v;
-      sr[ly][lx] = v;
-      s1[ly][lx] = v / diag;  // 0 + (1.0 * (rhs - A 0)) / diag
-    }
-  }
-  __syncthreads();
-  auto get1 = [&](int gx, int gy) { return s1[gy - by][gx - bx]; };
-  for (int i = threadIdx.x; i < MG_HX * MG_HY; i += 256) {
-    const int lx = i % MG_HX, ly = i / MG_HX;
-    const int gx = bx + lx, gy = by + ly;
-    if (lx >= 1 && ly >= 1 && lx < MG_HX - 1 && ly < MG_HY - 1 && gx >= 0 && gy >= 0 && gx < n && gy < n) {
-      const double ax = heat_apply(get1, gx, gy, n, hs);
-      const double v = sr[ly][lx] - ax;
-      s2[ly][lx] = s1[ly][lx] + 1.0 * v / diag;
-    }
-  }
-  __syncthreads();
-  auto get2 = [&](int gx, int gy) { return s2[gy - by][gx - bx]; };
-  S *xo = x2out + field * nn;
-  for (int i = threadIdx.x; i < MG_TX * MG_TY; i += 256) {
-    const int tx = i % MG_TX, ty = i / MG_TX;
-    const int gx = x0 + tx, gy = y0 + ty;
-    if (gx < n && gy < n) {
-      const double ax = heat_apply(get2, gx, gy, n, hs);
-      sres[ty][tx] = sr[ty + 2][tx + 2] - ax;
-      xo[gx + gy * n] = static_cast<S>(s2[ty + 2][tx + 2]);
-    }
-  }
-  __syncthreads();
-  const int nc = n / 2;
-  S *rc = rhsc + field * (size_t)nc * nc;
-  for (int i = threadIdx.x; i < (MG_TX / 2) * (MG_TY / 2); i += 256) {
-    const int cx = i % (MG_TX / 2), cy = i / (MG_TX / 2);
-    const int gcx = x0 / 2 + cx, gcy = y0 / 2 + cy;
-    if (gcx < nc && gcy < nc) {
-      double v = 0;
-      v += sres[2 * cy][2 * cx];
-      v += sres[2 * cy][2 * cx + 1];
-      v += sres[2 * cy + 1][2 * cx];
-      v += sres[2 * cy + 1][2 * cx + 1];
-      rc[gcx + gcy * nc] = static_cast<S>((1.0 / 4.0) * v);
-    }
-  }
-}
-
-template <typename S>
-__global__ void __launch_bounds__(256)
-k_mg_up(const S *__restrict__ x2in, const S *__restrict__ xc, const S *__restrict__ rhs, int negate,
-        S *__restrict__ xout, int n, const PairState *__restrict__ st, double sigma_default, int ntx, int nty,
-        int nfields, double *__restrict__ nrm_part) {
-  const int tiles = ntx * nty;
-  const int wg = xcd_remap(blockIdx.x, tiles * nfields);
-  const int field = wg / tiles, tile = wg - field * tiles;
-  const int pair = field >> 1;
-  if (st && !st[pair].active) return;
-  double hs, diag;
-  level_consts(st, pair, sigma_default, n, hs, diag);
-
-  __shared__ double sx[MG_HY][MG_HX];  // x on tile+2
-  __shared__ double sr[MG_HY][MG_HX];  // rhs on tile+1
-  __shared__ double s3[MG_HY][MG_HX];  // x3 on tile+1
-  __shared__ double red[4];
-  const size_t nn = (size_t)n * n;
-  const int nc = n / 2;
-  const S *x2p = x2in + field * nn, *rp = rhs + field * nn, *xcp = xc + field * (size_t)nc * nc;
-  const int x0 = (tile % ntx) * MG_TX, y0 = (tile / ntx) * MG_TY;
-  const int bx = x0 - 2, by = y0 - 2;
-
-  for (int i = threadIdx.x; i < MG_HX * MG_HY; i += 256) {
-    const int lx = i % MG_HX, ly = i / MG_HX;
-    const int gx = bx + lx, gy = by + ly;
-    if (gx >= 0 && gy >= 0 && gx < n && gy < n) {
-      sx[ly][lx] = ld(x2p, gx + gy * n) + ld(xcp, (gx >> 1) + (gy >> 1) * nc);
-      double v = ld(rp, gx + gy * n);
-      sr[ly][lx] = negate ? -v : v;
-    }
-  }
-  __syncthreads();
-  auto getx = [&](int gx, int gy) { return sx[gy - by][gx - bx]; };
-  for (int i = threadIdx.x; i < MG_HX * MG_HY; i += 256) {
-    const int lx = i % MG_HX, ly = i / MG_HX;
-    const int gx = bx + lx, gy = by + ly;
-    if (lx >= 1 && ly >= 1 && lx < MG_HX - 1 && ly < MG_HY - 1 && gx >= 0 && gy >= 0 && gx < n && gy < n) {
-      const double ax = heat_apply(getx, gx, gy, n, hs);
-      const double v = sr[ly][lx] - ax;
-      s3[ly][lx] = sx[ly][lx] + 1.0 * v / diag;
-    }
-  }
-  __syncthreads();
-  auto get3 = [&](int gx, int gy) { return s3[gy - by][gx - bx]; };
-  S *xo = xout + field * nn;
-  double nrm = 0;
-  for (int i = threadIdx.x; i < MG_TX * MG_TY; i += 256) {
-    const int tx = i % MG_TX, ty = i / MG_TX;
-    const int gx = x0 + tx, gy = y0 + ty;
-    if (gx < n && gy < n) {
-      const double ax = heat_apply(get3, gx, gy, n, hs);
-      const double v = sr[ty + 2][tx + 2] - ax;
-      const S o = static_cast<S>(s3[ty + 2][tx + 2] + 1.0 * v / diag);
-      xo[gx + gy * n] = o;
-      const double od = static_cast<double>(o);
-      nrm += od * od;
-    }
-  }
-  if (nrm_part) {
-    const double s = block_sum<256>(nrm, red);
-    if (threadIdx.x == 0) nrm_part[(size_t)field * tiles + tile] = s;
-  }
-}
-
-// Whole V-cycle below (and including) a level of at most 32x32 nodes in one workgroup, in LDS.
-// With n equal to the registration level this is the complete metric; otherwise it is the coarse
-// part of the cycle started by k_mg_down.
-constexpr int MGC_MAXL = 5;
-constexpr int MGC_OFF[MGC_MAXL + 2] = {0, 1, 5, 21, 85, 341, 1365};  // prefix sums of 4^l
-
-template <typename S>
-__global__ void __launch_bounds__(256)
-k_mg_coarse(const S *__restrict__ rhs, int negate, S *__restrict__ xout, int L, const PairState *__restrict__ st,
-            double sigma_default, int nfields, double *__restrict__ nrm_part) {
-  const int field = blockIdx.x;
-  const int pair = field >> 1;
-  if (st && !st[pair].active) return;
-  const double sigma = st ? st[pair].sigma : sigma_default;
-  const double tau = 0.5 * (sigma * sigma);
-  __shared__ double X[MGC_OFF[MGC_MAXL + 1]], B[MGC_OFF[MGC_MAXL + 1]], D[1024];
-  __shared__ double red[4];
-  const int n = 1 << L;
-  const size_t nn = (size_t)n * n;
-  const S *rp = rhs + field * nn;
-  for (int i = threadIdx.x; i < (int)nn; i += 256) {
-    double v = ld(rp, i);
-    B[MGC_OFF[L] + i] = negate ? -v : v;
-  }
-  __syncthreads();
-
-  // Jacobi sweep on level l: D = A x; x += (rhs - D)/diag
-  auto sweep = [&](int l) {
-    const int m = 1 << l;
-    const double s = (double)((m - 1) * (m - 1));
-    const double hs = tau * s, diag = 1 + 4 * tau * s;
-    double *x = X + MGC_OFF[l];
-    const double *b = B + MGC_OFF[l];
-    auto get = [&](int gx, int gy) { return x[gx + gy * m]; };
-    for (int i = threadIdx.x; i < m * m; i += 256) D[i] = heat_apply(get, i % m, i / m, m, hs);
-    __syncthreads();
-    for (int i = threadIdx.x; i < m * m; i += 256) {
-      const double v = b[i] - D[i];
-      x[i] += 1.0 * v / diag;
-    }
-    __syncthreads();
-  };
-
-  for (int l = L; l >= 1; --l) {
-    const int m = 1 << l;
-    double *x = X + MGC_OFF[l];
-    for (int i = threadIdx.x; i < m * m; i += 256) x[i] = 0;
-    __syncthreads();
-    sweep(l);
-    sweep(l);
-    const double s = (double)((m - 1) * (m - 1));
-    const double hs = tau * s;
-    auto get = [&](int gx, int gy) { return x[gx + gy * m]; };
-    for (int i = threadIdx.x; i < m * m; i += 256) D[i] = B[MGC_OFF[l] + i] - heat_apply(get, i % m, i / m, m, hs);
-    __syncthreads();
-    const int mc = m / 2;
-    for (int i = threadIdx.x; i < mc * mc; i += 256) {
-      const int cx = i % mc, cy = i / mc;
-      double v = 0;
-      v += D[2 * cx + 2 * cy * m];
-      v += D[2 * cx + 1 + 2 * cy * m];
-      v += D[2 * cx + (2 * cy + 1) * m];
-      v += D[2 * cx + 1 + (2 * cy + 1) * m];
-      B[MGC_OFF[l - 1] + i] = (1.0 / 4.0) * v;
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {  // 1x1 level: 100 sweeps of x += (rhs - x) / 1
-    double x = 0;
-    const double b = B[0];
-    for (int it = 0; it < 100; ++it) {
-      const double v = b - x;
-      x += 1.0 * v / 1.0;
-    }
-    X[0] = x;
-  }
-  __syncthreads();
-  for (int l = 1; l <= L; ++l) {
-    const int m = 1 << l, mc = m / 2;
-    double *x = X + MGC_OFF[l];
-    const double *xcs = X + MGC_OFF[l - 1];
-    for (int i = threadIdx.x; i < m * m; i += 256) x[i] += xcs[((i % m) >> 1) + ((i / m) >> 1) * mc];
-    __syncthreads();
-    sweep(l);
-    sweep(l);
-  }
-  S *xo = xout + field * nn;
-  double nrm = 0;
-  for (int i = threadIdx.x; i < (int)nn; i += 256) {
-    const S o = static_cast<S>(X[MGC_OFF[L] + i]);
-    xo[i] = o;
-    const double od = static_cast<double>(o);
-    nrm += od * od;
-  }
-  if (nrm_part) {
-    const double s = block_sum<256>(nrm, red);
-    if (threadIdx.x == 0) nrm_part[field] = s;
-  }
-}
-
-// ================================================================================================
-// Fletcher-Reeves direction d = z + beta * d_old (calcTaudAndDescentDir, gradientDescent.h:218-226)
-// and the slopes <DE, d>, <DE, z> (ArmijoLineSearchHelpFunction_evaluateDerivative, L176-182).
-// ================================================================================================
-constexpr int NCG_PER_THREAD = 4;
-
-template <typename S>
-__global__ void __launch_bounds__(256)
-k_ncg(const S *__restrict__ z, S *__restrict__ d, const S *__restrict__ DE, int n, const PairState *__restrict__ st,
-      double *__restrict__ part, int nblk_field, int nfields) {
-  const int total = nblk_field * nfields;
-  const int wg = xcd_remap(blockIdx.x, total);
-  const int field = wg / nblk_field, blk = wg - field * nblk_field;
-  const int pair = field >> 1;
-  if (!st[pair].active) return;
-  const double beta = st[pair].beta;
-  const bool zero_old = st[pair].olddir_zero != 0;
-  __shared__ double red[4];
-  const size_t nn = (size_t)n * n;
-  const S *zp = z + field * nn, *dep = DE + field * nn;
-  S *dp = d + field * nn;
-  double sd = 0, sz = 0;
-  const int base = blk * 256 * NCG_PER_THREAD;
-  for (int k = 0; k < NCG_PER_THREAD; ++k) {
-    const int i = base + k * 256 + threadIdx.x;
-    if (i < (int)nn) {
-      const double zv = ld(zp, i);
-      const double od = zero_old ? 0.0 : ld(dp, i);
-      const S dv = static_cast<S>(zv + beta * od);
-      dp[i] = dv;
-      const double de = ld(dep, i);
-      sd += de * static_cast<double>(dv);
-      sz += de * zv;
-    }
-  }
-  const double a = block_sum<256>(sd, red);
-  const double b = block_sum<256>(sz, red);
-  if (threadIdx.x == 0) {
-    part[((size_t)field * 2 + 0) * nblk_field + blk] = a;
-    part[((size_t)field * 2 + 1) * nblk_field + blk] = b;
-  }
-}
-
-// phi += tau * dir for pairs whose search accepted a step (GradientDescentBase::applySingle L382-385).
-template <typename S>
-__global__ void __launch_bounds__(256)
-k_update(S *__restrict__ phi, const S *__restrict__ dcg, const S *__restrict__ dgrad, int n,
-         const PairState *__restrict__ st, int nblk_pair, int npairs) {
-  const int total = nblk_pair * npairs;
-  const int wg = xcd_remap(blockIdx.x, total);
-  const int pair = wg / nblk_pair, blk = wg - pair * nblk_pair;
-  const double tau = st[pair].step_tau;
-  if (tau == 0) return;
-  const S *dir = (st[pair].step_dir == DIR_CG ? dcg : dgrad) + pair * 2 * (size_t)n * n;
-  S *p = phi + pair * 2 * (size_t)n * n;
-  const int m = 2 * n * n;
-  const int base = blk * 256 * NCG_PER_THREAD;
-  for (int k = 0; k < NCG_PER_THREAD; ++k) {
-    const int i = base + k * 256 + threadIdx.x;
-    if (i < m) p[i] = static_cast<S>(ld(p, i) + tau * ld(dir, i));
-  }
-}
-
-// ================================================================================================
 // Control: one workgroup per pair reduces the partial sums of the previous pass (fixed order) and
 // advances the pair's state machine. Phases:
 // ================================================================================================
@@ -721,13 +204,16 @@ __device__ void armijo_run(PairState &s, int spec, IterTrace *trace, int trace_c
   finish_iteration(s, 0.0, trace, trace_cap, pair);  // unreachable guard
 }
 
-__global__ void __launch_bounds__(256)
-k_control(PairState *__restrict__ st, const double *__restrict__ part_e, const double *__restrict__ part_n,
-          const double *__restrict__ part_c, CtrlArgs a, int *__restrict__ flags, IterTrace *__restrict__ trace) {
-  const int pair = blockIdx.x;
+// Runs one control phase for one pair with the whole workgroup (the reductions need every thread).
+// Called by k_control (one workgroup per pair) or by the last workgroup of the kernel that produced
+// the partial sums (device-driven iterations).
+__device__ void control_pair(int pair, PairState *__restrict__ st, const double *__restrict__ part_e,
+                             const double *__restrict__ part_n, const double *__restrict__ part_c, const CtrlArgs &a,
+                             int *__restrict__ flags, IterTrace *__restrict__ trace) {
   PairState &s = st[pair];
   __shared__ double red[4];
   __shared__ double sums[2 * MAX_CAND];
+  const bool participates = s.active && !s.searching && s.step_tau == 0;
 
   if (a.phase == PH_LEVEL_INIT || a.phase == PH_ARMIJO_STEP) {
     const int nc = s.ncand;
@@ -767,8 +253,7 @@ k_control(PairState *__restrict__ st, const double *__restrict__ part_e, const d
       }
     }
   } else if (a.phase == PH_BETA) {
-    if (threadIdx.x == 0) s.step_tau = 0;  // no update unless this iteration's search accepts a step
-    if (!s.active) return;
+    if (!participates) return;
     const double z0 = reduce_fixed(part_n + (size_t)(2 * pair) * a.nblk_norm, a.nblk_norm, red);
     const double z1 = reduce_fixed(part_n + (size_t)(2 * pair + 1) * a.nblk_norm, a.nblk_norm, red);
     if (threadIdx.x == 0) {
@@ -782,7 +267,7 @@ k_control(PairState *__restrict__ st, const double *__restrict__ part_e, const d
       s.passes = 0;
     }
   } else if (a.phase == PH_ARMIJO_INIT) {
-    if (!s.active) return;
+    if (!participates) return;
     const double d0 = reduce_fixed(part_c + ((size_t)(2 * pair) * 2 + 0) * a.nblk_ncg, a.nblk_ncg, red);
     const double d1 = reduce_fixed(part_c + ((size_t)(2 * pair + 1) * 2 + 0) * a.nblk_ncg, a.nblk_ncg, red);
     const double g0 = reduce_fixed(part_c + ((size_t)(2 * pair) * 2 + 1) * a.nblk_ncg, a.nblk_ncg, red);
@@ -803,6 +288,579 @@ k_control(PairState *__restrict__ st, const double *__restrict__ part_e, const d
     }
   }
   if (threadIdx.x == 0 && flags) flags[pair] = (s.searching ? 1 : 0) | (s.active ? 2 : 0);
+}
+
+__global__ void __launch_bounds__(256)
+k_control(PairState *__restrict__ st, const double *__restrict__ part_e, const double *__restrict__ part_n,
+          const double *__restrict__ part_c, CtrlArgs a, int *__restrict__ flags, IterTrace *__restrict__ trace) {
+  control_pair(blockIdx.x, st, part_e, part_n, part_c, a, flags, trace);
+}
+
+// True in exactly one workgroup of a group of `total` workgroups: the last to arrive. Every
+// workgroup's thread 0 must have stored its partial results before the call; the release/acquire
+// pair (agent scope) makes them visible to the last workgroup whatever XCD wrote them. The counter
+// is reset by the last workgroup, ready for the next launch.
+__device__ __forceinline__ bool last_block_of(unsigned *counter, unsigned total) {
+  __shared__ unsigned prev;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  const bool last = (prev == total - 1);
+  if (last) {
+    __threadfence();
+    if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+  }
+  return last;
+}
+
+// Where a kernel's control tail sends its results (device-driven iterations); null: no tail.
+struct Tail {
+  PairState *st;
+  const double *part_e, *part_n, *part_c;
+  CtrlArgs a;
+  int *flags;
+  IterTrace *trace;
+  unsigned *counter;  // one per pair
+};
+
+// A pair takes part in a new iteration when it is still iterating, no step search is pending
+// (device-driven iterations can leave one open across an iteration boundary) and its last step
+// has been applied.
+__device__ __forceinline__ bool participates(const PairState &s) {
+  return s.active && !s.searching && s.step_tau == 0;
+}
+
+__device__ __forceinline__ void run_tail(const Tail &t, int pair, unsigned total) {
+  if (t.st && last_block_of(t.counter + pair, total))
+    control_pair(pair, t.st, t.part_e, t.part_n, t.part_c, t.a, t.flags, t.trace);
+}
+
+
+// ================================================================================================
+// Energy E(phi + tau d) for up to MAX_CAND step sizes per pair, element-centric.
+// SSD part: SSDEnergy::evaluateIntegrand (registration.h:1607-1615) integrated as in
+// FENonlinIntegrationVectorInterface::applyAdd (FEOpInterface.h:2586-2613).
+// Regularisation part: DisplacementLengthEnergy (hyperelastic.h:178-183) = 1/2 sum_c psi_c^T K psi_c,
+// assembled element by element. Output: per block partial sums (ssd, reg) per candidate.
+// ================================================================================================
+constexpr int EN_TX = 64, EN_TY = 4;  // elements per workgroup tile (one per thread)
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_energy(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__ phi, const S *__restrict__ dcg,
+         const S *__restrict__ dgrad, int n, double h, int width, const PairState *st,
+         double *__restrict__ part, int ntx, int nty, int npairs, Tail tail) {
+  const int tiles = ntx * nty;
+  const int wg = xcd_remap(blockIdx.x, tiles * npairs);
+  const int pair = wg / tiles, tile = wg - pair * tiles;
+  const int nc = st[pair].ncand;
+  if (nc == 0) return;  // uniform across the workgroup
+  const int dm = st[pair].dirmode;
+  __shared__ double red[4];
+
+  const size_t nn = (size_t)n * n;
+  const S *Rp = R + pair * nn, *Tp = T + pair * nn;
+  const S *px = phi + pair * 2 * nn, *py = px + nn;
+  const S *dsel = (dm == DIR_CG) ? dcg : dgrad;
+  const S *dx = dsel + pair * 2 * nn, *dy = dx + nn;
+  const bool use_dir = (dm != DIR_NONE);
+
+  const int ex = (tile % ntx) * EN_TX + (threadIdx.x % EN_TX);
+  const int ey = (tile / ntx) * EN_TY + (threadIdx.x / EN_TX);
+  const bool valid = (ex < n - 1) && (ey < n - 1);
+
+  double accS[MAX_CAND], accR[MAX_CAND];
+#pragma unroll
+  for (int k = 0; k < MAX_CAND; ++k) accS[k] = accR[k] = 0.0;
+
+  if (valid) {
+    double f0[4], f1[4], d0[4], d1[4], rq[4];
+    double rn[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int g = l2g(ex, ey, b, n);
+      f0[b] = ld(px, g);
+      f1[b] = ld(py, g);
+      d0[b] = use_dir ? ld(dx, g) : 0.0;
+      d1[b] = use_dir ? ld(dy, g) : 0.0;
+      rn[b] = ld(Rp, g);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double w = 0.;
+      for (int b = 0; b < 4; ++b) w += rn[b] * bq(b, q);
+      rq[q] = w;
+    }
+    const double vol = h * h;
+#pragma unroll
+    for (int k = 0; k < MAX_CAND; ++k) {
+      if (k < nc) {
+        const double tau = st[pair].cand[k];
+        double p0[4], p1[4];
+        for (int b = 0; b < 4; ++b) {
+          p0[b] = use_dir ? rnd<S>(f0[b] + tau * d0[b]) : f0[b];
+          p1[b] = use_dir ? rnd<S>(f1[b] + tau * d1[b]) : f1[b];
+        }
+        double a = 0.;
+        for (int q = 0; q < 4; ++q) {
+          double ox = 0., oy = 0.;
+          for (int b = 0; b < 4; ++b) {
+            ox += p0[b] * bq(b, q);
+            oy += p1[b] * bq(b, q);
+          }
+          const Hit hp = deform_qp(ex, ey, q, ox, oy, n, h, width);
+          double t4[4];
+          const double t = eval_img(Tp, hp, n, t4) - rq[q];
+          a += 0.5 * (t * t) * 0.25;
+        }
+        a *= vol;
+        accS[k] = a;
+        accR[k] = 0.5 * (stiff_form(p0) + stiff_form(p1));
+      }
+    }
+  }
+  const int nblk = tiles;
+  for (int k = 0; k < nc; ++k) {
+    const double s = block_sum<256>(accS[k], red);
+    const double r = block_sum<256>(accR[k], red);
+    if (threadIdx.x == 0) {
+      part[((size_t)(pair * MAX_CAND + k) * 2 + 0) * nblk + tile] = s;
+      part[((size_t)(pair * MAX_CAND + k) * 2 + 1) * nblk + tile] = r;
+    }
+  }
+  run_tail(tail, pair, tiles);
+}
+
+// ================================================================================================
+// Variation DE = F_SSD + lambda K phi, node-centric with the element contributions staged in LDS.
+// F_SSD: SSDForce::getNonlinearity (registration.h:1646-1666) assembled as in
+// FENonlinVectorOpInterface::applyAdd (FEOpInterface.h:1803-1853); every node sums the
+// contributions of its (up to) four elements in the reference's element order.
+// ================================================================================================
+constexpr int GR_TX = 32, GR_TY = 8;  // nodes per workgroup tile
+constexpr int GR_EX = GR_TX + 1, GR_EY = GR_TY + 1;
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_grad(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__ phi, S *__restrict__ DE, int n,
+       double h, int width, const PairState *st, double lambda_default, int ntx, int nty, int npairs) {
+  const int tiles = ntx * nty;
+  const int wg = xcd_remap(blockIdx.x, tiles * npairs);
+  const int pair = wg / tiles, tile = wg - pair * tiles;
+  if (st && !participates(st[pair])) return;
+  const double lambda = st ? st[pair].lambda : lambda_default;
+
+  __shared__ double cf[GR_EY * GR_EX][4][2];  // force contribution per element, dof, component
+  __shared__ double ck[GR_EY * GR_EX][4][2];  // stiffness contribution
+
+  const size_t nn = (size_t)n * n;
+  const S *Rp = R + pair * nn, *Tp = T + pair * nn;
+  const S *px = phi + pair * 2 * nn, *py = px + nn;
+  const int x0 = (tile % ntx) * GR_TX, y0 = (tile / ntx) * GR_TY;
+  const double vol = h * h;
+
+  for (int e = threadIdx.x; e < GR_EX * GR_EY; e += 256) {
+    const int ex = x0 - 1 + (e % GR_EX), ey = y0 - 1 + (e / GR_EX);
+    if (ex < 0 || ey < 0 || ex >= n - 1 || ey >= n - 1) continue;
+    double f0[4], f1[4], rn[4];
+    for (int b = 0; b < 4; ++b) {
+      const int g = l2g(ex, ey, b, n);
+      f0[b] = ld(px, g);
+      f1[b] = ld(py, g);
+      rn[b] = ld(Rp, g);
+    }
+    double nl[4][2];
+    for (int q = 0; q < 4; ++q) {
+      double ox = 0., oy = 0., r = 0.;
+      for (int b = 0; b < 4; ++b) {
+        ox += f0[b] * bq(b, q);
+        oy += f1[b] * bq(b, q);
+      }
+      for (int b = 0; b < 4; ++b) r += rn[b] * bq(b, q);
+      const Hit hp = deform_qp(ex, ey, q, ox, oy, n, h, width);
+      double t4[4];
+      const double td = eval_img(Tp, hp, n, t4);
+      double g0 = 0., g1 = 0.;
+      for (int b = 0; b < 4; ++b) {
+        double v0 = q1dx(b, hp.ly) / h, v1 = q1dy(b, hp.lx) / h;
+        v0 *= t4[b];
+        v1 *= t4[b];
+        g0 += v0;
+        g1 += v1;
+      }
+      nl[q][0] = hp.inx ? g0 * (td - r) : 0;
+      nl[q][1] = hp.iny ? g1 * (td - r) : 0;
+    }
+    for (int dof = 0; dof < 4; ++dof) {
+      double a0 = 0, a1 = 0;
+      for (int q = 0; q < 4; ++q) {
+        const double fct = bq(dof, q) * 0.25;
+        a0 += fct * nl[q][0];
+        a1 += fct * nl[q][1];
+      }
+      cf[e][dof][0] = vol * a0;
+      cf[e][dof][1] = vol * a1;
+    }
+    // K_loc (1/6)[4 -1 -1 -2; -1 4 -2 -1; -1 -2 4 -1; -2 -1 -1 4] applied to the element's dofs
+    const double kl[4][4] = {{4. / 6, -1. / 6, -1. / 6, -2. / 6},
+                             {-1. / 6, 4. / 6, -2. / 6, -1. / 6},
+                             {-1. / 6, -2. / 6, 4. / 6, -1. / 6},
+                             {-2. / 6, -1. / 6, -1. / 6, 4. / 6}};
+    for (int i = 0; i < 4; ++i) {
+      double s0 = 0, s1 = 0;
+      for (int j = 0; j < 4; ++j) {
+        s0 += kl[i][j] * f0[j];
+        s1 += kl[i][j] * f1[j];
+      }
+      ck[e][i][0] = s0;
+      ck[e][i][1] = s1;
+    }
+  }
+  __syncthreads();
+  const int x = x0 + (threadIdx.x % GR_TX), y = y0 + (threadIdx.x / GR_TX);
+  if (x >= n || y >= n) return;
+  double F0 = 0, F1 = 0, K0 = 0, K1 = 0;
+  for (int ey = y - 1; ey <= y; ++ey)
+    for (int ex = x - 1; ex <= x; ++ex) {
+      if (ex < 0 || ey < 0 || ex >= n - 1 || ey >= n - 1) continue;
+      const int e = (ex - (x0 - 1)) + (ey - (y0 - 1)) * GR_EX;
+      const int dof = (x - ex) + 2 * (y - ey);
+      F0 += cf[e][dof][0];
+      F1 += cf[e][dof][1];
+      K0 += ck[e][dof][0];
+      K1 += ck[e][dof][1];
+    }
+  S *o = DE + pair * 2 * nn;
+  const int i = x + y * n;
+  o[i] = static_cast<S>(F0 + lambda * K0);
+  o[nn + i] = static_cast<S>(F1 + lambda * K1);
+}
+
+// ================================================================================================
+// H1 metric: one V-cycle of the FD heat operator (LinearSmoothOp::apply, linearSmoothOp.cpp:591-612,
+// MDMultigrid::solve L451-494, HeatMultigrid::jacobi L556-580) per field, 2 fields per pair.
+// Down-leg kernel: x1 = rhs/diag (first Jacobi sweep from zero), x2 = x1 + (rhs - A x1)/diag,
+// residual r = rhs - A x2, coarse rhs = 2x2 mean of r. Up-leg kernel: x = x2 + injected coarse
+// correction, then two Jacobi sweeps. Tiles with a 2-node halo live in LDS.
+// ================================================================================================
+constexpr int MG_TX = 32, MG_TY = 16;
+constexpr int MG_HX = MG_TX + 4, MG_HY = MG_TY + 4;
+
+__device__ __forceinline__ void level_consts(const PairState *st, int pair, double sigma_default, int n, double &hs,
+                                             double &diag) {
+  const double sigma = st ? st[pair].sigma : sigma_default;
+  const double tau = 0.5 * (sigma * sigma);  // LinearSmoothOp::setSigma (linearSmoothOp.h:50-52)
+  const double s = (double)((n - 1) * (n - 1));
+  hs = tau * s;
+  diag = 1 + 4 * tau * s;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_mg_down(const S *__restrict__ rhs, int negate, S *__restrict__ x2out, S *__restrict__ rhsc, int n,
+          const PairState *st, double sigma_default, int ntx, int nty, int nfields) {
+  const int tiles = ntx * nty;
+  const int wg = xcd_remap(blockIdx.x, tiles * nfields);
+  const int field = wg / tiles, tile = wg - field * tiles;
+  const int pair = field >> 1;
+  if (st && !participates(st[pair])) return;
+  double hs, diag;
+  level_consts(st, pair, sigma_default, n, hs, diag);
+
+  __shared__ double sr[MG_HY][MG_HX];   // rhs on tile+2
+  __shared__ double s1[MG_HY][MG_HX];   // x1 on tile+2
+  __shared__ double s2[MG_HY][MG_HX];   // x2 on tile+1
+  __shared__ double sres[MG_TY][MG_TX]; // residual on tile
+  const size_t nn = (size_t)n * n;
+  const S *rp = rhs + field * nn;
+  const int x0 = (tile % ntx) * MG_TX, y0 = (tile / ntx) * MG_TY;
+  const int bx = x0 - 2, by = y0 - 2;  // LDS origin
+
+  for (int i = threadIdx.x; i < MG_HX * MG_HY; i += 256) {
+    const int lx = i % MG_HX, ly = i / MG_HX;
+    const int gx = bx + lx, gy = by + ly;
+    if (gx >= 0 && gy >= 0 && gx < n && gy < n) {
+      double v = ld(rp, gx + gy * n);
+      if (negate) v = -v;
+      sr[ly][lx] = v;
+      s1[ly][lx] = v / diag;  // 0 + (1.0 * (rhs - A 0)) / diag
+    }
+  }
+  __syncthreads();
+  auto get1 = [&](int gx, int gy) { return s1[gy - by][gx - bx]; };
+  for (int i = threadIdx.x; i < MG_HX * MG_HY; i += 256) {
+    const int lx = i % MG_HX, ly = i / MG_HX;
+    const int gx = bx + lx, gy = by + ly;
+    if (lx >= 1 && ly >= 1 && lx < MG_HX - 1 && ly < MG_HY - 1 && gx >= 0 && gy >= 0 && gx < n && gy < n) {
+      const double ax = heat_apply(get1, gx, gy, n, hs);
+      const double v = sr[ly][lx] - ax;
+      s2[ly][lx] = s1[ly][lx] + 1.0 * v / diag;
+    }
+  }
+  __syncthreads();
+  auto get2 = [&](int gx, int gy) { return s2[gy - by][gx - bx]; };
+  S *xo = x2out + field * nn;
+  for (int i = threadIdx.x; i < MG_TX * MG_TY; i += 256) {
+    const int tx = i % MG_TX, ty = i / MG_TX;
+    const int gx = x0 + tx, gy = y0 + ty;
+    if (gx < n && gy < n) {
+      const double ax = heat_apply(get2, gx, gy, n, hs);
+      sres[ty][tx] = sr[ty + 2][tx + 2] - ax;
+      xo[gx + gy * n] = static_cast<S>(s2[ty + 2][tx + 2]);
+    }
+  }
+  __syncthreads();
+  const int nc = n / 2;
+  S *rc = rhsc + field * (size_t)nc * nc;
+  for (int i = threadIdx.x; i < (MG_TX / 2) * (MG_TY / 2); i += 256) {
+    const int cx = i % (MG_TX / 2), cy = i / (MG_TX / 2);
+    const int gcx = x0 / 2 + cx, gcy = y0 / 2 + cy;
+    if (gcx < nc && gcy < nc) {
+      double v = 0;
+      v += sres[2 * cy][2 * cx];
+      v += sres[2 * cy][2 * cx + 1];
+      v += sres[2 * cy + 1][2 * cx];
+      v += sres[2 * cy + 1][2 * cx + 1];
+      rc[gcx + gcy * nc] = static_cast<S>((1.0 / 4.0) * v);
+    }
+  }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_mg_up(const S *__restrict__ x2in, const S *__restrict__ xc, const S *__restrict__ rhs, int negate,
+        S *__restrict__ xout, int n, const PairState *st, double sigma_default, int ntx, int nty,
+        int nfields, double *__restrict__ nrm_part, Tail tail) {
+  const int tiles = ntx * nty;
+  const int wg = xcd_remap(blockIdx.x, tiles * nfields);
+  const int field = wg / tiles, tile = wg - field * tiles;
+  const int pair = field >> 1;
+  if (st && !participates(st[pair])) return;
+  double hs, diag;
+  level_consts(st, pair, sigma_default, n, hs, diag);
+
+  __shared__ double sx[MG_HY][MG_HX];  // x on tile+2
+  __shared__ double sr[MG_HY][MG_HX];  // rhs on tile+1
+  __shared__ double s3[MG_HY][MG_HX];  // x3 on tile+1
+  __shared__ double red[4];
+  const size_t nn = (size_t)n * n;
+  const int nc = n / 2;
+  const S *x2p = x2in + field * nn, *rp = rhs + field * nn, *xcp = xc + field * (size_t)nc * nc;
+  const int x0 = (tile % ntx) * MG_TX, y0 = (tile / ntx) * MG_TY;
+  const int bx = x0 - 2, by = y0 - 2;
+
+  for (int i = threadIdx.x; i < MG_HX * MG_HY; i += 256) {
+    const int lx = i % MG_HX, ly = i / MG_HX;
+    const int gx = bx + lx, gy = by + ly;
+    if (gx >= 0 && gy >= 0 && gx < n && gy < n) {
+      sx[ly][lx] = ld(x2p, gx + gy * n) + ld(xcp, (gx >> 1) + (gy >> 1) * nc);
+      double v = ld(rp, gx + gy * n);
+      sr[ly][lx] = negate ? -v : v;
+    }
+  }
+  __syncthreads();
+  auto getx = [&](int gx, int gy) { return sx[gy - by][gx - bx]; };
+  for (int i = threadIdx.x; i < MG_HX * MG_HY; i += 256) {
+    const int lx = i % MG_HX, ly = i / MG_HX;
+    const int gx = bx + lx, gy = by + ly;
+    if (lx >= 1 && ly >= 1 && lx < MG_HX - 1 && ly < MG_HY - 1 && gx >= 0 && gy >= 0 && gx < n && gy < n) {
+      const double ax = heat_apply(getx, gx, gy, n, hs);
+      const double v = sr[ly][lx] - ax;
+      s3[ly][lx] = sx[ly][lx] + 1.0 * v / diag;
+    }
+  }
+  __syncthreads();
+  auto get3 = [&](int gx, int gy) { return s3[gy - by][gx - bx]; };
+  S *xo = xout + field * nn;
+  double nrm = 0;
+  for (int i = threadIdx.x; i < MG_TX * MG_TY; i += 256) {
+    const int tx = i % MG_TX, ty = i / MG_TX;
+    const int gx = x0 + tx, gy = y0 + ty;
+    if (gx < n && gy < n) {
+      const double ax = heat_apply(get3, gx, gy, n, hs);
+      const double v = sr[ty + 2][tx + 2] - ax;
+      const S o = static_cast<S>(s3[ty + 2][tx + 2] + 1.0 * v / diag);
+      xo[gx + gy * n] = o;
+      const double od = static_cast<double>(o);
+      nrm += od * od;
+    }
+  }
+  if (nrm_part) {
+    const double s = block_sum<256>(nrm, red);
+    if (threadIdx.x == 0) nrm_part[(size_t)field * tiles + tile] = s;
+  }
+  run_tail(tail, pair, 2 * tiles);
+}
+
+// Whole V-cycle below (and including) a level of at most 32x32 nodes in one workgroup, in LDS.
+// With n equal to the registration level this is the complete metric; otherwise it is the coarse
+// part of the cycle started by k_mg_down.
+constexpr int MGC_MAXL = 5;
+constexpr int MGC_OFF[MGC_MAXL + 2] = {0, 1, 5, 21, 85, 341, 1365};  // prefix sums of 4^l
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_mg_coarse(const S *__restrict__ rhs, int negate, S *__restrict__ xout, int L, const PairState *st,
+            double sigma_default, int nfields, double *__restrict__ nrm_part, Tail tail) {
+  const int field = blockIdx.x;
+  const int pair = field >> 1;
+  if (st && !participates(st[pair])) return;
+  const double sigma = st ? st[pair].sigma : sigma_default;
+  const double tau = 0.5 * (sigma * sigma);
+  __shared__ double X[MGC_OFF[MGC_MAXL + 1]], B[MGC_OFF[MGC_MAXL + 1]], D[1024];
+  __shared__ double red[4];
+  const int n = 1 << L;
+  const size_t nn = (size_t)n * n;
+  const S *rp = rhs + field * nn;
+  for (int i = threadIdx.x; i < (int)nn; i += 256) {
+    double v = ld(rp, i);
+    B[MGC_OFF[L] + i] = negate ? -v : v;
+  }
+  __syncthreads();
+
+  // Jacobi sweep on level l: D = A x; x += (rhs - D)/diag
+  auto sweep = [&](int l) {
+    const int m = 1 << l;
+    const double s = (double)((m - 1) * (m - 1));
+    const double hs = tau * s, diag = 1 + 4 * tau * s;
+    double *x = X + MGC_OFF[l];
+    const double *b = B + MGC_OFF[l];
+    auto get = [&](int gx, int gy) { return x[gx + gy * m]; };
+    for (int i = threadIdx.x; i < m * m; i += 256) D[i] = heat_apply(get, i % m, i / m, m, hs);
+    __syncthreads();
+    for (int i = threadIdx.x; i < m * m; i += 256) {
+      const double v = b[i] - D[i];
+      x[i] += 1.0 * v / diag;
+    }
+    __syncthreads();
+  };
+
+  for (int l = L; l >= 1; --l) {
+    const int m = 1 << l;
+    double *x = X + MGC_OFF[l];
+    for (int i = threadIdx.x; i < m * m; i += 256) x[i] = 0;
+    __syncthreads();
+    sweep(l);
+    sweep(l);
+    const double s = (double)((m - 1) * (m - 1));
+    const double hs = tau * s;
+    auto get = [&](int gx, int gy) { return x[gx + gy * m]; };
+    for (int i = threadIdx.x; i < m * m; i += 256) D[i] = B[MGC_OFF[l] + i] - heat_apply(get, i % m, i / m, m, hs);
+    __syncthreads();
+    const int mc = m / 2;
+    for (int i = threadIdx.x; i < mc * mc; i += 256) {
+      const int cx = i % mc, cy = i / mc;
+      double v = 0;
+      v += D[2 * cx + 2 * cy * m];
+      v += D[2 * cx + 1 + 2 * cy * m];
+      v += D[2 * cx + (2 * cy + 1) * m];
+      v += D[2 * cx + 1 + (2 * cy + 1) * m];
+      B[MGC_OFF[l - 1] + i] = (1.0 / 4.0) * v;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {  // 1x1 level: 100 sweeps of x += (rhs - x) / 1
+    double x = 0;
+    const double b = B[0];
+    for (int it = 0; it < 100; ++it) {
+      const double v = b - x;
+      x += 1.0 * v / 1.0;
+    }
+    X[0] = x;
+  }
+  __syncthreads();
+  for (int l = 1; l <= L; ++l) {
+    const int m = 1 << l, mc = m / 2;
+    double *x = X + MGC_OFF[l];
+    const double *xcs = X + MGC_OFF[l - 1];
+    for (int i = threadIdx.x; i < m * m; i += 256) x[i] += xcs[((i % m) >> 1) + ((i / m) >> 1) * mc];
+    __syncthreads();
+    sweep(l);
+    sweep(l);
+  }
+  S *xo = xout + field * nn;
+  double nrm = 0;
+  for (int i = threadIdx.x; i < (int)nn; i += 256) {
+    const S o = static_cast<S>(X[MGC_OFF[L] + i]);
+    xo[i] = o;
+    const double od = static_cast<double>(o);
+    nrm += od * od;
+  }
+  if (nrm_part) {
+    const double s = block_sum<256>(nrm, red);
+    if (threadIdx.x == 0) nrm_part[field] = s;
+  }
+  run_tail(tail, pair, 2);
+}
+
+// ================================================================================================
+// Fletcher-Reeves direction d = z + beta * d_old (calcTaudAndDescentDir, gradientDescent.h:218-226)
+// and the slopes <DE, d>, <DE, z> (ArmijoLineSearchHelpFunction_evaluateDerivative, L176-182).
+// ================================================================================================
+constexpr int NCG_PER_THREAD = 4;
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_ncg(const S *__restrict__ z, S *__restrict__ d, const S *__restrict__ DE, int n, const PairState *st,
+      double *__restrict__ part, int nblk_field, int nfields, Tail tail) {
+  const int total = nblk_field * nfields;
+  const int wg = xcd_remap(blockIdx.x, total);
+  const int field = wg / nblk_field, blk = wg - field * nblk_field;
+  const int pair = field >> 1;
+  if (!participates(st[pair])) return;
+  const double beta = st[pair].beta;
+  const bool zero_old = st[pair].olddir_zero != 0;
+  __shared__ double red[4];
+  const size_t nn = (size_t)n * n;
+  const S *zp = z + field * nn, *dep = DE + field * nn;
+  S *dp = d + field * nn;
+  double sd = 0, sz = 0;
+  const int base = blk * 256 * NCG_PER_THREAD;
+  for (int k = 0; k < NCG_PER_THREAD; ++k) {
+    const int i = base + k * 256 + threadIdx.x;
+    if (i < (int)nn) {
+      const double zv = ld(zp, i);
+      const double od = zero_old ? 0.0 : ld(dp, i);
+      const S dv = static_cast<S>(zv + beta * od);
+      dp[i] = dv;
+      const double de = ld(dep, i);
+      sd += de * static_cast<double>(dv);
+      sz += de * zv;
+    }
+  }
+  const double a = block_sum<256>(sd, red);
+  const double b = block_sum<256>(sz, red);
+  if (threadIdx.x == 0) {
+    part[((size_t)field * 2 + 0) * nblk_field + blk] = a;
+    part[((size_t)field * 2 + 1) * nblk_field + blk] = b;
+  }
+  run_tail(tail, pair, 2 * nblk_field);
+}
+
+// phi += tau * dir for pairs whose search accepted a step (GradientDescentBase::applySingle L382-385).
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_update(S *__restrict__ phi, const S *__restrict__ dcg, const S *__restrict__ dgrad, int n, PairState *st,
+         int nblk_pair, int npairs, unsigned *counter) {
+  const int total = nblk_pair * npairs;
+  const int wg = xcd_remap(blockIdx.x, total);
+  const int pair = wg / nblk_pair, blk = wg - pair * nblk_pair;
+  const double tau = st[pair].step_tau;
+  if (tau == 0) return;  // no accepted step pending for this pair
+  const S *dir = (st[pair].step_dir == DIR_CG ? dcg : dgrad) + pair * 2 * (size_t)n * n;
+  S *p = phi + pair * 2 * (size_t)n * n;
+  const int m = 2 * n * n;
+  const int base = blk * 256 * NCG_PER_THREAD;
+  for (int k = 0; k < NCG_PER_THREAD; ++k) {
+    const int i = base + k * 256 + threadIdx.x;
+    if (i < m) p[i] = static_cast<S>(ld(p, i) + tau * ld(dir, i));
+  }
+  // the last workgroup of the pair marks the step as applied
+  if (last_block_of(counter + pair, nblk_pair) && threadIdx.x == 0) st[pair].step_tau = 0;
 }
 
 // ================================================================================================

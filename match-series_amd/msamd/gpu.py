@@ -75,6 +75,8 @@ SIGNATURES = {
                          ctypes.c_int),
     "msg_normalize": ([ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp], ctypes.c_int),
     "msg_release_cache": ([], ctypes.c_int),
+    "msg_ctx_set_mode": ([_vp, ctypes.c_int], ctypes.c_int),
+    "msg_set_default_mode": ([ctypes.c_int], ctypes.c_int),
     "msg_profile_enable": ([ctypes.c_uint], ctypes.c_int),
     "msg_profile_reset": ([], ctypes.c_int),
     "msg_profile_read": ([ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong), _dp], ctypes.c_int),
@@ -232,6 +234,9 @@ class Context:
         check(lib().msg_ctx_create(ctypes.byref(h), level, npairs, dtype))
         self.h = h
 
+    def set_mode(self, mode):
+        check(lib().msg_ctx_set_mode(self.h, mode))
+
     def close(self):
         if self.h:
             lib().msg_ctx_destroy(self.h)
@@ -339,3 +344,7 @@ def profile_read(cls):
     ms, n, b = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double()
     check(lib().msg_profile_read(KERNEL_CLASSES.index(cls), ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b)))
     return dict(ms=ms.value, launches=n.value, bytes=b.value)
+
+
+def set_default_mode(mode):
+    check(lib().msg_set_default_mode(mode))

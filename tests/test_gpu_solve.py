@@ -116,3 +116,32 @@ def test_match_series_f64(gpu, resample):
     np.testing.assert_array_equal(out["count"].cpu().numpy(), ref["count"])
     np.testing.assert_allclose(out["mean"].cpu().numpy(), ref["mean"], rtol=1e-9, atol=1e-12)
     np.testing.assert_allclose(out["median"].cpu().numpy(), ref["median"], rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("N,its,B", [(32, 40, 3), (128, 30, 2)])
+def test_device_driven_modes_match_host_mode(gpu, mode, N, its, B):
+    """The device-driven iteration (control in last-workgroup tails, 8 iterations per host check,
+    optionally one hipGraph) gives exactly the host-synchronised result."""
+    L = int(np.log2(N))
+    pairs = [stem_pair(N, 5 * k) for k in range(B)]
+    out = {}
+    for m in (0, mode):
+        ctx = gpu.Context(L, B, gpu.F64)
+        ctx.set_mode(m)
+        for k, (R, T) in enumerate(pairs):
+            ctx.set_reference(k, R)
+            ctx.set_template(k, T)
+        e, it, tr = ctx.solve_level(L, 10.0, its, -1.0, trace_cap=its)
+        out[m] = (e, it, [ctx.get_deformation(k) for k in range(B)], tr)
+        # a second solve reuses the captured graph
+        e2, it2, _ = ctx.solve_level(L, 10.0, its, -1.0, trace_cap=its)
+        np.testing.assert_array_equal(it2, it)
+        ctx.close()
+    e0, it0, phi0, tr0 = out[0]
+    e1, it1, phi1, tr1 = out[mode]
+    np.testing.assert_array_equal(it1, it0)
+    np.testing.assert_array_equal(e1, e0)
+    for k in range(B):
+        np.testing.assert_array_equal(phi1[k], phi0[k])
+        np.testing.assert_array_equal(tr1[k], tr0[k])
