@@ -38,6 +38,9 @@ const char *msg_version(void);
  * qc::StandardRegistrationMultilevelDescent (registration.h:1072-1132), one per pair.
  * ---------------------------------------------------------------------------------------------- */
 typedef struct msg_ctx msg_ctx;
+/* Streams: a context works on its own non-blocking HIP stream. Device arrays handed to a context
+ * call (or to msg_match_series) must be complete when the call is made — synchronise the stream
+ * that produced them — and the library has finished with every argument when the call returns. */
 
 /* RegistrationMultilevelDescentInterfaceBase(Parser) with precisionLevel = max_level
  * (registration.h:263-274); allocates every device buffer the solve needs. */
@@ -165,6 +168,9 @@ typedef struct {
   int dont_accumulate;                /* "dontAccumulateDeformation"                          */
   int spec;                           /* Armijo candidates per energy pass (0: default)       */
   double pre_smooth_sigma;            /* "preSmoothSigma" (pixels); <= 0: none                */
+  int pairwise_batch;                 /* pairwise solves per batch; 0: automatic. The chain   */
+                                      /* starts on the first finished batch and overlaps the  */
+                                      /* later ones (results do not depend on it)             */
 } msg_series_params;
 
 /* templates: n device images (dtype) at precision_level; reference: one device image.

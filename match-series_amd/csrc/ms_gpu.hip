@@ -6,10 +6,14 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <functional>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ms_kernels.h"
@@ -1239,6 +1243,116 @@ void ctx_release(msg_ctx *c) {
   g_ctx_cache.free.push_back(c);
 }
 
+struct CtxGuard {
+  msg_ctx *c;
+  ~CtxGuard() { ctx_release(c); }
+};
+
+// One background host thread for work that overlaps the caller's: msg_match_series runs its
+// pairwise batches there while the calling thread runs the chain. Never destroyed (a static
+// condition variable with a blocked waiter must not be destroyed at exit).
+struct Worker {
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<std::function<void()>> q;
+  bool started = false;
+  void post(std::function<void()> f) {
+    std::lock_guard<std::mutex> g(mu);
+    if (!started) {
+      std::thread([this] { loop(); }).detach();
+      started = true;
+    }
+    q.push_back(std::move(f));
+    cv.notify_one();
+  }
+  void loop() {
+    for (;;) {
+      std::function<void()> f;
+      {
+        std::unique_lock<std::mutex> l(mu);
+        cv.wait(l, [&] { return !q.empty(); });
+        f = std::move(q.front());
+        q.pop_front();
+      }
+      f();
+    }
+  }
+};
+Worker *worker() {
+  static Worker *w = new Worker;
+  return w;
+}
+
+// Progress of the pairwise batches as seen by the chain.
+struct Progress {
+  std::mutex mu;
+  std::condition_variable cv;
+  int done = 0;           // pairwise solves finished (a prefix of the pairs)
+  bool finished = false;  // the batch task has returned
+  bool abort = false;     // set by the chain after an error: no further batches
+  int rc = MSG_OK;
+  std::string err;
+};
+
+// Pairwise solves i0..i1-1 in one batched context: template i against its predecessor (the
+// reference for i = 0 or with dont_accumulate), SeriesMatching::matchAndAverageSeries
+// (matchSeries.h:136-176). Writes the fields to pw and the energies.
+template <typename S>
+int pairwise_batch(const msg_series_params *p, const msg_solve_params &sp, const S *tpl, const S *ref, int i0,
+                   int i1, S *pw, double *energies) {
+  const int L = p->precision_level;
+  const int N = 1 << L;
+  const size_t nn = (size_t)N * N;
+  const int dt = sizeof(S) == 8 ? MSG_F64 : MSG_F32;
+  const int nb = i1 - i0;
+  msg_ctx *pc = nullptr;
+  CHK(ctx_acquire(L, nb, dt, &pc));
+  CtxGuard gpc{pc};
+  hipStream_t s = pc->stream;
+  for (int i = i0; i < i1; ++i) {
+    const S *r = (i == 0 || p->dont_accumulate) ? ref : tpl + (size_t)(i - 1) * nn;
+    HIPCHK(hipMemcpyAsync((S *)pc->R[L] + (i - i0) * nn, r, nn * sizeof(S), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync((S *)pc->T[L] + (i - i0) * nn, tpl + (size_t)i * nn, nn * sizeof(S),
+                          hipMemcpyDeviceToDevice, s));
+  }
+  if (!p->dont_normalize) {
+    CHK(normalize_impl<S>(pc->R[L], N, nb, p->saturation_pct, p->normalize_min_to_zero, s));
+    CHK(normalize_impl<S>(pc->T[L], N, nb, p->saturation_pct, p->normalize_min_to_zero, s));
+  }
+  if (p->pre_smooth_sigma > 0) {  // prepareMultilevelArray's optional pre-smoothing (registration.h:356-362)
+    const double sig = p->pre_smooth_sigma / (double)(N - 1);
+    CHK(h1_impl<S>(pc->R[L], pc->R[L], N, nb, sig, s));
+    CHK(h1_impl<S>(pc->T[L], pc->T[L], N, nb, sig, s));
+  }
+  CHK(rebuild_pyramid<S>(pc, pc->R, -1, 1));
+  CHK(rebuild_pyramid<S>(pc, pc->T, -1, 1));
+  CHK(solve_impl<S>(pc, p->start_level, p->stop_level, &sp, energies + i0));
+  HIPCHK(hipMemcpyAsync(pw + (size_t)i0 * 2 * nn, pc->phi[L], sizeof(S) * 2 * nn * nb, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSG_OK;
+}
+
+// Runs the batches [k*b, min((k+1)*b, npw)) in order, publishing progress after each.
+template <typename S>
+void pairwise_all(const msg_series_params *p, const msg_solve_params &sp, const S *tpl, const S *ref, int npw,
+                  int b, S *pw, double *energies, Progress *pr) {
+  int rc = MSG_OK;
+  for (int i0 = 0; i0 < npw && rc == MSG_OK; i0 += b) {
+    {
+      std::lock_guard<std::mutex> g(pr->mu);
+      if (pr->abort) break;
+    }
+    rc = pairwise_batch<S>(p, sp, tpl, ref, i0, std::min(i0 + b, npw), pw, energies);
+    std::lock_guard<std::mutex> g(pr->mu);
+    if (rc == MSG_OK) pr->done = std::min(i0 + b, npw);
+    else { pr->rc = rc; pr->err = g_err; }
+    pr->cv.notify_all();
+  }
+  std::lock_guard<std::mutex> g(pr->mu);
+  pr->finished = true;
+  pr->cv.notify_all();
+}
+
 template <typename S>
 int match_series_impl(const msg_series_params *p, const void *templates, int n, const void *reference,
                       const void *pairwise_in, void *deformations, void *pairwise_out, double *energies, void *mean,
@@ -1258,99 +1372,120 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
   sp.spec = p->spec;
   const bool reuse = pairwise_in != nullptr;
   const int npw = reuse ? 1 : n;  // pairwise solves to run (pairs 0..npw-1)
+  const bool chain = !p->dont_accumulate;
+  // Batches: with a chain to overlap, about four (the chain starts when the first is done);
+  // otherwise one.
+  int b = p->pairwise_batch > 0 ? p->pairwise_batch : (chain ? (npw + 3) / 4 : npw);
+  b = std::max(1, std::min(b, npw));
+  const bool overlap = chain && b < npw;
 
-  // --- batched pairwise solves: template i against its predecessor (or the reference) ---
-  msg_ctx *pc = nullptr;
-  CHK(ctx_acquire(L, npw, dt, &pc));
-  struct Guard {
-    msg_ctx *c;
-    ~Guard() { ctx_release(c); }
-  } gpc{pc};
-  hipStream_t s = pc->stream;
-  for (int i = 0; i < npw; ++i) {
-    const S *r = (i == 0 || p->dont_accumulate) ? ref : tpl + (size_t)(i - 1) * nn;
-    HIPCHK(hipMemcpyAsync((S *)pc->R[L] + i * nn, r, nn * sizeof(S), hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipMemcpyAsync((S *)pc->T[L] + i * nn, tpl + (size_t)i * nn, nn * sizeof(S), hipMemcpyDeviceToDevice, s));
-  }
-  if (!p->dont_normalize) {
-    CHK(normalize_impl<S>(pc->R[L], N, npw, p->saturation_pct, p->normalize_min_to_zero, s));
-    CHK(normalize_impl<S>(pc->T[L], N, npw, p->saturation_pct, p->normalize_min_to_zero, s));
-  }
-  if (p->pre_smooth_sigma > 0) {  // prepareMultilevelArray's optional pre-smoothing (registration.h:356-362)
-    const double sig = p->pre_smooth_sigma / (double)(N - 1);
-    CHK(h1_impl<S>(pc->R[L], pc->R[L], N, npw, sig, s));
-    CHK(h1_impl<S>(pc->T[L], pc->T[L], N, npw, sig, s));
-  }
-  CHK(rebuild_pyramid<S>(pc, pc->R, -1, 1));
-  CHK(rebuild_pyramid<S>(pc, pc->T, -1, 1));
-  std::vector<double> epw(npw);
-  CHK(solve_impl<S>(pc, p->start_level, p->stop_level, &sp, epw.data()));
+  hipStream_t s = nullptr;  // the caller thread's stream for the reconstruction
+  msg_ctx *rc = nullptr;
+  if (chain) CHK(ctx_acquire(L, 1, dt, &rc));
+  CtxGuard grc{rc};
 
-  // pairwise deformation of every template
   DevBuf pw;
   HIPCHK(hipMalloc(&pw.p, sizeof(S) * 2 * nn * n));
   S *pwp = (S *)pw.p;
-  HIPCHK(hipMemcpyAsync(pwp, pc->phi[L], sizeof(S) * 2 * nn * npw, hipMemcpyDeviceToDevice, s));
-  if (reuse && n > 1)
-    HIPCHK(hipMemcpyAsync(pwp + 2 * nn, (const S *)pairwise_in + 2 * nn, sizeof(S) * 2 * nn * (n - 1),
-                          hipMemcpyDeviceToDevice, s));
-  if (pairwise_out) HIPCHK(hipMemcpyAsync(pairwise_out, pwp, sizeof(S) * 2 * nn * n, hipMemcpyDeviceToDevice, s));
-  for (int i = 0; i < n; ++i) energies[i] = (i < npw) ? epw[i] : NAN;
-  S *defs = (S *)deformations;
+  for (int i = 0; i < n; ++i) energies[i] = NAN;
+  if (reuse && n > 1) {
+    HIPCHK(hipMemcpy(pwp + 2 * nn, (const S *)pairwise_in + 2 * nn, sizeof(S) * 2 * nn * (n - 1),
+                     hipMemcpyDeviceToDevice));
+  }
 
-  if (p->dont_accumulate) {
-    HIPCHK(hipMemcpyAsync(defs, pwp, sizeof(S) * 2 * nn * n, hipMemcpyDeviceToDevice, s));
+  Progress pr;
+  if (overlap) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    worker()->post([=, &pr, &sp] {
+      const hipError_t e = hipSetDevice(dev);
+      if (e != hipSuccess) {
+        std::lock_guard<std::mutex> g(pr.mu);
+        pr.rc = MSG_ERR_HIP;
+        pr.err = std::string("hipSetDevice: ") + hipGetErrorString(e);
+        pr.finished = true;
+        pr.cv.notify_all();
+        return;
+      }
+      pairwise_all<S>(p, sp, tpl, ref, npw, b, pwp, energies, &pr);
+    });
+  } else {
+    pairwise_all<S>(p, sp, tpl, ref, npw, b, pwp, energies, &pr);
+    if (pr.rc != MSG_OK) return fail(pr.rc, pr.err);
+  }
+  // Waits (on the calling thread) until pairwise solve i is done; false after a batch failed.
+  auto wait_pairwise = [&](int i) {
+    std::unique_lock<std::mutex> l(pr.mu);
+    pr.cv.wait(l, [&] { return pr.done > i || pr.rc != MSG_OK || pr.finished; });
+    return pr.done > i;
+  };
+  // Stops the batches after an error on the chain and waits for the worker to let go of the buffers.
+  auto join = [&](int code) {
+    if (overlap) {
+      std::unique_lock<std::mutex> l(pr.mu);
+      pr.abort = true;
+      pr.cv.wait(l, [&] { return pr.finished; });
+    }
+    return code;
+  };
+
+  S *defs = (S *)deformations;
+  if (!chain) {
+    HIPCHK(hipMemcpy(defs, pwp, sizeof(S) * 2 * nn * n, hipMemcpyDeviceToDevice));
   } else {
     // --- sequential chain: accumulate by composition, refine against the reference ---
-    msg_ctx *rc = nullptr;
-    CHK(ctx_acquire(L, 1, dt, &rc));
-    Guard grc{rc};
-    HIPCHK(hipStreamSynchronize(s));
     hipStream_t rs = rc->stream;
-    HIPCHK(hipMemcpyAsync(rc->R[L], ref, nn * sizeof(S), hipMemcpyDeviceToDevice, rs));
-    if (!p->dont_normalize) CHK(normalize_impl<S>(rc->R[L], N, 1, p->saturation_pct, p->normalize_min_to_zero, rs));
-    if (p->pre_smooth_sigma > 0) CHK(h1_impl<S>(rc->R[L], rc->R[L], N, 1, p->pre_smooth_sigma / (double)(N - 1), rs));
-    CHK(rebuild_pyramid<S>(rc, rc->R, -1, 1));
-    DevBuf acc;
-    HIPCHK(hipMalloc(&acc.p, sizeof(S) * 2 * nn));
-    HIPCHK(hipMemsetAsync(acc.p, 0, sizeof(S) * 2 * nn, rs));
-    DevBuf prev;
-    HIPCHK(hipMalloc(&prev.p, sizeof(S) * 2 * nn));
-    for (int i = 0; i < n; ++i) {
-      HIPCHK(hipMemcpyAsync(prev.p, acc.p, sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, rs));
-      CHK(compose_impl<S>(pwp + (size_t)i * 2 * nn, prev.p, acc.p, N, rs));
-      if (i > 0) {
-        HIPCHK(hipMemcpyAsync(rc->T[L], tpl + (size_t)i * nn, nn * sizeof(S), hipMemcpyDeviceToDevice, rs));
-        if (!p->dont_normalize)
-          CHK(normalize_impl<S>(rc->T[L], N, 1, p->saturation_pct, p->normalize_min_to_zero, rs));
-        if (p->pre_smooth_sigma > 0)
-          CHK(h1_impl<S>(rc->T[L], rc->T[L], N, 1, p->pre_smooth_sigma / (double)(N - 1), rs));
-        CHK(rebuild_pyramid<S>(rc, rc->T, -1, 1));
-        HIPCHK(hipMemcpyAsync(rc->phi[L], acc.p, sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, rs));
-        CHK(rebuild_pyramid<S>(rc, rc->phi, -1, 2));
-        double e = 0;
-        CHK(solve_impl<S>(rc, p->refine_start_level, p->refine_stop_level, &sp, &e));
-        energies[i] = e;
-        HIPCHK(hipMemcpyAsync(acc.p, rc->phi[L], sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, rs));
+    s = rs;
+    const int code = [&]() -> int {
+      HIPCHK(hipMemcpyAsync(rc->R[L], ref, nn * sizeof(S), hipMemcpyDeviceToDevice, rs));
+      if (!p->dont_normalize) CHK(normalize_impl<S>(rc->R[L], N, 1, p->saturation_pct, p->normalize_min_to_zero, rs));
+      if (p->pre_smooth_sigma > 0) CHK(h1_impl<S>(rc->R[L], rc->R[L], N, 1, p->pre_smooth_sigma / (double)(N - 1), rs));
+      CHK(rebuild_pyramid<S>(rc, rc->R, -1, 1));
+      DevBuf acc;
+      HIPCHK(hipMalloc(&acc.p, sizeof(S) * 2 * nn));
+      HIPCHK(hipMemsetAsync(acc.p, 0, sizeof(S) * 2 * nn, rs));
+      DevBuf prev;
+      HIPCHK(hipMalloc(&prev.p, sizeof(S) * 2 * nn));
+      for (int i = 0; i < n; ++i) {
+        if (i < npw && !wait_pairwise(i)) return pr.rc;
+        HIPCHK(hipMemcpyAsync(prev.p, acc.p, sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, rs));
+        CHK(compose_impl<S>(pwp + (size_t)i * 2 * nn, prev.p, acc.p, N, rs));
+        if (i > 0) {
+          HIPCHK(hipMemcpyAsync(rc->T[L], tpl + (size_t)i * nn, nn * sizeof(S), hipMemcpyDeviceToDevice, rs));
+          if (!p->dont_normalize)
+            CHK(normalize_impl<S>(rc->T[L], N, 1, p->saturation_pct, p->normalize_min_to_zero, rs));
+          if (p->pre_smooth_sigma > 0)
+            CHK(h1_impl<S>(rc->T[L], rc->T[L], N, 1, p->pre_smooth_sigma / (double)(N - 1), rs));
+          CHK(rebuild_pyramid<S>(rc, rc->T, -1, 1));
+          HIPCHK(hipMemcpyAsync(rc->phi[L], acc.p, sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, rs));
+          CHK(rebuild_pyramid<S>(rc, rc->phi, -1, 2));
+          double e = 0;
+          CHK(solve_impl<S>(rc, p->refine_start_level, p->refine_stop_level, &sp, &e));
+          energies[i] = e;
+          HIPCHK(hipMemcpyAsync(acc.p, rc->phi[L], sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, rs));
+        }
+        HIPCHK(hipMemcpyAsync(defs + (size_t)i * 2 * nn, (i > 0) ? acc.p : (const void *)pwp, sizeof(S) * 2 * nn,
+                              hipMemcpyDeviceToDevice, rs));
       }
-      HIPCHK(hipMemcpyAsync(defs + (size_t)i * 2 * nn, (i > 0) ? acc.p : (const void *)pwp, sizeof(S) * 2 * nn,
-                            hipMemcpyDeviceToDevice, rs));
-    }
-    HIPCHK(hipStreamSynchronize(rs));
+      HIPCHK(hipStreamSynchronize(rs));
+      return MSG_OK;
+    }();
+    join(code);
+    if (code != MSG_OK) return code == pr.rc && !pr.err.empty() ? fail(code, pr.err) : code;
+    if (pr.rc != MSG_OK) return fail(pr.rc, pr.err);
   }
+  if (pairwise_out) HIPCHK(hipMemcpy(pairwise_out, pwp, sizeof(S) * 2 * nn * n, hipMemcpyDeviceToDevice));
 
   // --- reconstruction: warp the raw templates (Inf outside), then mean / count / median ---
+  if (n > msg::ST_MAXF) return fail(MSG_ERR_ARG, "series longer than 128 frames are not supported by the median kernel");
   DevBuf stack;
   HIPCHK(hipMalloc(&stack.p, sizeof(S) * nn * std::max(n, 1)));
-  if (n > 0) {
-    const size_t cnt = nn * n;
-    hipLaunchKernelGGL((msg::k_warp<S>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, tpl, (const S *)defs,
-                       (S *)stack.p, N, n, 0, 0, (double)INFINITY, 0);
-    HIPCHK(hipGetLastError());
-  }
+  const size_t cnt = nn * n;
+  hipLaunchKernelGGL((msg::k_warp<S>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, tpl, (const S *)defs,
+                     (S *)stack.p, N, n, 0, 0, (double)INFINITY, 0);
+  HIPCHK(hipGetLastError());
   double fill = 0;
-  if (n > 0) CHK(mean_value<S>(tpl + (size_t)(n - 1) * nn, (int)nn, &fill, s));
-  if (n > msg::ST_MAXF) return fail(MSG_ERR_ARG, "series longer than 128 frames are not supported by the median kernel");
+  CHK(mean_value<S>(tpl + (size_t)(n - 1) * nn, (int)nn, &fill, s));
   CHK(stats_impl<S>(stack.p, n, N, fill, mean, count, median, s));
   HIPCHK(hipStreamSynchronize(s));
   return MSG_OK;

@@ -116,6 +116,7 @@ msg_series_params seriesParams(const ParameterParser &p) {
   s.dont_accumulate = p.checkAndGetBool("dontAccumulateDeformation");
   s.spec = p.getIntOrDefault("gpuArmijoCandidates", 0);
   s.pre_smooth_sigma = p.getDoubleOrDefault("preSmoothSigma", 0.0);
+  s.pairwise_batch = p.getIntOrDefault("gpuPairwiseBatch", 0);
   return s;
 }
 

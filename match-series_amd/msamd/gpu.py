@@ -39,6 +39,7 @@ class SeriesParams(ctypes.Structure):
         ("saturation_pct", ctypes.c_double), ("normalize_min_to_zero", ctypes.c_int),
         ("dont_normalize", ctypes.c_int), ("resample_prolong", ctypes.c_int),
         ("dont_accumulate", ctypes.c_int), ("spec", ctypes.c_int), ("pre_smooth_sigma", ctypes.c_double),
+        ("pairwise_batch", ctypes.c_int),
     ]
 
 
@@ -130,6 +131,17 @@ def _dptr(a):
     return a.ctypes.data_as(_dp)
 
 
+def _stream():
+    """torch's current stream: the operator entry points run on it, ordered with torch's own work."""
+    return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def _ready():
+    """Context calls and msg_match_series run on the library's own streams: wait for torch's pending
+    work on their device inputs (and on freshly allocated outputs) first."""
+    _torch().cuda.current_stream().synchronize()
+
+
 def torch_dtype(code):
     torch = _torch()
     return torch.float64 if code == F64 else torch.float32
@@ -143,14 +155,14 @@ def energy(R, T, phi):
     B, n = R.shape[0], R.shape[-1]
     es = np.zeros(B)
     er = np.zeros(B)
-    check(lib().msg_energy(dtype_code(R), _ptr(R), _ptr(T), _ptr(phi), n, B, _dptr(es), _dptr(er), None))
+    check(lib().msg_energy(dtype_code(R), _ptr(R), _ptr(T), _ptr(phi), n, B, _dptr(es), _dptr(er), _stream()))
     return es, er
 
 
 def gradient(R, T, phi, lam):
     B, n = R.shape[0], R.shape[-1]
     out = _torch().empty_like(phi)
-    check(lib().msg_gradient(dtype_code(R), _ptr(R), _ptr(T), _ptr(phi), _ptr(out), n, B, lam, None))
+    check(lib().msg_gradient(dtype_code(R), _ptr(R), _ptr(T), _ptr(phi), _ptr(out), n, B, lam, _stream()))
     return out
 
 
@@ -159,7 +171,7 @@ def h1_smooth(d, sigma):
     n = d.shape[-1]
     k = d.numel() // (n * n)
     out = _torch().empty_like(d)
-    check(lib().msg_h1_smooth(dtype_code(d), _ptr(d), _ptr(out), n, k, sigma, None))
+    check(lib().msg_h1_smooth(dtype_code(d), _ptr(d), _ptr(out), n, k, sigma, _stream()))
     return out
 
 
@@ -169,7 +181,7 @@ def warp(img, phi, mode=0, ext=float("inf"), nearest=False):
     k = img.numel() // (n * n)
     shared = 1 if phi.dim() == 3 else 0
     out = _torch().empty_like(img)
-    check(lib().msg_warp(dtype_code(img), _ptr(img), _ptr(phi), _ptr(out), n, k, shared, mode, ext, int(nearest), None))
+    check(lib().msg_warp(dtype_code(img), _ptr(img), _ptr(phi), _ptr(out), n, k, shared, mode, ext, int(nearest), _stream()))
     return out
 
 
@@ -177,7 +189,7 @@ def restrict(fine):
     n = fine.shape[-1]
     k = fine.numel() // (n * n)
     out = _torch().empty(fine.shape[:-2] + (n // 2, n // 2), dtype=fine.dtype, device=fine.device)
-    check(lib().msg_restrict(dtype_code(fine), _ptr(fine), _ptr(out), n, k, None))
+    check(lib().msg_restrict(dtype_code(fine), _ptr(fine), _ptr(out), n, k, _stream()))
     return out
 
 
@@ -185,7 +197,7 @@ def prolong(coarse):
     nc = coarse.shape[-1]
     k = coarse.numel() // (nc * nc)
     out = _torch().empty(coarse.shape[:-2] + (2 * nc, 2 * nc), dtype=coarse.dtype, device=coarse.device)
-    check(lib().msg_prolong(dtype_code(coarse), _ptr(coarse), _ptr(out), nc, k, None))
+    check(lib().msg_prolong(dtype_code(coarse), _ptr(coarse), _ptr(out), nc, k, _stream()))
     return out
 
 
@@ -193,13 +205,13 @@ def resample(src, n):
     nc = src.shape[-1]
     k = src.numel() // (nc * nc)
     out = _torch().empty(src.shape[:-2] + (n, n), dtype=src.dtype, device=src.device)
-    check(lib().msg_resample(dtype_code(src), _ptr(src), _ptr(out), nc, n, k, None))
+    check(lib().msg_resample(dtype_code(src), _ptr(src), _ptr(out), nc, n, k, _stream()))
     return out
 
 
 def compose(phi1, phi2, start=None):
     out = _torch().zeros_like(phi1) if start is None else start.clone()
-    check(lib().msg_compose(dtype_code(phi1), _ptr(phi1), _ptr(phi2), _ptr(out), phi1.shape[-1], None))
+    check(lib().msg_compose(dtype_code(phi1), _ptr(phi1), _ptr(phi2), _ptr(out), phi1.shape[-1], _stream()))
     return out
 
 
@@ -209,7 +221,7 @@ def series_stats(stack, fill):
     mean = torch.empty((n, n), dtype=stack.dtype, device=stack.device)
     cnt = torch.empty_like(mean)
     med = torch.empty_like(mean)
-    check(lib().msg_series_stats(dtype_code(stack), _ptr(stack), k, n, fill, _ptr(mean), _ptr(cnt), _ptr(med), None))
+    check(lib().msg_series_stats(dtype_code(stack), _ptr(stack), k, n, fill, _ptr(mean), _ptr(cnt), _ptr(med), _stream()))
     return mean, cnt, med
 
 
@@ -217,7 +229,7 @@ def normalize(img, pct=-1.0, min_to_zero=True):
     """In place on (k, n, n) images."""
     n = img.shape[-1]
     k = img.numel() // (n * n)
-    check(lib().msg_normalize(dtype_code(img), _ptr(img), n, k, pct, int(min_to_zero), None))
+    check(lib().msg_normalize(dtype_code(img), _ptr(img), n, k, pct, int(min_to_zero), _stream()))
     return img
 
 
@@ -257,6 +269,7 @@ class Context:
         check(lib().msg_set_template(self.h, pair, _dptr(a)))
 
     def set_images_device(self, ref, tpl):
+        _ready()
         check(lib().msg_set_images_device(self.h, _ptr(ref) if ref is not None else None,
                                           _ptr(tpl) if tpl is not None else None,
                                           dtype_code(ref if ref is not None else tpl)))
@@ -274,9 +287,11 @@ class Context:
         check(lib().msg_set_deformation_zero(self.h, pair))
 
     def set_deformations_device(self, phi):
+        _ready()
         check(lib().msg_set_deformations_device(self.h, _ptr(phi)))
 
     def get_deformations_device(self, phi):
+        _ready()
         check(lib().msg_get_deformations_device(self.h, _ptr(phi)))
 
     def solve_level(self, level, lam, max_iter, stop_eps, spec=0, trace_cap=0):
@@ -317,6 +332,7 @@ def match_series(templates, reference, params, pairwise_in=None, want_pairwise=F
     mean = torch.zeros((N, N), dtype=templates.dtype, device=templates.device)
     cnt = torch.zeros_like(mean)
     med = torch.zeros_like(mean)
+    _ready()
     check(lib().msg_match_series(ctypes.byref(params), dtype_code(templates), _ptr(templates), n, _ptr(reference),
                                  _ptr(pairwise_in) if pairwise_in is not None else None, _ptr(defs),
                                  _ptr(pw) if pw is not None else None, _dptr(en), _ptr(mean), _ptr(cnt), _ptr(med)))

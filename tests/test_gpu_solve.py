@@ -145,3 +145,23 @@ def test_device_driven_modes_match_host_mode(gpu, mode, N, its, B):
     for k in range(B):
         np.testing.assert_array_equal(phi1[k], phi0[k])
         np.testing.assert_array_equal(tr1[k], tr0[k])
+
+
+def test_match_series_independent_of_pairwise_batching(gpu):
+    """The chain overlaps the later pairwise batches on a second host thread; the result is the same
+    for every batch size (one pair per batch up to all pairs in one)."""
+    from msamd import synth
+    N = 64
+    frames = synth.stem_series(N, 6, workload=0, spacing=6.0, sigma=1.5, jitter_step=0.15)
+    t = torch.tensor(frames, dtype=torch.float32, device="cuda")
+    outs = []
+    for b in (1, 2, 5, 0):
+        gp = gpu.SeriesParams(start_level=4, stop_level=6, refine_start_level=5, refine_stop_level=6,
+                              precision_level=6, lambda_=10.0, lambda_factor=-1, max_iter=12, stop_eps=-1.0,
+                              saturation_pct=0.15, normalize_min_to_zero=1, dont_normalize=0, resample_prolong=1,
+                              dont_accumulate=0, spec=0, pairwise_batch=b)
+        o = gpu.match_series(t[1:].contiguous(), t[0].contiguous(), gp)
+        outs.append({k: (v.cpu().numpy() if hasattr(v, "cpu") else np.asarray(v)) for k, v in o.items()})
+    for o in outs[1:]:
+        for k in ("deformations", "energies", "mean", "count", "median"):
+            np.testing.assert_array_equal(o[k], outs[0][k], err_msg=k)
