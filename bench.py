@@ -125,6 +125,19 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "launches": prof["launches"], "avg_launch_us": round(1000.0 * prof["ms"] / max(prof["launches"], 1), 2)}
 
+    # measured HBM ceiling: a plain device copy of 1 GiB (read + write), torch's copy kernel
+    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    roofline["copy_ceiling_gbs"] = round(10 * 2 * a.numel() * 4 / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
+    del a, b
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(N, args.cpu_iters, (levels, refine, args.iters), F)

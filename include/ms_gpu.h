@@ -173,14 +173,29 @@ typedef struct {
                                       /* later ones (results do not depend on it)             */
 } msg_series_params;
 
+/* Optional per-level records of msg_match_series: what the reference writes after every level of
+ * every solve as <i>/ or <i>-r/deformation_<LL>_<c>.dat.bz2 and energy_<LL>.txt
+ * (RegistrationConfigurator::writeCurrentRegistration via descentOnCurrentGrid,
+ * registration.h:1092-1112; GradientDescentBase::writeEnergy, aol/gradientDescent.h:254-271).
+ * Records are ordered by solve — the pairwise solves i = 0..npw-1 (npw = 1 with pairwise_in,
+ * else n) over levels start..stop, then the refinements i = 1..n-1 over
+ * refine_start..refine_stop (none with dont_accumulate) — and by ascending level within a solve.
+ * Every pointer may be NULL. All host memory. */
+typedef struct {
+  double *fields;         /* 2*N_l^2 doubles per record (x component, then y), back to back    */
+  msg_iter_trace *iters;  /* max_iter records per record slot (the first `iterations` are set) */
+  double *start;          /* 2 per record: E and lambda_l*E_reg at the start of the level     */
+  int *iterations;        /* 1 per record                                                       */
+} msg_series_levels;
+
 /* templates: n device images (dtype) at precision_level; reference: one device image.
  * pairwise_in: NULL or n device fields; entry i >= 1 replaces template i's pairwise solve
  * ("reuseStage1Results"). Outputs (device, dtype): deformations (n fields, those used for the
  * average), pairwise_out (n fields or NULL), mean/count/median (one image each); energies: host,
- * n values. */
+ * n values; levels: NULL or the per-level records above. */
 int msg_match_series(const msg_series_params *p, int dtype, const void *templates, int n, const void *reference,
                      const void *pairwise_in, void *deformations, void *pairwise_out, double *energies,
-                     void *mean, void *count, void *median);
+                     void *mean, void *count, void *median, const msg_series_levels *levels);
 /* msg_match_series keeps its device contexts for reuse by later calls of the same shape (no
  * counterpart in the reference); this frees them. */
 int msg_release_cache(void);

@@ -366,6 +366,7 @@ struct msg_ctx {
     hipGraphExec_t exec;
   };
   std::vector<GraphEntry> graphs;
+  std::vector<double> level_start;  // per pair: E and lambda*E_reg at the start of the last level solve
 };
 
 namespace {
@@ -537,6 +538,11 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
     c->cnt_evals += hs[p].total_evals;
     c->cnt_passes += hs[p].total_passes;
   }
+  c->level_start.resize(2 * B);
+  for (int p = 0; p < B; ++p) {
+    c->level_start[2 * p] = hs[p].energy_start;
+    c->level_start[2 * p + 1] = hs[p].ereg_start * lambda;
+  }
   if (tr) {
     static_assert(sizeof(msg::IterTrace) == sizeof(msg_iter_trace), "trace layout");
     HIPCHK(hipMemcpyAsync(trace, tr, sizeof(msg::IterTrace) * trace_cap * B, hipMemcpyDeviceToHost, s));
@@ -682,6 +688,11 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
     c->cnt_evals += hs[p].total_evals;
     c->cnt_passes += hs[p].total_passes;
   }
+  c->level_start.resize(2 * B);
+  for (int p = 0; p < B; ++p) {
+    c->level_start[2 * p] = hs[p].energy_start;
+    c->level_start[2 * p + 1] = hs[p].ereg_start * lambda;
+  }
   if (tr) {
     HIPCHK(hipMemcpyAsync(trace, tr, sizeof(msg::IterTrace) * trace_cap * B, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -710,13 +721,66 @@ int prolong_deformation(msg_ctx *c, int l, int resample) {
   return MSG_OK;
 }
 
+// Where the per-level records of a context's solves go (msg_series_levels): the context's pair k
+// at level l is record rec0 + k * (stop - start + 1) + (l - start); its field starts at double
+// offset field0 + k * per_solve + sum_{start <= m < l} 2 * 4^m.
+struct LevelOut {
+  const msg_series_levels *lv = nullptr;
+  long long rec0 = 0;
+  size_t field0 = 0;
+};
+
+size_t level_block(int start, int stop) {
+  size_t t = 0;
+  for (int l = start; l <= stop; ++l) t += (size_t)2 << (2 * l);
+  return t;
+}
+
 template <typename S>
-int solve_impl(msg_ctx *c, int start, int stop, const msg_solve_params *p, double *energies) {
+int record_level(msg_ctx *c, int level, int start, int stop, int max_iter, const LevelOut &o,
+                 const std::vector<msg_iter_trace> &tr, const std::vector<int> &its) {
+  const msg_series_levels *lv = o.lv;
+  const int nl = stop - start + 1;
+  const size_t nn2 = (size_t)2 << (2 * level);
+  const size_t per_solve = level_block(start, stop);
+  const size_t lev_off = level_block(start, level) - nn2;
+  std::vector<S> h;
+  if (lv->fields) {
+    h.resize(nn2 * c->B);
+    HIPCHK(hipMemcpyAsync(h.data(), c->phi[level], sizeof(S) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  for (int k = 0; k < c->B; ++k) {
+    const long long r = o.rec0 + (long long)k * nl + (level - start);
+    if (lv->fields) {
+      double *dst = lv->fields + o.field0 + k * per_solve + lev_off;
+      for (size_t i = 0; i < nn2; ++i) dst[i] = (double)h[k * nn2 + i];
+    }
+    if (lv->iters && max_iter > 0)
+      std::memcpy(lv->iters + r * max_iter, tr.data() + (size_t)k * max_iter, sizeof(msg_iter_trace) * max_iter);
+    if (lv->start) {
+      lv->start[2 * r] = c->level_start[2 * k];
+      lv->start[2 * r + 1] = c->level_start[2 * k + 1];
+    }
+    if (lv->iterations) lv->iterations[r] = its[k];
+  }
+  return MSG_OK;
+}
+
+template <typename S>
+int solve_impl(msg_ctx *c, int start, int stop, const msg_solve_params *p, double *energies,
+               const LevelOut *out = nullptr) {
   std::vector<double> e(c->B, NAN);
+  const bool rec = out && out->lv;
+  const int cap = rec && out->lv->iters ? p->max_iter : 0;
+  std::vector<msg_iter_trace> tr((size_t)c->B * std::max(cap, 0));
+  std::vector<int> its(c->B, 0);
   for (int level = start; level <= stop; ++level) {
     const double f = p->lambda_factor > 0 ? (double)std::pow(p->lambda_factor, p->lambda_ref_level - level) : 1;
     const double lam = f * p->lambda;
-    CHK(solve_level_any<S>(c, level, lam, p->max_iter, p->stop_eps, p->spec, e.data(), nullptr, nullptr, 0));
+    CHK(solve_level_any<S>(c, level, lam, p->max_iter, p->stop_eps, p->spec, e.data(), its.data(),
+                           cap > 0 ? tr.data() : nullptr, cap));
+    if (rec) CHK(record_level<S>(c, level, start, stop, cap, *out, tr, its));
     if (level < stop) CHK(prolong_deformation<S>(c, level, p->resample_prolong));
   }
   for (int l = stop; l < c->L; ++l) CHK(prolong_deformation<S>(c, l, p->resample_prolong));
@@ -1299,7 +1363,7 @@ struct Progress {
 // (matchSeries.h:136-176). Writes the fields to pw and the energies.
 template <typename S>
 int pairwise_batch(const msg_series_params *p, const msg_solve_params &sp, const S *tpl, const S *ref, int i0,
-                   int i1, S *pw, double *energies) {
+                   int i1, S *pw, double *energies, const msg_series_levels *lv) {
   const int L = p->precision_level;
   const int N = 1 << L;
   const size_t nn = (size_t)N * N;
@@ -1326,7 +1390,11 @@ int pairwise_batch(const msg_series_params *p, const msg_solve_params &sp, const
   }
   CHK(rebuild_pyramid<S>(pc, pc->R, -1, 1));
   CHK(rebuild_pyramid<S>(pc, pc->T, -1, 1));
-  CHK(solve_impl<S>(pc, p->start_level, p->stop_level, &sp, energies + i0));
+  LevelOut lo;
+  lo.lv = lv;
+  lo.rec0 = (long long)i0 * (p->stop_level - p->start_level + 1);
+  lo.field0 = (size_t)i0 * level_block(p->start_level, p->stop_level);
+  CHK(solve_impl<S>(pc, p->start_level, p->stop_level, &sp, energies + i0, &lo));
   HIPCHK(hipMemcpyAsync(pw + (size_t)i0 * 2 * nn, pc->phi[L], sizeof(S) * 2 * nn * nb, hipMemcpyDeviceToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
   return MSG_OK;
@@ -1335,14 +1403,14 @@ int pairwise_batch(const msg_series_params *p, const msg_solve_params &sp, const
 // Runs the batches [k*b, min((k+1)*b, npw)) in order, publishing progress after each.
 template <typename S>
 void pairwise_all(const msg_series_params *p, const msg_solve_params &sp, const S *tpl, const S *ref, int npw,
-                  int b, S *pw, double *energies, Progress *pr) {
+                  int b, S *pw, double *energies, const msg_series_levels *lv, Progress *pr) {
   int rc = MSG_OK;
   for (int i0 = 0; i0 < npw && rc == MSG_OK; i0 += b) {
     {
       std::lock_guard<std::mutex> g(pr->mu);
       if (pr->abort) break;
     }
-    rc = pairwise_batch<S>(p, sp, tpl, ref, i0, std::min(i0 + b, npw), pw, energies);
+    rc = pairwise_batch<S>(p, sp, tpl, ref, i0, std::min(i0 + b, npw), pw, energies, lv);
     std::lock_guard<std::mutex> g(pr->mu);
     if (rc == MSG_OK) pr->done = std::min(i0 + b, npw);
     else { pr->rc = rc; pr->err = g_err; }
@@ -1356,7 +1424,7 @@ void pairwise_all(const msg_series_params *p, const msg_solve_params &sp, const 
 template <typename S>
 int match_series_impl(const msg_series_params *p, const void *templates, int n, const void *reference,
                       const void *pairwise_in, void *deformations, void *pairwise_out, double *energies, void *mean,
-                      void *count, void *median) {
+                      void *count, void *median, const msg_series_levels *lv) {
   const int L = p->precision_level;
   const int N = 1 << L;
   const size_t nn = (size_t)N * N;
@@ -1407,10 +1475,10 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
         pr.cv.notify_all();
         return;
       }
-      pairwise_all<S>(p, sp, tpl, ref, npw, b, pwp, energies, &pr);
+      pairwise_all<S>(p, sp, tpl, ref, npw, b, pwp, energies, lv, &pr);
     });
   } else {
-    pairwise_all<S>(p, sp, tpl, ref, npw, b, pwp, energies, &pr);
+    pairwise_all<S>(p, sp, tpl, ref, npw, b, pwp, energies, lv, &pr);
     if (pr.rc != MSG_OK) return fail(pr.rc, pr.err);
   }
   // Waits (on the calling thread) until pairwise solve i is done; false after a batch failed.
@@ -1460,7 +1528,13 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
           HIPCHK(hipMemcpyAsync(rc->phi[L], acc.p, sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, rs));
           CHK(rebuild_pyramid<S>(rc, rc->phi, -1, 2));
           double e = 0;
-          CHK(solve_impl<S>(rc, p->refine_start_level, p->refine_stop_level, &sp, &e));
+          LevelOut lo;
+          lo.lv = lv;
+          lo.rec0 = (long long)npw * (p->stop_level - p->start_level + 1) +
+                    (long long)(i - 1) * (p->refine_stop_level - p->refine_start_level + 1);
+          lo.field0 = (size_t)npw * level_block(p->start_level, p->stop_level) +
+                      (size_t)(i - 1) * level_block(p->refine_start_level, p->refine_stop_level);
+          CHK(solve_impl<S>(rc, p->refine_start_level, p->refine_stop_level, &sp, &e, &lo));
           energies[i] = e;
           HIPCHK(hipMemcpyAsync(acc.p, rc->phi[L], sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, rs));
         }
@@ -1529,7 +1603,7 @@ int msg_normalize(int dtype, void *img, int n, int nimg, double pct, int min_to_
 
 int msg_match_series(const msg_series_params *p, int dtype, const void *templates, int n, const void *reference,
                      const void *pairwise_in, void *deformations, void *pairwise_out, double *energies, void *mean,
-                     void *count, void *median) {
+                     void *count, void *median, const msg_series_levels *levels) {
   if (!p || !templates || !reference || !deformations || !energies || !mean || !count || !median || n < 1)
     return fail(MSG_ERR_ARG, "invalid argument");
   CHK(check_dtype(dtype));
@@ -1542,9 +1616,9 @@ int msg_match_series(const msg_series_params *p, int dtype, const void *template
     return fail(MSG_ERR_ARG, "invalid refineStartLevel/refineStopLevel");
   if (dtype == MSG_F64)
     return match_series_impl<double>(p, templates, n, reference, pairwise_in, deformations, pairwise_out, energies,
-                                     mean, count, median);
+                                     mean, count, median, levels);
   return match_series_impl<float>(p, templates, n, reference, pairwise_in, deformations, pairwise_out, energies, mean,
-                                  count, median);
+                                  count, median, levels);
 }
 
 }  // extern "C"

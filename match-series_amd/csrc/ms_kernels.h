@@ -234,8 +234,8 @@ __device__ void control_pair(int pair, PairState *__restrict__ st, const double 
         s.cand_R[k] = sums[2 * k + 1];
       }
       if (a.phase == PH_LEVEL_INIT) {
-        s.energy = s.energy_new = s.cand_E[0];
-        s.ereg_new = s.cand_R[0];
+        s.energy = s.energy_new = s.energy_start = s.cand_E[0];
+        s.ereg_new = s.ereg_start = s.cand_R[0];
         s.tau = 1.0;
         s.sigma = 1.0;
         s.iter = 0;

@@ -20,6 +20,7 @@ struct PairState {
   double Df, Dfz;     // <DE, d>, <DE, z>
   double lambda;      // regularisation weight of the level
   double stop_eps;
+  double energy_start, ereg_start;  // E and E_reg (unweighted) at the start of the level
   // Armijo machine (getTimestepWidthWithArmijoLineSearch)
   double a_tau;       // tau of the sequential algorithm
   double a_Df;        // slope of the active direction

@@ -207,25 +207,60 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
     }
   }
   std::vector<double> energies(n);
+  // per-level records: every level of every solve, as the reference writes them
+  const bool accumulate = !p.checkAndGetBool("dontAccumulateDeformation");
+  const int npw = pin ? 1 : n;
+  const int nlp = sp.stop_level - sp.start_level + 1, nlr = sp.refine_stop_level - sp.refine_start_level + 1;
+  const int nrec = npw * nlp + (accumulate ? (n - 1) * nlr : 0);
+  size_t nfld = 0;
+  for (int l = sp.start_level; l <= sp.stop_level; ++l) nfld += (size_t)npw * (2u << (2 * l));
+  if (accumulate)
+    for (int l = sp.refine_start_level; l <= sp.refine_stop_level; ++l) nfld += (size_t)(n - 1) * (2u << (2 * l));
+  const int maxIt = std::max(sp.max_iter, 0);
+  std::vector<double> lvFields(nfld), lvStart(2 * nrec);
+  std::vector<msg_iter_trace> lvIters((size_t)std::max(nrec * maxIt, 1));
+  std::vector<int> lvIts(nrec);
+  msg_series_levels lv = {lvFields.data(), lvIters.data(), lvStart.data(), lvIts.data()};
   const auto t0 = std::chrono::steady_clock::now();
   msgCheck(msg_match_series(&sp, dt, tpl.p, n, ref.p, pin ? pin->p : nullptr, defs.p, pw.p, energies.data(), mean.p,
-                            cnt.p, med.p));
+                            cnt.p, med.p, &lv));
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::cerr << "Registered " << n << " templates on the GPU in " << secs << " s\n";
 
   std::vector<double> hdefs(2 * nn * n), hpw(2 * nn * n);
   download(hdefs.data(), defs.p, 2 * nn * n, dt);
   download(hpw.data(), pw.p, 2 * nn * n, dt);
+  // Writes the level records r0 .. r0+nl-1 of one solve into `sub` (deformation_<LL>_<c>.dat.bz2 and
+  // energy_<LL>.txt as GradientDescentBase::writeEnergy prints them: "k lambda*E_reg E", k from -1).
+  size_t fldOff = 0;
+  int recNext = 0;
+  auto writeLevels = [&](const std::string &sub, int l0, int l1) {
+    for (int l = l0; l <= l1; ++l, ++recNext) {
+      const int r = recNext, m = 1 << l;
+      saveField(sub + strprintf("deformation_%02d_%%d.dat.bz2", l), lvFields.data() + fldOff, m);
+      fldOff += (size_t)2 * m * m;
+      const double lam = sp.lambda_factor > 0 ? sp.lambda * std::pow(sp.lambda_factor, sp.stop_level - l) : sp.lambda;
+      std::ofstream e(sub + strprintf("energy_%02d.txt", l));
+      e << -1 << " " << lvStart[2 * r + 1] << " " << lvStart[2 * r] << "\n";
+      for (int k = 0; k < lvIts[r] && k < maxIt; ++k) {
+        const msg_iter_trace &t = lvIters[(size_t)r * maxIt + k];
+        e << k << " " << lam * t.ereg << " " << t.energy << "\n";
+      }
+    }
+  };
   std::ofstream defNorms, energiesFile(dir + "/energies.txt");
   if (!loadStageOneResults) defNorms.open(dir + "/defNorms.txt");
-  const bool accumulate = !p.checkAndGetBool("dontAccumulateDeformation");
   const int refineStop = sp.refine_stop_level;
+  for (int i = 0; i < npw; ++i) {  // pairwise records come first
+    const std::string sub = strprintf("%s%d/", dir.c_str(), i);
+    makeDirectory(sub);
+    writeLevels(sub, sp.start_level, sp.stop_level);
+  }
   for (int i = 0; i < n; ++i) {
     const double *phw = hpw.data() + (size_t)i * 2 * nn;
     if (!loadStageOneResults || i == 0) {
       const std::string sub = strprintf("%s%d/", dir.c_str(), i);
-      makeDirectory(sub);
-      saveField(sub + strprintf("deformation_%02d_%%d.dat.bz2", L), phw, N);
+      if (sp.stop_level < L) saveField(sub + strprintf("deformation_%02d_%%d.dat.bz2", L), phw, N);
       if (!loadStageOneResults) {
         double norm = 0;
         for (int c = 0; c < 2; ++c) {
@@ -240,8 +275,8 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
       const std::string sub = strprintf("%s%d-r/", dir.c_str(), i);
       makeDirectory(sub);
       const double *phd = hdefs.data() + (size_t)i * 2 * nn;
-      saveField(sub + strprintf("deformation_%02d_%%d.dat.bz2", L), phd, N);
-      if (refineStop < L) saveField(sub + strprintf("deformation_%02d_%%d.dat.bz2", refineStop), phd, N);
+      writeLevels(sub, sp.refine_start_level, refineStop);
+      if (refineStop < L) saveField(sub + strprintf("deformation_%02d_%%d.dat.bz2", L), phd, N);
     }
     energiesFile << i << " " << energies[i] << "\n";
   }

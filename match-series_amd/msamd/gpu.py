@@ -43,6 +43,11 @@ class SeriesParams(ctypes.Structure):
     ]
 
 
+class SeriesLevels(ctypes.Structure):
+    _fields_ = [("fields", _dp), ("iters", ctypes.POINTER(IterTrace)), ("start", _dp),
+                ("iterations", ctypes.POINTER(ctypes.c_int))]
+
+
 # exported symbols with their argument types (also the list the C-ABI test checks)
 SIGNATURES = {
     "msg_last_error": ([], ctypes.c_char_p),
@@ -82,7 +87,7 @@ SIGNATURES = {
     "msg_profile_reset": ([], ctypes.c_int),
     "msg_profile_read": ([ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong), _dp], ctypes.c_int),
     "msg_match_series": ([ctypes.POINTER(SeriesParams), ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _dp, _vp,
-                          _vp, _vp], ctypes.c_int),
+                          _vp, _vp, ctypes.POINTER(SeriesLevels)], ctypes.c_int),
 }
 
 _lib = None
@@ -322,10 +327,31 @@ class Context:
         return a.value, b.value, c.value
 
 
-def match_series(templates, reference, params, pairwise_in=None, want_pairwise=False):
-    """One stage of series matching on device tensors. templates: (n, N, N); reference: (N, N)."""
+def series_level_layout(params, n, reuse=False):
+    """(solve kind, frame index, level) of every per-level record of match_series, in order."""
+    npw = 1 if reuse else n
+    recs = [("pairwise", i, l) for i in range(npw) for l in range(params.start_level, params.stop_level + 1)]
+    if not params.dont_accumulate:
+        recs += [("refine", i, l) for i in range(1, n)
+                 for l in range(params.refine_start_level, params.refine_stop_level + 1)]
+    return recs
+
+
+def match_series(templates, reference, params, pairwise_in=None, want_pairwise=False, want_levels=False):
+    """One stage of series matching on device tensors. templates: (n, N, N); reference: (N, N).
+    want_levels: also return the per-level deformations and iteration records of every solve."""
     torch = _torch()
     n, N = templates.shape[0], templates.shape[-1]
+    lv = None
+    if want_levels:
+        recs = series_level_layout(params, n, pairwise_in is not None)
+        nf = sum(2 << (2 * l) for _, _, l in recs)
+        mi = max(params.max_iter, 0)
+        fields = np.zeros(nf)
+        iters = (IterTrace * max(len(recs) * mi, 1))()
+        start = np.zeros(2 * len(recs))
+        its = np.zeros(len(recs), dtype=np.int32)
+        lv = SeriesLevels(_dptr(fields), iters, _dptr(start), its.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
     defs = torch.zeros((n, 2, N, N), dtype=templates.dtype, device=templates.device)
     pw = torch.zeros_like(defs) if want_pairwise else None
     en = np.zeros(n)
@@ -335,8 +361,20 @@ def match_series(templates, reference, params, pairwise_in=None, want_pairwise=F
     _ready()
     check(lib().msg_match_series(ctypes.byref(params), dtype_code(templates), _ptr(templates), n, _ptr(reference),
                                  _ptr(pairwise_in) if pairwise_in is not None else None, _ptr(defs),
-                                 _ptr(pw) if pw is not None else None, _dptr(en), _ptr(mean), _ptr(cnt), _ptr(med)))
-    return dict(deformations=defs, pairwise=pw, energies=en, mean=mean, count=cnt, median=med)
+                                 _ptr(pw) if pw is not None else None, _dptr(en), _ptr(mean), _ptr(cnt), _ptr(med),
+                                 ctypes.byref(lv) if lv is not None else None))
+    out = dict(deformations=defs, pairwise=pw, energies=en, mean=mean, count=cnt, median=med)
+    if lv is not None:
+        levels, off = [], 0
+        tr = np.ctypeslib.as_array(iters).view(np.float64).reshape(-1, 8) if mi > 0 else np.zeros((0, 8))
+        for r, (kind, i, l) in enumerate(recs):
+            m = 1 << l
+            levels.append(dict(kind=kind, frame=i, level=l, field=fields[off:off + 2 * m * m].reshape(2, m, m),
+                               start=(start[2 * r], start[2 * r + 1]), iterations=int(its[r]),
+                               trace=tr[r * mi:r * mi + int(its[r])]))
+            off += 2 * m * m
+        out["levels"] = levels
+    return out
 
 
 # ------------------------------------------------------------------------------------------------

@@ -7,7 +7,7 @@ Runs in the build container only (it needs the reference build of oracle/_ref/, 
     complete level solve with its step-size trace);
   * oracle/_ref/matchSeries: the reference program itself on a seeded 4-frame 64x64 STEM series with
     two stages, reading frames written here in its P9 format and writing its usual output tree.
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [--only name,name]
 """
 import bz2
 import os
@@ -172,10 +172,25 @@ def main():
                 e2e[f"s{st_}_{nm}"] = read_p9(os.path.join(od, f"stage{st_}", f"{nm}.q2bz"))
             en = np.loadtxt(os.path.join(od, f"stage{st_}", "energies.txt"))
             e2e[f"s{st_}_energies"] = en[:, 1]
+            # per-level records: deformation_<LL> and energy_<LL>.txt of every solve directory
+            for sub in sorted(os.listdir(os.path.join(od, f"stage{st_}"))):
+                sd = os.path.join(od, f"stage{st_}", sub)
+                if not os.path.isdir(sd):
+                    continue
+                for fn in sorted(os.listdir(sd)):
+                    key = f"s{st_}_{sub.replace('-', '_')}_"
+                    if fn.startswith("energy_"):
+                        e2e[key + fn[:-4]] = np.loadtxt(os.path.join(sd, fn)).reshape(-1, 3)
+                    elif fn.startswith("deformation_") and fn.endswith("_0.dat.bz2"):
+                        lv = fn[len("deformation_"):len("deformation_") + 2]
+                        e2e[key + f"deformation_{lv}"] = np.stack(
+                            [read_p9(os.path.join(sd, f"deformation_{lv}_{c}.dat.bz2")) for c in range(2)])
         e2e["par"] = np.frombuffer(par.encode(), dtype=np.uint8)
         out["e2e64"] = e2e
+    only = sys.argv[sys.argv.index("--only") + 1].split(",") if "--only" in sys.argv else None
     for key, d in out.items():
-        np.savez_compressed(os.path.join(HERE, f"{key}.npz"), **d)
+        if only is None or key in only:
+            np.savez_compressed(os.path.join(HERE, f"{key}.npz"), **d)
     print("wrote", len(out), "fixtures")
 
 

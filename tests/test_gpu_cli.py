@@ -63,5 +63,20 @@ def test_matchseries_two_stages(tmp_path, precision, tol_s1, tol_s2):
         assert np.abs(med - g[f"s{st}_median"])[ok].max() < (1e-6 if st == 1 and precision == 64 else 1e-2)
         en = np.loadtxt(str(out / f"stage{st}" / "energies.txt"))[:, 1]
         np.testing.assert_allclose(en, g[f"s{st}_energies"], rtol=1e-4 if precision == 64 else 1e-3)
+        # per-level records of every solve: deformation_<LL> and energy_<LL>.txt ("k lambda*E_reg E")
+        subs = [k for k in g if k.startswith(f"s{st}_") and "_energy_" in k]
+        assert len(subs) == (9 + 4 if st == 1 else 3 + 6)
+        for k in subs:
+            sub, lv = k[len(f"s{st}_"):].split("_energy_")
+            sd = out / f"stage{st}" / sub.replace("_", "-")
+            mine = np.loadtxt(str(sd / f"energy_{lv}.txt")).reshape(-1, 3)
+            ref = g[k]
+            np.testing.assert_array_equal(mine[:, 0], ref[:, 0])
+            rt = (2e-5 if st == 1 else 1e-3) if precision == 64 else 2e-3
+            np.testing.assert_allclose(mine[:, 2], ref[:, 2], rtol=rt, err_msg=k)
+            np.testing.assert_allclose(mine[:, 1], ref[:, 1], rtol=10 * rt, atol=1e-9, err_msg=k)
+            m = 1 << int(lv)
+            d = np.stack([read_p9(str(sd / f"deformation_{lv}_{c}.dat.bz2")) for c in range(2)])
+            assert np.abs(d - g[k.replace("_energy_", "_deformation_")]).max() * (m - 1) < tol, k
     assert (out / "log.txt").exists() and (out / "stage1" / "parameter-dump.txt").exists()
     assert (out / "stage1" / "defNorms.txt").exists()
