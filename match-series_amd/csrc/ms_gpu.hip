@@ -148,6 +148,13 @@ struct ProfScope {
   }
 };
 
+// Adds algorithmic bytes to a class whose launches cannot know them (device-decided work).
+void prof_add_bytes(int cls, double bytes) {
+  if (!((g_prof.mask >> cls) & 1u)) return;
+  std::lock_guard<std::mutex> g(g_prof.mu);
+  g_prof.bytes[cls] += bytes;
+}
+
 void prof_resolve() {
   for (auto &r : g_prof.pending) {
     (void)hipEventSynchronize(r.b);
@@ -480,8 +487,7 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
 
   hipLaunchKernelGGL(k_state_init, dim3(cdiv(B, 64)), dim3(64), 0, s, c->st, B, lambda);
   HIPCHK(hipGetLastError());
-  const double field_bytes = (double)n * n * sizeof(S);
-  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, 4.0 * field_bytes * B));
+  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s));
   a.phase = msg::PH_LEVEL_INIT;
   hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
   HIPCHK(hipGetLastError());
@@ -504,7 +510,7 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
     int rounds = 0;
     int nsearch = nactive;  // pairs evaluating in this round (upper bound for the first round)
     while (searching) {
-      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, 6.0 * field_bytes * nsearch));
+      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s));
       a.phase = msg::PH_ARMIJO_STEP;
       hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d,
                          tr);
@@ -538,6 +544,11 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
     c->cnt_evals += hs[p].total_evals;
     c->cnt_passes += hs[p].total_passes;
   }
+  // algorithmic bytes of the level's energy passes: the first reads R, T and phi, every later one
+  // also the direction (the device counts the passes)
+  double ebytes = 0;
+  for (int p = 0; p < B; ++p) ebytes += (4.0 + 6.0 * hs[p].total_passes) * (double)n * n * sizeof(S);
+  prof_add_bytes(KC_ENERGY, ebytes);
   c->level_start.resize(2 * B);
   for (int p = 0; p < B; ++p) {
     c->level_start[2 * p] = hs[p].energy_start;
@@ -574,7 +585,8 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
   if (spec <= 0) spec = 3;
   if (spec > msg::MAX_CAND) spec = msg::MAX_CAND;
   const int Rr = 2, K = 8;
-  const bool use_graph = (c->mode == 2);
+  // event timing cannot bracket launches inside a graph: a profiled energy class runs uncaptured
+  const bool use_graph = (c->mode == 2) && !((g_prof.mask >> KC_ENERGY) & 1u);
 
   msg::IterTrace *tr = nullptr;
   int tcap = 0;
@@ -617,8 +629,7 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
 
   hipLaunchKernelGGL(k_state_init, dim3(cdiv(B, 64)), dim3(64), 0, s, c->st, B, lambda);
   HIPCHK(hipGetLastError());
-  const double field_bytes = (double)n * n * sizeof(S);
-  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, 4.0 * field_bytes * B, t_level));
+  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, 0.0, t_level));
 
   auto iteration = [&]() -> int {
     CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s));
@@ -688,6 +699,11 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
     c->cnt_evals += hs[p].total_evals;
     c->cnt_passes += hs[p].total_passes;
   }
+  // algorithmic bytes of the level's energy passes: the first reads R, T and phi, every later one
+  // also the direction (the device counts the passes)
+  double ebytes = 0;
+  for (int p = 0; p < B; ++p) ebytes += (4.0 + 6.0 * hs[p].total_passes) * (double)n * n * sizeof(S);
+  prof_add_bytes(KC_ENERGY, ebytes);
   c->level_start.resize(2 * B);
   for (int p = 0; p < B; ++p) {
     c->level_start[2 * p] = hs[p].energy_start;
