@@ -35,8 +35,9 @@ def series_params(gpu, levels=(7, 10), refine=(9, 10), its=100, lam=10.0, spec=0
                             dont_normalize=0, resample_prolong=1, dont_accumulate=0, spec=spec)
 
 
-def cpu_baseline(N, its_sample, schedule, frames):
-    """Reference level solve at N x N for its_sample iterations on one host core, extrapolated."""
+def cpu_baseline(N, its_sample, schedule, frames, cores=1):
+    """Reference level solve at N x N for its_sample iterations on one host core, extrapolated; with
+    cores > 1 also the throughput with that many concurrent copies."""
     from msamd import synth
     harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
     kind = "reference"
@@ -57,16 +58,35 @@ def cpu_baseline(N, its_sample, schedule, frames):
             t0 = time.perf_counter()
             O.level_solve(R, T, np.zeros((2, N, N)), 10.0, its_sample, -1.0)
             t = time.perf_counter() - t0
+        # the same sample on every host core at once (independent processes): the pairwise solves
+        # of a series are independent, the accumulate-and-refine chain is not
+        par = None
+        if os.path.exists(harness) and cores > 1:
+            t0 = time.perf_counter()
+            procs = [subprocess.Popen([harness, "solve", str(N), "10", str(its_sample), "-1", fin, fout + f".{k}"],
+                                      stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL) for k in range(cores)]
+            rcs = [q.wait() for q in procs]
+            if all(rc == 0 for rc in rcs):
+                par = time.perf_counter() - t0
     t_iter = t / its_sample  # includes the initial energy and setup: slightly pessimistic for the CPU
     L = int(np.log2(N))
     (l0, l1), (r0, r1), its = schedule
     per_pair = sum(its * t_iter * 4.0 ** (l - L) for l in range(l0, l1 + 1))
     per_refine = sum(its * t_iter * 4.0 ** (l - L) for l in range(r0, r1 + 1))
     total = (frames - 1) * per_pair + (frames - 2) * per_refine
-    return dict(value=frames / total, unit="frames/s", cores=1, kind=kind,
-                sample=f"{its_sample} gradient-flow iterations of one {N}x{N} pair (level {L}) in "
-                       f"{'oracle/_ref/ref_harness solve' if kind == 'reference' else 'the oracle'} "
-                       f"= {t_iter:.3f} s/iteration, extrapolated to the step's schedule with cost ~ N^2")
+    what = "oracle/_ref/ref_harness solve" if kind == "reference" else "the oracle"
+    one = dict(value=frames / total, unit="frames/s", cores=1, kind=kind,
+               sample=f"{its_sample} gradient-flow iterations of one {N}x{N} pair (level {L}) in {what} "
+                      f"= {t_iter:.3f} s/iteration, extrapolated to the step's schedule with cost ~ N^2")
+    allc = None
+    if par is not None:
+        t_par = par / its_sample  # wall time per iteration with every core busy
+        per_pair_p = sum(its * t_par * 4.0 ** (l - L) for l in range(l0, l1 + 1))
+        total_p = (frames - 1) * per_pair_p / cores + (frames - 2) * per_refine
+        allc = dict(value=frames / total_p, unit="frames/s", cores=cores, kind=kind,
+                    sample=f"{cores} concurrent copies of the same sample = {t_par:.3f} s/iteration each; pairwise "
+                           f"solves spread over {cores} cores, the chain on one")
+    return one, allc
 
 
 def main():
@@ -147,9 +167,10 @@ def main():
     roofline["copy_ceiling_gbs"] = round(10 * 2 * a.numel() * 4 / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
     del a, b
 
-    cpu = None
+    cpu = cpu_all = None
     if not args.no_cpu:
-        cpu = cpu_baseline(N, args.cpu_iters, (levels, refine, args.iters), F)
+        ncores = min(16, len(os.sched_getaffinity(0)))
+        cpu, cpu_all = cpu_baseline(N, args.cpu_iters, (levels, refine, args.iters), F, ncores)
 
     finite = bool(torch.isfinite(out["median"]).all().item()) if out is not None else False
     line = {
@@ -163,6 +184,7 @@ def main():
                    "iteration_mode": args.mode, "armijo_candidates": args.spec or 3},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
         "energies_final_mean": float(np.mean(out["energies"])) if out is not None else None,
         "median_finite": finite,
     }
