@@ -321,6 +321,171 @@ void msor_stiff_apply(const double *u, int N, double *out) {
     }
 }
 
+/* Assembled Q1 mass matrix applied to u (MassOp<ConfType>(grid, ASSEMBLED), FEOpInterface.h:2382-2397).
+ * Local matrix of FELinScalarWeightedMassInterface::prepareLocalMatrix (FEOpInterface.h:1446-1475):
+ * sum over the quadrature points of (b_i b_j * 1) * w_q, then times the element volume h^2. Rows are
+ * assembled element by element and applied in column order, as for the stiffness matrix. */
+void msor_mass_apply(const double *u, int N, double *out) {
+  initBQ();
+  const double h = gridH(N), vol = h * h;
+  double ML[4][4] = {{0}};
+  for (int q = 0; q < 4; ++q)
+    for (int i = 0; i < 4; ++i) {
+      const double bi = BQ[i][q];
+      for (int j = i; j < 4; ++j) ML[i][j] += bi * BQ[j][q] * 1. * QW;
+    }
+  for (int i = 0; i < 4; ++i)
+    for (int j = i; j < 4; ++j) ML[j][i] = ML[i][j] = ML[i][j] * vol;
+  for (int y = 0; y < N; ++y)
+    for (int x = 0; x < N; ++x) {
+      double c[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int ey = y - 1; ey <= y; ++ey)
+        for (int ex = x - 1; ex <= x; ++ex) {
+          if (ex < 0 || ey < 0 || ex >= N - 1 || ey >= N - 1) continue;
+          const int bi = (x - ex) + 2 * (y - ey);
+          for (int bj = 0; bj < 4; ++bj) {
+            const int jx = ex + (bj & 1), jy = ey + ((bj & 2) >> 1);
+            c[jy - y + 1][jx - x + 1] += ML[bi][bj];
+          }
+        }
+      double s = 0;
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int jx = x + dx, jy = y + dy;
+          if (jx < 0 || jy < 0 || jx >= N || jy >= N) continue;
+          s += c[dy + 1][dx + 1] * u[IDX(jx, jy, N)];
+        }
+      out[IDX(x, y, N)] = s;
+    }
+}
+
+/* NormalizedCrossCorrelationEnergy::normalizeImageForNCC (registration.h:1934-1947): subtract the
+ * mass-weighted mean, divide by sqrt(u.Mu) unless that is within 1e-15 of zero (appeqAbsolute,
+ * aol.h:779-794). Returns sqrt(u.Mu). */
+double msor_ncc_normalize(const double *img, int N, double *out) {
+  const int NN = N * N;
+  double *tmp = (double *)malloc(sizeof(double) * NN);
+  msor_mass_apply(img, N, tmp);
+  double mean = 0;
+  for (int i = 0; i < NN; ++i) mean += tmp[i];                 /* Vector::sum, vec.cpp:847-870 */
+  for (int i = 0; i < NN; ++i) out[i] = img[i];
+  for (int i = 0; i < NN; ++i) out[i] += -mean;                /* addToAll, vec.h:633-639 */
+  msor_mass_apply(out, N, tmp);
+  double dot = 0;
+  for (int i = 0; i < NN; ++i) dot += out[i] * tmp[i];         /* ScalarProduct, vec.cpp:33-40 */
+  const double var = sqrt(dot);
+  if (!(fabs(var - 0.) < 1e-15))
+    for (int i = 0; i < NN; ++i) out[i] /= var;                /* vec.cpp:729-734 */
+  free(tmp);
+  return var;
+}
+
+/* NormalizedCrossCorrelationEnergy::computeEnergy (registration.h:1953-1964) with the reference
+ * already normalised: -(M Tn) . Rn, Tn the normalised FastDeformImage of T. Writes Tn and returns
+ * the energy; *var_t receives the variance of the deformed template. */
+static double nccEnergyNormalized(const double *Rn, const double *T, const double *phi, int N, double *Tn,
+                                  double *var_t) {
+  const int NN = N * N;
+  double *def = (double *)malloc(sizeof(double) * NN);
+  msor_fast_deform_image(T, phi, N, def);
+  *var_t = msor_ncc_normalize(def, N, Tn);
+  msor_mass_apply(Tn, N, def);
+  double dot = 0;
+  for (int i = 0; i < NN; ++i) dot += def[i] * Rn[i];
+  free(def);
+  return -dot;
+}
+
+double msor_ncc_energy(const double *R, const double *T, const double *phi, int N) {
+  const int NN = N * N;
+  double *Rn = (double *)malloc(sizeof(double) * NN), *Tn = (double *)malloc(sizeof(double) * NN);
+  double var_t;
+  msor_ncc_normalize(R, N, Rn);
+  const double e = nccEnergyNormalized(Rn, T, phi, N, Tn, &var_t);
+  free(Rn);
+  free(Tn);
+  return e;
+}
+
+/* transformCoord<ConfType, false> (deformations.h:623-668, 681-693): grid coordinates of the quad
+ * point plus offset/h; false when the point leaves the grid on the first axis tested. */
+static int transformNoClip(int ex, int ey, int q, double offx, double offy, int N, int *tel, double *tloc) {
+  const double h = gridH(N);
+  double coord[2];
+  coord[0] = (double)ex + QP[q / 2];
+  coord[1] = (double)ey + QP[q % 2];
+  const double off[2] = {offx, offy};
+  for (int i = 0; i < 2; ++i) {
+    const double tc = coord[i] + off[i] / h;
+    int el = (short)tc;
+    double loc = tc - el;
+    if (el == N - 1 && loc == 0) {
+      el = N - 2;
+      loc = 1;
+    }
+    tel[i] = el;
+    tloc[i] = loc;
+    if (tc < 0. || el < 0 || el >= N - 1) return 0;
+  }
+  return 1;
+}
+
+/* NormalizedCrossCorrelationEnergy::applyDerivative (registration.h:1974-1984): w = (Rn + E Tn)/var_T
+ * (no division when var_T is within 1e-15 of zero), then CrossCorrelationForce(w, T)
+ * (registration.h:1826-1862): -w(x_q) grad T(x_q + phi(x_q)) at every quad point that stays in
+ * the grid, zero elsewhere, assembled by FENonlinVectorOpInterface::applyAdd (FEOpInterface.h:1803-1853). */
+static void nccForceNormalized(const double *Rn, const double *T, const double *phi, int N, double *out) {
+  initBQ();
+  const int NN = N * N;
+  const double h = gridH(N), vol = h * h;
+  double *Tn = (double *)malloc(sizeof(double) * NN), *w = (double *)malloc(sizeof(double) * NN);
+  double var_t;
+  const double e = nccEnergyNormalized(Rn, T, phi, N, Tn, &var_t);
+  for (int i = 0; i < NN; ++i) w[i] = Rn[i];
+  for (int i = 0; i < NN; ++i) w[i] += e * Tn[i];              /* addMultiple, vec.h */
+  if (!(fabs(var_t - 0.) < 1e-15))
+    for (int i = 0; i < NN; ++i) w[i] /= var_t;
+  memset(out, 0, sizeof(double) * 2 * NN);
+  for (int ey = 0; ey < N - 1; ++ey)
+    for (int ex = 0; ex < N - 1; ++ex) {
+      double nl[4][2];
+      for (int q = 0; q < 4; ++q) {
+        const double ox = evalAtQP(phi, ex, ey, q, N), oy = evalAtQP(phi + NN, ex, ey, q, N);
+        int tel[2];
+        double tloc[2];
+        if (!transformNoClip(ex, ey, q, ox, oy, N, tel, tloc)) {
+          nl[q][0] = nl[q][1] = 0;
+          continue;
+        }
+        double g[2];
+        evalGradAt(T, tel, tloc, N, g);
+        const double wq = -evalAtQP(w, ex, ey, q, N);
+        nl[q][0] = g[0] * wq;
+        nl[q][1] = g[1] * wq;
+      }
+      for (int dof = 0; dof < 4; ++dof) {
+        double a0 = 0, a1 = 0;
+        for (int q = 0; q < 4; ++q) {
+          const double f = BQ[dof][q] * QW;
+          a0 += f * nl[q][0];
+          a1 += f * nl[q][1];
+        }
+        const int gi = l2g(ex, ey, dof, N);
+        out[gi] += vol * a0;
+        out[NN + gi] += vol * a1;
+      }
+    }
+  free(Tn);
+  free(w);
+}
+
+void msor_ncc_force(const double *R, const double *T, const double *phi, int N, double *out) {
+  double *Rn = (double *)malloc(sizeof(double) * N * N);
+  msor_ncc_normalize(R, N, Rn);
+  nccForceNormalized(Rn, T, phi, N, out);
+  free(Rn);
+}
+
 /* DisplacementLengthEnergy::apply (hyperelastic.h:178-183): 1/2 sum_c phi_c . K phi_c, the
  * MultiVector dot product summed component by component (multiVector.h:570-578, vec.cpp:33-40). */
 double msor_reg_energy(const double *phi, int N) {
@@ -458,21 +623,32 @@ typedef struct {
   double lambda;
   double *tmp; /* 2*N*N scratch for positions */
   int evals;
+  int data_term;
+  double *Rn, *Tn; /* NCC: the normalised reference (set once per level, as the reference's
+                      energy object does in its constructor) and a scratch image */
 } Energy;
 
-/* E = (0 + E_SSD) + lambda * E_reg (LinCombOp::applyAdd, op.h:255-272) */
+/* E = (0 + E_data) + lambda * E_reg (LinCombOp::applyAdd, op.h:255-272) */
 static double energyAt(Energy *e, const double *phi) {
   double E = 0;
-  E += msor_ssd_energy(e->R, e->T, phi, e->N);
+  if (e->data_term == MSOR_NCC) {
+    double var_t;
+    E += nccEnergyNormalized(e->Rn, e->T, phi, e->N, e->Tn, &var_t);
+  } else {
+    E += msor_ssd_energy(e->R, e->T, phi, e->N);
+  }
   E += e->lambda * msor_reg_energy(phi, e->N);
   return E;
 }
 
-/* DE = (0 + F_SSD) + lambda * K phi, componentwise */
+/* DE = (0 + DE_data) + lambda * K phi, componentwise */
 static void gradientAt(Energy *e, const double *phi, double *out) {
   const int NN = e->N * e->N;
   double *k = (double *)malloc(sizeof(double) * NN);
-  msor_ssd_force(e->R, e->T, phi, e->N, out);
+  if (e->data_term == MSOR_NCC)
+    nccForceNormalized(e->Rn, e->T, phi, e->N, out);
+  else
+    msor_ssd_force(e->R, e->T, phi, e->N, out);
   for (int c = 0; c < 2; ++c) {
     msor_stiff_apply(phi + c * NN, e->N, k);
     for (int i = 0; i < NN; ++i) out[c * NN + i] += e->lambda * k[i];
@@ -539,7 +715,12 @@ int msor_level_solve(const double *R, const double *T, double *phi, int N, const
                      double *energy_trace, double *tau_trace, double *sigma_trace, int *evals_trace,
                      int trace_cap, int *n_armijo_calls, double *final_energy) {
   const int NN = N * N, n2 = 2 * NN;
-  Energy e = {R, T, N, p->lambda, (double *)malloc(sizeof(double) * n2), 0};
+  Energy e = {R, T, N, p->lambda, (double *)malloc(sizeof(double) * n2), 0, p->data_term, NULL, NULL};
+  if (p->data_term == MSOR_NCC) {
+    e.Rn = (double *)malloc(sizeof(double) * NN);
+    e.Tn = (double *)malloc(sizeof(double) * NN);
+    msor_ncc_normalize(R, N, e.Rn);
+  }
   double *de = (double *)malloc(sizeof(double) * n2);
   double *dir = (double *)malloc(sizeof(double) * n2);
   double *oldDir = (double *)calloc(n2, sizeof(double));
@@ -556,7 +737,7 @@ int msor_level_solve(const double *R, const double *T, double *phi, int N, const
   if (p->max_iter <= 0) {
     if (final_energy) *final_energy = energy;
     if (n_armijo_calls) *n_armijo_calls = 0;
-    free(e.tmp); free(de); free(dir); free(oldDir); free(oldGrad);
+    free(e.tmp); free(e.Rn); free(e.Tn); free(de); free(dir); free(oldDir); free(oldGrad);
     return 0;
   }
   int stopping;
@@ -608,7 +789,7 @@ int msor_level_solve(const double *R, const double *T, double *phi, int N, const
 
   if (final_energy) *final_energy = energyNew;
   if (n_armijo_calls) *n_armijo_calls = calls;
-  free(e.tmp); free(de); free(dir); free(oldDir); free(oldGrad);
+  free(e.tmp); free(e.Rn); free(e.Tn); free(de); free(dir); free(oldDir); free(oldGrad);
   return iter;
 }
 
@@ -847,7 +1028,7 @@ static double solveML(const Pyramid *R, const Pyramid *T, DefPyramid *d, int sta
                       const msor_series_params *p) {
   double energy = NAN;
   for (int level = start; level <= stop; ++level) {
-    msor_level_params lp = {levelLambda(p, level), p->max_iter, p->stop_eps};
+    msor_level_params lp = {levelLambda(p, level), p->max_iter, p->stop_eps, p->data_term};
     msor_level_solve(R->lev[level], T->lev[level], d->lev[level], 1 << level, &lp, NULL, NULL, NULL, NULL, 0,
                      NULL, &energy);
     if (level < stop) defProlong(d, level, p->resample_prolong);

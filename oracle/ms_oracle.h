@@ -33,6 +33,11 @@ double msor_ssd_energy(const double *R, const double *T, const double *phi, int 
 void msor_ssd_force(const double *R, const double *T, const double *phi, int N, double *out);
 void msor_stiff_apply(const double *u, int N, double *out);           /* one component, K u      */
 double msor_reg_energy(const double *phi, int N);                     /* 1/2 sum_c phi_c K phi_c */
+void msor_mass_apply(const double *u, int N, double *out);            /* one component, M u      */
+/* NCC data term (deformationModel 0; registration.h:1909-2018) */
+double msor_ncc_normalize(const double *img, int N, double *out);     /* returns sqrt(u.Mu)      */
+double msor_ncc_energy(const double *R, const double *T, const double *phi, int N);
+void msor_ncc_force(const double *R, const double *T, const double *phi, int N, double *out);
 
 /* --- H1 metric: one FD heat-equation multigrid V-cycle (linearSmoothOp.cpp:11-147,451-612) --- */
 void msor_heat_apply(const double *u, int N, double tau, double *out);
@@ -42,10 +47,13 @@ void msor_vcycle(const double *d, int N, double sigma, double *out);
  * phi (2*N*N) is updated in place. Traces (may be NULL) are written per Armijo call
  * (tau, sigma at the call, energy evaluations of the call) and per iteration (energy after the
  * step, energy_trace[0] = initial energy); trace_cap bounds every trace. Returns iterations. */
+enum { MSOR_SSD = 0, MSOR_NCC = 1 }; /* data term: deformationModel 22 / deformationModel 0 */
+
 typedef struct {
   double lambda;      /* regularisation weight of this level            */
   int max_iter;       /* maxGDIterations                                */
   double stop_eps;    /* stopEpsilon                                    */
+  int data_term;      /* MSOR_SSD or MSOR_NCC                           */
 } msor_level_params;
 
 int msor_level_solve(const double *R, const double *T, double *phi, int N, const msor_level_params *p,
@@ -86,6 +94,7 @@ typedef struct {
   int dont_normalize;                 /* dontNormalizeInputImages                             */
   int resample_prolong;               /* resampleInsteadOfProlongateDeformation               */
   int dont_accumulate;                /* dontAccumulateDeformation                            */
+  int data_term;                      /* MSOR_SSD or MSOR_NCC                                 */
 } msor_series_params;
 
 int msor_match_series(const double *templates, int n, const double *reference, int N,

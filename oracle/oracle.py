@@ -16,8 +16,12 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 
 
+SSD, NCC = 0, 1  # data terms: deformationModel 22 / deformationModel 0
+
+
 class LevelParams(ctypes.Structure):
-    _fields_ = [("lambda_", ctypes.c_double), ("max_iter", ctypes.c_int), ("stop_eps", ctypes.c_double)]
+    _fields_ = [("lambda_", ctypes.c_double), ("max_iter", ctypes.c_int), ("stop_eps", ctypes.c_double),
+                ("data_term", ctypes.c_int)]
 
 
 class SeriesParams(ctypes.Structure):
@@ -29,7 +33,7 @@ class SeriesParams(ctypes.Structure):
         ("max_iter", ctypes.c_int), ("stop_eps", ctypes.c_double),
         ("saturation_pct", ctypes.c_double), ("normalize_min_to_zero", ctypes.c_int),
         ("dont_normalize", ctypes.c_int), ("resample_prolong", ctypes.c_int),
-        ("dont_accumulate", ctypes.c_int),
+        ("dont_accumulate", ctypes.c_int), ("data_term", ctypes.c_int),
     ]
 
 
@@ -45,6 +49,8 @@ def lib():
         L.msor_interpolate.restype = ctypes.c_double
         L.msor_ssd_energy.restype = ctypes.c_double
         L.msor_reg_energy.restype = ctypes.c_double
+        L.msor_ncc_normalize.restype = ctypes.c_double
+        L.msor_ncc_energy.restype = ctypes.c_double
         L.msor_level_solve.restype = ctypes.c_int
         L.msor_match_series.restype = ctypes.c_int
         _lib = L
@@ -121,6 +127,33 @@ def reg_energy(phi):
     return lib().msor_reg_energy(_p(phi), phi.shape[-1])
 
 
+def mass_apply(u):
+    u = _f64(u)
+    out = np.empty_like(u)
+    lib().msor_mass_apply(_p(u), u.shape[-1], _p(out))
+    return out
+
+
+def ncc_normalize(img):
+    """(normalised image, sqrt(u.Mu)) as normalizeImageForNCC computes them."""
+    img = _f64(img)
+    out = np.empty_like(img)
+    var = lib().msor_ncc_normalize(_p(img), img.shape[-1], _p(out))
+    return out, var
+
+
+def ncc_energy(R, T, phi):
+    R, T, phi = _f64(R), _f64(T), _f64(phi)
+    return lib().msor_ncc_energy(_p(R), _p(T), _p(phi), R.shape[-1])
+
+
+def ncc_force(R, T, phi):
+    R, T, phi = _f64(R), _f64(T), _f64(phi)
+    out = np.empty_like(phi)
+    lib().msor_ncc_force(_p(R), _p(T), _p(phi), R.shape[-1], _p(out))
+    return out
+
+
 def heat_apply(u, tau):
     u = _f64(u)
     out = np.empty_like(u)
@@ -135,7 +168,7 @@ def vcycle(d, sigma):
     return out
 
 
-def level_solve(R, T, phi0, lam, max_iter, stop_eps, trace_cap=None):
+def level_solve(R, T, phi0, lam, max_iter, stop_eps, trace_cap=None, data_term=SSD):
     """Returns (phi, info) with info: iterations, final energy, energy trace per iteration and
     tau / sigma / evaluations per Armijo call."""
     R, T = _f64(R), _f64(T)
@@ -148,7 +181,7 @@ def level_solve(R, T, phi0, lam, max_iter, stop_eps, trace_cap=None):
     ev = np.zeros(cap, dtype=np.int32)
     calls = ctypes.c_int(0)
     fe = ctypes.c_double(0)
-    p = LevelParams(lam, max_iter, stop_eps)
+    p = LevelParams(lam, max_iter, stop_eps, data_term)
     it = lib().msor_level_solve(_p(R), _p(T), _p(phi), N, ctypes.byref(p), _p(et), _p(tt), _p(st),
                                 ev.ctypes.data_as(_ip), cap, ctypes.byref(calls), ctypes.byref(fe))
     nc = calls.value

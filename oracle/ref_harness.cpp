@@ -122,6 +122,24 @@ int main(int argc, char **argv) {
       F.apply(phi, dssd);
       regDE.apply(phi, kphi);
       append(out, dssd[0]); append(out, dssd[1]); append(out, kphi[0]); append(out, kphi[1]);
+    } else if (cmd == "ncc") {
+      // in: R, T, phix, phiy ; out: E_ncc, DE_ncc (2N^2), M R (N^2), normalised R (N^2), var(R)
+      // (NormalizedCrossCorrelationEnergy / VariationOf... registration.h:1909-2018; MassOp assembled)
+      aol::Vector<RType> R(NN), T(NN), mr(NN), rn(NN);
+      aol::MultiVector<RType> phi(2, NN), dncc(2, NN);
+      toVec(&in[0], R); toVec(&in[NN], T); toVec(&in[2 * NN], phi[0]); toVec(&in[3 * NN], phi[1]);
+      qc::NormalizedCrossCorrelationEnergy<ConfType> E(grid, R, T);
+      qc::VariationOfNormalizedCrossCorrelationEnergy<ConfType> DE(grid, R, T);
+      aol::Scalar<RType> e;
+      E.apply(phi, e);
+      out.push_back(e[0]);
+      DE.apply(phi, dncc);
+      append(out, dncc[0]); append(out, dncc[1]);
+      E.getMassOpRef().apply(R, mr);
+      append(out, mr);
+      const RType var = qc::NormalizedCrossCorrelationEnergy<ConfType>::normalizeImageForNCC(E.getMassOpRef(), R, rn);
+      append(out, rn);
+      out.push_back(var);
     } else if (cmd == "smooth") {
       // params: sigma ; in: d ; out: one V-cycle of the heat operator applied to d
       const double sigma = atof(argv[3]);
@@ -203,7 +221,7 @@ int main(int argc, char **argv) {
       out.push_back(a.getNumY());
       append(out, a);
     } else if (cmd == "solve") {
-      // params: lambda maxIt stopEps ; in: R, T, phix0, phiy0
+      // params: lambda maxIt stopEps [deformationModel: 22 SSD (default), 0 NCC] ; in: R, T, phix0, phiy0
       // out: phix, phiy, iterations, E_final, then per iteration k: E_total(k), tau(k), sigma(k), evals(k)
       // (E_total(0) is the initial energy; tau/sigma/evals are recorded per Armijo call).
       const double lambda = atof(argv[3]);
@@ -212,16 +230,24 @@ int main(int argc, char **argv) {
       aol::Vector<RType> R(NN), T(NN);
       aol::MultiVector<RType> phi(2, NN);
       toVec(&in[0], R); toVec(&in[NN], T); toVec(&in[2 * NN], phi[0]); toVec(&in[3 * NN], phi[1]);
+      const int model = argc > 8 ? atoi(argv[6]) : 22;  // deformationModel: 22 SSD, 0 NCC
       aol::StiffOp<ConfType> stiff(grid, aol::ASSEMBLED);
       qc::DisplacementLengthEnergy<ConfType> regE(stiff);
       aol::DiagonalBlockOp<RType> regDE(stiff);
       qc::SSDEnergy<ConfType> ssdE(grid, R, T);
       qc::SSDForce<ConfType> ssdF(grid, R, T);
+      qc::NormalizedCrossCorrelationEnergy<ConfType> nccE(grid, R, T);
+      qc::VariationOfNormalizedCrossCorrelationEnergy<ConfType> nccDE(grid, R, T);
       aol::LinCombOp<aol::MultiVector<RType>, aol::Scalar<RType> > E;
-      E.appendReference(ssdE);
-      E.appendReference(regE, lambda);
       aol::LinCombOp<aol::MultiVector<RType> > DE;
-      DE.appendReference(ssdF);
+      if (model == 0) {
+        E.appendReference(nccE);
+        DE.appendReference(nccDE);
+      } else {
+        E.appendReference(ssdE);
+        DE.appendReference(ssdF);
+      }
+      E.appendReference(regE, lambda);
       DE.appendReference(regDE, lambda);
       TracingGD gd(grid, E, DE, maxIt, 1.0, stopEps);
       gd.setConfigurationFlags(GDType::USE_NONLINEAR_CG | GDType::DO_NOT_WRITE_CONSOLE_OUTPUT);

@@ -85,6 +85,14 @@ def main():
             nn = N * N
             out[f"ssd{N}"] = dict(R=R, T=T, phi=phi, e_ssd=o[0], e_reg=o[1], de_ssd=o[2:2 + 2 * nn].reshape(2, N, N),
                                   kphi=o[2 + 2 * nn:].reshape(2, N, N))
+        # NCC data term (deformationModel 0): energy, variation, mass matrix, normalised reference
+        for N in (8, 32, 64):
+            R, T, phi = rand_problem(N, N)
+            o = harness("ncc", N, [], [R, T, phi], tmp)
+            nn = N * N
+            out[f"ncc{N}"] = dict(R=R, T=T, phi=phi, e_ncc=o[0], de_ncc=o[1:1 + 2 * nn].reshape(2, N, N),
+                                  mass_r=o[1 + 2 * nn:1 + 3 * nn].reshape(N, N),
+                                  r_norm=o[1 + 3 * nn:1 + 4 * nn].reshape(N, N), var_r=o[1 + 4 * nn])
         for N in (8, 64):
             d = np.random.default_rng(100 + N).standard_normal((N, N))
             for k, s in enumerate((1.0, 2.0 ** -5)):
@@ -122,13 +130,14 @@ def main():
         out["image129"] = dict(nx=int(o[0]), ny=int(o[1]), data=o[2:].reshape(int(o[1]), int(o[0])))
         # a P9 file written by the reference (stage-1 median of the end-to-end run) is covered below
         # complete level solves with their traces
-        for N, lam, its, eps in ((32, 10.0, 50, -1.0), (64, 10.0, 50, -1.0), (64, 1.0, 200, 1e-6)):
+        for N, lam, its, eps, model in ((32, 10.0, 50, -1.0, 22), (64, 10.0, 50, -1.0, 22), (64, 1.0, 200, 1e-6, 22),
+                                        (32, 10.0, 40, -1.0, 0), (64, 1.0, 60, -1.0, 0)):
             s = stem_pair(N)
             o = harness("normalize", N, [0.15, 1], [s[0]], tmp)
             R = o[:N * N].reshape(N, N)
             o = harness("normalize", N, [0.15, 1], [s[1]], tmp)
             T = o[:N * N].reshape(N, N)
-            o = harness("solve", N, [lam, its, eps], [R, T, np.zeros((2, N, N))], tmp)
+            o = harness("solve", N, [lam, its, eps, model], [R, T, np.zeros((2, N, N))], tmp)
             nn = N * N
             k = 2 * nn
             iters = int(o[k])
@@ -138,7 +147,8 @@ def main():
             k += iters + 1
             nc = int(o[k])
             k += 1
-            out[f"solve{N}_lam{lam:g}"] = dict(R=R, T=T, lam=lam, max_iter=its, stop_eps=eps,
+            out[f"solve{N}_lam{lam:g}" + ("_ncc" if model == 0 else "")] = dict(
+                                             R=R, T=T, lam=lam, max_iter=its, stop_eps=eps, model=model,
                                              phi=o[:2 * nn].reshape(2, N, N), iterations=iters, energy=e_fin,
                                              energies=energies, taus=o[k:k + nc], sigmas=o[k + nc:k + 2 * nc],
                                              evals=o[k + 2 * nc:k + 3 * nc])

@@ -79,12 +79,26 @@ def test_series_stats():
     assert np.array_equal(med, g["median"])
 
 
-@pytest.mark.parametrize("name", ["solve32_lam10", "solve64_lam10", "solve64_lam1"])
+@pytest.mark.parametrize("N", [8, 32, 64])
+def test_ncc_energy_variation_mass(N):
+    """deformationModel 0: NormalizedCrossCorrelationEnergy / its variation and the assembled mass
+    matrix, against the reference's own classes (bit for bit)."""
+    g = load(f"ncc{N}")
+    assert O.ncc_energy(g["R"], g["T"], g["phi"]) == g["e_ncc"]
+    np.testing.assert_array_equal(O.ncc_force(g["R"], g["T"], g["phi"]), g["de_ncc"])
+    np.testing.assert_array_equal(O.mass_apply(g["R"]), g["mass_r"])
+    rn, var = O.ncc_normalize(g["R"])
+    np.testing.assert_array_equal(rn, g["r_norm"])
+    assert var == g["var_r"]
+
+
+@pytest.mark.parametrize("name", ["solve32_lam10", "solve64_lam10", "solve64_lam1", "solve32_lam10_ncc",
+                                  "solve64_lam1_ncc"])
 def test_level_solve_trace(name):
     g = load(name)
     N = g["R"].shape[0]
     phi, info = O.level_solve(g["R"], g["T"], np.zeros((2, N, N)), float(g["lam"]), int(g["max_iter"]),
-                              float(g["stop_eps"]))
+                              float(g["stop_eps"]), data_term=O.NCC if int(g["model"]) == 0 else O.SSD)
     assert info["iterations"] == int(g["iterations"])
     np.testing.assert_array_equal(info["taus"], g["taus"])
     np.testing.assert_array_equal(info["sigmas"], g["sigmas"])

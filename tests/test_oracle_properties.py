@@ -40,6 +40,55 @@ def test_ssd_force_is_the_derivative():
         assert abs(fd - np.sum(F * v)) <= 1e-5 * abs(fd)
 
 
+@pytest.mark.parametrize("N", [2, 8, 32])
+def test_mass_closed_forms(N):
+    x, y, h = grid(N)
+    # interior stencil h^2/36 [1 4 1; 4 16 4; 1 4 1], total mass 1, exact second moment of x
+    if N >= 4:
+        u = np.zeros((N, N))
+        u[N // 2, N // 2] = 1.0
+        blk = O.mass_apply(u)[N // 2 - 1:N // 2 + 2, N // 2 - 1:N // 2 + 2]
+        np.testing.assert_allclose(blk, h * h / 36 * np.array([[1, 4, 1], [4, 16, 4], [1, 4, 1]]), rtol=1e-14)
+    assert abs(O.mass_apply(np.ones((N, N))).sum() - 1.0) < 1e-14
+    assert abs(np.sum(O.mass_apply(x * h) * (x * h)) - 1.0 / 3.0) < 1e-14
+    rng = np.random.default_rng(N)
+    a, b = rng.random((N, N)), rng.random((N, N))
+    assert abs(np.sum(O.mass_apply(a) * b) - np.sum(a * O.mass_apply(b))) < 1e-14
+
+
+@pytest.mark.parametrize("N", [8, 64])
+def test_ncc_closed_forms(N):
+    # phi = 0: T an increasing affine function of R gives -1, a decreasing one +1; the normalised
+    # reference has unit M-norm and zero M-mean
+    rng = np.random.default_rng(N + 1)
+    R = rng.random((N, N))
+    z = np.zeros((2, N, N))
+    assert abs(O.ncc_energy(R, 3.0 * R + 2.0, z) + 1.0) < 1e-12
+    assert abs(O.ncc_energy(R, 0.5 - 2.0 * R, z) - 1.0) < 1e-12
+    T = rng.random((N, N))
+    assert abs(O.ncc_energy(R, 7.0 * T - 1.0, z) - O.ncc_energy(R, T, z)) < 1e-12
+    rn, var = O.ncc_normalize(R)
+    assert abs(np.sum(O.mass_apply(rn) * rn) - 1.0) < 1e-12 and abs(O.mass_apply(rn).sum()) < 1e-12
+    # a constant template has no variance: no division, energy 0
+    assert abs(O.ncc_energy(R, np.full((N, N), 0.7), z)) < 1e-12
+
+
+def test_ncc_force_along_a_translation():
+    # the variation is the continuous one (quadrature-point gradients), the energy warps nodally:
+    # they agree to O(h) along smooth directions, and a wrong sign or scale fails this
+    N = 64
+    x, y, h = grid(N)
+    R = np.sin(4 * np.pi * x / N) * np.cos(4 * np.pi * y / N) + 0.5
+    T = np.sin(4 * np.pi * (x + 0.7) / N) * np.cos(4 * np.pi * (y - 0.4) / N) + 0.5
+    phi = 0.3 * h * np.random.default_rng(0).standard_normal((2, N, N))
+    F = O.ncc_force(R, T, phi)
+    for c in range(2):
+        v = np.zeros((2, N, N))
+        v[c] = 0.5 * h
+        fd = (O.ncc_energy(R, T, phi + 1e-6 * v) - O.ncc_energy(R, T, phi - 1e-6 * v)) / 2e-6
+        assert abs(fd - np.sum(F * v)) <= 0.05 * abs(fd)
+
+
 @pytest.mark.parametrize("N", [2, 8, 33 - 1])
 def test_stiffness_closed_forms(N):
     x, y, h = grid(N)
