@@ -25,6 +25,9 @@ extern "C" {
 
 enum { MSG_OK = 0, MSG_ERR_ARG = 1, MSG_ERR_HIP = 2, MSG_ERR_STATE = 3, MSG_ERR_ALLOC = 4 };
 enum { MSG_F32 = 0, MSG_F64 = 1 };
+/* Data terms: SSD (matchSeries deformationModel 22, SSDRegistrationConfigurator) and normalised
+ * cross-correlation (deformationModel 0, NCCRegistrationConfigurator; registration.h:1766-1781). */
+enum { MSG_SSD = 0, MSG_NCC = 1 };
 
 /* Description of the last error raised on the calling thread ("" if none). */
 const char *msg_last_error(void);
@@ -97,6 +100,9 @@ int msg_solve(msg_ctx *ctx, int start_level, int stop_level, const msg_solve_par
  *    every 8 iterations; 2: as 1 with the 8 iterations replayed as one hipGraph. Default 0.
  * msg_set_default_mode applies to contexts created later and to msg_match_series. */
 int msg_ctx_set_mode(msg_ctx *ctx, int mode);
+/* The registration configurator of the context's solves (StandardRegistrationMultilevelDescent's
+ * RegistrationConfiguratorType, matchSeries.cpp:17-21 and 194-197): MSG_SSD (default) or MSG_NCC. */
+int msg_ctx_set_data_term(msg_ctx *ctx, int data_term);
 int msg_set_default_mode(int mode);
 
 /* Totals of the last msg_solve / msg_solve_level: iterations, reference-equivalent energy
@@ -119,6 +125,13 @@ int msg_energy(int dtype, const void *R, const void *T, const void *phi, int n, 
  * registration.h:677-710): the variation DE of the registration energy. */
 int msg_gradient(int dtype, const void *R, const void *T, const void *phi, void *DE, int n, int batch,
                  double lambda, void *stream);
+/* NormalizedCrossCorrelationEnergy (registration.h:1909-1990; FastDeformImage warp, assembled mass
+ * matrix) per pair: e_ncc (and e_reg as msg_energy); its variation VariationOfNormalizedCross-
+ * CorrelationEnergy (L1998-2018, CrossCorrelationForce L1826-1862) plus lambda K phi. */
+int msg_ncc_energy(int dtype, const void *R, const void *T, const void *phi, int n, int batch, double *e_ncc,
+                   double *e_reg, void *stream);
+int msg_ncc_gradient(int dtype, const void *R, const void *T, const void *phi, void *DE, int n, int batch,
+                     double lambda, void *stream);
 
 /* LinearSmoothOp::apply (quoc/linearSmoothOp.cpp:591-612): one FD heat multigrid V-cycle with
  * tau = sigma^2/2 on each of nfields fields (in and out may alias). */
@@ -171,6 +184,7 @@ typedef struct {
   int pairwise_batch;                 /* pairwise solves per batch; 0: automatic. The chain   */
                                       /* starts on the first finished batch and overlaps the  */
                                       /* later ones (results do not depend on it)             */
+  int data_term;                      /* MSG_SSD ("deformationModel 22") or MSG_NCC (0)       */
 } msg_series_params;
 
 /* Optional per-level records of msg_match_series: what the reference writes after every level of

@@ -103,6 +103,36 @@ __device__ __forceinline__ Hit deform_qp(int ex, int ey, int q, double ox, doubl
   return r;
 }
 
+// transformCoord<ConfType, false> (deformations.h:623-668, 681-693): the quad point in grid
+// coordinates plus offset/h, without clipping; false when it leaves the grid on either axis (the
+// reference stops at the first such axis; only the flag matters then). The element / local
+// coordinate of a point exactly on the last node line is moved into the last element.
+__device__ __forceinline__ bool transform_noclip(int ex, int ey, int q, double ox, double oy, int n, double h,
+                                                 Hit &r) {
+  const double c[2] = {(double)ex + qpx(q), (double)ey + qpy(q)};
+  const double off[2] = {ox, oy};
+  int el[2];
+  double loc[2];
+  for (int i = 0; i < 2; ++i) {
+    const double tc = c[i] + off[i] / h;
+    if (!(tc >= 0.)) return false;  // also NaN
+    if (tc >= (double)(n - 1)) {
+      if (tc != (double)(n - 1)) return false;
+      el[i] = n - 2;
+      loc[i] = 1;
+    } else {
+      el[i] = static_cast<int>(tc);
+      loc[i] = tc - el[i];
+    }
+  }
+  r.ex = el[0];
+  r.ey = el[1];
+  r.lx = loc[0];
+  r.ly = loc[1];
+  r.inx = r.iny = true;
+  return true;
+}
+
 // T(el, loc): DiscreteFunctionDefault::evaluate (discreteFunction.h:229-236)
 template <typename S>
 __device__ __forceinline__ double eval_img(const S *__restrict__ T, const Hit &p, int n, double t[4]) {
@@ -114,6 +144,55 @@ __device__ __forceinline__ double eval_img(const S *__restrict__ T, const Hit &p
   double w = 0.;
   for (int b = 0; b < 4; ++b) w += t[b] * q1(b, p.lx, p.ly);
   return w;
+}
+
+// ScalarArray<QC_2D>::interpolate (scalarArray.h:499-529) including the Y == 0 branch.
+template <typename S>
+__device__ __forceinline__ double interp(const S *__restrict__ img, int n, double X, double Y) {
+  if (X != X) X = 0.;  // NaN coordinates (undefined in the reference) read inside the image
+  if (Y != Y) Y = 0.;
+  int xL = (int)X, yL = (int)Y;
+  if (X >= (double)(n - 1)) xL = n - 2;
+  if (Y >= (double)(n - 1)) yL = n - 2;
+  if (X < 0.) xL = 0;
+  if (Y < 0.) yL = 0;
+  X -= xL;
+  Y -= yL;
+  const int xU = xL + 1, yU = yL + 1;
+  const double v0 = ld(img, xL + yL * n), v1 = ld(img, xU + yL * n);
+  const double v2 = ld(img, xL + yU * n), v3 = ld(img, xU + yU * n);
+  if (Y == 0) return (1 - X) * v0 + X * v1;
+  return (1 - X) * (1 - Y) * v0 + X * (1 - Y) * v1 + (1 - X) * Y * v2 + X * Y * v3;
+}
+
+
+// Nodal warp of FastDeformImage (deformations.h:140-162): T at node (x, y) displaced by (px, py)
+// in domain units, clamped to the grid (the second test also sends NaN to 0, keeping the gather
+// inside the image).
+template <typename S>
+__device__ __forceinline__ double fast_warp(const S *__restrict__ T, int n, double h, int x, int y, double px,
+                                            double py) {
+  const double hi = (double)(n - 1);
+  double X = x + px / h, Y = y + py / h;
+  X = X > hi ? hi : X;
+  X = X >= 0. ? X : 0.;
+  Y = Y > hi ? hi : Y;
+  Y = Y >= 0. ? Y : 0.;
+  return interp(T, n, X, Y);
+}
+
+// Q1 element mass matrix times the element volume (FELinScalarWeightedMassInterface::prepareLocalMatrix,
+// FEOpInterface.h:1446-1475): sum over the quad points of (b_i b_j * 1) * w_q, then * vol.
+__device__ __forceinline__ void mass_local(double vol, double ml[4][4]) {
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) ml[i][j] = 0.;
+  for (int q = 0; q < 4; ++q)
+    for (int i = 0; i < 4; ++i) {
+      const double bi = bq(i, q);
+      for (int j = i; j < 4; ++j) ml[i][j] += bi * bq(j, q) * 1. * 0.25;
+    }
+  for (int i = 0; i < 4; ++i)
+    for (int j = i; j < 4; ++j) ml[j][i] = ml[i][j] = ml[i][j] * vol;
 }
 
 // Element quadratic form psi^T K_loc psi of the Q1 stiffness matrix written in differences:

@@ -221,28 +221,42 @@ EnergyGeom energy_geom(int n) {
   return g;
 }
 
+// doubles of the energy pass's partial sums for npairs pairs at grid size n (either data term)
+size_t energy_part_size(int n, int npairs) {
+  return (size_t)npairs * (msg::MAX_CAND * 4 + 2) * energy_geom(n).nblk;
+}
+
 template <typename S>
 int launch_energy(const S *R, const S *T, const S *phi, const S *dcg, const S *dgrad, int n, const PairState *st,
-                  double *part, int npairs, hipStream_t s, double bytes = 0, const msg::Tail &tail = msg::Tail{}) {
+                  double *part, int npairs, hipStream_t s, int data_term, double bytes = 0,
+                  const msg::Tail &tail = msg::Tail{}) {
   ProfScope ps(KC_ENERGY, s, bytes);
   const EnergyGeom g = energy_geom(n);
   const double h = 1.0 / (double)(n - 1);
   const int width = (int)((n - 1) * h);
-  hipLaunchKernelGGL((msg::k_energy<S>), dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, dcg, dgrad, n, h, width,
-                     st, part, g.ntx, g.nty, npairs, tail);
+  if (data_term == msg::DT_NCC)
+    hipLaunchKernelGGL((msg::k_energy_ncc<S>), dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, dcg, dgrad, n, h,
+                       st, part, g.ntx, g.nty, npairs, tail);
+  else
+    hipLaunchKernelGGL((msg::k_energy<S>), dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, dcg, dgrad, n, h, width,
+                       st, part, g.ntx, g.nty, npairs, tail);
   HIPCHK(hipGetLastError());
   return MSG_OK;
 }
 
 template <typename S>
 int launch_grad(const S *R, const S *T, const S *phi, S *DE, int n, const PairState *st, double lambda, int npairs,
-                hipStream_t s) {
+                hipStream_t s, int data_term) {
   ProfScope ps(KC_GRAD, s, 6.0 * n * n * sizeof(S) * npairs);
   const int ntx = cdiv(n, msg::GR_TX), nty = cdiv(n, msg::GR_TY);
   const double h = 1.0 / (double)(n - 1);
   const int width = (int)((n - 1) * h);
-  hipLaunchKernelGGL((msg::k_grad<S>), dim3(ntx * nty * npairs), dim3(256), 0, s, R, T, phi, DE, n, h, width, st,
-                     lambda, ntx, nty, npairs);
+  if (data_term == msg::DT_NCC)
+    hipLaunchKernelGGL((msg::k_grad<S, msg::DT_NCC>), dim3(ntx * nty * npairs), dim3(256), 0, s, R, T, phi, DE, n, h,
+                       width, st, lambda, ntx, nty, npairs);
+  else
+    hipLaunchKernelGGL((msg::k_grad<S, msg::DT_SSD>), dim3(ntx * nty * npairs), dim3(256), 0, s, R, T, phi, DE, n, h,
+                       width, st, lambda, ntx, nty, npairs);
   HIPCHK(hipGetLastError());
   return MSG_OK;
 }
@@ -374,6 +388,7 @@ struct msg_ctx {
   };
   std::vector<GraphEntry> graphs;
   std::vector<double> level_start;  // per pair: E and lambda*E_reg at the start of the last level solve
+  int data_term = msg::DT_SSD;
 };
 
 namespace {
@@ -484,10 +499,12 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
   a.max_iter = max_iter;
   a.spec = spec;
   a.trace_cap = tcap;
+  a.data_term = c->data_term;
+  a.npairs = B;
 
   hipLaunchKernelGGL(k_state_init, dim3(cdiv(B, 64)), dim3(64), 0, s, c->st, B, lambda);
   HIPCHK(hipGetLastError());
-  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s));
+  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term));
   a.phase = msg::PH_LEVEL_INIT;
   hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
   HIPCHK(hipGetLastError());
@@ -495,7 +512,7 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
   bool any_active = max_iter > 0;
   int nactive = max_iter > 0 ? B : 0;
   while (any_active) {
-    CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s));
+    CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s, c->data_term));
     CHK(launch_vcycle<S>(DE, 1, z, l, mg, c->st, 1.0, 2 * B, c->part_n, s));
     a.phase = msg::PH_BETA;
     hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
@@ -510,7 +527,7 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
     int rounds = 0;
     int nsearch = nactive;  // pairs evaluating in this round (upper bound for the first round)
     while (searching) {
-      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s));
+      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term));
       a.phase = msg::PH_ARMIJO_STEP;
       hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d,
                          tr);
@@ -611,6 +628,8 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
   a.max_iter = max_iter;
   a.spec = spec;
   a.trace_cap = tcap;
+  a.data_term = c->data_term;
+  a.npairs = B;
   auto mk = [&](int phase, int slot) {
     msg::Tail t;
     t.st = c->st;
@@ -629,15 +648,16 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
 
   hipLaunchKernelGGL(k_state_init, dim3(cdiv(B, 64)), dim3(64), 0, s, c->st, B, lambda);
   HIPCHK(hipGetLastError());
-  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, 0.0, t_level));
+  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, t_level));
 
   auto iteration = [&]() -> int {
-    CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s));
+    CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s, c->data_term));
     CHK(launch_vcycle<S>(DE, 1, z, l, mg, c->st, 1.0, 2 * B, c->part_n, s, t_beta));
     hipLaunchKernelGGL((msg::k_ncg<S>), dim3(nblk_ncg * 2 * B), dim3(256), 0, s, z, d, DE, n, c->st, c->part_c, nblk_ncg,
                        2 * B, t_init);
     HIPCHK(hipGetLastError());
-    for (int r = 0; r < Rr; ++r) CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, 0.0, t_step));
+    for (int r = 0; r < Rr; ++r)
+      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, t_step));
     hipLaunchKernelGGL((msg::k_update<S>), dim3(nblk_upd * B), dim3(256), 0, s, phi, d, z, n, c->st, nblk_upd, B,
                        c->counters + 3 * B);
     HIPCHK(hipGetLastError());
@@ -855,7 +875,8 @@ int msg_ctx_create(msg_ctx **out, int max_level, int npairs, int dtype) {
   A(&c->x2, B * 2 * NN * c->es);
   const int N = 1 << max_level;
   const EnergyGeom eg = energy_geom(N);
-  A((void **)&c->part_e, sizeof(double) * B * msg::MAX_CAND * 2 * eg.nblk);
+  (void)eg;
+  A((void **)&c->part_e, sizeof(double) * energy_part_size(N, (int)B));
   A((void **)&c->part_n, sizeof(double) * B * 2 * std::max(1, mg_norm_blocks(max_level)));
   A((void **)&c->part_c, sizeof(double) * B * 2 * 2 * cdiv(N * N, 256 * msg::NCG_PER_THREAD));
   A((void **)&c->st, sizeof(PairState) * B);
@@ -1022,27 +1043,68 @@ int msg_ctx_buffer(msg_ctx *c, int which, int level, void **ptr) {
 
 namespace {
 template <typename S>
-int energy_impl(const void *R, const void *T, const void *phi, int n, int batch, double *e_ssd,
-                       double *e_reg, hipStream_t s) {
+// Energy pass at phi for every pair, reduced on the host. SSD: e_data = E_SSD. NCC: e_data = E_NCC and,
+// if st_out is given, the pair states the NCC variation reads (reference and template statistics,
+// E_NCC at phi).
+int energy_impl(const void *R, const void *T, const void *phi, int n, int batch, int data_term, double *e_data,
+                       double *e_reg, hipStream_t s, std::vector<PairState> *st_out = nullptr) {
   const EnergyGeom g = energy_geom(n);
   void *stp = nullptr, *part = nullptr;
   CHK(scratch_get(0, sizeof(PairState) * batch, &stp));
-  CHK(scratch_get(1, sizeof(double) * batch * msg::MAX_CAND * 2 * g.nblk, &part));
+  CHK(scratch_get(1, sizeof(double) * energy_part_size(n, batch), &part));
   hipLaunchKernelGGL(k_state_init, dim3(cdiv(batch, 64)), dim3(64), 0, s, (PairState *)stp, batch, 0.0);
   HIPCHK(hipGetLastError());
   CHK(launch_energy<S>((const S *)R, (const S *)T, (const S *)phi, (const S *)phi, (const S *)phi, n,
-                       (const PairState *)stp, (double *)part, batch, s));
-  std::vector<double> hp((size_t)batch * msg::MAX_CAND * 2 * g.nblk);
+                       (const PairState *)stp, (double *)part, batch, s, data_term));
+  std::vector<double> hp(energy_part_size(n, batch));
   HIPCHK(hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  auto sum = [&](size_t off) {
+    double a = 0;
+    for (int i = 0; i < g.nblk; ++i) a += hp[off + i];
+    return a;
+  };
+  if (st_out) st_out->assign(batch, PairState{});
   for (int p = 0; p < batch; ++p) {
-    double a = 0, b = 0;
-    const double *pp = hp.data() + (size_t)p * msg::MAX_CAND * 2 * g.nblk;
-    for (int i = 0; i < g.nblk; ++i) a += pp[i];
-    for (int i = 0; i < g.nblk; ++i) b += pp[g.nblk + i];
-    if (e_ssd) e_ssd[p] = a;
-    if (e_reg) e_reg[p] = b;
+    if (data_term == msg::DT_NCC) {
+      const double r1 = sum(msg::part_index_ref(batch, p, 0, g.nblk)), r2 = sum(msg::part_index_ref(batch, p, 1, g.nblk));
+      const double rv = std::sqrt(std::fmax(r2 - r1 * r1, 0.0));
+      const double s1 = sum(msg::part_index(msg::DT_NCC, p, 0, 0, g.nblk));
+      double vt;
+      const double d = msg::ncc_value(s1, sum(msg::part_index(msg::DT_NCC, p, 0, 1, g.nblk)),
+                                      sum(msg::part_index(msg::DT_NCC, p, 0, 2, g.nblk)), r1, rv, &vt);
+      if (e_data) e_data[p] = d;
+      if (e_reg) e_reg[p] = sum(msg::part_index(msg::DT_NCC, p, 0, 3, g.nblk));
+      if (st_out) {
+        PairState &q = (*st_out)[p];
+        q.active = 1;
+        q.ncc_rm = r1;
+        q.ncc_rv = rv;
+        q.ncc_m = s1;
+        q.ncc_v = vt;
+        q.ncc_d = d;
+      }
+    } else {
+      if (e_data) e_data[p] = sum(msg::part_index(msg::DT_SSD, p, 0, 0, g.nblk));
+      if (e_reg) e_reg[p] = sum(msg::part_index(msg::DT_SSD, p, 0, 1, g.nblk));
+    }
   }
+  return MSG_OK;
+}
+
+// DE = DE_NCC + lambda K phi: an energy pass at phi supplies the statistics the variation needs.
+template <typename S>
+int ncc_gradient_impl(const void *R, const void *T, const void *phi, void *DE, int n, int batch, double lambda,
+                      hipStream_t s) {
+  std::vector<PairState> hs;
+  CHK(energy_impl<S>(R, T, phi, n, batch, msg::DT_NCC, nullptr, nullptr, s, &hs));
+  for (auto &q : hs) q.lambda = lambda;
+  void *stp = nullptr;
+  CHK(scratch_get(2, sizeof(PairState) * batch, &stp));
+  HIPCHK(hipMemcpyAsync(stp, hs.data(), sizeof(PairState) * batch, hipMemcpyHostToDevice, s));
+  CHK(launch_grad<S>((const S *)R, (const S *)T, (const S *)phi, (S *)DE, n, (const PairState *)stp, lambda, batch, s,
+                     msg::DT_NCC));
+  HIPCHK(hipStreamSynchronize(s));
   return MSG_OK;
 }
 }  // namespace
@@ -1054,8 +1116,26 @@ int msg_energy(int dtype, const void *R, const void *T, const void *phi, int n, 
   CHK(check_dtype(dtype));
   CHK(check_n(n));
   if (!R || !T || !phi || batch < 1) return fail(MSG_ERR_ARG, "invalid argument");
-  return dtype == MSG_F64 ? energy_impl<double>(R, T, phi, n, batch, e_ssd, e_reg, strm(stream))
-                          : energy_impl<float>(R, T, phi, n, batch, e_ssd, e_reg, strm(stream));
+  return dtype == MSG_F64 ? energy_impl<double>(R, T, phi, n, batch, msg::DT_SSD, e_ssd, e_reg, strm(stream))
+                          : energy_impl<float>(R, T, phi, n, batch, msg::DT_SSD, e_ssd, e_reg, strm(stream));
+}
+
+int msg_ncc_energy(int dtype, const void *R, const void *T, const void *phi, int n, int batch, double *e_ncc,
+                   double *e_reg, void *stream) {
+  CHK(check_dtype(dtype));
+  CHK(check_n(n));
+  if (!R || !T || !phi || batch < 1) return fail(MSG_ERR_ARG, "invalid argument");
+  return dtype == MSG_F64 ? energy_impl<double>(R, T, phi, n, batch, msg::DT_NCC, e_ncc, e_reg, strm(stream))
+                          : energy_impl<float>(R, T, phi, n, batch, msg::DT_NCC, e_ncc, e_reg, strm(stream));
+}
+
+int msg_ncc_gradient(int dtype, const void *R, const void *T, const void *phi, void *DE, int n, int batch,
+                     double lambda, void *stream) {
+  CHK(check_dtype(dtype));
+  CHK(check_n(n));
+  if (!R || !T || !phi || !DE || batch < 1) return fail(MSG_ERR_ARG, "invalid argument");
+  return dtype == MSG_F64 ? ncc_gradient_impl<double>(R, T, phi, DE, n, batch, lambda, strm(stream))
+                          : ncc_gradient_impl<float>(R, T, phi, DE, n, batch, lambda, strm(stream));
 }
 
 int msg_gradient(int dtype, const void *R, const void *T, const void *phi, void *DE, int n, int batch, double lambda,
@@ -1065,10 +1145,10 @@ int msg_gradient(int dtype, const void *R, const void *T, const void *phi, void 
   if (!R || !T || !phi || !DE || batch < 1) return fail(MSG_ERR_ARG, "invalid argument");
   if (dtype == MSG_F64)
     CHK(launch_grad<double>((const double *)R, (const double *)T, (const double *)phi, (double *)DE, n, nullptr,
-                            lambda, batch, strm(stream)));
+                            lambda, batch, strm(stream), msg::DT_SSD));
   else
     CHK(launch_grad<float>((const float *)R, (const float *)T, (const float *)phi, (float *)DE, n, nullptr, lambda,
-                           batch, strm(stream)));
+                           batch, strm(stream), msg::DT_SSD));
   HIPCHK(hipStreamSynchronize(strm(stream)));
   return MSG_OK;
 }
@@ -1388,6 +1468,7 @@ int pairwise_batch(const msg_series_params *p, const msg_solve_params &sp, const
   msg_ctx *pc = nullptr;
   CHK(ctx_acquire(L, nb, dt, &pc));
   CtxGuard gpc{pc};
+  CHK(msg_ctx_set_data_term(pc, p->data_term));
   hipStream_t s = pc->stream;
   for (int i = i0; i < i1; ++i) {
     const S *r = (i == 0 || p->dont_accumulate) ? ref : tpl + (size_t)(i - 1) * nn;
@@ -1406,6 +1487,10 @@ int pairwise_batch(const msg_series_params *p, const msg_solve_params &sp, const
   }
   CHK(rebuild_pyramid<S>(pc, pc->R, -1, 1));
   CHK(rebuild_pyramid<S>(pc, pc->T, -1, 1));
+  // every pairwise solve starts from the zero deformation (a context taken from the cache still
+  // holds its previous solution)
+  for (int l = 0; l <= L; ++l)
+    HIPCHK(hipMemsetAsync(pc->phi[l], 0, sizeof(S) * ((size_t)2 << (2 * l)) * nb, s));
   LevelOut lo;
   lo.lv = lv;
   lo.rec0 = (long long)i0 * (p->stop_level - p->start_level + 1);
@@ -1467,6 +1552,7 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
   msg_ctx *rc = nullptr;
   if (chain) CHK(ctx_acquire(L, 1, dt, &rc));
   CtxGuard grc{rc};
+  if (chain) CHK(msg_ctx_set_data_term(rc, p->data_term));
 
   DevBuf pw;
   HIPCHK(hipMalloc(&pw.p, sizeof(S) * 2 * nn * n));
@@ -1688,6 +1774,17 @@ int msg_ctx_set_mode(msg_ctx *c, int mode) {
   if (!c) return fail(MSG_ERR_ARG, "null context");
   if (mode < 0 || mode > 2) return fail(MSG_ERR_ARG, "mode must be 0, 1 or 2");
   c->mode = mode;
+  return MSG_OK;
+}
+
+int msg_ctx_set_data_term(msg_ctx *c, int data_term) {
+  if (!c) return fail(MSG_ERR_ARG, "null context");
+  if (data_term != MSG_SSD && data_term != MSG_NCC) return fail(MSG_ERR_ARG, "data term must be MSG_SSD or MSG_NCC");
+  if (c->data_term != data_term) {
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);  // the captured kernels depend on it
+    c->graphs.clear();
+  }
+  c->data_term = data_term;
   return MSG_OK;
 }
 

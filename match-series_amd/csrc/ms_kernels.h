@@ -33,7 +33,31 @@ struct CtrlArgs {
   int max_iter;
   int spec;          // candidates per energy pass (1..MAX_CAND)
   int trace_cap;
+  int data_term;     // DT_SSD or DT_NCC
+  int npairs;        // pairs of the batch (NCC: the reference statistics follow the candidates' partials)
 };
+
+// Where the energy pass stores its partial sums. SSD: 2 terms per candidate (E_SSD, E_reg). NCC: 4
+// per candidate (1'Mt, t'Mt, t'Mr, E_reg) and, after all candidates of all pairs, 2 per pair
+// (1'Mr, r'Mr) for the reference's statistics.
+__device__ __host__ __forceinline__ size_t part_index(int data_term, int pair, int k, int term, int nblk) {
+  return ((size_t)(pair * MAX_CAND + k) * (data_term == DT_NCC ? 4 : 2) + term) * nblk;
+}
+__device__ __host__ __forceinline__ size_t part_index_ref(int npairs, int pair, int term, int nblk) {
+  return ((size_t)npairs * MAX_CAND * 4 + (size_t)pair * 2 + term) * nblk;
+}
+// NCC energy from the moments of one evaluation (NormalizedCrossCorrelationEnergy::computeEnergy,
+// registration.h:1953-1964, with normalizeImageForNCC, L1934-1947, written in moments): t and r
+// have M-means m_t = 1'Mt, m_r = 1'Mr and deviations v = sqrt(u'Mu - m^2) (1'M1 = 1); a deviation
+// within 1e-15 of zero is not divided by (appeqAbsolute, aol.h:779-794).
+__device__ __host__ __forceinline__ double ncc_value(double s1, double s2, double s3, double rm, double rv,
+                                                    double *vt_out) {
+  const double vt = sqrt(fmax(s2 - s1 * s1, 0.0));
+  *vt_out = vt;
+  const double dt = fabs(vt - 0.0) < 1e-15 ? 1.0 : vt;
+  const double dr = fabs(rv - 0.0) < 1e-15 ? 1.0 : rv;
+  return -((s3 - s1 * rm) / (dt * dr));
+}
 
 __device__ __forceinline__ double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
@@ -64,29 +88,39 @@ __device__ void armijo_plan(PairState &s, int spec) {
   s.passes++;
 }
 
-__device__ bool armijo_lookup(const PairState &s, double t, double &E, double &Rg) {
+// One energy evaluation: total E, unweighted E_reg and (NCC) the data energy and the warped
+// template's mean / deviation.
+struct EvalRec {
+  double E, R, D, M, V;
+};
+
+__device__ bool armijo_lookup(const PairState &s, double t, EvalRec &ev) {
   for (int j = 0; j < s.ncand; ++j)
     if (s.cand[j] == t) {
-      E = s.cand_E[j];
-      Rg = s.cand_R[j];
+      ev = {s.cand_E[j], s.cand_R[j], s.cand_D[j], s.cand_M[j], s.cand_V[j]};
       return true;
     }
   for (int j = 0; j < 2; ++j)
     if (s.last_tau[j] == t) {
-      E = s.last_E[j];
-      Rg = s.last_R[j];
+      ev = {s.last_E[j], s.last_R[j], s.last_D[j], s.last_M[j], s.last_V[j]};
       return true;
     }
   return false;
 }
 
-__device__ void armijo_consume(PairState &s, double t, double E, double Rg) {
+__device__ void armijo_consume(PairState &s, double t, const EvalRec &ev) {
   s.last_tau[1] = s.last_tau[0];
   s.last_E[1] = s.last_E[0];
   s.last_R[1] = s.last_R[0];
+  s.last_D[1] = s.last_D[0];
+  s.last_M[1] = s.last_M[0];
+  s.last_V[1] = s.last_V[0];
   s.last_tau[0] = t;
-  s.last_E[0] = E;
-  s.last_R[0] = Rg;
+  s.last_E[0] = ev.E;
+  s.last_R[0] = ev.R;
+  s.last_D[0] = ev.D;
+  s.last_M[0] = ev.M;
+  s.last_V[0] = ev.V;
   s.evals++;
 }
 
@@ -105,10 +139,13 @@ __device__ void finish_iteration(PairState &s, double t, IterTrace *trace, int t
   s.step_dir = s.dirmode;
   double En = s.energy, Rn = s.ereg_new;
   if (t != 0) {
-    double E, Rg;
-    if (armijo_lookup(s, t, E, Rg)) {
-      En = E;
-      Rn = Rg;
+    EvalRec ev;
+    if (armijo_lookup(s, t, ev)) {
+      En = ev.E;
+      Rn = ev.R;
+      s.ncc_d = ev.D;  // the NCC variation at the new position needs these
+      s.ncc_m = ev.M;
+      s.ncc_v = ev.V;
     } else {
       En = __builtin_nan("");  // cannot happen: the accepted step is one of the last two evaluations
     }
@@ -148,29 +185,32 @@ __device__ void finish_iteration(PairState &s, double t, IterTrace *trace, int t
 __device__ void armijo_run(PairState &s, int spec, IterTrace *trace, int trace_cap, int pair) {
   const double f = s.energy;
   for (int guard = 0; guard < 256; ++guard) {
-    double t = s.a_tau, E = 0, Rg = 0;
+    double t = s.a_tau;
+    EvalRec ev;
     double result = -1;  // >= 0 when the search ends
     if (s.a_phase == AP_FIRST) {
       if (s.a_Df >= 0 || !isfinite(f)) {
         s.evals++;  // the reference evaluates E(tau0) before these checks
         result = 0;
       } else {
-        if (!armijo_lookup(s, t, E, Rg)) {
+        if (!armijo_lookup(s, t, ev)) {
           armijo_plan(s, spec);
           return;
         }
-        armijo_consume(s, t, E, Rg);
+        armijo_consume(s, t, ev);
+        const double E = ev.E;
         const double G = (E - f) / (s.a_Df * t);
         s.a_phase = (G >= ARMIJO_SIGMA && f >= E) ? AP_GROW : AP_SHRINK;
         continue;
       }
     } else if (s.a_phase == AP_GROW) {
       const double tn = t * 2.;
-      if (!armijo_lookup(s, tn, E, Rg)) {
+      if (!armijo_lookup(s, tn, ev)) {
         armijo_plan(s, spec);
         return;
       }
-      armijo_consume(s, tn, E, Rg);
+      armijo_consume(s, tn, ev);
+      const double E = ev.E;
       s.a_tau = tn;
       const double G = (E - f) / (s.a_Df * tn);
       if (G >= ARMIJO_SIGMA && f >= E && tn <= TAU_MAX) continue;
@@ -178,11 +218,12 @@ __device__ void armijo_run(PairState &s, int spec, IterTrace *trace, int trace_c
       result = s.a_tau > TAU_MIN ? s.a_tau : 0.;
     } else {  // AP_SHRINK
       const double tn = t > TAU_MIN ? t * 0.5 : t;
-      if (!armijo_lookup(s, tn, E, Rg)) {
+      if (!armijo_lookup(s, tn, ev)) {
         armijo_plan(s, spec);
         return;
       }
-      armijo_consume(s, tn, E, Rg);
+      armijo_consume(s, tn, ev);
+      const double E = ev.E;
       s.a_tau = tn;
       const double G = (E - f) / (s.a_Df * tn);
       if ((!isfinite(E) || (G < ARMIJO_SIGMA || f < E)) && (tn > TAU_MIN)) continue;
@@ -212,10 +253,43 @@ __device__ void control_pair(int pair, PairState *__restrict__ st, const double 
                              int *__restrict__ flags, IterTrace *__restrict__ trace) {
   PairState &s = st[pair];
   __shared__ double red[4];
-  __shared__ double sums[2 * MAX_CAND];
+  __shared__ double sums[4 * MAX_CAND];
   const bool participates = s.active && !s.searching && s.step_tau == 0;
 
-  if (a.phase == PH_LEVEL_INIT || a.phase == PH_ARMIJO_STEP) {
+  if ((a.phase == PH_LEVEL_INIT || a.phase == PH_ARMIJO_STEP) && a.data_term == DT_NCC) {
+    const int nc = s.ncand;
+    const int nb = a.nblk_energy;
+    double rm = 0, rv = 0;
+    if (a.phase == PH_LEVEL_INIT) {
+      const double r1 = reduce_fixed(part_e + part_index_ref(a.npairs, pair, 0, nb), nb, red);
+      const double r2 = reduce_fixed(part_e + part_index_ref(a.npairs, pair, 1, nb), nb, red);
+      if (threadIdx.x == 0) {
+        s.ncc_rm = r1;
+        s.ncc_rv = sqrt(fmax(r2 - r1 * r1, 0.0));
+      }
+    }
+    for (int k = 0; k < nc; ++k)
+      for (int t = 0; t < 4; ++t) {
+        const double v = reduce_fixed(part_e + part_index(DT_NCC, pair, k, t, nb), nb, red);
+        if (threadIdx.x == 0) sums[4 * k + t] = v;
+      }
+    if (threadIdx.x == 0) {
+      rm = s.ncc_rm;
+      rv = s.ncc_rv;
+      for (int k = 0; k < nc; ++k) {
+        double vt;
+        const double D = ncc_value(sums[4 * k], sums[4 * k + 1], sums[4 * k + 2], rm, rv, &vt);
+        double E = 0;
+        E += D;
+        E += s.lambda * sums[4 * k + 3];  // LinCombOp: (0 + E_NCC) + lambda E_reg
+        s.cand_E[k] = E;
+        s.cand_R[k] = sums[4 * k + 3];
+        s.cand_D[k] = D;
+        s.cand_M[k] = sums[4 * k];
+        s.cand_V[k] = vt;
+      }
+    }
+  } else if (a.phase == PH_LEVEL_INIT || a.phase == PH_ARMIJO_STEP) {
     const int nc = s.ncand;
     for (int k = 0; k < nc; ++k) {
       const double e = reduce_fixed(part_e + ((size_t)(pair * MAX_CAND + k) * 2 + 0) * a.nblk_energy, a.nblk_energy, red);
@@ -232,8 +306,16 @@ __device__ void control_pair(int pair, PairState *__restrict__ st, const double 
         E += s.lambda * sums[2 * k + 1];  // LinCombOp: (0 + E_SSD) + lambda E_reg
         s.cand_E[k] = E;
         s.cand_R[k] = sums[2 * k + 1];
+        s.cand_D[k] = s.cand_M[k] = s.cand_V[k] = 0;
       }
+    }
+  }
+  if (a.phase == PH_LEVEL_INIT || a.phase == PH_ARMIJO_STEP) {
+    if (threadIdx.x == 0) {
       if (a.phase == PH_LEVEL_INIT) {
+        s.ncc_d = s.cand_D[0];
+        s.ncc_m = s.cand_M[0];
+        s.ncc_v = s.cand_V[0];
         s.energy = s.energy_new = s.energy_start = s.cand_E[0];
         s.ereg_new = s.ereg_start = s.cand_R[0];
         s.tau = 1.0;
@@ -424,12 +506,116 @@ k_energy(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__
     }
   }
   const int nblk = tiles;
-  for (int k = 0; k < nc; ++k) {
-    const double s = block_sum<256>(accS[k], red);
-    const double r = block_sum<256>(accR[k], red);
+#pragma unroll
+  for (int k = 0; k < MAX_CAND; ++k) {  // static indices keep acc* in registers; nc is uniform
+    if (k < nc) {
+      const double s = block_sum<256>(accS[k], red);
+      const double r = block_sum<256>(accR[k], red);
+      if (threadIdx.x == 0) {
+        part[((size_t)(pair * MAX_CAND + k) * 2 + 0) * nblk + tile] = s;
+        part[((size_t)(pair * MAX_CAND + k) * 2 + 1) * nblk + tile] = r;
+      }
+    }
+  }
+  run_tail(tail, pair, tiles);
+}
+
+// ================================================================================================
+// NCC energy pass (deformationModel 0) for up to MAX_CAND step sizes per pair, element-centric.
+// NormalizedCrossCorrelationEnergy::computeEnergy (registration.h:1953-1964) in moments of the
+// assembled mass matrix: with t the nodal FastDeformImage of T at phi + tau d and r the reference,
+// every element adds its share of 1'Mt, t'Mt, t'Mr (the element matrix of the assembled M), and at
+// a level's first pass (no direction) also 1'Mr, r'Mr. The control combines them (ncc_value).
+// ================================================================================================
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_energy_ncc(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__ phi, const S *__restrict__ dcg,
+             const S *__restrict__ dgrad, int n, double h, const PairState *st, double *__restrict__ part, int ntx,
+             int nty, int npairs, Tail tail) {
+  const int tiles = ntx * nty;
+  const int wg = xcd_remap(blockIdx.x, tiles * npairs);
+  const int pair = wg / tiles, tile = wg - pair * tiles;
+  const int nc = st[pair].ncand;
+  if (nc == 0) return;  // uniform across the workgroup
+  const int dm = st[pair].dirmode;
+  __shared__ double red[4];
+
+  const size_t nn = (size_t)n * n;
+  const S *Rp = R + pair * nn, *Tp = T + pair * nn;
+  const S *px = phi + pair * 2 * nn, *py = px + nn;
+  const S *dsel = (dm == DIR_CG) ? dcg : dgrad;
+  const S *dx = dsel + pair * 2 * nn, *dy = dx + nn;
+  const bool use_dir = (dm != DIR_NONE);
+
+  const int ex = (tile % ntx) * EN_TX + (threadIdx.x % EN_TX);
+  const int ey = (tile / ntx) * EN_TY + (threadIdx.x / EN_TX);
+  const bool valid = (ex < n - 1) && (ey < n - 1);
+
+  double acc[MAX_CAND][4];
+#pragma unroll
+  for (int k = 0; k < MAX_CAND; ++k)
+    for (int t = 0; t < 4; ++t) acc[k][t] = 0.0;
+  double accR1 = 0.0, accR2 = 0.0;
+
+  if (valid) {
+    double f0[4], f1[4], d0[4], d1[4], r[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int g = l2g(ex, ey, b, n);
+      f0[b] = ld(px, g);
+      f1[b] = ld(py, g);
+      d0[b] = use_dir ? ld(dx, g) : 0.0;
+      d1[b] = use_dir ? ld(dy, g) : 0.0;
+      r[b] = ld(Rp, g);
+    }
+    double ml[4][4];
+    mass_local(h * h, ml);
+    if (!use_dir) {
+      for (int i = 0; i < 4; ++i) {
+        double m = 0.;
+        for (int j = 0; j < 4; ++j) m += ml[i][j] * r[j];
+        accR1 += m;
+        accR2 += r[i] * m;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < MAX_CAND; ++k) {
+      if (k < nc) {
+        const double tau = st[pair].cand[k];
+        double p0[4], p1[4], t[4];
+        for (int b = 0; b < 4; ++b) {
+          p0[b] = use_dir ? rnd<S>(f0[b] + tau * d0[b]) : f0[b];
+          p1[b] = use_dir ? rnd<S>(f1[b] + tau * d1[b]) : f1[b];
+          t[b] = fast_warp(Tp, n, h, ex + (b & 1), ey + ((b >> 1) & 1), p0[b], p1[b]);
+        }
+        for (int i = 0; i < 4; ++i) {
+          double m = 0.;
+          for (int j = 0; j < 4; ++j) m += ml[i][j] * t[j];
+          acc[k][0] += m;
+          acc[k][1] += t[i] * m;
+          acc[k][2] += r[i] * m;
+        }
+        acc[k][3] = 0.5 * (stiff_form(p0) + stiff_form(p1));
+      }
+    }
+  }
+  const int nblk = tiles;
+#pragma unroll
+  for (int k = 0; k < MAX_CAND; ++k) {  // static indices keep acc in registers; nc is uniform
+    if (k < nc) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const double v = block_sum<256>(acc[k][t], red);
+        if (threadIdx.x == 0) part[part_index(DT_NCC, pair, k, t, nblk) + tile] = v;
+      }
+    }
+  }
+  if (!use_dir) {
+    const double a1 = block_sum<256>(accR1, red);
+    const double a2 = block_sum<256>(accR2, red);
     if (threadIdx.x == 0) {
-      part[((size_t)(pair * MAX_CAND + k) * 2 + 0) * nblk + tile] = s;
-      part[((size_t)(pair * MAX_CAND + k) * 2 + 1) * nblk + tile] = r;
+      part[part_index_ref(npairs, pair, 0, nblk) + tile] = a1;
+      part[part_index_ref(npairs, pair, 1, nblk) + tile] = a2;
     }
   }
   run_tail(tail, pair, tiles);
@@ -444,7 +630,7 @@ k_energy(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__
 constexpr int GR_TX = 32, GR_TY = 8;  // nodes per workgroup tile
 constexpr int GR_EX = GR_TX + 1, GR_EY = GR_TY + 1;
 
-template <typename S>
+template <typename S, int DT>
 __global__ void __launch_bounds__(256)
 k_grad(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__ phi, S *__restrict__ DE, int n,
        double h, int width, const PairState *st, double lambda_default, int ntx, int nty, int npairs) {
@@ -474,26 +660,73 @@ k_grad(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__ p
       rn[b] = ld(Rp, g);
     }
     double nl[4][2];
-    for (int q = 0; q < 4; ++q) {
-      double ox = 0., oy = 0., r = 0.;
+    if (DT == DT_NCC) {
+      // NormalizedCrossCorrelationEnergy::applyDerivative (registration.h:1974-1984): nodal weight
+      // w = (Rn + E Tn) / v_t with Rn, Tn the normalised reference and warped template, then
+      // CrossCorrelationForce (L1826-1862): -w(x_q) grad T at the unclipped deformed quad point,
+      // zero where it leaves the grid
+      const PairState &ps = st[pair];
+      const bool divr = !(fabs(ps.ncc_rv - 0.0) < 1e-15), divt = !(fabs(ps.ncc_v - 0.0) < 1e-15);
+      double w[4];
       for (int b = 0; b < 4; ++b) {
-        ox += f0[b] * bq(b, q);
-        oy += f1[b] * bq(b, q);
+        double rh = rn[b] + (-ps.ncc_rm);
+        if (divr) rh /= ps.ncc_rv;
+        double th = fast_warp(Tp, n, h, ex + (b & 1), ey + ((b >> 1) & 1), f0[b], f1[b]) + (-ps.ncc_m);
+        if (divt) th /= ps.ncc_v;
+        double wb = rh;
+        wb += ps.ncc_d * th;
+        if (divt) wb /= ps.ncc_v;
+        w[b] = wb;
       }
-      for (int b = 0; b < 4; ++b) r += rn[b] * bq(b, q);
-      const Hit hp = deform_qp(ex, ey, q, ox, oy, n, h, width);
-      double t4[4];
-      const double td = eval_img(Tp, hp, n, t4);
-      double g0 = 0., g1 = 0.;
-      for (int b = 0; b < 4; ++b) {
-        double v0 = q1dx(b, hp.ly) / h, v1 = q1dy(b, hp.lx) / h;
-        v0 *= t4[b];
-        v1 *= t4[b];
-        g0 += v0;
-        g1 += v1;
+      for (int q = 0; q < 4; ++q) {
+        double ox = 0., oy = 0.;
+        for (int b = 0; b < 4; ++b) {
+          ox += f0[b] * bq(b, q);
+          oy += f1[b] * bq(b, q);
+        }
+        Hit hp;
+        if (!transform_noclip(ex, ey, q, ox, oy, n, h, hp)) {
+          nl[q][0] = nl[q][1] = 0.;
+          continue;
+        }
+        double t4[4];
+        (void)eval_img(Tp, hp, n, t4);
+        double g0 = 0., g1 = 0.;
+        for (int b = 0; b < 4; ++b) {
+          double v0 = q1dx(b, hp.ly) / h, v1 = q1dy(b, hp.lx) / h;
+          v0 *= t4[b];
+          v1 *= t4[b];
+          g0 += v0;
+          g1 += v1;
+        }
+        double wq = 0.;
+        for (int b = 0; b < 4; ++b) wq += w[b] * bq(b, q);
+        const double sc = -wq;
+        nl[q][0] = g0 * sc;
+        nl[q][1] = g1 * sc;
       }
-      nl[q][0] = hp.inx ? g0 * (td - r) : 0;
-      nl[q][1] = hp.iny ? g1 * (td - r) : 0;
+    } else {
+      for (int q = 0; q < 4; ++q) {
+        double ox = 0., oy = 0., r = 0.;
+        for (int b = 0; b < 4; ++b) {
+          ox += f0[b] * bq(b, q);
+          oy += f1[b] * bq(b, q);
+        }
+        for (int b = 0; b < 4; ++b) r += rn[b] * bq(b, q);
+        const Hit hp = deform_qp(ex, ey, q, ox, oy, n, h, width);
+        double t4[4];
+        const double td = eval_img(Tp, hp, n, t4);
+        double g0 = 0., g1 = 0.;
+        for (int b = 0; b < 4; ++b) {
+          double v0 = q1dx(b, hp.ly) / h, v1 = q1dy(b, hp.lx) / h;
+          v0 *= t4[b];
+          v1 *= t4[b];
+          g0 += v0;
+          g1 += v1;
+        }
+        nl[q][0] = hp.inx ? g0 * (td - r) : 0;
+        nl[q][1] = hp.iny ? g1 * (td - r) : 0;
+      }
     }
     for (int dof = 0; dof < 4; ++dof) {
       double a0 = 0, a1 = 0;
@@ -897,23 +1130,6 @@ __global__ void k_prolong(const S *__restrict__ coarse, S *__restrict__ fine, in
   const int j = (int)(i - (size_t)f * cnt);
   const int x = j % n, y = j / n;
   fine[i] = coarse[(size_t)f * nc * nc + (x >> 1) + (y >> 1) * nc];
-}
-
-// ScalarArray<QC_2D>::interpolate (scalarArray.h:499-529) including the Y == 0 branch.
-template <typename S>
-__device__ __forceinline__ double interp(const S *__restrict__ img, int n, double X, double Y) {
-  int xL = (int)X, yL = (int)Y;
-  if (X >= (double)(n - 1)) xL = n - 2;
-  if (Y >= (double)(n - 1)) yL = n - 2;
-  if (X < 0.) xL = 0;
-  if (Y < 0.) yL = 0;
-  X -= xL;
-  Y -= yL;
-  const int xU = xL + 1, yU = yL + 1;
-  const double v0 = ld(img, xL + yL * n), v1 = ld(img, xU + yL * n);
-  const double v2 = ld(img, xL + yU * n), v3 = ld(img, xU + yU * n);
-  if (Y == 0) return (1 - X) * v0 + X * v1;
-  return (1 - X) * (1 - Y) * v0 + X * (1 - Y) * v1 + (1 - X) * Y * v2 + X * Y * v3;
 }
 
 // ScalarArray<QC_2D>::resampleFrom (scalarArray.h:308-331, 1052-1060); coord[k] = k * (nc-1)/(n-1)

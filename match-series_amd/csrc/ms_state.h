@@ -7,6 +7,7 @@ namespace msg {
 enum ArmijoPhase { AP_IDLE = 0, AP_FIRST = 1, AP_GROW = 2, AP_SHRINK = 3 };
 enum DirMode { DIR_CG = 0, DIR_GRAD = 1, DIR_NONE = 2 };
 enum { MAX_CAND = 4 };
+enum DataTerm { DT_SSD = 0, DT_NCC = 1 };  // deformationModel 22 / deformationModel 0
 
 struct PairState {
   // gradient-flow state (GradientDescentBase::applySingle)
@@ -27,6 +28,13 @@ struct PairState {
   double cand[MAX_CAND];
   double cand_E[MAX_CAND], cand_R[MAX_CAND];
   double last_tau[2], last_E[2], last_R[2];  // the last two consumed evaluations
+  // NCC data term: per evaluation the data energy and the mass-weighted mean / deviation of the
+  // warped template; the reference's mean / deviation of the level; the values at the current
+  // position (what the variation needs)
+  double cand_D[MAX_CAND], cand_M[MAX_CAND], cand_V[MAX_CAND];
+  double last_D[2], last_M[2], last_V[2];
+  double ncc_rm, ncc_rv;
+  double ncc_d, ncc_m, ncc_v;
   double step_tau;    // tau of the update (0: no update)
   double tau_first;   // result of the CG-direction search this iteration (trace)
   int ncand;          // candidates to evaluate in the next energy pass (0: none)

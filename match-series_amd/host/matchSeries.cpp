@@ -1,8 +1,8 @@
 // matchSeries <file.par> [ACTION] — the reference's command line (projects/electronMicroscopy/
 // matchSeries.cpp:11-209, matchSeries.h:1249-1320) over the gfx950 library's C ABI.
 //
-// deformationModel 22 (SSD data term, Dirichlet regulariser, H1 gradient flow) is computed on the
-// GPU; ACTION 0 (match and average) and 1 (only average) are supported. The output tree is the
+// deformationModel 22 (SSD data term) and 0 (normalised cross-correlation), each with the Dirichlet
+// regulariser and the H1 gradient flow, are computed on the GPU; ACTION 0 (match and average) and 1 (only average) are supported. The output tree is the
 // reference's: <saveDirectory>/{log.txt, parameter-dump.txt, stage<k>/...}. Errors are reported as
 // the reference does (message on stderr, exit code 0); MSG_STRICT=1 makes them exit with 1.
 #include <hip/hip_runtime_api.h>
@@ -117,6 +117,7 @@ msg_series_params seriesParams(const ParameterParser &p) {
   s.spec = p.getIntOrDefault("gpuArmijoCandidates", 0);
   s.pre_smooth_sigma = p.getDoubleOrDefault("preSmoothSigma", 0.0);
   s.pairwise_batch = p.getIntOrDefault("gpuPairwiseBatch", 0);
+  s.data_term = p.getIntOrDefault("deformationModel", 0) == 22 ? MSG_SSD : MSG_NCC;
   return s;
 }
 
@@ -240,11 +241,17 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
       saveField(sub + strprintf("deformation_%02d_%%d.dat.bz2", l), lvFields.data() + fldOff, m);
       fldOff += (size_t)2 * m * m;
       const double lam = sp.lambda_factor > 0 ? sp.lambda * std::pow(sp.lambda_factor, sp.stop_level - l) : sp.lambda;
+      // NCC is a GenEnergyOp too: its last value comes first (E - lambda*E_reg)
       std::ofstream e(sub + strprintf("energy_%02d.txt", l));
-      e << -1 << " " << lvStart[2 * r + 1] << " " << lvStart[2 * r] << "\n";
+      auto line = [&](int k, double regw, double E) {
+        e << k << " ";
+        if (sp.data_term == MSG_NCC) e << E - regw << " ";
+        e << regw << " " << E << "\n";
+      };
+      line(-1, lvStart[2 * r + 1], lvStart[2 * r]);
       for (int k = 0; k < lvIts[r] && k < maxIt; ++k) {
         const msg_iter_trace &t = lvIters[(size_t)r * maxIt + k];
-        e << k << " " << lam * t.ereg << " " << t.energy << "\n";
+        line(k, lam * t.ereg, t.energy);
       }
     }
   };
@@ -369,8 +376,9 @@ int main(int argc, char **argv) {
   try {
     ParameterParser parser(argc > 1 ? argv[1] : "matchSeries.par");
     const int model = parser.getIntOrDefault("deformationModel", 0);
-    if (model != 22)
-      MSH_THROW(strprintf("deformationModel %d is not supported by this build (22: SSD + Dirichlet regulariser)", model));
+    if (model != 22 && model != 0)
+      MSH_THROW(strprintf("deformationModel %d is not supported by this build (0: NCC, 22: SSD; both with the "
+                          "Dirichlet regulariser)", model));
     runMatchSeries(parser, argc, argv);
   } catch (Exception &e) {
     e.dump(std::cerr);

@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libmsgpu.so")
 
 F32, F64 = 0, 1
+SSD, NCC = 0, 1  # data terms: deformationModel 22 / deformationModel 0
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -39,7 +40,7 @@ class SeriesParams(ctypes.Structure):
         ("saturation_pct", ctypes.c_double), ("normalize_min_to_zero", ctypes.c_int),
         ("dont_normalize", ctypes.c_int), ("resample_prolong", ctypes.c_int),
         ("dont_accumulate", ctypes.c_int), ("spec", ctypes.c_int), ("pre_smooth_sigma", ctypes.c_double),
-        ("pairwise_batch", ctypes.c_int),
+        ("pairwise_batch", ctypes.c_int), ("data_term", ctypes.c_int),
     ]
 
 
@@ -82,6 +83,10 @@ SIGNATURES = {
     "msg_normalize": ([ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp], ctypes.c_int),
     "msg_release_cache": ([], ctypes.c_int),
     "msg_ctx_set_mode": ([_vp, ctypes.c_int], ctypes.c_int),
+    "msg_ctx_set_data_term": ([_vp, ctypes.c_int], ctypes.c_int),
+    "msg_ncc_energy": ([ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _vp], ctypes.c_int),
+    "msg_ncc_gradient": ([ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp],
+                         ctypes.c_int),
     "msg_set_default_mode": ([ctypes.c_int], ctypes.c_int),
     "msg_profile_enable": ([ctypes.c_uint], ctypes.c_int),
     "msg_profile_reset": ([], ctypes.c_int),
@@ -162,6 +167,22 @@ def energy(R, T, phi):
     er = np.zeros(B)
     check(lib().msg_energy(dtype_code(R), _ptr(R), _ptr(T), _ptr(phi), n, B, _dptr(es), _dptr(er), _stream()))
     return es, er
+
+
+def ncc_energy(R, T, phi):
+    """Per-pair (E_NCC, 1/2 sum_c phi_c K phi_c) of the deformationModel 0 data term."""
+    B, n = R.shape[0], R.shape[-1]
+    es = np.zeros(B)
+    er = np.zeros(B)
+    check(lib().msg_ncc_energy(dtype_code(R), _ptr(R), _ptr(T), _ptr(phi), n, B, _dptr(es), _dptr(er), _stream()))
+    return es, er
+
+
+def ncc_gradient(R, T, phi, lam):
+    B, n = R.shape[0], R.shape[-1]
+    out = _torch().empty_like(phi)
+    check(lib().msg_ncc_gradient(dtype_code(R), _ptr(R), _ptr(T), _ptr(phi), _ptr(out), n, B, lam, _stream()))
+    return out
 
 
 def gradient(R, T, phi, lam):
@@ -250,6 +271,9 @@ class Context:
         h = ctypes.c_void_p()
         check(lib().msg_ctx_create(ctypes.byref(h), level, npairs, dtype))
         self.h = h
+
+    def set_data_term(self, data_term):
+        check(lib().msg_ctx_set_data_term(self.h, data_term))
 
     def set_mode(self, mode):
         check(lib().msg_ctx_set_mode(self.h, mode))

@@ -78,6 +78,42 @@ def test_gradient(gpu, N, code, tol):
         assert rel(out[b], ref) < max(tol, 1e-12) * 10
 
 
+@pytest.mark.parametrize("N", [4, 8, 32, 64, 128])
+@pytest.mark.parametrize("code,tol", DT)
+def test_ncc_energy_and_gradient(gpu, N, code, tol):
+    """deformationModel 0: the moment form on the GPU against the oracle's vector form (which is
+    bit-exact with the reference's classes)."""
+    probs = [random_problem(N, 20 + s) for s in range(3)]
+    dt = gpu.torch_dtype(code)
+    R = dev(np.stack([p[0] for p in probs]), dt)
+    T = dev(np.stack([p[1] for p in probs]), dt)
+    phi = dev(np.stack([p[2] for p in probs]), dt)
+    lam = 10.0
+    es, er = gpu.ncc_energy(R, T, phi)
+    out = host(gpu.ncc_gradient(R, T, phi, lam))
+    for b in range(len(probs)):
+        r, t, ph = host(R[b]), host(T[b]), host(phi[b])
+        eo = O.ncc_energy(r, t, ph)
+        assert abs(es[b] - eo) <= 1e-11 * abs(eo)
+        assert abs(er[b] - O.reg_energy(ph)) <= 1e-12 * O.reg_energy(ph)
+        ref = O.ncc_force(r, t, ph) + lam * np.stack([O.stiff_apply(ph[0]), O.stiff_apply(ph[1])])
+        assert rel(out[b], ref) < (1e-10 if code == 1 else 1e-6)
+
+
+def test_ncc_golden(gpu):
+    """The reference's own NormalizedCrossCorrelationEnergy and its variation (fixtures)."""
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    for N in (8, 32, 64):
+        with np.load(os.path.join(here, f"ncc{N}.npz")) as z:
+            g = {k: z[k] for k in z.files}
+        R, T, phi = dev(g["R"][None], torch.float64), dev(g["T"][None], torch.float64), dev(g["phi"][None], torch.float64)
+        es, _ = gpu.ncc_energy(R, T, phi)
+        assert abs(es[0] - g["e_ncc"]) <= 1e-11 * abs(g["e_ncc"])
+        de = host(gpu.ncc_gradient(R, T, phi, 0.0))[0]
+        assert rel(de, g["de_ncc"]) < 1e-10
+
+
 @pytest.mark.parametrize("N", [2, 8, 32, 64, 128, 256])
 @pytest.mark.parametrize("sigma", [1.0, 2.0 ** -5])
 @pytest.mark.parametrize("code,tol", DT)

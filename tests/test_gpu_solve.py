@@ -61,6 +61,25 @@ def test_level_solve_f64_trajectory(gpu, N, lam, its, eps):
     assert np.abs(phi - phi_o).max() * (N - 1) < 1e-8
 
 
+@pytest.mark.parametrize("N,lam,its", [(32, 10.0, 40), (64, 1.0, 60), (128, 10.0, 25)])
+def test_level_solve_ncc_f64_trajectory(gpu, N, lam, its):
+    """deformationModel 0 (NCC): the same step sizes as the oracle at every Armijo call."""
+    R, T = stem_pair(N)
+    L = int(np.log2(N))
+    ctx = gpu.Context(L, 1, gpu.F64)
+    ctx.set_data_term(gpu.NCC)
+    ctx.set_reference(0, R)
+    ctx.set_template(0, T)
+    e, it, tr = ctx.solve_level(L, lam, its, -1.0, trace_cap=its)
+    phi = ctx.get_deformation(0)
+    ctx.close()
+    phi_o, info = O.level_solve(R, T, np.zeros((2, N, N)), lam, its, -1.0, data_term=O.NCC)
+    assert it[0] == info["iterations"]
+    np.testing.assert_array_equal(gpu_calls(tr[0]), info["taus"])
+    np.testing.assert_allclose(tr[0][:, 0], info["energies"][1:], rtol=1e-10)
+    assert np.abs(phi - phi_o).max() * (N - 1) < 1e-8
+
+
 @pytest.mark.parametrize("N,lam,its", [(64, 10.0, 50), (256, 10.0, 30)])
 def test_level_solve_f32(gpu, N, lam, its):
     R, T = stem_pair(N)
@@ -95,20 +114,23 @@ def test_batched_pairs_are_independent(gpu):
     ctx.close()
 
 
-@pytest.mark.parametrize("resample", [0, 1])
-def test_match_series_f64(gpu, resample):
+@pytest.mark.parametrize("resample,dterm", [(0, 0), (1, 0), (1, 1)])
+def test_match_series_f64(gpu, resample, dterm):
     from msamd import synth
     N, L = 64, 6
     frames = synth.stem_series(N, 4, workload=0, spacing=6.0, sigma=1.5, jitter_step=0.15)
     p = O.SeriesParams(start_level=4, stop_level=6, refine_start_level=5, refine_stop_level=6, precision_level=6,
                        lambda_=10.0, lambda_factor=-1, max_iter=15, stop_eps=-1.0, saturation_pct=0.15,
-                       normalize_min_to_zero=1, dont_normalize=0, resample_prolong=resample, dont_accumulate=0)
+                       normalize_min_to_zero=1, dont_normalize=0, resample_prolong=resample, dont_accumulate=0,
+                       data_term=dterm)
     ref = O.match_series(frames[1:], frames[0], p)
     gp = gpu.SeriesParams(start_level=4, stop_level=6, refine_start_level=5, refine_stop_level=6, precision_level=6,
                           lambda_=10.0, lambda_factor=-1, max_iter=15, stop_eps=-1.0, saturation_pct=0.15,
                           normalize_min_to_zero=1, dont_normalize=0, resample_prolong=resample, dont_accumulate=0,
-                          spec=0)
+                          spec=0, data_term=dterm)
     t = torch.tensor(frames, dtype=torch.float64, device="cuda")
+    out = gpu.match_series(t[1:].contiguous(), t[0].contiguous(), gp)
+    # twice: contexts come from the cache the second time and must not carry state over
     out = gpu.match_series(t[1:].contiguous(), t[0].contiguous(), gp)
     d = out["deformations"].cpu().numpy()
     assert np.abs(d - ref["deformations"]).max() * (N - 1) < 1e-6
@@ -119,8 +141,8 @@ def test_match_series_f64(gpu, resample):
 
 
 @pytest.mark.parametrize("mode", [1, 2])
-@pytest.mark.parametrize("N,its,B", [(32, 40, 3), (128, 30, 2)])
-def test_device_driven_modes_match_host_mode(gpu, mode, N, its, B):
+@pytest.mark.parametrize("N,its,B,dterm", [(32, 40, 3, 0), (128, 30, 2, 0), (64, 30, 2, 1)])
+def test_device_driven_modes_match_host_mode(gpu, mode, N, its, B, dterm):
     """The device-driven iteration (control in last-workgroup tails, 8 iterations per host check,
     optionally one hipGraph) gives exactly the host-synchronised result."""
     L = int(np.log2(N))
@@ -129,6 +151,7 @@ def test_device_driven_modes_match_host_mode(gpu, mode, N, its, B):
     for m in (0, mode):
         ctx = gpu.Context(L, B, gpu.F64)
         ctx.set_mode(m)
+        ctx.set_data_term(dterm)
         for k, (R, T) in enumerate(pairs):
             ctx.set_reference(k, R)
             ctx.set_template(k, T)

@@ -88,9 +88,9 @@ def cli(tmp_path, par_text, strict=False):
 
 def test_cli_reports_errors_like_the_reference(tmp_path):
     # unsupported model: message on stderr, exit code 0 (matchSeries.cpp:204-208); strict mode: 1
-    r = cli(tmp_path, "deformationModel 0\n")
-    assert r.returncode == 0 and "deformationModel 0" in r.stderr
-    r = cli(tmp_path, "deformationModel 0\n", strict=True)
+    r = cli(tmp_path, "deformationModel 3\n")
+    assert r.returncode == 0 and "deformationModel 3" in r.stderr
+    r = cli(tmp_path, "deformationModel 3\n", strict=True)
     assert r.returncode == 1
 
 
