@@ -152,56 +152,62 @@ def main():
                                              phi=o[:2 * nn].reshape(2, N, N), iterations=iters, energy=e_fin,
                                              energies=energies, taus=o[k:k + nc], sigmas=o[k + nc:k + 2 * nc],
                                              evals=o[k + 2 * nc:k + 3 * nc])
-        # the reference program end to end: 4 frames of 64x64, two stages
+        # the reference program end to end: 4 frames of 64x64, two stages, SSD and NCC models
         frames = synth.stem_series(64, 4, workload=0, spacing=6.0, sigma=1.5, jitter_step=0.15)
-        wd = os.path.join(tmp, "e2e")
-        os.makedirs(os.path.join(wd, "in"))
-        for i, f in enumerate(frames):
-            write_p9(os.path.join(wd, "in", f"frame_{i:02d}.dat.bz2"), f)
-        par = E2E_PAR
-        with open(os.path.join(wd, "run.par"), "w") as f:
-            f.write(par)
-        env = dict(os.environ, QUOC_NO_SYSTEM_PAUSE="ON")
-        subprocess.check_call([MATCHSERIES, "run.par"], cwd=wd, env=env, stdout=subprocess.DEVNULL,
-                              stderr=subprocess.DEVNULL)
-        od = os.path.join(wd, "out")
-        e2e = dict(frames=frames)
-        for st_ in (1, 2):
-            nt = 3 if st_ == 1 else 4
-            defs = []
-            for i in range(nt):
-                sub = f"{i}" if i == 0 else f"{i}-r"
-                defs.append(np.stack([read_p9(os.path.join(od, f"stage{st_}", sub, f"deformation_06_{c}.dat.bz2"))
-                                      for c in range(2)]))
-            e2e[f"s{st_}_deformations"] = np.stack(defs)
-            if st_ == 1:
-                e2e["s1_pairwise"] = np.stack([np.stack([read_p9(os.path.join(od, "stage1", f"{i}",
-                                                                                f"deformation_06_{c}.dat.bz2"))
-                                                         for c in range(2)]) for i in range(nt)])
-            for nm in ("average", "median", "numSamples"):
-                e2e[f"s{st_}_{nm}"] = read_p9(os.path.join(od, f"stage{st_}", f"{nm}.q2bz"))
-            en = np.loadtxt(os.path.join(od, f"stage{st_}", "energies.txt"))
-            e2e[f"s{st_}_energies"] = en[:, 1]
-            # per-level records: deformation_<LL> and energy_<LL>.txt of every solve directory
-            for sub in sorted(os.listdir(os.path.join(od, f"stage{st_}"))):
-                sd = os.path.join(od, f"stage{st_}", sub)
-                if not os.path.isdir(sd):
-                    continue
-                for fn in sorted(os.listdir(sd)):
-                    key = f"s{st_}_{sub.replace('-', '_')}_"
-                    if fn.startswith("energy_"):
-                        e2e[key + fn[:-4]] = np.loadtxt(os.path.join(sd, fn)).reshape(-1, 3)
-                    elif fn.startswith("deformation_") and fn.endswith("_0.dat.bz2"):
-                        lv = fn[len("deformation_"):len("deformation_") + 2]
-                        e2e[key + f"deformation_{lv}"] = np.stack(
-                            [read_p9(os.path.join(sd, f"deformation_{lv}_{c}.dat.bz2")) for c in range(2)])
-        e2e["par"] = np.frombuffer(par.encode(), dtype=np.uint8)
-        out["e2e64"] = e2e
+        out["e2e64"] = run_e2e(frames, E2E_PAR, os.path.join(tmp, "e2e"))
+        out["e2e64_ncc"] = run_e2e(frames, E2E_PAR.replace("deformationModel 22", "deformationModel 0"),
+                                   os.path.join(tmp, "e2e_ncc"))
     only = sys.argv[sys.argv.index("--only") + 1].split(",") if "--only" in sys.argv else None
     for key, d in out.items():
         if only is None or key in only:
             np.savez_compressed(os.path.join(HERE, f"{key}.npz"), **d)
     print("wrote", len(out), "fixtures")
+
+
+def run_e2e(frames, par, wd):
+    """Runs the reference program on the frames with the parameter file; returns its outputs."""
+    os.makedirs(os.path.join(wd, "in"))
+    for i, f in enumerate(frames):
+        write_p9(os.path.join(wd, "in", f"frame_{i:02d}.dat.bz2"), f)
+    with open(os.path.join(wd, "run.par"), "w") as f:
+        f.write(par)
+    env = dict(os.environ, QUOC_NO_SYSTEM_PAUSE="ON")
+    subprocess.check_call([MATCHSERIES, "run.par"], cwd=wd, env=env, stdout=subprocess.DEVNULL,
+                          stderr=subprocess.DEVNULL)
+    od = os.path.join(wd, "out")
+    e2e = dict(frames=frames)
+    for st_ in (1, 2):
+        nt = 3 if st_ == 1 else 4
+        defs = []
+        for i in range(nt):
+            sub = f"{i}" if i == 0 else f"{i}-r"
+            defs.append(np.stack([read_p9(os.path.join(od, f"stage{st_}", sub, f"deformation_06_{c}.dat.bz2"))
+                                  for c in range(2)]))
+        e2e[f"s{st_}_deformations"] = np.stack(defs)
+        if st_ == 1:
+            e2e["s1_pairwise"] = np.stack([np.stack([read_p9(os.path.join(od, "stage1", f"{i}",
+                                                                            f"deformation_06_{c}.dat.bz2"))
+                                                     for c in range(2)]) for i in range(nt)])
+        for nm in ("average", "median", "numSamples"):
+            e2e[f"s{st_}_{nm}"] = read_p9(os.path.join(od, f"stage{st_}", f"{nm}.q2bz"))
+        en = np.loadtxt(os.path.join(od, f"stage{st_}", "energies.txt"))
+        e2e[f"s{st_}_energies"] = en[:, 1]
+        # per-level records: deformation_<LL> and energy_<LL>.txt of every solve directory
+        for sub in sorted(os.listdir(os.path.join(od, f"stage{st_}"))):
+            sd = os.path.join(od, f"stage{st_}", sub)
+            if not os.path.isdir(sd):
+                continue
+            for fn in sorted(os.listdir(sd)):
+                key = f"s{st_}_{sub.replace('-', '_')}_"
+                if fn.startswith("energy_"):
+                    a = np.loadtxt(os.path.join(sd, fn))
+                    e2e[key + fn[:-4]] = a.reshape(-1, a.shape[-1])
+                elif fn.startswith("deformation_") and fn.endswith("_0.dat.bz2"):
+                    lv = fn[len("deformation_"):len("deformation_") + 2]
+                    e2e[key + f"deformation_{lv}"] = np.stack(
+                        [read_p9(os.path.join(sd, f"deformation_{lv}_{c}.dat.bz2")) for c in range(2)])
+    e2e["par"] = np.frombuffer(par.encode(), dtype=np.uint8)
+    return e2e
 
 
 E2E_PAR = """templateNamePattern in/frame_%02d.dat.bz2

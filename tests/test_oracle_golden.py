@@ -107,10 +107,31 @@ def test_level_solve_trace(name):
     assert np.abs(phi - g["phi"]).max() * (N - 1) < 1e-10
 
 
-def e2e_params(lam):
+def e2e_params(lam, data_term=O.SSD):
     return O.SeriesParams(start_level=4, stop_level=6, refine_start_level=5, refine_stop_level=6, precision_level=6,
                           lambda_=lam, lambda_factor=1.0, max_iter=15, stop_eps=-1.0, saturation_pct=0.15,
-                          normalize_min_to_zero=1, dont_normalize=0, resample_prolong=1, dont_accumulate=0)
+                          normalize_min_to_zero=1, dont_normalize=0, resample_prolong=1, dont_accumulate=0,
+                          data_term=data_term)
+
+
+def test_reference_program_two_stages_ncc():
+    """deformationModel 0 end to end against the reference program's output tree."""
+    g = load("e2e64_ncc")
+    fr = g["frames"]
+    N = fr.shape[-1]
+    r1 = O.match_series(fr[1:], fr[0], e2e_params(10.0, O.NCC))
+    assert np.abs(r1["deformations"] - g["s1_deformations"]).max() * (N - 1) < 1e-10
+    assert np.abs(r1["pairwise"] - g["s1_pairwise"]).max() * (N - 1) < 1e-10
+    assert np.array_equal(r1["count"], g["s1_numSamples"])
+    np.testing.assert_allclose(r1["median"], g["s1_median"], atol=1e-10)
+    np.testing.assert_allclose(r1["energies"], g["s1_energies"], rtol=1e-5)
+    pw = np.zeros((4, 2, N, N))
+    pw[1:] = g["s1_pairwise"]
+    r2 = O.match_series(fr, g["s1_median"], e2e_params(1.0, O.NCC), pw)
+    assert np.abs(r2["deformations"] - g["s2_deformations"]).max() * (N - 1) < 1e-7
+    assert np.array_equal(r2["count"], g["s2_numSamples"])
+    np.testing.assert_allclose(r2["median"], g["s2_median"], atol=1e-8)
+    np.testing.assert_allclose(r2["energies"], g["s2_energies"], rtol=1e-5)
 
 
 def test_reference_program_two_stages():
