@@ -185,6 +185,9 @@ typedef struct {
                                       /* starts on the first finished batch and overlaps the  */
                                       /* later ones (results do not depend on it)             */
   int data_term;                      /* MSG_SSD ("deformationModel 22") or MSG_NCC (0)       */
+  int average_ext_mean;               /* "deformForAverageExtendedWithMean": outside the      */
+                                      /* domain a frame takes its own mean instead of no sample*/
+  int average_nearest;                /* "deformForAverageUsingNearestNeighborInterpolation"  */
 } msg_series_params;
 
 /* Optional per-level records of msg_match_series: what the reference writes after every level of

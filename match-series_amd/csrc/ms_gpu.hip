@@ -1656,10 +1656,23 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
   if (n > msg::ST_MAXF) return fail(MSG_ERR_ARG, "series longer than 128 frames are not supported by the median kernel");
   DevBuf stack;
   HIPCHK(hipMalloc(&stack.p, sizeof(S) * nn * std::max(n, 1)));
-  const size_t cnt = nn * n;
-  hipLaunchKernelGGL((msg::k_warp<S>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, tpl, (const S *)defs,
-                     (S *)stack.p, N, n, 0, 0, (double)INFINITY, 0);
-  HIPCHK(hipGetLastError());
+  // averageSeries (matchSeries.h:353-451): DeformImage of each raw template, +Inf (no sample) or the
+  // template's own mean outside the domain, optionally nearest-neighbour
+  const int nearest = p->average_nearest ? 1 : 0;
+  if (p->average_ext_mean) {
+    for (int i = 0; i < n; ++i) {
+      double ext = 0;
+      CHK(mean_value<S>(tpl + (size_t)i * nn, (int)nn, &ext, s));
+      hipLaunchKernelGGL((msg::k_warp<S>), dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, tpl + (size_t)i * nn,
+                         (const S *)defs + (size_t)i * 2 * nn, (S *)stack.p + (size_t)i * nn, N, 1, 0, 0, ext, nearest);
+      HIPCHK(hipGetLastError());
+    }
+  } else {
+    const size_t cnt = nn * n;
+    hipLaunchKernelGGL((msg::k_warp<S>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, tpl, (const S *)defs,
+                       (S *)stack.p, N, n, 0, 0, (double)INFINITY, nearest);
+    HIPCHK(hipGetLastError());
+  }
   double fill = 0;
   CHK(mean_value<S>(tpl + (size_t)(n - 1) * nn, (int)nn, &fill, s));
   CHK(stats_impl<S>(stack.p, n, N, fill, mean, count, median, s));

@@ -86,12 +86,15 @@ std::string dirOf(const ParameterParser &p) { return p.getStringExpandTilde("sav
 void rejectUnsupported(const ParameterParser &p) {
   const char *flags[] = {"reduceDeformations", "useCorrelationToInitTranslation", "reverseRolesInSeriesMatching",
                          "calcInverseDeformation", "cropInput", "resizeInput", "calcIQ", "calcPrecision",
-                         "performNoiseAnalysis"};
+                         "performNoiseAnalysis", "saveNamedDeformedTemplates", "saveNamedDeformations"};
   for (const char *f : flags)
     if (p.checkAndGetBool(f)) MSH_THROW(strprintf("\"%s 1\" is not supported by this build", f));
   if (p.hasVariable("altStartLevel") && p.getInt("altStartLevel") != p.getInt("startLevel"))
     MSH_THROW("\"altStartLevel\" is not supported by this build");
   if (p.hasVariable("initial-deformation")) MSH_THROW("\"initial-deformation\" is not supported by this build");
+  if (p.hasVariable("averageSaveIncrement")) MSH_THROW("\"averageSaveIncrement\" is not supported by this build");
+  // the reference itself does not implement a positive threshold (registration.h:604-610)
+  if (p.getDoubleOrDefault("averageXDerivNormThreshold", 0) > 0) MSH_THROW("averageXDerivNormThreshold: not implemented");
   if (p.hasVariable("minimizationAlgo") && p.getString("minimizationAlgo") != "GradientDescent")
     MSH_THROW("only \"minimizationAlgo GradientDescent\" is supported by this build");
 }
@@ -118,6 +121,8 @@ msg_series_params seriesParams(const ParameterParser &p) {
   s.pre_smooth_sigma = p.getDoubleOrDefault("preSmoothSigma", 0.0);
   s.pairwise_batch = p.getIntOrDefault("gpuPairwiseBatch", 0);
   s.data_term = p.getIntOrDefault("deformationModel", 0) == 22 ? MSG_SSD : MSG_NCC;
+  s.average_ext_mean = p.checkAndGetBool("deformForAverageExtendedWithMean");
+  s.average_nearest = p.checkAndGetBool("deformForAverageUsingNearestNeighborInterpolation");
   return s;
 }
 
@@ -155,19 +160,27 @@ void averageOnly(const ParameterParser &p) {
   const int n = (int)files.size();
   DeviceBuffer img(es * nn * n), defs(es * 2 * nn * n), stack(es * nn * n), mean(es * nn), cnt(es * nn), med(es * nn);
   double lastMean = 0;
+  const bool extMean = p.checkAndGetBool("deformForAverageExtendedWithMean");
+  const int nearest = p.checkAndGetBool("deformForAverageUsingNearestNeighborInterpolation") ? 1 : 0;
+  std::vector<double> means(n);
   for (int i = 0; i < n; ++i) {
     const Image im = loadChecked(files[i], N);
     upload((char *)img.p + (size_t)i * es * nn, im.data.data(), nn, dt);
     std::vector<double> phi;
     loadField(deformationBase(p, dir, i, L) + "_%d.dat.bz2", N, phi);
     upload((char *)defs.p + (size_t)i * es * 2 * nn, phi.data(), 2 * nn, dt);
-    if (i == n - 1) {
-      double s = 0;
-      for (double v : im.data) s += v;
-      lastMean = s / nn;
-    }
+    double s = 0;
+    for (double v : im.data) s += v;
+    means[i] = s / nn;  // ScalarArray::getMeanValue
+    if (i == n - 1) lastMean = means[i];
   }
-  msgCheck(msg_warp(dt, img.p, defs.p, stack.p, N, n, 0, 0, INFINITY, 0, nullptr));
+  if (extMean) {
+    for (int i = 0; i < n; ++i)
+      msgCheck(msg_warp(dt, (char *)img.p + (size_t)i * es * nn, (char *)defs.p + (size_t)i * es * 2 * nn,
+                        (char *)stack.p + (size_t)i * es * nn, N, 1, 0, 0, means[i], nearest, nullptr));
+  } else {
+    msgCheck(msg_warp(dt, img.p, defs.p, stack.p, N, n, 0, 0, INFINITY, nearest, nullptr));
+  }
   msgCheck(msg_series_stats(dt, stack.p, n, N, lastMean, mean.p, cnt.p, med.p, nullptr));
   std::vector<double> hm(nn), hc(nn), hmed(nn);
   download(hm.data(), mean.p, nn, dt);

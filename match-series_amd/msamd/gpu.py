@@ -41,6 +41,7 @@ class SeriesParams(ctypes.Structure):
         ("dont_normalize", ctypes.c_int), ("resample_prolong", ctypes.c_int),
         ("dont_accumulate", ctypes.c_int), ("spec", ctypes.c_int), ("pre_smooth_sigma", ctypes.c_double),
         ("pairwise_batch", ctypes.c_int), ("data_term", ctypes.c_int),
+        ("average_ext_mean", ctypes.c_int), ("average_nearest", ctypes.c_int),
     ]
 
 

@@ -188,3 +188,26 @@ def test_match_series_independent_of_pairwise_batching(gpu):
     for o in outs[1:]:
         for k in ("deformations", "energies", "mean", "count", "median"):
             np.testing.assert_array_equal(o[k], outs[0][k], err_msg=k)
+
+
+@pytest.mark.parametrize("ext_mean,nearest", [(1, 0), (0, 1), (1, 1)])
+def test_match_series_averaging_options(gpu, ext_mean, nearest):
+    """deformForAverageExtendedWithMean / ...UsingNearestNeighborInterpolation: the reconstruction from
+    the returned deformations, restated with the oracle's DeformImage and statistics."""
+    from msamd import synth
+    N = 64
+    frames = synth.stem_series(N, 4, workload=0, spacing=6.0, sigma=1.5, jitter_step=0.15)
+    gp = gpu.SeriesParams(start_level=4, stop_level=6, refine_start_level=5, refine_stop_level=6, precision_level=6,
+                          lambda_=10.0, lambda_factor=-1, max_iter=10, stop_eps=-1.0, saturation_pct=0.15,
+                          normalize_min_to_zero=1, dont_normalize=0, resample_prolong=1, dont_accumulate=0,
+                          average_ext_mean=ext_mean, average_nearest=nearest)
+    t = torch.tensor(frames, dtype=torch.float64, device="cuda")
+    out = gpu.match_series(t[1:].contiguous(), t[0].contiguous(), gp)
+    d = out["deformations"].cpu().numpy()
+    tpl = frames[1:]
+    stack = np.stack([O.deform_image(tpl[i], d[i], ext=tpl[i].mean() if ext_mean else np.inf, nearest=bool(nearest))
+                      for i in range(len(tpl))])
+    mean, cnt, med = O.series_stats(stack, tpl[-1].mean())
+    np.testing.assert_array_equal(out["count"].cpu().numpy(), cnt)
+    np.testing.assert_allclose(out["mean"].cpu().numpy(), mean, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(out["median"].cpu().numpy(), med, rtol=1e-12, atol=1e-12)
