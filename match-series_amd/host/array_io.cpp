@@ -1,6 +1,7 @@
 #include "array_io.h"
 
 #include <bzlib.h>
+#include <zlib.h>
 
 #include <algorithm>
 #include <cmath>
@@ -183,6 +184,60 @@ void loadField(const std::string &mask, int n, std::vector<double> &phi) {
 
 void saveField(const std::string &mask, const double *phi, int n) {
   for (int c = 0; c < 2; ++c) saveP9(strprintf(mask.c_str(), c), phi + (size_t)c * n * n, n, n);
+}
+
+
+namespace {
+void putU32(std::string &b, uint32_t v) {
+  for (int k = 3; k >= 0; --k) b.push_back((char)((v >> (8 * k)) & 0xff));
+}
+void pngChunk(std::string &out, const char *type, const std::string &data) {
+  putU32(out, (uint32_t)data.size());
+  std::string td(type, 4);
+  td += data;
+  out += td;
+  putU32(out, (uint32_t)crc32(0L, reinterpret_cast<const Bytef *>(td.data()), (uInt)td.size()));
+}
+}  // namespace
+
+void savePNGScaled(const std::string &fileName, const double *data, int nx, int ny) {
+  if (nx <= 0 || ny <= 0) MSH_THROW("Cannot save ScalarArray of size 0 as PNG.");
+  double mn = data[0], mx = data[0];
+  const size_t cnt = (size_t)nx * ny;
+  for (size_t i = 1; i < cnt; ++i) {
+    mn = std::min(mn, data[i]);
+    mx = std::max(mx, data[i]);
+  }
+  const double range = mx - mn;
+  std::string raw;
+  raw.reserve(cnt + ny);
+  for (int y = 0; y < ny; ++y) {
+    raw.push_back(0);  // filter type 0 per scanline
+    for (int x = 0; x < nx; ++x) {
+      double v = data[(size_t)y * nx + x];
+      v = v < mn ? mn : (v > mx ? mx : v);
+      const double s = range > 0 ? (v - mn) / (1.0 * range) * 255 : 0.0;
+      raw.push_back((char)static_cast<unsigned char>(s));
+    }
+  }
+  uLongf zlen = compressBound((uLong)raw.size());
+  std::string z(zlen, '\0');
+  if (compress2(reinterpret_cast<Bytef *>(&z[0]), &zlen, reinterpret_cast<const Bytef *>(raw.data()), (uLong)raw.size(),
+                Z_DEFAULT_COMPRESSION) != Z_OK)
+    MSH_THROW("PNG compression failed");
+  z.resize(zlen);
+  std::string out("\x89PNG\r\n\x1a\n", 8);
+  std::string ihdr;
+  putU32(ihdr, (uint32_t)nx);
+  putU32(ihdr, (uint32_t)ny);
+  ihdr += std::string("\x08\x00\x00\x00\x00", 5);  // 8-bit grey, deflate, no filter, no interlace
+  pngChunk(out, "IHDR", ihdr);
+  pngChunk(out, "IDAT", z);
+  pngChunk(out, "IEND", std::string());
+  std::ofstream f(fileName, std::ios::binary);
+  if (!f) MSH_THROW("cannot open " + fileName + " for writing");
+  f.write(out.data(), (std::streamsize)out.size());
+  if (!f) MSH_THROW("error writing " + fileName);
 }
 
 }  // namespace mshost

@@ -18,6 +18,11 @@ struct Image {
 Image loadImage(const std::string &fileName);
 // PGM_DOUBLE_BINARY (P9), the type the reference writes deformations and reconstructions with.
 void saveP9(const std::string &fileName, const double *data, int nx, int ny, const char *comment = nullptr);
+// 8-bit grey PNG of the array scaled from its value range [min, max] to [0, 255] by truncation, as
+// DefaultArraySaver(true, ...) writes the reconstruction previews (quocTimestepSaver.h:417-441 with
+// setOverflowHandlingToCurrentValueRange; CLIP_THEN_SCALE in Vector::createOverflowHandledData,
+// vec.cpp:240-252; PNGSaveInterface::loadDataFromScalarArray2d, pngInterface.cpp:136-159).
+void savePNGScaled(const std::string &fileName, const double *data, int nx, int ny);
 
 // Component-wise load/save of a displacement field from a "%d" file-name mask
 // (qc::MultiArray::load / save, quoc/multiArray.h:272-306).

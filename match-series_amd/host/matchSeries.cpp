@@ -148,6 +148,9 @@ void writeReconstruction(const std::string &dir, const std::vector<double> &mean
   saveP9(dir + "average.q2bz", mean.data(), N, N);
   saveP9(dir + "median.q2bz", median.data(), N, N);
   saveP9(dir + "numSamples.q2bz", count.data(), N, N);
+  savePNGScaled(dir + "average.png", mean.data(), N, N);
+  savePNGScaled(dir + "median.png", median.data(), N, N);
+  savePNGScaled(dir + "numSamples.png", count.data(), N, N);
 }
 
 // ACTION 1: SeriesMatching::averageSeries (matchSeries.h:353-451) from saved deformations.

@@ -1,6 +1,8 @@
 // msgio — array file utility over the reference's formats (cf. the reference's image converters):
 //   msgio convert <in> <out>     read any supported array (P2/P5/P7/P8/P9/Pa/Pb, optionally .bz2/.q2bz)
 //                                and write it as P9 (float64), bzip2-compressed when <out> ends in .bz2/.q2bz
+//   msgio png <in> <out.png>     8-bit grey preview scaled from the value range (the reference's
+//                                average/median/numSamples .png)
 //   msgio info <in>              print "W H min max mean"
 #include <algorithm>
 #include <cstdio>
@@ -17,6 +19,11 @@ int main(int argc, char **argv) {
       mshost::saveP9(argv[3], im.data.data(), im.nx, im.ny);
       return 0;
     }
+    if (argc >= 4 && std::string(argv[1]) == "png") {
+      const mshost::Image im = mshost::loadImage(argv[2]);
+      mshost::savePNGScaled(argv[3], im.data.data(), im.nx, im.ny);
+      return 0;
+    }
     if (argc >= 3 && std::string(argv[1]) == "info") {
       const mshost::Image im = mshost::loadImage(argv[2]);
       const auto mm = std::minmax_element(im.data.begin(), im.data.end());
@@ -24,7 +31,7 @@ int main(int argc, char **argv) {
       printf("%d %d %.17g %.17g %.17g\n", im.nx, im.ny, *mm.first, *mm.second, mean);
       return 0;
     }
-    std::cerr << "usage: msgio convert <in> <out> | msgio info <in>\n";
+    std::cerr << "usage: msgio convert <in> <out> | msgio png <in> <out.png> | msgio info <in>\n";
     return 2;
   } catch (mshost::Exception &e) {
     e.dump(std::cerr);

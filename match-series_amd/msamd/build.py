@@ -42,14 +42,15 @@ def build_cli():
            "-idirafter", "/opt/conda/include",  # bzlib.h (the library itself is the system libbz2)
            "-D__HIP_PLATFORM_AMD__"]
     bz2 = "/usr/lib/x86_64-linux-gnu/libbz2.so.1.0"
+    zlib = "/usr/lib/x86_64-linux-gnu/libz.so.1"
     outs = []
     out = os.path.join(PKG, "bin", "matchSeries")
     run(["g++", "-std=c++17", "-O2", "-Wall"] + inc + common + [os.path.join(host, "matchSeries.cpp"), "-o", out,
          "-L", libdir, "-lmsgpu", "-Wl,-rpath,$ORIGIN/../lib", "-L/opt/rocm/lib", "-lamdhip64",
-         "-Wl,-rpath,/opt/rocm/lib", bz2])
+         "-Wl,-rpath,/opt/rocm/lib", bz2, zlib])
     outs.append(out)
     out = os.path.join(PKG, "bin", "msgio")
-    run(["g++", "-std=c++17", "-O2", "-Wall"] + inc + common + [os.path.join(host, "msgio.cpp"), "-o", out, bz2])
+    run(["g++", "-std=c++17", "-O2", "-Wall"] + inc + common + [os.path.join(host, "msgio.cpp"), "-o", out, bz2, zlib])
     outs.append(out)
     return outs
 

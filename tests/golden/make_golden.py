@@ -21,7 +21,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "match-series_amd"))
+sys.path.insert(0, HERE)
 from msamd import synth  # noqa: E402
+from pngread import read_png_gray8  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref")
 HARNESS = os.path.join(REF, "ref_harness")
@@ -190,6 +192,7 @@ def run_e2e(frames, par, wd):
                                                      for c in range(2)]) for i in range(nt)])
         for nm in ("average", "median", "numSamples"):
             e2e[f"s{st_}_{nm}"] = read_p9(os.path.join(od, f"stage{st_}", f"{nm}.q2bz"))
+            e2e[f"s{st_}_{nm}_png"] = read_png_gray8(os.path.join(od, f"stage{st_}", f"{nm}.png"))
         en = np.loadtxt(os.path.join(od, f"stage{st_}", "energies.txt"))
         e2e[f"s{st_}_energies"] = en[:, 1]
         # per-level records: deformation_<LL> and energy_<LL>.txt of every solve directory

@@ -4,6 +4,8 @@ import bz2
 import os
 import subprocess
 
+import sys
+
 import numpy as np
 import pytest
 
@@ -12,6 +14,8 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "match-series_amd", "bin", "matchSeries")
 GOLD = os.path.join(ROOT, "tests", "golden")
+sys.path.insert(0, GOLD)
+from pngread import read_png_gray8  # noqa: E402
 
 
 def read_p9(path):
@@ -59,6 +63,12 @@ def test_matchseries_two_stages(tmp_path, fixture, precision, tol_s1, tol_s2):
             assert np.abs(d - g[f"s{st}_deformations"][i]).max() * (N - 1) < tol, (st, i)
         cnt = read_p9(str(out / f"stage{st}" / "numSamples.q2bz"))
         assert np.abs(cnt - g[f"s{st}_numSamples"]).sum() <= (0 if st == 1 and precision == 64 else 8)
+        # PNG previews: the same 8-bit scaling of the value range; a pixel may differ by one step
+        # where the data differ below the reference's rounding
+        for nm in ("average", "median", "numSamples"):
+            png = read_png_gray8(str(out / f"stage{st}" / f"{nm}.png")).astype(int)
+            diff = np.abs(png - g[f"s{st}_{nm}_png"].astype(int))
+            assert diff.max() <= 1 and (diff > 0).mean() < 0.01, nm
         med = read_p9(str(out / f"stage{st}" / "median.q2bz"))
         ok = cnt == g[f"s{st}_numSamples"]
         assert np.abs(med - g[f"s{st}_median"])[ok].max() < (1e-6 if st == 1 and precision == 64 else 1e-2)

@@ -111,3 +111,22 @@ def test_par_grammar_accepts_comments_quotes_lists(tmp_path):
     r = cli(tmp_path, text, strict=True)
     # parsing succeeds; the run then stops at the unsupported model
     assert r.returncode == 1 and "deformationModel 5" in r.stderr
+
+
+@pytest.mark.parametrize("name", ["median", "average", "numSamples"])
+def test_png_preview_matches_the_reference(tmp_path, name):
+    """The reconstruction preview: the reference's own PNG of its stage-1 output (fixture), rebuilt from
+    the same array."""
+    import sys
+    sys.path.insert(0, GOLD)
+    from pngread import read_png_gray8
+    with np.load(os.path.join(GOLD, "e2e64.npz")) as z:
+        arr, png = z[f"s1_{name}"], z[f"s1_{name}_png"]
+    src = str(tmp_path / "a.dat")
+    with open(src, "wb") as f:
+        f.write(f"P9\n{arr.shape[1]} {arr.shape[0]}\n1\n".encode() + np.ascontiguousarray(arr, "<f8").tobytes())
+    out = str(tmp_path / "a.png")
+    r = msgio("png", src, out)
+    assert r.returncode == 0, r.stderr
+    np.testing.assert_array_equal(read_png_gray8(out), png)
+
