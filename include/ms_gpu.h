@@ -188,6 +188,11 @@ typedef struct {
   int average_ext_mean;               /* "deformForAverageExtendedWithMean": outside the      */
                                       /* domain a frame takes its own mean instead of no sample*/
   int average_nearest;                /* "deformForAverageUsingNearestNeighborInterpolation"  */
+  int reduce_deformations;            /* "reduceDeformations": after the matching, the        */
+  int reduce_max_iter;                /* deformation reducing the consistency energy of all   */
+  double reduce_stop_eps;             /* frames is found and applied first when averaging     */
+  const void *pair0_init;             /* NULL or a device field: template 0's starting        */
+                                      /* deformation (stage > 1 with reduceDeformations)      */
 } msg_series_params;
 
 /* Optional per-level records of msg_match_series: what the reference writes after every level of
@@ -209,10 +214,18 @@ typedef struct {
  * pairwise_in: NULL or n device fields; entry i >= 1 replaces template i's pairwise solve
  * ("reuseStage1Results"). Outputs (device, dtype): deformations (n fields, those used for the
  * average), pairwise_out (n fields or NULL), mean/count/median (one image each); energies: host,
- * n values; levels: NULL or the per-level records above. */
+ * n values; levels: NULL or the per-level records above; reduced: NULL or one device field, the
+ * reduced deformation (reduce_deformations). */
 int msg_match_series(const msg_series_params *p, int dtype, const void *templates, int n, const void *reference,
                      const void *pairwise_in, void *deformations, void *pairwise_out, double *energies,
-                     void *mean, void *count, void *median, const msg_series_levels *levels);
+                     void *mean, void *count, void *median, const msg_series_levels *levels, void *reduced);
+/* SeriesMatching::reduceDeformations (matchSeries.h:678-704): from zero, the displacement phi
+ * minimising sum_i 1/2 int |psi_i(x + phi(x)) + phi(x)|^2 (ConsistencyEnergyOp and its variation,
+ * AmbrosioTortorelli.h:687-800) by H1GradientDescent with filter width sqrt(10 h), nonlinear CG and
+ * gradient-norm stopping (stop_eps; the reference's default 5e-7). deformations: n device fields at
+ * level `level`; phi: device output field. energy/iterations: host outputs (may be NULL). */
+int msg_reduce_deformations(int dtype, const void *deformations, int n, int level, int max_iter, double stop_eps,
+                            void *phi, double *energy, int *iterations);
 /* msg_match_series keeps its device contexts for reuse by later calls of the same shape (no
  * counterpart in the reference); this frees them. */
 int msg_release_cache(void);

@@ -226,15 +226,19 @@ size_t energy_part_size(int n, int npairs) {
   return (size_t)npairs * (msg::MAX_CAND * 4 + 2) * energy_geom(n).nblk;
 }
 
+// The images of the data term: R, T (SSD, NCC) or, for DT_CONS, the npsi deformations in T.
 template <typename S>
 int launch_energy(const S *R, const S *T, const S *phi, const S *dcg, const S *dgrad, int n, const PairState *st,
                   double *part, int npairs, hipStream_t s, int data_term, double bytes = 0,
-                  const msg::Tail &tail = msg::Tail{}) {
+                  const msg::Tail &tail = msg::Tail{}, int npsi = 0) {
   ProfScope ps(KC_ENERGY, s, bytes);
   const EnergyGeom g = energy_geom(n);
   const double h = 1.0 / (double)(n - 1);
   const int width = (int)((n - 1) * h);
-  if (data_term == msg::DT_NCC)
+  if (data_term == msg::DT_CONS)
+    hipLaunchKernelGGL((msg::k_energy_cons<S>), dim3(g.nblk * npairs), dim3(256), 0, s, T, npsi, phi, dcg, dgrad, n, h,
+                       width, st, part, g.ntx, g.nty, npairs, tail);
+  else if (data_term == msg::DT_NCC)
     hipLaunchKernelGGL((msg::k_energy_ncc<S>), dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, dcg, dgrad, n, h,
                        st, part, g.ntx, g.nty, npairs, tail);
   else
@@ -246,17 +250,20 @@ int launch_energy(const S *R, const S *T, const S *phi, const S *dcg, const S *d
 
 template <typename S>
 int launch_grad(const S *R, const S *T, const S *phi, S *DE, int n, const PairState *st, double lambda, int npairs,
-                hipStream_t s, int data_term) {
+                hipStream_t s, int data_term, int npsi = 0) {
   ProfScope ps(KC_GRAD, s, 6.0 * n * n * sizeof(S) * npairs);
   const int ntx = cdiv(n, msg::GR_TX), nty = cdiv(n, msg::GR_TY);
   const double h = 1.0 / (double)(n - 1);
   const int width = (int)((n - 1) * h);
-  if (data_term == msg::DT_NCC)
+  if (data_term == msg::DT_CONS)
+    hipLaunchKernelGGL((msg::k_grad<S, msg::DT_CONS>), dim3(ntx * nty * npairs), dim3(256), 0, s, R, T, phi, DE, n, h,
+                       width, st, lambda, ntx, nty, npairs, npsi);
+  else if (data_term == msg::DT_NCC)
     hipLaunchKernelGGL((msg::k_grad<S, msg::DT_NCC>), dim3(ntx * nty * npairs), dim3(256), 0, s, R, T, phi, DE, n, h,
-                       width, st, lambda, ntx, nty, npairs);
+                       width, st, lambda, ntx, nty, npairs, 0);
   else
     hipLaunchKernelGGL((msg::k_grad<S, msg::DT_SSD>), dim3(ntx * nty * npairs), dim3(256), 0, s, R, T, phi, DE, n, h,
-                       width, st, lambda, ntx, nty, npairs);
+                       width, st, lambda, ntx, nty, npairs, 0);
   HIPCHK(hipGetLastError());
   return MSG_OK;
 }
@@ -389,6 +396,12 @@ struct msg_ctx {
   std::vector<GraphEntry> graphs;
   std::vector<double> level_start;  // per pair: E and lambda*E_reg at the start of the last level solve
   int data_term = msg::DT_SSD;
+  // reduceDeformations: the deformations (device, npsi fields at level L) the consistency energy reads,
+  // and the descent configuration of plain H1GradientDescent
+  const void *psi = nullptr;
+  int npsi = 0;
+  int sigma_fixed = 0, grad_stop = 0;
+  double sigma0 = 1.0;
 };
 
 namespace {
@@ -466,7 +479,9 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
                      int *iterations, msg_iter_trace *trace, int trace_cap) {
   const int n = 1 << l, B = c->B;
   hipStream_t s = c->stream;
-  const S *R = (const S *)c->R[l], *T = (const S *)c->T[l];
+  const bool cons = c->data_term == msg::DT_CONS;  // the consistency energy reads the deformations
+  const S *R = cons ? (const S *)c->psi : (const S *)c->R[l], *T = cons ? (const S *)c->psi : (const S *)c->T[l];
+  const int npsi = c->npsi;
   S *phi = (S *)c->phi[l];
   S *DE = (S *)c->DE, *z = (S *)c->z, *d = (S *)c->d;
   const MGBufs<S> mg = ctx_mg<S>(c);
@@ -501,10 +516,13 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
   a.trace_cap = tcap;
   a.data_term = c->data_term;
   a.npairs = B;
+  a.sigma_fixed = c->sigma_fixed;
+  a.sigma0 = c->sigma0;
+  a.grad_stop = c->grad_stop;
 
   hipLaunchKernelGGL(k_state_init, dim3(cdiv(B, 64)), dim3(64), 0, s, c->st, B, lambda);
   HIPCHK(hipGetLastError());
-  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term));
+  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, msg::Tail{}, npsi));
   a.phase = msg::PH_LEVEL_INIT;
   hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
   HIPCHK(hipGetLastError());
@@ -512,7 +530,7 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
   bool any_active = max_iter > 0;
   int nactive = max_iter > 0 ? B : 0;
   while (any_active) {
-    CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s, c->data_term));
+    CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s, c->data_term, npsi));
     CHK(launch_vcycle<S>(DE, 1, z, l, mg, c->st, 1.0, 2 * B, c->part_n, s));
     a.phase = msg::PH_BETA;
     hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
@@ -527,7 +545,7 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
     int rounds = 0;
     int nsearch = nactive;  // pairs evaluating in this round (upper bound for the first round)
     while (searching) {
-      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term));
+      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, msg::Tail{}, npsi));
       a.phase = msg::PH_ARMIJO_STEP;
       hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d,
                          tr);
@@ -592,7 +610,9 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
                    int *iterations, msg_iter_trace *trace, int trace_cap) {
   const int n = 1 << l, B = c->B;
   hipStream_t s = c->stream;
-  const S *R = (const S *)c->R[l], *T = (const S *)c->T[l];
+  const bool cons = c->data_term == msg::DT_CONS;  // the consistency energy reads the deformations
+  const S *R = cons ? (const S *)c->psi : (const S *)c->R[l], *T = cons ? (const S *)c->psi : (const S *)c->T[l];
+  const int npsi = c->npsi;
   S *phi = (S *)c->phi[l];
   S *DE = (S *)c->DE, *z = (S *)c->z, *d = (S *)c->d;
   const MGBufs<S> mg = ctx_mg<S>(c);
@@ -630,6 +650,9 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
   a.trace_cap = tcap;
   a.data_term = c->data_term;
   a.npairs = B;
+  a.sigma_fixed = c->sigma_fixed;
+  a.sigma0 = c->sigma0;
+  a.grad_stop = c->grad_stop;
   auto mk = [&](int phase, int slot) {
     msg::Tail t;
     t.st = c->st;
@@ -648,16 +671,16 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
 
   hipLaunchKernelGGL(k_state_init, dim3(cdiv(B, 64)), dim3(64), 0, s, c->st, B, lambda);
   HIPCHK(hipGetLastError());
-  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, t_level));
+  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, t_level, npsi));
 
   auto iteration = [&]() -> int {
-    CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s, c->data_term));
+    CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s, c->data_term, npsi));
     CHK(launch_vcycle<S>(DE, 1, z, l, mg, c->st, 1.0, 2 * B, c->part_n, s, t_beta));
     hipLaunchKernelGGL((msg::k_ncg<S>), dim3(nblk_ncg * 2 * B), dim3(256), 0, s, z, d, DE, n, c->st, c->part_c, nblk_ncg,
                        2 * B, t_init);
     HIPCHK(hipGetLastError());
     for (int r = 0; r < Rr; ++r)
-      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, t_step));
+      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, t_step, npsi));
     hipLaunchKernelGGL((msg::k_update<S>), dim3(nblk_upd * B), dim3(256), 0, s, phi, d, z, n, c->st, nblk_upd, B,
                        c->counters + 3 * B);
     HIPCHK(hipGetLastError());
@@ -878,7 +901,7 @@ int msg_ctx_create(msg_ctx **out, int max_level, int npairs, int dtype) {
   (void)eg;
   A((void **)&c->part_e, sizeof(double) * energy_part_size(N, (int)B));
   A((void **)&c->part_n, sizeof(double) * B * 2 * std::max(1, mg_norm_blocks(max_level)));
-  A((void **)&c->part_c, sizeof(double) * B * 2 * 2 * cdiv(N * N, 256 * msg::NCG_PER_THREAD));
+  A((void **)&c->part_c, sizeof(double) * B * 2 * msg::NCG_TERMS * cdiv(N * N, 256 * msg::NCG_PER_THREAD));
   A((void **)&c->st, sizeof(PairState) * B);
   A((void **)&c->flags_d, sizeof(int) * B);
   A((void **)&c->counters, sizeof(unsigned) * 4 * B);
@@ -1408,6 +1431,49 @@ struct CtxGuard {
   ~CtxGuard() { ctx_release(c); }
 };
 
+// SeriesMatching::reduceDeformations (matchSeries.h:678-704) on n device fields at level L: a one-pair
+// context whose data term is the consistency energy of the n deformations, plain H1GradientDescent
+// (filter width sqrt(10 h) fixed, gradientDescent.h:639-650) and gradient-norm stopping.
+template <typename S>
+int reduce_impl(const void *defs, int n, int L, int max_iter, double stop_eps, void *phi_out, double *energy,
+                int *iterations) {
+  const int N = 1 << L;
+  const size_t nn = (size_t)N * N;
+  const int dt = sizeof(S) == 8 ? MSG_F64 : MSG_F32;
+  msg_ctx *c = nullptr;
+  CHK(ctx_acquire(L, 1, dt, &c));
+  CtxGuard g{c};
+  auto reset = [c]() {  // captured graphs hold the deformations' address: never keep them
+    for (auto &ge : c->graphs) (void)hipGraphExecDestroy(ge.exec);
+    c->graphs.clear();
+  };
+  reset();
+  c->data_term = msg::DT_CONS;
+  c->psi = defs;
+  c->npsi = n;
+  c->sigma_fixed = 1;
+  c->sigma0 = std::sqrt(10.0 / (double)(N - 1));
+  c->grad_stop = 1;
+  HIPCHK(hipMemsetAsync(c->phi[L], 0, sizeof(S) * 2 * nn, c->stream));
+  double e = 0;
+  int it = 0;
+  const int rc = solve_level_any<S>(c, L, 0.0, max_iter, stop_eps, 0, &e, &it, nullptr, 0);
+  if (rc == MSG_OK && phi_out)
+    (void)hipMemcpyAsync(phi_out, c->phi[L], sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, c->stream);
+  const hipError_t es = hipStreamSynchronize(c->stream);
+  reset();
+  c->data_term = msg::DT_SSD;
+  c->psi = nullptr;
+  c->npsi = 0;
+  c->sigma_fixed = c->grad_stop = 0;
+  c->sigma0 = 1.0;
+  CHK(rc);
+  HIPCHK(es);
+  if (energy) *energy = e;
+  if (iterations) *iterations = it;
+  return MSG_OK;
+}
+
 // One background host thread for work that overlaps the caller's: msg_match_series runs its
 // pairwise batches there while the calling thread runs the chain. Never destroyed (a static
 // condition variable with a blocked waiter must not be destroyed at exit).
@@ -1459,7 +1525,7 @@ struct Progress {
 // (matchSeries.h:136-176). Writes the fields to pw and the energies.
 template <typename S>
 int pairwise_batch(const msg_series_params *p, const msg_solve_params &sp, const S *tpl, const S *ref, int i0,
-                   int i1, S *pw, double *energies, const msg_series_levels *lv) {
+                   int i1, S *pw, double *energies, const msg_series_levels *lv, const S *init0) {
   const int L = p->precision_level;
   const int N = 1 << L;
   const size_t nn = (size_t)N * N;
@@ -1491,6 +1557,10 @@ int pairwise_batch(const msg_series_params *p, const msg_solve_params &sp, const
   // holds its previous solution)
   for (int l = 0; l <= L; ++l)
     HIPCHK(hipMemsetAsync(pc->phi[l], 0, sizeof(S) * ((size_t)2 << (2 * l)) * nb, s));
+  if (init0 && i0 == 0) {  // template 0 starts from the given deformation (matchSeries.h:123-135)
+    HIPCHK(hipMemcpyAsync(pc->phi[L], init0, sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, s));
+    CHK(rebuild_pyramid<S>(pc, pc->phi, 0, 2));
+  }
   LevelOut lo;
   lo.lv = lv;
   lo.rec0 = (long long)i0 * (p->stop_level - p->start_level + 1);
@@ -1504,14 +1574,14 @@ int pairwise_batch(const msg_series_params *p, const msg_solve_params &sp, const
 // Runs the batches [k*b, min((k+1)*b, npw)) in order, publishing progress after each.
 template <typename S>
 void pairwise_all(const msg_series_params *p, const msg_solve_params &sp, const S *tpl, const S *ref, int npw,
-                  int b, S *pw, double *energies, const msg_series_levels *lv, Progress *pr) {
+                  int b, S *pw, double *energies, const msg_series_levels *lv, const S *init0, Progress *pr) {
   int rc = MSG_OK;
   for (int i0 = 0; i0 < npw && rc == MSG_OK; i0 += b) {
     {
       std::lock_guard<std::mutex> g(pr->mu);
       if (pr->abort) break;
     }
-    rc = pairwise_batch<S>(p, sp, tpl, ref, i0, std::min(i0 + b, npw), pw, energies, lv);
+    rc = pairwise_batch<S>(p, sp, tpl, ref, i0, std::min(i0 + b, npw), pw, energies, lv, init0);
     std::lock_guard<std::mutex> g(pr->mu);
     if (rc == MSG_OK) pr->done = std::min(i0 + b, npw);
     else { pr->rc = rc; pr->err = g_err; }
@@ -1525,7 +1595,7 @@ void pairwise_all(const msg_series_params *p, const msg_solve_params &sp, const 
 template <typename S>
 int match_series_impl(const msg_series_params *p, const void *templates, int n, const void *reference,
                       const void *pairwise_in, void *deformations, void *pairwise_out, double *energies, void *mean,
-                      void *count, void *median, const msg_series_levels *lv) {
+                      void *count, void *median, const msg_series_levels *lv, void *reduced) {
   const int L = p->precision_level;
   const int N = 1 << L;
   const size_t nn = (size_t)N * N;
@@ -1541,6 +1611,7 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
   sp.spec = p->spec;
   const bool reuse = pairwise_in != nullptr;
   const int npw = reuse ? 1 : n;  // pairwise solves to run (pairs 0..npw-1)
+  const S *init0 = (const S *)p->pair0_init;
   const bool chain = !p->dont_accumulate;
   // Batches: with a chain to overlap, about four (the chain starts when the first is done);
   // otherwise one.
@@ -1577,10 +1648,10 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
         pr.cv.notify_all();
         return;
       }
-      pairwise_all<S>(p, sp, tpl, ref, npw, b, pwp, energies, lv, &pr);
+      pairwise_all<S>(p, sp, tpl, ref, npw, b, pwp, energies, lv, init0, &pr);
     });
   } else {
-    pairwise_all<S>(p, sp, tpl, ref, npw, b, pwp, energies, lv, &pr);
+    pairwise_all<S>(p, sp, tpl, ref, npw, b, pwp, energies, lv, init0, &pr);
     if (pr.rc != MSG_OK) return fail(pr.rc, pr.err);
   }
   // Waits (on the calling thread) until pairwise solve i is done; false after a batch failed.
@@ -1656,20 +1727,39 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
   if (n > msg::ST_MAXF) return fail(MSG_ERR_ARG, "series longer than 128 frames are not supported by the median kernel");
   DevBuf stack;
   HIPCHK(hipMalloc(&stack.p, sizeof(S) * nn * std::max(n, 1)));
-  // averageSeries (matchSeries.h:353-451): DeformImage of each raw template, +Inf (no sample) or the
-  // template's own mean outside the domain, optionally nearest-neighbour
+  // reduceDeformations (matchSeries.h:565-570, 678-704): the reduced deformation is applied to every
+  // template first (DeformImage, +Inf outside), then the template's own deformation (L386-392)
+  DevBuf red, pre;
+  const S *src = tpl;
+  if (p->reduce_deformations) {
+    void *rp = reduced;
+    if (!rp) {
+      HIPCHK(hipMalloc(&red.p, sizeof(S) * 2 * nn));
+      rp = red.p;
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    CHK(reduce_impl<S>(defs, n, L, p->reduce_max_iter, p->reduce_stop_eps, rp, nullptr, nullptr));
+    HIPCHK(hipMalloc(&pre.p, sizeof(S) * nn * n));
+    const size_t cnt = nn * n;
+    hipLaunchKernelGGL((msg::k_warp<S>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, tpl, (const S *)rp,
+                       (S *)pre.p, N, n, 1, 0, (double)INFINITY, 0);
+    HIPCHK(hipGetLastError());
+    src = (const S *)pre.p;
+  }
+  // averageSeries (matchSeries.h:353-451): DeformImage of each template, +Inf (no sample) or the
+  // image's own mean outside the domain, optionally nearest-neighbour
   const int nearest = p->average_nearest ? 1 : 0;
   if (p->average_ext_mean) {
     for (int i = 0; i < n; ++i) {
       double ext = 0;
-      CHK(mean_value<S>(tpl + (size_t)i * nn, (int)nn, &ext, s));
-      hipLaunchKernelGGL((msg::k_warp<S>), dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, tpl + (size_t)i * nn,
+      CHK(mean_value<S>(src + (size_t)i * nn, (int)nn, &ext, s));
+      hipLaunchKernelGGL((msg::k_warp<S>), dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, src + (size_t)i * nn,
                          (const S *)defs + (size_t)i * 2 * nn, (S *)stack.p + (size_t)i * nn, N, 1, 0, 0, ext, nearest);
       HIPCHK(hipGetLastError());
     }
   } else {
     const size_t cnt = nn * n;
-    hipLaunchKernelGGL((msg::k_warp<S>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, tpl, (const S *)defs,
+    hipLaunchKernelGGL((msg::k_warp<S>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, src, (const S *)defs,
                        (S *)stack.p, N, n, 0, 0, (double)INFINITY, nearest);
     HIPCHK(hipGetLastError());
   }
@@ -1718,7 +1808,7 @@ int msg_normalize(int dtype, void *img, int n, int nimg, double pct, int min_to_
 
 int msg_match_series(const msg_series_params *p, int dtype, const void *templates, int n, const void *reference,
                      const void *pairwise_in, void *deformations, void *pairwise_out, double *energies, void *mean,
-                     void *count, void *median, const msg_series_levels *levels) {
+                     void *count, void *median, const msg_series_levels *levels, void *reduced) {
   if (!p || !templates || !reference || !deformations || !energies || !mean || !count || !median || n < 1)
     return fail(MSG_ERR_ARG, "invalid argument");
   CHK(check_dtype(dtype));
@@ -1731,9 +1821,18 @@ int msg_match_series(const msg_series_params *p, int dtype, const void *template
     return fail(MSG_ERR_ARG, "invalid refineStartLevel/refineStopLevel");
   if (dtype == MSG_F64)
     return match_series_impl<double>(p, templates, n, reference, pairwise_in, deformations, pairwise_out, energies,
-                                     mean, count, median, levels);
+                                     mean, count, median, levels, reduced);
   return match_series_impl<float>(p, templates, n, reference, pairwise_in, deformations, pairwise_out, energies, mean,
-                                  count, median, levels);
+                                  count, median, levels, reduced);
+}
+
+int msg_reduce_deformations(int dtype, const void *deformations, int n, int level, int max_iter, double stop_eps,
+                            void *phi, double *energy, int *iterations) {
+  CHK(check_dtype(dtype));
+  if (!deformations || !phi || n < 1) return fail(MSG_ERR_ARG, "invalid argument");
+  if (level < 1 || level > MAXL) return fail(MSG_ERR_ARG, "level out of range");
+  if (dtype == MSG_F64) return reduce_impl<double>(deformations, n, level, max_iter, stop_eps, phi, energy, iterations);
+  return reduce_impl<float>(deformations, n, level, max_iter, stop_eps, phi, energy, iterations);
 }
 
 }  // extern "C"

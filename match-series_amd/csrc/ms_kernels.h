@@ -33,9 +33,13 @@ struct CtrlArgs {
   int max_iter;
   int spec;          // candidates per energy pass (1..MAX_CAND)
   int trace_cap;
-  int data_term;     // DT_SSD or DT_NCC
+  int data_term;     // DT_SSD, DT_NCC or DT_CONS
   int npairs;        // pairs of the batch (NCC: the reference statistics follow the candidates' partials)
+  int sigma_fixed;   // 1: filter width fixed at sigma0 (H1GradientDescent, gradientDescent.h:639-650)
+  double sigma0;
+  int grad_stop;     // 1: USE_GRADIENT_BASED_STOPPING (gradientDescent.h:416)
 };
+constexpr int NCG_TERMS = 3;  // per field and block: <DE,d>, <DE,z>, <DE,DE>
 
 // Where the energy pass stores its partial sums. SSD: 2 terms per candidate (E_SSD, E_reg). NCC: 4
 // per candidate (1'Mt, t'Mt, t'Mr, E_reg) and, after all candidates of all pairs, 2 per pair
@@ -155,8 +159,11 @@ __device__ void finish_iteration(PairState &s, double t, IterTrace *trace, int t
   s.ereg_new = Rn;
   s.tau = t;
   const double sigma_used = s.sigma;
-  bool stopping = ((energy - En) > s.stop_eps) && (s.iter < s.max_iter);
-  if ((0.5 * s.sigma > 0.0001) && ((energy - En) < 0.00001)) {
+  // the norm is of DE at the position the iteration started from (gradientDescent.h:416)
+  const double crit = s.grad_stop ? sqrt(s.dd) : energy - En;
+  bool stopping = (crit > s.stop_eps) && (s.iter < s.max_iter);
+  if (s.sigma_fixed && t == 0) stopping = false;  // L418-422
+  if (!s.sigma_fixed && (0.5 * s.sigma > 0.0001) && ((energy - En) < 0.00001)) {
     s.sigma *= 0.5;
     s.tau = 1.;
     stopping = (s.sigma > 0.0001) && (s.iter < s.max_iter);
@@ -319,7 +326,10 @@ __device__ void control_pair(int pair, PairState *__restrict__ st, const double 
         s.energy = s.energy_new = s.energy_start = s.cand_E[0];
         s.ereg_new = s.ereg_start = s.cand_R[0];
         s.tau = 1.0;
-        s.sigma = 1.0;
+        s.sigma = a.sigma_fixed ? a.sigma0 : 1.0;
+        s.sigma_fixed = a.sigma_fixed;
+        s.grad_stop = a.grad_stop;
+        s.dd = 0;
         s.iter = 0;
         s.max_iter = a.max_iter;
         s.stop_eps = a.stop_eps;
@@ -350,10 +360,15 @@ __device__ void control_pair(int pair, PairState *__restrict__ st, const double 
     }
   } else if (a.phase == PH_ARMIJO_INIT) {
     if (!participates) return;
-    const double d0 = reduce_fixed(part_c + ((size_t)(2 * pair) * 2 + 0) * a.nblk_ncg, a.nblk_ncg, red);
-    const double d1 = reduce_fixed(part_c + ((size_t)(2 * pair + 1) * 2 + 0) * a.nblk_ncg, a.nblk_ncg, red);
-    const double g0 = reduce_fixed(part_c + ((size_t)(2 * pair) * 2 + 1) * a.nblk_ncg, a.nblk_ncg, red);
-    const double g1 = reduce_fixed(part_c + ((size_t)(2 * pair + 1) * 2 + 1) * a.nblk_ncg, a.nblk_ncg, red);
+    const double d0 = reduce_fixed(part_c + ((size_t)(2 * pair) * NCG_TERMS + 0) * a.nblk_ncg, a.nblk_ncg, red);
+    const double d1 = reduce_fixed(part_c + ((size_t)(2 * pair + 1) * NCG_TERMS + 0) * a.nblk_ncg, a.nblk_ncg, red);
+    const double g0 = reduce_fixed(part_c + ((size_t)(2 * pair) * NCG_TERMS + 1) * a.nblk_ncg, a.nblk_ncg, red);
+    const double g1 = reduce_fixed(part_c + ((size_t)(2 * pair + 1) * NCG_TERMS + 1) * a.nblk_ncg, a.nblk_ncg, red);
+    double n0 = 0, n1 = 0;
+    if (a.grad_stop) {
+      n0 = reduce_fixed(part_c + ((size_t)(2 * pair) * NCG_TERMS + 2) * a.nblk_ncg, a.nblk_ncg, red);
+      n1 = reduce_fixed(part_c + ((size_t)(2 * pair + 1) * NCG_TERMS + 2) * a.nblk_ncg, a.nblk_ncg, red);
+    }
     if (threadIdx.x == 0) {
       double Df = 0, Dfz = 0;
       Df += d0;
@@ -362,6 +377,12 @@ __device__ void control_pair(int pair, PairState *__restrict__ st, const double 
       Dfz += g1;
       s.Df = Df;
       s.Dfz = Dfz;
+      if (a.grad_stop) {
+        double dd = 0;
+        dd += n0;
+        dd += n1;
+        s.dd = dd;
+      }
       s.olddir_zero = 0;  // d now holds this iteration's CG direction
       s.tau_first = 0;
       armijo_start(s, DIR_CG, Df, s.tau);
@@ -521,6 +542,94 @@ k_energy(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__
 }
 
 // ================================================================================================
+// Consistency energy pass of reduceDeformations (matchSeries.h:678-704): for up to MAX_CAND step
+// sizes, the sum over the n deformations psi_i (2 fields each) of ConsistencyEnergyOp
+// (AmbrosioTortorelli.h:687-735), 1/2 int |psi_i(x + phi(x)) + phi(x)|^2 at the clipped deformed quad
+// point. Partials as the SSD pass (term 1, the regulariser, is absent: 0).
+// ================================================================================================
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_energy_cons(const S *__restrict__ psi, int npsi, const S *__restrict__ phi, const S *__restrict__ dcg,
+              const S *__restrict__ dgrad, int n, double h, int width, const PairState *st, double *__restrict__ part,
+              int ntx, int nty, int npairs, Tail tail) {
+  const int tiles = ntx * nty;
+  const int wg = xcd_remap(blockIdx.x, tiles * npairs);
+  const int pair = wg / tiles, tile = wg - pair * tiles;
+  const int nc = st[pair].ncand;
+  if (nc == 0) return;  // uniform across the workgroup
+  const int dm = st[pair].dirmode;
+  __shared__ double red[4];
+
+  const size_t nn = (size_t)n * n;
+  const S *px = phi + pair * 2 * nn, *py = px + nn;
+  const S *dsel = (dm == DIR_CG) ? dcg : dgrad;
+  const S *dx = dsel + pair * 2 * nn, *dy = dx + nn;
+  const bool use_dir = (dm != DIR_NONE);
+
+  const int ex = (tile % ntx) * EN_TX + (threadIdx.x % EN_TX);
+  const int ey = (tile / ntx) * EN_TY + (threadIdx.x / EN_TX);
+  const bool valid = (ex < n - 1) && (ey < n - 1);
+
+  double acc[MAX_CAND];
+#pragma unroll
+  for (int k = 0; k < MAX_CAND; ++k) acc[k] = 0.0;
+
+  if (valid) {
+    double f0[4], f1[4], d0[4], d1[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int g = l2g(ex, ey, b, n);
+      f0[b] = ld(px, g);
+      f1[b] = ld(py, g);
+      d0[b] = use_dir ? ld(dx, g) : 0.0;
+      d1[b] = use_dir ? ld(dy, g) : 0.0;
+    }
+    const double vol = h * h;
+#pragma unroll
+    for (int k = 0; k < MAX_CAND; ++k) {
+      if (k < nc) {
+        const double tau = st[pair].cand[k];
+        double p0[4], p1[4];
+        for (int b = 0; b < 4; ++b) {
+          p0[b] = use_dir ? rnd<S>(f0[b] + tau * d0[b]) : f0[b];
+          p1[b] = use_dir ? rnd<S>(f1[b] + tau * d1[b]) : f1[b];
+        }
+        double a = 0.;
+        for (int q = 0; q < 4; ++q) {
+          double ox = 0., oy = 0.;
+          for (int b = 0; b < 4; ++b) {
+            ox += p0[b] * bq(b, q);
+            oy += p1[b] * bq(b, q);
+          }
+          const Hit hp = deform_qp(ex, ey, q, ox, oy, n, h, width);
+          double e = 0.;
+          for (int i = 0; i < npsi; ++i) {
+            const S *pi = psi + (size_t)i * 2 * nn;
+            double u4[4];
+            const double vx = eval_img(pi, hp, n, u4) + ox, vy = eval_img(pi + nn, hp, n, u4) + oy;
+            e += vx * vx + vy * vy;
+          }
+          a += 0.5 * e * 0.25;
+        }
+        acc[k] = a * vol;
+      }
+    }
+  }
+  const int nblk = tiles;
+#pragma unroll
+  for (int k = 0; k < MAX_CAND; ++k) {
+    if (k < nc) {
+      const double s = block_sum<256>(acc[k], red);
+      if (threadIdx.x == 0) {
+        part[((size_t)(pair * MAX_CAND + k) * 2 + 0) * nblk + tile] = s;
+        part[((size_t)(pair * MAX_CAND + k) * 2 + 1) * nblk + tile] = 0.0;
+      }
+    }
+  }
+  run_tail(tail, pair, tiles);
+}
+
+// ================================================================================================
 // NCC energy pass (deformationModel 0) for up to MAX_CAND step sizes per pair, element-centric.
 // NormalizedCrossCorrelationEnergy::computeEnergy (registration.h:1953-1964) in moments of the
 // assembled mass matrix: with t the nodal FastDeformImage of T at phi + tau d and r the reference,
@@ -633,7 +742,7 @@ constexpr int GR_EX = GR_TX + 1, GR_EY = GR_TY + 1;
 template <typename S, int DT>
 __global__ void __launch_bounds__(256)
 k_grad(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__ phi, S *__restrict__ DE, int n,
-       double h, int width, const PairState *st, double lambda_default, int ntx, int nty, int npairs) {
+       double h, int width, const PairState *st, double lambda_default, int ntx, int nty, int npairs, int npsi) {
   const int tiles = ntx * nty;
   const int wg = xcd_remap(blockIdx.x, tiles * npairs);
   const int pair = wg / tiles, tile = wg - pair * tiles;
@@ -660,7 +769,41 @@ k_grad(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__ p
       rn[b] = ld(Rp, g);
     }
     double nl[4][2];
-    if (DT == DT_NCC) {
+    if (DT == DT_CONS) {
+      // reduceDeformations: sum over the n deformations psi_i (passed as T, 2 fields each) of
+      // VariationOfConsistencyEnergyOp::getNonlinearity (AmbrosioTortorelli.h:755-799):
+      // (Dpsi_i + I)^T (psi_i + offset) at the clipped deformed quad point, zero per clipped axis
+      for (int q = 0; q < 4; ++q) {
+        double ox = 0., oy = 0.;
+        for (int b = 0; b < 4; ++b) {
+          ox += f0[b] * bq(b, q);
+          oy += f1[b] * bq(b, q);
+        }
+        const Hit hp = deform_qp(ex, ey, q, ox, oy, n, h, width);
+        double a0 = 0., a1 = 0.;
+        for (int k = 0; k < npsi; ++k) {
+          const S *pk = T + (size_t)k * 2 * nn;
+          double u4[4], w4[4];
+          double vx = eval_img(pk, hp, n, u4), vy = eval_img(pk + nn, hp, n, w4);
+          double j00 = 0., j01 = 0., j10 = 0., j11 = 0.;  // j[c][d] = d psi_c / d x_d
+          for (int b = 0; b < 4; ++b) {
+            const double gx = q1dx(b, hp.ly) / h, gy = q1dy(b, hp.lx) / h;
+            j00 += gx * u4[b];
+            j01 += gy * u4[b];
+            j10 += gx * w4[b];
+            j11 += gy * w4[b];
+          }
+          vx += ox;
+          vy += oy;
+          j00 += 1.;
+          j11 += 1.;
+          a0 += j00 * vx + j10 * vy;
+          a1 += j01 * vx + j11 * vy;
+        }
+        nl[q][0] = hp.inx ? a0 : 0;
+        nl[q][1] = hp.iny ? a1 : 0;
+      }
+    } else if (DT == DT_NCC) {
       // NormalizedCrossCorrelationEnergy::applyDerivative (registration.h:1974-1984): nodal weight
       // w = (Rn + E Tn) / v_t with Rn, Tn the normalised reference and warped template, then
       // CrossCorrelationForce (L1826-1862): -w(x_q) grad T at the unclipped deformed quad point,
@@ -1051,7 +1194,7 @@ k_ncg(const S *__restrict__ z, S *__restrict__ d, const S *__restrict__ DE, int 
   const size_t nn = (size_t)n * n;
   const S *zp = z + field * nn, *dep = DE + field * nn;
   S *dp = d + field * nn;
-  double sd = 0, sz = 0;
+  double sd = 0, sz = 0, sn = 0;
   const int base = blk * 256 * NCG_PER_THREAD;
   for (int k = 0; k < NCG_PER_THREAD; ++k) {
     const int i = base + k * 256 + threadIdx.x;
@@ -1063,13 +1206,16 @@ k_ncg(const S *__restrict__ z, S *__restrict__ d, const S *__restrict__ DE, int 
       const double de = ld(dep, i);
       sd += de * static_cast<double>(dv);
       sz += de * zv;
+      sn += de * de;
     }
   }
   const double a = block_sum<256>(sd, red);
   const double b = block_sum<256>(sz, red);
+  const double c = block_sum<256>(sn, red);
   if (threadIdx.x == 0) {
-    part[((size_t)field * 2 + 0) * nblk_field + blk] = a;
-    part[((size_t)field * 2 + 1) * nblk_field + blk] = b;
+    part[((size_t)field * NCG_TERMS + 0) * nblk_field + blk] = a;
+    part[((size_t)field * NCG_TERMS + 1) * nblk_field + blk] = b;
+    part[((size_t)field * NCG_TERMS + 2) * nblk_field + blk] = c;
   }
   run_tail(tail, pair, 2 * nblk_field);
 }

@@ -7,7 +7,8 @@ namespace msg {
 enum ArmijoPhase { AP_IDLE = 0, AP_FIRST = 1, AP_GROW = 2, AP_SHRINK = 3 };
 enum DirMode { DIR_CG = 0, DIR_GRAD = 1, DIR_NONE = 2 };
 enum { MAX_CAND = 4 };
-enum DataTerm { DT_SSD = 0, DT_NCC = 1 };  // deformationModel 22 / deformationModel 0
+// deformationModel 22 / deformationModel 0 / reduceDeformations' consistency energy
+enum DataTerm { DT_SSD = 0, DT_NCC = 1, DT_CONS = 2 };
 
 struct PairState {
   // gradient-flow state (GradientDescentBase::applySingle)
@@ -35,6 +36,9 @@ struct PairState {
   double last_D[2], last_M[2], last_V[2];
   double ncc_rm, ncc_rv;
   double ncc_d, ncc_m, ncc_v;
+  double dd;          // |DE|^2 at the current position (gradient-norm stopping)
+  int sigma_fixed;    // 1: filter width fixed (plain H1GradientDescent), 0: the automatic schedule
+  int grad_stop;      // 1: stop on |DE| <= stopEpsilon, 0: on the energy decrease
   double step_tau;    // tau of the update (0: no update)
   double tau_first;   // result of the CG-direction search this iteration (trace)
   int ncand;          // candidates to evaluate in the next energy pass (0: none)

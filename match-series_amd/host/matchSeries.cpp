@@ -84,7 +84,7 @@ std::string dirOf(const ParameterParser &p) { return p.getStringExpandTilde("sav
 
 // Options of the reference that change the computation but are not implemented here.
 void rejectUnsupported(const ParameterParser &p) {
-  const char *flags[] = {"reduceDeformations", "useCorrelationToInitTranslation", "reverseRolesInSeriesMatching",
+  const char *flags[] = {"useCorrelationToInitTranslation", "reverseRolesInSeriesMatching",
                          "calcInverseDeformation", "cropInput", "resizeInput", "calcIQ", "calcPrecision",
                          "performNoiseAnalysis", "saveNamedDeformedTemplates", "saveNamedDeformations"};
   for (const char *f : flags)
@@ -123,6 +123,10 @@ msg_series_params seriesParams(const ParameterParser &p) {
   s.data_term = p.getIntOrDefault("deformationModel", 0) == 22 ? MSG_SSD : MSG_NCC;
   s.average_ext_mean = p.checkAndGetBool("deformForAverageExtendedWithMean");
   s.average_nearest = p.checkAndGetBool("deformForAverageUsingNearestNeighborInterpolation");
+  s.reduce_deformations = p.checkAndGetBool("reduceDeformations");
+  s.reduce_max_iter = p.getIntOrDefault("maxGDIterations", 1000);  // SeriesMatching::reduceDeformations
+  s.reduce_stop_eps = p.getDoubleOrDefault("stopEpsilon", 5e-7);   // (matchSeries.h:697)
+  s.pair0_init = nullptr;
   return s;
 }
 
@@ -177,6 +181,22 @@ void averageOnly(const ParameterParser &p) {
     means[i] = s / nn;  // ScalarArray::getMeanValue
     if (i == n - 1) lastMean = means[i];
   }
+  if (p.checkAndGetBool("reduceDeformations")) {  // the reduced deformation first (matchSeries.h:381-392)
+    std::vector<double> red;
+    loadField(dir + "reduceDef_%d.dat.bz2", N, red);
+    DeviceBuffer redD(es * 2 * nn), pre(es * nn * n);
+    upload(redD.p, red.data(), 2 * nn, dt);
+    msgCheck(msg_warp(dt, img.p, redD.p, pre.p, N, n, 1, 0, INFINITY, 0, nullptr));
+    hipCheck(hipMemcpy(img.p, pre.p, es * nn * n, hipMemcpyDeviceToDevice), "copy");
+    if (extMean)  // the mean of the reduced-deformed image (+Inf where it has no sample)
+      for (int i = 0; i < n; ++i) {
+        std::vector<double> h(nn);
+        download(h.data(), (char *)img.p + (size_t)i * es * nn, nn, dt);
+        double s = 0;
+        for (double v : h) s += v;
+        means[i] = s / nn;
+      }
+  }
   if (extMean) {
     for (int i = 0; i < n; ++i)
       msgCheck(msg_warp(dt, (char *)img.p + (size_t)i * es * nn, (char *)defs.p + (size_t)i * es * 2 * nn,
@@ -201,7 +221,27 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
   const std::vector<std::string> files = createTemplateFileNameList(p);
   const int n = (int)files.size();
   if (n == 0) MSH_THROW("no template images");
-  const msg_series_params sp = seriesParams(p);
+  msg_series_params sp = seriesParams(p);
+
+  // reduceDeformations in a later stage: template 0 starts from the previous stage's reduced
+  // deformation, from stage 3 on composed with the previous stage's deformation of template 0
+  // (matchSeries.h:123-135)
+  std::unique_ptr<DeviceBuffer> init0;
+  if (sp.reduce_deformations && stage > 1) {
+    std::vector<double> red;
+    loadField(strprintf("%s../stage%d/reduceDef_%%d.dat.bz2", dir.c_str(), stage - 1), N, red);
+    init0.reset(new DeviceBuffer(es * 2 * nn));
+    upload(init0->p, red.data(), 2 * nn, dt);
+    if (stage > 2) {
+      std::vector<double> last;
+      loadField(strprintf("%s../stage%d/0/deformation_%02d_%%d.dat.bz2", dir.c_str(), stage - 1, L), N, last);
+      DeviceBuffer lastD(es * 2 * nn), redD(es * 2 * nn);
+      upload(lastD.p, last.data(), 2 * nn, dt);
+      upload(redD.p, red.data(), 2 * nn, dt);
+      msgCheck(msg_compose(dt, lastD.p, redD.p, init0->p, N, nullptr));  // start value: the reduced deformation
+    }
+    sp.pair0_init = init0->p;
+  }
 
   DeviceBuffer tpl(es * nn * n), ref(es * nn), defs(es * 2 * nn * n), pw(es * 2 * nn * n), mean(es * nn),
       cnt(es * nn), med(es * nn);
@@ -239,8 +279,9 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
   std::vector<int> lvIts(nrec);
   msg_series_levels lv = {lvFields.data(), lvIters.data(), lvStart.data(), lvIts.data()};
   const auto t0 = std::chrono::steady_clock::now();
+  DeviceBuffer reduced(es * 2 * nn);
   msgCheck(msg_match_series(&sp, dt, tpl.p, n, ref.p, pin ? pin->p : nullptr, defs.p, pw.p, energies.data(), mean.p,
-                            cnt.p, med.p, &lv));
+                            cnt.p, med.p, &lv, sp.reduce_deformations ? reduced.p : nullptr));
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::cerr << "Registered " << n << " templates on the GPU in " << secs << " s\n";
 
@@ -303,12 +344,16 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
     }
     energiesFile << i << " " << energies[i] << "\n";
   }
+  if (sp.reduce_deformations) {  // "<saveDirectory>reduceDef_<c>.dat.bz2" (matchSeries.h:701)
+    std::vector<double> hr(2 * nn);
+    download(hr.data(), reduced.p, 2 * nn, dt);
+    saveField(dir + "reduceDef_%d.dat.bz2", hr.data(), N);
+  }
   std::vector<double> hm(nn), hc(nn), hmed(nn);
   download(hm.data(), mean.p, nn, dt);
   download(hc.data(), cnt.p, nn, dt);
   download(hmed.data(), med.p, nn, dt);
   writeReconstruction(dir, hm, hmed, hc, N);
-  (void)stage;
 }
 
 void doAction(ParameterParser &p, int action, bool loadStageOneResults, int stage) {

@@ -42,6 +42,8 @@ class SeriesParams(ctypes.Structure):
         ("dont_accumulate", ctypes.c_int), ("spec", ctypes.c_int), ("pre_smooth_sigma", ctypes.c_double),
         ("pairwise_batch", ctypes.c_int), ("data_term", ctypes.c_int),
         ("average_ext_mean", ctypes.c_int), ("average_nearest", ctypes.c_int),
+        ("reduce_deformations", ctypes.c_int), ("reduce_max_iter", ctypes.c_int), ("reduce_stop_eps", ctypes.c_double),
+        ("pair0_init", ctypes.c_void_p),
     ]
 
 
@@ -93,7 +95,9 @@ SIGNATURES = {
     "msg_profile_reset": ([], ctypes.c_int),
     "msg_profile_read": ([ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong), _dp], ctypes.c_int),
     "msg_match_series": ([ctypes.POINTER(SeriesParams), ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _dp, _vp,
-                          _vp, _vp, ctypes.POINTER(SeriesLevels)], ctypes.c_int),
+                          _vp, _vp, ctypes.POINTER(SeriesLevels), _vp], ctypes.c_int),
+    "msg_reduce_deformations": ([ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _dp,
+                                 ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
 }
 
 _lib = None
@@ -362,6 +366,18 @@ def series_level_layout(params, n, reuse=False):
     return recs
 
 
+def reduce_deformations(defs, max_iter=1000, stop_eps=5e-7):
+    """reduceDeformations on device fields defs (n, 2, N, N): (phi, energy, iterations)."""
+    n, N = defs.shape[0], defs.shape[-1]
+    phi = _torch().zeros_like(defs[0])
+    e = np.zeros(1)
+    it = ctypes.c_int(0)
+    _ready()
+    check(lib().msg_reduce_deformations(dtype_code(defs), _ptr(defs), n, int(np.log2(N)), max_iter, stop_eps, _ptr(phi),
+                                        _dptr(e), ctypes.byref(it)))
+    return phi, float(e[0]), it.value
+
+
 def match_series(templates, reference, params, pairwise_in=None, want_pairwise=False, want_levels=False):
     """One stage of series matching on device tensors. templates: (n, N, N); reference: (N, N).
     want_levels: also return the per-level deformations and iteration records of every solve."""
@@ -384,11 +400,12 @@ def match_series(templates, reference, params, pairwise_in=None, want_pairwise=F
     cnt = torch.zeros_like(mean)
     med = torch.zeros_like(mean)
     _ready()
+    red = torch.zeros((2, N, N), dtype=templates.dtype, device=templates.device) if params.reduce_deformations else None
     check(lib().msg_match_series(ctypes.byref(params), dtype_code(templates), _ptr(templates), n, _ptr(reference),
                                  _ptr(pairwise_in) if pairwise_in is not None else None, _ptr(defs),
                                  _ptr(pw) if pw is not None else None, _dptr(en), _ptr(mean), _ptr(cnt), _ptr(med),
-                                 ctypes.byref(lv) if lv is not None else None))
-    out = dict(deformations=defs, pairwise=pw, energies=en, mean=mean, count=cnt, median=med)
+                                 ctypes.byref(lv) if lv is not None else None, _ptr(red) if red is not None else None))
+    out = dict(deformations=defs, pairwise=pw, energies=en, mean=mean, count=cnt, median=med, reduced=red)
     if lv is not None:
         levels, off = [], 0
         tr = np.ctypeslib.as_array(iters).view(np.float64).reshape(-1, 8) if mi > 0 else np.zeros((0, 8))
@@ -427,3 +444,8 @@ def profile_read(cls):
 
 def set_default_mode(mode):
     check(lib().msg_set_default_mode(mode))
+
+
+def release_cache():
+    """Frees the contexts msg_match_series keeps for reuse."""
+    check(lib().msg_release_cache())

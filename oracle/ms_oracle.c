@@ -614,6 +614,83 @@ void msor_vcycle(const double *d, int N, double sigma, double *out) {
 }
 
 /* ---------------------------------------------------------------------------------------------
+ * Consistency energy of reduceDeformations (AmbrosioTortorelli.h:680-800)
+ * ------------------------------------------------------------------------------------------- */
+
+/* ConsistencyEnergyOp::evaluateIntegrand (AmbrosioTortorelli.h:705-728) integrated as
+ * FENonlinIntegrationVectorInterface::applyAdd: 1/2 int |psi(x + phi(x)) + phi(x)|^2 with the clipped
+ * deformed quad point of transformAndClipCoord (deformations.h:764-787). */
+double msor_consistency_energy(const double *psi, const double *phi, int N) {
+  initBQ();
+  const int NN = N * N;
+  const double h = gridH(N), vol = h * h;
+  double res = 0.;
+  for (int ey = 0; ey < N - 1; ++ey)
+    for (int ex = 0; ex < N - 1; ++ex) {
+      double a = 0.;
+      for (int q = 0; q < 4; ++q) {
+        const double off[2] = {evalAtQP(phi, ex, ey, q, N), evalAtQP(phi + NN, ex, ey, q, N)};
+        int tel[2], within[2];
+        double tloc[2];
+        transformAndClip(ex, ey, q, off[0], off[1], N, tel, tloc, within);
+        double energy = 0.;
+        for (int i = 0; i < 2; ++i) {
+          const double v = evalAt(psi + i * NN, tel, tloc, N) + off[i];
+          energy += v * v;
+        }
+        a += 0.5 * energy * QW;
+      }
+      a *= vol;
+      res += a;
+    }
+  return 0. + res;
+}
+
+/* VariationOfConsistencyEnergyOp::getNonlinearity (AmbrosioTortorelli.h:755-799): (Dpsi + I)^T
+ * (psi + offset) at the clipped deformed quad point, zero per clipped axis; assembled as
+ * FENonlinVectorOpInterface::applyAdd. */
+void msor_consistency_force(const double *psi, const double *phi, int N, double *out) {
+  initBQ();
+  const int NN = N * N;
+  const double h = gridH(N), vol = h * h;
+  memset(out, 0, sizeof(double) * 2 * NN);
+  for (int ey = 0; ey < N - 1; ++ey)
+    for (int ex = 0; ex < N - 1; ++ex) {
+      double nl[4][2];
+      for (int q = 0; q < 4; ++q) {
+        const double off[2] = {evalAtQP(phi, ex, ey, q, N), evalAtQP(phi + NN, ex, ey, q, N)};
+        int tel[2], within[2];
+        double tloc[2];
+        transformAndClip(ex, ey, q, off[0], off[1], N, tel, tloc, within);
+        double v[2], J[2][2];
+        for (int c = 0; c < 2; ++c) {
+          v[c] = evalAt(psi + c * NN, tel, tloc, N);
+          evalGradAt(psi + c * NN, tel, tloc, N, J[c]);
+        }
+        for (int c = 0; c < 2; ++c) v[c] += off[c];
+        for (int c = 0; c < 2; ++c) J[c][c] += 1.;
+        for (int i = 0; i < 2; ++i) {                   /* Mat::multTransposed, smallMat.h:358-364 */
+          nl[q][i] = 0;
+          for (int j = 0; j < 2; ++j) nl[q][i] += J[j][i] * v[j];
+        }
+        for (int i = 0; i < 2; ++i)
+          if (!within[i]) nl[q][i] = 0;
+      }
+      for (int dof = 0; dof < 4; ++dof) {
+        double a0 = 0, a1 = 0;
+        for (int q = 0; q < 4; ++q) {
+          const double f = BQ[dof][q] * QW;
+          a0 += f * nl[q][0];
+          a1 += f * nl[q][1];
+        }
+        const int gi = l2g(ex, ey, dof, N);
+        out[gi] += vol * a0;
+        out[NN + gi] += vol * a1;
+      }
+    }
+}
+
+/* ---------------------------------------------------------------------------------------------
  * One level of the H1 gradient flow with nonlinear CG and Armijo step control
  * ------------------------------------------------------------------------------------------- */
 
@@ -626,11 +703,19 @@ typedef struct {
   int data_term;
   double *Rn, *Tn; /* NCC: the normalised reference (set once per level, as the reference's
                       energy object does in its constructor) and a scratch image */
+  const double *psi; /* consistency: the n deformations, 2*N*N each */
+  int npsi;
 } Energy;
+
+enum { DT_CONSISTENCY = 100 }; /* internal: reduceDeformations' energy */
 
 /* E = (0 + E_data) + lambda * E_reg (LinCombOp::applyAdd, op.h:255-272) */
 static double energyAt(Energy *e, const double *phi) {
   double E = 0;
+  if (e->data_term == DT_CONSISTENCY) {  /* LinCombOp of the n consistency energies, coefficient 1 */
+    for (int i = 0; i < e->npsi; ++i) E += msor_consistency_energy(e->psi + (size_t)i * 2 * e->N * e->N, phi, e->N);
+    return E;
+  }
   if (e->data_term == MSOR_NCC) {
     double var_t;
     E += nccEnergyNormalized(e->Rn, e->T, phi, e->N, e->Tn, &var_t);
@@ -644,6 +729,16 @@ static double energyAt(Energy *e, const double *phi) {
 /* DE = (0 + DE_data) + lambda * K phi, componentwise */
 static void gradientAt(Energy *e, const double *phi, double *out) {
   const int NN = e->N * e->N;
+  if (e->data_term == DT_CONSISTENCY) {  /* Dest = 0; Dest += tmp_i (LinCombOp::applyAdd, op.h:254-272) */
+    double *tmp = (double *)malloc(sizeof(double) * 2 * NN);
+    memset(out, 0, sizeof(double) * 2 * NN);
+    for (int i = 0; i < e->npsi; ++i) {
+      msor_consistency_force(e->psi + (size_t)i * 2 * NN, phi, e->N, tmp);
+      for (int k = 0; k < 2 * NN; ++k) out[k] += tmp[k];
+    }
+    free(tmp);
+    return;
+  }
   double *k = (double *)malloc(sizeof(double) * NN);
   if (e->data_term == MSOR_NCC)
     nccForceNormalized(e->Rn, e->T, phi, e->N, out);
@@ -711,33 +806,31 @@ static double armijo(Energy *e, const double *x, const double *d, const double *
  * the energy decrease drops below 1e-5 (GradientDescentWithAutomaticFilterWidth L675-691; L416-439).
  * An energy output stream is attached in the reference run, so the energy at the new position is
  * re-evaluated after each update instead of being taken from the Armijo cache (L361-396). */
-int msor_level_solve(const double *R, const double *T, double *phi, int N, const msor_level_params *p,
-                     double *energy_trace, double *tau_trace, double *sigma_trace, int *evals_trace,
-                     int trace_cap, int *n_armijo_calls, double *final_energy) {
-  const int NN = N * N, n2 = 2 * NN;
-  Energy e = {R, T, N, p->lambda, (double *)malloc(sizeof(double) * n2), 0, p->data_term, NULL, NULL};
-  if (p->data_term == MSOR_NCC) {
-    e.Rn = (double *)malloc(sizeof(double) * NN);
-    e.Tn = (double *)malloc(sizeof(double) * NN);
-    msor_ncc_normalize(R, N, e.Rn);
-  }
+/* The descent loop of GradientDescentBase::applySingle for an energy e, from phi (updated in place).
+ * auto_sigma: GradientDescentWithAutomaticFilterWidth's schedule from sigma 1, else sigma0 fixed
+ * (H1GradientDescent alone, gradientDescent.h:639-650). grad_stop: USE_GRADIENT_BASED_STOPPING
+ * (continue while |DE at the old position| > stop_eps, L416), else the energy decrease. */
+static int descent(Energy *e, double *phi, int max_iter, double stop_eps, double sigma0, int auto_sigma,
+                   int grad_stop, double *energy_trace, double *tau_trace, double *sigma_trace, int *evals_trace,
+                   int trace_cap, int *n_armijo_calls, double *final_energy) {
+  const int N = e->N, NN = N * N, n2 = 2 * NN;
   double *de = (double *)malloc(sizeof(double) * n2);
   double *dir = (double *)malloc(sizeof(double) * n2);
   double *oldDir = (double *)calloc(n2, sizeof(double));
   double *oldGrad = (double *)calloc(n2, sizeof(double));
   double tau = 1.0;
-  double filterWidth = 1.0;
+  double filterWidth = auto_sigma ? 1.0 : sigma0;
   const double stopFilterWidth = 0.0001, updateTol = 0.00001;
   int calls = 0;
 
-  double energy = energyAt(&e, phi);
+  double energy = energyAt(e, phi);
   double energyNew = energy;
   int iter = 0;
   if (energy_trace && trace_cap > 0) energy_trace[0] = energyNew;
-  if (p->max_iter <= 0) {
+  if (max_iter <= 0) {
     if (final_energy) *final_energy = energy;
     if (n_armijo_calls) *n_armijo_calls = 0;
-    free(e.tmp); free(e.Rn); free(e.Tn); free(de); free(dir); free(oldDir); free(oldGrad);
+    free(de); free(dir); free(oldDir); free(oldGrad);
     return 0;
   }
   int stopping;
@@ -745,51 +838,83 @@ int msor_level_solve(const double *R, const double *T, double *phi, int N, const
     iter++;
     energy = energyNew;
     /* calcTaudAndDescentDir */
-    gradientAt(&e, phi, de);
+    gradientAt(e, phi, de);
     for (int i = 0; i < n2; ++i) dir[i] = -de[i];
     for (int c = 0; c < 2; ++c) msor_vcycle(dir + c * NN, N, filterWidth, dir + c * NN);
     const double beta = ((iter - 1) % 10) ? (dot2(dir, dir, NN) / dot2(oldGrad, oldGrad, NN)) : 0.0;
     memcpy(oldGrad, dir, sizeof(double) * n2);
     for (int i = 0; i < n2; ++i) dir[i] += beta * oldDir[i];
     memcpy(oldDir, dir, sizeof(double) * n2);
-    e.evals = 0;
-    tau = armijo(&e, phi, dir, de, tau, energy);
+    e->evals = 0;
+    tau = armijo(e, phi, dir, de, tau, energy);
     if (tau_trace && calls < trace_cap) {
       tau_trace[calls] = tau;
       if (sigma_trace) sigma_trace[calls] = filterWidth;
-      if (evals_trace) evals_trace[calls] = e.evals;
+      if (evals_trace) evals_trace[calls] = e->evals;
     }
     calls++;
     for (int i = 0; i < n2; ++i) dir[i] *= tau;
     if (tau == 0) {
       memcpy(dir, oldGrad, sizeof(double) * n2);
       memset(oldDir, 0, sizeof(double) * n2);
-      e.evals = 0;
-      tau = armijo(&e, phi, dir, de, tau, energy);
+      e->evals = 0;
+      tau = armijo(e, phi, dir, de, tau, energy);
       if (tau_trace && calls < trace_cap) {
         tau_trace[calls] = tau;
         if (sigma_trace) sigma_trace[calls] = filterWidth;
-        if (evals_trace) evals_trace[calls] = e.evals;
+        if (evals_trace) evals_trace[calls] = e->evals;
       }
       calls++;
       for (int i = 0; i < n2; ++i) dir[i] *= tau;
     }
     if (tau != 0)
       for (int i = 0; i < n2; ++i) phi[i] += dir[i];
-    energyNew = energyAt(&e, phi);
+    energyNew = energyAt(e, phi);
     if (energy_trace && iter < trace_cap) energy_trace[iter] = energyNew;
 
-    stopping = ((energy - energyNew) > p->stop_eps) && (iter < p->max_iter);
-    if ((0.5 * filterWidth > stopFilterWidth) && ((energy - energyNew) < updateTol)) {
+    const double crit = grad_stop ? sqrt(dot2(de, de, NN)) : energy - energyNew; /* MultiVector::norm */
+    stopping = (crit > stop_eps) && (iter < max_iter);
+    if (!auto_sigma && tau == 0) stopping = 0;
+    if (auto_sigma && (0.5 * filterWidth > stopFilterWidth) && ((energy - energyNew) < updateTol)) {
       filterWidth *= 0.5;
       tau = 1.;
-      stopping = (filterWidth > stopFilterWidth) && (iter < p->max_iter);
+      stopping = (filterWidth > stopFilterWidth) && (iter < max_iter);
     }
   } while (stopping);
 
   if (final_energy) *final_energy = energyNew;
   if (n_armijo_calls) *n_armijo_calls = calls;
-  free(e.tmp); free(e.Rn); free(e.Tn); free(de); free(dir); free(oldDir); free(oldGrad);
+  free(de); free(dir); free(oldDir); free(oldGrad);
+  return iter;
+}
+
+int msor_level_solve(const double *R, const double *T, double *phi, int N, const msor_level_params *p,
+                     double *energy_trace, double *tau_trace, double *sigma_trace, int *evals_trace,
+                     int trace_cap, int *n_armijo_calls, double *final_energy) {
+  const int NN = N * N, n2 = 2 * NN;
+  Energy e = {R, T, N, p->lambda, (double *)malloc(sizeof(double) * n2), 0, p->data_term, NULL, NULL, NULL, 0};
+  if (p->data_term == MSOR_NCC) {
+    e.Rn = (double *)malloc(sizeof(double) * NN);
+    e.Tn = (double *)malloc(sizeof(double) * NN);
+    msor_ncc_normalize(R, N, e.Rn);
+  }
+  const int iter = descent(&e, phi, p->max_iter, p->stop_eps, 1.0, 1, 0, energy_trace, tau_trace, sigma_trace,
+                           evals_trace, trace_cap, n_armijo_calls, final_energy);
+  free(e.tmp); free(e.Rn); free(e.Tn);
+  return iter;
+}
+
+/* SeriesMatching::reduceDeformations (matchSeries.h:678-704): from phi = 0, minimise the sum of the
+ * consistency energies of the n deformations with H1GradientDescent (fixed filter width sqrt(10 h),
+ * gradientDescent.h:639-650), nonlinear CG and gradient-norm stopping. */
+int msor_reduce_deformations(const double *psi, int n, int N, int max_iter, double stop_eps, double *phi,
+                             double *final_energy) {
+  const int n2 = 2 * N * N;
+  Energy e = {NULL, NULL, N, 0.0, (double *)malloc(sizeof(double) * n2), 0, DT_CONSISTENCY, NULL, NULL, psi, n};
+  memset(phi, 0, sizeof(double) * n2);
+  const int iter = descent(&e, phi, max_iter, stop_eps, sqrt(10 * gridH(N)), 0, 1, NULL, NULL, NULL, NULL, 0, NULL,
+                           final_energy);
+  free(e.tmp);
   return iter;
 }
 
@@ -1045,7 +1170,7 @@ static void prepared(const double *raw, int NN, const msor_series_params *p, dou
 int msor_match_series(const double *templates, int n, const double *reference, int N,
                       const msor_series_params *p, const double *pairwise_in,
                       double *deformations, double *pairwise_out, double *energies,
-                      double *mean, double *count, double *median) {
+                      double *mean, double *count, double *median, double *reduced) {
   int L = 0;
   while ((1 << L) < N) ++L;
   const size_t NN = (size_t)N * N;
@@ -1063,6 +1188,7 @@ int msor_match_series(const double *templates, int n, const double *reference, i
     double *zero = (double *)calloc(2 * NN, sizeof(double));
     defSet(&def, zero); /* setTransformationToZero */
     free(zero);
+    if (i == 0 && p->pair0_init) defSet(&def, p->pair0_init); /* reduceDeformations, stage > 1 (L123-135) */
     prepared(templates + i * NN, (int)NN, p, img);
     pyrBuild(&tpl, img, L);
     if (pairwise_in == NULL || i == 0) {
@@ -1088,11 +1214,24 @@ int msor_match_series(const double *templates, int n, const double *reference, i
     pyrFree(&tpl);
   }
 
-  /* averageSeries (matchSeries.h:353-451): raw templates, Inf outside, fill = last template mean */
+  /* reduceDeformations (L565-570, 678-704), then averageSeries (matchSeries.h:353-451): raw templates
+   * (first deformed by the reduced deformation), Inf outside, fill = last template mean */
+  double *red = NULL, *pre = NULL;
+  if (p->reduce_deformations && n > 0) {
+    red = (double *)malloc(sizeof(double) * 2 * NN);
+    pre = (double *)malloc(sizeof(double) * NN);
+    msor_reduce_deformations(deformations, n, N, p->reduce_max_iter, p->reduce_stop_eps, red, NULL);
+    if (reduced) memcpy(reduced, red, sizeof(double) * 2 * NN);
+  }
   double *stack = (double *)malloc(sizeof(double) * NN * (n > 0 ? n : 1));
   double lastMean = 0;
   for (int i = 0; i < n; ++i) {
-    msor_deform_image(templates + i * NN, deformations + i * 2 * NN, N, INFINITY, 0, stack + i * NN);
+    if (red) {
+      msor_deform_image(templates + i * NN, red, N, INFINITY, 0, pre);
+      msor_deform_image(pre, deformations + i * 2 * NN, N, INFINITY, 0, stack + i * NN);
+    } else {
+      msor_deform_image(templates + i * NN, deformations + i * 2 * NN, N, INFINITY, 0, stack + i * NN);
+    }
     if (i == n - 1) {
       double s = 0;
       for (size_t j = 0; j < NN; ++j) s += templates[i * NN + j];
@@ -1101,6 +1240,8 @@ int msor_match_series(const double *templates, int n, const double *reference, i
   }
   msor_series_stats(stack, n, (int)NN, lastMean, mean, count, median);
   free(stack);
+  free(red);
+  free(pre);
   pyrFree(&ref0);
   pyrFree(&ref);
   defFree(&def);

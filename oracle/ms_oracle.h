@@ -60,6 +60,13 @@ int msor_level_solve(const double *R, const double *T, double *phi, int N, const
                      double *energy_trace, double *tau_trace, double *sigma_trace, int *evals_trace,
                      int trace_cap, int *n_armijo_calls, double *final_energy);
 
+/* --- reduceDeformations (matchSeries.h:678-704; AmbrosioTortorelli.h:680-800) ---
+ * psi: n displacement fields (2*N*N each); phi (out): the reduced deformation. Returns iterations. */
+double msor_consistency_energy(const double *psi, const double *phi, int N);
+void msor_consistency_force(const double *psi, const double *phi, int N, double *out);
+int msor_reduce_deformations(const double *psi, int n, int N, int max_iter, double stop_eps, double *phi,
+                             double *final_energy);
+
 /* --- input normalisation and image pyramid (registration.h:331-367; vec.h:864-870,1084-1089) ---
  * pct < 0 means "enhanceContrastSaturationPercentage" absent. */
 void msor_normalize(double *img, int count, double pct, int min_to_zero);
@@ -95,12 +102,16 @@ typedef struct {
   int resample_prolong;               /* resampleInsteadOfProlongateDeformation               */
   int dont_accumulate;                /* dontAccumulateDeformation                            */
   int data_term;                      /* MSOR_SSD or MSOR_NCC                                 */
+  int reduce_deformations;            /* reduceDeformations                                   */
+  int reduce_max_iter;
+  double reduce_stop_eps;
+  const double *pair0_init;           /* NULL or template 0's starting deformation            */
 } msor_series_params;
 
 int msor_match_series(const double *templates, int n, const double *reference, int N,
                       const msor_series_params *p, const double *pairwise_in,
                       double *deformations, double *pairwise_out, double *energies,
-                      double *mean, double *count, double *median);
+                      double *mean, double *count, double *median, double *reduced);
 
 #ifdef __cplusplus
 }

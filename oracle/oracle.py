@@ -34,6 +34,8 @@ class SeriesParams(ctypes.Structure):
         ("saturation_pct", ctypes.c_double), ("normalize_min_to_zero", ctypes.c_int),
         ("dont_normalize", ctypes.c_int), ("resample_prolong", ctypes.c_int),
         ("dont_accumulate", ctypes.c_int), ("data_term", ctypes.c_int),
+        ("reduce_deformations", ctypes.c_int), ("reduce_max_iter", ctypes.c_int), ("reduce_stop_eps", ctypes.c_double),
+        ("pair0_init", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -51,6 +53,7 @@ def lib():
         L.msor_reg_energy.restype = ctypes.c_double
         L.msor_ncc_normalize.restype = ctypes.c_double
         L.msor_ncc_energy.restype = ctypes.c_double
+        L.msor_consistency_energy.restype = ctypes.c_double
         L.msor_level_solve.restype = ctypes.c_int
         L.msor_match_series.restype = ctypes.c_int
         _lib = L
@@ -213,9 +216,31 @@ def series_stats(stack, fill):
     return mean, cnt, med
 
 
-def match_series(templates, reference, params, pairwise_in=None):
+def consistency_energy(psi, phi):
+    psi, phi = _f64(psi), _f64(phi)
+    return lib().msor_consistency_energy(_p(psi), _p(phi), psi.shape[-1])
+
+
+def consistency_force(psi, phi):
+    psi, phi = _f64(psi), _f64(phi)
+    out = np.empty_like(phi)
+    lib().msor_consistency_force(_p(psi), _p(phi), psi.shape[-1], _p(out))
+    return out
+
+
+def reduce_deformations(psis, max_iter=1000, stop_eps=5e-7):
+    """(phi, energy, iterations) of SeriesMatching::reduceDeformations over psis (n, 2, N, N)."""
+    psis = _f64(psis)
+    n, N = psis.shape[0], psis.shape[-1]
+    phi = np.zeros((2, N, N))
+    e = ctypes.c_double(0)
+    it = lib().msor_reduce_deformations(_p(psis), n, N, max_iter, ctypes.c_double(stop_eps), _p(phi), ctypes.byref(e))
+    return phi, e.value, it
+
+
+def match_series(templates, reference, params, pairwise_in=None, pair0_init=None):
     """One stage of series matching; returns dict of deformations, pairwise deformations,
-    energies, mean, count, median."""
+    energies, mean, count, median (and the reduced deformation with reduce_deformations)."""
     T = _f64(templates)
     n, N = T.shape[0], T.shape[-1]
     ref = _f64(reference)
@@ -226,6 +251,11 @@ def match_series(templates, reference, params, pairwise_in=None):
     cnt = np.zeros((N, N))
     med = np.zeros((N, N))
     pin = None if pairwise_in is None else _p(_f64(pairwise_in))
+    init = None if pair0_init is None else _f64(pair0_init).copy()
+    params.pair0_init = _p(init) if init is not None else None
+    red = np.zeros((2, N, N))
     lib().msor_match_series(_p(T), n, _p(ref), N, ctypes.byref(params), pin, _p(defs), _p(pw), _p(en),
-                            _p(mean), _p(cnt), _p(med))
-    return dict(deformations=defs, pairwise=pw, energies=en, mean=mean, count=cnt, median=med)
+                            _p(mean), _p(cnt), _p(med), _p(red))
+    params.pair0_init = None
+    return dict(deformations=defs, pairwise=pw, energies=en, mean=mean, count=cnt, median=med,
+                reduced=red if params.reduce_deformations else None)

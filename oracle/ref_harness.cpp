@@ -16,6 +16,7 @@
 #include <linearSmoothOp.h>
 #include <gradientDescent.h>
 #include <multilevelArray.h>
+#include <AmbrosioTortorelli.h>
 
 #include <cstdio>
 #include <cstdlib>
@@ -140,6 +141,49 @@ int main(int argc, char **argv) {
       const RType var = qc::NormalizedCrossCorrelationEnergy<ConfType>::normalizeImageForNCC(E.getMassOpRef(), R, rn);
       append(out, rn);
       out.push_back(var);
+    } else if (cmd == "consistency") {
+      // in: psi (2N^2), phi (2N^2) ; out: E, DE (2N^2) of ConsistencyEnergyOp / VariationOf...
+      aol::MultiVector<RType> psi(2, NN), phi(2, NN), de(2, NN);
+      toVec(&in[0], psi[0]); toVec(&in[NN], psi[1]); toVec(&in[2 * NN], phi[0]); toVec(&in[3 * NN], phi[1]);
+      qc::ConsistencyEnergyOp<ConfType> E(grid, psi);
+      qc::VariationOfConsistencyEnergyOp<ConfType> DE(grid, psi);
+      aol::Scalar<RType> e;
+      E.apply(phi, e);
+      out.push_back(e[0]);
+      DE.apply(phi, de);
+      append(out, de[0]); append(out, de[1]);
+    } else if (cmd == "reduce") {
+      // params: n maxIt stopEps ; in: n deformations (2N^2 each) ; out: phi (2N^2), E, iterations
+      // (SeriesMatching::reduceDeformations, matchSeries.h:678-704)
+      const int n = atoi(argv[3]);
+      const int maxIt = atoi(argv[4]);
+      const double stopEps = atof(argv[5]);
+      std::vector<aol::MultiVector<RType> *> defs;
+      aol::RandomAccessContainer<const qc::ConsistencyEnergyOp<ConfType> > energies;
+      aol::RandomAccessContainer<const qc::VariationOfConsistencyEnergyOp<ConfType> > derivatives;
+      aol::LinCombOp<aol::MultiVector<RType>, aol::Scalar<RType> > E;
+      aol::LinCombOp<aol::MultiVector<RType> > DE;
+      for (int i = 0; i < n; ++i) {
+        defs.push_back(new aol::MultiVector<RType>(2, NN));
+        toVec(&in[(2 * i) * NN], (*defs[i])[0]);
+        toVec(&in[(2 * i + 1) * NN], (*defs[i])[1]);
+      }
+      for (int i = 0; i < n; ++i) {
+        energies.constructDatumAndPushBack(grid, *defs[i]);
+        E.appendReference(energies[i]);
+        derivatives.constructDatumAndPushBack(grid, *defs[i]);
+        DE.appendReference(derivatives[i]);
+      }
+      typedef aol::H1GradientDescent<ConfType, aol::MultiVector<RType>, SmoothType> PlainH1;
+      PlainH1 solver(grid, E, DE, maxIt, 1, stopEps);
+      solver.setConfigurationFlags(PlainH1::USE_NONLINEAR_CG | PlainH1::LOG_GRADIENT_NORM_AT_OLD_POSITION |
+                                   PlainH1::USE_GRADIENT_BASED_STOPPING | PlainH1::DO_NOT_WRITE_CONSOLE_OUTPUT);
+      aol::MultiVector<RType> phi(2, NN);
+      solver.applySingle(phi);
+      append(out, phi[0]); append(out, phi[1]);
+      out.push_back(solver.getEnergyAtLastPosition());
+      out.push_back(solver.getIterations());
+      for (auto *d : defs) delete d;
     } else if (cmd == "smooth") {
       // params: sigma ; in: d ; out: one V-cycle of the heat operator applied to d
       const double sigma = atof(argv[3]);

@@ -159,6 +159,23 @@ def main():
         out["e2e64"] = run_e2e(frames, E2E_PAR, os.path.join(tmp, "e2e"))
         out["e2e64_ncc"] = run_e2e(frames, E2E_PAR.replace("deformationModel 22", "deformationModel 0"),
                                    os.path.join(tmp, "e2e_ncc"))
+        out["e2e64_reduce"] = run_e2e(frames, E2E_PAR.replace("reduceDeformations 0", "reduceDeformations 1"),
+                                      os.path.join(tmp, "e2e_reduce"))
+        # reduceDeformations' consistency energy and its minimisation
+        for N in (8, 32, 64):
+            rng = np.random.default_rng(300 + N)
+            h = 1.0 / (N - 1)
+            psi = (rng.random((2, N, N)) - 0.5) * 6 * h
+            phi = (rng.random((2, N, N)) - 0.5) * 6 * h
+            o = harness("consistency", N, [], [psi, phi], tmp)
+            out[f"consistency{N}"] = dict(psi=psi, phi=phi, e=o[0], de=o[1:].reshape(2, N, N))
+        N, nd = 32, 4
+        y, x = np.mgrid[0:N, 0:N] / (N - 1.0)
+        psis = np.stack([np.stack([0.02 * np.sin(2 * np.pi * (y + 0.1 * k)) + 0.01 * k,
+                                   0.015 * np.cos(2 * np.pi * x) + 0.005 * k]) for k in range(nd)])
+        o = harness("reduce", N, [nd, 200, 5e-7], [psis], tmp)
+        out["reduce32"] = dict(psi=psis, max_iter=200, stop_eps=5e-7, phi=o[:2 * N * N].reshape(2, N, N),
+                               energy=o[2 * N * N], iterations=int(o[2 * N * N + 1]))
     only = sys.argv[sys.argv.index("--only") + 1].split(",") if "--only" in sys.argv else None
     for key, d in out.items():
         if only is None or key in only:
@@ -190,6 +207,9 @@ def run_e2e(frames, par, wd):
             e2e["s1_pairwise"] = np.stack([np.stack([read_p9(os.path.join(od, "stage1", f"{i}",
                                                                             f"deformation_06_{c}.dat.bz2"))
                                                      for c in range(2)]) for i in range(nt)])
+        red = os.path.join(od, f"stage{st_}", "reduceDef_0.dat.bz2")
+        if os.path.exists(red):
+            e2e[f"s{st_}_reduceDef"] = np.stack([read_p9(red.replace("_0.dat", f"_{c}.dat")) for c in range(2)])
         for nm in ("average", "median", "numSamples"):
             e2e[f"s{st_}_{nm}"] = read_p9(os.path.join(od, f"stage{st_}", f"{nm}.q2bz"))
             e2e[f"s{st_}_{nm}_png"] = read_png_gray8(os.path.join(od, f"stage{st_}", f"{nm}.png"))

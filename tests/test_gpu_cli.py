@@ -39,7 +39,8 @@ def write_p9(path, img):
 
 
 @pytest.mark.parametrize("fixture,precision,tol_s1,tol_s2", [("e2e64", 64, 1e-6, 1e-3), ("e2e64", 32, 1e-3, 1e-2),
-                                                               ("e2e64_ncc", 64, 1e-6, 1e-3)])
+                                                               ("e2e64_ncc", 64, 1e-6, 1e-3),
+                                                               ("e2e64_reduce", 64, 1e-6, 1e-3)])
 def test_matchseries_two_stages(tmp_path, fixture, precision, tol_s1, tol_s2):
     if not os.path.exists(EXE):
         pytest.skip("bin/matchSeries not built")
@@ -89,5 +90,8 @@ def test_matchseries_two_stages(tmp_path, fixture, precision, tol_s1, tol_s2):
             m = 1 << int(lv)
             d = np.stack([read_p9(str(sd / f"deformation_{lv}_{c}.dat.bz2")) for c in range(2)])
             assert np.abs(d - g[k.replace("_energy_", "_deformation_")]).max() * (m - 1) < tol, k
+        if f"s{st}_reduceDef" in g:
+            red = np.stack([read_p9(str(out / f"stage{st}" / f"reduceDef_{c}.dat.bz2")) for c in range(2)])
+            assert np.abs(red - g[f"s{st}_reduceDef"]).max() * (N - 1) < tol, st
     assert (out / "log.txt").exists() and (out / "stage1" / "parameter-dump.txt").exists()
     assert (out / "stage1" / "defNorms.txt").exists()

@@ -211,3 +211,45 @@ def test_match_series_averaging_options(gpu, ext_mean, nearest):
     np.testing.assert_array_equal(out["count"].cpu().numpy(), cnt)
     np.testing.assert_allclose(out["mean"].cpu().numpy(), mean, rtol=1e-12, atol=1e-12)
     np.testing.assert_allclose(out["median"].cpu().numpy(), med, rtol=1e-12, atol=1e-12)
+
+
+def test_reduce_deformations_f64(gpu):
+    """reduceDeformations: plain H1 gradient descent (fixed filter width, gradient-norm stopping) on
+    the consistency energy of the given deformations, against the oracle (bit-exact with the
+    reference, fixture reduce32)."""
+    import os
+    with np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reduce32.npz")) as z:
+        g = {k: z[k] for k in z.files}
+    N = g["psi"].shape[-1]
+    psi = torch.tensor(g["psi"], dtype=torch.float64, device="cuda")
+    for mode in (0, 2):
+        gpu.set_default_mode(mode)
+        gpu.release_cache()
+        phi, e, it = gpu.reduce_deformations(psi, int(g["max_iter"]), float(g["stop_eps"]))
+        assert it == int(g["iterations"])
+        assert abs(e - g["energy"]) <= 1e-10 * abs(g["energy"])
+        assert np.abs(phi.cpu().numpy() - g["phi"]).max() * (N - 1) < 1e-8
+    gpu.set_default_mode(0)
+    gpu.release_cache()
+
+
+def test_match_series_reduce_f64(gpu):
+    from msamd import synth
+    N = 64
+    frames = synth.stem_series(N, 4, workload=0, spacing=6.0, sigma=1.5, jitter_step=0.15)
+    kw = dict(start_level=4, stop_level=6, refine_start_level=5, refine_stop_level=6, precision_level=6, lambda_=10.0,
+              lambda_factor=-1, max_iter=12, stop_eps=-1.0, saturation_pct=0.15, normalize_min_to_zero=1,
+              dont_normalize=0, resample_prolong=1, dont_accumulate=0, reduce_deformations=1, reduce_max_iter=12,
+              reduce_stop_eps=-1.0)
+    t = torch.tensor(frames, dtype=torch.float64, device="cuda")
+    init = np.zeros((2, N, N))
+    init[0] = 0.01 * np.sin(np.linspace(0, 3, N))[None, :]
+    ref = O.match_series(frames[1:], frames[0], O.SeriesParams(**kw), pair0_init=init)
+    ti = torch.tensor(init, dtype=torch.float64, device="cuda")
+    gp = gpu.SeriesParams(**kw)
+    gp.pair0_init = ti.data_ptr()
+    out = gpu.match_series(t[1:].contiguous(), t[0].contiguous(), gp)
+    assert np.abs(out["deformations"].cpu().numpy() - ref["deformations"]).max() * (N - 1) < 1e-6
+    assert np.abs(out["reduced"].cpu().numpy() - ref["reduced"]).max() * (N - 1) < 1e-6
+    np.testing.assert_array_equal(out["count"].cpu().numpy(), ref["count"])
+    np.testing.assert_allclose(out["median"].cpu().numpy(), ref["median"], rtol=1e-9, atol=1e-10)

@@ -153,3 +153,46 @@ def test_reference_program_two_stages():
     assert np.array_equal(r2["count"], g["s2_numSamples"])
     np.testing.assert_allclose(r2["median"], g["s2_median"], atol=1e-4)
     np.testing.assert_allclose(r2["energies"], g["s2_energies"], rtol=1e-5)
+
+
+@pytest.mark.parametrize("N", [8, 32, 64])
+def test_consistency_energy_and_variation(N):
+    """reduceDeformations' ConsistencyEnergyOp and its variation, bit for bit."""
+    g = load(f"consistency{N}")
+    assert O.consistency_energy(g["psi"], g["phi"]) == g["e"]
+    np.testing.assert_array_equal(O.consistency_force(g["psi"], g["phi"]), g["de"])
+
+
+def test_reduce_deformations_solve():
+    g = load("reduce32")
+    phi, e, it = O.reduce_deformations(g["psi"], int(g["max_iter"]), float(g["stop_eps"]))
+    assert it == int(g["iterations"])
+    assert e == g["energy"]
+    np.testing.assert_array_equal(phi, g["phi"])
+
+
+def test_reference_program_two_stages_reduce():
+    """reduceDeformations 1 end to end: the reduced deformation of each stage, the averages it enters,
+    and stage 2's template 0 starting from stage 1's reduced deformation."""
+    g = load("e2e64_reduce")
+    fr = g["frames"]
+    N = fr.shape[-1]
+
+    def params(lam):
+        p = e2e_params(lam)
+        p.reduce_deformations, p.reduce_max_iter, p.reduce_stop_eps = 1, 15, -1.0
+        return p
+    r1 = O.match_series(fr[1:], fr[0], params(10.0))
+    assert np.abs(r1["deformations"] - g["s1_deformations"]).max() * (N - 1) < 1e-10
+    assert np.abs(r1["reduced"] - g["s1_reduceDef"]).max() * (N - 1) < 1e-10
+    assert np.array_equal(r1["count"], g["s1_numSamples"])
+    np.testing.assert_allclose(r1["mean"], g["s1_average"], atol=1e-11)
+    np.testing.assert_allclose(r1["median"], g["s1_median"], atol=1e-11)
+    pw = np.zeros((4, 2, N, N))
+    pw[1:] = g["s1_pairwise"]
+    r2 = O.match_series(fr, g["s1_median"], params(1.0), pw, pair0_init=g["s1_reduceDef"])
+    assert np.abs(r2["deformations"] - g["s2_deformations"]).max() * (N - 1) < 1e-4
+    assert np.abs(r2["reduced"] - g["s2_reduceDef"]).max() * (N - 1) < 1e-4
+    assert np.array_equal(r2["count"], g["s2_numSamples"])
+    np.testing.assert_allclose(r2["median"], g["s2_median"], atol=1e-4)
+

@@ -89,6 +89,29 @@ def test_ncc_force_along_a_translation():
         assert abs(fd - np.sum(F * v)) <= 0.05 * abs(fd)
 
 
+def test_consistency_closed_forms_and_derivative():
+    # psi = 0, phi = c: 1/2 |c|^2 over the unit square; psi = -c, phi = c: 0
+    N = 16
+    c = np.array([0.013, -0.021])
+    phi = np.ones((2, N, N)) * c[:, None, None]
+    assert abs(O.consistency_energy(np.zeros((2, N, N)), phi) - 0.5 * np.sum(c * c)) < 1e-15
+    assert abs(O.consistency_energy(-phi, phi)) < 1e-30
+    # the variation is the derivative of the quadrature energy (central differences, no clipping)
+    N = 32
+    x, y, h = grid(N)
+    psi = 0.02 * np.stack([np.sin(2 * np.pi * y * h), np.cos(2 * np.pi * x * h)])
+    rng = np.random.default_rng(1)
+    phi = 0.2 * h * rng.standard_normal((2, N, N)) + 0.01
+    phi[:, :2, :], phi[:, -2:, :], phi[:, :, :2], phi[:, :, -2:] = 0, 0, 0, 0  # stay inside the domain
+    F = O.consistency_force(psi, phi)
+    for k in range(3):
+        v = rng.standard_normal((2, N, N)) * h
+        v[:, :2, :], v[:, -2:, :], v[:, :, :2], v[:, :, -2:] = 0, 0, 0, 0
+        eps = 1e-7
+        fd = (O.consistency_energy(psi, phi + eps * v) - O.consistency_energy(psi, phi - eps * v)) / (2 * eps)
+        assert abs(fd - np.sum(F * v)) <= 1e-5 * abs(fd)
+
+
 @pytest.mark.parametrize("N", [2, 8, 33 - 1])
 def test_stiffness_closed_forms(N):
     x, y, h = grid(N)
