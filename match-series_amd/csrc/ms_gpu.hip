@@ -175,15 +175,16 @@ void prof_resolve() {
 __global__ void k_state_init(PairState *st, int npairs, double lambda) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npairs) return;
-  PairState s;
-  memset(&s, 0, sizeof(s));
+  static_assert(sizeof(PairState) % 8 == 0, "PairState is a whole number of doubles");
+  double *w = reinterpret_cast<double *>(st + p);  // zero in place (a local copy would live in scratch)
+  for (size_t i = 0; i < sizeof(PairState) / 8; ++i) w[i] = 0.0;
+  PairState &s = st[p];
   s.ncand = 1;
   s.cand[0] = 0.0;
   s.dirmode = msg::DIR_NONE;
   s.lambda = lambda;
   s.active = 1;
   s.sigma = 1.0;
-  st[p] = s;
 }
 
 template <typename D, typename S>
