@@ -102,6 +102,8 @@ def main():
     ap.add_argument("--mode", type=int, default=2, help="0 host-synchronised, 1 device-driven, 2 + hipGraph")
     ap.add_argument("--cpu-iters", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=0, help="pairwise solves per batch (0: automatic)")
+    ap.add_argument("--classes", action="store_true", help="also time every kernel class in the profiled step")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -122,6 +124,7 @@ def main():
     dev = torch.tensor(frames, dtype=dt, device="cuda")
     tpl, ref = dev[1:].contiguous(), dev[0].contiguous()
     p = series_params(gpu, levels, refine, args.iters, spec=args.spec)
+    p.pairwise_batch = args.batch
 
     out = None
     for _ in range(args.warmup):
@@ -140,13 +143,14 @@ def main():
     # launching stream; graph capture is off while a class is timed) and the algorithmic bytes the
     # device counted (6 fields per pass with a direction, 4 for a level's first pass)
     gpu.profile_reset()
-    gpu.profile_enable(("energy",))
+    gpu.profile_enable(gpu.KERNEL_CLASSES if args.classes else ("energy",))
     t2 = time.perf_counter()
     gpu.match_series(tpl, ref, p)
     torch.cuda.synchronize()
     t3 = time.perf_counter()
     gpu.profile_enable(())
     prof = gpu.profile_read("energy")
+    classes = {c: gpu.profile_read(c) for c in gpu.KERNEL_CLASSES} if args.classes else None
 
     achieved = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9 if prof["ms"] > 0 else 0.0
     roofline = {"bound": "hbm", "kernel": "msg::k_energy", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -185,6 +189,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
+        "kernel_classes": classes,
         "energies_final_mean": float(np.mean(out["energies"])) if out is not None else None,
         "median_finite": finite,
     }
