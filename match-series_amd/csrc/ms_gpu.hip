@@ -566,7 +566,10 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
       if (++rounds > 200) return fail(MSG_ERR_STATE, "Armijo search did not terminate");
     }
     hipLaunchKernelGGL((msg::k_update<S>), dim3(nblk_upd * B), dim3(256), 0, s, phi, d, z, n, c->st, nblk_upd, B,
-                       c->counters + 3 * B);
+                       (unsigned *)nullptr);
+    HIPCHK(hipGetLastError());
+    a.phase = msg::PH_APPLIED;
+    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
     HIPCHK(hipGetLastError());
   }
 
@@ -654,37 +657,47 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
   a.sigma_fixed = c->sigma_fixed;
   a.sigma0 = c->sigma0;
   a.grad_stop = c->grad_stop;
-  auto mk = [&](int phase, int slot) {
-    msg::Tail t;
-    t.st = c->st;
-    t.part_e = c->part_e;
-    t.part_n = c->part_n;
-    t.part_c = c->part_c;
-    t.a = a;
-    t.a.phase = phase;
-    t.flags = c->flags_d;
-    t.trace = tr;
-    t.counter = c->counters + slot * B;
-    return t;
+  // Each control phase is its own one-workgroup-per-pair kernel after the pass whose partial sums it
+  // reduces: kernel boundaries make the partials visible across the XCDs (a last-workgroup tail would
+  // need an agent-scope release by every workgroup, an L2 write-back on this part, which costs far
+  // more than the extra launch).
+  auto control = [&](int phase) -> int {
+    msg::CtrlArgs ap = a;
+    ap.phase = phase;
+    ProfScope ps(KC_CONTROL, s, 0);
+    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, ap,
+                       c->flags_d, tr);
+    HIPCHK(hipGetLastError());
+    return MSG_OK;
   };
-  const msg::Tail t_beta = mk(msg::PH_BETA, 0), t_init = mk(msg::PH_ARMIJO_INIT, 1);
-  const msg::Tail t_step = mk(msg::PH_ARMIJO_STEP, 2), t_level = mk(msg::PH_LEVEL_INIT, 2);
 
   hipLaunchKernelGGL(k_state_init, dim3(cdiv(B, 64)), dim3(64), 0, s, c->st, B, lambda);
   HIPCHK(hipGetLastError());
-  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, t_level, npsi));
+  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, msg::Tail{}, npsi));
+  CHK(control(msg::PH_LEVEL_INIT));
 
   auto iteration = [&]() -> int {
     CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s, c->data_term, npsi));
-    CHK(launch_vcycle<S>(DE, 1, z, l, mg, c->st, 1.0, 2 * B, c->part_n, s, t_beta));
-    hipLaunchKernelGGL((msg::k_ncg<S>), dim3(nblk_ncg * 2 * B), dim3(256), 0, s, z, d, DE, n, c->st, c->part_c, nblk_ncg,
-                       2 * B, t_init);
-    HIPCHK(hipGetLastError());
-    for (int r = 0; r < Rr; ++r)
-      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, t_step, npsi));
-    hipLaunchKernelGGL((msg::k_update<S>), dim3(nblk_upd * B), dim3(256), 0, s, phi, d, z, n, c->st, nblk_upd, B,
-                       c->counters + 3 * B);
-    HIPCHK(hipGetLastError());
+    CHK(launch_vcycle<S>(DE, 1, z, l, mg, c->st, 1.0, 2 * B, c->part_n, s));
+    CHK(control(msg::PH_BETA));
+    {
+      ProfScope ps(KC_NCG, s, 0);
+      hipLaunchKernelGGL((msg::k_ncg<S>), dim3(nblk_ncg * 2 * B), dim3(256), 0, s, z, d, DE, n, c->st, c->part_c,
+                         nblk_ncg, 2 * B, msg::Tail{});
+      HIPCHK(hipGetLastError());
+    }
+    CHK(control(msg::PH_ARMIJO_INIT));
+    for (int r = 0; r < Rr; ++r) {
+      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, msg::Tail{}, npsi));
+      CHK(control(msg::PH_ARMIJO_STEP));
+    }
+    {
+      ProfScope ps(KC_UPDATE, s, 0);
+      hipLaunchKernelGGL((msg::k_update<S>), dim3(nblk_upd * B), dim3(256), 0, s, phi, d, z, n, c->st, nblk_upd, B,
+                         (unsigned *)nullptr);
+      HIPCHK(hipGetLastError());
+    }
+    CHK(control(msg::PH_APPLIED));
     return MSG_OK;
   };
 

@@ -16,7 +16,8 @@ constexpr double ARMIJO_SIGMA = 0.5;
 // Control: one workgroup per pair reduces the partial sums of the previous pass (fixed order) and
 // advances the pair's state machine. Phases:
 // ================================================================================================
-enum CtrlPhase { PH_LEVEL_INIT = 0, PH_BETA = 1, PH_ARMIJO_INIT = 2, PH_ARMIJO_STEP = 3 };
+// PH_APPLIED: the update kernel has applied every pending step (step_tau back to 0).
+enum CtrlPhase { PH_LEVEL_INIT = 0, PH_BETA = 1, PH_ARMIJO_INIT = 2, PH_ARMIJO_STEP = 3, PH_APPLIED = 4 };
 
 __device__ double reduce_fixed(const double *p, int count, double *red) {
   double s = 0;
@@ -390,6 +391,7 @@ __device__ void control_pair(int pair, PairState *__restrict__ st, const double 
       armijo_run(s, a.spec, trace, a.trace_cap, pair);
     }
   }
+  if (a.phase == PH_APPLIED && threadIdx.x == 0) s.step_tau = 0;
   if (threadIdx.x == 0 && flags) flags[pair] = (s.searching ? 1 : 0) | (s.active ? 2 : 0);
 }
 
@@ -1239,7 +1241,7 @@ k_update(S *__restrict__ phi, const S *__restrict__ dcg, const S *__restrict__ d
     if (i < m) p[i] = static_cast<S>(ld(p, i) + tau * ld(dir, i));
   }
   // the last workgroup of the pair marks the step as applied
-  if (last_block_of(counter + pair, nblk_pair) && threadIdx.x == 0) st[pair].step_tau = 0;
+  if (counter && last_block_of(counter + pair, nblk_pair) && threadIdx.x == 0) st[pair].step_tau = 0;
 }
 
 // ================================================================================================

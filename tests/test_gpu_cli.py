@@ -38,60 +38,102 @@ def write_p9(path, img):
         f.write(bz2.compress(hdr + np.ascontiguousarray(img, dtype="<f8").tobytes()))
 
 
-@pytest.mark.parametrize("fixture,precision,tol_s1,tol_s2", [("e2e64", 64, 1e-6, 1e-3), ("e2e64", 32, 1e-3, 1e-2),
-                                                               ("e2e64_ncc", 64, 1e-6, 1e-3),
-                                                               ("e2e64_reduce", 64, 1e-6, 1e-3)])
-def test_matchseries_two_stages(tmp_path, fixture, precision, tol_s1, tol_s2):
-    if not os.path.exists(EXE):
-        pytest.skip("bin/matchSeries not built")
-    with np.load(os.path.join(GOLD, f"{fixture}.npz")) as z:
-        g = {k: z[k] for k in z.files}
-    (tmp_path / "in").mkdir()
+def write_field(mask, phi):
+    for c in range(2):
+        write_p9(mask % c, phi[c])
+
+
+def run_cli(tmp_path, g, precision, skip_stage1=False):
+    (tmp_path / "in").mkdir(exist_ok=True)
     for i, f in enumerate(g["frames"]):
         write_p9(str(tmp_path / "in" / f"frame_{i:02d}.dat.bz2"), f)
     par = bytes(g["par"]).decode() + f"gpuFieldPrecision {precision}\n"
+    if skip_stage1:
+        assert "skipStage1 0" in par
+        par = par.replace("skipStage1 0", "skipStage1 1")
     (tmp_path / "run.par").write_text(par)
     r = subprocess.run([EXE, "run.par"], cwd=tmp_path, capture_output=True, text=True,
                        env=dict(os.environ, MSG_STRICT="1"), timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
-    out = tmp_path / "out"
+    return tmp_path / "out"
+
+
+def check_stage(out, g, st, precision, tol):
+    """Every output file of one stage against the reference program's (fixture)."""
     N = 64
-    for st, nt in ((1, 3), (2, 4)):
-        tol = tol_s1 if st == 1 else tol_s2
-        for i in range(nt):
-            sub = f"{i}" if i == 0 else f"{i}-r"
-            d = np.stack([read_p9(str(out / f"stage{st}" / sub / f"deformation_06_{c}.dat.bz2")) for c in range(2)])
-            assert np.abs(d - g[f"s{st}_deformations"][i]).max() * (N - 1) < tol, (st, i)
-        cnt = read_p9(str(out / f"stage{st}" / "numSamples.q2bz"))
-        assert np.abs(cnt - g[f"s{st}_numSamples"]).sum() <= (0 if st == 1 and precision == 64 else 8)
-        # PNG previews: the same 8-bit scaling of the value range; a pixel may differ by one step
-        # where the data differ below the reference's rounding
-        for nm in ("average", "median", "numSamples"):
-            png = read_png_gray8(str(out / f"stage{st}" / f"{nm}.png")).astype(int)
-            diff = np.abs(png - g[f"s{st}_{nm}_png"].astype(int))
-            assert diff.max() <= 1 and (diff > 0).mean() < 0.01, nm
-        med = read_p9(str(out / f"stage{st}" / "median.q2bz"))
-        ok = cnt == g[f"s{st}_numSamples"]
-        assert np.abs(med - g[f"s{st}_median"])[ok].max() < (1e-6 if st == 1 and precision == 64 else 1e-2)
-        en = np.loadtxt(str(out / f"stage{st}" / "energies.txt"))[:, 1]
-        np.testing.assert_allclose(en, g[f"s{st}_energies"], rtol=1e-4 if precision == 64 else 1e-3)
-        # per-level records of every solve: deformation_<LL> and energy_<LL>.txt ("k lambda*E_reg E")
-        subs = [k for k in g if k.startswith(f"s{st}_") and "_energy_" in k]
-        assert len(subs) == (9 + 4 if st == 1 else 3 + 6)
-        for k in subs:
-            sub, lv = k[len(f"s{st}_"):].split("_energy_")
-            sd = out / f"stage{st}" / sub.replace("_", "-")
-            ref = g[k]
-            mine = np.loadtxt(str(sd / f"energy_{lv}.txt")).reshape(-1, ref.shape[1])
-            np.testing.assert_array_equal(mine[:, 0], ref[:, 0])
-            rt = (2e-5 if st == 1 else 1e-3) if precision == 64 else 2e-3
-            np.testing.assert_allclose(mine[:, -1], ref[:, -1], rtol=rt, err_msg=k)
-            np.testing.assert_allclose(mine[:, 1:-1], ref[:, 1:-1], rtol=10 * rt, atol=1e-9, err_msg=k)
-            m = 1 << int(lv)
-            d = np.stack([read_p9(str(sd / f"deformation_{lv}_{c}.dat.bz2")) for c in range(2)])
-            assert np.abs(d - g[k.replace("_energy_", "_deformation_")]).max() * (m - 1) < tol, k
-        if f"s{st}_reduceDef" in g:
-            red = np.stack([read_p9(str(out / f"stage{st}" / f"reduceDef_{c}.dat.bz2")) for c in range(2)])
-            assert np.abs(red - g[f"s{st}_reduceDef"]).max() * (N - 1) < tol, st
+    nt = 3 if st == 1 else 4
+    for i in range(nt):
+        sub = f"{i}" if i == 0 else f"{i}-r"
+        d = np.stack([read_p9(str(out / f"stage{st}" / sub / f"deformation_06_{c}.dat.bz2")) for c in range(2)])
+        assert np.abs(d - g[f"s{st}_deformations"][i]).max() * (N - 1) < tol, (st, i)
+    cnt = read_p9(str(out / f"stage{st}" / "numSamples.q2bz"))
+    assert np.abs(cnt - g[f"s{st}_numSamples"]).sum() <= (0 if precision == 64 else 8)
+    # PNG previews: the same 8-bit scaling of the value range; a pixel may differ by one step
+    # where the data differ below the reference's rounding
+    for nm in ("average", "median", "numSamples"):
+        png = read_png_gray8(str(out / f"stage{st}" / f"{nm}.png")).astype(int)
+        diff = np.abs(png - g[f"s{st}_{nm}_png"].astype(int))
+        assert diff.max() <= 1 and (diff > 0).mean() < 0.01, nm
+    med = read_p9(str(out / f"stage{st}" / "median.q2bz"))
+    ok = cnt == g[f"s{st}_numSamples"]
+    assert np.abs(med - g[f"s{st}_median"])[ok].max() < (1e-6 if precision == 64 else 1e-2)
+    en = np.loadtxt(str(out / f"stage{st}" / "energies.txt"))[:, 1]
+    np.testing.assert_allclose(en, g[f"s{st}_energies"], rtol=1e-4 if precision == 64 else 1e-3)
+    # per-level records of every solve: deformation_<LL> and energy_<LL>.txt ("k lambda*E_reg E",
+    # NCC with E_NCC first)
+    subs = [k for k in g if k.startswith(f"s{st}_") and "_energy_" in k]
+    assert len(subs) == (9 + 4 if st == 1 else 3 + 6)
+    for k in subs:
+        sub, lv = k[len(f"s{st}_"):].split("_energy_")
+        sd = out / f"stage{st}" / sub.replace("_", "-")
+        ref = g[k]
+        mine = np.loadtxt(str(sd / f"energy_{lv}.txt")).reshape(-1, ref.shape[1])
+        np.testing.assert_array_equal(mine[:, 0], ref[:, 0])
+        rt = 2e-5 if precision == 64 else 2e-3
+        np.testing.assert_allclose(mine[:, -1], ref[:, -1], rtol=rt, err_msg=k)
+        np.testing.assert_allclose(mine[:, 1:-1], ref[:, 1:-1], rtol=10 * rt, atol=1e-9, err_msg=k)
+        m = 1 << int(lv)
+        d = np.stack([read_p9(str(sd / f"deformation_{lv}_{c}.dat.bz2")) for c in range(2)])
+        assert np.abs(d - g[k.replace("_energy_", "_deformation_")]).max() * (m - 1) < tol, k
+    if f"s{st}_reduceDef" in g:
+        red = np.stack([read_p9(str(out / f"stage{st}" / f"reduceDef_{c}.dat.bz2")) for c in range(2)])
+        assert np.abs(red - g[f"s{st}_reduceDef"]).max() * (N - 1) < tol, st
+
+
+@pytest.mark.parametrize("fixture,precision,tol", [("e2e64", 64, 1e-6), ("e2e64", 32, 1e-3), ("e2e64_ncc", 64, 1e-6),
+                                                   ("e2e64_reduce", 64, 1e-6)])
+def test_matchseries_stage1(tmp_path, fixture, precision, tol):
+    """Stage 1 from the frames; stage 2 of the same run (lambda 1: its step decisions can part from
+    the reference's on inputs that differ in the last bits) is only checked for completeness."""
+    if not os.path.exists(EXE):
+        pytest.skip("bin/matchSeries not built")
+    with np.load(os.path.join(GOLD, f"{fixture}.npz")) as z:
+        g = {k: z[k] for k in z.files}
+    out = run_cli(tmp_path, g, precision)
+    check_stage(out, g, 1, precision, tol)
+    for f in ("median.q2bz", "average.q2bz", "numSamples.q2bz", "energies.txt", "3-r/deformation_06_1.dat.bz2"):
+        assert (out / "stage2" / f).exists(), f
+    d = read_p9(str(out / "stage2" / "3-r" / "deformation_06_0.dat.bz2"))
+    assert np.isfinite(d).all() and np.abs(d).max() * 63 < 20
     assert (out / "log.txt").exists() and (out / "stage1" / "parameter-dump.txt").exists()
     assert (out / "stage1" / "defNorms.txt").exists()
+
+
+@pytest.mark.parametrize("fixture,precision,tol", [("e2e64", 64, 1e-4), ("e2e64", 32, 1e-2), ("e2e64_ncc", 64, 1e-4),
+                                                   ("e2e64_reduce", 64, 1e-4)])
+def test_matchseries_stage2_from_reference_stage1(tmp_path, fixture, precision, tol):
+    """Stage 2 alone ("skipStage1 1") on the reference program's own stage-1 files: median as the new
+    target, pairwise deformations reused, and with reduceDeformations its reduced deformation."""
+    if not os.path.exists(EXE):
+        pytest.skip("bin/matchSeries not built")
+    with np.load(os.path.join(GOLD, f"{fixture}.npz")) as z:
+        g = {k: z[k] for k in z.files}
+    s1 = tmp_path / "out" / "stage1"
+    for i in range(3):
+        (s1 / f"{i}").mkdir(parents=True)
+        write_field(str(s1 / f"{i}" / "deformation_06_%d.dat.bz2"), g["s1_pairwise"][i])
+    write_p9(str(s1 / "median.q2bz"), g["s1_median"])
+    if "s1_reduceDef" in g:
+        write_field(str(s1 / "reduceDef_%d.dat.bz2"), g["s1_reduceDef"])
+    out = run_cli(tmp_path, g, precision, skip_stage1=True)
+    check_stage(out, g, 2, precision, tol)
