@@ -269,7 +269,26 @@ int launch_grad(const S *R, const S *T, const S *phi, S *DE, int n, const PairSt
   return MSG_OK;
 }
 
-// Buffers of one V-cycle over fields of level l: per multigrid level m < l the coarse rhs, the
+// The reference statistics of the NCC energy for every pair of a level (two passes with their
+// control phases); the first energy pass of the level follows.
+template <typename S>
+int ncc_reference_stats(const S *R, int n, PairState *st, double *part, double *part_n, double *part_c, int B,
+                        msg::CtrlArgs a, int *flags, msg::IterTrace *tr, hipStream_t s) {
+  const EnergyGeom g = energy_geom(n);
+  const double h = 1.0 / (double)(n - 1);
+  for (int pass = 0; pass < 2; ++pass) {
+    hipLaunchKernelGGL((msg::k_ncc_ref<S>), dim3(g.nblk * B), dim3(256), 0, s, R, n, h, st, part, g.ntx, g.nty, B, pass);
+    HIPCHK(hipGetLastError());
+    a.phase = pass ? msg::PH_NCC_REF1 : msg::PH_NCC_REF0;
+    a.nblk_energy = g.nblk;
+    a.npairs = B;
+    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, st, part, part_n, part_c, a, flags, tr);
+    HIPCHK(hipGetLastError());
+  }
+  return MSG_OK;
+}
+
+// Buffers of one V-cycle over fields of level l// Buffers of one V-cycle over fields of level l: per multigrid level m < l the coarse rhs, the
 // pre-smoothed iterate and the final iterate.
 template <typename S>
 struct MGBufs {
@@ -523,6 +542,8 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
 
   hipLaunchKernelGGL(k_state_init, dim3(cdiv(B, 64)), dim3(64), 0, s, c->st, B, lambda);
   HIPCHK(hipGetLastError());
+  if (c->data_term == msg::DT_NCC)
+    CHK(ncc_reference_stats<S>(R, n, c->st, c->part_e, c->part_n, c->part_c, B, a, c->flags_d, tr, s));
   CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, msg::Tail{}, npsi));
   a.phase = msg::PH_LEVEL_INIT;
   hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
@@ -673,6 +694,8 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
 
   hipLaunchKernelGGL(k_state_init, dim3(cdiv(B, 64)), dim3(64), 0, s, c->st, B, lambda);
   HIPCHK(hipGetLastError());
+  if (c->data_term == msg::DT_NCC)
+    CHK(ncc_reference_stats<S>(R, n, c->st, c->part_e, c->part_n, c->part_c, B, a, c->flags_d, tr, s));
   CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, msg::Tail{}, npsi));
   CHK(control(msg::PH_LEVEL_INIT));
 
@@ -1084,7 +1107,7 @@ template <typename S>
 // if st_out is given, the pair states the NCC variation reads (reference and template statistics,
 // E_NCC at phi).
 int energy_impl(const void *R, const void *T, const void *phi, int n, int batch, int data_term, double *e_data,
-                       double *e_reg, hipStream_t s, std::vector<PairState> *st_out = nullptr) {
+                       double *e_reg, hipStream_t s) {
   const EnergyGeom g = energy_geom(n);
   void *stp = nullptr, *part = nullptr;
   CHK(scratch_get(0, sizeof(PairState) * batch, &stp));
@@ -1101,31 +1124,47 @@ int energy_impl(const void *R, const void *T, const void *phi, int n, int batch,
     for (int i = 0; i < g.nblk; ++i) a += hp[off + i];
     return a;
   };
-  if (st_out) st_out->assign(batch, PairState{});
   for (int p = 0; p < batch; ++p) {
-    if (data_term == msg::DT_NCC) {
-      const double r1 = sum(msg::part_index_ref(batch, p, 0, g.nblk)), r2 = sum(msg::part_index_ref(batch, p, 1, g.nblk));
-      const double rv = std::sqrt(std::fmax(r2 - r1 * r1, 0.0));
-      const double s1 = sum(msg::part_index(msg::DT_NCC, p, 0, 0, g.nblk));
-      double vt;
-      const double d = msg::ncc_value(s1, sum(msg::part_index(msg::DT_NCC, p, 0, 1, g.nblk)),
-                                      sum(msg::part_index(msg::DT_NCC, p, 0, 2, g.nblk)), r1, rv, &vt);
-      if (e_data) e_data[p] = d;
-      if (e_reg) e_reg[p] = sum(msg::part_index(msg::DT_NCC, p, 0, 3, g.nblk));
-      if (st_out) {
-        PairState &q = (*st_out)[p];
-        q.active = 1;
-        q.ncc_rm = r1;
-        q.ncc_rv = rv;
-        q.ncc_m = s1;
-        q.ncc_v = vt;
-        q.ncc_d = d;
-      }
-    } else {
-      if (e_data) e_data[p] = sum(msg::part_index(msg::DT_SSD, p, 0, 0, g.nblk));
-      if (e_reg) e_reg[p] = sum(msg::part_index(msg::DT_SSD, p, 0, 1, g.nblk));
-    }
+    if (e_data) e_data[p] = sum(msg::part_index(msg::DT_SSD, p, 0, 0, g.nblk));
+    if (e_reg) e_reg[p] = sum(msg::part_index(msg::DT_SSD, p, 0, 1, g.nblk));
   }
+  return MSG_OK;
+}
+
+// NCC at phi for every pair through the level solve's own statistics passes and control (the pair
+// states are what the NCC variation then reads).
+template <typename S>
+int ncc_energy_impl(const void *R, const void *T, const void *phi, int n, int batch, double *e_data, double *e_reg,
+                    hipStream_t s, std::vector<PairState> *st_out = nullptr) {
+  const EnergyGeom g = energy_geom(n);
+  void *stp = nullptr, *part = nullptr;
+  CHK(scratch_get(0, sizeof(PairState) * batch, &stp));
+  CHK(scratch_get(1, sizeof(double) * energy_part_size(n, batch), &part));
+  PairState *st = (PairState *)stp;
+  hipLaunchKernelGGL(k_state_init, dim3(cdiv(batch, 64)), dim3(64), 0, s, st, batch, 0.0);
+  HIPCHK(hipGetLastError());
+  msg::CtrlArgs a{};
+  a.nblk_energy = g.nblk;
+  a.max_iter = 1;  // leaves the pairs active: the variation kernel then takes part
+  a.stop_eps = 0;
+  a.spec = 1;
+  a.data_term = msg::DT_NCC;
+  a.npairs = batch;
+  CHK(ncc_reference_stats<S>((const S *)R, n, st, (double *)part, nullptr, nullptr, batch, a, nullptr, nullptr, s));
+  CHK(launch_energy<S>((const S *)R, (const S *)T, (const S *)phi, (const S *)phi, (const S *)phi, n, st,
+                       (double *)part, batch, s, msg::DT_NCC));
+  a.phase = msg::PH_LEVEL_INIT;
+  hipLaunchKernelGGL(msg::k_control, dim3(batch), dim3(256), 0, s, st, (const double *)part, (const double *)nullptr,
+                     (const double *)nullptr, a, (int *)nullptr, (msg::IterTrace *)nullptr);
+  HIPCHK(hipGetLastError());
+  std::vector<PairState> hs(batch);
+  HIPCHK(hipMemcpyAsync(hs.data(), st, sizeof(PairState) * batch, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int p = 0; p < batch; ++p) {
+    if (e_data) e_data[p] = hs[p].cand_D[0];
+    if (e_reg) e_reg[p] = hs[p].cand_R[0];
+  }
+  if (st_out) *st_out = hs;
   return MSG_OK;
 }
 
@@ -1134,7 +1173,7 @@ template <typename S>
 int ncc_gradient_impl(const void *R, const void *T, const void *phi, void *DE, int n, int batch, double lambda,
                       hipStream_t s) {
   std::vector<PairState> hs;
-  CHK(energy_impl<S>(R, T, phi, n, batch, msg::DT_NCC, nullptr, nullptr, s, &hs));
+  CHK(ncc_energy_impl<S>(R, T, phi, n, batch, nullptr, nullptr, s, &hs));
   for (auto &q : hs) q.lambda = lambda;
   void *stp = nullptr;
   CHK(scratch_get(2, sizeof(PairState) * batch, &stp));
@@ -1162,8 +1201,8 @@ int msg_ncc_energy(int dtype, const void *R, const void *T, const void *phi, int
   CHK(check_dtype(dtype));
   CHK(check_n(n));
   if (!R || !T || !phi || batch < 1) return fail(MSG_ERR_ARG, "invalid argument");
-  return dtype == MSG_F64 ? energy_impl<double>(R, T, phi, n, batch, msg::DT_NCC, e_ncc, e_reg, strm(stream))
-                          : energy_impl<float>(R, T, phi, n, batch, msg::DT_NCC, e_ncc, e_reg, strm(stream));
+  return dtype == MSG_F64 ? ncc_energy_impl<double>(R, T, phi, n, batch, e_ncc, e_reg, strm(stream))
+                          : ncc_energy_impl<float>(R, T, phi, n, batch, e_ncc, e_reg, strm(stream));
 }
 
 int msg_ncc_gradient(int dtype, const void *R, const void *T, const void *phi, void *DE, int n, int batch,

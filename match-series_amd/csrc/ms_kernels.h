@@ -17,7 +17,9 @@ constexpr double ARMIJO_SIGMA = 0.5;
 // advances the pair's state machine. Phases:
 // ================================================================================================
 // PH_APPLIED: the update kernel has applied every pending step (step_tau back to 0).
-enum CtrlPhase { PH_LEVEL_INIT = 0, PH_BETA = 1, PH_ARMIJO_INIT = 2, PH_ARMIJO_STEP = 3, PH_APPLIED = 4 };
+// PH_NCC_REF0 / PH_NCC_REF1: the reference's mean, then its centred moments (NCC, once per level).
+enum CtrlPhase { PH_LEVEL_INIT = 0, PH_BETA = 1, PH_ARMIJO_INIT = 2, PH_ARMIJO_STEP = 3, PH_APPLIED = 4,
+                 PH_NCC_REF0 = 5, PH_NCC_REF1 = 6 };
 
 __device__ double reduce_fixed(const double *p, int count, double *red) {
   double s = 0;
@@ -43,8 +45,8 @@ struct CtrlArgs {
 constexpr int NCG_TERMS = 3;  // per field and block: <DE,d>, <DE,z>, <DE,DE>
 
 // Where the energy pass stores its partial sums. SSD: 2 terms per candidate (E_SSD, E_reg). NCC: 4
-// per candidate (1'Mt, t'Mt, t'Mr, E_reg) and, after all candidates of all pairs, 2 per pair
-// (1'Mr, r'Mr) for the reference's statistics.
+// per candidate (1'Mw, w'Mw, w'Mu, E_reg with w = t - c, u = r - m_r) and, after all candidates of
+// all pairs, 2 per pair for the reference's statistics.
 __device__ __host__ __forceinline__ size_t part_index(int data_term, int pair, int k, int term, int nblk) {
   return ((size_t)(pair * MAX_CAND + k) * (data_term == DT_NCC ? 4 : 2) + term) * nblk;
 }
@@ -52,16 +54,18 @@ __device__ __host__ __forceinline__ size_t part_index_ref(int npairs, int pair, 
   return ((size_t)npairs * MAX_CAND * 4 + (size_t)pair * 2 + term) * nblk;
 }
 // NCC energy from the moments of one evaluation (NormalizedCrossCorrelationEnergy::computeEnergy,
-// registration.h:1953-1964, with normalizeImageForNCC, L1934-1947, written in moments): t and r
-// have M-means m_t = 1'Mt, m_r = 1'Mr and deviations v = sqrt(u'Mu - m^2) (1'M1 = 1); a deviation
-// within 1e-15 of zero is not divided by (appeqAbsolute, aol.h:779-794).
-__device__ __host__ __forceinline__ double ncc_value(double s1, double s2, double s3, double rm, double rv,
+// registration.h:1953-1964, with normalizeImageForNCC, L1934-1947, written in moments). The template
+// t enters shifted, w = t - c, and the reference centred, u = r - m_r, so that no moment cancels:
+// s1 = 1'Mw = m_t - c, s2 = w'Mw, s3 = w'Mu; then m_t = c + s1, v_t = sqrt(s2 - s1^2) (1'M1 = 1) and
+// (t - m_t)'M(r - m_r) = s3 - s1 * rb with rb = 1'Mu. A deviation within 1e-15 of zero is not divided
+// by (appeqAbsolute, aol.h:779-794).
+__device__ __host__ __forceinline__ double ncc_value(double s1, double s2, double s3, double rb, double rv,
                                                     double *vt_out) {
   const double vt = sqrt(fmax(s2 - s1 * s1, 0.0));
   *vt_out = vt;
   const double dt = fabs(vt - 0.0) < 1e-15 ? 1.0 : vt;
   const double dr = fabs(rv - 0.0) < 1e-15 ? 1.0 : rv;
-  return -((s3 - s1 * rm) / (dt * dr));
+  return -((s3 - s1 * rb) / (dt * dr));
 }
 
 __device__ __forceinline__ double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -151,6 +155,7 @@ __device__ void finish_iteration(PairState &s, double t, IterTrace *trace, int t
       s.ncc_d = ev.D;  // the NCC variation at the new position needs these
       s.ncc_m = ev.M;
       s.ncc_v = ev.V;
+      if (ev.M == ev.M) s.ncc_c = ev.M;  // later moments about the current mean
     } else {
       En = __builtin_nan("");  // cannot happen: the accepted step is one of the last two evaluations
     }
@@ -264,36 +269,38 @@ __device__ void control_pair(int pair, PairState *__restrict__ st, const double 
   __shared__ double sums[4 * MAX_CAND];
   const bool participates = s.active && !s.searching && s.step_tau == 0;
 
-  if ((a.phase == PH_LEVEL_INIT || a.phase == PH_ARMIJO_STEP) && a.data_term == DT_NCC) {
-    const int nc = s.ncand;
+  if (a.phase == PH_NCC_REF0 || a.phase == PH_NCC_REF1) {
     const int nb = a.nblk_energy;
-    double rm = 0, rv = 0;
-    if (a.phase == PH_LEVEL_INIT) {
-      const double r1 = reduce_fixed(part_e + part_index_ref(a.npairs, pair, 0, nb), nb, red);
-      const double r2 = reduce_fixed(part_e + part_index_ref(a.npairs, pair, 1, nb), nb, red);
-      if (threadIdx.x == 0) {
-        s.ncc_rm = r1;
-        s.ncc_rv = sqrt(fmax(r2 - r1 * r1, 0.0));
+    const double r1 = reduce_fixed(part_e + part_index_ref(a.npairs, pair, 0, nb), nb, red);
+    const double r2 = a.phase == PH_NCC_REF1 ? reduce_fixed(part_e + part_index_ref(a.npairs, pair, 1, nb), nb, red) : 0;
+    if (threadIdx.x == 0) {
+      if (a.phase == PH_NCC_REF0) {
+        s.ncc_rm = r1;  // 1'Mr
+      } else {
+        s.ncc_rb = r1;  // 1'M(r - m_r)
+        s.ncc_rv = sqrt(fmax(r2, 0.0));
+        s.ncc_c = s.ncc_rm;  // the first template moments are taken about the reference's mean
       }
     }
+  } else if ((a.phase == PH_LEVEL_INIT || a.phase == PH_ARMIJO_STEP) && a.data_term == DT_NCC) {
+    const int nc = s.ncand;
+    const int nb = a.nblk_energy;
     for (int k = 0; k < nc; ++k)
       for (int t = 0; t < 4; ++t) {
         const double v = reduce_fixed(part_e + part_index(DT_NCC, pair, k, t, nb), nb, red);
         if (threadIdx.x == 0) sums[4 * k + t] = v;
       }
     if (threadIdx.x == 0) {
-      rm = s.ncc_rm;
-      rv = s.ncc_rv;
       for (int k = 0; k < nc; ++k) {
         double vt;
-        const double D = ncc_value(sums[4 * k], sums[4 * k + 1], sums[4 * k + 2], rm, rv, &vt);
+        const double D = ncc_value(sums[4 * k], sums[4 * k + 1], sums[4 * k + 2], s.ncc_rb, s.ncc_rv, &vt);
         double E = 0;
         E += D;
         E += s.lambda * sums[4 * k + 3];  // LinCombOp: (0 + E_NCC) + lambda E_reg
         s.cand_E[k] = E;
         s.cand_R[k] = sums[4 * k + 3];
         s.cand_D[k] = D;
-        s.cand_M[k] = sums[4 * k];
+        s.cand_M[k] = s.ncc_c + sums[4 * k];
         s.cand_V[k] = vt;
       }
     }
@@ -322,7 +329,7 @@ __device__ void control_pair(int pair, PairState *__restrict__ st, const double 
     if (threadIdx.x == 0) {
       if (a.phase == PH_LEVEL_INIT) {
         s.ncc_d = s.cand_D[0];
-        s.ncc_m = s.cand_M[0];
+        s.ncc_m = s.ncc_c = s.cand_M[0];
         s.ncc_v = s.cand_V[0];
         s.energy = s.energy_new = s.energy_start = s.cand_E[0];
         s.ereg_new = s.ereg_start = s.cand_R[0];
@@ -666,10 +673,10 @@ k_energy_ncc(const S *__restrict__ R, const S *__restrict__ T, const S *__restri
 #pragma unroll
   for (int k = 0; k < MAX_CAND; ++k)
     for (int t = 0; t < 4; ++t) acc[k][t] = 0.0;
-  double accR1 = 0.0, accR2 = 0.0;
 
   if (valid) {
-    double f0[4], f1[4], d0[4], d1[4], r[4];
+    const double rm = st[pair].ncc_rm, c = st[pair].ncc_c;
+    double f0[4], f1[4], d0[4], d1[4], u[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int g = l2g(ex, ey, b, n);
@@ -677,34 +684,26 @@ k_energy_ncc(const S *__restrict__ R, const S *__restrict__ T, const S *__restri
       f1[b] = ld(py, g);
       d0[b] = use_dir ? ld(dx, g) : 0.0;
       d1[b] = use_dir ? ld(dy, g) : 0.0;
-      r[b] = ld(Rp, g);
+      u[b] = ld(Rp, g) + (-rm);  // the reference centred as normalizeImageForNCC does (addToAll)
     }
     double ml[4][4];
     mass_local(h * h, ml);
-    if (!use_dir) {
-      for (int i = 0; i < 4; ++i) {
-        double m = 0.;
-        for (int j = 0; j < 4; ++j) m += ml[i][j] * r[j];
-        accR1 += m;
-        accR2 += r[i] * m;
-      }
-    }
 #pragma unroll
     for (int k = 0; k < MAX_CAND; ++k) {
       if (k < nc) {
         const double tau = st[pair].cand[k];
-        double p0[4], p1[4], t[4];
+        double p0[4], p1[4], w[4];
         for (int b = 0; b < 4; ++b) {
           p0[b] = use_dir ? rnd<S>(f0[b] + tau * d0[b]) : f0[b];
           p1[b] = use_dir ? rnd<S>(f1[b] + tau * d1[b]) : f1[b];
-          t[b] = fast_warp(Tp, n, h, ex + (b & 1), ey + ((b >> 1) & 1), p0[b], p1[b]);
+          w[b] = fast_warp(Tp, n, h, ex + (b & 1), ey + ((b >> 1) & 1), p0[b], p1[b]) - c;
         }
         for (int i = 0; i < 4; ++i) {
           double m = 0.;
-          for (int j = 0; j < 4; ++j) m += ml[i][j] * t[j];
+          for (int j = 0; j < 4; ++j) m += ml[i][j] * w[j];
           acc[k][0] += m;
-          acc[k][1] += t[i] * m;
-          acc[k][2] += r[i] * m;
+          acc[k][1] += w[i] * m;
+          acc[k][2] += u[i] * m;
         }
         acc[k][3] = 0.5 * (stiff_form(p0) + stiff_form(p1));
       }
@@ -721,15 +720,44 @@ k_energy_ncc(const S *__restrict__ R, const S *__restrict__ T, const S *__restri
       }
     }
   }
-  if (!use_dir) {
-    const double a1 = block_sum<256>(accR1, red);
-    const double a2 = block_sum<256>(accR2, red);
-    if (threadIdx.x == 0) {
-      part[part_index_ref(npairs, pair, 0, nblk) + tile] = a1;
-      part[part_index_ref(npairs, pair, 1, nblk) + tile] = a2;
+  run_tail(tail, pair, tiles);
+}
+
+// The reference's statistics for the NCC pass, once per level: pass 0 the M-mean 1'Mr, pass 1 (with
+// the mean in the pair state) the centred moments 1'Mu and u'Mu of u = r - m_r (normalizeImageForNCC,
+// registration.h:1934-1947). Partials in the reference slots of the energy layout.
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_ncc_ref(const S *__restrict__ R, int n, double h, const PairState *st, double *__restrict__ part, int ntx, int nty,
+          int npairs, int centred) {
+  const int tiles = ntx * nty;
+  const int wg = xcd_remap(blockIdx.x, tiles * npairs);
+  const int pair = wg / tiles, tile = wg - pair * tiles;
+  __shared__ double red[4];
+  const size_t nn = (size_t)n * n;
+  const S *Rp = R + pair * nn;
+  const int ex = (tile % ntx) * EN_TX + (threadIdx.x % EN_TX);
+  const int ey = (tile / ntx) * EN_TY + (threadIdx.x / EN_TX);
+  double a1 = 0., a2 = 0.;
+  if ((ex < n - 1) && (ey < n - 1)) {
+    const double rm = centred ? st[pair].ncc_rm : 0.0;
+    double u[4];
+    for (int b = 0; b < 4; ++b) u[b] = centred ? ld(Rp, l2g(ex, ey, b, n)) + (-rm) : ld(Rp, l2g(ex, ey, b, n));
+    double ml[4][4];
+    mass_local(h * h, ml);
+    for (int i = 0; i < 4; ++i) {
+      double m = 0.;
+      for (int j = 0; j < 4; ++j) m += ml[i][j] * u[j];
+      a1 += m;
+      a2 += u[i] * m;
     }
   }
-  run_tail(tail, pair, tiles);
+  const double s1 = block_sum<256>(a1, red);
+  const double s2 = block_sum<256>(a2, red);
+  if (threadIdx.x == 0) {
+    part[part_index_ref(npairs, pair, 0, tiles) + tile] = s1;
+    part[part_index_ref(npairs, pair, 1, tiles) + tile] = s2;
+  }
 }
 
 // ================================================================================================

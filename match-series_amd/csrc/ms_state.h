@@ -34,7 +34,8 @@ struct PairState {
   // position (what the variation needs)
   double cand_D[MAX_CAND], cand_M[MAX_CAND], cand_V[MAX_CAND];
   double last_D[2], last_M[2], last_V[2];
-  double ncc_rm, ncc_rv;
+  double ncc_rm, ncc_rv, ncc_rb;  // reference: M-mean, deviation, 1'M(r - mean) (0 up to rounding)
+  double ncc_c;                   // shift of the template moments (a recent template mean)
   double ncc_d, ncc_m, ncc_v;
   double dd;          // |DE|^2 at the current position (gradient-norm stopping)
   int sigma_fixed;    // 1: filter width fixed (plain H1GradientDescent), 0: the automatic schedule
