@@ -641,9 +641,11 @@ k_energy_cons(const S *__restrict__ psi, int npsi, const S *__restrict__ phi, co
 // ================================================================================================
 // NCC energy pass (deformationModel 0) for up to MAX_CAND step sizes per pair, element-centric.
 // NormalizedCrossCorrelationEnergy::computeEnergy (registration.h:1953-1964) in moments of the
-// assembled mass matrix: with t the nodal FastDeformImage of T at phi + tau d and r the reference,
-// every element adds its share of 1'Mt, t'Mt, t'Mr (the element matrix of the assembled M), and at
-// a level's first pass (no direction) also 1'Mr, r'Mr. The control combines them (ncc_value).
+// assembled mass matrix: with t the nodal FastDeformImage of T at phi + tau d, w = t - c for a shift
+// c near the M-mean of t (the last accepted one) and u = r - m_r the centred reference, every element
+// adds its share of 1'Mw, w'Mw, w'Mu (the element matrix of the assembled M). Moments about a nearby
+// shift keep the differences between the candidates of one pass resolvable in FP64. The control
+// combines them (ncc_value).
 // ================================================================================================
 template <typename S>
 __global__ void __launch_bounds__(256)
