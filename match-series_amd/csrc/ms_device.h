@@ -61,6 +61,20 @@ __device__ __forceinline__ double rnd(double v) {
   return static_cast<double>(static_cast<S>(v));
 }
 
+// phi + tau * d as stored in S: FP64 fields in FP64 arithmetic, FP32 fields by one fused multiply-add
+// (the step sizes are powers of two, exact in FP32). The update kernel and every energy pass use this
+// one definition, so an accepted candidate is bitwise the position the update stores.
+template <typename S>
+__device__ __forceinline__ S step_at(S f, double tau, S d);
+template <>
+__device__ __forceinline__ double step_at<double>(double f, double tau, double d) {
+  return f + tau * d;
+}
+template <>
+__device__ __forceinline__ float step_at<float>(float f, double tau, float d) {
+  return __fmaf_rn((float)tau, d, f);
+}
+
 // Deformed quad point: transformAndClipCoord (deformations.h:795-821) followed by
 // getLocalCoordsRegularRectangularGrid (configurators.h:32-50), in domain units as the reference.
 struct Hit {

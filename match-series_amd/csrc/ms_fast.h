@@ -1,0 +1,531 @@
+// FP32-arithmetic kernels for FP32 fields: the SSD energy pass, the SSD variation and the V-cycle
+// of the H1 metric. Same launch geometry, partial-sum layout and pair-state protocol as the FP64
+// kernels in ms_kernels.h, which FP64 fields keep (they reproduce the reference's arithmetic).
+//
+// Numerics: the warp and the stencils run in FP32 with fused multiply-adds; energy sums, slopes and
+// the step decisions stay FP64 (per-element values enter FP64 accumulators). A deformed quad point
+// is computed in grid units relative to its element, e + u with u = quad offset + displacement / h,
+// so the local coordinate keeps FP32 precision of the displacement (about 1e-6 px at 10 px) rather
+// than of the absolute position (about 6e-5 px at 1023). Division by h becomes a multiplication by
+// n - 1 and the Jacobi division by the diagonal a multiplication by its reciprocal.
+#pragma once
+
+#include "ms_kernels.h"
+
+namespace msg {
+
+__device__ __forceinline__ float qpxf(int q) { return (q >> 1) ? (float)MSG_QP1 : (float)MSG_QP0; }
+__device__ __forceinline__ float qpyf(int q) { return (q & 1) ? (float)MSG_QP1 : (float)MSG_QP0; }
+// basis value of dof b at quad point q (compile-time constants once the loops are unrolled)
+__device__ __forceinline__ float bqf(int b, int q) { return (float)bq(b, q); }
+
+// One axis of a deformed quad point: the position e + u in grid units (e the element's lower node)
+// clipped to [0, n-1] as transformAndClipCoord does; `in` is false on a clipped axis. The element
+// index stays in [0, n-2] for every input (NaN included).
+__device__ __forceinline__ void clip_rel(int e, float u, int n, int &el, float &loc, bool &in) {
+  const float lo = -(float)e, hi = (float)(n - 1 - e);
+  in = (u >= lo) && (u < hi);
+  const float uc = fmaxf(u, lo);  // NaN -> lo
+  const float fl = floorf(uc);
+  if (uc < hi) {
+    el = e + (int)fl;
+    loc = uc - fl;
+  } else {
+    el = n - 2;
+    loc = 1.f;
+  }
+}
+
+struct HitF {
+  int i0;  // lower-left node of the element hit
+  float lx, ly;
+  bool inx, iny;
+};
+
+// quad point q of element (ex, ey) displaced by (ox, oy) in domain units
+__device__ __forceinline__ HitF deform_qp_f(int ex, int ey, int q, float ox, float oy, int n, float nm1) {
+#pragma clang fp contract(fast)
+  HitF r;
+  int elx, ely;
+  clip_rel(ex, ox * nm1 + qpxf(q), n, elx, r.lx, r.inx);
+  clip_rel(ey, oy * nm1 + qpyf(q), n, ely, r.ly, r.iny);
+  r.i0 = elx + ely * n;
+  return r;
+}
+
+// bilinear value at the hit; t receives the 4 nodal values
+__device__ __forceinline__ float bilin_f(const float *__restrict__ T, const HitF &p, int n, float t[4]) {
+#pragma clang fp contract(fast)
+  t[0] = T[p.i0];
+  t[1] = T[p.i0 + 1];
+  t[2] = T[p.i0 + n];
+  t[3] = T[p.i0 + n + 1];
+  const float a = p.lx * (t[1] - t[0]) + t[0];
+  const float b = p.lx * (t[3] - t[2]) + t[2];
+  return p.ly * (b - a) + a;
+}
+
+// psi^T K_loc psi of the Q1 stiffness matrix (see stiff_form)
+__device__ __forceinline__ float stiff_form_f(const float p[4]) {
+#pragma clang fp contract(fast)
+  const float e0 = p[0] - p[1], e1 = p[0] - p[2], e2 = p[1] - p[3], e3 = p[2] - p[3];
+  const float d0 = p[0] - p[3], d1 = p[1] - p[2];
+  return (e0 * e0 + e1 * e1 + e2 * e2 + e3 * e3) * (1.0f / 6.0f) + (d0 * d0 + d1 * d1) * (2.0f / 6.0f);
+}
+
+// Sums of NV values over a block of 256 threads in one pass (valid in thread 0); red holds 4*NV.
+template <int NV>
+__device__ __forceinline__ void block_sum_n(double (&v)[NV], double *red) {
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[wid * NV + k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = ((red[k] + red[NV + k]) + red[2 * NV + k]) + red[3 * NV + k];
+}
+
+// ================================================================================================
+// SSD energy pass for up to MAX_CAND step sizes (k_energy in FP32 arithmetic).
+// ================================================================================================
+__global__ void __launch_bounds__(256)
+k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const float *__restrict__ phi,
+             const float *__restrict__ dcg, const float *__restrict__ dgrad, int n, double vol,
+             const PairState *st, double *__restrict__ part, int ntx, int nty, int npairs) {
+#pragma clang fp contract(fast)
+  const int tiles = ntx * nty;
+  const int wg = xcd_remap(blockIdx.x, tiles * npairs);
+  const int pair = wg / tiles, tile = wg - pair * tiles;
+  const int nc = st[pair].ncand;
+  if (nc == 0) return;  // uniform across the workgroup
+  const int dm = st[pair].dirmode;
+  __shared__ double red[4 * 2 * MAX_CAND];
+
+  const size_t nn = (size_t)n * n;
+  const float *Rp = R + pair * nn, *Tp = T + pair * nn;
+  const float *px = phi + pair * 2 * nn, *py = px + nn;
+  const float *dsel = (dm == DIR_CG) ? dcg : dgrad;
+  const float *dx = dsel + pair * 2 * nn, *dy = dx + nn;
+  const bool use_dir = (dm != DIR_NONE);
+  const float nm1 = (float)(n - 1);
+
+  const int ex = (tile % ntx) * EN_TX + (threadIdx.x % EN_TX);
+  const int ey = (tile / ntx) * EN_TY + (threadIdx.x / EN_TX);
+  const bool valid = (ex < n - 1) && (ey < n - 1);
+
+  double acc[2 * MAX_CAND];
+#pragma unroll
+  for (int k = 0; k < 2 * MAX_CAND; ++k) acc[k] = 0.0;
+
+  if (valid) {
+    float f0[4], f1[4], d0[4], d1[4], rq[4];
+    float rn[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int g = l2g(ex, ey, b, n);
+      f0[b] = px[g];
+      f1[b] = py[g];
+      d0[b] = use_dir ? dx[g] : 0.f;
+      d1[b] = use_dir ? dy[g] : 0.f;
+      rn[b] = Rp[g];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) rq[q] = rn[0] * bqf(0, q) + rn[1] * bqf(1, q) + rn[2] * bqf(2, q) + rn[3] * bqf(3, q);
+#pragma unroll
+    for (int k = 0; k < MAX_CAND; ++k) {
+      if (k < nc) {
+        const double tau = st[pair].cand[k];
+        float p0[4], p1[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          p0[b] = use_dir ? step_at<float>(f0[b], tau, d0[b]) : f0[b];
+          p1[b] = use_dir ? step_at<float>(f1[b], tau, d1[b]) : f1[b];
+        }
+        float a = 0.f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float ox = p0[0] * bqf(0, q) + p0[1] * bqf(1, q) + p0[2] * bqf(2, q) + p0[3] * bqf(3, q);
+          const float oy = p1[0] * bqf(0, q) + p1[1] * bqf(1, q) + p1[2] * bqf(2, q) + p1[3] * bqf(3, q);
+          const HitF hp = deform_qp_f(ex, ey, q, ox, oy, n, nm1);
+          float t4[4];
+          const float t = bilin_f(Tp, hp, n, t4) - rq[q];
+          a += t * t;
+        }
+        acc[2 * k] = (double)a * (0.125 * vol);  // 1/2 t^2 * quad weight 1/4 * vol
+        acc[2 * k + 1] = 0.5 * ((double)stiff_form_f(p0) + (double)stiff_form_f(p1));
+      }
+    }
+  }
+  const int nblk = tiles;
+  block_sum_n<2 * MAX_CAND>(acc, red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < MAX_CAND; ++k)
+      if (k < nc) {
+        part[((size_t)(pair * MAX_CAND + k) * 2 + 0) * nblk + tile] = acc[2 * k];
+        part[((size_t)(pair * MAX_CAND + k) * 2 + 1) * nblk + tile] = acc[2 * k + 1];
+      }
+  }
+}
+
+// ================================================================================================
+// SSD variation DE = F_SSD + lambda K phi (k_grad<S, DT_SSD> in FP32 arithmetic).
+// ================================================================================================
+__global__ void __launch_bounds__(256)
+k_grad_f32(const float *__restrict__ R, const float *__restrict__ T, const float *__restrict__ phi,
+           float *__restrict__ DE, int n, double vol_d, const PairState *st, double lambda_default, int ntx, int nty,
+           int npairs) {
+#pragma clang fp contract(fast)
+  const int tiles = ntx * nty;
+  const int wg = xcd_remap(blockIdx.x, tiles * npairs);
+  const int pair = wg / tiles, tile = wg - pair * tiles;
+  if (st && !participates(st[pair])) return;
+  const float lambda = (float)(st ? st[pair].lambda : lambda_default);
+
+  __shared__ float cf[GR_EY * GR_EX][4][2];  // force contribution per element, dof, component
+  __shared__ float ck[GR_EY * GR_EX][4][2];  // stiffness contribution
+
+  const size_t nn = (size_t)n * n;
+  const float *Rp = R + pair * nn, *Tp = T + pair * nn;
+  const float *px = phi + pair * 2 * nn, *py = px + nn;
+  const int x0 = (tile % ntx) * GR_TX, y0 = (tile / ntx) * GR_TY;
+  const float nm1 = (float)(n - 1);
+  const float wv = (float)(0.25 * vol_d);  // quad weight * vol
+
+  for (int e = threadIdx.x; e < GR_EX * GR_EY; e += 256) {
+    const int ex = x0 - 1 + (e % GR_EX), ey = y0 - 1 + (e / GR_EX);
+    if (ex < 0 || ey < 0 || ex >= n - 1 || ey >= n - 1) continue;
+    float f0[4], f1[4], rn[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int g = l2g(ex, ey, b, n);
+      f0[b] = px[g];
+      f1[b] = py[g];
+      rn[b] = Rp[g];
+    }
+    float nl[4][2];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float ox = f0[0] * bqf(0, q) + f0[1] * bqf(1, q) + f0[2] * bqf(2, q) + f0[3] * bqf(3, q);
+      const float oy = f1[0] * bqf(0, q) + f1[1] * bqf(1, q) + f1[2] * bqf(2, q) + f1[3] * bqf(3, q);
+      const float r = rn[0] * bqf(0, q) + rn[1] * bqf(1, q) + rn[2] * bqf(2, q) + rn[3] * bqf(3, q);
+      const HitF hp = deform_qp_f(ex, ey, q, ox, oy, n, nm1);
+      float t4[4];
+      const float diff = bilin_f(Tp, hp, n, t4) - r;
+      // gradient of the bilinear T at the hit, in domain units
+      const float ax = t4[1] - t4[0], bx = t4[3] - t4[2];
+      const float ay = t4[2] - t4[0], by = t4[3] - t4[1];
+      const float gx = (hp.ly * (bx - ax) + ax) * nm1;
+      const float gy = (hp.lx * (by - ay) + ay) * nm1;
+      nl[q][0] = hp.inx ? gx * diff : 0.f;
+      nl[q][1] = hp.iny ? gy * diff : 0.f;
+    }
+#pragma unroll
+    for (int dof = 0; dof < 4; ++dof) {
+      float a0 = 0.f, a1 = 0.f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        a0 += bqf(dof, q) * nl[q][0];
+        a1 += bqf(dof, q) * nl[q][1];
+      }
+      cf[e][dof][0] = wv * a0;
+      cf[e][dof][1] = wv * a1;
+    }
+    // K_loc (1/6)[4 -1 -1 -2; -1 4 -2 -1; -1 -2 4 -1; -2 -1 -1 4] applied to the element's dofs. Its
+    // rows sum to zero, so row i is sum over j != i of K_ij (f_j - f_i): differences of close values
+    // instead of a sum that cancels to the FP32 ulp of f for smooth fields.
+    const float kl[4][4] = {{4.f / 6, -1.f / 6, -1.f / 6, -2.f / 6},
+                            {-1.f / 6, 4.f / 6, -2.f / 6, -1.f / 6},
+                            {-1.f / 6, -2.f / 6, 4.f / 6, -1.f / 6},
+                            {-2.f / 6, -1.f / 6, -1.f / 6, 4.f / 6}};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (j == i) continue;
+        s0 += kl[i][j] * (f0[j] - f0[i]);
+        s1 += kl[i][j] * (f1[j] - f1[i]);
+      }
+      ck[e][i][0] = s0;
+      ck[e][i][1] = s1;
+    }
+  }
+  __syncthreads();
+  const int x = x0 + (threadIdx.x % GR_TX), y = y0 + (threadIdx.x / GR_TX);
+  if (x >= n || y >= n) return;
+  float F0 = 0.f, F1 = 0.f, K0 = 0.f, K1 = 0.f;
+#pragma unroll
+  for (int ey = y - 1; ey <= y; ++ey)
+#pragma unroll
+    for (int ex = x - 1; ex <= x; ++ex) {
+      if (ex < 0 || ey < 0 || ex >= n - 1 || ey >= n - 1) continue;
+      const int e = (ex - (x0 - 1)) + (ey - (y0 - 1)) * GR_EX;
+      const int dof = (x - ex) + 2 * (y - ey);
+      F0 += cf[e][dof][0];
+      F1 += cf[e][dof][1];
+      K0 += ck[e][dof][0];
+      K1 += ck[e][dof][1];
+    }
+  float *o = DE + pair * 2 * nn;
+  const int i = x + y * n;
+  o[i] = lambda * K0 + F0;
+  o[nn + i] = lambda * K1 + F1;
+}
+
+// ================================================================================================
+// V-cycle of the H1 metric in FP32 arithmetic (k_mg_down / k_mg_up / k_mg_coarse).
+// ================================================================================================
+__device__ __forceinline__ void level_consts_f(const PairState *st, int pair, double sigma_default, int n, float &hs,
+                                               float &inv_diag) {
+  const double sigma = st ? st[pair].sigma : sigma_default;
+  const double tau = 0.5 * (sigma * sigma);
+  const double s = (double)((n - 1) * (n - 1));
+  hs = (float)(tau * s);
+  inv_diag = (float)(1.0 / (1 + 4 * tau * s));
+}
+
+// A u of the FD heat operator (heat_apply) in FP32. The stencil 4u - (neighbours) is summed as
+// differences u - u_nb: hs grows like n^2, and 4u - sum(u_nb) of a smooth field would cancel to a
+// few FP32 ulps of u (a 1e-4 relative error of A u at n = 256), while each difference of close
+// values is exact.
+template <typename Get>
+__device__ __forceinline__ float heat_apply_f(const Get &get, int x, int y, int n, float hs) {
+#pragma clang fp contract(fast)
+  if (n == 1) return get(0, 0);
+  const float u = get(x, y);
+  const int e = n - 1;
+  const bool bx0 = (x == 0), bx1 = (x == e), by0 = (y == 0), by1 = (y == e);
+  float lap;
+  if (!(bx0 || bx1 || by0 || by1)) {
+    lap = ((u - get(x + 1, y)) + (u - get(x - 1, y))) + ((u - get(x, y + 1)) + (u - get(x, y - 1)));
+  } else if ((bx0 || bx1) && (by0 || by1)) {  // corner: the inward neighbours count twice
+    lap = 2.f * ((u - get(x + (bx0 ? 1 : -1), y)) + (u - get(x, y + (by0 ? 1 : -1))));
+  } else if (by0 || by1) {  // edge parallel to the x axis
+    lap = 2.f * (u - get(x, y + (by0 ? 1 : -1))) + ((u - get(x + 1, y)) + (u - get(x - 1, y)));
+  } else {  // edge parallel to the y axis
+    lap = 2.f * (u - get(x + (bx0 ? 1 : -1), y)) + ((u - get(x, y + 1)) + (u - get(x, y - 1)));
+  }
+  return hs * lap + u;
+}
+
+__global__ void __launch_bounds__(256)
+k_mg_down_f32(const float *__restrict__ rhs, int negate, float *__restrict__ x2out, float *__restrict__ rhsc, int n,
+              const PairState *st, double sigma_default, int ntx, int nty, int nfields) {
+#pragma clang fp contract(fast)
+  const int tiles = ntx * nty;
+  const int wg = xcd_remap(blockIdx.x, tiles * nfields);
+  const int field = wg / tiles, tile = wg - field * tiles;
+  const int pair = field >> 1;
+  if (st && !participates(st[pair])) return;
+  float hs, idg;
+  level_consts_f(st, pair, sigma_default, n, hs, idg);
+
+  __shared__ float sr[MG_HY][MG_HX];    // rhs on tile+2
+  __shared__ float s1[MG_HY][MG_HX];    // x1 on tile+2
+  __shared__ float s2[MG_HY][MG_HX];    // x2 on tile+1
+  __shared__ float sres[MG_TY][MG_TX];  // residual on tile
+  const size_t nn = (size_t)n * n;
+  const float *rp = rhs + field * nn;
+  const int x0 = (tile % ntx) * MG_TX, y0 = (tile / ntx) * MG_TY;
+  const int bx = x0 - 2, by = y0 - 2;
+  const float sg = negate ? -1.f : 1.f;
+
+  for (int i = threadIdx.x; i < MG_HX * MG_HY; i += 256) {
+    const int lx = i % MG_HX, ly = i / MG_HX;
+    const int gx = bx + lx, gy = by + ly;
+    if (gx >= 0 && gy >= 0 && gx < n && gy < n) {
+      const float v = sg * rp[gx + gy * n];
+      sr[ly][lx] = v;
+      s1[ly][lx] = v * idg;  // first Jacobi sweep from zero
+    }
+  }
+  __syncthreads();
+  auto get1 = [&](int gx, int gy) { return s1[gy - by][gx - bx]; };
+  for (int i = threadIdx.x; i < MG_HX * MG_HY; i += 256) {
+    const int lx = i % MG_HX, ly = i / MG_HX;
+    const int gx = bx + lx, gy = by + ly;
+    if (lx >= 1 && ly >= 1 && lx < MG_HX - 1 && ly < MG_HY - 1 && gx >= 0 && gy >= 0 && gx < n && gy < n)
+      s2[ly][lx] = (sr[ly][lx] - heat_apply_f(get1, gx, gy, n, hs)) * idg + s1[ly][lx];
+  }
+  __syncthreads();
+  auto get2 = [&](int gx, int gy) { return s2[gy - by][gx - bx]; };
+  float *xo = x2out + field * nn;
+  for (int i = threadIdx.x; i < MG_TX * MG_TY; i += 256) {
+    const int tx = i % MG_TX, ty = i / MG_TX;
+    const int gx = x0 + tx, gy = y0 + ty;
+    if (gx < n && gy < n) {
+      sres[ty][tx] = sr[ty + 2][tx + 2] - heat_apply_f(get2, gx, gy, n, hs);
+      xo[gx + gy * n] = s2[ty + 2][tx + 2];
+    }
+  }
+  __syncthreads();
+  const int nc = n / 2;
+  float *rc = rhsc + field * (size_t)nc * nc;
+  for (int i = threadIdx.x; i < (MG_TX / 2) * (MG_TY / 2); i += 256) {
+    const int cx = i % (MG_TX / 2), cy = i / (MG_TX / 2);
+    const int gcx = x0 / 2 + cx, gcy = y0 / 2 + cy;
+    if (gcx < nc && gcy < nc)
+      rc[gcx + gcy * nc] =
+          0.25f * ((sres[2 * cy][2 * cx] + sres[2 * cy][2 * cx + 1]) + (sres[2 * cy + 1][2 * cx] + sres[2 * cy + 1][2 * cx + 1]));
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_mg_up_f32(const float *__restrict__ x2in, const float *__restrict__ xc, const float *__restrict__ rhs, int negate,
+            float *__restrict__ xout, int n, const PairState *st, double sigma_default, int ntx, int nty, int nfields,
+            double *__restrict__ nrm_part) {
+#pragma clang fp contract(fast)
+  const int tiles = ntx * nty;
+  const int wg = xcd_remap(blockIdx.x, tiles * nfields);
+  const int field = wg / tiles, tile = wg - field * tiles;
+  const int pair = field >> 1;
+  if (st && !participates(st[pair])) return;
+  float hs, idg;
+  level_consts_f(st, pair, sigma_default, n, hs, idg);
+
+  __shared__ float sx[MG_HY][MG_HX];  // x on tile+2
+  __shared__ float sr[MG_HY][MG_HX];  // rhs on tile+1
+  __shared__ float s3[MG_HY][MG_HX];  // x3 on tile+1
+  __shared__ double red[4];
+  const size_t nn = (size_t)n * n;
+  const int nc = n / 2;
+  const float *x2p = x2in + field * nn, *rp = rhs + field * nn, *xcp = xc + field * (size_t)nc * nc;
+  const int x0 = (tile % ntx) * MG_TX, y0 = (tile / ntx) * MG_TY;
+  const int bx = x0 - 2, by = y0 - 2;
+  const float sg = negate ? -1.f : 1.f;
+
+  for (int i = threadIdx.x; i < MG_HX * MG_HY; i += 256) {
+    const int lx = i % MG_HX, ly = i / MG_HX;
+    const int gx = bx + lx, gy = by + ly;
+    if (gx >= 0 && gy >= 0 && gx < n && gy < n) {
+      sx[ly][lx] = x2p[gx + gy * n] + xcp[(gx >> 1) + (gy >> 1) * nc];
+      sr[ly][lx] = sg * rp[gx + gy * n];
+    }
+  }
+  __syncthreads();
+  auto getx = [&](int gx, int gy) { return sx[gy - by][gx - bx]; };
+  for (int i = threadIdx.x; i < MG_HX * MG_HY; i += 256) {
+    const int lx = i % MG_HX, ly = i / MG_HX;
+    const int gx = bx + lx, gy = by + ly;
+    if (lx >= 1 && ly >= 1 && lx < MG_HX - 1 && ly < MG_HY - 1 && gx >= 0 && gy >= 0 && gx < n && gy < n)
+      s3[ly][lx] = (sr[ly][lx] - heat_apply_f(getx, gx, gy, n, hs)) * idg + sx[ly][lx];
+  }
+  __syncthreads();
+  auto get3 = [&](int gx, int gy) { return s3[gy - by][gx - bx]; };
+  float *xo = xout + field * nn;
+  double nrm = 0;
+  for (int i = threadIdx.x; i < MG_TX * MG_TY; i += 256) {
+    const int tx = i % MG_TX, ty = i / MG_TX;
+    const int gx = x0 + tx, gy = y0 + ty;
+    if (gx < n && gy < n) {
+      const float o = (sr[ty + 2][tx + 2] - heat_apply_f(get3, gx, gy, n, hs)) * idg + s3[ty + 2][tx + 2];
+      xo[gx + gy * n] = o;
+      nrm += (double)o * (double)o;
+    }
+  }
+  if (nrm_part) {
+    const double s = block_sum<256>(nrm, red);
+    if (threadIdx.x == 0) nrm_part[(size_t)field * tiles + tile] = s;
+  }
+}
+
+// Whole V-cycle below (and including) a level of at most 32x32 nodes in one workgroup of
+// MGC_THREADS threads (k_mg_coarse). The Jacobi sweeps write a second buffer instead of updating in
+// place, the residual is restricted as it is formed and the prolongation is read on the fly by the
+// first sweep of the up leg, so a level costs 3 barriers down and 2 up.
+constexpr int MGC_THREADS = 1024;
+
+__global__ void __launch_bounds__(MGC_THREADS)
+k_mg_coarse_f32(const float *__restrict__ rhs, int negate, float *__restrict__ xout, int L, const PairState *st,
+                double sigma_default, int nfields, double *__restrict__ nrm_part) {
+#pragma clang fp contract(fast)
+  const int field = blockIdx.x;
+  const int pair = field >> 1;
+  if (st && !participates(st[pair])) return;
+  const double sigma = st ? st[pair].sigma : sigma_default;
+  const double tau = 0.5 * (sigma * sigma);
+  __shared__ float X[MGC_OFF[MGC_MAXL + 1]], B[MGC_OFF[MGC_MAXL + 1]], Y[1024];
+  __shared__ double red[MGC_THREADS / 64];
+  const int n = 1 << L;
+  const size_t nn = (size_t)n * n;
+  const float *rp = rhs + field * nn;
+  const float sg = negate ? -1.f : 1.f;
+  for (int i = threadIdx.x; i < (int)nn; i += MGC_THREADS) B[MGC_OFF[L] + i] = sg * rp[i];
+  __syncthreads();
+
+  auto consts = [&](int l, float &hs, float &idg) {
+    const int m = 1 << l;
+    const double s = (double)((m - 1) * (m - 1));
+    hs = (float)(tau * s);
+    idg = (float)(1.0 / (1 + 4 * tau * s));
+  };
+
+  for (int l = L; l >= 1; --l) {
+    const int m = 1 << l, mm = m * m, mc = m / 2;
+    float hs, idg;
+    consts(l, hs, idg);
+    float *x = X + MGC_OFF[l];
+    const float *b = B + MGC_OFF[l];
+    // first sweep from zero, into Y
+    for (int i = threadIdx.x; i < mm; i += MGC_THREADS) Y[i] = b[i] * idg;
+    __syncthreads();
+    // second sweep, into x (kept for the up leg)
+    auto gy = [&](int gx, int gyy) { return Y[gx + gyy * m]; };
+    for (int i = threadIdx.x; i < mm; i += MGC_THREADS)
+      x[i] = (b[i] - heat_apply_f(gy, i % m, i / m, m, hs)) * idg + Y[i];
+    __syncthreads();
+    // residual at the 4 children of every coarse node, averaged
+    auto gx2 = [&](int px, int py) { return x[px + py * m]; };
+    for (int i = threadIdx.x; i < mc * mc; i += MGC_THREADS) {
+      const int cx = i % mc, cy = i / mc;
+      float r[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int fx = 2 * cx + (c & 1), fy = 2 * cy + (c >> 1);
+        r[c] = b[fx + fy * m] - heat_apply_f(gx2, fx, fy, m, hs);
+      }
+      B[MGC_OFF[l - 1] + i] = 0.25f * ((r[0] + r[1]) + (r[2] + r[3]));
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) X[0] = B[0];  // 1x1 level: A = 1, the sweeps converge in one step
+  __syncthreads();
+  for (int l = 1; l <= L; ++l) {
+    const int m = 1 << l, mm = m * m, mc = m / 2;
+    float hs, idg;
+    consts(l, hs, idg);
+    float *x = X + MGC_OFF[l];
+    const float *xc = X + MGC_OFF[l - 1];
+    const float *b = B + MGC_OFF[l];
+    // x + injected coarse correction, read on the fly; first sweep into Y
+    auto gp = [&](int px, int py) { return x[px + py * m] + xc[(px >> 1) + (py >> 1) * mc]; };
+    for (int i = threadIdx.x; i < mm; i += MGC_THREADS) {
+      const int px = i % m, py = i / m;
+      Y[i] = (b[i] - heat_apply_f(gp, px, py, m, hs)) * idg + gp(px, py);
+    }
+    __syncthreads();
+    // second sweep into x
+    auto gy = [&](int gx, int gyy) { return Y[gx + gyy * m]; };
+    for (int i = threadIdx.x; i < mm; i += MGC_THREADS)
+      x[i] = (b[i] - heat_apply_f(gy, i % m, i / m, m, hs)) * idg + Y[i];
+    __syncthreads();
+  }
+  float *xo = xout + field * nn;
+  double nrm = 0;
+  for (int i = threadIdx.x; i < (int)nn; i += MGC_THREADS) {
+    const float o = X[MGC_OFF[L] + i];
+    xo[i] = o;
+    nrm += (double)o * (double)o;
+  }
+  if (nrm_part) {
+    const double s = block_sum<MGC_THREADS>(nrm, red);
+    if (threadIdx.x == 0) nrm_part[field] = s;
+  }
+}
+
+}  // namespace msg

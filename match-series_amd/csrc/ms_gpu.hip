@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "ms_kernels.h"
+#include "ms_fast.h"
 #include "ms_compose.h"
 #include "ms_stats.h"
 
@@ -230,21 +231,23 @@ size_t energy_part_size(int n, int npairs) {
 // The images of the data term: R, T (SSD, NCC) or, for DT_CONS, the npsi deformations in T.
 template <typename S>
 int launch_energy(const S *R, const S *T, const S *phi, const S *dcg, const S *dgrad, int n, const PairState *st,
-                  double *part, int npairs, hipStream_t s, int data_term, double bytes = 0,
-                  const msg::Tail &tail = msg::Tail{}, int npsi = 0) {
+                  double *part, int npairs, hipStream_t s, int data_term, double bytes = 0, int npsi = 0) {
   ProfScope ps(KC_ENERGY, s, bytes);
   const EnergyGeom g = energy_geom(n);
   const double h = 1.0 / (double)(n - 1);
   const int width = (int)((n - 1) * h);
   if (data_term == msg::DT_CONS)
     hipLaunchKernelGGL((msg::k_energy_cons<S>), dim3(g.nblk * npairs), dim3(256), 0, s, T, npsi, phi, dcg, dgrad, n, h,
-                       width, st, part, g.ntx, g.nty, npairs, tail);
+                       width, st, part, g.ntx, g.nty, npairs);
   else if (data_term == msg::DT_NCC)
     hipLaunchKernelGGL((msg::k_energy_ncc<S>), dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, dcg, dgrad, n, h,
-                       st, part, g.ntx, g.nty, npairs, tail);
+                       st, part, g.ntx, g.nty, npairs);
+  else if constexpr (std::is_same<S, float>::value)
+    hipLaunchKernelGGL(msg::k_energy_f32, dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, dcg, dgrad, n, h * h, st,
+                       part, g.ntx, g.nty, npairs);
   else
     hipLaunchKernelGGL((msg::k_energy<S>), dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, dcg, dgrad, n, h, width,
-                       st, part, g.ntx, g.nty, npairs, tail);
+                       st, part, g.ntx, g.nty, npairs);
   HIPCHK(hipGetLastError());
   return MSG_OK;
 }
@@ -262,6 +265,9 @@ int launch_grad(const S *R, const S *T, const S *phi, S *DE, int n, const PairSt
   else if (data_term == msg::DT_NCC)
     hipLaunchKernelGGL((msg::k_grad<S, msg::DT_NCC>), dim3(ntx * nty * npairs), dim3(256), 0, s, R, T, phi, DE, n, h,
                        width, st, lambda, ntx, nty, npairs, 0);
+  else if constexpr (std::is_same<S, float>::value)
+    hipLaunchKernelGGL(msg::k_grad_f32, dim3(ntx * nty * npairs), dim3(256), 0, s, R, T, phi, DE, n, h * h, st, lambda,
+                       ntx, nty, npairs);
   else
     hipLaunchKernelGGL((msg::k_grad<S, msg::DT_SSD>), dim3(ntx * nty * npairs), dim3(256), 0, s, R, T, phi, DE, n, h,
                        width, st, lambda, ntx, nty, npairs, 0);
@@ -282,7 +288,7 @@ int ncc_reference_stats(const S *R, int n, PairState *st, double *part, double *
     a.phase = pass ? msg::PH_NCC_REF1 : msg::PH_NCC_REF0;
     a.nblk_energy = g.nblk;
     a.npairs = B;
-    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, st, part, part_n, part_c, a, flags, tr);
+    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, st, part, part_n, part_c, a, flags, tr);
     HIPCHK(hipGetLastError());
   }
   return MSG_OK;
@@ -304,14 +310,44 @@ int mg_norm_blocks(int l) {
   return cdiv(n, msg::MG_TX) * cdiv(n, msg::MG_TY);
 }
 
+// The V-cycle kernels of a field type: FP32 arithmetic for FP32 fields, FP64 for FP64 fields.
+template <typename S>
+void mg_coarse(dim3 g, hipStream_t s, const S *rhs, int negate, S *xout, int L, const PairState *st, double sigma,
+               int nfields, double *nrm_part) {
+  if constexpr (std::is_same<S, float>::value)
+    hipLaunchKernelGGL(msg::k_mg_coarse_f32, g, dim3(msg::MGC_THREADS), 0, s, rhs, negate, xout, L, st, sigma, nfields,
+                       nrm_part);
+  else
+    hipLaunchKernelGGL((msg::k_mg_coarse<S>), g, dim3(256), 0, s, rhs, negate, xout, L, st, sigma, nfields, nrm_part);
+}
+template <typename S>
+void mg_down(dim3 g, hipStream_t s, const S *rhs, int negate, S *x2out, S *rhsc, int n, const PairState *st,
+             double sigma, int ntx, int nty, int nfields) {
+  if constexpr (std::is_same<S, float>::value)
+    hipLaunchKernelGGL(msg::k_mg_down_f32, g, dim3(256), 0, s, rhs, negate, x2out, rhsc, n, st, sigma, ntx, nty,
+                       nfields);
+  else
+    hipLaunchKernelGGL((msg::k_mg_down<S>), g, dim3(256), 0, s, rhs, negate, x2out, rhsc, n, st, sigma, ntx, nty,
+                       nfields);
+}
+template <typename S>
+void mg_up(dim3 g, hipStream_t s, const S *x2in, const S *xc, const S *rhs, int negate, S *xout, int n,
+           const PairState *st, double sigma, int ntx, int nty, int nfields, double *nrm_part) {
+  if constexpr (std::is_same<S, float>::value)
+    hipLaunchKernelGGL(msg::k_mg_up_f32, g, dim3(256), 0, s, x2in, xc, rhs, negate, xout, n, st, sigma, ntx, nty,
+                       nfields, nrm_part);
+  else
+    hipLaunchKernelGGL((msg::k_mg_up<S>), g, dim3(256), 0, s, x2in, xc, rhs, negate, xout, n, st, sigma, ntx, nty,
+                       nfields, nrm_part);
+}
+
 // out = V-cycle(sign * in) for nfields fields of level l (LinearSmoothOp::apply).
 template <typename S>
 int launch_vcycle(const S *in, int negate, S *out, int l, const MGBufs<S> &b, const PairState *st, double sigma,
-                  int nfields, double *nrm_part, hipStream_t s, const msg::Tail &tail = msg::Tail{}) {
+                  int nfields, double *nrm_part, hipStream_t s) {
   ProfScope ps(KC_VCYCLE, s, 0);
   if (l <= msg::MGC_MAXL) {
-    hipLaunchKernelGGL((msg::k_mg_coarse<S>), dim3(nfields), dim3(256), 0, s, in, negate, out, l, st, sigma, nfields,
-                       nrm_part, tail);
+    mg_coarse<S>(dim3(nfields), s, in, negate, out, l, st, sigma, nfields, nrm_part);
     HIPCHK(hipGetLastError());
     return MSG_OK;
   }
@@ -322,12 +358,11 @@ int launch_vcycle(const S *in, int negate, S *out, int l, const MGBufs<S> &b, co
     const S *src = (m == l) ? in : b.rhs[m];
     const int neg = (m == l) ? negate : 0;
     S *x2 = (m == l) ? b.x2_top : b.x2[m];
-    hipLaunchKernelGGL((msg::k_mg_down<S>), dim3(ntx * nty * nfields), dim3(256), 0, s, src, neg, x2, b.rhs[m - 1], n,
-                       st, sigma, ntx, nty, nfields);
+    mg_down<S>(dim3(ntx * nty * nfields), s, src, neg, x2, b.rhs[m - 1], n, st, sigma, ntx, nty, nfields);
     HIPCHK(hipGetLastError());
   }
-  hipLaunchKernelGGL((msg::k_mg_coarse<S>), dim3(nfields), dim3(256), 0, s, b.rhs[msg::MGC_MAXL], 0,
-                     b.x[msg::MGC_MAXL], msg::MGC_MAXL, st, sigma, nfields, (double *)nullptr, msg::Tail{});
+  mg_coarse<S>(dim3(nfields), s, b.rhs[msg::MGC_MAXL], 0, b.x[msg::MGC_MAXL], msg::MGC_MAXL, st, sigma, nfields,
+               (double *)nullptr);
   HIPCHK(hipGetLastError());
   for (int m = msg::MGC_MAXL + 1; m <= l; ++m) {
     const int n = 1 << m;
@@ -336,9 +371,8 @@ int launch_vcycle(const S *in, int negate, S *out, int l, const MGBufs<S> &b, co
     const S *rhs = (m == l) ? in : b.rhs[m];
     const int neg = (m == l) ? negate : 0;
     S *xo = (m == l) ? out : b.x[m];
-    hipLaunchKernelGGL((msg::k_mg_up<S>), dim3(ntx * nty * nfields), dim3(256), 0, s, x2, b.x[m - 1], rhs, neg, xo, n,
-                       st, sigma, ntx, nty, nfields, (m == l) ? nrm_part : (double *)nullptr,
-                       (m == l) ? tail : msg::Tail{});
+    mg_up<S>(dim3(ntx * nty * nfields), s, x2, b.x[m - 1], rhs, neg, xo, n, st, sigma, ntx, nty, nfields,
+             (m == l) ? nrm_part : (double *)nullptr);
     HIPCHK(hipGetLastError());
   }
   return MSG_OK;
@@ -399,7 +433,6 @@ struct msg_ctx {
   double *coord = nullptr;
   PairState *st = nullptr;
   int *flags_d = nullptr, *flags_h = nullptr;
-  unsigned *counters = nullptr;  // last-workgroup counters: [4][B] (V-cycle, CG direction, energy, update)
   msg::IterTrace *trace_d = nullptr;
   int trace_cap = 0;
   hipStream_t stream = nullptr;
@@ -544,32 +577,34 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
   HIPCHK(hipGetLastError());
   if (c->data_term == msg::DT_NCC)
     CHK(ncc_reference_stats<S>(R, n, c->st, c->part_e, c->part_n, c->part_c, B, a, c->flags_d, tr, s));
-  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, msg::Tail{}, npsi));
+  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi));
   a.phase = msg::PH_LEVEL_INIT;
-  hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
+  a.slot = 0;
+  hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
   HIPCHK(hipGetLastError());
 
   bool any_active = max_iter > 0;
   int nactive = max_iter > 0 ? B : 0;
-  while (any_active) {
+  for (int it = 0; any_active; ++it) {
+    a.slot = it & 1;
     CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s, c->data_term, npsi));
     CHK(launch_vcycle<S>(DE, 1, z, l, mg, c->st, 1.0, 2 * B, c->part_n, s));
     a.phase = msg::PH_BETA;
-    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
+    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL((msg::k_ncg<S>), dim3(nblk_ncg * 2 * B), dim3(256), 0, s, z, d, DE, n, c->st, c->part_c, nblk_ncg,
-                       2 * B, msg::Tail{});
+                       2 * B);
     HIPCHK(hipGetLastError());
     a.phase = msg::PH_ARMIJO_INIT;
-    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
+    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
     HIPCHK(hipGetLastError());
     bool searching = true;
     int rounds = 0;
     int nsearch = nactive;  // pairs evaluating in this round (upper bound for the first round)
     while (searching) {
-      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, msg::Tail{}, npsi));
+      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi));
       a.phase = msg::PH_ARMIJO_STEP;
-      hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d,
+      hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d,
                          tr);
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(c->flags_h, c->flags_d, sizeof(int) * B, hipMemcpyDeviceToHost, s));
@@ -587,10 +622,7 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
       if (++rounds > 200) return fail(MSG_ERR_STATE, "Armijo search did not terminate");
     }
     hipLaunchKernelGGL((msg::k_update<S>), dim3(nblk_upd * B), dim3(256), 0, s, phi, d, z, n, c->st, nblk_upd, B,
-                       (unsigned *)nullptr);
-    HIPCHK(hipGetLastError());
-    a.phase = msg::PH_APPLIED;
-    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
+                       a.slot);
     HIPCHK(hipGetLastError());
   }
 
@@ -623,9 +655,10 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
 }
 
 // ------------------------------------------------------------------------------------------------
-// Device-driven level solve: every control phase runs in the last workgroup of the kernel that
-// produced its partial sums, so one iteration is a fixed launch sequence (variation, V-cycle,
-// CG direction, R energy passes, update) with no host round trip. The host replays K iterations
+// Device-driven level solve: every control phase is a one-workgroup-per-pair kernel after the pass
+// whose partial sums it reduces, so one iteration is a fixed launch sequence (variation, V-cycle,
+// control, CG direction, control, R energy passes each with its control, update) with no host round
+// trip. The host replays K iterations
 // (as one hipGraph in mode 2) and then reads the per-pair flags once. A pair whose step search is
 // still open after the R passes simply sits out the next iteration's gradient work and finishes the
 // search in its energy passes, so the result is that of the host-synchronised loop.
@@ -678,15 +711,17 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
   a.sigma_fixed = c->sigma_fixed;
   a.sigma0 = c->sigma0;
   a.grad_stop = c->grad_stop;
+  a.slot = 0;
   // Each control phase is its own one-workgroup-per-pair kernel after the pass whose partial sums it
   // reduces: kernel boundaries make the partials visible across the XCDs (a last-workgroup tail would
   // need an agent-scope release by every workgroup, an L2 write-back on this part, which costs far
   // more than the extra launch).
-  auto control = [&](int phase) -> int {
+  auto control = [&](int phase, int slot) -> int {
     msg::CtrlArgs ap = a;
     ap.phase = phase;
+    ap.slot = slot;
     ProfScope ps(KC_CONTROL, s, 0);
-    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(256), 0, s, c->st, c->part_e, c->part_n, c->part_c, ap,
+    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, c->st, c->part_e, c->part_n, c->part_c, ap,
                        c->flags_d, tr);
     HIPCHK(hipGetLastError());
     return MSG_OK;
@@ -696,31 +731,32 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
   HIPCHK(hipGetLastError());
   if (c->data_term == msg::DT_NCC)
     CHK(ncc_reference_stats<S>(R, n, c->st, c->part_e, c->part_n, c->part_c, B, a, c->flags_d, tr, s));
-  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, msg::Tail{}, npsi));
-  CHK(control(msg::PH_LEVEL_INIT));
+  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi));
+  CHK(control(msg::PH_LEVEL_INIT, 0));
 
-  auto iteration = [&]() -> int {
+  // iteration k of a replay uses step slot k & 1 (K is even, so the slots keep alternating)
+  auto iteration = [&](int k) -> int {
+    const int slot = k & 1;
     CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s, c->data_term, npsi));
     CHK(launch_vcycle<S>(DE, 1, z, l, mg, c->st, 1.0, 2 * B, c->part_n, s));
-    CHK(control(msg::PH_BETA));
+    CHK(control(msg::PH_BETA, slot));
     {
       ProfScope ps(KC_NCG, s, 0);
       hipLaunchKernelGGL((msg::k_ncg<S>), dim3(nblk_ncg * 2 * B), dim3(256), 0, s, z, d, DE, n, c->st, c->part_c,
-                         nblk_ncg, 2 * B, msg::Tail{});
+                         nblk_ncg, 2 * B);
       HIPCHK(hipGetLastError());
     }
-    CHK(control(msg::PH_ARMIJO_INIT));
+    CHK(control(msg::PH_ARMIJO_INIT, slot));
     for (int r = 0; r < Rr; ++r) {
-      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, msg::Tail{}, npsi));
-      CHK(control(msg::PH_ARMIJO_STEP));
+      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi));
+      CHK(control(msg::PH_ARMIJO_STEP, slot));
     }
     {
       ProfScope ps(KC_UPDATE, s, 0);
       hipLaunchKernelGGL((msg::k_update<S>), dim3(nblk_upd * B), dim3(256), 0, s, phi, d, z, n, c->st, nblk_upd, B,
-                         (unsigned *)nullptr);
+                         slot);
       HIPCHK(hipGetLastError());
     }
-    CHK(control(msg::PH_APPLIED));
     return MSG_OK;
   };
 
@@ -734,7 +770,7 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
       HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       g_capturing = true;
       int rc = MSG_OK;
-      for (int k = 0; k < K && rc == MSG_OK; ++k) rc = iteration();
+      for (int k = 0; k < K && rc == MSG_OK; ++k) rc = iteration(k);
       g_capturing = false;
       hipGraph_t graph = nullptr;
       const hipError_t ec = hipStreamEndCapture(s, &graph);
@@ -760,7 +796,7 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
     if (use_graph) {
       HIPCHK(hipGraphLaunch(exec, s));
     } else {
-      for (int k = 0; k < K; ++k) CHK(iteration());
+      for (int k = 0; k < K; ++k) CHK(iteration(k));
     }
     HIPCHK(hipMemcpyAsync(c->flags_h, c->flags_d, sizeof(int) * B, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -941,7 +977,6 @@ int msg_ctx_create(msg_ctx **out, int max_level, int npairs, int dtype) {
   A((void **)&c->part_c, sizeof(double) * B * 2 * msg::NCG_TERMS * cdiv(N * N, 256 * msg::NCG_PER_THREAD));
   A((void **)&c->st, sizeof(PairState) * B);
   A((void **)&c->flags_d, sizeof(int) * B);
-  A((void **)&c->counters, sizeof(unsigned) * 4 * B);
   if (rc != MSG_OK) {
     msg_ctx_destroy(c);
     return rc;
@@ -954,7 +989,6 @@ int msg_ctx_create(msg_ctx **out, int max_level, int npairs, int dtype) {
     msg_ctx_destroy(c);
     return fail(MSG_ERR_HIP, "stream creation failed");
   }
-  (void)hipMemsetAsync(c->counters, 0, sizeof(unsigned) * 4 * B, c->stream);
   // zero images and deformations so that a context is usable before it is filled
   for (int l = 0; l <= max_level; ++l) {
     const size_t nn = (size_t)1 << (2 * l);
@@ -1154,7 +1188,7 @@ int ncc_energy_impl(const void *R, const void *T, const void *phi, int n, int ba
   CHK(launch_energy<S>((const S *)R, (const S *)T, (const S *)phi, (const S *)phi, (const S *)phi, n, st,
                        (double *)part, batch, s, msg::DT_NCC));
   a.phase = msg::PH_LEVEL_INIT;
-  hipLaunchKernelGGL(msg::k_control, dim3(batch), dim3(256), 0, s, st, (const double *)part, (const double *)nullptr,
+  hipLaunchKernelGGL(msg::k_control, dim3(batch), dim3(msg::CTRL_THREADS), 0, s, st, (const double *)part, (const double *)nullptr,
                      (const double *)nullptr, a, (int *)nullptr, (msg::IterTrace *)nullptr);
   HIPCHK(hipGetLastError());
   std::vector<PairState> hs(batch);
@@ -1331,51 +1365,74 @@ namespace {
 
 template <typename S>
 int compose_impl(const void *phi1, const void *phi2, void *result, int n, hipStream_t s) {
+  ProfScope ps(KC_COMPOSE, s, 0);
   const int nn = n * n;
   const int nblk = cdiv(nn, 256);
-  void *bc, *rhs, *invd, *x, *g, *hb, *d, *def, *part, *part2, *scp;
+  void *bc, *rhs, *invd, *x, *g, *hb, *h2, *d0, *d1, *def, *part, *parta, *partb, *U, *nsel, *sel_tmp, *stp, *donep;
   CHK(scratch_get(70, sizeof(double) * 2 * nn, &bc));
   CHK(scratch_get(71, sizeof(double) * 2 * nn, &rhs));
   CHK(scratch_get(72, sizeof(double) * nn, &invd));
   CHK(scratch_get(73, sizeof(double) * 2 * nn, &x));
   CHK(scratch_get(74, sizeof(double) * 2 * nn, &g));
   CHK(scratch_get(75, sizeof(double) * 2 * nn, &hb));
-  CHK(scratch_get(76, sizeof(double) * 2 * nn, &d));
-  CHK(scratch_get(77, nn, &def));
-  CHK(scratch_get(78, sizeof(double) * 4 * nblk, &part));
-  CHK(scratch_get(79, sizeof(double) * 2 * nblk, &part2));
-  CHK(scratch_get(80, sizeof(msg::PcgScalars) * 2, &scp));
-  msg::PcgScalars *sc = (msg::PcgScalars *)scp;
-  const dim3 g1(nblk), g2(nblk, 2), b(256);
+  CHK(scratch_get(76, sizeof(double) * 2 * nn, &h2));
+  CHK(scratch_get(77, sizeof(double) * 2 * nn, &d0));
+  CHK(scratch_get(78, sizeof(double) * 2 * nn, &d1));
+  CHK(scratch_get(79, nn, &def));
+  CHK(scratch_get(80, sizeof(double) * 6 * nblk, &part));
+  CHK(scratch_get(91, sizeof(double) * 2 * nblk, &parta));
+  CHK(scratch_get(92, sizeof(double) * 4 * nblk, &partb));
+  CHK(scratch_get(93, sizeof(int) * nn, &U));
+  CHK(scratch_get(94, sizeof(int) * 4, &nsel));
+  CHK(scratch_get(95, sizeof(msg::CgState) * 4, &stp));
+  CHK(scratch_get(96, sizeof(int) * 2, &donep));
+  msg::CgState *sa = (msg::CgState *)stp, *sb = sa + 2;
+  int *done = (int *)donep;
+  const dim3 g1(nblk), b(256);
   hipLaunchKernelGGL((msg::k_compose_init<S>), g1, b, 0, s, (const S *)phi1, (const S *)phi2, (double *)bc,
                      (unsigned char *)def, n);
   hipLaunchKernelGGL(msg::k_compose_rhs, dim3(cdiv(2 * nn, 256)), b, 0, s, (const double *)bc,
                      (const unsigned char *)def, (double *)rhs, (double *)invd, n);
-  CHK(convert(result, sizeof(S) == 8 ? MSG_F64 : MSG_F32, x, MSG_F64, 2 * (size_t)nn, s));
-  hipLaunchKernelGGL((msg::k_pcg_start<S>), g2, b, 0, s, (const S *)nullptr, (double *)x, (const double *)rhs,
-                     (const unsigned char *)def, (const double *)invd, (double *)g, (double *)hb, (double *)d,
-                     (double *)part, n, nblk);
-  hipLaunchKernelGGL(msg::k_dot_blocks, g2, b, 0, s, (const double *)g, (const double *)hb, (double *)part2, n, nblk);
-  hipLaunchKernelGGL(msg::k_pcg_scalar, dim3(2), b, 0, s, sc, (const double *)part, nblk, 0, (const double *)part2);
   HIPCHK(hipGetLastError());
-  msg::PcgScalars hsc[2];
-  for (int it = 0; it < 1000;) {
-    HIPCHK(hipMemcpyAsync(hsc, sc, sizeof(hsc), hipMemcpyDeviceToHost, s));
+  // the undefined nodes, in index order
+  size_t tbytes = 0;
+  const msg::UndefinedNode pred{(const unsigned char *)def};
+  HIPCHK(hipcub::DeviceSelect::If(nullptr, tbytes, hipcub::CountingInputIterator<int>(0), (int *)U, (int *)nsel, nn,
+                                  pred, s));
+  CHK(scratch_get(97, tbytes, &sel_tmp));
+  HIPCHK(hipcub::DeviceSelect::If(sel_tmp, tbytes, hipcub::CountingInputIterator<int>(0), (int *)U, (int *)nsel, nn,
+                                  pred, s));
+  // the start value is the result's previous content (PCGInverse::apply starts from its Dest)
+  CHK(convert(result, sizeof(S) == 8 ? MSG_F64 : MSG_F32, x, MSG_F64, 2 * (size_t)nn, s));
+  hipLaunchKernelGGL(msg::k_cg_start, dim3(nblk, 2), b, 0, s, (const double *)x, (const double *)rhs,
+                     (const unsigned char *)def, (const double *)invd, (double *)g, (double *)hb, (double *)d0,
+                     (double *)part, n, nblk);
+  hipLaunchKernelGGL(msg::k_cg_init, dim3(2), b, 0, s, (const double *)part, nblk, sb, done);
+  HIPCHK(hipGetLastError());
+  int nu = 0;
+  HIPCHK(hipMemcpyAsync(&nu, nsel, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const int nbu = std::max(1, cdiv(nu, 256));
+  double *dbuf[2] = {(double *)d0, (double *)d1};
+  int hdone[2] = {0, 0};
+  for (int k = 0; k <= 1000;) {
+    HIPCHK(hipMemcpyAsync(hdone, done, sizeof(hdone), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    if (hsc[0].done && hsc[1].done) break;
-    const int chunk = 8;
-    for (int k = 0; k < chunk; ++k, ++it) {
-      hipLaunchKernelGGL(msg::k_pcg_apply, g2, b, 0, s, (const double *)d, (const unsigned char *)def, (double *)hb,
-                         (double *)part, (const msg::PcgScalars *)sc, n, nblk);
-      hipLaunchKernelGGL(msg::k_pcg_scalar, dim3(2), b, 0, s, sc, (const double *)part, nblk, 1, (const double *)nullptr);
-      hipLaunchKernelGGL(msg::k_pcg_update, g2, b, 0, s, (double *)x, (const double *)d, (double *)g, (double *)hb,
-                         (const double *)invd, (double *)part, (const msg::PcgScalars *)sc, n, nblk);
-      hipLaunchKernelGGL(msg::k_pcg_scalar, dim3(2), b, 0, s, sc, (const double *)part, nblk, 2, (const double *)nullptr);
-      hipLaunchKernelGGL(msg::k_pcg_dir, g2, b, 0, s, (double *)d, (const double *)hb, (const msg::PcgScalars *)sc, n);
+    if (hdone[0] && hdone[1]) break;
+    for (int r = 0; r < 8 && k <= 1000; ++r, ++k) {
+      const double *dp = dbuf[k > 0 ? (k - 1) & 1 : 0];
+      double *dc = dbuf[k & 1];
+      hipLaunchKernelGGL(msg::k_cg_a, dim3(nbu, 2), b, 0, s, k, (const int *)U, nu, (const unsigned char *)def, dp, dc,
+                         (const double *)hb, (double *)h2, (const double *)partb, (double *)parta, nbu,
+                         (const msg::CgState *)sb, sa, done, n);
+      hipLaunchKernelGGL(msg::k_cg_b, dim3(nbu, 2), b, 0, s, (const int *)U, nu, (double *)x, (const double *)dc,
+                         (double *)g, (double *)hb, (const double *)h2, (const double *)invd, (const double *)parta,
+                         (double *)partb, nbu, (const msg::CgState *)sa, sb, (const int *)done, n);
     }
     HIPCHK(hipGetLastError());
   }
-  hipLaunchKernelGGL((msg::k_store<S>), dim3(cdiv(2 * nn, 256)), b, 0, s, (const double *)x, (S *)result, 2 * nn);
+  hipLaunchKernelGGL((msg::k_cg_finish<S>), dim3(cdiv(2 * nn, 256)), b, 0, s, (const double *)x, (const double *)rhs,
+                     (const unsigned char *)def, (const msg::CgState *)sb, (S *)result, n);
   HIPCHK(hipGetLastError());
   return MSG_OK;
 }

@@ -16,15 +16,55 @@ constexpr double ARMIJO_SIGMA = 0.5;
 // Control: one workgroup per pair reduces the partial sums of the previous pass (fixed order) and
 // advances the pair's state machine. Phases:
 // ================================================================================================
-// PH_APPLIED: the update kernel has applied every pending step (step_tau back to 0).
 // PH_NCC_REF0 / PH_NCC_REF1: the reference's mean, then its centred moments (NCC, once per level).
-enum CtrlPhase { PH_LEVEL_INIT = 0, PH_BETA = 1, PH_ARMIJO_INIT = 2, PH_ARMIJO_STEP = 3, PH_APPLIED = 4,
-                 PH_NCC_REF0 = 5, PH_NCC_REF1 = 6 };
+// An accepted step goes to the step slot of its iteration (iterations alternate between two slots);
+// the update kernel of that iteration applies it and the next iteration's PH_BETA clears the slot,
+// so no phase of its own is needed after the update.
+enum CtrlPhase { PH_LEVEL_INIT = 0, PH_BETA = 1, PH_ARMIJO_INIT = 2, PH_ARMIJO_STEP = 3, PH_NCC_REF0 = 5,
+                 PH_NCC_REF1 = 6 };
 
-__device__ double reduce_fixed(const double *p, int count, double *red) {
-  double s = 0;
-  for (int i = threadIdx.x; i < count; i += 256) s += p[i];
-  return block_sum<256>(s, red);  // valid in thread 0
+constexpr int CTRL_THREADS = 1024;  // threads of a control workgroup
+constexpr int CTRL_MAXT = 16;       // partial arrays one control sweep can reduce
+
+// Sums of nt consecutive partial arrays base[t * len .. t * len + len) (nt <= CTRL_MAXT) in one
+// sweep of the control workgroup. The partials were just written by a kernel on other XCDs, so
+// every load is a memory round trip: each thread takes a contiguous run of CTRL_RUN entries of every
+// array and issues all of its loads before adding, which keeps the sweep at one or two round trips
+// for the partial counts of a 1024^2 level (at most 4096 per array). Fixed order; the sums land in
+// out[0 .. nt) (shared memory), valid for every thread after the call.
+constexpr int CTRL_RUN = 4;
+__device__ void reduce_arrays(const double *base, int nt, int len, double *out) {
+  __shared__ double red[CTRL_THREADS / 64][CTRL_MAXT];
+  double acc[CTRL_MAXT];
+#pragma unroll
+  for (int t = 0; t < CTRL_MAXT; ++t) acc[t] = 0.;
+  for (int i0 = threadIdx.x * CTRL_RUN; i0 < len; i0 += CTRL_THREADS * CTRL_RUN) {
+    double v[CTRL_MAXT][CTRL_RUN];
+#pragma unroll
+    for (int t = 0; t < CTRL_MAXT; ++t)
+#pragma unroll
+      for (int r = 0; r < CTRL_RUN; ++r) v[t][r] = (t < nt && i0 + r < len) ? base[(size_t)t * len + i0 + r] : 0.;
+#pragma unroll
+    for (int t = 0; t < CTRL_MAXT; ++t)
+#pragma unroll
+      for (int r = 0; r < CTRL_RUN; ++r) acc[t] += v[t][r];
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int t = 0; t < CTRL_MAXT; ++t)
+    if (t < nt) acc[t] = wave_sum(acc[t]);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int t = 0; t < CTRL_MAXT; ++t)
+      if (t < nt) red[wid][t] = acc[t];
+  __syncthreads();
+  if (threadIdx.x < nt) {
+    double v = 0.;
+    for (int w = 0; w < CTRL_THREADS / 64; ++w) v += red[w][threadIdx.x];
+    out[threadIdx.x] = v;
+  }
+  __syncthreads();
 }
 
 struct CtrlArgs {
@@ -41,6 +81,7 @@ struct CtrlArgs {
   int sigma_fixed;   // 1: filter width fixed at sigma0 (H1GradientDescent, gradientDescent.h:639-650)
   double sigma0;
   int grad_stop;     // 1: USE_GRADIENT_BASED_STOPPING (gradientDescent.h:416)
+  int slot;          // step slot of the current iteration (0 or 1)
 };
 constexpr int NCG_TERMS = 3;  // per field and block: <DE,d>, <DE,z>, <DE,DE>
 
@@ -143,9 +184,9 @@ __device__ void armijo_start(PairState &s, int dirmode, double Df, double oldTau
 }
 
 // End of an iteration (GradientDescentBase::applySingle L361-439).
-__device__ void finish_iteration(PairState &s, double t, IterTrace *trace, int trace_cap, int pair) {
-  s.step_tau = t;
-  s.step_dir = s.dirmode;
+__device__ void finish_iteration(PairState &s, double t, IterTrace *trace, int trace_cap, int pair, int slot) {
+  s.step_tau[slot] = t;
+  s.step_dir[slot] = s.dirmode;
   double En = s.energy, Rn = s.ereg_new;
   if (t != 0) {
     EvalRec ev;
@@ -195,7 +236,7 @@ __device__ void finish_iteration(PairState &s, double t, IterTrace *trace, int t
 
 // Replays getTimestepWidthWithArmijoLineSearch (ArmijoSearch.h:118-200) over the evaluations
 // available, including the fall-back to the smoothed gradient (gradientDescent.h:231-235).
-__device__ void armijo_run(PairState &s, int spec, IterTrace *trace, int trace_cap, int pair) {
+__device__ void armijo_run(PairState &s, int spec, IterTrace *trace, int trace_cap, int pair, int slot) {
   const double f = s.energy;
   for (int guard = 0; guard < 256; ++guard) {
     double t = s.a_tau;
@@ -252,27 +293,24 @@ __device__ void armijo_run(PairState &s, int spec, IterTrace *trace, int trace_c
         continue;
       }
     }
-    finish_iteration(s, result, trace, trace_cap, pair);
+    finish_iteration(s, result, trace, trace_cap, pair, slot);
     return;
   }
-  finish_iteration(s, 0.0, trace, trace_cap, pair);  // unreachable guard
+  finish_iteration(s, 0.0, trace, trace_cap, pair, slot);  // unreachable guard
 }
 
-// Runs one control phase for one pair with the whole workgroup (the reductions need every thread).
-// Called by k_control (one workgroup per pair) or by the last workgroup of the kernel that produced
-// the partial sums (device-driven iterations).
-__device__ void control_pair(int pair, PairState *__restrict__ st, const double *__restrict__ part_e,
+// Runs one control phase for one pair with the whole workgroup (the reductions need every thread)
+// on the workgroup's copy s of the pair state.
+__device__ void control_pair(int pair, PairState &s, const double *__restrict__ part_e,
                              const double *__restrict__ part_n, const double *__restrict__ part_c, const CtrlArgs &a,
                              int *__restrict__ flags, IterTrace *__restrict__ trace) {
-  PairState &s = st[pair];
-  __shared__ double red[4];
-  __shared__ double sums[4 * MAX_CAND];
-  const bool participates = s.active && !s.searching && s.step_tau == 0;
+  __shared__ double sums[CTRL_MAXT];
+  const bool participates = s.active && !s.searching;
 
   if (a.phase == PH_NCC_REF0 || a.phase == PH_NCC_REF1) {
     const int nb = a.nblk_energy;
-    const double r1 = reduce_fixed(part_e + part_index_ref(a.npairs, pair, 0, nb), nb, red);
-    const double r2 = a.phase == PH_NCC_REF1 ? reduce_fixed(part_e + part_index_ref(a.npairs, pair, 1, nb), nb, red) : 0;
+    reduce_arrays(part_e + part_index_ref(a.npairs, pair, 0, nb), a.phase == PH_NCC_REF1 ? 2 : 1, nb, sums);
+    const double r1 = sums[0], r2 = a.phase == PH_NCC_REF1 ? sums[1] : 0;
     if (threadIdx.x == 0) {
       if (a.phase == PH_NCC_REF0) {
         s.ncc_rm = r1;  // 1'Mr
@@ -285,11 +323,7 @@ __device__ void control_pair(int pair, PairState *__restrict__ st, const double 
   } else if ((a.phase == PH_LEVEL_INIT || a.phase == PH_ARMIJO_STEP) && a.data_term == DT_NCC) {
     const int nc = s.ncand;
     const int nb = a.nblk_energy;
-    for (int k = 0; k < nc; ++k)
-      for (int t = 0; t < 4; ++t) {
-        const double v = reduce_fixed(part_e + part_index(DT_NCC, pair, k, t, nb), nb, red);
-        if (threadIdx.x == 0) sums[4 * k + t] = v;
-      }
+    reduce_arrays(part_e + part_index(DT_NCC, pair, 0, 0, nb), 4 * nc, nb, sums);  // candidate k, term t at 4k+t
     if (threadIdx.x == 0) {
       for (int k = 0; k < nc; ++k) {
         double vt;
@@ -306,14 +340,7 @@ __device__ void control_pair(int pair, PairState *__restrict__ st, const double 
     }
   } else if (a.phase == PH_LEVEL_INIT || a.phase == PH_ARMIJO_STEP) {
     const int nc = s.ncand;
-    for (int k = 0; k < nc; ++k) {
-      const double e = reduce_fixed(part_e + ((size_t)(pair * MAX_CAND + k) * 2 + 0) * a.nblk_energy, a.nblk_energy, red);
-      const double r = reduce_fixed(part_e + ((size_t)(pair * MAX_CAND + k) * 2 + 1) * a.nblk_energy, a.nblk_energy, red);
-      if (threadIdx.x == 0) {
-        sums[2 * k] = e;
-        sums[2 * k + 1] = r;
-      }
-    }
+    reduce_arrays(part_e + part_index(DT_SSD, pair, 0, 0, a.nblk_energy), 2 * nc, a.nblk_energy, sums);
     if (threadIdx.x == 0) {
       for (int k = 0; k < nc; ++k) {
         double E = 0;
@@ -347,15 +374,17 @@ __device__ void control_pair(int pair, PairState *__restrict__ st, const double 
         s.olddir_zero = 1;
         s.zz_old = 0;
         s.total_evals = s.total_passes = 0;
-        s.step_tau = 0;
+        s.step_tau[0] = s.step_tau[1] = 0;
       } else if (s.searching) {
-        armijo_run(s, a.spec, trace, a.trace_cap, pair);
+        armijo_run(s, a.spec, trace, a.trace_cap, pair, a.slot);
       }
     }
   } else if (a.phase == PH_BETA) {
+    // the previous iteration's update has run (stream order): free its slot
+    if (threadIdx.x == 0) s.step_tau[a.slot ^ 1] = 0;
     if (!participates) return;
-    const double z0 = reduce_fixed(part_n + (size_t)(2 * pair) * a.nblk_norm, a.nblk_norm, red);
-    const double z1 = reduce_fixed(part_n + (size_t)(2 * pair + 1) * a.nblk_norm, a.nblk_norm, red);
+    reduce_arrays(part_n + (size_t)(2 * pair) * a.nblk_norm, 2, a.nblk_norm, sums);
+    const double z0 = sums[0], z1 = sums[1];
     if (threadIdx.x == 0) {
       double zz = 0;
       zz += z0;
@@ -368,15 +397,10 @@ __device__ void control_pair(int pair, PairState *__restrict__ st, const double 
     }
   } else if (a.phase == PH_ARMIJO_INIT) {
     if (!participates) return;
-    const double d0 = reduce_fixed(part_c + ((size_t)(2 * pair) * NCG_TERMS + 0) * a.nblk_ncg, a.nblk_ncg, red);
-    const double d1 = reduce_fixed(part_c + ((size_t)(2 * pair + 1) * NCG_TERMS + 0) * a.nblk_ncg, a.nblk_ncg, red);
-    const double g0 = reduce_fixed(part_c + ((size_t)(2 * pair) * NCG_TERMS + 1) * a.nblk_ncg, a.nblk_ncg, red);
-    const double g1 = reduce_fixed(part_c + ((size_t)(2 * pair + 1) * NCG_TERMS + 1) * a.nblk_ncg, a.nblk_ncg, red);
-    double n0 = 0, n1 = 0;
-    if (a.grad_stop) {
-      n0 = reduce_fixed(part_c + ((size_t)(2 * pair) * NCG_TERMS + 2) * a.nblk_ncg, a.nblk_ncg, red);
-      n1 = reduce_fixed(part_c + ((size_t)(2 * pair + 1) * NCG_TERMS + 2) * a.nblk_ncg, a.nblk_ncg, red);
-    }
+    // field f, term t at f * NCG_TERMS + t
+    reduce_arrays(part_c + (size_t)(2 * pair) * NCG_TERMS * a.nblk_ncg, 2 * NCG_TERMS, a.nblk_ncg, sums);
+    const double d0 = sums[0], g0 = sums[1], d1 = sums[NCG_TERMS], g1 = sums[NCG_TERMS + 1];
+    const double n0 = a.grad_stop ? sums[2] : 0, n1 = a.grad_stop ? sums[NCG_TERMS + 2] : 0;
     if (threadIdx.x == 0) {
       double Df = 0, Dfz = 0;
       Df += d0;
@@ -395,60 +419,36 @@ __device__ void control_pair(int pair, PairState *__restrict__ st, const double 
       s.tau_first = 0;
       armijo_start(s, DIR_CG, Df, s.tau);
       s.searching = 1;
-      armijo_run(s, a.spec, trace, a.trace_cap, pair);
+      armijo_run(s, a.spec, trace, a.trace_cap, pair, a.slot);
     }
   }
-  if (a.phase == PH_APPLIED && threadIdx.x == 0) s.step_tau = 0;
   if (threadIdx.x == 0 && flags) flags[pair] = (s.searching ? 1 : 0) | (s.active ? 2 : 0);
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(CTRL_THREADS)
 k_control(PairState *__restrict__ st, const double *__restrict__ part_e, const double *__restrict__ part_n,
           const double *__restrict__ part_c, CtrlArgs a, int *__restrict__ flags, IterTrace *__restrict__ trace) {
-  control_pair(blockIdx.x, st, part_e, part_n, part_c, a, flags, trace);
+  // The state machine runs on an LDS copy of the pair state: one coalesced load and one store
+  // instead of a chain of dependent global accesses by the thread that advances it.
+  static_assert(sizeof(PairState) % 4 == 0, "pair state copied in 32-bit words");
+  constexpr int NW = sizeof(PairState) / 4;
+  __shared__ PairState ss;
+  const int pair = blockIdx.x;
+  unsigned *g = reinterpret_cast<unsigned *>(st + pair), *l = reinterpret_cast<unsigned *>(&ss);
+  for (int i = threadIdx.x; i < NW; i += CTRL_THREADS) l[i] = g[i];
+  __syncthreads();
+  control_pair(pair, ss, part_e, part_n, part_c, a, flags, trace);
+  __syncthreads();
+  for (int i = threadIdx.x; i < NW; i += CTRL_THREADS) g[i] = l[i];
 }
 
-// True in exactly one workgroup of a group of `total` workgroups: the last to arrive. Every
-// workgroup's thread 0 must have stored its partial results before the call; the release/acquire
-// pair (agent scope) makes them visible to the last workgroup whatever XCD wrote them. The counter
-// is reset by the last workgroup, ready for the next launch.
-__device__ __forceinline__ bool last_block_of(unsigned *counter, unsigned total) {
-  __shared__ unsigned prev;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __syncthreads();
-  const bool last = (prev == total - 1);
-  if (last) {
-    __threadfence();
-    if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-  }
-  return last;
-}
 
-// Where a kernel's control tail sends its results (device-driven iterations); null: no tail.
-struct Tail {
-  PairState *st;
-  const double *part_e, *part_n, *part_c;
-  CtrlArgs a;
-  int *flags;
-  IterTrace *trace;
-  unsigned *counter;  // one per pair
-};
 
-// A pair takes part in a new iteration when it is still iterating, no step search is pending
-// (device-driven iterations can leave one open across an iteration boundary) and its last step
-// has been applied.
+// A pair takes part in a new iteration when it is still iterating and no step search is pending
+// (device-driven iterations can leave one open across an iteration boundary); the update of its
+// last step has run before (stream order).
 __device__ __forceinline__ bool participates(const PairState &s) {
-  return s.active && !s.searching && s.step_tau == 0;
-}
-
-__device__ __forceinline__ void run_tail(const Tail &t, int pair, unsigned total) {
-  if (t.st && last_block_of(t.counter + pair, total))
-    control_pair(pair, t.st, t.part_e, t.part_n, t.part_c, t.a, t.flags, t.trace);
+  return s.active && !s.searching;
 }
 
 
@@ -465,7 +465,7 @@ template <typename S>
 __global__ void __launch_bounds__(256)
 k_energy(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__ phi, const S *__restrict__ dcg,
          const S *__restrict__ dgrad, int n, double h, int width, const PairState *st,
-         double *__restrict__ part, int ntx, int nty, int npairs, Tail tail) {
+         double *__restrict__ part, int ntx, int nty, int npairs) {
   const int tiles = ntx * nty;
   const int wg = xcd_remap(blockIdx.x, tiles * npairs);
   const int pair = wg / tiles, tile = wg - pair * tiles;
@@ -514,8 +514,8 @@ k_energy(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__
         const double tau = st[pair].cand[k];
         double p0[4], p1[4];
         for (int b = 0; b < 4; ++b) {
-          p0[b] = use_dir ? rnd<S>(f0[b] + tau * d0[b]) : f0[b];
-          p1[b] = use_dir ? rnd<S>(f1[b] + tau * d1[b]) : f1[b];
+          p0[b] = use_dir ? (double)step_at<S>((S)f0[b], tau, (S)d0[b]) : f0[b];
+          p1[b] = use_dir ? (double)step_at<S>((S)f1[b], tau, (S)d1[b]) : f1[b];
         }
         double a = 0.;
         for (int q = 0; q < 4; ++q) {
@@ -547,7 +547,6 @@ k_energy(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__
       }
     }
   }
-  run_tail(tail, pair, tiles);
 }
 
 // ================================================================================================
@@ -560,7 +559,7 @@ template <typename S>
 __global__ void __launch_bounds__(256)
 k_energy_cons(const S *__restrict__ psi, int npsi, const S *__restrict__ phi, const S *__restrict__ dcg,
               const S *__restrict__ dgrad, int n, double h, int width, const PairState *st, double *__restrict__ part,
-              int ntx, int nty, int npairs, Tail tail) {
+              int ntx, int nty, int npairs) {
   const int tiles = ntx * nty;
   const int wg = xcd_remap(blockIdx.x, tiles * npairs);
   const int pair = wg / tiles, tile = wg - pair * tiles;
@@ -600,8 +599,8 @@ k_energy_cons(const S *__restrict__ psi, int npsi, const S *__restrict__ phi, co
         const double tau = st[pair].cand[k];
         double p0[4], p1[4];
         for (int b = 0; b < 4; ++b) {
-          p0[b] = use_dir ? rnd<S>(f0[b] + tau * d0[b]) : f0[b];
-          p1[b] = use_dir ? rnd<S>(f1[b] + tau * d1[b]) : f1[b];
+          p0[b] = use_dir ? (double)step_at<S>((S)f0[b], tau, (S)d0[b]) : f0[b];
+          p1[b] = use_dir ? (double)step_at<S>((S)f1[b], tau, (S)d1[b]) : f1[b];
         }
         double a = 0.;
         for (int q = 0; q < 4; ++q) {
@@ -635,7 +634,6 @@ k_energy_cons(const S *__restrict__ psi, int npsi, const S *__restrict__ phi, co
       }
     }
   }
-  run_tail(tail, pair, tiles);
 }
 
 // ================================================================================================
@@ -651,7 +649,7 @@ template <typename S>
 __global__ void __launch_bounds__(256)
 k_energy_ncc(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__ phi, const S *__restrict__ dcg,
              const S *__restrict__ dgrad, int n, double h, const PairState *st, double *__restrict__ part, int ntx,
-             int nty, int npairs, Tail tail) {
+             int nty, int npairs) {
   const int tiles = ntx * nty;
   const int wg = xcd_remap(blockIdx.x, tiles * npairs);
   const int pair = wg / tiles, tile = wg - pair * tiles;
@@ -696,8 +694,8 @@ k_energy_ncc(const S *__restrict__ R, const S *__restrict__ T, const S *__restri
         const double tau = st[pair].cand[k];
         double p0[4], p1[4], w[4];
         for (int b = 0; b < 4; ++b) {
-          p0[b] = use_dir ? rnd<S>(f0[b] + tau * d0[b]) : f0[b];
-          p1[b] = use_dir ? rnd<S>(f1[b] + tau * d1[b]) : f1[b];
+          p0[b] = use_dir ? (double)step_at<S>((S)f0[b], tau, (S)d0[b]) : f0[b];
+          p1[b] = use_dir ? (double)step_at<S>((S)f1[b], tau, (S)d1[b]) : f1[b];
           w[b] = fast_warp(Tp, n, h, ex + (b & 1), ey + ((b >> 1) & 1), p0[b], p1[b]) - c;
         }
         for (int i = 0; i < 4; ++i) {
@@ -722,7 +720,6 @@ k_energy_ncc(const S *__restrict__ R, const S *__restrict__ T, const S *__restri
       }
     }
   }
-  run_tail(tail, pair, tiles);
 }
 
 // The reference's statistics for the NCC pass, once per level: pass 0 the M-mean 1'Mr, pass 1 (with
@@ -1042,7 +1039,7 @@ template <typename S>
 __global__ void __launch_bounds__(256)
 k_mg_up(const S *__restrict__ x2in, const S *__restrict__ xc, const S *__restrict__ rhs, int negate,
         S *__restrict__ xout, int n, const PairState *st, double sigma_default, int ntx, int nty,
-        int nfields, double *__restrict__ nrm_part, Tail tail) {
+        int nfields, double *__restrict__ nrm_part) {
   const int tiles = ntx * nty;
   const int wg = xcd_remap(blockIdx.x, tiles * nfields);
   const int field = wg / tiles, tile = wg - field * tiles;
@@ -1101,7 +1098,6 @@ k_mg_up(const S *__restrict__ x2in, const S *__restrict__ xc, const S *__restric
     const double s = block_sum<256>(nrm, red);
     if (threadIdx.x == 0) nrm_part[(size_t)field * tiles + tile] = s;
   }
-  run_tail(tail, pair, 2 * tiles);
 }
 
 // Whole V-cycle below (and including) a level of at most 32x32 nodes in one workgroup, in LDS.
@@ -1113,7 +1109,7 @@ constexpr int MGC_OFF[MGC_MAXL + 2] = {0, 1, 5, 21, 85, 341, 1365};  // prefix s
 template <typename S>
 __global__ void __launch_bounds__(256)
 k_mg_coarse(const S *__restrict__ rhs, int negate, S *__restrict__ xout, int L, const PairState *st,
-            double sigma_default, int nfields, double *__restrict__ nrm_part, Tail tail) {
+            double sigma_default, int nfields, double *__restrict__ nrm_part) {
   const int field = blockIdx.x;
   const int pair = field >> 1;
   if (st && !participates(st[pair])) return;
@@ -1202,7 +1198,6 @@ k_mg_coarse(const S *__restrict__ rhs, int negate, S *__restrict__ xout, int L, 
     const double s = block_sum<256>(nrm, red);
     if (threadIdx.x == 0) nrm_part[field] = s;
   }
-  run_tail(tail, pair, 2);
 }
 
 // ================================================================================================
@@ -1214,7 +1209,7 @@ constexpr int NCG_PER_THREAD = 4;
 template <typename S>
 __global__ void __launch_bounds__(256)
 k_ncg(const S *__restrict__ z, S *__restrict__ d, const S *__restrict__ DE, int n, const PairState *st,
-      double *__restrict__ part, int nblk_field, int nfields, Tail tail) {
+      double *__restrict__ part, int nblk_field, int nfields) {
   const int total = nblk_field * nfields;
   const int wg = xcd_remap(blockIdx.x, total);
   const int field = wg / nblk_field, blk = wg - field * nblk_field;
@@ -1249,29 +1244,26 @@ k_ncg(const S *__restrict__ z, S *__restrict__ d, const S *__restrict__ DE, int 
     part[((size_t)field * NCG_TERMS + 1) * nblk_field + blk] = b;
     part[((size_t)field * NCG_TERMS + 2) * nblk_field + blk] = c;
   }
-  run_tail(tail, pair, 2 * nblk_field);
 }
 
 // phi += tau * dir for pairs whose search accepted a step (GradientDescentBase::applySingle L382-385).
 template <typename S>
 __global__ void __launch_bounds__(256)
-k_update(S *__restrict__ phi, const S *__restrict__ dcg, const S *__restrict__ dgrad, int n, PairState *st,
-         int nblk_pair, int npairs, unsigned *counter) {
+k_update(S *__restrict__ phi, const S *__restrict__ dcg, const S *__restrict__ dgrad, int n, const PairState *st,
+         int nblk_pair, int npairs, int slot) {
   const int total = nblk_pair * npairs;
   const int wg = xcd_remap(blockIdx.x, total);
   const int pair = wg / nblk_pair, blk = wg - pair * nblk_pair;
-  const double tau = st[pair].step_tau;
-  if (tau == 0) return;  // no accepted step pending for this pair
-  const S *dir = (st[pair].step_dir == DIR_CG ? dcg : dgrad) + pair * 2 * (size_t)n * n;
+  const double tau = st[pair].step_tau[slot];
+  if (tau == 0) return;  // no step accepted in this iteration
+  const S *dir = (st[pair].step_dir[slot] == DIR_CG ? dcg : dgrad) + pair * 2 * (size_t)n * n;
   S *p = phi + pair * 2 * (size_t)n * n;
   const int m = 2 * n * n;
   const int base = blk * 256 * NCG_PER_THREAD;
   for (int k = 0; k < NCG_PER_THREAD; ++k) {
     const int i = base + k * 256 + threadIdx.x;
-    if (i < m) p[i] = static_cast<S>(ld(p, i) + tau * ld(dir, i));
+    if (i < m) p[i] = step_at<S>(p[i], tau, dir[i]);
   }
-  // the last workgroup of the pair marks the step as applied
-  if (counter && last_block_of(counter + pair, nblk_pair) && threadIdx.x == 0) st[pair].step_tau = 0;
 }
 
 // ================================================================================================

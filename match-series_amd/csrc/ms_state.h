@@ -40,12 +40,12 @@ struct PairState {
   double dd;          // |DE|^2 at the current position (gradient-norm stopping)
   int sigma_fixed;    // 1: filter width fixed (plain H1GradientDescent), 0: the automatic schedule
   int grad_stop;      // 1: stop on |DE| <= stopEpsilon, 0: on the energy decrease
-  double step_tau;    // tau of the update (0: no update)
+  double step_tau[2];  // per step slot: tau of the accepted step (0: none)
   double tau_first;   // result of the CG-direction search this iteration (trace)
   int ncand;          // candidates to evaluate in the next energy pass (0: none)
   int a_phase;
   int dirmode;        // direction the energy pass / update uses
-  int step_dir;
+  int step_dir[2];
   int iter, max_iter;
   int active;         // still iterating at this level
   int searching;      // Armijo search in progress

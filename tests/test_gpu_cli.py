@@ -69,14 +69,15 @@ def check_stage(out, g, st, precision, tol, med_tol):
         assert np.abs(d - g[f"s{st}_deformations"][i]).max() * (N - 1) < tol, (st, i)
     cnt = read_p9(str(out / f"stage{st}" / "numSamples.q2bz"))
     assert np.abs(cnt - g[f"s{st}_numSamples"]).sum() <= (0 if precision == 64 else 8)
+    # pixels with the same sample count (FP32 fields can move a sample point across the image edge)
+    ok = cnt == g[f"s{st}_numSamples"]
     # PNG previews: the same 8-bit scaling of the value range; a pixel may differ by one step
     # where the data differ below the reference's rounding
     for nm in ("average", "median", "numSamples"):
         png = read_png_gray8(str(out / f"stage{st}" / f"{nm}.png")).astype(int)
-        diff = np.abs(png - g[f"s{st}_{nm}_png"].astype(int))
+        diff = np.abs(png - g[f"s{st}_{nm}_png"].astype(int))[ok]
         assert diff.max() <= 1 and (diff > 0).mean() < 0.01, nm
     med = read_p9(str(out / f"stage{st}" / "median.q2bz"))
-    ok = cnt == g[f"s{st}_numSamples"]
     assert np.abs(med - g[f"s{st}_median"])[ok].max() < med_tol
     en = np.loadtxt(str(out / f"stage{st}" / "energies.txt"))[:, 1]
     np.testing.assert_allclose(en, g[f"s{st}_energies"], rtol=1e-4 if precision == 64 else 1e-3)
