@@ -26,45 +26,61 @@ enum CtrlPhase { PH_LEVEL_INIT = 0, PH_BETA = 1, PH_ARMIJO_INIT = 2, PH_ARMIJO_S
 constexpr int CTRL_THREADS = 1024;  // threads of a control workgroup
 constexpr int CTRL_MAXT = 16;       // partial arrays one control sweep can reduce
 
-// Sums of nt consecutive partial arrays base[t * len .. t * len + len) (nt <= CTRL_MAXT) in one
-// sweep of the control workgroup. The partials were just written by a kernel on other XCDs, so
-// every load is a memory round trip: each thread takes a contiguous run of CTRL_RUN entries of every
-// array and issues all of its loads before adding, which keeps the sweep at one or two round trips
-// for the partial counts of a 1024^2 level (at most 4096 per array). Fixed order; the sums land in
-// out[0 .. nt) (shared memory), valid for every thread after the call.
-constexpr int CTRL_RUN = 4;
-__device__ void reduce_arrays(const double *base, int nt, int len, double *out) {
-  __shared__ double red[CTRL_THREADS / 64][CTRL_MAXT];
-  double acc[CTRL_MAXT];
+// Sums of NT consecutive partial arrays base[t * len .. t * len + len) in one sweep of the control
+// workgroup. The partials were just written by a kernel on other XCDs, so a load is a memory round
+// trip: the loads of a sweep (CTRL_RUN per thread and array) are unconditional, from clamped
+// indices, with out-of-range values dropped by a select, so none waits for another. Fixed order;
+// the sums land in out[0 .. NT) (shared memory), valid for every thread after the call.
+constexpr int CTRL_RUN = 4;  // one sweep covers the 4096 partials of a 1024^2 level
+template <int NT>
+__device__ void reduce_arrays_n(const double *base, int len, double *out) {
+  __shared__ double red[CTRL_THREADS / 64][NT];
+  double acc[NT];
 #pragma unroll
-  for (int t = 0; t < CTRL_MAXT; ++t) acc[t] = 0.;
-  for (int i0 = threadIdx.x * CTRL_RUN; i0 < len; i0 += CTRL_THREADS * CTRL_RUN) {
-    double v[CTRL_MAXT][CTRL_RUN];
+  for (int t = 0; t < NT; ++t) acc[t] = 0.;
+  for (int i0 = 0; i0 < len; i0 += CTRL_THREADS * CTRL_RUN) {
+    double v[NT][CTRL_RUN];
 #pragma unroll
-    for (int t = 0; t < CTRL_MAXT; ++t)
+    for (int t = 0; t < NT; ++t)
 #pragma unroll
-      for (int r = 0; r < CTRL_RUN; ++r) v[t][r] = (t < nt && i0 + r < len) ? base[(size_t)t * len + i0 + r] : 0.;
+      for (int r = 0; r < CTRL_RUN; ++r) {
+        const int k = i0 + r * CTRL_THREADS + (int)threadIdx.x;
+        v[t][r] = base[(size_t)t * len + (k < len ? k : len - 1)];
+      }
 #pragma unroll
-    for (int t = 0; t < CTRL_MAXT; ++t)
+    for (int t = 0; t < NT; ++t)
 #pragma unroll
-      for (int r = 0; r < CTRL_RUN; ++r) acc[t] += v[t][r];
+      for (int r = 0; r < CTRL_RUN; ++r) {
+        const int k = i0 + r * CTRL_THREADS + (int)threadIdx.x;
+        acc[t] += (k < len) ? v[t][r] : 0.;
+      }
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-  for (int t = 0; t < CTRL_MAXT; ++t)
-    if (t < nt) acc[t] = wave_sum(acc[t]);
+  for (int t = 0; t < NT; ++t) acc[t] = wave_sum(acc[t]);
   __syncthreads();
   if (lane == 0)
 #pragma unroll
-    for (int t = 0; t < CTRL_MAXT; ++t)
-      if (t < nt) red[wid][t] = acc[t];
+    for (int t = 0; t < NT; ++t) red[wid][t] = acc[t];
   __syncthreads();
-  if (threadIdx.x < nt) {
+  if (threadIdx.x < NT) {
     double v = 0.;
     for (int w = 0; w < CTRL_THREADS / 64; ++w) v += red[w][threadIdx.x];
     out[threadIdx.x] = v;
   }
   __syncthreads();
+}
+
+// nt (<= CTRL_MAXT) arrays: sweeps of at most 4 arrays
+__device__ void reduce_arrays(const double *base, int nt, int len, double *out) {
+  for (int t0 = 0; t0 < nt; t0 += 4) {
+    const int m = nt - t0 < 4 ? nt - t0 : 4;
+    const double *b = base + (size_t)t0 * len;
+    if (m == 4) reduce_arrays_n<4>(b, len, out + t0);
+    else if (m == 3) reduce_arrays_n<3>(b, len, out + t0);
+    else if (m == 2) reduce_arrays_n<2>(b, len, out + t0);
+    else reduce_arrays_n<1>(b, len, out + t0);
+  }
 }
 
 struct CtrlArgs {
