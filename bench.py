@@ -6,10 +6,20 @@ accumulate-and-refine chain against the first frame (levels 9-10), and the serie
 100 gradient-flow iterations per level (stopEpsilon -1: fixed schedule), lambda 10. Fields FP32,
 energies / slopes / step decisions FP64.
 
-Prints one JSON line with frames/s, the roofline of the dominant kernel (the energy pass, timed
-with HIP events on its own stream during one extra step after the timed ones) and a CPU baseline: the reference's own
-level solve (oracle/_ref/ref_harness, compiled from the reference sources) timed for a few
-iterations at 1024^2 on this host, extrapolated to the same schedule (cost proportional to N^2).
+Prints one JSON line with frames/s, the roofline of the dominant kernel and a CPU baseline.
+
+Roofline: the energy pass (k_energy_f32) at the workload's dominant shape, one pairwise batch of 8
+pairs at 1024^2 evaluating 3 step sizes along a real CG direction (the state two iterations of a
+level solve leave), timed with HIP events around `reps` back-to-back launches on the context's
+stream. Algorithmic bytes: an energy evaluation reads the deformation (2 fields), the direction (2),
+reference and template, 6 FP32 fields of N^2; one launch performs 3 evaluations of 8 pairs, so
+3 * 6 * 4 B * 8 * 1024^2 = 576 MiB per launch. The pass reads its inputs once for all its step sizes,
+which the measured `traffic` shows (rocprofv3 FETCH_SIZE + WRITE_SIZE of the same launch, calibrated
+on a plain copy, from profiles/k_energy_f32_8pairs_1024_traffic.json). `--roofline-only` runs just this measurement (for rocprofv3).
+
+CPU baseline: the reference's own level solve (oracle/_ref/ref_harness, compiled from the reference
+sources) timed for a few iterations at 1024^2 on this host, extrapolated to the same schedule (cost
+proportional to N^2).
 """
 import argparse
 import json
@@ -89,6 +99,42 @@ def cpu_baseline(N, its_sample, schedule, frames, cores=1):
     return one, allc
 
 
+def unit_range(img):
+    lo, hi = float(img.min()), float(img.max())
+    return (img - lo) / (hi - lo if hi > lo else 1.0)
+
+
+def energy_roofline(gpu, frames, defs=None, batch=8, reps=100):
+    """Mean launch time and algorithmic bytes of the energy pass over `batch` pairs (frames k, k+1)."""
+    N = frames.shape[-1]
+    L = int(np.log2(N))
+    ctx = gpu.Context(L, batch, gpu.F32)
+    try:
+        for p in range(batch):
+            ctx.set_reference(p, unit_range(frames[p]))
+            ctx.set_template(p, unit_range(frames[p + 1]))
+            if defs is not None:
+                ctx.set_deformation(p, defs[p])
+        # a real deformation and CG direction: 2 iterations from the series' deformations, or 30 from 0
+        ctx.solve_level(L, 10.0, 2 if defs is not None else 30, -1.0)
+        ms, nbytes = ctx.time_energy_pass(L, 3, reps)
+    finally:
+        ctx.close()
+    evals = 3
+    algo = evals * nbytes  # nbytes: the 6 fields of one evaluation for every pair
+    gbs = algo / (ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "k_energy_f32_8pairs_1024_traffic.json")
+    if os.path.exists(tf) and batch == 8 and N == 1024:
+        with open(tf) as f:
+            traffic = json.load(f).get("traffic_bytes_per_launch")
+    return {"bound": "hbm", "kernel": "msg::k_energy_f32", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic, "launches": reps,
+            "avg_launch_us": round(1000.0 * ms, 2), "bytes_per_launch": algo, "evals_per_launch": evals,
+            "bytes_per_evaluation": nbytes,
+            "shape": f"{batch} pairs x {N}^2, {evals} step sizes, FP32 fields"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,7 +149,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=0, help="pairwise solves per batch (0: automatic)")
-    ap.add_argument("--classes", action="store_true", help="also time every kernel class in the profiled step")
+    ap.add_argument("--classes", action="store_true", help="also time every kernel class in one extra step")
+    ap.add_argument("--roofline-only", action="store_true", help="only the energy-pass roofline measurement")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -121,6 +168,17 @@ def main():
     levels, refine = (L - 3, L), (L - 1, L)
     dt = torch.float32 if args.precision == "f32" else torch.float64
     frames = synth.stem_series(N, F, workload=1, spacing=16.0, sigma=3.0, jitter_step=0.4)
+    if args.roofline_only:
+        r = energy_roofline(gpu, frames)
+        # calibration for HBM byte counters: plain copies of 256 MiB (read 256 MiB, write 256 MiB each)
+        a = torch.rand(1 << 26, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        r["calibration_copy_bytes"] = a.numel() * 4
+        print(json.dumps({"roofline": r}), flush=True)
+        return
     dev = torch.tensor(frames, dtype=dt, device="cuda")
     tpl, ref = dev[1:].contiguous(), dev[0].contiguous()
     p = series_params(gpu, levels, refine, args.iters, spec=args.spec)
@@ -139,24 +197,16 @@ def main():
     ms_step = 1000.0 * dt_s / args.steps
     value = F * args.steps / dt_s
 
-    # the energy kernel's roofline: one more step with HIP events around every energy launch (on the
-    # launching stream; graph capture is off while a class is timed) and the algorithmic bytes the
-    # device counted (6 fields per pass with a direction, 4 for a level's first pass)
-    gpu.profile_reset()
-    gpu.profile_enable(gpu.KERNEL_CLASSES if args.classes else ("energy",))
-    t2 = time.perf_counter()
-    gpu.match_series(tpl, ref, p)
-    torch.cuda.synchronize()
-    t3 = time.perf_counter()
-    gpu.profile_enable(())
-    prof = gpu.profile_read("energy")
-    classes = {c: gpu.profile_read(c) for c in gpu.KERNEL_CLASSES} if args.classes else None
+    roofline = energy_roofline(gpu, frames, out["deformations"][1:9].double().cpu().numpy())
 
-    achieved = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9 if prof["ms"] > 0 else 0.0
-    roofline = {"bound": "hbm", "kernel": "msg::k_energy", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "launches": prof["launches"], "avg_launch_us": round(1000.0 * prof["ms"] / max(prof["launches"], 1), 2),
-                "profiled_step_ms": round(1000.0 * (t3 - t2), 2)}
+    classes = None
+    if args.classes:  # every kernel class bracketed by HIP events during one more step (graphs off meanwhile)
+        gpu.profile_reset()
+        gpu.profile_enable(gpu.KERNEL_CLASSES)
+        gpu.match_series(tpl, ref, p)
+        torch.cuda.synchronize()
+        gpu.profile_enable(())
+        classes = {c: gpu.profile_read(c) for c in gpu.KERNEL_CLASSES}
 
     # measured HBM ceiling: a plain device copy of 1 GiB (read + write), torch's copy kernel
     a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")

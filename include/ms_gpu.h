@@ -239,6 +239,14 @@ int msg_release_cache(void);
 int msg_profile_enable(unsigned mask);
 int msg_profile_reset(void);
 int msg_profile_read(int cls, double *ms, long long *launches, double *bytes);
+/* Times the energy pass of a context's level solve (no counterpart in the reference; used by the
+ * benchmark's roofline): every pair evaluates `ncand` step sizes (1, 1/2, ...) along the direction the
+ * last solve at this level left in the context, `reps` launches back to back on the context's stream
+ * between two HIP events. Outputs the mean launch time (ms) and the algorithmic bytes of one launch
+ * (reference, template, deformation and direction: 6 fields per pair). Leaves the pair state unusable
+ * for resuming; the next solve starts afresh as always. */
+int msg_time_energy_pass(msg_ctx *ctx, int level, int ncand, int reps, double *ms_per_launch,
+                         double *bytes_per_launch);
 
 #ifdef __cplusplus
 }

@@ -243,6 +243,27 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
   return s;
 }
 
+// Sums of NV values over a block of BLOCK threads in one pass (valid in thread 0); red holds
+// (BLOCK / 64) * NV doubles. Fixed order.
+template <int NV, int BLOCK = 256>
+__device__ __forceinline__ void block_sum_n(double (&v)[NV], double *red) {
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[wid * NV + k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double a = red[k];
+      for (int w = 1; w < BLOCK / 64; ++w) a += red[w * NV + k];
+      v[k] = a;
+    }
+}
+
 // Map a linear workgroup id so that each of the 8 XCDs receives a contiguous range of tiles
 // (workgroups are dispatched round-robin over XCDs); neighbouring tiles then share an L2.
 __device__ __forceinline__ int xcd_remap(int wg, int total) {

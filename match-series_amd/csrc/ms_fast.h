@@ -73,28 +73,12 @@ __device__ __forceinline__ float stiff_form_f(const float p[4]) {
   return (e0 * e0 + e1 * e1 + e2 * e2 + e3 * e3) * (1.0f / 6.0f) + (d0 * d0 + d1 * d1) * (2.0f / 6.0f);
 }
 
-// Sums of NV values over a block of 256 threads in one pass (valid in thread 0); red holds 4*NV.
-template <int NV>
-__device__ __forceinline__ void block_sum_n(double (&v)[NV], double *red) {
-#pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0)
-#pragma unroll
-    for (int k = 0; k < NV; ++k) red[wid * NV + k] = v[k];
-  __syncthreads();
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = ((red[k] + red[NV + k]) + red[2 * NV + k]) + red[3 * NV + k];
-}
-
 // ================================================================================================
 // SSD energy pass for up to MAX_CAND step sizes (k_energy in FP32 arithmetic).
 // ================================================================================================
 __global__ void __launch_bounds__(256)
 k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const float *__restrict__ phi,
-             const float *__restrict__ dcg, const float *__restrict__ dgrad, int n, double vol,
+             const float *__restrict__ z, const float *__restrict__ dold, int n, double vol,
              const PairState *st, double *__restrict__ part, int ntx, int nty, int npairs) {
 #pragma clang fp contract(fast)
   const int tiles = ntx * nty;
@@ -108,8 +92,9 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
   const size_t nn = (size_t)n * n;
   const float *Rp = R + pair * nn, *Tp = T + pair * nn;
   const float *px = phi + pair * 2 * nn, *py = px + nn;
-  const float *dsel = (dm == DIR_CG) ? dcg : dgrad;
-  const float *dx = dsel + pair * 2 * nn, *dy = dx + nn;
+  const float *zx = z + pair * 2 * nn, *zy = zx + nn, *ox_ = dold + pair * 2 * nn, *oy_ = ox_ + nn;
+  const bool cg = (dm == DIR_CG), use_old = st[pair].cg_old != 0;
+  const double beta = st[pair].beta;
   const bool use_dir = (dm != DIR_NONE);
   const float nm1 = (float)(n - 1);
 
@@ -129,8 +114,8 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
       const int g = l2g(ex, ey, b, n);
       f0[b] = px[g];
       f1[b] = py[g];
-      d0[b] = use_dir ? dx[g] : 0.f;
-      d1[b] = use_dir ? dy[g] : 0.f;
+      d0[b] = use_dir ? search_dir<float>(zx, ox_, g, cg, use_old, beta) : 0.f;
+      d1[b] = use_dir ? search_dir<float>(zy, oy_, g, cg, use_old, beta) : 0.f;
       rn[b] = Rp[g];
     }
 #pragma unroll
@@ -378,7 +363,7 @@ k_mg_down_f32(const float *__restrict__ rhs, int negate, float *__restrict__ x2o
 __global__ void __launch_bounds__(256)
 k_mg_up_f32(const float *__restrict__ x2in, const float *__restrict__ xc, const float *__restrict__ rhs, int negate,
             float *__restrict__ xout, int n, const PairState *st, double sigma_default, int ntx, int nty, int nfields,
-            double *__restrict__ nrm_part) {
+            const float *__restrict__ dold, double *__restrict__ nrm_part) {
 #pragma clang fp contract(fast)
   const int tiles = ntx * nty;
   const int wg = xcd_remap(blockIdx.x, tiles * nfields);
@@ -391,7 +376,7 @@ k_mg_up_f32(const float *__restrict__ x2in, const float *__restrict__ xc, const 
   __shared__ float sx[MG_HY][MG_HX];  // x on tile+2
   __shared__ float sr[MG_HY][MG_HX];  // rhs on tile+1
   __shared__ float s3[MG_HY][MG_HX];  // x3 on tile+1
-  __shared__ double red[4];
+  __shared__ double red[4 * NORM_TERMS];
   const size_t nn = (size_t)n * n;
   const int nc = n / 2;
   const float *x2p = x2in + field * nn, *rp = rhs + field * nn, *xcp = xc + field * (size_t)nc * nc;
@@ -418,39 +403,48 @@ k_mg_up_f32(const float *__restrict__ x2in, const float *__restrict__ xc, const 
   __syncthreads();
   auto get3 = [&](int gx, int gy) { return s3[gy - by][gx - bx]; };
   float *xo = xout + field * nn;
-  double nrm = 0;
+  double acc[NORM_TERMS] = {0., 0., 0., 0.};  // <z,z>, <DE,z>, <DE,d_old>, <DE,DE> (DE = the raw input)
   for (int i = threadIdx.x; i < MG_TX * MG_TY; i += 256) {
     const int tx = i % MG_TX, ty = i / MG_TX;
     const int gx = x0 + tx, gy = y0 + ty;
     if (gx < n && gy < n) {
       const float o = (sr[ty + 2][tx + 2] - heat_apply_f(get3, gx, gy, n, hs)) * idg + s3[ty + 2][tx + 2];
       xo[gx + gy * n] = o;
-      nrm += (double)o * (double)o;
+      if (nrm_part) {
+        const double od = (double)o, de = (double)(sg * sr[ty + 2][tx + 2]);  // the raw input
+        acc[0] += od * od;
+        acc[1] += de * od;
+        acc[2] += de * (double)dold[field * nn + gx + gy * n];
+        acc[3] += de * de;
+      }
     }
   }
   if (nrm_part) {
-    const double s = block_sum<256>(nrm, red);
-    if (threadIdx.x == 0) nrm_part[(size_t)field * tiles + tile] = s;
+    block_sum_n<NORM_TERMS>(acc, red);
+    if (threadIdx.x == 0)
+      for (int t = 0; t < NORM_TERMS; ++t) nrm_part[((size_t)field * NORM_TERMS + t) * tiles + tile] = acc[t];
   }
 }
 
-// Whole V-cycle below (and including) a level of at most 32x32 nodes in one workgroup of
-// MGC_THREADS threads (k_mg_coarse). The Jacobi sweeps write a second buffer instead of updating in
-// place, the residual is restricted as it is formed and the prolongation is read on the fly by the
-// first sweep of the up leg, so a level costs 3 barriers down and 2 up.
+// Whole V-cycle below (and including) a level of at most 64x64 nodes in one workgroup of
+// MGC_THREADS threads (k_mg_coarse, one level deeper: FP32 arrays of level 6 and below fit in LDS).
+// The Jacobi sweeps write a second buffer instead of updating in place, the residual is restricted as
+// it is formed and the prolongation is read on the fly by the first sweep of the up leg, so a level
+// costs 3 barriers down and 2 up.
 constexpr int MGC_THREADS = 1024;
+constexpr int MGC_MAXL_F32 = 6;
 
 __global__ void __launch_bounds__(MGC_THREADS)
 k_mg_coarse_f32(const float *__restrict__ rhs, int negate, float *__restrict__ xout, int L, const PairState *st,
-                double sigma_default, int nfields, double *__restrict__ nrm_part) {
+                double sigma_default, int nfields, const float *__restrict__ dold, double *__restrict__ nrm_part) {
 #pragma clang fp contract(fast)
   const int field = blockIdx.x;
   const int pair = field >> 1;
   if (st && !participates(st[pair])) return;
   const double sigma = st ? st[pair].sigma : sigma_default;
   const double tau = 0.5 * (sigma * sigma);
-  __shared__ float X[MGC_OFF[MGC_MAXL + 1]], B[MGC_OFF[MGC_MAXL + 1]], Y[1024];
-  __shared__ double red[MGC_THREADS / 64];
+  __shared__ float X[MGC_OFF[MGC_MAXL_F32 + 1]], B[MGC_OFF[MGC_MAXL_F32 + 1]], Y[1 << (2 * MGC_MAXL_F32)];
+  __shared__ double red[(MGC_THREADS / 64) * NORM_TERMS];
   const int n = 1 << L;
   const size_t nn = (size_t)n * n;
   const float *rp = rhs + field * nn;
@@ -516,15 +510,22 @@ k_mg_coarse_f32(const float *__restrict__ rhs, int negate, float *__restrict__ x
     __syncthreads();
   }
   float *xo = xout + field * nn;
-  double nrm = 0;
+  double acc[NORM_TERMS] = {0., 0., 0., 0.};  // <z,z>, <DE,z>, <DE,d_old>, <DE,DE> (DE = the raw input)
   for (int i = threadIdx.x; i < (int)nn; i += MGC_THREADS) {
     const float o = X[MGC_OFF[L] + i];
     xo[i] = o;
-    nrm += (double)o * (double)o;
+    if (nrm_part) {
+      const double od = (double)o, de = (double)rp[i];
+      acc[0] += od * od;
+      acc[1] += de * od;
+      acc[2] += de * (double)dold[field * nn + i];
+      acc[3] += de * de;
+    }
   }
   if (nrm_part) {
-    const double s = block_sum<MGC_THREADS>(nrm, red);
-    if (threadIdx.x == 0) nrm_part[field] = s;
+    block_sum_n<NORM_TERMS, MGC_THREADS>(acc, red);
+    if (threadIdx.x == 0)
+      for (int t = 0; t < NORM_TERMS; ++t) nrm_part[(size_t)field * NORM_TERMS + t] = acc[t];
   }
 }
 

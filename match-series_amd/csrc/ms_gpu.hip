@@ -230,23 +230,23 @@ size_t energy_part_size(int n, int npairs) {
 
 // The images of the data term: R, T (SSD, NCC) or, for DT_CONS, the npsi deformations in T.
 template <typename S>
-int launch_energy(const S *R, const S *T, const S *phi, const S *dcg, const S *dgrad, int n, const PairState *st,
+int launch_energy(const S *R, const S *T, const S *phi, const S *z, const S *dold, int n, const PairState *st,
                   double *part, int npairs, hipStream_t s, int data_term, double bytes = 0, int npsi = 0) {
   ProfScope ps(KC_ENERGY, s, bytes);
   const EnergyGeom g = energy_geom(n);
   const double h = 1.0 / (double)(n - 1);
   const int width = (int)((n - 1) * h);
   if (data_term == msg::DT_CONS)
-    hipLaunchKernelGGL((msg::k_energy_cons<S>), dim3(g.nblk * npairs), dim3(256), 0, s, T, npsi, phi, dcg, dgrad, n, h,
+    hipLaunchKernelGGL((msg::k_energy_cons<S>), dim3(g.nblk * npairs), dim3(256), 0, s, T, npsi, phi, z, dold, n, h,
                        width, st, part, g.ntx, g.nty, npairs);
   else if (data_term == msg::DT_NCC)
-    hipLaunchKernelGGL((msg::k_energy_ncc<S>), dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, dcg, dgrad, n, h,
+    hipLaunchKernelGGL((msg::k_energy_ncc<S>), dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, z, dold, n, h,
                        st, part, g.ntx, g.nty, npairs);
   else if constexpr (std::is_same<S, float>::value)
-    hipLaunchKernelGGL(msg::k_energy_f32, dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, dcg, dgrad, n, h * h, st,
+    hipLaunchKernelGGL(msg::k_energy_f32, dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, z, dold, n, h * h, st,
                        part, g.ntx, g.nty, npairs);
   else
-    hipLaunchKernelGGL((msg::k_energy<S>), dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, dcg, dgrad, n, h, width,
+    hipLaunchKernelGGL((msg::k_energy<S>), dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, z, dold, n, h, width,
                        st, part, g.ntx, g.nty, npairs);
   HIPCHK(hipGetLastError());
   return MSG_OK;
@@ -278,7 +278,7 @@ int launch_grad(const S *R, const S *T, const S *phi, S *DE, int n, const PairSt
 // The reference statistics of the NCC energy for every pair of a level (two passes with their
 // control phases); the first energy pass of the level follows.
 template <typename S>
-int ncc_reference_stats(const S *R, int n, PairState *st, double *part, double *part_n, double *part_c, int B,
+int ncc_reference_stats(const S *R, int n, PairState *st, double *part, double *part_n, int B,
                         msg::CtrlArgs a, int *flags, msg::IterTrace *tr, hipStream_t s) {
   const EnergyGeom g = energy_geom(n);
   const double h = 1.0 / (double)(n - 1);
@@ -288,7 +288,7 @@ int ncc_reference_stats(const S *R, int n, PairState *st, double *part, double *
     a.phase = pass ? msg::PH_NCC_REF1 : msg::PH_NCC_REF0;
     a.nblk_energy = g.nblk;
     a.npairs = B;
-    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, st, part, part_n, part_c, a, flags, tr);
+    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, st, part, part_n, a, flags, tr);
     HIPCHK(hipGetLastError());
   }
   return MSG_OK;
@@ -304,8 +304,15 @@ struct MGBufs {
   S *x2_top;  // pre-smoothed iterate of the top level
 };
 
-int mg_norm_blocks(int l) {
-  if (l <= msg::MGC_MAXL) return 1;
+// Deepest level the single-workgroup coarse V-cycle handles for a field type.
+template <typename S>
+constexpr int mgc_maxl() {
+  return std::is_same<S, float>::value ? msg::MGC_MAXL_F32 : msg::MGC_MAXL;
+}
+
+// Partial sums per field and term of the V-cycle's last pass at level l.
+int mg_norm_blocks(int l, int coarse_max) {
+  if (l <= coarse_max) return 1;
   const int n = 1 << l;
   return cdiv(n, msg::MG_TX) * cdiv(n, msg::MG_TY);
 }
@@ -313,12 +320,13 @@ int mg_norm_blocks(int l) {
 // The V-cycle kernels of a field type: FP32 arithmetic for FP32 fields, FP64 for FP64 fields.
 template <typename S>
 void mg_coarse(dim3 g, hipStream_t s, const S *rhs, int negate, S *xout, int L, const PairState *st, double sigma,
-               int nfields, double *nrm_part) {
+               int nfields, const S *dold, double *nrm_part) {
   if constexpr (std::is_same<S, float>::value)
     hipLaunchKernelGGL(msg::k_mg_coarse_f32, g, dim3(msg::MGC_THREADS), 0, s, rhs, negate, xout, L, st, sigma, nfields,
-                       nrm_part);
+                       dold, nrm_part);
   else
-    hipLaunchKernelGGL((msg::k_mg_coarse<S>), g, dim3(256), 0, s, rhs, negate, xout, L, st, sigma, nfields, nrm_part);
+    hipLaunchKernelGGL((msg::k_mg_coarse<S>), g, dim3(256), 0, s, rhs, negate, xout, L, st, sigma, nfields, dold,
+                       nrm_part);
 }
 template <typename S>
 void mg_down(dim3 g, hipStream_t s, const S *rhs, int negate, S *x2out, S *rhsc, int n, const PairState *st,
@@ -332,27 +340,29 @@ void mg_down(dim3 g, hipStream_t s, const S *rhs, int negate, S *x2out, S *rhsc,
 }
 template <typename S>
 void mg_up(dim3 g, hipStream_t s, const S *x2in, const S *xc, const S *rhs, int negate, S *xout, int n,
-           const PairState *st, double sigma, int ntx, int nty, int nfields, double *nrm_part) {
+           const PairState *st, double sigma, int ntx, int nty, int nfields, const S *dold, double *nrm_part) {
   if constexpr (std::is_same<S, float>::value)
     hipLaunchKernelGGL(msg::k_mg_up_f32, g, dim3(256), 0, s, x2in, xc, rhs, negate, xout, n, st, sigma, ntx, nty,
-                       nfields, nrm_part);
+                       nfields, dold, nrm_part);
   else
     hipLaunchKernelGGL((msg::k_mg_up<S>), g, dim3(256), 0, s, x2in, xc, rhs, negate, xout, n, st, sigma, ntx, nty,
-                       nfields, nrm_part);
+                       nfields, dold, nrm_part);
 }
 
-// out = V-cycle(sign * in) for nfields fields of level l (LinearSmoothOp::apply).
+// out = V-cycle(sign * in) for nfields fields of level l (LinearSmoothOp::apply). With nrm_part, the
+// last pass also leaves the partial sums <out,out>, <in,out>, <in,dold>, <in,in> per field and block.
 template <typename S>
 int launch_vcycle(const S *in, int negate, S *out, int l, const MGBufs<S> &b, const PairState *st, double sigma,
-                  int nfields, double *nrm_part, hipStream_t s) {
+                  int nfields, double *nrm_part, const S *dold, hipStream_t s) {
   ProfScope ps(KC_VCYCLE, s, 0);
-  if (l <= msg::MGC_MAXL) {
-    mg_coarse<S>(dim3(nfields), s, in, negate, out, l, st, sigma, nfields, nrm_part);
+  constexpr int CM = mgc_maxl<S>();
+  if (l <= CM) {
+    mg_coarse<S>(dim3(nfields), s, in, negate, out, l, st, sigma, nfields, dold, nrm_part);
     HIPCHK(hipGetLastError());
     return MSG_OK;
   }
-  // down leg: levels l .. MGC_MAXL+1
-  for (int m = l; m > msg::MGC_MAXL; --m) {
+  // down leg: levels l .. CM+1
+  for (int m = l; m > CM; --m) {
     const int n = 1 << m;
     const int ntx = cdiv(n, msg::MG_TX), nty = cdiv(n, msg::MG_TY);
     const S *src = (m == l) ? in : b.rhs[m];
@@ -361,17 +371,17 @@ int launch_vcycle(const S *in, int negate, S *out, int l, const MGBufs<S> &b, co
     mg_down<S>(dim3(ntx * nty * nfields), s, src, neg, x2, b.rhs[m - 1], n, st, sigma, ntx, nty, nfields);
     HIPCHK(hipGetLastError());
   }
-  mg_coarse<S>(dim3(nfields), s, b.rhs[msg::MGC_MAXL], 0, b.x[msg::MGC_MAXL], msg::MGC_MAXL, st, sigma, nfields,
-               (double *)nullptr);
+  mg_coarse<S>(dim3(nfields), s, b.rhs[CM], 0, b.x[CM], CM, st, sigma, nfields,
+               (const S *)nullptr, (double *)nullptr);
   HIPCHK(hipGetLastError());
-  for (int m = msg::MGC_MAXL + 1; m <= l; ++m) {
+  for (int m = CM + 1; m <= l; ++m) {
     const int n = 1 << m;
     const int ntx = cdiv(n, msg::MG_TX), nty = cdiv(n, msg::MG_TY);
     const S *x2 = (m == l) ? b.x2_top : b.x2[m];
     const S *rhs = (m == l) ? in : b.rhs[m];
     const int neg = (m == l) ? negate : 0;
     S *xo = (m == l) ? out : b.x[m];
-    mg_up<S>(dim3(ntx * nty * nfields), s, x2, b.x[m - 1], rhs, neg, xo, n, st, sigma, ntx, nty, nfields,
+    mg_up<S>(dim3(ntx * nty * nfields), s, x2, b.x[m - 1], rhs, neg, xo, n, st, sigma, ntx, nty, nfields, dold,
              (m == l) ? nrm_part : (double *)nullptr);
     HIPCHK(hipGetLastError());
   }
@@ -429,7 +439,7 @@ struct msg_ctx {
   void *R[MAXL + 1] = {}, *T[MAXL + 1] = {}, *phi[MAXL + 1] = {};
   void *DE = nullptr, *z = nullptr, *d = nullptr, *x2 = nullptr;
   void *mg_rhs[MAXL + 1] = {}, *mg_x2[MAXL + 1] = {}, *mg_x[MAXL + 1] = {};
-  double *part_e = nullptr, *part_n = nullptr, *part_c = nullptr;
+  double *part_e = nullptr, *part_n = nullptr;
   double *coord = nullptr;
   PairState *st = nullptr;
   int *flags_d = nullptr, *flags_h = nullptr;
@@ -539,9 +549,8 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
   S *DE = (S *)c->DE, *z = (S *)c->z, *d = (S *)c->d;
   const MGBufs<S> mg = ctx_mg<S>(c);
   const EnergyGeom eg = energy_geom(n);
-  const int nblk_norm = mg_norm_blocks(l);
-  const int nblk_ncg = cdiv(n * n, 256 * msg::NCG_PER_THREAD);
-  const int nblk_upd = cdiv(2 * n * n, 256 * msg::NCG_PER_THREAD);
+  const int nblk_norm = mg_norm_blocks(l, mgc_maxl<S>());
+  const int nblk_upd = cdiv(2 * n * n, 256 * msg::UPD_PER_THREAD);
   if (spec <= 0) spec = 3;
   if (spec > msg::MAX_CAND) spec = msg::MAX_CAND;
 
@@ -561,7 +570,6 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
   msg::CtrlArgs a;
   a.nblk_energy = eg.nblk;
   a.nblk_norm = nblk_norm;
-  a.nblk_ncg = nblk_ncg;
   a.lambda = lambda;
   a.stop_eps = stop_eps;
   a.max_iter = max_iter;
@@ -576,11 +584,11 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
   hipLaunchKernelGGL(k_state_init, dim3(cdiv(B, 64)), dim3(64), 0, s, c->st, B, lambda);
   HIPCHK(hipGetLastError());
   if (c->data_term == msg::DT_NCC)
-    CHK(ncc_reference_stats<S>(R, n, c->st, c->part_e, c->part_n, c->part_c, B, a, c->flags_d, tr, s));
-  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi));
+    CHK(ncc_reference_stats<S>(R, n, c->st, c->part_e, c->part_n, B, a, c->flags_d, tr, s));
+  CHK(launch_energy<S>(R, T, phi, z, d, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi));
   a.phase = msg::PH_LEVEL_INIT;
   a.slot = 0;
-  hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
+  hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, c->st, c->part_e, c->part_n, a, c->flags_d, tr);
   HIPCHK(hipGetLastError());
 
   bool any_active = max_iter > 0;
@@ -588,23 +596,17 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
   for (int it = 0; any_active; ++it) {
     a.slot = it & 1;
     CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s, c->data_term, npsi));
-    CHK(launch_vcycle<S>(DE, 1, z, l, mg, c->st, 1.0, 2 * B, c->part_n, s));
-    a.phase = msg::PH_BETA;
-    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL((msg::k_ncg<S>), dim3(nblk_ncg * 2 * B), dim3(256), 0, s, z, d, DE, n, c->st, c->part_c, nblk_ncg,
-                       2 * B);
-    HIPCHK(hipGetLastError());
+    CHK(launch_vcycle<S>(DE, 1, z, l, mg, c->st, 1.0, 2 * B, c->part_n, d, s));
     a.phase = msg::PH_ARMIJO_INIT;
-    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d, tr);
+    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, c->st, c->part_e, c->part_n, a, c->flags_d, tr);
     HIPCHK(hipGetLastError());
     bool searching = true;
     int rounds = 0;
     int nsearch = nactive;  // pairs evaluating in this round (upper bound for the first round)
     while (searching) {
-      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi));
+      CHK(launch_energy<S>(R, T, phi, z, d, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi));
       a.phase = msg::PH_ARMIJO_STEP;
-      hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, c->st, c->part_e, c->part_n, c->part_c, a, c->flags_d,
+      hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, c->st, c->part_e, c->part_n, a, c->flags_d,
                          tr);
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(c->flags_h, c->flags_d, sizeof(int) * B, hipMemcpyDeviceToHost, s));
@@ -675,8 +677,7 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
   S *DE = (S *)c->DE, *z = (S *)c->z, *d = (S *)c->d;
   const MGBufs<S> mg = ctx_mg<S>(c);
   const EnergyGeom eg = energy_geom(n);
-  const int nblk_ncg = cdiv(n * n, 256 * msg::NCG_PER_THREAD);
-  const int nblk_upd = cdiv(2 * n * n, 256 * msg::NCG_PER_THREAD);
+  const int nblk_upd = cdiv(2 * n * n, 256 * msg::UPD_PER_THREAD);
   if (spec <= 0) spec = 3;
   if (spec > msg::MAX_CAND) spec = msg::MAX_CAND;
   const int Rr = 2, K = 8;
@@ -699,8 +700,7 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
   }
   msg::CtrlArgs a;
   a.nblk_energy = eg.nblk;
-  a.nblk_norm = mg_norm_blocks(l);
-  a.nblk_ncg = nblk_ncg;
+  a.nblk_norm = mg_norm_blocks(l, mgc_maxl<S>());
   a.lambda = lambda;
   a.stop_eps = stop_eps;
   a.max_iter = max_iter;
@@ -721,7 +721,7 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
     ap.phase = phase;
     ap.slot = slot;
     ProfScope ps(KC_CONTROL, s, 0);
-    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, c->st, c->part_e, c->part_n, c->part_c, ap,
+    hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, c->st, c->part_e, c->part_n, ap,
                        c->flags_d, tr);
     HIPCHK(hipGetLastError());
     return MSG_OK;
@@ -730,25 +730,18 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
   hipLaunchKernelGGL(k_state_init, dim3(cdiv(B, 64)), dim3(64), 0, s, c->st, B, lambda);
   HIPCHK(hipGetLastError());
   if (c->data_term == msg::DT_NCC)
-    CHK(ncc_reference_stats<S>(R, n, c->st, c->part_e, c->part_n, c->part_c, B, a, c->flags_d, tr, s));
-  CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi));
+    CHK(ncc_reference_stats<S>(R, n, c->st, c->part_e, c->part_n, B, a, c->flags_d, tr, s));
+  CHK(launch_energy<S>(R, T, phi, z, d, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi));
   CHK(control(msg::PH_LEVEL_INIT, 0));
 
   // iteration k of a replay uses step slot k & 1 (K is even, so the slots keep alternating)
   auto iteration = [&](int k) -> int {
     const int slot = k & 1;
     CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s, c->data_term, npsi));
-    CHK(launch_vcycle<S>(DE, 1, z, l, mg, c->st, 1.0, 2 * B, c->part_n, s));
-    CHK(control(msg::PH_BETA, slot));
-    {
-      ProfScope ps(KC_NCG, s, 0);
-      hipLaunchKernelGGL((msg::k_ncg<S>), dim3(nblk_ncg * 2 * B), dim3(256), 0, s, z, d, DE, n, c->st, c->part_c,
-                         nblk_ncg, 2 * B);
-      HIPCHK(hipGetLastError());
-    }
+    CHK(launch_vcycle<S>(DE, 1, z, l, mg, c->st, 1.0, 2 * B, c->part_n, d, s));
     CHK(control(msg::PH_ARMIJO_INIT, slot));
     for (int r = 0; r < Rr; ++r) {
-      CHK(launch_energy<S>(R, T, phi, d, z, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi));
+      CHK(launch_energy<S>(R, T, phi, z, d, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi));
       CHK(control(msg::PH_ARMIJO_STEP, slot));
     }
     {
@@ -927,6 +920,43 @@ int solve_impl(msg_ctx *c, int start, int stop, const msg_solve_params *p, doubl
 // =================================================================================================
 // C ABI
 // =================================================================================================
+// Every pair evaluates nc step sizes tau0 * 2^-k along its CG direction in the next energy pass.
+__global__ void k_state_bench(PairState *st, int npairs, int nc, double tau0) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  st[p].dirmode = msg::DIR_CG;
+  st[p].ncand = nc;
+  for (int k = 0; k < nc; ++k) st[p].cand[k] = tau0 / (double)(1 << k);
+}
+
+template <typename S>
+int time_energy_pass(msg_ctx *c, int l, int nc, int reps, double *ms, double *bytes) {
+  const int n = 1 << l, B = c->B;
+  hipStream_t s = c->stream;
+  const bool cons = c->data_term == msg::DT_CONS;
+  const S *R = cons ? (const S *)c->psi : (const S *)c->R[l], *T = cons ? (const S *)c->psi : (const S *)c->T[l];
+  hipLaunchKernelGGL(k_state_bench, dim3(cdiv(B, 64)), dim3(64), 0, s, c->st, B, nc, 1.0);
+  HIPCHK(hipGetLastError());
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, s));
+  int rc = MSG_OK;
+  for (int r = 0; r < reps && rc == MSG_OK; ++r)
+    rc = launch_energy<S>(R, T, (const S *)c->phi[l], (const S *)c->z, (const S *)c->d, n, c->st, c->part_e, B, s,
+                          c->data_term, 0.0, c->npsi);
+  HIPCHK(hipEventRecord(e1, s));
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  CHK(rc);
+  if (ms) *ms = (double)t / reps;
+  if (bytes) *bytes = 6.0 * (double)n * n * sizeof(S) * B;
+  return MSG_OK;
+}
+
 extern "C" {
 
 const char *msg_last_error(void) { return g_err.c_str(); }
@@ -973,8 +1003,7 @@ int msg_ctx_create(msg_ctx **out, int max_level, int npairs, int dtype) {
   const EnergyGeom eg = energy_geom(N);
   (void)eg;
   A((void **)&c->part_e, sizeof(double) * energy_part_size(N, (int)B));
-  A((void **)&c->part_n, sizeof(double) * B * 2 * std::max(1, mg_norm_blocks(max_level)));
-  A((void **)&c->part_c, sizeof(double) * B * 2 * msg::NCG_TERMS * cdiv(N * N, 256 * msg::NCG_PER_THREAD));
+  A((void **)&c->part_n, sizeof(double) * B * 2 * msg::NORM_TERMS * std::max(1, mg_norm_blocks(max_level, msg::MGC_MAXL)));
   A((void **)&c->st, sizeof(PairState) * B);
   A((void **)&c->flags_d, sizeof(int) * B);
   if (rc != MSG_OK) {
@@ -1120,6 +1149,14 @@ int msg_last_counts(msg_ctx *c, long long *iterations, long long *evals, long lo
   return MSG_OK;
 }
 
+int msg_time_energy_pass(msg_ctx *c, int level, int ncand, int reps, double *ms_per_launch,
+                         double *bytes_per_launch) {
+  if (!c) return fail(MSG_ERR_ARG, "null context");
+  if (level < 1 || level > c->L) return fail(MSG_ERR_ARG, "level out of range");
+  if (ncand < 1 || ncand > msg::MAX_CAND || reps < 1) return fail(MSG_ERR_ARG, "ncand must be 1..4 and reps >= 1");
+  return DISPATCH(c, time_energy_pass, c, level, ncand, reps, ms_per_launch, bytes_per_launch);
+}
+
 int msg_ctx_buffer(msg_ctx *c, int which, int level, void **ptr) {
   if (!c || !ptr) return fail(MSG_ERR_ARG, "null argument");
   if (level < 0 || level > c->L) return fail(MSG_ERR_ARG, "level out of range");
@@ -1184,11 +1221,11 @@ int ncc_energy_impl(const void *R, const void *T, const void *phi, int n, int ba
   a.spec = 1;
   a.data_term = msg::DT_NCC;
   a.npairs = batch;
-  CHK(ncc_reference_stats<S>((const S *)R, n, st, (double *)part, nullptr, nullptr, batch, a, nullptr, nullptr, s));
+  CHK(ncc_reference_stats<S>((const S *)R, n, st, (double *)part, nullptr, batch, a, nullptr, nullptr, s));
   CHK(launch_energy<S>((const S *)R, (const S *)T, (const S *)phi, (const S *)phi, (const S *)phi, n, st,
                        (double *)part, batch, s, msg::DT_NCC));
   a.phase = msg::PH_LEVEL_INIT;
-  hipLaunchKernelGGL(msg::k_control, dim3(batch), dim3(msg::CTRL_THREADS), 0, s, st, (const double *)part, (const double *)nullptr,
+  hipLaunchKernelGGL(msg::k_control, dim3(batch), dim3(msg::CTRL_THREADS), 0, s, st, (const double *)part,
                      (const double *)nullptr, a, (int *)nullptr, (msg::IterTrace *)nullptr);
   HIPCHK(hipGetLastError());
   std::vector<PairState> hs(batch);
@@ -1286,7 +1323,7 @@ int h1_impl(const void *in, void *out, int n, int nfields, double sigma, hipStre
   CHK(scratch_get(3, (size_t)n * n * nfields * sizeof(S), &tmpout));
   b.x2_top = (S *)top;
   // write into a temporary so that in == out is allowed
-  CHK(launch_vcycle<S>((const S *)in, 0, (S *)tmpout, l, b, nullptr, sigma, nfields, nullptr, s));
+  CHK(launch_vcycle<S>((const S *)in, 0, (S *)tmpout, l, b, nullptr, sigma, nfields, nullptr, (const S *)nullptr, s));
   HIPCHK(hipMemcpyAsync(out, tmpout, (size_t)n * n * nfields * sizeof(S), hipMemcpyDeviceToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
   return MSG_OK;

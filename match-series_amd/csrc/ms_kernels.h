@@ -18,23 +18,28 @@ constexpr double ARMIJO_SIGMA = 0.5;
 // ================================================================================================
 // PH_NCC_REF0 / PH_NCC_REF1: the reference's mean, then its centred moments (NCC, once per level).
 // An accepted step goes to the step slot of its iteration (iterations alternate between two slots);
-// the update kernel of that iteration applies it and the next iteration's PH_BETA clears the slot,
-// so no phase of its own is needed after the update.
-enum CtrlPhase { PH_LEVEL_INIT = 0, PH_BETA = 1, PH_ARMIJO_INIT = 2, PH_ARMIJO_STEP = 3, PH_NCC_REF0 = 5,
-                 PH_NCC_REF1 = 6 };
+// the update kernel of that iteration applies it and the next iteration's PH_ARMIJO_INIT clears the
+// slot, so no phase of its own is needed after the update.
+// PH_ARMIJO_INIT also forms the Fletcher-Reeves coefficient from the V-cycle's partial sums; the CG
+// direction itself is never stored before the step: the energy passes and the update form it on
+// the fly from z and the previous direction, and the update stores it for the next iteration.
+enum CtrlPhase { PH_LEVEL_INIT = 0, PH_ARMIJO_INIT = 2, PH_ARMIJO_STEP = 3, PH_NCC_REF0 = 5, PH_NCC_REF1 = 6 };
 
 constexpr int CTRL_THREADS = 1024;  // threads of a control workgroup
 constexpr int CTRL_MAXT = 16;       // partial arrays one control sweep can reduce
 
-// Sums of NT consecutive partial arrays base[t * len .. t * len + len) in one sweep of the control
-// workgroup. The partials were just written by a kernel on other XCDs, so a load is a memory round
-// trip: the loads of a sweep (CTRL_RUN per thread and array) are unconditional, from clamped
-// indices, with out-of-range values dropped by a select, so none waits for another. Fixed order;
-// the sums land in out[0 .. NT) (shared memory), valid for every thread after the call.
+// Sums of NT (<= 8) consecutive partial arrays base[t * len .. t * len + len) in one sweep of the
+// control workgroup. The partials were just written by a kernel on other XCDs, so a load is a memory
+// round trip: the loads of the sweep (CTRL_RUN per thread and array) are unconditional, from clamped
+// indices, with out-of-range values dropped by a select, so none waits for another. Each wave sums
+// its lanes; wave 0 then sums the 16 wave results of every array with lane-group shuffles. Fixed
+// order; the sums land in out[0 .. NT) (shared memory), valid for every thread after the call.
 constexpr int CTRL_RUN = 4;  // one sweep covers the 4096 partials of a 1024^2 level
+constexpr int CTRL_WAVES = CTRL_THREADS / 64;
+static_assert(CTRL_WAVES == 16, "the final sum assumes 16 waves");
 template <int NT>
-__device__ void reduce_arrays_n(const double *base, int len, double *out) {
-  __shared__ double red[CTRL_THREADS / 64][NT];
+__device__ __forceinline__ void reduce_arrays_n(const double *base, int len, double *out) {
+  __shared__ double red[NT][CTRL_WAVES];
   double acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = 0.;
@@ -58,36 +63,46 @@ __device__ void reduce_arrays_n(const double *base, int len, double *out) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = wave_sum(acc[t]);
-  __syncthreads();
   if (lane == 0)
 #pragma unroll
-    for (int t = 0; t < NT; ++t) red[wid][t] = acc[t];
+    for (int t = 0; t < NT; ++t) red[t][wid] = acc[t];
   __syncthreads();
-  if (threadIdx.x < NT) {
-    double v = 0.;
-    for (int w = 0; w < CTRL_THREADS / 64; ++w) v += red[w][threadIdx.x];
-    out[threadIdx.x] = v;
+  if (wid == 0) {
+    // lane = 16 g + w holds wave w's sum of array t = 4 h + g (h: the lane's pass over the arrays)
+#pragma unroll
+    for (int h = 0; h < (NT + 3) / 4; ++h) {
+      const int t = 4 * h + (lane >> 4), w = lane & 15;
+      double x = (t < NT) ? red[t < NT ? t : 0][w] : 0.;
+#pragma unroll
+      for (int off = 8; off > 0; off >>= 1) x += __shfl_down(x, off, 16);
+      if (w == 0 && t < NT) out[t] = x;
+    }
   }
   __syncthreads();
 }
 
-// nt (<= CTRL_MAXT) arrays: sweeps of at most 4 arrays
-__device__ void reduce_arrays(const double *base, int nt, int len, double *out) {
-  for (int t0 = 0; t0 < nt; t0 += 4) {
-    const int m = nt - t0 < 4 ? nt - t0 : 4;
+// nt (<= CTRL_MAXT) arrays: sweeps of at most 8 arrays
+__device__ __forceinline__ void reduce_arrays(const double *base, int nt, int len, double *out) {
+  for (int t0 = 0; t0 < nt; t0 += 8) {
+    const int m = nt - t0 < 8 ? nt - t0 : 8;
     const double *b = base + (size_t)t0 * len;
-    if (m == 4) reduce_arrays_n<4>(b, len, out + t0);
-    else if (m == 3) reduce_arrays_n<3>(b, len, out + t0);
-    else if (m == 2) reduce_arrays_n<2>(b, len, out + t0);
-    else reduce_arrays_n<1>(b, len, out + t0);
+    switch (m) {
+      case 8: reduce_arrays_n<8>(b, len, out + t0); break;
+      case 7: reduce_arrays_n<7>(b, len, out + t0); break;
+      case 6: reduce_arrays_n<6>(b, len, out + t0); break;
+      case 5: reduce_arrays_n<5>(b, len, out + t0); break;
+      case 4: reduce_arrays_n<4>(b, len, out + t0); break;
+      case 3: reduce_arrays_n<3>(b, len, out + t0); break;
+      case 2: reduce_arrays_n<2>(b, len, out + t0); break;
+      default: reduce_arrays_n<1>(b, len, out + t0); break;
+    }
   }
 }
 
 struct CtrlArgs {
   int phase;
   int nblk_energy;   // partial count per candidate and term
-  int nblk_norm;     // partials per field of the norm pass
-  int nblk_ncg;      // partials per field of the ncg pass
+  int nblk_norm;     // partials per field and term of the V-cycle's last pass
   double lambda, stop_eps;
   int max_iter;
   int spec;          // candidates per energy pass (1..MAX_CAND)
@@ -99,7 +114,28 @@ struct CtrlArgs {
   int grad_stop;     // 1: USE_GRADIENT_BASED_STOPPING (gradientDescent.h:416)
   int slot;          // step slot of the current iteration (0 or 1)
 };
-constexpr int NCG_TERMS = 3;  // per field and block: <DE,d>, <DE,z>, <DE,DE>
+// Partial sums of the V-cycle's last pass, per field and block: <z,z>, <DE,z>, <DE,d_old>, <DE,DE>.
+constexpr int NORM_TERMS = 4;
+
+// The search direction at element i: the smoothed gradient z, or the Fletcher-Reeves direction
+// d = z + beta d_old (calcTaudAndDescentDir, gradientDescent.h:218-226) as the FP64 sum rounded to S,
+// with d_old taken as zero after a fall-back to the gradient.
+template <typename S>
+__device__ __forceinline__ S search_dir(const S *__restrict__ z, const S *__restrict__ dold, size_t i, bool cg,
+                                        bool use_old, double beta) {
+  const S zv = z[i];
+  if (!cg) return zv;
+  const double od = use_old ? (double)dold[i] : 0.0;
+  return static_cast<S>((double)zv + beta * od);
+}
+// FP32 fields: one fused multiply-add in FP32 (the same expression in every kernel that forms d)
+template <>
+__device__ __forceinline__ float search_dir<float>(const float *__restrict__ z, const float *__restrict__ dold,
+                                                   size_t i, bool cg, bool use_old, double beta) {
+  const float zv = z[i];
+  if (!cg || !use_old) return zv;
+  return __fmaf_rn((float)beta, dold[i], zv);
+}
 
 // Where the energy pass stores its partial sums. SSD: 2 terms per candidate (E_SSD, E_reg). NCC: 4
 // per candidate (1'Mw, w'Mw, w'Mu, E_reg with w = t - c, u = r - m_r) and, after all candidates of
@@ -318,8 +354,8 @@ __device__ void armijo_run(PairState &s, int spec, IterTrace *trace, int trace_c
 // Runs one control phase for one pair with the whole workgroup (the reductions need every thread)
 // on the workgroup's copy s of the pair state.
 __device__ void control_pair(int pair, PairState &s, const double *__restrict__ part_e,
-                             const double *__restrict__ part_n, const double *__restrict__ part_c, const CtrlArgs &a,
-                             int *__restrict__ flags, IterTrace *__restrict__ trace) {
+                             const double *__restrict__ part_n, const CtrlArgs &a, int *__restrict__ flags,
+                             IterTrace *__restrict__ trace) {
   __shared__ double sums[CTRL_MAXT];
   const bool participates = s.active && !s.searching;
 
@@ -395,43 +431,37 @@ __device__ void control_pair(int pair, PairState &s, const double *__restrict__ 
         armijo_run(s, a.spec, trace, a.trace_cap, pair, a.slot);
       }
     }
-  } else if (a.phase == PH_BETA) {
+  } else if (a.phase == PH_ARMIJO_INIT) {
     // the previous iteration's update has run (stream order): free its slot
     if (threadIdx.x == 0) s.step_tau[a.slot ^ 1] = 0;
     if (!participates) return;
-    reduce_arrays(part_n + (size_t)(2 * pair) * a.nblk_norm, 2, a.nblk_norm, sums);
-    const double z0 = sums[0], z1 = sums[1];
+    // field f, term t at f * NORM_TERMS + t
+    reduce_arrays(part_n + (size_t)(2 * pair) * NORM_TERMS * a.nblk_norm, 2 * NORM_TERMS, a.nblk_norm, sums);
     if (threadIdx.x == 0) {
-      double zz = 0;
-      zz += z0;
-      zz += z1;
+      double zz = 0, Dfz = 0, Dfd = 0;
+      zz += sums[0];
+      zz += sums[NORM_TERMS];
+      Dfz += sums[1];
+      Dfz += sums[NORM_TERMS + 1];
+      Dfd += sums[2];
+      Dfd += sums[NORM_TERMS + 2];
       s.iter += 1;
       s.beta = ((s.iter - 1) % 10) ? (zz / s.zz_old) : 0.0;
       s.zz_old = zz;
       s.evals = 0;
       s.passes = 0;
-    }
-  } else if (a.phase == PH_ARMIJO_INIT) {
-    if (!participates) return;
-    // field f, term t at f * NCG_TERMS + t
-    reduce_arrays(part_c + (size_t)(2 * pair) * NCG_TERMS * a.nblk_ncg, 2 * NCG_TERMS, a.nblk_ncg, sums);
-    const double d0 = sums[0], g0 = sums[1], d1 = sums[NCG_TERMS], g1 = sums[NCG_TERMS + 1];
-    const double n0 = a.grad_stop ? sums[2] : 0, n1 = a.grad_stop ? sums[NCG_TERMS + 2] : 0;
-    if (threadIdx.x == 0) {
-      double Df = 0, Dfz = 0;
-      Df += d0;
-      Df += d1;
-      Dfz += g0;
-      Dfz += g1;
+      // <DE, d> of d = z + beta d_old, or of d = z when the old direction counts as zero
+      s.cg_old = s.olddir_zero ? 0 : 1;
+      const double Df = s.cg_old ? Dfz + s.beta * Dfd : Dfz;
       s.Df = Df;
       s.Dfz = Dfz;
       if (a.grad_stop) {
         double dd = 0;
-        dd += n0;
-        dd += n1;
+        dd += sums[3];
+        dd += sums[NORM_TERMS + 3];
         s.dd = dd;
       }
-      s.olddir_zero = 0;  // d now holds this iteration's CG direction
+      s.olddir_zero = 0;  // the update stores this iteration's CG direction
       s.tau_first = 0;
       armijo_start(s, DIR_CG, Df, s.tau);
       s.searching = 1;
@@ -442,8 +472,8 @@ __device__ void control_pair(int pair, PairState &s, const double *__restrict__ 
 }
 
 __global__ void __launch_bounds__(CTRL_THREADS)
-k_control(PairState *__restrict__ st, const double *__restrict__ part_e, const double *__restrict__ part_n,
-          const double *__restrict__ part_c, CtrlArgs a, int *__restrict__ flags, IterTrace *__restrict__ trace) {
+k_control(PairState *__restrict__ st, const double *__restrict__ part_e, const double *__restrict__ part_n, CtrlArgs a,
+          int *__restrict__ flags, IterTrace *__restrict__ trace) {
   // The state machine runs on an LDS copy of the pair state: one coalesced load and one store
   // instead of a chain of dependent global accesses by the thread that advances it.
   static_assert(sizeof(PairState) % 4 == 0, "pair state copied in 32-bit words");
@@ -453,7 +483,7 @@ k_control(PairState *__restrict__ st, const double *__restrict__ part_e, const d
   unsigned *g = reinterpret_cast<unsigned *>(st + pair), *l = reinterpret_cast<unsigned *>(&ss);
   for (int i = threadIdx.x; i < NW; i += CTRL_THREADS) l[i] = g[i];
   __syncthreads();
-  control_pair(pair, ss, part_e, part_n, part_c, a, flags, trace);
+  control_pair(pair, ss, part_e, part_n, a, flags, trace);
   __syncthreads();
   for (int i = threadIdx.x; i < NW; i += CTRL_THREADS) g[i] = l[i];
 }
@@ -479,8 +509,8 @@ constexpr int EN_TX = 64, EN_TY = 4;  // elements per workgroup tile (one per th
 
 template <typename S>
 __global__ void __launch_bounds__(256)
-k_energy(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__ phi, const S *__restrict__ dcg,
-         const S *__restrict__ dgrad, int n, double h, int width, const PairState *st,
+k_energy(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__ phi, const S *__restrict__ z,
+         const S *__restrict__ dold, int n, double h, int width, const PairState *st,
          double *__restrict__ part, int ntx, int nty, int npairs) {
   const int tiles = ntx * nty;
   const int wg = xcd_remap(blockIdx.x, tiles * npairs);
@@ -493,8 +523,9 @@ k_energy(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__
   const size_t nn = (size_t)n * n;
   const S *Rp = R + pair * nn, *Tp = T + pair * nn;
   const S *px = phi + pair * 2 * nn, *py = px + nn;
-  const S *dsel = (dm == DIR_CG) ? dcg : dgrad;
-  const S *dx = dsel + pair * 2 * nn, *dy = dx + nn;
+  const S *zx = z + pair * 2 * nn, *zy = zx + nn, *ox_ = dold + pair * 2 * nn, *oy_ = ox_ + nn;
+  const bool cg = (dm == DIR_CG), use_old = st[pair].cg_old != 0;
+  const double beta = st[pair].beta;
   const bool use_dir = (dm != DIR_NONE);
 
   const int ex = (tile % ntx) * EN_TX + (threadIdx.x % EN_TX);
@@ -513,8 +544,8 @@ k_energy(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__
       const int g = l2g(ex, ey, b, n);
       f0[b] = ld(px, g);
       f1[b] = ld(py, g);
-      d0[b] = use_dir ? ld(dx, g) : 0.0;
-      d1[b] = use_dir ? ld(dy, g) : 0.0;
+      d0[b] = use_dir ? (double)search_dir<S>(zx, ox_, g, cg, use_old, beta) : 0.0;
+      d1[b] = use_dir ? (double)search_dir<S>(zy, oy_, g, cg, use_old, beta) : 0.0;
       rn[b] = ld(Rp, g);
     }
 #pragma unroll
@@ -573,8 +604,8 @@ k_energy(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__
 // ================================================================================================
 template <typename S>
 __global__ void __launch_bounds__(256)
-k_energy_cons(const S *__restrict__ psi, int npsi, const S *__restrict__ phi, const S *__restrict__ dcg,
-              const S *__restrict__ dgrad, int n, double h, int width, const PairState *st, double *__restrict__ part,
+k_energy_cons(const S *__restrict__ psi, int npsi, const S *__restrict__ phi, const S *__restrict__ z,
+              const S *__restrict__ dold, int n, double h, int width, const PairState *st, double *__restrict__ part,
               int ntx, int nty, int npairs) {
   const int tiles = ntx * nty;
   const int wg = xcd_remap(blockIdx.x, tiles * npairs);
@@ -586,8 +617,9 @@ k_energy_cons(const S *__restrict__ psi, int npsi, const S *__restrict__ phi, co
 
   const size_t nn = (size_t)n * n;
   const S *px = phi + pair * 2 * nn, *py = px + nn;
-  const S *dsel = (dm == DIR_CG) ? dcg : dgrad;
-  const S *dx = dsel + pair * 2 * nn, *dy = dx + nn;
+  const S *zx = z + pair * 2 * nn, *zy = zx + nn, *ox_ = dold + pair * 2 * nn, *oy_ = ox_ + nn;
+  const bool cg = (dm == DIR_CG), use_old = st[pair].cg_old != 0;
+  const double beta = st[pair].beta;
   const bool use_dir = (dm != DIR_NONE);
 
   const int ex = (tile % ntx) * EN_TX + (threadIdx.x % EN_TX);
@@ -605,8 +637,8 @@ k_energy_cons(const S *__restrict__ psi, int npsi, const S *__restrict__ phi, co
       const int g = l2g(ex, ey, b, n);
       f0[b] = ld(px, g);
       f1[b] = ld(py, g);
-      d0[b] = use_dir ? ld(dx, g) : 0.0;
-      d1[b] = use_dir ? ld(dy, g) : 0.0;
+      d0[b] = use_dir ? (double)search_dir<S>(zx, ox_, g, cg, use_old, beta) : 0.0;
+      d1[b] = use_dir ? (double)search_dir<S>(zy, oy_, g, cg, use_old, beta) : 0.0;
     }
     const double vol = h * h;
 #pragma unroll
@@ -663,8 +695,8 @@ k_energy_cons(const S *__restrict__ psi, int npsi, const S *__restrict__ phi, co
 // ================================================================================================
 template <typename S>
 __global__ void __launch_bounds__(256)
-k_energy_ncc(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__ phi, const S *__restrict__ dcg,
-             const S *__restrict__ dgrad, int n, double h, const PairState *st, double *__restrict__ part, int ntx,
+k_energy_ncc(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__ phi, const S *__restrict__ z,
+             const S *__restrict__ dold, int n, double h, const PairState *st, double *__restrict__ part, int ntx,
              int nty, int npairs) {
   const int tiles = ntx * nty;
   const int wg = xcd_remap(blockIdx.x, tiles * npairs);
@@ -677,8 +709,9 @@ k_energy_ncc(const S *__restrict__ R, const S *__restrict__ T, const S *__restri
   const size_t nn = (size_t)n * n;
   const S *Rp = R + pair * nn, *Tp = T + pair * nn;
   const S *px = phi + pair * 2 * nn, *py = px + nn;
-  const S *dsel = (dm == DIR_CG) ? dcg : dgrad;
-  const S *dx = dsel + pair * 2 * nn, *dy = dx + nn;
+  const S *zx = z + pair * 2 * nn, *zy = zx + nn, *ox_ = dold + pair * 2 * nn, *oy_ = ox_ + nn;
+  const bool cg = (dm == DIR_CG), use_old = st[pair].cg_old != 0;
+  const double beta = st[pair].beta;
   const bool use_dir = (dm != DIR_NONE);
 
   const int ex = (tile % ntx) * EN_TX + (threadIdx.x % EN_TX);
@@ -698,8 +731,8 @@ k_energy_ncc(const S *__restrict__ R, const S *__restrict__ T, const S *__restri
       const int g = l2g(ex, ey, b, n);
       f0[b] = ld(px, g);
       f1[b] = ld(py, g);
-      d0[b] = use_dir ? ld(dx, g) : 0.0;
-      d1[b] = use_dir ? ld(dy, g) : 0.0;
+      d0[b] = use_dir ? (double)search_dir<S>(zx, ox_, g, cg, use_old, beta) : 0.0;
+      d1[b] = use_dir ? (double)search_dir<S>(zy, oy_, g, cg, use_old, beta) : 0.0;
       u[b] = ld(Rp, g) + (-rm);  // the reference centred as normalizeImageForNCC does (addToAll)
     }
     double ml[4][4];
@@ -1055,7 +1088,7 @@ template <typename S>
 __global__ void __launch_bounds__(256)
 k_mg_up(const S *__restrict__ x2in, const S *__restrict__ xc, const S *__restrict__ rhs, int negate,
         S *__restrict__ xout, int n, const PairState *st, double sigma_default, int ntx, int nty,
-        int nfields, double *__restrict__ nrm_part) {
+        int nfields, const S *__restrict__ dold, double *__restrict__ nrm_part) {
   const int tiles = ntx * nty;
   const int wg = xcd_remap(blockIdx.x, tiles * nfields);
   const int field = wg / tiles, tile = wg - field * tiles;
@@ -1067,7 +1100,7 @@ k_mg_up(const S *__restrict__ x2in, const S *__restrict__ xc, const S *__restric
   __shared__ double sx[MG_HY][MG_HX];  // x on tile+2
   __shared__ double sr[MG_HY][MG_HX];  // rhs on tile+1
   __shared__ double s3[MG_HY][MG_HX];  // x3 on tile+1
-  __shared__ double red[4];
+  __shared__ double red[4 * NORM_TERMS];
   const size_t nn = (size_t)n * n;
   const int nc = n / 2;
   const S *x2p = x2in + field * nn, *rp = rhs + field * nn, *xcp = xc + field * (size_t)nc * nc;
@@ -1097,7 +1130,7 @@ k_mg_up(const S *__restrict__ x2in, const S *__restrict__ xc, const S *__restric
   __syncthreads();
   auto get3 = [&](int gx, int gy) { return s3[gy - by][gx - bx]; };
   S *xo = xout + field * nn;
-  double nrm = 0;
+  double acc[NORM_TERMS] = {0., 0., 0., 0.};  // <z,z>, <DE,z>, <DE,d_old>, <DE,DE> (DE = the raw input)
   for (int i = threadIdx.x; i < MG_TX * MG_TY; i += 256) {
     const int tx = i % MG_TX, ty = i / MG_TX;
     const int gx = x0 + tx, gy = y0 + ty;
@@ -1106,13 +1139,19 @@ k_mg_up(const S *__restrict__ x2in, const S *__restrict__ xc, const S *__restric
       const double v = sr[ty + 2][tx + 2] - ax;
       const S o = static_cast<S>(s3[ty + 2][tx + 2] + 1.0 * v / diag);
       xo[gx + gy * n] = o;
-      const double od = static_cast<double>(o);
-      nrm += od * od;
+      if (nrm_part) {
+        const double od = static_cast<double>(o), de = ld(rp, gx + gy * n);
+        acc[0] += od * od;
+        acc[1] += de * od;
+        acc[2] += de * ld(dold + field * nn, gx + gy * n);
+        acc[3] += de * de;
+      }
     }
   }
   if (nrm_part) {
-    const double s = block_sum<256>(nrm, red);
-    if (threadIdx.x == 0) nrm_part[(size_t)field * tiles + tile] = s;
+    block_sum_n<NORM_TERMS>(acc, red);
+    if (threadIdx.x == 0)
+      for (int t = 0; t < NORM_TERMS; ++t) nrm_part[((size_t)field * NORM_TERMS + t) * tiles + tile] = acc[t];
   }
 }
 
@@ -1120,19 +1159,19 @@ k_mg_up(const S *__restrict__ x2in, const S *__restrict__ xc, const S *__restric
 // With n equal to the registration level this is the complete metric; otherwise it is the coarse
 // part of the cycle started by k_mg_down.
 constexpr int MGC_MAXL = 5;
-constexpr int MGC_OFF[MGC_MAXL + 2] = {0, 1, 5, 21, 85, 341, 1365};  // prefix sums of 4^l
+constexpr int MGC_OFF[8] = {0, 1, 5, 21, 85, 341, 1365, 5461};  // prefix sums of 4^l
 
 template <typename S>
 __global__ void __launch_bounds__(256)
 k_mg_coarse(const S *__restrict__ rhs, int negate, S *__restrict__ xout, int L, const PairState *st,
-            double sigma_default, int nfields, double *__restrict__ nrm_part) {
+            double sigma_default, int nfields, const S *__restrict__ dold, double *__restrict__ nrm_part) {
   const int field = blockIdx.x;
   const int pair = field >> 1;
   if (st && !participates(st[pair])) return;
   const double sigma = st ? st[pair].sigma : sigma_default;
   const double tau = 0.5 * (sigma * sigma);
   __shared__ double X[MGC_OFF[MGC_MAXL + 1]], B[MGC_OFF[MGC_MAXL + 1]], D[1024];
-  __shared__ double red[4];
+  __shared__ double red[4 * NORM_TERMS];
   const int n = 1 << L;
   const size_t nn = (size_t)n * n;
   const S *rp = rhs + field * nn;
@@ -1203,82 +1242,55 @@ k_mg_coarse(const S *__restrict__ rhs, int negate, S *__restrict__ xout, int L, 
     sweep(l);
   }
   S *xo = xout + field * nn;
-  double nrm = 0;
+  double acc[NORM_TERMS] = {0., 0., 0., 0.};  // <z,z>, <DE,z>, <DE,d_old>, <DE,DE> (DE = the raw input)
   for (int i = threadIdx.x; i < (int)nn; i += 256) {
     const S o = static_cast<S>(X[MGC_OFF[L] + i]);
     xo[i] = o;
-    const double od = static_cast<double>(o);
-    nrm += od * od;
-  }
-  if (nrm_part) {
-    const double s = block_sum<256>(nrm, red);
-    if (threadIdx.x == 0) nrm_part[field] = s;
-  }
-}
-
-// ================================================================================================
-// Fletcher-Reeves direction d = z + beta * d_old (calcTaudAndDescentDir, gradientDescent.h:218-226)
-// and the slopes <DE, d>, <DE, z> (ArmijoLineSearchHelpFunction_evaluateDerivative, L176-182).
-// ================================================================================================
-constexpr int NCG_PER_THREAD = 4;
-
-template <typename S>
-__global__ void __launch_bounds__(256)
-k_ncg(const S *__restrict__ z, S *__restrict__ d, const S *__restrict__ DE, int n, const PairState *st,
-      double *__restrict__ part, int nblk_field, int nfields) {
-  const int total = nblk_field * nfields;
-  const int wg = xcd_remap(blockIdx.x, total);
-  const int field = wg / nblk_field, blk = wg - field * nblk_field;
-  const int pair = field >> 1;
-  if (!participates(st[pair])) return;
-  const double beta = st[pair].beta;
-  const bool zero_old = st[pair].olddir_zero != 0;
-  __shared__ double red[4];
-  const size_t nn = (size_t)n * n;
-  const S *zp = z + field * nn, *dep = DE + field * nn;
-  S *dp = d + field * nn;
-  double sd = 0, sz = 0, sn = 0;
-  const int base = blk * 256 * NCG_PER_THREAD;
-  for (int k = 0; k < NCG_PER_THREAD; ++k) {
-    const int i = base + k * 256 + threadIdx.x;
-    if (i < (int)nn) {
-      const double zv = ld(zp, i);
-      const double od = zero_old ? 0.0 : ld(dp, i);
-      const S dv = static_cast<S>(zv + beta * od);
-      dp[i] = dv;
-      const double de = ld(dep, i);
-      sd += de * static_cast<double>(dv);
-      sz += de * zv;
-      sn += de * de;
+    if (nrm_part) {
+      const double od = static_cast<double>(o), de = ld(rp, i);
+      acc[0] += od * od;
+      acc[1] += de * od;
+      acc[2] += de * ld(dold + field * nn, i);
+      acc[3] += de * de;
     }
   }
-  const double a = block_sum<256>(sd, red);
-  const double b = block_sum<256>(sz, red);
-  const double c = block_sum<256>(sn, red);
-  if (threadIdx.x == 0) {
-    part[((size_t)field * NCG_TERMS + 0) * nblk_field + blk] = a;
-    part[((size_t)field * NCG_TERMS + 1) * nblk_field + blk] = b;
-    part[((size_t)field * NCG_TERMS + 2) * nblk_field + blk] = c;
+  if (nrm_part) {
+    block_sum_n<NORM_TERMS>(acc, red);
+    if (threadIdx.x == 0)
+      for (int t = 0; t < NORM_TERMS; ++t) nrm_part[(size_t)field * NORM_TERMS + t] = acc[t];
   }
 }
 
-// phi += tau * dir for pairs whose search accepted a step (GradientDescentBase::applySingle L382-385).
+// ================================================================================================
+// Update phi += tau * dir for pairs whose search accepted a step (GradientDescentBase::applySingle
+// L382-385). Along the CG direction the kernel forms d = z + beta d_old (search_dir) and stores it
+// in place of d_old: it is the next iteration's old direction.
+// ================================================================================================
+constexpr int UPD_PER_THREAD = 4;
+
 template <typename S>
 __global__ void __launch_bounds__(256)
-k_update(S *__restrict__ phi, const S *__restrict__ dcg, const S *__restrict__ dgrad, int n, const PairState *st,
-         int nblk_pair, int npairs, int slot) {
+k_update(S *__restrict__ phi, S *__restrict__ d, const S *__restrict__ z, int n, const PairState *st, int nblk_pair,
+         int npairs, int slot) {
   const int total = nblk_pair * npairs;
   const int wg = xcd_remap(blockIdx.x, total);
   const int pair = wg / nblk_pair, blk = wg - pair * nblk_pair;
   const double tau = st[pair].step_tau[slot];
   if (tau == 0) return;  // no step accepted in this iteration
-  const S *dir = (st[pair].step_dir[slot] == DIR_CG ? dcg : dgrad) + pair * 2 * (size_t)n * n;
-  S *p = phi + pair * 2 * (size_t)n * n;
+  const bool cg = st[pair].step_dir[slot] == DIR_CG, use_old = st[pair].cg_old != 0;
+  const double beta = st[pair].beta;
+  const size_t off = pair * 2 * (size_t)n * n;
+  S *p = phi + off, *dp = d + off;
+  const S *zp = z + off;
   const int m = 2 * n * n;
-  const int base = blk * 256 * NCG_PER_THREAD;
-  for (int k = 0; k < NCG_PER_THREAD; ++k) {
+  const int base = blk * 256 * UPD_PER_THREAD;
+  for (int k = 0; k < UPD_PER_THREAD; ++k) {
     const int i = base + k * 256 + threadIdx.x;
-    if (i < m) p[i] = step_at<S>(p[i], tau, dir[i]);
+    if (i < m) {
+      const S dv = search_dir<S>(zp, dp, i, cg, use_old, beta);
+      if (cg) dp[i] = dv;
+      p[i] = step_at<S>(p[i], tau, dv);
+    }
   }
 }
 

@@ -50,6 +50,7 @@ struct PairState {
   int active;         // still iterating at this level
   int searching;      // Armijo search in progress
   int olddir_zero;    // the previous search fell back: treat the old CG direction as zero
+  int cg_old;         // this iteration's CG direction is z + beta d_old (0: z; the old direction is zero)
   int evals;          // evaluations consumed by this iteration's searches (reference count)
   int passes;         // energy passes launched for this iteration
   int total_evals, total_passes;

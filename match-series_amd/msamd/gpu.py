@@ -94,6 +94,7 @@ SIGNATURES = {
     "msg_profile_enable": ([ctypes.c_uint], ctypes.c_int),
     "msg_profile_reset": ([], ctypes.c_int),
     "msg_profile_read": ([ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong), _dp], ctypes.c_int),
+    "msg_time_energy_pass": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp], ctypes.c_int),
     "msg_match_series": ([ctypes.POINTER(SeriesParams), ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _dp, _vp,
                           _vp, _vp, ctypes.POINTER(SeriesLevels), _vp], ctypes.c_int),
     "msg_reduce_deformations": ([ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _dp,
@@ -354,6 +355,13 @@ class Context:
         a, b, c = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
         check(lib().msg_last_counts(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
+
+    def time_energy_pass(self, level, ncand=3, reps=50):
+        """Mean time (ms) and algorithmic bytes of one energy pass over all pairs of the context, with
+        `ncand` step sizes along the direction of the last solve at this level."""
+        ms, b = ctypes.c_double(), ctypes.c_double()
+        check(lib().msg_time_energy_pass(self.h, level, ncand, reps, ctypes.byref(ms), ctypes.byref(b)))
+        return ms.value, b.value
 
 
 def series_level_layout(params, n, reuse=False):

@@ -72,11 +72,11 @@ def check_stage(out, g, st, precision, tol, med_tol):
     # pixels with the same sample count (FP32 fields can move a sample point across the image edge)
     ok = cnt == g[f"s{st}_numSamples"]
     # PNG previews: the same 8-bit scaling of the value range; a pixel may differ by one step
-    # where the data differ below the reference's rounding
+    # where the data differ below the reference's rounding (FP32 fields: on up to 5 % of the pixels)
     for nm in ("average", "median", "numSamples"):
         png = read_png_gray8(str(out / f"stage{st}" / f"{nm}.png")).astype(int)
         diff = np.abs(png - g[f"s{st}_{nm}_png"].astype(int))[ok]
-        assert diff.max() <= 1 and (diff > 0).mean() < 0.01, nm
+        assert diff.max() <= 1 and (diff > 0).mean() < (0.01 if precision == 64 else 0.05), nm
     med = read_p9(str(out / f"stage{st}" / "median.q2bz"))
     assert np.abs(med - g[f"s{st}_median"])[ok].max() < med_tol
     en = np.loadtxt(str(out / f"stage{st}" / "energies.txt"))[:, 1]
