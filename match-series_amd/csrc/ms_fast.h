@@ -42,35 +42,61 @@ struct HitF {
   bool inx, iny;
 };
 
-// quad point q of element (ex, ey) displaced by (ox, oy) in domain units
-__device__ __forceinline__ HitF deform_qp_f(int ex, int ey, int q, float ox, float oy, int n, float nm1) {
-#pragma clang fp contract(fast)
+// quad point of element (ex, ey) at ux, uy grid units from the element's lower node
+__device__ __forceinline__ HitF hit_rel(int ex, int ey, float ux, float uy, int n) {
   HitF r;
   int elx, ely;
-  clip_rel(ex, ox * nm1 + qpxf(q), n, elx, r.lx, r.inx);
-  clip_rel(ey, oy * nm1 + qpyf(q), n, ely, r.ly, r.iny);
+  clip_rel(ex, ux, n, elx, r.lx, r.inx);
+  clip_rel(ey, uy, n, ely, r.ly, r.iny);
   r.i0 = elx + ely * n;
   return r;
 }
 
-// bilinear value at the hit; t receives the 4 nodal values
-__device__ __forceinline__ float bilin_f(const float *__restrict__ T, const HitF &p, int n, float t[4]) {
+// Values of the Q1 interpolant of the element's nodal values p at the 4 quad points (q = 2 qx + qy),
+// interpolating along x, then y.
+__device__ __forceinline__ void quad_values(const float p[4], float v[4]) {
 #pragma clang fp contract(fast)
-  t[0] = T[p.i0];
-  t[1] = T[p.i0 + 1];
-  t[2] = T[p.i0 + n];
-  t[3] = T[p.i0 + n + 1];
+  const float q0 = (float)MSG_QP0, q1 = (float)MSG_QP1;
+  const float dx0 = p[1] - p[0], dx1 = p[3] - p[2];
+  const float a0 = q0 * dx0 + p[0], a1 = q1 * dx0 + p[0];  // row y = 0 at qx = QP0, QP1
+  const float b0 = q0 * dx1 + p[2], b1 = q1 * dx1 + p[2];  // row y = 1
+  v[0] = q0 * (b0 - a0) + a0;
+  v[1] = q1 * (b0 - a0) + a0;
+  v[2] = q0 * (b1 - a1) + a1;
+  v[3] = q1 * (b1 - a1) + a1;
+}
+
+// Gathers go through a buffer resource over one image: 32-bit offsets (no 64-bit address arithmetic
+// per load), neighbouring nodes merged into one two-dword load, and reads past the image return 0.
+// Resource word 3 is the gfx9-family raw-buffer format (32-bit data, no swizzle).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t image_rsrc(const float *p, int count) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), (short)0, count * 4, 0x00020000);
+}
+__device__ __forceinline__ float ld_rsrc(__amdgpu_buffer_rsrc_t r, unsigned idx) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, idx * 4u, 0, 0));
+}
+
+// bilinear value at the hit; t receives the 4 nodal values
+__device__ __forceinline__ float bilin_f(__amdgpu_buffer_rsrc_t T, const HitF &p, int n, float t[4]) {
+#pragma clang fp contract(fast)
+  const unsigned o = (unsigned)p.i0, on = o + (unsigned)n;
+  t[0] = ld_rsrc(T, o);
+  t[1] = ld_rsrc(T, o + 1u);
+  t[2] = ld_rsrc(T, on);
+  t[3] = ld_rsrc(T, on + 1u);
   const float a = p.lx * (t[1] - t[0]) + t[0];
   const float b = p.lx * (t[3] - t[2]) + t[2];
   return p.ly * (b - a) + a;
 }
 
-// psi^T K_loc psi of the Q1 stiffness matrix (see stiff_form)
-__device__ __forceinline__ float stiff_form_f(const float p[4]) {
+// a^T K_loc b of the Q1 stiffness matrix in the difference form of stiff_form
+__device__ __forceinline__ float stiff_bi_f(const float a[4], const float b[4]) {
 #pragma clang fp contract(fast)
-  const float e0 = p[0] - p[1], e1 = p[0] - p[2], e2 = p[1] - p[3], e3 = p[2] - p[3];
-  const float d0 = p[0] - p[3], d1 = p[1] - p[2];
-  return (e0 * e0 + e1 * e1 + e2 * e2 + e3 * e3) * (1.0f / 6.0f) + (d0 * d0 + d1 * d1) * (2.0f / 6.0f);
+  const float ea0 = a[0] - a[1], ea1 = a[0] - a[2], ea2 = a[1] - a[3], ea3 = a[2] - a[3];
+  const float da0 = a[0] - a[3], da1 = a[1] - a[2];
+  const float eb0 = b[0] - b[1], eb1 = b[0] - b[2], eb2 = b[1] - b[3], eb3 = b[2] - b[3];
+  const float db0 = b[0] - b[3], db1 = b[1] - b[2];
+  return (ea0 * eb0 + ea1 * eb1 + ea2 * eb2 + ea3 * eb3) * (1.0f / 6.0f) + (da0 * db0 + da1 * db1) * (2.0f / 6.0f);
 }
 
 // ================================================================================================
@@ -91,6 +117,7 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
 
   const size_t nn = (size_t)n * n;
   const float *Rp = R + pair * nn, *Tp = T + pair * nn;
+  const __amdgpu_buffer_rsrc_t Trs = image_rsrc(Tp, (int)nn);
   const float *px = phi + pair * 2 * nn, *py = px + nn;
   const float *zx = z + pair * 2 * nn, *zy = zx + nn, *ox_ = dold + pair * 2 * nn, *oy_ = ox_ + nn;
   const bool cg = (dm == DIR_CG), use_old = st[pair].cg_old != 0;
@@ -107,8 +134,11 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
   for (int k = 0; k < 2 * MAX_CAND; ++k) acc[k] = 0.0;
 
   if (valid) {
-    float f0[4], f1[4], d0[4], d1[4], rq[4];
-    float rn[4];
+    // Positions are linear in the step size: the quad points of phi + tau d sit at u_f + tau u_d
+    // (grid units relative to the element), and psi^T K psi of phi + tau d is a quadratic in tau. One
+    // interpolation of phi and d per element serves every candidate; positions differ from those of
+    // the stored phi + tau d (rounded per node) by that rounding, about 3e-7 px at 10 px.
+    float f0[4], f1[4], d0[4], d1[4], rn[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int g = l2g(ex, ey, b, n);
@@ -118,30 +148,37 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
       d1[b] = use_dir ? search_dir<float>(zy, oy_, g, cg, use_old, beta) : 0.f;
       rn[b] = Rp[g];
     }
+    float rq[4], ufx[4], ufy[4], udx[4], udy[4];
+    quad_values(rn, rq);
+    quad_values(f0, ufx);
+    quad_values(f1, ufy);
+    quad_values(d0, udx);
+    quad_values(d1, udy);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) rq[q] = rn[0] * bqf(0, q) + rn[1] * bqf(1, q) + rn[2] * bqf(2, q) + rn[3] * bqf(3, q);
+    for (int q = 0; q < 4; ++q) {
+      ufx[q] = ufx[q] * nm1 + qpxf(q);
+      ufy[q] = ufy[q] * nm1 + qpyf(q);
+      udx[q] *= nm1;
+      udy[q] *= nm1;
+    }
+    const double sff = (double)stiff_bi_f(f0, f0) + (double)stiff_bi_f(f1, f1);
+    const double sfd = (double)stiff_bi_f(f0, d0) + (double)stiff_bi_f(f1, d1);
+    const double sdd = (double)stiff_bi_f(d0, d0) + (double)stiff_bi_f(d1, d1);
 #pragma unroll
     for (int k = 0; k < MAX_CAND; ++k) {
       if (k < nc) {
-        const double tau = st[pair].cand[k];
-        float p0[4], p1[4];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          p0[b] = use_dir ? step_at<float>(f0[b], tau, d0[b]) : f0[b];
-          p1[b] = use_dir ? step_at<float>(f1[b], tau, d1[b]) : f1[b];
-        }
+        const double taud = st[pair].cand[k];
+        const float tau = (float)taud;
         float a = 0.f;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const float ox = p0[0] * bqf(0, q) + p0[1] * bqf(1, q) + p0[2] * bqf(2, q) + p0[3] * bqf(3, q);
-          const float oy = p1[0] * bqf(0, q) + p1[1] * bqf(1, q) + p1[2] * bqf(2, q) + p1[3] * bqf(3, q);
-          const HitF hp = deform_qp_f(ex, ey, q, ox, oy, n, nm1);
+          const HitF hp = hit_rel(ex, ey, tau * udx[q] + ufx[q], tau * udy[q] + ufy[q], n);
           float t4[4];
-          const float t = bilin_f(Tp, hp, n, t4) - rq[q];
+          const float t = bilin_f(Trs, hp, n, t4) - rq[q];
           a += t * t;
         }
         acc[2 * k] = (double)a * (0.125 * vol);  // 1/2 t^2 * quad weight 1/4 * vol
-        acc[2 * k + 1] = 0.5 * ((double)stiff_form_f(p0) + (double)stiff_form_f(p1));
+        acc[2 * k + 1] = 0.5 * (sff + taud * (2.0 * sfd + taud * sdd));
       }
     }
   }
@@ -176,6 +213,7 @@ k_grad_f32(const float *__restrict__ R, const float *__restrict__ T, const float
 
   const size_t nn = (size_t)n * n;
   const float *Rp = R + pair * nn, *Tp = T + pair * nn;
+  const __amdgpu_buffer_rsrc_t Trs = image_rsrc(Tp, (int)nn);
   const float *px = phi + pair * 2 * nn, *py = px + nn;
   const int x0 = (tile % ntx) * GR_TX, y0 = (tile / ntx) * GR_TY;
   const float nm1 = (float)(n - 1);
@@ -192,15 +230,15 @@ k_grad_f32(const float *__restrict__ R, const float *__restrict__ T, const float
       f1[b] = py[g];
       rn[b] = Rp[g];
     }
-    float nl[4][2];
+    float nl[4][2], ox4[4], oy4[4], r4[4];
+    quad_values(f0, ox4);
+    quad_values(f1, oy4);
+    quad_values(rn, r4);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const float ox = f0[0] * bqf(0, q) + f0[1] * bqf(1, q) + f0[2] * bqf(2, q) + f0[3] * bqf(3, q);
-      const float oy = f1[0] * bqf(0, q) + f1[1] * bqf(1, q) + f1[2] * bqf(2, q) + f1[3] * bqf(3, q);
-      const float r = rn[0] * bqf(0, q) + rn[1] * bqf(1, q) + rn[2] * bqf(2, q) + rn[3] * bqf(3, q);
-      const HitF hp = deform_qp_f(ex, ey, q, ox, oy, n, nm1);
+      const HitF hp = hit_rel(ex, ey, ox4[q] * nm1 + qpxf(q), oy4[q] * nm1 + qpyf(q), n);
       float t4[4];
-      const float diff = bilin_f(Tp, hp, n, t4) - r;
+      const float diff = bilin_f(Trs, hp, n, t4) - r4[q];
       // gradient of the bilinear T at the hit, in domain units
       const float ax = t4[1] - t4[0], bx = t4[3] - t4[2];
       const float ay = t4[2] - t4[0], by = t4[3] - t4[1];

@@ -128,13 +128,14 @@ __device__ __forceinline__ S search_dir(const S *__restrict__ z, const S *__rest
   const double od = use_old ? (double)dold[i] : 0.0;
   return static_cast<S>((double)zv + beta * od);
 }
-// FP32 fields: one fused multiply-add in FP32 (the same expression in every kernel that forms d)
+// FP32 fields: one fused multiply-add in FP32 (the same expression in every kernel that forms d);
+// both loads unconditional and the choice a select, so the loads of neighbouring nodes overlap.
 template <>
 __device__ __forceinline__ float search_dir<float>(const float *__restrict__ z, const float *__restrict__ dold,
                                                    size_t i, bool cg, bool use_old, double beta) {
-  const float zv = z[i];
-  if (!cg || !use_old) return zv;
-  return __fmaf_rn((float)beta, dold[i], zv);
+  const float zv = z[i], ov = dold[i];
+  const float dv = __fmaf_rn((float)beta, ov, zv);
+  return (cg && use_old) ? dv : zv;
 }
 
 // Where the energy pass stores its partial sums. SSD: 2 terms per candidate (E_SSD, E_reg). NCC: 4
