@@ -497,23 +497,21 @@ k_mg_coarse_f32(const float *__restrict__ rhs, int negate, float *__restrict__ x
     idg = (float)(1.0 / (1 + 4 * tau * s));
   };
 
-  for (int l = L; l >= 1; --l) {
+  // Down leg of level l with nthr threads (thread tid): two Jacobi sweeps from zero, the second kept
+  // in X for the up leg, then the residual restricted to level l - 1.
+  auto down_level = [&](int l, int tid, int nthr, auto sync) {
     const int m = 1 << l, mm = m * m, mc = m / 2;
     float hs, idg;
     consts(l, hs, idg);
     float *x = X + MGC_OFF[l];
     const float *b = B + MGC_OFF[l];
-    // first sweep from zero, into Y
-    for (int i = threadIdx.x; i < mm; i += MGC_THREADS) Y[i] = b[i] * idg;
-    __syncthreads();
-    // second sweep, into x (kept for the up leg)
+    for (int i = tid; i < mm; i += nthr) Y[i] = b[i] * idg;  // first sweep from zero, into Y
+    sync();
     auto gy = [&](int gx, int gyy) { return Y[gx + gyy * m]; };
-    for (int i = threadIdx.x; i < mm; i += MGC_THREADS)
-      x[i] = (b[i] - heat_apply_f(gy, i % m, i / m, m, hs)) * idg + Y[i];
-    __syncthreads();
-    // residual at the 4 children of every coarse node, averaged
+    for (int i = tid; i < mm; i += nthr) x[i] = (b[i] - heat_apply_f(gy, i % m, i / m, m, hs)) * idg + Y[i];
+    sync();
     auto gx2 = [&](int px, int py) { return x[px + py * m]; };
-    for (int i = threadIdx.x; i < mc * mc; i += MGC_THREADS) {
+    for (int i = tid; i < mc * mc; i += nthr) {  // residual at the 4 children of a coarse node, averaged
       const int cx = i % mc, cy = i / mc;
       float r[4];
 #pragma unroll
@@ -523,30 +521,46 @@ k_mg_coarse_f32(const float *__restrict__ rhs, int negate, float *__restrict__ x
       }
       B[MGC_OFF[l - 1] + i] = 0.25f * ((r[0] + r[1]) + (r[2] + r[3]));
     }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) X[0] = B[0];  // 1x1 level: A = 1, the sweeps converge in one step
-  __syncthreads();
-  for (int l = 1; l <= L; ++l) {
+    sync();
+  };
+  // Up leg of level l: the injected coarse correction read on the fly by the first sweep, the
+  // second sweep into X.
+  auto up_level = [&](int l, int tid, int nthr, auto sync) {
     const int m = 1 << l, mm = m * m, mc = m / 2;
     float hs, idg;
     consts(l, hs, idg);
     float *x = X + MGC_OFF[l];
     const float *xc = X + MGC_OFF[l - 1];
     const float *b = B + MGC_OFF[l];
-    // x + injected coarse correction, read on the fly; first sweep into Y
     auto gp = [&](int px, int py) { return x[px + py * m] + xc[(px >> 1) + (py >> 1) * mc]; };
-    for (int i = threadIdx.x; i < mm; i += MGC_THREADS) {
+    for (int i = tid; i < mm; i += nthr) {
       const int px = i % m, py = i / m;
       Y[i] = (b[i] - heat_apply_f(gp, px, py, m, hs)) * idg + gp(px, py);
     }
-    __syncthreads();
-    // second sweep into x
+    sync();
     auto gy = [&](int gx, int gyy) { return Y[gx + gyy * m]; };
-    for (int i = threadIdx.x; i < mm; i += MGC_THREADS)
-      x[i] = (b[i] - heat_apply_f(gy, i % m, i / m, m, hs)) * idg + Y[i];
-    __syncthreads();
+    for (int i = tid; i < mm; i += nthr) x[i] = (b[i] - heat_apply_f(gy, i % m, i / m, m, hs)) * idg + Y[i];
+    sync();
+  };
+  auto block_sync = [] { __syncthreads(); };
+  // Levels of at most 64 nodes run in wave 0 alone: its instructions, LDS accesses included, execute
+  // in order for all lanes, so a wavefront fence (a compiler barrier) replaces the workgroup barrier.
+  auto wave_sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  constexpr int WAVE_LEVELS = 3;  // 8x8 nodes and below
+  for (int l = L; l > WAVE_LEVELS; --l) down_level(l, (int)threadIdx.x, MGC_THREADS, block_sync);
+  if (threadIdx.x < 64) {
+    const int lw = L < WAVE_LEVELS ? L : WAVE_LEVELS;
+    for (int l = lw; l >= 1; --l) down_level(l, (int)threadIdx.x, 64, wave_sync);
+    if (threadIdx.x == 0) X[0] = B[0];  // 1x1 level: A = 1, the sweeps converge in one step
+    wave_sync();
+    for (int l = 1; l <= lw; ++l) up_level(l, (int)threadIdx.x, 64, wave_sync);
   }
+  __syncthreads();
+  for (int l = WAVE_LEVELS + 1; l <= L; ++l) up_level(l, (int)threadIdx.x, MGC_THREADS, block_sync);
   float *xo = xout + field * nn;
   double acc[NORM_TERMS] = {0., 0., 0., 0.};  // <z,z>, <DE,z>, <DE,d_old>, <DE,DE> (DE = the raw input)
   for (int i = threadIdx.x; i < (int)nn; i += MGC_THREADS) {
