@@ -421,6 +421,15 @@ k_mg_up_f32(const float *__restrict__ x2in, const float *__restrict__ xc, const 
   const int x0 = (tile % ntx) * MG_TX, y0 = (tile / ntx) * MG_TY;
   const int bx = x0 - 2, by = y0 - 2;
   const float sg = negate ? -1.f : 1.f;
+  // the previous direction at this thread's output nodes, loaded now so the load overlaps the sweeps
+  constexpr int OUT_PER_THREAD = MG_TX * MG_TY / 256;
+  float dprev[OUT_PER_THREAD];
+#pragma unroll
+  for (int k = 0; k < OUT_PER_THREAD; ++k) {
+    const int i = threadIdx.x + 256 * k;
+    const int gx = x0 + i % MG_TX, gy = y0 + i / MG_TX;
+    dprev[k] = (nrm_part && gx < n && gy < n) ? dold[field * nn + gx + gy * n] : 0.f;
+  }
 
   for (int i = threadIdx.x; i < MG_HX * MG_HY; i += 256) {
     const int lx = i % MG_HX, ly = i / MG_HX;
@@ -442,7 +451,9 @@ k_mg_up_f32(const float *__restrict__ x2in, const float *__restrict__ xc, const 
   auto get3 = [&](int gx, int gy) { return s3[gy - by][gx - bx]; };
   float *xo = xout + field * nn;
   double acc[NORM_TERMS] = {0., 0., 0., 0.};  // <z,z>, <DE,z>, <DE,d_old>, <DE,DE> (DE = the raw input)
-  for (int i = threadIdx.x; i < MG_TX * MG_TY; i += 256) {
+#pragma unroll
+  for (int k = 0; k < OUT_PER_THREAD; ++k) {
+    const int i = threadIdx.x + 256 * k;
     const int tx = i % MG_TX, ty = i / MG_TX;
     const int gx = x0 + tx, gy = y0 + ty;
     if (gx < n && gy < n) {
@@ -452,7 +463,7 @@ k_mg_up_f32(const float *__restrict__ x2in, const float *__restrict__ xc, const 
         const double od = (double)o, de = (double)(sg * sr[ty + 2][tx + 2]);  // the raw input
         acc[0] += od * od;
         acc[1] += de * od;
-        acc[2] += de * (double)dold[field * nn + gx + gy * n];
+        acc[2] += de * (double)dprev[k];
         acc[3] += de * de;
       }
     }

@@ -1624,36 +1624,6 @@ int reduce_impl(const void *defs, int n, int L, int max_iter, double stop_eps, v
 // One background host thread for work that overlaps the caller's: msg_match_series runs its
 // pairwise batches there while the calling thread runs the chain. Never destroyed (a static
 // condition variable with a blocked waiter must not be destroyed at exit).
-// Stream of the pairwise batches that run beside the chain: with MSG_CHAIN_RESERVED_CUS = k > 0 it is
-// limited (CU mask) to all but k compute units spread over the device, so that the chain's
-// one-workgroup kernels always find a free unit; nullptr (the context's own stream) otherwise.
-// Default priority like every stream here. One per device, kept until the process ends.
-hipStream_t pairwise_stream(int dev) {
-  static std::mutex mu;
-  static hipStream_t streams[16] = {};
-  static bool made[16] = {};
-  const char *e = getenv("MSG_CHAIN_RESERVED_CUS");
-  const int reserve = e ? atoi(e) : 0;
-  if (reserve <= 0 || dev < 0 || dev >= 16) return nullptr;
-  std::lock_guard<std::mutex> g(mu);
-  if (made[dev]) return streams[dev];
-  made[dev] = true;
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return nullptr;
-  const int ncu = prop.multiProcessorCount;
-  if (reserve >= ncu) return nullptr;
-  std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
-  for (int i = 0; i < ncu; ++i) mask[i / 32] |= 1u << (i % 32);
-  for (int k = 0; k < reserve; ++k) {
-    const int i = (int)((long long)k * ncu / reserve);
-    mask[i / 32] &= ~(1u << (i % 32));
-  }
-  hipStream_t st = nullptr;
-  if (hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()) != hipSuccess) return nullptr;
-  streams[dev] = st;
-  return st;
-}
-
 struct Worker {
   std::mutex mu;
   std::condition_variable cv;
@@ -1702,8 +1672,7 @@ struct Progress {
 // (matchSeries.h:136-176). Writes the fields to pw and the energies.
 template <typename S>
 int pairwise_batch(const msg_series_params *p, const msg_solve_params &sp, const S *tpl, const S *ref, int i0,
-                   int i1, S *pw, double *energies, const msg_series_levels *lv, const S *init0,
-                   hipStream_t on_stream) {
+                   int i1, S *pw, double *energies, const msg_series_levels *lv, const S *init0) {
   const int L = p->precision_level;
   const int N = 1 << L;
   const size_t nn = (size_t)N * N;
@@ -1713,14 +1682,6 @@ int pairwise_batch(const msg_series_params *p, const msg_solve_params &sp, const
   CHK(ctx_acquire(L, nb, dt, &pc));
   CtxGuard gpc{pc};
   CHK(msg_ctx_set_data_term(pc, p->data_term));
-  // the batch may run on a given stream (the context is ours until it is released; the batch ends
-  // with that stream synchronised)
-  struct StreamSwap {
-    msg_ctx *c;
-    hipStream_t saved;
-    ~StreamSwap() { c->stream = saved; }
-  } swap{pc, pc->stream};
-  if (on_stream) pc->stream = on_stream;
   hipStream_t s = pc->stream;
   for (int i = i0; i < i1; ++i) {
     const S *r = (i == 0 || p->dont_accumulate) ? ref : tpl + (size_t)(i - 1) * nn;
@@ -1760,15 +1721,14 @@ int pairwise_batch(const msg_series_params *p, const msg_solve_params &sp, const
 // Runs the batches [k*b, min((k+1)*b, npw)) in order, publishing progress after each.
 template <typename S>
 void pairwise_all(const msg_series_params *p, const msg_solve_params &sp, const S *tpl, const S *ref, int npw,
-                  int b, S *pw, double *energies, const msg_series_levels *lv, const S *init0, Progress *pr,
-                  hipStream_t on_stream) {
+                  int b, S *pw, double *energies, const msg_series_levels *lv, const S *init0, Progress *pr) {
   int rc = MSG_OK;
   for (int i0 = 0; i0 < npw && rc == MSG_OK; i0 += b) {
     {
       std::lock_guard<std::mutex> g(pr->mu);
       if (pr->abort) break;
     }
-    rc = pairwise_batch<S>(p, sp, tpl, ref, i0, std::min(i0 + b, npw), pw, energies, lv, init0, on_stream);
+    rc = pairwise_batch<S>(p, sp, tpl, ref, i0, std::min(i0 + b, npw), pw, energies, lv, init0);
     std::lock_guard<std::mutex> g(pr->mu);
     if (rc == MSG_OK) pr->done = std::min(i0 + b, npw);
     else { pr->rc = rc; pr->err = g_err; }
@@ -1835,10 +1795,10 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
         pr.cv.notify_all();
         return;
       }
-      pairwise_all<S>(p, sp, tpl, ref, npw, b, pwp, energies, lv, init0, &pr, pairwise_stream(dev));
+      pairwise_all<S>(p, sp, tpl, ref, npw, b, pwp, energies, lv, init0, &pr);
     });
   } else {
-    pairwise_all<S>(p, sp, tpl, ref, npw, b, pwp, energies, lv, init0, &pr, nullptr);
+    pairwise_all<S>(p, sp, tpl, ref, npw, b, pwp, energies, lv, init0, &pr);
     if (pr.rc != MSG_OK) return fail(pr.rc, pr.err);
   }
   // Waits (on the calling thread) until pairwise solve i is done; false after a batch failed.
