@@ -11,6 +11,7 @@
 #include <fstream>
 #include <sstream>
 
+#include "image_formats.h"
 #include "params.h"
 
 namespace mshost {
@@ -85,6 +86,18 @@ std::string token(const std::string &b, size_t &pos) {
 }  // namespace
 
 Image loadImage(const std::string &fileName) {
+  // the reference's dispatch on the (case-sensitive) suffix, quoc/scalarArray.cpp:547-569
+  auto ends = [&](const char *s) {
+    const size_t n = strlen(s);
+    return fileName.size() >= n && fileName.compare(fileName.size() - n, n, s) == 0;
+  };
+  if (ends(".png")) return decodePNG(readFile(fileName), fileName);
+  if (ends(".dm3") || ends(".dm4")) return decodeDM(readFile(fileName), fileName);
+  if (ends(".tif") || ends(".tiff")) return decodeTIFF(readFile(fileName), fileName);
+  if (ends(".mhd")) return loadMetaImage(fileName);
+  if (ends(".nc"))
+    MSH_THROW(strprintf("%s: reading NetCDF files needs the netCDF library, which this build does not link",
+                        fileName.c_str()));
   std::string b = readFile(fileName);
   if (hasBzipSuffix(fileName)) b = bunzip(b, fileName);
   size_t pos = 0;

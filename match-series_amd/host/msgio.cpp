@@ -1,6 +1,6 @@
 // msgio — array file utility over the reference's formats (cf. the reference's image converters):
-//   msgio convert <in> <out>     read any supported array (P2/P5/P7/P8/P9/Pa/Pb, optionally .bz2/.q2bz)
-//                                and write it as P9 (float64), bzip2-compressed when <out> ends in .bz2/.q2bz
+//   msgio convert <in> <out>     read any supported array (P2/P5/P7/P8/P9/Pa/Pb, optionally .bz2/.q2bz;
+//                                .png, .tif/.tiff, .dm3/.dm4, .mhd by suffix) and write it as P9 (float64), bzip2-compressed when <out> ends in .bz2/.q2bz
 //   msgio png <in> <out.png>     8-bit grey preview scaled from the value range (the reference's
 //                                average/median/numSamples .png)
 //   msgio info <in>              print "W H min max mean"
@@ -35,6 +35,9 @@ int main(int argc, char **argv) {
     return 2;
   } catch (mshost::Exception &e) {
     e.dump(std::cerr);
+    return 1;
+  } catch (std::exception &e) {
+    std::cerr << "error: " << e.what() << "\n";
     return 1;
   }
 }

@@ -35,7 +35,7 @@ def build_lib():
 def build_cli():
     """bin/matchSeries (the drop-in CLI) and bin/msgio (array file utility), sharing params/array_io."""
     host = os.path.join(PKG, "host")
-    common = [os.path.join(host, f) for f in ("params.cpp", "array_io.cpp")]
+    common = [os.path.join(host, f) for f in ("params.cpp", "array_io.cpp", "image_formats.cpp")]
     os.makedirs(os.path.join(PKG, "bin"), exist_ok=True)
     libdir = os.path.join(PKG, "lib")
     inc = ["-I", os.path.join(ROOT, "include"), "-I", host, "-I", "/opt/rocm/include",

@@ -16,6 +16,7 @@ EXE = os.path.join(ROOT, "match-series_amd", "bin", "matchSeries")
 GOLD = os.path.join(ROOT, "tests", "golden")
 sys.path.insert(0, GOLD)
 from pngread import read_png_gray8  # noqa: E402
+from make_formats import make_tiff  # noqa: E402
 
 
 def read_p9(path):
@@ -43,11 +44,17 @@ def write_field(mask, phi):
         write_p9(mask % c, phi[c])
 
 
-def run_cli(tmp_path, g, precision, skip_stage1=False):
+def run_cli(tmp_path, g, precision, skip_stage1=False, frame_format="p9"):
     (tmp_path / "in").mkdir(exist_ok=True)
-    for i, f in enumerate(g["frames"]):
-        write_p9(str(tmp_path / "in" / f"frame_{i:02d}.dat.bz2"), f)
     par = bytes(g["par"]).decode() + f"gpuFieldPrecision {precision}\n"
+    for i, f in enumerate(g["frames"]):
+        if frame_format == "p9":
+            write_p9(str(tmp_path / "in" / f"frame_{i:02d}.dat.bz2"), f)
+        else:  # 64-bit float TIFF, LZW with the floating-point predictor: the same values
+            (tmp_path / "in" / f"frame_{i:02d}.tif").write_bytes(make_tiff(f, "f8", compression=5, predictor=3,
+                                                                          rows_per_strip=16))
+    if frame_format != "p9":
+        par = par.replace("in/frame_%02d.dat.bz2", "in/frame_%02d.tif")
     if skip_stage1:
         assert "skipStage1 0" in par
         par = par.replace("skipStage1 0", "skipStage1 1")
@@ -125,6 +132,17 @@ def test_matchseries_stage1(tmp_path, fixture, precision, tol):
     assert np.isfinite(d).all() and np.abs(d).max() * 63 < 20
     assert (out / "log.txt").exists() and (out / "stage1" / "parameter-dump.txt").exists()
     assert (out / "stage1" / "defNorms.txt").exists()
+
+
+def test_matchseries_reads_tiff_frames(tmp_path):
+    """The frames as TIFF files (the loader picks the format by suffix, as the reference's does):
+    stage 1 matches the reference's run on the same values."""
+    if not os.path.exists(EXE):
+        pytest.skip("bin/matchSeries not built")
+    with np.load(os.path.join(GOLD, "e2e64.npz")) as z:
+        g = {k: z[k] for k in z.files}
+    out = run_cli(tmp_path, g, 64, frame_format="tif")
+    check_stage(out, g, 1, 64, 1e-6, 1e-6)
 
 
 @pytest.mark.parametrize("fixture,precision,tol", [("e2e64", 64, 1e-4), ("e2e64", 32, 1.0), ("e2e64_ncc", 64, 1e-4),
