@@ -165,6 +165,15 @@ int msg_series_stats(int dtype, const void *stack, int nimg, int n, double fill,
  * (aol/vec.h:864-870, 1084-1089). In place on nimg device images. */
 int msg_normalize(int dtype, void *img, int n, int nimg, double pct, int min_to_zero, void *stream);
 
+/* PhaseCorrelationRegistration::registerImages (quoc/paramReg.h:2057-2096) over
+ * Convolution::phaseCorrelation (quoc/convolution.h:1092-1114): the integer translation b (pixels,
+ * shift[0] along x, shift[1] along y, each in (-n/2, n/2]) with reference(. + b) ~ templ, taken at
+ * the peak of the phase correlation (the first maximum in x-fastest order); max_shift > 0 searches
+ * |b|_inf <= max_shift (< n) on the plain cross-correlation instead. Device images of n x n,
+ * n <= 4096; corr: NULL or n*n device doubles receiving the correlation surface. */
+int msg_register_translation(int dtype, const void *reference, const void *templ, int n, int max_shift, int *shift,
+                             double *corr, void *stream);
+
 /* ------------------------------------------------------------------------------------------------
  * Series matching: one stage of SeriesMatching::matchAndAverageSeries (matchSeries.h:87-269,
  * 353-451, 565-570) on device-resident frames.
@@ -193,6 +202,10 @@ typedef struct {
   double reduce_stop_eps;             /* frames is found and applied first when averaging     */
   const void *pair0_init;             /* NULL or a device field: template 0's starting        */
                                       /* deformation (stage > 1 with reduceDeformations)      */
+  int corr_init;                      /* "useCorrelationToInitTranslation": every pairwise    */
+  int max_corr_shift;                 /* solve starts from the translation found by phase     */
+                                      /* correlation of its two frames ("maxCorrShift" > 0:   */
+                                      /* the plain correlation peak within that many pixels)  */
 } msg_series_params;
 
 /* Optional per-level records of msg_match_series: what the reference writes after every level of

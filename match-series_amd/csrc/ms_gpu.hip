@@ -20,6 +20,7 @@
 #include "ms_fast.h"
 #include "ms_compose.h"
 #include "ms_stats.h"
+#include "ms_fft.h"
 
 using msg::PairState;
 
@@ -1667,6 +1668,113 @@ struct Progress {
   std::string err;
 };
 
+// Device buffers of the phase correlation of n x n frames, allocated on first use and kept for the
+// following pairs of the same size.
+struct TranslationWork {
+  int n = 0;
+  DevBuf w, x, y, z, corr, part;
+  int ensure(int nn_side) {
+    if (n == nn_side) return MSG_OK;
+    const size_t nn = (size_t)nn_side * nn_side;
+    for (DevBuf *b : {&w, &x, &y, &z, &corr, &part})
+      if (b->p) {
+        HIPCHK(hipFree(b->p));
+        b->p = nullptr;
+      }
+    n = 0;
+    HIPCHK(hipMalloc(&w.p, sizeof(double2) * (nn_side / 2)));
+    HIPCHK(hipMalloc(&x.p, sizeof(double2) * nn));
+    HIPCHK(hipMalloc(&y.p, sizeof(double2) * nn));
+    HIPCHK(hipMalloc(&z.p, sizeof(double2) * nn));
+    HIPCHK(hipMalloc(&corr.p, sizeof(double) * nn));
+    HIPCHK(hipMalloc(&part.p, (sizeof(double) + sizeof(long long)) * 1024));
+    n = nn_side;
+    return MSG_OK;
+  }
+};
+
+// PhaseCorrelationRegistration::registerImages(Reference, Template, MaxInfAbsShift)
+// (quoc/paramReg.h:2057-2096): FFT2 of both frames, the (normalised) cross power spectrum
+// FFT(reference) * conj(FFT(template)), inverse FFT scaled by 1/n^2 (convolution.h:1049-1114), then
+// the first maximum, or with max_shift > 0 the strict maximum over the window in x-fastest order
+// from (0, 0). The shift is returned signed, (-n/2, n/2].
+template <typename S>
+int translation_impl(const S *refimg, const S *tplimg, int n, int max_shift, int *shift, double *corr_out,
+                     TranslationWork &wk, hipStream_t s) {
+  const int logn = log2i(n);
+  if (logn > msg::FFT_MAXLOG) return fail(MSG_ERR_ARG, "phase correlation supports frames of at most 4096 x 4096");
+  if (max_shift >= n) return fail(MSG_ERR_ARG, "maxCorrShift must be smaller than the frame size");
+  CHK(wk.ensure(n));
+  const size_t nn = (size_t)n * n;
+  double2 *W = (double2 *)wk.w.p, *X = (double2 *)wk.x.p, *Y = (double2 *)wk.y.p, *Z = (double2 *)wk.z.p;
+  double *corr = corr_out ? corr_out : (double *)wk.corr.p;
+  hipLaunchKernelGGL(msg::k_fft_twiddles, dim3(cdiv(n / 2, 256)), dim3(256), 0, s, W, n);
+  const int thr = std::min(1024, std::max(64, n / 2));
+  const size_t lds = sizeof(double2) * n;
+  const dim3 tg(cdiv(n, 32), cdiv(n, 32)), tb(32, 8);
+  const unsigned eg = (unsigned)std::min<size_t>((nn + 255) / 256, 8192);
+  // rows, transpose, rows: Y = FFT2(reference)^T, Z = FFT2(template)^T
+  hipLaunchKernelGGL((msg::k_fft_rows<S>), dim3(n), dim3(thr), lds, s, refimg, X, W, n, logn, 0);
+  hipLaunchKernelGGL((msg::k_fft_rows<S>), dim3(n), dim3(thr), lds, s, tplimg, Y, W, n, logn, 0);
+  hipLaunchKernelGGL(msg::k_transpose_c, tg, tb, 0, s, X, Z, n);
+  hipLaunchKernelGGL(msg::k_transpose_c, tg, tb, 0, s, Y, X, n);
+  hipLaunchKernelGGL((msg::k_fft_rows<double2>), dim3(n), dim3(thr), lds, s, Z, Y, W, n, logn, 0);
+  hipLaunchKernelGGL((msg::k_fft_rows<double2>), dim3(n), dim3(thr), lds, s, X, Z, W, n, logn, 0);
+  hipLaunchKernelGGL(msg::k_cross_power, dim3(eg), dim3(256), 0, s, Y, Z, nn, max_shift > 0 ? 0 : 1);
+  // inverse: rows, transpose, rows, real part / n^2
+  hipLaunchKernelGGL((msg::k_fft_rows<double2>), dim3(n), dim3(thr), lds, s, Y, X, W, n, logn, 1);
+  hipLaunchKernelGGL(msg::k_transpose_c, tg, tb, 0, s, X, Z, n);
+  hipLaunchKernelGGL((msg::k_fft_rows<double2>), dim3(n), dim3(thr), lds, s, Z, Y, W, n, logn, 1);
+  hipLaunchKernelGGL(msg::k_real_part, dim3(eg), dim3(256), 0, s, Y, corr, nn, 1.0 / (double)nn);
+  HIPCHK(hipGetLastError());
+  int sx = 0, sy = 0;
+  if (max_shift <= 0) {
+    const int nb = (int)std::min<size_t>((nn + 255) / 256, 1024);
+    double *bv = (double *)wk.part.p;
+    long long *bi = (long long *)(bv + 1024);
+    hipLaunchKernelGGL(msg::k_argmax_part, dim3(nb), dim3(256), 0, s, corr, nn, bv, bi);
+    HIPCHK(hipGetLastError());
+    std::vector<double> hv(nb);
+    std::vector<long long> hi(nb);
+    HIPCHK(hipMemcpyAsync(hv.data(), bv, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hi.data(), bi, sizeof(long long) * nb, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    double best = 0;
+    long long at = -1;
+    for (int b = 0; b < nb; ++b)
+      if (hi[b] >= 0 && (at < 0 || hv[b] > best || (hv[b] == best && hi[b] < at))) {
+        best = hv[b];
+        at = hi[b];
+      }
+    if (at < 0) at = 0;
+    sx = (int)(at % n);
+    sy = (int)(at / n);
+  } else {
+    const int side = 2 * max_shift + 1;
+    const size_t cnt = (size_t)side * side;
+    double *win = (double *)wk.x.p;  // free by now; holds 2 n^2 >= (2 max_shift + 1)^2 doubles
+    hipLaunchKernelGGL(msg::k_gather_window, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, corr, n, max_shift,
+                       win);
+    HIPCHK(hipGetLastError());
+    std::vector<double> hw(cnt);
+    HIPCHK(hipMemcpyAsync(hw.data(), win, sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    double best = hw[(size_t)max_shift * side + max_shift];  // the value at shift (0, 0)
+    for (int dy = -max_shift; dy <= max_shift; ++dy)
+      for (int dx = -max_shift; dx <= max_shift; ++dx) {
+        const double v = hw[(size_t)(dy + max_shift) * side + (dx + max_shift)];
+        if (v > best) {
+          best = v;
+          sx = (dx + n) % n;
+          sy = (dy + n) % n;
+        }
+      }
+  }
+  shift[0] = sx <= n / 2 ? sx : sx - n;
+  shift[1] = sy <= n / 2 ? sy : sy - n;
+  return MSG_OK;
+}
+
 // Pairwise solves i0..i1-1 in one batched context: template i against its predecessor (the
 // reference for i = 0 or with dont_accumulate), SeriesMatching::matchAndAverageSeries
 // (matchSeries.h:136-176). Writes the fields to pw and the energies.
@@ -1707,6 +1815,23 @@ int pairwise_batch(const msg_series_params *p, const msg_solve_params &sp, const
   if (init0 && i0 == 0) {  // template 0 starts from the given deformation (matchSeries.h:123-135)
     HIPCHK(hipMemcpyAsync(pc->phi[L], init0, sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, s));
     CHK(rebuild_pyramid<S>(pc, pc->phi, 0, 2));
+  }
+  if (p->corr_init) {
+    // "useCorrelationToInitTranslation": the translation found by phase correlation of the
+    // template (as registerImages' reference) against the reference, in grid units, on every
+    // level (setTransformationToTranslation, registration.h:578-582; matchSeries.h:145-153)
+    TranslationWork wk;
+    const double h = 1.0 / (double)(N - 1);
+    for (int i = i0; i < i1; ++i) {
+      const int k = i - i0;
+      int sh[2];
+      CHK(translation_impl<S>((const S *)pc->T[L] + k * nn, (const S *)pc->R[L] + k * nn, N, p->max_corr_shift, sh,
+                              nullptr, wk, s));
+      hipLaunchKernelGGL((msg::k_fill_translation<S>), dim3((unsigned)std::min<size_t>((nn + 255) / 256, 4096)),
+                         dim3(256), 0, s, (S *)pc->phi[L] + k * 2 * nn, nn, (double)sh[0] * h, (double)sh[1] * h);
+      HIPCHK(hipGetLastError());
+      CHK(rebuild_pyramid<S>(pc, pc->phi, k, 2));
+    }
   }
   LevelOut lo;
   lo.lv = lv;
@@ -1939,6 +2064,22 @@ int msg_series_stats(int dtype, const void *stack, int nimg, int n, double fill,
     return fail(MSG_ERR_ARG, "invalid argument (at most 128 images)");
   if (dtype == MSG_F64) CHK(stats_impl<double>(stack, nimg, n, fill, mean, count, median, strm(stream)));
   else CHK(stats_impl<float>(stack, nimg, n, fill, mean, count, median, strm(stream)));
+  HIPCHK(hipStreamSynchronize(strm(stream)));
+  return MSG_OK;
+}
+
+int msg_register_translation(int dtype, const void *reference, const void *templ, int n, int max_shift, int *shift,
+                             double *corr, void *stream) {
+  CHK(check_dtype(dtype));
+  CHK(check_n(n));
+  if (!reference || !templ || !shift) return fail(MSG_ERR_ARG, "invalid argument");
+  TranslationWork wk;
+  if (dtype == MSG_F64)
+    CHK(translation_impl<double>((const double *)reference, (const double *)templ, n, max_shift, shift, corr, wk,
+                                 strm(stream)));
+  else
+    CHK(translation_impl<float>((const float *)reference, (const float *)templ, n, max_shift, shift, corr, wk,
+                                strm(stream)));
   HIPCHK(hipStreamSynchronize(strm(stream)));
   return MSG_OK;
 }

@@ -84,7 +84,7 @@ std::string dirOf(const ParameterParser &p) { return p.getStringExpandTilde("sav
 
 // Options of the reference that change the computation but are not implemented here.
 void rejectUnsupported(const ParameterParser &p) {
-  const char *flags[] = {"useCorrelationToInitTranslation", "reverseRolesInSeriesMatching",
+  const char *flags[] = {"reverseRolesInSeriesMatching",
                          "calcInverseDeformation", "cropInput", "resizeInput", "calcIQ", "calcPrecision",
                          "performNoiseAnalysis", "saveNamedDeformedTemplates", "saveNamedDeformations"};
   for (const char *f : flags)
@@ -127,6 +127,9 @@ msg_series_params seriesParams(const ParameterParser &p) {
   s.reduce_max_iter = p.getIntOrDefault("maxGDIterations", 1000);  // SeriesMatching::reduceDeformations
   s.reduce_stop_eps = p.getDoubleOrDefault("stopEpsilon", 5e-7);   // (matchSeries.h:697)
   s.pair0_init = nullptr;
+  // every new pairing starts from the phase-correlation translation of its frames (matchSeries.h:145-153)
+  s.corr_init = p.checkAndGetBool("useCorrelationToInitTranslation");
+  s.max_corr_shift = s.corr_init ? p.getInt("maxCorrShift") : 0;
   return s;
 }
 
@@ -414,6 +417,8 @@ void runMatchSeries(ParameterParser &parser, int argc, char **argv) {
     }
     // the reference passes lambda through "%f"
     stage2.changeVariableValue("lambda", strprintf("%f", parser.getDouble("lambda") * extraStagesLambdaFactor));
+    if (parser.checkAndGetBool("useCorrelationToInitTranslation"))
+      stage2.changeVariableValue("useCorrelationToInitTranslation", 0);
     for (int i = 0; i < numExtraStages; ++i) {
       ParameterParser stageN(stage2);
       stageN.changeVariableValue("saveDirectory", strprintf("%sstage%d/", saveDir.c_str(), i + 2));

@@ -43,7 +43,7 @@ class SeriesParams(ctypes.Structure):
         ("pairwise_batch", ctypes.c_int), ("data_term", ctypes.c_int),
         ("average_ext_mean", ctypes.c_int), ("average_nearest", ctypes.c_int),
         ("reduce_deformations", ctypes.c_int), ("reduce_max_iter", ctypes.c_int), ("reduce_stop_eps", ctypes.c_double),
-        ("pair0_init", ctypes.c_void_p),
+        ("pair0_init", ctypes.c_void_p), ("corr_init", ctypes.c_int), ("max_corr_shift", ctypes.c_int),
     ]
 
 
@@ -97,6 +97,7 @@ SIGNATURES = {
     "msg_time_energy_pass": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp], ctypes.c_int),
     "msg_match_series": ([ctypes.POINTER(SeriesParams), ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _dp, _vp,
                           _vp, _vp, ctypes.POINTER(SeriesLevels), _vp], ctypes.c_int),
+    "msg_register_translation": ([ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _ip, _vp, _vp], ctypes.c_int),
     "msg_reduce_deformations": ([ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _dp,
                                  ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
 }
@@ -263,6 +264,19 @@ def normalize(img, pct=-1.0, min_to_zero=True):
     k = img.numel() // (n * n)
     check(lib().msg_normalize(dtype_code(img), _ptr(img), n, k, pct, int(min_to_zero), _stream()))
     return img
+
+
+def register_translation(reference, templ, max_shift=0, want_corr=False):
+    """Integer translation (bx, by) in pixels with reference(. + b) ~ templ, by phase correlation
+    (max_shift > 0: plain correlation within |b|_inf <= max_shift); with want_corr also the
+    correlation surface as an (n, n) float64 device tensor."""
+    n = reference.shape[-1]
+    sh = (ctypes.c_int * 2)()
+    corr = _torch().empty((n, n), dtype=_torch().float64, device=reference.device) if want_corr else None
+    check(lib().msg_register_translation(dtype_code(reference), _ptr(reference), _ptr(templ), n, int(max_shift), sh,
+                                         _ptr(corr) if want_corr else None, _stream()))
+    b = (int(sh[0]), int(sh[1]))
+    return (b, corr) if want_corr else b
 
 
 # ------------------------------------------------------------------------------------------------

@@ -1162,6 +1162,88 @@ static double solveML(const Pyramid *R, const Pyramid *T, DefPyramid *d, int sta
   return energy;
 }
 
+/* In-place radix-2 transform of n = 2^k complex values (re, im interleaved), unnormalised,
+ * exp(-2 pi i jk/n) forward and exp(+2 pi i jk/n) inverse; every twiddle from cos/sin directly. */
+static void fft1(double *a, int n, int inverse) {
+  for (int i = 1, j = 0; i < n; ++i) {
+    int bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) {
+      double t = a[2 * i]; a[2 * i] = a[2 * j]; a[2 * j] = t;
+      t = a[2 * i + 1]; a[2 * i + 1] = a[2 * j + 1]; a[2 * j + 1] = t;
+    }
+  }
+  for (int len = 2; len <= n; len <<= 1) {
+    for (int k = 0; k < len / 2; ++k) {
+      const double ang = 2.0 * 3.14159265358979323846 * k / len, wr = cos(ang), wi = inverse ? sin(ang) : -sin(ang);
+      for (int i = 0; i < n; i += len) {
+        double *u = a + 2 * (i + k), *v = a + 2 * (i + k + len / 2);
+        const double vr = v[0] * wr - v[1] * wi, vi = v[0] * wi + v[1] * wr;
+        v[0] = u[0] - vr; v[1] = u[1] - vi;
+        u[0] += vr; u[1] += vi;
+      }
+    }
+  }
+}
+
+/* 2-D transform of an N x N complex array (x fastest): rows, then columns */
+static void fft2(double *a, int N, int inverse) {
+  double *col = (double *)malloc(sizeof(double) * 2 * N);
+  for (int y = 0; y < N; ++y) fft1(a + (size_t)2 * y * N, N, inverse);
+  for (int x = 0; x < N; ++x) {
+    for (int y = 0; y < N; ++y) { col[2 * y] = a[2 * ((size_t)y * N + x)]; col[2 * y + 1] = a[2 * ((size_t)y * N + x) + 1]; }
+    fft1(col, N, inverse);
+    for (int y = 0; y < N; ++y) { a[2 * ((size_t)y * N + x)] = col[2 * y]; a[2 * ((size_t)y * N + x) + 1] = col[2 * y + 1]; }
+  }
+  free(col);
+}
+
+void msor_phase_correlation(const double *a, const double *b, int N, int normalise, double *out) {
+  const size_t NN = (size_t)N * N;
+  double *fa = (double *)calloc(2 * NN, sizeof(double)), *fb = (double *)calloc(2 * NN, sizeof(double));
+  for (size_t i = 0; i < NN; ++i) { fa[2 * i] = a[i]; fb[2 * i] = b[i]; }
+  fft2(fa, N, 0);
+  fft2(fb, N, 0);
+  for (size_t i = 0; i < NN; ++i) {
+    /* FFT(a) * conj(FFT(b)), convolution.h:1095-1102 */
+    const double tx = fa[2 * i] * fb[2 * i] + fa[2 * i + 1] * fb[2 * i + 1];
+    const double ty = fa[2 * i + 1] * fb[2 * i] - fa[2 * i] * fb[2 * i + 1];
+    const double nrm = normalise ? sqrt(tx * tx + ty * ty) : 1.0;
+    const double scale = fabs(nrm) < 1e-15 ? 0.0 : 1.0 / nrm;
+    fa[2 * i] = tx * scale;
+    fa[2 * i + 1] = ty * scale;
+  }
+  fft2(fa, N, 1);
+  for (size_t i = 0; i < NN; ++i) out[i] = fa[2 * i] * (1.0 / (double)NN);
+  free(fa);
+  free(fb);
+}
+
+void msor_register_translation(const double *reference, const double *templ, int N, int max_shift, int *shift) {
+  const size_t NN = (size_t)N * N;
+  double *pc = (double *)malloc(sizeof(double) * NN);
+  msor_phase_correlation(reference, templ, N, max_shift <= 0, pc);
+  int sx = 0, sy = 0;
+  if (max_shift <= 0) { /* getMaxIndexAndValue: the first maximum */
+    size_t at = 0;
+    for (size_t i = 1; i < NN; ++i)
+      if (pc[i] > pc[at]) at = i;
+    sx = (int)(at % N);
+    sy = (int)(at / N);
+  } else { /* window |shift|_inf <= max_shift from (0, 0), x fastest, strict improvement */
+    double best = pc[0];
+    for (int dy = -max_shift; dy <= max_shift; ++dy)
+      for (int dx = -max_shift; dx <= max_shift; ++dx) {
+        const int px = (dx + N) % N, py = (dy + N) % N;
+        if (pc[(size_t)py * N + px] > best) { best = pc[(size_t)py * N + px]; sx = px; sy = py; }
+      }
+  }
+  shift[0] = sx <= N / 2 ? sx : sx - N;
+  shift[1] = sy <= N / 2 ? sy : sy - N;
+  free(pc);
+}
+
 static void prepared(const double *raw, int NN, const msor_series_params *p, double *out) {
   memcpy(out, raw, sizeof(double) * NN);
   if (!p->dont_normalize) msor_normalize(out, NN, p->saturation_pct, p->normalize_min_to_zero);
@@ -1191,6 +1273,15 @@ int msor_match_series(const double *templates, int n, const double *reference, i
     if (i == 0 && p->pair0_init) defSet(&def, p->pair0_init); /* reduceDeformations, stage > 1 (L123-135) */
     prepared(templates + i * NN, (int)NN, p, img);
     pyrBuild(&tpl, img, L);
+    if (p->corr_init && (pairwise_in == NULL || i == 0)) {
+      /* useCorrelationToInitTranslation (matchSeries.h:145-153): registerImages(template,
+       * reference) in pixels, times h, on every level (setTransformationToTranslation) */
+      int sh[2];
+      msor_register_translation(tpl.lev[L], ref.lev[L], N, p->max_corr_shift, sh);
+      const double h = 1.0 / (double)(N - 1);
+      for (size_t k = 0; k < NN; ++k) { tmp[k] = sh[0] * h; tmp[NN + k] = sh[1] * h; }
+      defSet(&def, tmp);
+    }
     if (pairwise_in == NULL || i == 0) {
       energies[i] = solveML(&ref, &tpl, &def, p->start_level, p->stop_level, p);
     } else {

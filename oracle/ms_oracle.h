@@ -106,7 +106,16 @@ typedef struct {
   int reduce_max_iter;
   double reduce_stop_eps;
   const double *pair0_init;           /* NULL or template 0's starting deformation            */
+  int corr_init;                      /* useCorrelationToInitTranslation                      */
+  int max_corr_shift;                 /* maxCorrShift                                         */
 } msor_series_params;
+
+/* Convolution::phaseCorrelation (quoc/convolution.h:1049-1114): the inverse FFT of
+ * FFT(a) * conj(FFT(b)), scaled to unit modulus per frequency when `normalise`, times 1/N^2. */
+void msor_phase_correlation(const double *a, const double *b, int N, int normalise, double *out);
+/* PhaseCorrelationRegistration::registerImages (quoc/paramReg.h:2057-2096): signed integer shift,
+ * x then y. */
+void msor_register_translation(const double *reference, const double *templ, int N, int max_shift, int *shift);
 
 int msor_match_series(const double *templates, int n, const double *reference, int N,
                       const msor_series_params *p, const double *pairwise_in,

@@ -35,7 +35,7 @@ class SeriesParams(ctypes.Structure):
         ("dont_normalize", ctypes.c_int), ("resample_prolong", ctypes.c_int),
         ("dont_accumulate", ctypes.c_int), ("data_term", ctypes.c_int),
         ("reduce_deformations", ctypes.c_int), ("reduce_max_iter", ctypes.c_int), ("reduce_stop_eps", ctypes.c_double),
-        ("pair0_init", ctypes.POINTER(ctypes.c_double)),
+        ("pair0_init", ctypes.POINTER(ctypes.c_double)), ("corr_init", ctypes.c_int), ("max_corr_shift", ctypes.c_int),
     ]
 
 
@@ -236,6 +236,24 @@ def reduce_deformations(psis, max_iter=1000, stop_eps=5e-7):
     e = ctypes.c_double(0)
     it = lib().msor_reduce_deformations(_p(psis), n, N, max_iter, ctypes.c_double(stop_eps), _p(phi), ctypes.byref(e))
     return phi, e.value, it
+
+
+def phase_correlation(a, b, normalise=True):
+    """Convolution::phaseCorrelation (quoc/convolution.h:1049-1114): real inverse FFT of
+    FFT(a) * conj(FFT(b)) (unit modulus per frequency when normalise) / N^2, shape (N, N)."""
+    a, b = _f64(a), _f64(b)
+    out = np.zeros_like(a)
+    lib().msor_phase_correlation(_p(a), _p(b), a.shape[-1], int(bool(normalise)), _p(out))
+    return out
+
+
+def register_translation(reference, templ, max_shift=0):
+    """PhaseCorrelationRegistration::registerImages (quoc/paramReg.h:2057-2096): (bx, by) in
+    pixels with reference(. + b) ~ templ."""
+    a, b = _f64(reference), _f64(templ)
+    sh = (ctypes.c_int * 2)()
+    lib().msor_register_translation(_p(a), _p(b), a.shape[-1], int(max_shift), sh)
+    return int(sh[0]), int(sh[1])
 
 
 def match_series(templates, reference, params, pairwise_in=None, pair0_init=None):

@@ -145,6 +145,37 @@ def test_matchseries_reads_tiff_frames(tmp_path):
     check_stage(out, g, 1, 64, 1e-6, 1e-6)
 
 
+@pytest.mark.parametrize("max_shift", [0, 6])
+def test_matchseries_translation_init(tmp_path, max_shift):
+    """useCorrelationToInitTranslation on frames drifting by (-5, 4) px per frame: stage 1 against
+    the oracle's series loop with the same option (the reference needs FFTW for it, which the
+    reference build here lacks)."""
+    if not os.path.exists(EXE):
+        pytest.skip("bin/matchSeries not built")
+    import oracle as O
+    from msamd import synth
+    with np.load(os.path.join(GOLD, "e2e64.npz")) as z:
+        par = bytes(z["par"]).decode()
+    N = 64
+    big = synth.stem_frame(N + 64, synth.frame_seed(3, 0), spacing=6.0, sigma=1.5, jitter_step=0.1)
+    frames = np.stack([big[32 + 4 * f:32 + 4 * f + N, 32 - 5 * f:32 - 5 * f + N] for f in range(4)])
+    g = {"frames": frames, "par": np.frombuffer(
+        (par.replace("numExtraStages 1", "numExtraStages 0")
+         + f"useCorrelationToInitTranslation 1\nmaxCorrShift {max_shift}\n").encode(), dtype=np.uint8)}
+    out = run_cli(tmp_path, g, 64)
+    p = O.SeriesParams(start_level=4, stop_level=6, refine_start_level=5, refine_stop_level=6, precision_level=6,
+                       lambda_=10.0, lambda_factor=1.0, max_iter=15, stop_eps=-1.0, saturation_pct=0.15,
+                       normalize_min_to_zero=1, dont_normalize=0, resample_prolong=1, dont_accumulate=0,
+                       data_term=O.SSD, corr_init=1, max_corr_shift=max_shift)
+    ref = O.match_series(frames[1:], frames[0], p)
+    for i in range(3):
+        sub = f"{i}" if i == 0 else f"{i}-r"
+        d = np.stack([read_p9(str(out / "stage1" / sub / f"deformation_06_{c}.dat.bz2")) for c in range(2)])
+        assert np.abs(d - ref["deformations"][i]).max() * (N - 1) < 1e-6, i
+    en = np.loadtxt(str(out / "stage1" / "energies.txt"))[:, 1]
+    np.testing.assert_allclose(en, ref["energies"], rtol=1e-4)
+
+
 @pytest.mark.parametrize("fixture,precision,tol", [("e2e64", 64, 1e-4), ("e2e64", 32, 1.0), ("e2e64_ncc", 64, 1e-4),
                                                    ("e2e64_reduce", 64, 1e-4)])
 def test_matchseries_stage2_from_reference_stage1(tmp_path, fixture, precision, tol):

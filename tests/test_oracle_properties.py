@@ -250,3 +250,39 @@ def test_level_solve_decreases_energy_monotonically():
     e = info["energies"]
     assert np.all(np.diff(e) <= 1e-15)   # Armijo accepts only descent steps
     assert e[-1] < 0.2 * e[0]
+
+
+@pytest.mark.parametrize("N", [8, 16])
+def test_phase_correlation_is_the_circular_cross_correlation(N):
+    # un-normalised: out(s) = sum_x a(x + s) b(x) (circular), by brute force; normalised: the same
+    # formula on unit-modulus spectra, by numpy's independent FFT
+    rng = np.random.default_rng(N)
+    a, b = rng.random((N, N)), rng.random((N, N))
+    brute = np.array([[np.sum(np.roll(a, (-sy, -sx), axis=(0, 1)) * b) for sx in range(N)] for sy in range(N)])
+    np.testing.assert_allclose(O.phase_correlation(a, b, False), brute, rtol=1e-13, atol=1e-12)
+    g = np.fft.fft2(a) * np.conj(np.fft.fft2(b))
+    np.testing.assert_allclose(O.phase_correlation(a, b, True), np.real(np.fft.ifft2(g / np.abs(g))), atol=1e-14)
+    # a pure circular shift gives a unit delta at the shift
+    pc = O.phase_correlation(a, np.roll(a, (-2, 3), axis=(0, 1)), True)
+    assert abs(pc[2, N - 3] - 1.0) < 1e-12 and np.abs(np.delete(pc.ravel(), 2 * N + N - 3)).max() < 1e-12
+
+
+def test_register_translation_conventions():
+    # reference(x + b) = templ(x): b is returned signed in (-N/2, N/2], x first
+    rng = np.random.default_rng(1)
+    N = 32
+    a = rng.random((N, N))
+    for bx, by in ((0, 0), (3, -5), (-7, 2), (N // 2, 1), (1 - N // 2, -N // 2 + 1)):
+        assert O.register_translation(a, np.roll(a, (-by, -bx), axis=(0, 1))) == (bx, by)
+    # a window that holds the shift finds it on the plain correlation; one that does not returns the
+    # best position inside it (brute force over the window, first strict maximum in x-fastest order)
+    b = np.roll(a, (-2, -3), axis=(0, 1))
+    assert O.register_translation(a, b, 4) == (3, 2)
+    m = 1
+    cc = O.phase_correlation(a, b, False)
+    best, at = cc[0, 0], (0, 0)
+    for dy in range(-m, m + 1):
+        for dx in range(-m, m + 1):
+            if cc[dy % N, dx % N] > best:
+                best, at = cc[dy % N, dx % N], (dx, dy)
+    assert O.register_translation(a, b, m) == at
