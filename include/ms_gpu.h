@@ -165,6 +165,13 @@ int msg_series_stats(int dtype, const void *stack, int nimg, int n, double fill,
  * (aol/vec.h:864-870, 1084-1089). In place on nimg device images. */
 int msg_normalize(int dtype, void *img, int n, int nimg, double pct, int min_to_zero, void *stream);
 
+/* qc::approxInverseDeformation (quoc/deformations.h:341-355, TransformFunction<QC_2D>::transform,
+ * quoc/auxiliary.cpp:33-176): the identity pushed forward through the displacement phi (two
+ * triangles per element, the first covering triangle in element order wins), minus the identity;
+ * uncovered nodes take the midpoint of the covered values' range. phi, inv: distinct device fields
+ * (x component, then y) of n x n. */
+int msg_approx_inverse(int dtype, const void *phi, void *inv, int n, void *stream);
+
 /* PhaseCorrelationRegistration::registerImages (quoc/paramReg.h:2057-2096) over
  * Convolution::phaseCorrelation (quoc/convolution.h:1092-1114): the integer translation b (pixels,
  * shift[0] along x, shift[1] along y, each in (-n/2, n/2]) with reference(. + b) ~ templ, taken at
@@ -206,6 +213,11 @@ typedef struct {
   int max_corr_shift;                 /* solve starts from the translation found by phase     */
                                       /* correlation of its two frames ("maxCorrShift" > 0:   */
                                       /* the plain correlation peak within that many pixels)  */
+  int calc_inverse;                   /* "calcInverseDeformation": after frame i's (refined)  */
+                                      /* deformation, a solve over the refine levels with the */
+                                      /* two images' roles swapped, started from its          */
+                                      /* approximate inverse (matchSeries.h:247-259)          */
+  void *inverse_out;                  /* NULL or n device fields: those inverse solves' result*/
 } msg_series_params;
 
 /* Optional per-level records of msg_match_series: what the reference writes after every level of
@@ -214,7 +226,8 @@ typedef struct {
  * registration.h:1092-1112; GradientDescentBase::writeEnergy, aol/gradientDescent.h:254-271).
  * Records are ordered by solve — the pairwise solves i = 0..npw-1 (npw = 1 with pairwise_in,
  * else n) over levels start..stop, then the refinements i = 1..n-1 over
- * refine_start..refine_stop (none with dont_accumulate) — and by ascending level within a solve.
+ * refine_start..refine_stop (none with dont_accumulate), then with calc_inverse the inverse solves
+ * i = 0..n-1 over refine_start..refine_stop — and by ascending level within a solve.
  * Every pointer may be NULL. All host memory. */
 typedef struct {
   double *fields;         /* 2*N_l^2 doubles per record (x component, then y), back to back    */

@@ -21,6 +21,7 @@
 #include "ms_compose.h"
 #include "ms_stats.h"
 #include "ms_fft.h"
+#include "ms_inverse.h"
 
 using msg::PairState;
 
@@ -1775,6 +1776,42 @@ int translation_impl(const S *refimg, const S *tplimg, int n, int max_shift, int
   return MSG_OK;
 }
 
+// qc::approxInverseDeformation (deformations.h:341-355): keys of the first covering triangle, the
+// interpolated identity minus the identity, the covered range's midpoint where nothing covers.
+template <typename S>
+int inverse_impl(const S *phi, S *out, int n, hipStream_t s) {
+  const size_t nn = (size_t)n * n;
+  DevBuf key, part;
+  const int nb = (int)std::min<size_t>((nn + 255) / 256, 1024);
+  HIPCHK(hipMalloc(&key.p, sizeof(unsigned) * nn));
+  HIPCHK(hipMalloc(&part.p, sizeof(double) * 4 * nb));
+  unsigned *kp = (unsigned *)key.p;
+  const double H = 1.0 / (double)(n - 1);
+  const unsigned eg = (unsigned)std::min<size_t>((nn + 255) / 256, 4096);
+  hipLaunchKernelGGL(msg::k_fill_u32, dim3(eg), dim3(256), 0, s, kp, nn, msg::INV_NONE);
+  const long long ne = (long long)(n - 1) * (n - 1);
+  hipLaunchKernelGGL((msg::k_inv_keys<S>), dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, phi, n, H, kp);
+  hipLaunchKernelGGL((msg::k_inv_values<S>), dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, phi, n, H,
+                     (const unsigned *)kp, out);
+  hipLaunchKernelGGL((msg::k_inv_minmax<S>), dim3(nb), dim3(256), 0, s, (const S *)out, (const unsigned *)kp, nn,
+                     (double *)part.p);
+  HIPCHK(hipGetLastError());
+  std::vector<double> hp(4 * (size_t)nb);
+  HIPCHK(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  double mn[2] = {DBL_MAX, DBL_MAX}, mx[2] = {-DBL_MAX, -DBL_MAX};
+  for (int b = 0; b < nb; ++b)
+    for (int c = 0; c < 2; ++c) {
+      mn[c] = std::min(mn[c], hp[4 * b + 2 * c]);
+      mx[c] = std::max(mx[c], hp[4 * b + 2 * c + 1]);
+    }
+  hipLaunchKernelGGL((msg::k_inv_fill<S>), dim3(eg), dim3(256), 0, s, out, (const unsigned *)kp, nn,
+                     0.5 * (mn[0] + mx[0]), 0.5 * (mn[1] + mx[1]));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));  // the key buffer is freed on return
+  return MSG_OK;
+}
+
 // Pairwise solves i0..i1-1 in one batched context: template i against its predecessor (the
 // reference for i = 0 or with dont_accumulate), SeriesMatching::matchAndAverageSeries
 // (matchSeries.h:136-176). Writes the fields to pw and the energies.
@@ -1957,8 +1994,9 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
       DevBuf acc;
       HIPCHK(hipMalloc(&acc.p, sizeof(S) * 2 * nn));
       HIPCHK(hipMemsetAsync(acc.p, 0, sizeof(S) * 2 * nn, rs));
-      DevBuf prev;
+      DevBuf prev, swp;
       HIPCHK(hipMalloc(&prev.p, sizeof(S) * 2 * nn));
+      if (p->calc_inverse) HIPCHK(hipMalloc(&swp.p, sizeof(S) * nn));
       for (int i = 0; i < n; ++i) {
         if (i < npw && !wait_pairwise(i)) return pr.rc;
         HIPCHK(hipMemcpyAsync(prev.p, acc.p, sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, rs));
@@ -1982,6 +2020,44 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
           CHK(solve_impl<S>(rc, p->refine_start_level, p->refine_stop_level, &sp, &e, &lo));
           energies[i] = e;
           HIPCHK(hipMemcpyAsync(acc.p, rc->phi[L], sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, rs));
+        }
+        if (p->calc_inverse) {
+          // calcInverseDeformation (matchSeries.h:247-259): reference and template swapped, the
+          // solve over the refine levels started from the approximate inverse of the accumulated
+          // deformation. The swap copies the pyramids' contents (captured level graphs hold the
+          // buffers' addresses) and is undone afterwards.
+          if (i == 0) {
+            HIPCHK(hipMemcpyAsync(rc->T[L], tpl, nn * sizeof(S), hipMemcpyDeviceToDevice, rs));
+            if (!p->dont_normalize)
+              CHK(normalize_impl<S>(rc->T[L], N, 1, p->saturation_pct, p->normalize_min_to_zero, rs));
+            if (p->pre_smooth_sigma > 0)
+              CHK(h1_impl<S>(rc->T[L], rc->T[L], N, 1, p->pre_smooth_sigma / (double)(N - 1), rs));
+            CHK(rebuild_pyramid<S>(rc, rc->T, -1, 1));
+          }
+          auto swap_images = [&]() -> int {
+            for (int l = 0; l <= L; ++l) {
+              const size_t bytes = sizeof(S) * ((size_t)1 << (2 * l));
+              HIPCHK(hipMemcpyAsync(swp.p, rc->R[l], bytes, hipMemcpyDeviceToDevice, rs));
+              HIPCHK(hipMemcpyAsync(rc->R[l], rc->T[l], bytes, hipMemcpyDeviceToDevice, rs));
+              HIPCHK(hipMemcpyAsync(rc->T[l], swp.p, bytes, hipMemcpyDeviceToDevice, rs));
+            }
+            return MSG_OK;
+          };
+          CHK(swap_images());
+          CHK(inverse_impl<S>((const S *)acc.p, (S *)rc->phi[L], N, rs));
+          CHK(rebuild_pyramid<S>(rc, rc->phi, -1, 2));
+          const long long nlr = p->refine_stop_level - p->refine_start_level + 1;
+          LevelOut lo;
+          lo.lv = lv;
+          lo.rec0 = (long long)npw * (p->stop_level - p->start_level + 1) + (long long)(n - 1 + i) * nlr;
+          lo.field0 = (size_t)npw * level_block(p->start_level, p->stop_level) +
+                      (size_t)(n - 1 + i) * level_block(p->refine_start_level, p->refine_stop_level);
+          double e = 0;
+          CHK(solve_impl<S>(rc, p->refine_start_level, p->refine_stop_level, &sp, &e, &lo));
+          if (p->inverse_out)
+            HIPCHK(hipMemcpyAsync((S *)p->inverse_out + (size_t)i * 2 * nn, rc->phi[L], sizeof(S) * 2 * nn,
+                                  hipMemcpyDeviceToDevice, rs));
+          CHK(swap_images());
         }
         HIPCHK(hipMemcpyAsync(defs + (size_t)i * 2 * nn, (i > 0) ? acc.p : (const void *)pwp, sizeof(S) * 2 * nn,
                               hipMemcpyDeviceToDevice, rs));
@@ -2065,6 +2141,15 @@ int msg_series_stats(int dtype, const void *stack, int nimg, int n, double fill,
   if (dtype == MSG_F64) CHK(stats_impl<double>(stack, nimg, n, fill, mean, count, median, strm(stream)));
   else CHK(stats_impl<float>(stack, nimg, n, fill, mean, count, median, strm(stream)));
   HIPCHK(hipStreamSynchronize(strm(stream)));
+  return MSG_OK;
+}
+
+int msg_approx_inverse(int dtype, const void *phi, void *inv, int n, void *stream) {
+  CHK(check_dtype(dtype));
+  CHK(check_n(n));
+  if (!phi || !inv || phi == inv) return fail(MSG_ERR_ARG, "invalid argument (phi and inv must be distinct)");
+  if (dtype == MSG_F64) CHK(inverse_impl<double>((const double *)phi, (double *)inv, n, strm(stream)));
+  else CHK(inverse_impl<float>((const float *)phi, (float *)inv, n, strm(stream)));
   return MSG_OK;
 }
 

@@ -84,8 +84,7 @@ std::string dirOf(const ParameterParser &p) { return p.getStringExpandTilde("sav
 
 // Options of the reference that change the computation but are not implemented here.
 void rejectUnsupported(const ParameterParser &p) {
-  const char *flags[] = {"reverseRolesInSeriesMatching",
-                         "calcInverseDeformation", "cropInput", "resizeInput", "calcIQ", "calcPrecision",
+  const char *flags[] = {"reverseRolesInSeriesMatching", "cropInput", "resizeInput", "calcIQ", "calcPrecision",
                          "performNoiseAnalysis", "saveNamedDeformedTemplates", "saveNamedDeformations"};
   for (const char *f : flags)
     if (p.checkAndGetBool(f)) MSH_THROW(strprintf("\"%s 1\" is not supported by this build", f));
@@ -130,6 +129,9 @@ msg_series_params seriesParams(const ParameterParser &p) {
   // every new pairing starts from the phase-correlation translation of its frames (matchSeries.h:145-153)
   s.corr_init = p.checkAndGetBool("useCorrelationToInitTranslation");
   s.max_corr_shift = s.corr_init ? p.getInt("maxCorrShift") : 0;
+  // after each frame, the inverse problem solved from the approximate inverse (matchSeries.h:247-259)
+  s.calc_inverse = p.checkAndGetBool("calcInverseDeformation");
+  s.inverse_out = nullptr;
   return s;
 }
 
@@ -271,11 +273,13 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
   const bool accumulate = !p.checkAndGetBool("dontAccumulateDeformation");
   const int npw = pin ? 1 : n;
   const int nlp = sp.stop_level - sp.start_level + 1, nlr = sp.refine_stop_level - sp.refine_start_level + 1;
-  const int nrec = npw * nlp + (accumulate ? (n - 1) * nlr : 0);
+  const bool inverse = accumulate && sp.calc_inverse;
+  const int nrec = npw * nlp + (accumulate ? (n - 1) * nlr : 0) + (inverse ? n * nlr : 0);
   size_t nfld = 0;
   for (int l = sp.start_level; l <= sp.stop_level; ++l) nfld += (size_t)npw * (2u << (2 * l));
   if (accumulate)
-    for (int l = sp.refine_start_level; l <= sp.refine_stop_level; ++l) nfld += (size_t)(n - 1) * (2u << (2 * l));
+    for (int l = sp.refine_start_level; l <= sp.refine_stop_level; ++l)
+      nfld += (size_t)(n - 1 + (inverse ? n : 0)) * (2u << (2 * l));
   const int maxIt = std::max(sp.max_iter, 0);
   std::vector<double> lvFields(nfld), lvStart(2 * nrec);
   std::vector<msg_iter_trace> lvIters((size_t)std::max(nrec * maxIt, 1));
@@ -346,6 +350,11 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
       if (refineStop < L) saveField(sub + strprintf("deformation_%02d_%%d.dat.bz2", L), phd, N);
     }
     energiesFile << i << " " << energies[i] << "\n";
+  }
+  for (int i = 0; inverse && i < n; ++i) {  // the inverse solves' records come last
+    const std::string sub = strprintf("%s%d-r-inv/", dir.c_str(), i);
+    makeDirectory(sub);
+    writeLevels(sub, sp.refine_start_level, refineStop);
   }
   if (sp.reduce_deformations) {  // "<saveDirectory>reduceDef_<c>.dat.bz2" (matchSeries.h:701)
     std::vector<double> hr(2 * nn);

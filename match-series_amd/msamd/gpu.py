@@ -44,6 +44,7 @@ class SeriesParams(ctypes.Structure):
         ("average_ext_mean", ctypes.c_int), ("average_nearest", ctypes.c_int),
         ("reduce_deformations", ctypes.c_int), ("reduce_max_iter", ctypes.c_int), ("reduce_stop_eps", ctypes.c_double),
         ("pair0_init", ctypes.c_void_p), ("corr_init", ctypes.c_int), ("max_corr_shift", ctypes.c_int),
+        ("calc_inverse", ctypes.c_int), ("inverse_out", ctypes.c_void_p),
     ]
 
 
@@ -97,6 +98,7 @@ SIGNATURES = {
     "msg_time_energy_pass": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp], ctypes.c_int),
     "msg_match_series": ([ctypes.POINTER(SeriesParams), ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _dp, _vp,
                           _vp, _vp, ctypes.POINTER(SeriesLevels), _vp], ctypes.c_int),
+    "msg_approx_inverse": ([ctypes.c_int, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "msg_register_translation": ([ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _ip, _vp, _vp], ctypes.c_int),
     "msg_reduce_deformations": ([ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _dp,
                                  ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
@@ -385,6 +387,9 @@ def series_level_layout(params, n, reuse=False):
     if not params.dont_accumulate:
         recs += [("refine", i, l) for i in range(1, n)
                  for l in range(params.refine_start_level, params.refine_stop_level + 1)]
+        if params.calc_inverse:
+            recs += [("inverse", i, l) for i in range(n)
+                     for l in range(params.refine_start_level, params.refine_stop_level + 1)]
     return recs
 
 
@@ -400,9 +405,17 @@ def reduce_deformations(defs, max_iter=1000, stop_eps=5e-7):
     return phi, float(e[0]), it.value
 
 
+def approx_inverse(phi):
+    """qc::approxInverseDeformation of a (2, n, n) device displacement field."""
+    out = _torch().empty_like(phi)
+    check(lib().msg_approx_inverse(dtype_code(phi), _ptr(phi), _ptr(out), phi.shape[-1], _stream()))
+    return out
+
+
 def match_series(templates, reference, params, pairwise_in=None, want_pairwise=False, want_levels=False):
     """One stage of series matching on device tensors. templates: (n, N, N); reference: (N, N).
-    want_levels: also return the per-level deformations and iteration records of every solve."""
+    want_levels: also return the per-level deformations and iteration records of every solve; with
+    params.calc_inverse the inverse solves' deformations come back as "inverses"."""
     torch = _torch()
     n, N = templates.shape[0], templates.shape[-1]
     lv = None
@@ -423,11 +436,15 @@ def match_series(templates, reference, params, pairwise_in=None, want_pairwise=F
     med = torch.zeros_like(mean)
     _ready()
     red = torch.zeros((2, N, N), dtype=templates.dtype, device=templates.device) if params.reduce_deformations else None
+    inv = torch.zeros_like(defs) if params.calc_inverse and not params.dont_accumulate else None
+    params.inverse_out = ctypes.c_void_p(inv.data_ptr()) if inv is not None else None
     check(lib().msg_match_series(ctypes.byref(params), dtype_code(templates), _ptr(templates), n, _ptr(reference),
                                  _ptr(pairwise_in) if pairwise_in is not None else None, _ptr(defs),
                                  _ptr(pw) if pw is not None else None, _dptr(en), _ptr(mean), _ptr(cnt), _ptr(med),
                                  ctypes.byref(lv) if lv is not None else None, _ptr(red) if red is not None else None))
-    out = dict(deformations=defs, pairwise=pw, energies=en, mean=mean, count=cnt, median=med, reduced=red)
+    params.inverse_out = None
+    out = dict(deformations=defs, pairwise=pw, energies=en, mean=mean, count=cnt, median=med, reduced=red,
+               inverses=inv)
     if lv is not None:
         levels, off = [], 0
         tr = np.ctypeslib.as_array(iters).view(np.float64).reshape(-1, 8) if mi > 0 else np.zeros((0, 8))

@@ -1244,6 +1244,92 @@ void msor_register_translation(const double *reference, const double *templ, int
   free(pc);
 }
 
+/* TransformFunction<QC_2D>::interpolateTriangle (quoc/auxiliary.cpp:115-176): nodes of the bounding
+ * box not yet set whose barycentric coordinates pass the 1e-6 tolerance take the affine
+ * interpolation of the corner values (when finite) and are marked set. */
+static void invTriangle(const double c[3][2], const double val[2][3], int N, unsigned char *set, double *dest) {
+  int miny = N, maxy = -1, minx = N, maxx = -1;
+  for (int k = 0; k < 3; ++k) {
+    int qx = (int)c[k][0], qy = (int)c[k][1];
+    if (qy > maxy) maxy = qy;
+    if (qy < c[k][1]) ++qy;
+    if (qy < miny) miny = qy;
+    if (qx > maxx) maxx = qx;
+    if (qx < c[k][0]) ++qx;
+    if (qx < minx) minx = qx;
+  }
+  if (minx < 0) minx = 0;
+  if (miny < 0) miny = 0;
+  if (maxx > N - 1) maxx = N - 1;
+  if (maxy > N - 1) maxy = N - 1;
+  const double m00 = c[0][0] - c[2][0], m10 = c[0][1] - c[2][1];
+  const double m01 = c[1][0] - c[2][0], m11 = c[1][1] - c[2][1];
+  const double det = m00 * m11 - m10 * m01;
+  if (det == 0 || !isfinite(det)) return;
+  const double i00 = m11 / det, i01 = -m01 / det, i10 = -m10 / det, i11 = m00 / det;
+  const size_t NN = (size_t)N * N;
+  for (int x = minx; x <= maxx; ++x)
+    for (int y = miny; y <= maxy; ++y) {
+      const size_t at = (size_t)y * N + x;
+      if (set[at]) continue;
+      const double r0 = (double)x - c[2][0], r1 = (double)y - c[2][1];
+      const double l0 = i00 * r0 + i01 * r1, l1 = i10 * r0 + i11 * r1;
+      if (l0 < -1.e-6 || l1 < -1.e-6 || (l0 + l1) > 1. + 1.e-6) continue;
+      set[at] = 1;
+      for (int k = 0; k < 2; ++k) {
+        const double v = l0 * val[k][0] + l1 * val[k][1] + (1.0 - l0 - l1) * val[k][2];
+        if (isfinite(v)) dest[k * NN + at] = v;
+      }
+    }
+}
+
+void msor_approx_inverse(const double *phi, int N, double *inv) {
+  const size_t NN = (size_t)N * N;
+  const double H = 1.0 / (double)(N - 1);
+  unsigned char *set = (unsigned char *)calloc(NN, 1);
+  for (size_t i = 0; i < 2 * NN; ++i) inv[i] = 0.0; /* the extension image: zeros */
+  for (int ey = 0; ey < N - 1; ++ey)
+    for (int ex = 0; ex < N - 1; ++ex) {
+      const int px[4] = {ex, ex + 1, ex, ex + 1}, py[4] = {ey, ey, ey + 1, ey + 1};
+      double d[4][2], idv[2][4];
+      for (int i = 0; i < 4; ++i) {
+        const size_t at = (size_t)py[i] * N + px[i];
+        double dx = (double)px[i] + phi[at] / H, dy = (double)py[i] + phi[NN + at] / H;
+        dx = dx < (double)(N - 1) ? dx : (double)(N - 1); /* aol::Min, then aol::Max */
+        dy = dy < (double)(N - 1) ? dy : (double)(N - 1);
+        dx = dx < 0.0 ? 0.0 : dx;
+        dy = dy < 0.0 ? 0.0 : dy;
+        d[i][0] = dx;
+        d[i][1] = dy;
+        idv[0][i] = 1.0 * ((double)px[i] * H - 0.0); /* generateIdentity */
+        idv[1][i] = 1.0 * ((double)py[i] * H - 0.0);
+      }
+      double c[3][2] = {{d[0][0], d[0][1]}, {d[1][0], d[1][1]}, {d[2][0], d[2][1]}};
+      double val[2][3] = {{idv[0][0], idv[0][1], idv[0][2]}, {idv[1][0], idv[1][1], idv[1][2]}};
+      invTriangle(c, val, N, set, inv);
+      c[0][0] = d[3][0];
+      c[0][1] = d[3][1];
+      val[0][0] = idv[0][3];
+      val[1][0] = idv[1][3];
+      invTriangle(c, val, N, set, inv);
+    }
+  for (int k = 0; k < 2; ++k) {
+    double mn = DBL_MAX, mx = -DBL_MAX;
+    for (size_t i = 0; i < NN; ++i) {
+      const int y = (int)(i / N), x = (int)(i % N);
+      inv[k * NN + i] -= 1.0 * ((double)(k == 0 ? x : y) * H - 0.0);
+      if (set[i]) {
+        if (inv[k * NN + i] < mn) mn = inv[k * NN + i];
+        if (inv[k * NN + i] > mx) mx = inv[k * NN + i];
+      }
+    }
+    const double fill = 0.5 * (mn + mx);
+    for (size_t i = 0; i < NN; ++i)
+      if (!set[i]) inv[k * NN + i] = fill;
+  }
+  free(set);
+}
+
 static void prepared(const double *raw, int NN, const msor_series_params *p, double *out) {
   memcpy(out, raw, sizeof(double) * NN);
   if (!p->dont_normalize) msor_normalize(out, NN, p->saturation_pct, p->normalize_min_to_zero);
@@ -1302,6 +1388,16 @@ int msor_match_series(const double *templates, int n, const double *reference, i
       pyrBuild(&ref, tpl.lev[L], L); /* setReference(template) */
     }
     memcpy(deformations + i * 2 * NN, (!p->dont_accumulate && i > 0) ? acc : def.lev[L], sizeof(double) * 2 * NN);
+    if (!p->dont_accumulate && p->calc_inverse) {
+      /* calcInverseDeformation (matchSeries.h:247-259): the template as reference, the reference
+       * as template, from the approximate inverse of the accumulated deformation, refine levels */
+      double *inv0 = (double *)malloc(sizeof(double) * 2 * NN);
+      msor_approx_inverse(acc, N, inv0);
+      defSet(&def, inv0);
+      free(inv0);
+      solveML(&tpl, &ref0, &def, p->refine_start_level, p->refine_stop_level, p);
+      if (p->inverse_out) memcpy(p->inverse_out + i * 2 * NN, def.lev[L], sizeof(double) * 2 * NN);
+    }
     pyrFree(&tpl);
   }
 

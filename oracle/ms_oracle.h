@@ -108,6 +108,8 @@ typedef struct {
   const double *pair0_init;           /* NULL or template 0's starting deformation            */
   int corr_init;                      /* useCorrelationToInitTranslation                      */
   int max_corr_shift;                 /* maxCorrShift                                         */
+  int calc_inverse;                   /* calcInverseDeformation                               */
+  double *inverse_out;                /* NULL or n fields: the inverse solves' deformations   */
 } msor_series_params;
 
 /* Convolution::phaseCorrelation (quoc/convolution.h:1049-1114): the inverse FFT of
@@ -116,6 +118,12 @@ void msor_phase_correlation(const double *a, const double *b, int N, int normali
 /* PhaseCorrelationRegistration::registerImages (quoc/paramReg.h:2057-2096): signed integer shift,
  * x then y. */
 void msor_register_translation(const double *reference, const double *templ, int N, int max_shift, int *shift);
+
+/* qc::approxInverseDeformation (quoc/deformations.h:341-355): the identity pushed through phi by
+ * TransformFunction<QC_2D>::transform (quoc/auxiliary.cpp:33-176: two triangles per element, x-fastest
+ * element order, the first triangle covering a node wins), minus the identity; nodes no triangle
+ * covers get the midpoint of the covered values' range. phi, inv: (2, N, N). */
+void msor_approx_inverse(const double *phi, int N, double *inv);
 
 int msor_match_series(const double *templates, int n, const double *reference, int N,
                       const msor_series_params *p, const double *pairwise_in,

@@ -36,6 +36,7 @@ class SeriesParams(ctypes.Structure):
         ("dont_accumulate", ctypes.c_int), ("data_term", ctypes.c_int),
         ("reduce_deformations", ctypes.c_int), ("reduce_max_iter", ctypes.c_int), ("reduce_stop_eps", ctypes.c_double),
         ("pair0_init", ctypes.POINTER(ctypes.c_double)), ("corr_init", ctypes.c_int), ("max_corr_shift", ctypes.c_int),
+        ("calc_inverse", ctypes.c_int), ("inverse_out", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -256,6 +257,14 @@ def register_translation(reference, templ, max_shift=0):
     return int(sh[0]), int(sh[1])
 
 
+def approx_inverse(phi):
+    """qc::approxInverseDeformation (quoc/deformations.h:341-355) of a (2, N, N) displacement."""
+    phi = _f64(phi)
+    out = np.zeros_like(phi)
+    lib().msor_approx_inverse(_p(phi), phi.shape[-1], _p(out))
+    return out
+
+
 def match_series(templates, reference, params, pairwise_in=None, pair0_init=None):
     """One stage of series matching; returns dict of deformations, pairwise deformations,
     energies, mean, count, median (and the reduced deformation with reduce_deformations)."""
@@ -272,8 +281,11 @@ def match_series(templates, reference, params, pairwise_in=None, pair0_init=None
     init = None if pair0_init is None else _f64(pair0_init).copy()
     params.pair0_init = _p(init) if init is not None else None
     red = np.zeros((2, N, N))
+    inv = np.zeros((n, 2, N, N)) if params.calc_inverse and not params.dont_accumulate else None
+    params.inverse_out = _p(inv) if inv is not None else None
     lib().msor_match_series(_p(T), n, _p(ref), N, ctypes.byref(params), pin, _p(defs), _p(pw), _p(en),
                             _p(mean), _p(cnt), _p(med), _p(red))
     params.pair0_init = None
+    params.inverse_out = None
     return dict(deformations=defs, pairwise=pw, energies=en, mean=mean, count=cnt, median=med,
-                reduced=red if params.reduce_deformations else None)
+                reduced=red if params.reduce_deformations else None, inverses=inv)

@@ -221,6 +221,12 @@ int main(int argc, char **argv) {
       toVec(&in[0], p1[0]); toVec(&in[NN], p1[1]); toVec(&in[2 * NN], p2[0]); toVec(&in[3 * NN], p2[1]);
       qc::ConcatAndSmoothlyExtendDeformations<ConfType>(p1, p2, grid, res);
       append(out, res[0]); append(out, res[1]);
+    } else if (cmd == "inverse") {
+      // in: phi (2N^2) ; out: qc::approxInverseDeformation of phi
+      aol::MultiVector<RType> phi(2, NN), inv(2, NN);
+      toVec(&in[0], phi[0]); toVec(&in[NN], phi[1]);
+      qc::approxInverseDeformation<ConfType>(grid, phi, inv);
+      append(out, inv[0]); append(out, inv[1]);
     } else if (cmd == "normalize") {
       // params: pct (negative: no saturation) normMinToZero ; in: img ; out: prepared img + pyramid
       const double pct = atof(argv[3]);

@@ -145,6 +145,31 @@ def test_matchseries_reads_tiff_frames(tmp_path):
     check_stage(out, g, 1, 64, 1e-6, 1e-6)
 
 
+def test_matchseries_inverse_deformation(tmp_path):
+    """calcInverseDeformation 1: the <i>-r-inv/ solves of stage 1 (per-level deformations and energy
+    logs) against the reference program's (tests/golden/e2e64_inv.npz), FP64 fields."""
+    if not os.path.exists(EXE):
+        pytest.skip("bin/matchSeries not built")
+    with np.load(os.path.join(GOLD, "e2e64_inv.npz")) as z:
+        g = {k: z[k] for k in z.files}
+    out = run_cli(tmp_path, g, 64)
+    for i in range(3):
+        sub = f"{i}" if i == 0 else f"{i}-r"
+        d = np.stack([read_p9(str(out / "stage1" / sub / f"deformation_06_{c}.dat.bz2")) for c in range(2)])
+        assert np.abs(d - g["s1_deformations"][i]).max() * 63 < 1e-6
+        for lv in ("05", "06"):
+            sd = out / "stage1" / f"{i}-r-inv"
+            d = np.stack([read_p9(str(sd / f"deformation_{lv}_{c}.dat.bz2")) for c in range(2)])
+            m = 1 << int(lv)
+            assert np.abs(d - g[f"s1_{i}_r_inv_deformation_{lv}"]).max() * (m - 1) < 1e-6, (i, lv)
+            ref = g[f"s1_{i}_r_inv_energy_{lv}"]
+            mine = np.loadtxt(str(sd / f"energy_{lv}.txt")).reshape(-1, ref.shape[1])
+            np.testing.assert_array_equal(mine[:, 0], ref[:, 0])
+            np.testing.assert_allclose(mine[:, 1:], ref[:, 1:], rtol=2e-4, atol=1e-9)
+    for i in range(4):
+        assert (out / "stage2" / f"{i}-r-inv" / "deformation_06_1.dat.bz2").exists(), i
+
+
 @pytest.mark.parametrize("max_shift", [0, 6])
 def test_matchseries_translation_init(tmp_path, max_shift):
     """useCorrelationToInitTranslation on frames drifting by (-5, 4) px per frame: stage 1 against

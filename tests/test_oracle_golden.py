@@ -196,3 +196,24 @@ def test_reference_program_two_stages_reduce():
     assert np.array_equal(r2["count"], g["s2_numSamples"])
     np.testing.assert_allclose(r2["median"], g["s2_median"], atol=1e-4)
 
+
+
+@pytest.mark.parametrize("N", [8, 32, 64])
+def test_approx_inverse(N):
+    # qc::approxInverseDeformation through the reference harness: smooth, noisy and folding fields
+    g = load("inverse")
+    for k in range(3):
+        np.testing.assert_array_equal(O.approx_inverse(g[f"phi{N}_{k}"]), g[f"inv{N}_{k}"])
+
+
+def test_reference_program_inverse_solves():
+    # calcInverseDeformation: the reference program's <i>-r-inv/ solves of stage 1
+    g = load("e2e64_inv")
+    fr = g["frames"]
+    N = fr.shape[-1]
+    p = e2e_params(10.0)
+    p.calc_inverse = 1
+    r1 = O.match_series(fr[1:], fr[0], p)
+    assert np.abs(r1["deformations"] - g["s1_deformations"]).max() * (N - 1) < 1e-10
+    for i in range(3):
+        assert np.abs(r1["inverses"][i] - g[f"s1_{i}_r_inv_deformation_06"]).max() * (N - 1) < 1e-10, i

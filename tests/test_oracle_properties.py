@@ -286,3 +286,35 @@ def test_register_translation_conventions():
             if cc[dy % N, dx % N] > best:
                 best, at = cc[dy % N, dx % N], (dx, dy)
     assert O.register_translation(a, b, m) == at
+
+
+def test_approx_inverse_of_translations():
+    # phi = t (whole pixels): every node x - t inside the grid is covered, with value -t; the
+    # others take the midpoint of the covered range, also -t. The zero field inverts to zero.
+    N = 16
+    h = 1.0 / (N - 1)
+    assert np.abs(O.approx_inverse(np.zeros((2, N, N)))).max() < 1e-15
+    for tx, ty in ((2, 0), (-3, 1), (1, -4)):
+        phi = np.stack([np.full((N, N), tx * h), np.full((N, N), ty * h)])
+        inv = O.approx_inverse(phi)
+        np.testing.assert_allclose(inv[0], -tx * h, atol=1e-14)
+        np.testing.assert_allclose(inv[1], -ty * h, atol=1e-14)
+
+
+def test_approx_inverse_composes_to_identity():
+    # a small smooth deformation: phi(x + inv(x)) + inv(x) ~ 0 (the inverse is exact at the nodes up
+    # to the piecewise-linear interpolation of phi), checked on the interior
+    N = 64
+    h = 1.0 / (N - 1)
+    y, x = np.mgrid[0:N, 0:N] * h
+    phi = np.stack([1.5 * h * np.sin(2 * np.pi * y), 1.5 * h * np.cos(2 * np.pi * x)])
+    inv = O.approx_inverse(phi)
+    px, py = x + inv[0], y + inv[1]
+    i, j = np.clip((px / h).astype(int), 0, N - 2), np.clip((py / h).astype(int), 0, N - 2)
+    fx, fy = px / h - i, py / h - j
+
+    def bil(f):
+        return ((1 - fx) * (1 - fy) * f[j, i] + fx * (1 - fy) * f[j, i + 1] + (1 - fx) * fy * f[j + 1, i]
+                + fx * fy * f[j + 1, i + 1])
+    r = np.stack([bil(phi[0]) + inv[0], bil(phi[1]) + inv[1]])[:, 4:-4, 4:-4]
+    assert np.abs(r).max() < 0.05 * h
