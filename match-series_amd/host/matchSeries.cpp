@@ -85,13 +85,12 @@ std::string dirOf(const ParameterParser &p) { return p.getStringExpandTilde("sav
 // Options of the reference that change the computation but are not implemented here.
 void rejectUnsupported(const ParameterParser &p) {
   const char *flags[] = {"reverseRolesInSeriesMatching", "cropInput", "resizeInput", "calcIQ", "calcPrecision",
-                         "performNoiseAnalysis", "saveNamedDeformedTemplates", "saveNamedDeformations"};
+                         "performNoiseAnalysis", "saveNamedDeformedDMXTemplatesAsDMX"};
   for (const char *f : flags)
     if (p.checkAndGetBool(f)) MSH_THROW(strprintf("\"%s 1\" is not supported by this build", f));
   if (p.hasVariable("altStartLevel") && p.getInt("altStartLevel") != p.getInt("startLevel"))
     MSH_THROW("\"altStartLevel\" is not supported by this build");
   if (p.hasVariable("initial-deformation")) MSH_THROW("\"initial-deformation\" is not supported by this build");
-  if (p.hasVariable("averageSaveIncrement")) MSH_THROW("\"averageSaveIncrement\" is not supported by this build");
   // the reference itself does not implement a positive threshold (registration.h:604-610)
   if (p.getDoubleOrDefault("averageXDerivNormThreshold", 0) > 0) MSH_THROW("averageXDerivNormThreshold: not implemented");
   if (p.hasVariable("minimizationAlgo") && p.getString("minimizationAlgo") != "GradientDescent")
@@ -162,59 +161,154 @@ void writeReconstruction(const std::string &dir, const std::vector<double> &mean
   savePNGScaled(dir + "numSamples.png", count.data(), N, N);
 }
 
+// aol::getBaseFileName: the file name without directory and last extension
+std::string baseFileName(const std::string &f) {
+  const size_t slash = f.rfind('/');
+  std::string b = slash == std::string::npos ? f : f.substr(slash + 1);
+  const size_t dot = b.rfind('.');
+  return dot == std::string::npos ? b : b.substr(0, dot);
+}
+
+double meanOf(const std::vector<double> &v) {
+  double s = 0;
+  for (double x : v) s += x;
+  return v.empty() ? 0.0 : s / v.size();  // ScalarArray::getMeanValue
+}
+
+// The deformed templates SeriesMatching::averageSeries averages (matchSeries.h:381-392): with
+// reduceDeformations the reduced deformation first (+Inf outside), then each template's own
+// deformation, +Inf outside (or the image's mean with deformForAverageExtendedWithMean), optionally
+// nearest-neighbour. img: the n raw templates on the device (the reduced pre-warp replaces them).
+void deformForAverage(const ParameterParser &p, int n, int N, int dt, void *img, const void *defs,
+                      const void *reduced, void *stack) {
+  const size_t nn = (size_t)N * N;
+  const int es = dt == MSG_F64 ? 8 : 4;
+  const bool extMean = p.checkAndGetBool("deformForAverageExtendedWithMean");
+  const int nearest = p.checkAndGetBool("deformForAverageUsingNearestNeighborInterpolation") ? 1 : 0;
+  if (reduced) {
+    DeviceBuffer pre(es * nn * n);
+    msgCheck(msg_warp(dt, img, reduced, pre.p, N, n, 1, 0, INFINITY, 0, nullptr));
+    hipCheck(hipMemcpy(img, pre.p, es * nn * n, hipMemcpyDeviceToDevice), "copy");
+  }
+  if (extMean) {
+    for (int i = 0; i < n; ++i) {
+      std::vector<double> h(nn);
+      download(h.data(), (char *)img + (size_t)i * es * nn, nn, dt);
+      msgCheck(msg_warp(dt, (char *)img + (size_t)i * es * nn, (const char *)defs + (size_t)i * es * 2 * nn,
+                        (char *)stack + (size_t)i * es * nn, N, 1, 0, 0, meanOf(h), nearest, nullptr));
+    }
+  } else {
+    msgCheck(msg_warp(dt, img, defs, stack, N, n, 0, 0, INFINITY, nearest, nullptr));
+  }
+}
+
+// mean / count / median of the first k deformed templates, `fill` where no sample, written with
+// the iteration suffix "_<k>" (three digits) unless iter < 0 (saveAverageMedianAndNumSamples,
+// matchSeries.h:271-279).
+void reconstructAndWrite(const std::string &dir, const void *stack, int k, int N, int dt, double fill, int iter) {
+  const size_t nn = (size_t)N * N;
+  const int es = dt == MSG_F64 ? 8 : 4;
+  DeviceBuffer mean(es * nn), cnt(es * nn), med(es * nn);
+  msgCheck(msg_series_stats(dt, stack, k, N, fill, mean.p, cnt.p, med.p, nullptr));
+  std::vector<double> hm(nn), hc(nn), hmed(nn);
+  download(hm.data(), mean.p, nn, dt);
+  download(hc.data(), cnt.p, nn, dt);
+  download(hmed.data(), med.p, nn, dt);
+  const std::string sfx = iter < 0 ? std::string() : strprintf("_%03d", iter);
+  for (int f = 0; f < 3; ++f) {
+    const char *nm = f == 0 ? "average" : f == 1 ? "median" : "numSamples";
+    const std::vector<double> &a = f == 0 ? hm : f == 1 ? hmed : hc;
+    saveP9(dir + nm + sfx + ".q2bz", a.data(), N, N);
+    savePNGScaled(dir + nm + sfx + ".png", a.data(), N, N);
+  }
+}
+
+// The optional outputs of SeriesMatching::averageSeries (matchSeries.h:365-423) besides the final
+// reconstruction: <basename>.q2bz, each template deformed by its own deformation with 0 (or its mean)
+// outside (saveNamedDeformedTemplates; in stage 1 also the undeformed reference);
+// <basename>_def_<c>.dat.bz2, its deformation (saveNamedDeformations); the reconstruction of the
+// first k templates after every averageSaveIncrement of them. img: raw templates (device), defs:
+// their deformations (device), reduced: NULL or the reduced deformation (device).
+void seriesExtras(const ParameterParser &p, int stage, const std::vector<std::string> &files, const void *img,
+                  const void *defs, const void *reduced, int N, int dt) {
+  const std::string dir = dirOf(p);
+  const int n = (int)files.size();
+  const size_t nn = (size_t)N * N;
+  const int es = dt == MSG_F64 ? 8 : 4;
+  const bool namedTpl = p.checkAndGetBool("saveNamedDeformedTemplates");
+  const bool namedDef = p.checkAndGetBool("saveNamedDeformations");
+  const int inc = p.hasVariable("averageSaveIncrement") ? p.getInt("averageSaveIncrement") : n;
+  if (inc <= 0) MSH_THROW("'averageSaveIncrement' must be bigger than zero.");
+  if (namedTpl && stage == 1) {
+    const std::string ref = p.getStringExpandTilde("reference");
+    const Image r = loadChecked(ref, N);
+    saveP9(dir + baseFileName(ref) + ".q2bz", r.data.data(), N, N);
+  }
+  if (namedTpl) {
+    const bool extMean = p.checkAndGetBool("saveNamedDeformedTemplatesExtendedWithMean");
+    const int nearest = p.checkAndGetBool("saveNamedDeformedTemplatesUsingNearestNeighborInterpolation") ? 1 : 0;
+    DeviceBuffer out(es * nn);
+    std::vector<double> h(nn);
+    for (int i = 0; i < n; ++i) {
+      const char *im = (const char *)img + (size_t)i * es * nn;
+      double ext = 0;
+      if (extMean) {
+        download(h.data(), im, nn, dt);
+        ext = meanOf(h);
+      }
+      msgCheck(msg_warp(dt, im, (const char *)defs + (size_t)i * es * 2 * nn, out.p, N, 1, 0, 0, ext, nearest, nullptr));
+      download(h.data(), out.p, nn, dt);
+      saveP9(dir + baseFileName(files[i]) + ".q2bz", h.data(), N, N);
+    }
+  }
+  if (namedDef) {
+    std::vector<double> h(2 * nn);
+    for (int i = 0; i < n; ++i) {
+      download(h.data(), (const char *)defs + (size_t)i * es * 2 * nn, 2 * nn, dt);
+      saveField(dir + baseFileName(files[i]) + "_def_%d.dat.bz2", h.data(), N);
+    }
+  }
+  if (inc < n) {
+    DeviceBuffer work(es * nn * n), stack(es * nn * n);
+    hipCheck(hipMemcpy(work.p, img, es * nn * n, hipMemcpyDeviceToDevice), "copy");
+    deformForAverage(p, n, N, dt, work.p, defs, reduced, stack.p);
+    std::vector<double> h(nn);
+    for (int k = inc; k < n; k += inc) {
+      download(h.data(), (const char *)img + (size_t)(k - 1) * es * nn, nn, dt);
+      reconstructAndWrite(dir, stack.p, k, N, dt, meanOf(h), k);
+    }
+  }
+}
+
 // ACTION 1: SeriesMatching::averageSeries (matchSeries.h:353-451) from saved deformations.
-void averageOnly(const ParameterParser &p) {
+void averageOnly(const ParameterParser &p, int stage) {
   const std::string dir = dirOf(p);
   const int L = p.getInt("precisionLevel"), N = 1 << L;
   const size_t nn = (size_t)N * N;
   const int dt = fieldType(p), es = dt == MSG_F64 ? 8 : 4;
   const std::vector<std::string> files = createTemplateFileNameList(p);
   const int n = (int)files.size();
-  DeviceBuffer img(es * nn * n), defs(es * 2 * nn * n), stack(es * nn * n), mean(es * nn), cnt(es * nn), med(es * nn);
+  DeviceBuffer img(es * nn * n), raw(es * nn * n), defs(es * 2 * nn * n), stack(es * nn * n);
   double lastMean = 0;
-  const bool extMean = p.checkAndGetBool("deformForAverageExtendedWithMean");
-  const int nearest = p.checkAndGetBool("deformForAverageUsingNearestNeighborInterpolation") ? 1 : 0;
-  std::vector<double> means(n);
   for (int i = 0; i < n; ++i) {
     const Image im = loadChecked(files[i], N);
     upload((char *)img.p + (size_t)i * es * nn, im.data.data(), nn, dt);
     std::vector<double> phi;
     loadField(deformationBase(p, dir, i, L) + "_%d.dat.bz2", N, phi);
     upload((char *)defs.p + (size_t)i * es * 2 * nn, phi.data(), 2 * nn, dt);
-    double s = 0;
-    for (double v : im.data) s += v;
-    means[i] = s / nn;  // ScalarArray::getMeanValue
-    if (i == n - 1) lastMean = means[i];
+    if (i == n - 1) lastMean = meanOf(im.data);
   }
+  hipCheck(hipMemcpy(raw.p, img.p, es * nn * n, hipMemcpyDeviceToDevice), "copy");
+  std::unique_ptr<DeviceBuffer> redD;
   if (p.checkAndGetBool("reduceDeformations")) {  // the reduced deformation first (matchSeries.h:381-392)
     std::vector<double> red;
     loadField(dir + "reduceDef_%d.dat.bz2", N, red);
-    DeviceBuffer redD(es * 2 * nn), pre(es * nn * n);
-    upload(redD.p, red.data(), 2 * nn, dt);
-    msgCheck(msg_warp(dt, img.p, redD.p, pre.p, N, n, 1, 0, INFINITY, 0, nullptr));
-    hipCheck(hipMemcpy(img.p, pre.p, es * nn * n, hipMemcpyDeviceToDevice), "copy");
-    if (extMean)  // the mean of the reduced-deformed image (+Inf where it has no sample)
-      for (int i = 0; i < n; ++i) {
-        std::vector<double> h(nn);
-        download(h.data(), (char *)img.p + (size_t)i * es * nn, nn, dt);
-        double s = 0;
-        for (double v : h) s += v;
-        means[i] = s / nn;
-      }
+    redD.reset(new DeviceBuffer(es * 2 * nn));
+    upload(redD->p, red.data(), 2 * nn, dt);
   }
-  if (extMean) {
-    for (int i = 0; i < n; ++i)
-      msgCheck(msg_warp(dt, (char *)img.p + (size_t)i * es * nn, (char *)defs.p + (size_t)i * es * 2 * nn,
-                        (char *)stack.p + (size_t)i * es * nn, N, 1, 0, 0, means[i], nearest, nullptr));
-  } else {
-    msgCheck(msg_warp(dt, img.p, defs.p, stack.p, N, n, 0, 0, INFINITY, nearest, nullptr));
-  }
-  msgCheck(msg_series_stats(dt, stack.p, n, N, lastMean, mean.p, cnt.p, med.p, nullptr));
-  std::vector<double> hm(nn), hc(nn), hmed(nn);
-  download(hm.data(), mean.p, nn, dt);
-  download(hc.data(), cnt.p, nn, dt);
-  download(hmed.data(), med.p, nn, dt);
-  writeReconstruction(dir, hm, hmed, hc, N);
+  deformForAverage(p, n, N, dt, img.p, defs.p, redD ? redD->p : nullptr, stack.p);
+  seriesExtras(p, stage, files, raw.p, defs.p, redD ? redD->p : nullptr, N, dt);
+  reconstructAndWrite(dir, stack.p, n, N, dt, lastMean, -1);
 }
 
 // ACTION 0 of one stage: SeriesMatching::matchAndAverageSeries (matchSeries.h:565-570).
@@ -366,6 +460,7 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
   download(hc.data(), cnt.p, nn, dt);
   download(hmed.data(), med.p, nn, dt);
   writeReconstruction(dir, hm, hmed, hc, N);
+  seriesExtras(p, stage, files, tpl.p, defs.p, sp.reduce_deformations ? reduced.p : nullptr, N, dt);
 }
 
 void doAction(ParameterParser &p, int action, bool loadStageOneResults, int stage) {
@@ -374,7 +469,7 @@ void doAction(ParameterParser &p, int action, bool loadStageOneResults, int stag
   p.dumpToFile("parameter-dump.txt", dirOf(p));
   switch (action) {
     case 0: matchAndAverage(p, loadStageOneResults, stage); break;
-    case 1: averageOnly(p); break;
+    case 1: averageOnly(p, stage); break;
     default: MSH_THROW(strprintf("ACTION %d is not supported by this build (0: match and average, 1: average)", action));
   }
 }

@@ -1,0 +1,83 @@
+"""Write the golden fixtures of matchSeries' optional outputs from the reference itself:
+  * inverse.npz: qc::approxInverseDeformation (oracle/_ref/ref_harness "inverse") of smooth, noisy
+    and folding displacements on 8x8, 32x32 and 64x64 grids;
+  * e2e64_inv.npz: the reference program on the 4-frame 64x64 series of e2e64.npz with
+    "calcInverseDeformation 1" (its <i>-r-inv/ solves among the per-level records);
+  * e2e64_named_<a|b>.npz: the same series with saveNamedDeformedTemplates (plain; extended with the
+    mean and nearest-neighbour with reduceDeformations), saveNamedDeformations and
+    averageSaveIncrement 2 / 1: every extra file of both stages.
+Runs in the build container only (needs oracle/_ref). Usage: python tests/golden/make_options.py
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+import make_golden as G  # noqa: E402
+
+
+def main():
+    if not os.path.exists(G.HARNESS):
+        sys.exit("oracle/_ref is not built (make -C oracle ref)")
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for N in (8, 32, 64):
+            rng = np.random.default_rng(500 + N)
+            h = 1.0 / (N - 1)
+            y, x = np.mgrid[0:N, 0:N] / (N - 1.0)
+            for k, amp in enumerate((0.5, 3.0, 8.0)):
+                phi = np.stack([amp * h * np.sin(2 * np.pi * y + 1) + 0.3 * amp * h * rng.standard_normal((N, N)),
+                                amp * h * np.cos(2 * np.pi * x) + 2 * h])
+                out[f"phi{N}_{k}"] = phi
+                out[f"inv{N}_{k}"] = G.harness("inverse", N, [], [phi], tmp).reshape(2, N, N)
+        np.savez_compressed(os.path.join(HERE, "inverse.npz"), **out)
+        with np.load(os.path.join(HERE, "e2e64.npz")) as z:
+            frames = z["frames"]
+        e2e = G.run_e2e(frames, G.E2E_PAR + "calcInverseDeformation 1\n", os.path.join(tmp, "e2e_inv"))
+        np.savez_compressed(os.path.join(HERE, "e2e64_inv.npz"), **e2e)
+        for tag, make_par in NAMED.items():
+            np.savez_compressed(os.path.join(HERE, f"e2e64_named_{tag}.npz"),
+                                **run_named(frames, make_par(G.E2E_PAR), os.path.join(tmp, f"named_{tag}")))
+    print("wrote inverse.npz, e2e64_inv.npz and e2e64_named_*.npz")
+
+
+NAMED = {
+    "a": lambda par: par + "saveNamedDeformations 1\nsaveNamedDeformedTemplates 1\naverageSaveIncrement 2\n",
+    "b": lambda par: par.replace("reduceDeformations 0", "reduceDeformations 1") + (
+        "saveNamedDeformedTemplates 1\nsaveNamedDeformedTemplatesExtendedWithMean 1\n"
+        "saveNamedDeformedTemplatesUsingNearestNeighborInterpolation 1\ndeformForAverageExtendedWithMean 1\n"
+        "averageSaveIncrement 1\n"),
+}
+
+
+def run_named(frames, par, wd):
+    """The reference program with the extra options; every file of its stage directories that the
+    plain run does not write (named deformed templates and deformations, intermediate
+    reconstructions), keyed "s<stage>/<file name>"."""
+    os.makedirs(os.path.join(wd, "in"))
+    for i, f in enumerate(frames):
+        G.write_p9(os.path.join(wd, "in", f"frame_{i:02d}.dat.bz2"), f)
+    with open(os.path.join(wd, "run.par"), "w") as f:
+        f.write(par)
+    env = dict(os.environ, QUOC_NO_SYSTEM_PAUSE="ON")
+    subprocess.check_call([G.MATCHSERIES, "run.par"], cwd=wd, env=env, stdout=subprocess.DEVNULL,
+                          stderr=subprocess.DEVNULL)
+    out = dict(frames=frames, par=np.frombuffer(par.encode(), dtype=np.uint8))
+    for st in (1, 2):
+        sd = os.path.join(wd, "out", f"stage{st}")
+        for fn in sorted(os.listdir(sd)):
+            path = os.path.join(sd, fn)
+            if fn.startswith("frame_") or "_00" in fn:
+                if fn.endswith(".png"):
+                    out[f"s{st}/{fn}"] = G.read_png_gray8(path)
+                elif fn.endswith((".q2bz", ".bz2")):
+                    out[f"s{st}/{fn}"] = G.read_p9(path)
+    return out
+
+
+if __name__ == "__main__":
+    main()
