@@ -218,6 +218,10 @@ typedef struct {
                                       /* two images' roles swapped, started from its          */
                                       /* approximate inverse (matchSeries.h:247-259)          */
   void *inverse_out;                  /* NULL or n device fields: those inverse solves' result*/
+  const void *raw_templates;          /* NULL, or n device images the reconstruction warps    */
+                                      /* instead of `templates` (when the caller prepared the */
+                                      /* matching inputs itself: normalised at their own      */
+                                      /* resolution, then restricted, registration.h:369-399) */
 } msg_series_params;
 
 /* Optional per-level records of msg_match_series: what the reference writes after every level of

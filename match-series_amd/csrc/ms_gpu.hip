@@ -2078,7 +2078,10 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
   // reduceDeformations (matchSeries.h:565-570, 678-704): the reduced deformation is applied to every
   // template first (DeformImage, +Inf outside), then the template's own deformation (L386-392)
   DevBuf red, pre;
-  const S *src = tpl;
+  // the frames as the reconstruction reads them: raw (raw_templates, when the caller prepared the
+  // matching inputs itself), else the templates given
+  const S *raw = p->raw_templates ? (const S *)p->raw_templates : tpl;
+  const S *src = raw;
   if (p->reduce_deformations) {
     void *rp = reduced;
     if (!rp) {
@@ -2089,7 +2092,7 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
     CHK(reduce_impl<S>(defs, n, L, p->reduce_max_iter, p->reduce_stop_eps, rp, nullptr, nullptr));
     HIPCHK(hipMalloc(&pre.p, sizeof(S) * nn * n));
     const size_t cnt = nn * n;
-    hipLaunchKernelGGL((msg::k_warp<S>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, tpl, (const S *)rp,
+    hipLaunchKernelGGL((msg::k_warp<S>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, raw, (const S *)rp,
                        (S *)pre.p, N, n, 1, 0, (double)INFINITY, 0);
     HIPCHK(hipGetLastError());
     src = (const S *)pre.p;
@@ -2112,7 +2115,7 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
     HIPCHK(hipGetLastError());
   }
   double fill = 0;
-  CHK(mean_value<S>(tpl + (size_t)(n - 1) * nn, (int)nn, &fill, s));
+  CHK(mean_value<S>(raw + (size_t)(n - 1) * nn, (int)nn, &fill, s));
   CHK(stats_impl<S>(stack.p, n, N, fill, mean, count, median, s));
   HIPCHK(hipStreamSynchronize(s));
   return MSG_OK;

@@ -84,7 +84,7 @@ std::string dirOf(const ParameterParser &p) { return p.getStringExpandTilde("sav
 
 // Options of the reference that change the computation but are not implemented here.
 void rejectUnsupported(const ParameterParser &p) {
-  const char *flags[] = {"reverseRolesInSeriesMatching", "cropInput", "resizeInput", "calcIQ", "calcPrecision",
+  const char *flags[] = {"reverseRolesInSeriesMatching", "resizeInput", "calcIQ", "calcPrecision",
                          "performNoiseAnalysis", "saveNamedDeformedDMXTemplatesAsDMX"};
   for (const char *f : flags)
     if (p.checkAndGetBool(f)) MSH_THROW(strprintf("\"%s 1\" is not supported by this build", f));
@@ -136,12 +136,59 @@ msg_series_params seriesParams(const ParameterParser &p) {
 
 int fieldType(const ParameterParser &p) { return p.getIntOrDefault("gpuFieldPrecision", 32) == 64 ? MSG_F64 : MSG_F32; }
 
-Image loadChecked(const std::string &f, int N) {
-  Image im = loadImage(f);
-  if (im.nx != N || im.ny != N)
-    MSH_THROW(strprintf("Input data size (%dx%d of %s) doesn't match the size of the computational grid (%dx%d).",
-                        im.nx, im.ny, f.c_str(), N, N));
-  return im;
+// An input image as RegistrationMultilevelDescentInterface::initMultilevelArrayFromImage prepares it
+// (registration.h:330-399): cut out of the input at (cropStartX, cropStartY) with cropInput, else
+// taken at its own dyadic level k >= L; normalised there (scale: prepareMultilevelArray's
+// saturation clamp or min/max scaling), pre-smoothed there (smooth: preSmoothSigma in pixels of
+// that level), then restricted to level L. Writes 2^L x 2^L values of the field type to dst.
+void prepareFrame(const ParameterParser &p, const Image &im, const std::string &name, int L, int dt, bool scale,
+                  bool smooth, bool noCrop, void *dst) {
+  const int N = 1 << L, es = dt == MSG_F64 ? 8 : 4;
+  int k = L;
+  std::vector<double> full;
+  if (!noCrop && p.checkAndGetBool("cropInput")) {
+    const int sx = p.getInt("cropStartX"), sy = p.getInt("cropStartY");
+    const int w = std::min(im.nx, N), h = std::min(im.ny, N);
+    if (sx < 0 || sy < 0 || sx + w > im.nx || sy + h > im.ny)
+      MSH_THROW(strprintf("cropInput: the %dx%d block at (%d, %d) is not inside %s (%dx%d)", w, h, sx, sy,
+                          name.c_str(), im.nx, im.ny));
+    full.assign((size_t)N * N, 0.0);
+    for (int y = 0; y < h; ++y)
+      for (int x = 0; x < w; ++x) full[(size_t)y * N + x] = im.data[(size_t)(sy + y) * im.nx + sx + x];
+  } else {
+    k = 0;
+    while ((2 << k) <= im.nx) ++k;  // qc::logBaseTwo
+    if (im.nx != (1 << k) || im.ny != (1 << k))
+      MSH_THROW(strprintf("Input data size (%dx%d of %s) doesn't match the size of the computational grid. Either "
+                          "use a different type of grid or change the size with the input data with \"resizeInput\" "
+                          "or \"cropInput\".", im.nx, im.ny, name.c_str()));
+    if (k < L)
+      MSH_THROW("Input data size too small. Note: Unless \"resizeInput\" is used, \"precisionLevel\" may not be bigger "
+                "than the grid level of the input data.");
+    full = im.data;
+  }
+  const int n = 1 << k;
+  std::unique_ptr<DeviceBuffer> cur(new DeviceBuffer((size_t)es * n * n));
+  upload(cur->p, full.data(), (size_t)n * n, dt);
+  if (scale)
+    msgCheck(msg_normalize(dt, cur->p, n, 1,
+                           p.hasVariable("enhanceContrastSaturationPercentage")
+                               ? p.getDouble("enhanceContrastSaturationPercentage") : -1.0,
+                           p.checkAndGetBool("normalizeMinToZero"), nullptr));
+  const double sigma = p.getDoubleOrDefault("preSmoothSigma", 0.0);
+  if (smooth && sigma > 0) msgCheck(msg_h1_smooth(dt, cur->p, cur->p, n, 1, sigma / (double)(n - 1), nullptr));
+  for (int m = n; m > N; m /= 2) {
+    std::unique_ptr<DeviceBuffer> half(new DeviceBuffer((size_t)es * (m / 2) * (m / 2)));
+    msgCheck(msg_restrict(dt, cur->p, half->p, m, 1, nullptr));
+    cur.swap(half);
+  }
+  hipCheck(hipMemcpy(dst, cur->p, (size_t)es * N * N, hipMemcpyDeviceToDevice), "copy");
+}
+
+// The raw frame at level L (loadAndPrepareImage(..., NoScaling, ..., NoSmoothing), as averageSeries
+// reads the frames) into a device buffer of the field type.
+void loadRaw(const ParameterParser &p, const std::string &f, int L, int dt, void *dst, bool noCrop = false) {
+  prepareFrame(p, loadImage(f), f, L, dt, false, false, noCrop, dst);
 }
 
 // <dir><i>[-r]/deformation_<LL> (SeriesMatching::createDeformationBaseFileName, matchSeries.h:58-67)
@@ -241,8 +288,13 @@ void seriesExtras(const ParameterParser &p, int stage, const std::vector<std::st
   if (inc <= 0) MSH_THROW("'averageSaveIncrement' must be bigger than zero.");
   if (namedTpl && stage == 1) {
     const std::string ref = p.getStringExpandTilde("reference");
-    const Image r = loadChecked(ref, N);
-    saveP9(dir + baseFileName(ref) + ".q2bz", r.data.data(), N, N);
+    int L = 0;
+    while ((2 << L) <= N) ++L;
+    DeviceBuffer r(es * nn);
+    loadRaw(p, ref, L, dt, r.p);
+    std::vector<double> h(nn);
+    download(h.data(), r.p, nn, dt);
+    saveP9(dir + baseFileName(ref) + ".q2bz", h.data(), N, N);
   }
   if (namedTpl) {
     const bool extMean = p.checkAndGetBool("saveNamedDeformedTemplatesExtendedWithMean");
@@ -291,12 +343,15 @@ void averageOnly(const ParameterParser &p, int stage) {
   DeviceBuffer img(es * nn * n), raw(es * nn * n), defs(es * 2 * nn * n), stack(es * nn * n);
   double lastMean = 0;
   for (int i = 0; i < n; ++i) {
-    const Image im = loadChecked(files[i], N);
-    upload((char *)img.p + (size_t)i * es * nn, im.data.data(), nn, dt);
+    loadRaw(p, files[i], L, dt, (char *)img.p + (size_t)i * es * nn);
     std::vector<double> phi;
     loadField(deformationBase(p, dir, i, L) + "_%d.dat.bz2", N, phi);
     upload((char *)defs.p + (size_t)i * es * 2 * nn, phi.data(), 2 * nn, dt);
-    if (i == n - 1) lastMean = meanOf(im.data);
+    if (i == n - 1) {
+      std::vector<double> h(nn);
+      download(h.data(), (char *)img.p + (size_t)i * es * nn, nn, dt);
+      lastMean = meanOf(h);
+    }
   }
   hipCheck(hipMemcpy(raw.p, img.p, es * nn * n, hipMemcpyDeviceToDevice), "copy");
   std::unique_ptr<DeviceBuffer> redD;
@@ -344,14 +399,33 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
 
   DeviceBuffer tpl(es * nn * n), ref(es * nn), defs(es * 2 * nn * n), pw(es * 2 * nn * n), mean(es * nn),
       cnt(es * nn), med(es * nn);
-  for (int i = 0; i < n; ++i) {
-    const Image im = loadChecked(files[i], N);
-    upload((char *)tpl.p + (size_t)i * es * nn, im.data.data(), nn, dt);
+  // Frames of the grid's size are normalised inside the library. Larger dyadic frames and cropped
+  // ones are prepared here as the reference prepares them (normalised at their own resolution, then
+  // restricted); the reconstruction then reads the raw frames restricted alike.
+  std::vector<Image> ims;
+  for (int i = 0; i < n; ++i) ims.push_back(loadImage(files[i]));
+  const std::string refName = p.getStringExpandTilde("reference");
+  const Image refIm = loadImage(refName);
+  const bool refNoCrop = p.checkAndGetBool("dontResizeOrCropReference");
+  bool prepare = p.checkAndGetBool("cropInput") || refIm.nx != N || refIm.ny != N;
+  for (const Image &im : ims) prepare = prepare || im.nx != N || im.ny != N;
+  std::unique_ptr<DeviceBuffer> rawTpl;
+  if (!prepare) {
+    for (int i = 0; i < n; ++i) upload((char *)tpl.p + (size_t)i * es * nn, ims[i].data.data(), nn, dt);
+    upload(ref.p, refIm.data.data(), nn, dt);
+  } else {
+    const bool scale = !p.checkAndGetBool("dontNormalizeInputImages");
+    rawTpl.reset(new DeviceBuffer(es * nn * n));
+    for (int i = 0; i < n; ++i) {
+      prepareFrame(p, ims[i], files[i], L, dt, scale, true, false, (char *)tpl.p + (size_t)i * es * nn);
+      prepareFrame(p, ims[i], files[i], L, dt, false, false, false, (char *)rawTpl->p + (size_t)i * es * nn);
+    }
+    prepareFrame(p, refIm, refName, L, dt, scale, true, refNoCrop, ref.p);
+    sp.dont_normalize = 1;
+    sp.pre_smooth_sigma = 0;
+    sp.raw_templates = rawTpl->p;
   }
-  {
-    const Image r = loadChecked(p.getStringExpandTilde("reference"), N);
-    upload(ref.p, r.data.data(), nn, dt);
-  }
+  ims.clear();
   std::unique_ptr<DeviceBuffer> pin;
   if (loadStageOneResults && n > 1) {
     // pairwise deformation of template i is stage 1's deformation of its template i-1
@@ -460,7 +534,8 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
   download(hc.data(), cnt.p, nn, dt);
   download(hmed.data(), med.p, nn, dt);
   writeReconstruction(dir, hm, hmed, hc, N);
-  seriesExtras(p, stage, files, tpl.p, defs.p, sp.reduce_deformations ? reduced.p : nullptr, N, dt);
+  seriesExtras(p, stage, files, rawTpl ? rawTpl->p : tpl.p, defs.p, sp.reduce_deformations ? reduced.p : nullptr, N,
+               dt);
 }
 
 void doAction(ParameterParser &p, int action, bool loadStageOneResults, int stage) {
@@ -521,6 +596,8 @@ void runMatchSeries(ParameterParser &parser, int argc, char **argv) {
     }
     // the reference passes lambda through "%f"
     stage2.changeVariableValue("lambda", strprintf("%f", parser.getDouble("lambda") * extraStagesLambdaFactor));
+    if (parser.checkAndGetBool("cropInput") || parser.checkAndGetBool("resizeInput"))
+      stage2.changeVariableValue("dontResizeOrCropReference", 1);
     if (parser.checkAndGetBool("useCorrelationToInitTranslation"))
       stage2.changeVariableValue("useCorrelationToInitTranslation", 0);
     for (int i = 0; i < numExtraStages; ++i) {

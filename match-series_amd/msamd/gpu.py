@@ -44,7 +44,7 @@ class SeriesParams(ctypes.Structure):
         ("average_ext_mean", ctypes.c_int), ("average_nearest", ctypes.c_int),
         ("reduce_deformations", ctypes.c_int), ("reduce_max_iter", ctypes.c_int), ("reduce_stop_eps", ctypes.c_double),
         ("pair0_init", ctypes.c_void_p), ("corr_init", ctypes.c_int), ("max_corr_shift", ctypes.c_int),
-        ("calc_inverse", ctypes.c_int), ("inverse_out", ctypes.c_void_p),
+        ("calc_inverse", ctypes.c_int), ("inverse_out", ctypes.c_void_p), ("raw_templates", ctypes.c_void_p),
     ]
 
 

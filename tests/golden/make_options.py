@@ -39,6 +39,17 @@ def main():
             frames = z["frames"]
         e2e = G.run_e2e(frames, G.E2E_PAR + "calcInverseDeformation 1\n", os.path.join(tmp, "e2e_inv"))
         np.savez_compressed(os.path.join(HERE, "e2e64_inv.npz"), **e2e)
+        # frames larger than the grid (128^2 at precisionLevel 6: normalised there, then restricted)
+        # and frames cropped to it (80x72, cropInput at (5, 3))
+        from msamd import synth
+        big = synth.stem_series(128, 4, workload=0, spacing=12.0, sigma=3.0, jitter_step=0.3)
+        np.savez_compressed(os.path.join(HERE, "e2e128_to64.npz"),
+                            **G.run_e2e(big, G.E2E_PAR, os.path.join(tmp, "e2e128")))
+        crop = np.ascontiguousarray(G.synth.stem_series(96, 4, workload=0, spacing=6.0, sigma=1.5,
+                                                        jitter_step=0.15)[:, :72, :80])
+        np.savez_compressed(os.path.join(HERE, "e2e_crop64.npz"), **G.run_e2e(
+            crop, G.E2E_PAR + "cropInput 1\ncropStartX 5\ncropStartY 3\ndontResizeOrCropReference 0\n",
+            os.path.join(tmp, "e2e_crop")))
         for tag, make_par in NAMED.items():
             np.savez_compressed(os.path.join(HERE, f"e2e64_named_{tag}.npz"),
                                 **run_named(frames, make_par(G.E2E_PAR), os.path.join(tmp, f"named_{tag}")))
