@@ -165,6 +165,11 @@ int msg_series_stats(int dtype, const void *stack, int nimg, int n, double fill,
  * (aol/vec.h:864-870, 1084-1089). In place on nimg device images. */
 int msg_normalize(int dtype, void *img, int n, int nimg, double pct, int min_to_zero, void *stream);
 
+/* ScalarArray<QC_2D>::resampleFrom (quoc/scalarArray.h:308-331) of a w x h source (w, h >= 2) onto an
+ * n x n array: bilinear interpolation at x (w-1)/(n-1), y (h-1)/(n-1) ("resizeInput",
+ * registration.h:369-372). Device arrays, x fastest. */
+int msg_resample_rect(int dtype, const void *src, int w, int h, void *dst, int n, void *stream);
+
 /* qc::approxInverseDeformation (quoc/deformations.h:341-355, TransformFunction<QC_2D>::transform,
  * quoc/auxiliary.cpp:33-176): the identity pushed forward through the displacement phi (two
  * triangles per element, the first covering triangle in element order wins), minus the identity;

@@ -160,23 +160,29 @@ __device__ __forceinline__ double eval_img(const S *__restrict__ T, const Hit &p
   return w;
 }
 
-// ScalarArray<QC_2D>::interpolate (scalarArray.h:499-529) including the Y == 0 branch.
+// ScalarArray<QC_2D>::interpolate (scalarArray.h:499-529) including the Y == 0 branch, on a w x h
+// array (w, h >= 2).
 template <typename S>
-__device__ __forceinline__ double interp(const S *__restrict__ img, int n, double X, double Y) {
+__device__ __forceinline__ double interp_wh(const S *__restrict__ img, int w, int h, double X, double Y) {
   if (X != X) X = 0.;  // NaN coordinates (undefined in the reference) read inside the image
   if (Y != Y) Y = 0.;
   int xL = (int)X, yL = (int)Y;
-  if (X >= (double)(n - 1)) xL = n - 2;
-  if (Y >= (double)(n - 1)) yL = n - 2;
+  if (X >= (double)(w - 1)) xL = w - 2;
+  if (Y >= (double)(h - 1)) yL = h - 2;
   if (X < 0.) xL = 0;
   if (Y < 0.) yL = 0;
   X -= xL;
   Y -= yL;
   const int xU = xL + 1, yU = yL + 1;
-  const double v0 = ld(img, xL + yL * n), v1 = ld(img, xU + yL * n);
-  const double v2 = ld(img, xL + yU * n), v3 = ld(img, xU + yU * n);
+  const double v0 = ld(img, xL + yL * w), v1 = ld(img, xU + yL * w);
+  const double v2 = ld(img, xL + yU * w), v3 = ld(img, xU + yU * w);
   if (Y == 0) return (1 - X) * v0 + X * v1;
   return (1 - X) * (1 - Y) * v0 + X * (1 - Y) * v1 + (1 - X) * Y * v2 + X * Y * v3;
+}
+
+template <typename S>
+__device__ __forceinline__ double interp(const S *__restrict__ img, int n, double X, double Y) {
+  return interp_wh(img, n, n, X, Y);
 }
 
 

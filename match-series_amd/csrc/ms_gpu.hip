@@ -408,10 +408,11 @@ int launch_prolong(const S *coarse, S *fine, int nc, int nfields, hipStream_t s)
   return MSG_OK;
 }
 
-// Sampling positions k * (nc-1)/(n-1) of ResampleCopier, with the step held in long double.
+// Sampling positions k * (nc-1)/(n-1) of ResampleCopier (scalarArray.h:311-330): the step divided in
+// double and held in long double, each product rounded to double.
 int resample_coords(int nc, int n, double **dcoord, hipStream_t s) {
   std::vector<double> c(n);
-  const long double hx = (long double)(nc - 1) / (n - 1);
+  const long double hx = (long double)((double)(nc - 1) / (double)(n - 1));
   for (int k = 0; k < n; ++k) c[k] = (double)(k * hx);
   void *p = nullptr;
   CHK(scratch_get(15, sizeof(double) * n, &p));
@@ -2144,6 +2145,35 @@ int msg_series_stats(int dtype, const void *stack, int nimg, int n, double fill,
   if (dtype == MSG_F64) CHK(stats_impl<double>(stack, nimg, n, fill, mean, count, median, strm(stream)));
   else CHK(stats_impl<float>(stack, nimg, n, fill, mean, count, median, strm(stream)));
   HIPCHK(hipStreamSynchronize(strm(stream)));
+  return MSG_OK;
+}
+
+int msg_resample_rect(int dtype, const void *src, int w, int h, void *dst, int n, void *stream) {
+  CHK(check_dtype(dtype));
+  CHK(check_n(n));
+  if (!src || !dst || w < 2 || h < 2) return fail(MSG_ERR_ARG, "invalid argument (source of at least 2 x 2)");
+  hipStream_t s = strm(stream);
+  // ResampleCopier's positions: the steps divided in double, held in long double, products rounded
+  std::vector<double> c(2 * (size_t)n);
+  const long double hx = (long double)((double)(w - 1) / (double)(n - 1));
+  const long double hy = (long double)((double)(h - 1) / (double)(n - 1));
+  for (int k = 0; k < n; ++k) {
+    c[k] = (double)(k * hx);
+    c[n + k] = (double)(k * hy);
+  }
+  DevBuf dc;
+  HIPCHK(hipMalloc(&dc.p, sizeof(double) * 2 * n));
+  HIPCHK(hipMemcpyAsync(dc.p, c.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice, s));
+  const size_t cnt = (size_t)n * n;
+  const double *cx = (const double *)dc.p, *cy = cx + n;
+  if (dtype == MSG_F64)
+    hipLaunchKernelGGL((msg::k_resample_rect<double>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
+                       (const double *)src, w, h, (double *)dst, n, cx, cy);
+  else
+    hipLaunchKernelGGL((msg::k_resample_rect<float>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
+                       (const float *)src, w, h, (float *)dst, n, cx, cy);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));  // the coordinates are freed on return
   return MSG_OK;
 }
 

@@ -1344,6 +1344,17 @@ __global__ void k_resample(const S *__restrict__ src, S *__restrict__ dst, int n
   dst[i] = static_cast<S>(interp(src + (size_t)f * nc * nc, nc, coord[j % n], coord[j / n]));
 }
 
+// ScalarArray<QC_2D>::resampleFrom of a w x h source onto n x n nodes: node (x, y) takes the source
+// interpolated at (cx[x], cy[y]).
+template <typename S>
+__global__ void k_resample_rect(const S *__restrict__ src, int w, int h, S *__restrict__ dst, int n,
+                                const double *__restrict__ cx, const double *__restrict__ cy) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)n * n) return;
+  const int x = (int)(i % n), y = (int)(i / n);
+  dst[i] = static_cast<S>(interp_wh(src, w, h, cx[x], cy[y]));
+}
+
 // DeformImage (deformations.h:62-101; ExtendWithConstant, optional nearest-neighbour rounding),
 // FastDeformImage (L140-162; clamp) and the NaN-extended variant of L226-256 (mode 2).
 // img: nimg images; phi: one field per image (or one shared field if phi_shared).

@@ -84,7 +84,7 @@ std::string dirOf(const ParameterParser &p) { return p.getStringExpandTilde("sav
 
 // Options of the reference that change the computation but are not implemented here.
 void rejectUnsupported(const ParameterParser &p) {
-  const char *flags[] = {"reverseRolesInSeriesMatching", "resizeInput", "calcIQ", "calcPrecision",
+  const char *flags[] = {"reverseRolesInSeriesMatching", "calcIQ", "calcPrecision",
                          "performNoiseAnalysis", "saveNamedDeformedDMXTemplatesAsDMX"};
   for (const char *f : flags)
     if (p.checkAndGetBool(f)) MSH_THROW(strprintf("\"%s 1\" is not supported by this build", f));
@@ -137,8 +137,9 @@ msg_series_params seriesParams(const ParameterParser &p) {
 int fieldType(const ParameterParser &p) { return p.getIntOrDefault("gpuFieldPrecision", 32) == 64 ? MSG_F64 : MSG_F32; }
 
 // An input image as RegistrationMultilevelDescentInterface::initMultilevelArrayFromImage prepares it
-// (registration.h:330-399): cut out of the input at (cropStartX, cropStartY) with cropInput, else
-// taken at its own dyadic level k >= L; normalised there (scale: prepareMultilevelArray's
+// (registration.h:330-399): resampled onto the grid of level max(L, log2 width) with resizeInput, cut
+// out of the input at (cropStartX, cropStartY) with cropInput, else taken at its own dyadic level
+// k >= L; normalised there (scale: prepareMultilevelArray's
 // saturation clamp or min/max scaling), pre-smoothed there (smooth: preSmoothSigma in pixels of
 // that level), then restricted to level L. Writes 2^L x 2^L values of the field type to dst.
 void prepareFrame(const ParameterParser &p, const Image &im, const std::string &name, int L, int dt, bool scale,
@@ -146,7 +147,18 @@ void prepareFrame(const ParameterParser &p, const Image &im, const std::string &
   const int N = 1 << L, es = dt == MSG_F64 ? 8 : 4;
   int k = L;
   std::vector<double> full;
-  if (!noCrop && p.checkAndGetBool("cropInput")) {
+  std::unique_ptr<DeviceBuffer> cur;
+  if (!noCrop && p.checkAndGetBool("resizeInput")) {
+    int inLevel = 0;
+    while ((2 << inLevel) <= im.nx) ++inLevel;  // qc::logBaseTwo
+    k = std::max(L, inLevel);
+    const int n = 1 << k;
+    if (im.nx < 2 || im.ny < 2) MSH_THROW(strprintf("resizeInput: %s is too small", name.c_str()));
+    DeviceBuffer src((size_t)es * im.nx * im.ny);
+    upload(src.p, im.data.data(), (size_t)im.nx * im.ny, dt);
+    cur.reset(new DeviceBuffer((size_t)es * n * n));
+    msgCheck(msg_resample_rect(dt, src.p, im.nx, im.ny, cur->p, n, nullptr));
+  } else if (!noCrop && p.checkAndGetBool("cropInput")) {
     const int sx = p.getInt("cropStartX"), sy = p.getInt("cropStartY");
     const int w = std::min(im.nx, N), h = std::min(im.ny, N);
     if (sx < 0 || sy < 0 || sx + w > im.nx || sy + h > im.ny)
@@ -168,8 +180,10 @@ void prepareFrame(const ParameterParser &p, const Image &im, const std::string &
     full = im.data;
   }
   const int n = 1 << k;
-  std::unique_ptr<DeviceBuffer> cur(new DeviceBuffer((size_t)es * n * n));
-  upload(cur->p, full.data(), (size_t)n * n, dt);
+  if (!cur) {
+    cur.reset(new DeviceBuffer((size_t)es * n * n));
+    upload(cur->p, full.data(), (size_t)n * n, dt);
+  }
   if (scale)
     msgCheck(msg_normalize(dt, cur->p, n, 1,
                            p.hasVariable("enhanceContrastSaturationPercentage")
@@ -407,7 +421,7 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
   const std::string refName = p.getStringExpandTilde("reference");
   const Image refIm = loadImage(refName);
   const bool refNoCrop = p.checkAndGetBool("dontResizeOrCropReference");
-  bool prepare = p.checkAndGetBool("cropInput") || refIm.nx != N || refIm.ny != N;
+  bool prepare = p.checkAndGetBool("cropInput") || p.checkAndGetBool("resizeInput") || refIm.nx != N || refIm.ny != N;
   for (const Image &im : ims) prepare = prepare || im.nx != N || im.ny != N;
   std::unique_ptr<DeviceBuffer> rawTpl;
   if (!prepare) {

@@ -98,6 +98,7 @@ SIGNATURES = {
     "msg_time_energy_pass": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp], ctypes.c_int),
     "msg_match_series": ([ctypes.POINTER(SeriesParams), ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _dp, _vp,
                           _vp, _vp, ctypes.POINTER(SeriesLevels), _vp], ctypes.c_int),
+    "msg_resample_rect": ([ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "msg_approx_inverse": ([ctypes.c_int, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "msg_register_translation": ([ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _ip, _vp, _vp], ctypes.c_int),
     "msg_reduce_deformations": ([ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _dp,
@@ -403,6 +404,14 @@ def reduce_deformations(defs, max_iter=1000, stop_eps=5e-7):
     check(lib().msg_reduce_deformations(dtype_code(defs), _ptr(defs), n, int(np.log2(N)), max_iter, stop_eps, _ptr(phi),
                                         _dptr(e), ctypes.byref(it)))
     return phi, float(e[0]), it.value
+
+
+def resample_rect(src, n):
+    """ScalarArray<QC_2D>::resampleFrom of an (h, w) device image onto (n, n)."""
+    h, w = src.shape[-2], src.shape[-1]
+    out = _torch().empty((n, n), dtype=src.dtype, device=src.device)
+    check(lib().msg_resample_rect(dtype_code(src), _ptr(src), w, h, _ptr(out), n, _stream()))
+    return out
 
 
 def approx_inverse(phi):

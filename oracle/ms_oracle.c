@@ -112,11 +112,35 @@ double msor_interpolate(const double *img, int N, double X, double Y) {
   return (1 - X) * (1 - Y) * v0 + X * (1 - Y) * v1 + (1 - X) * Y * v2 + X * Y * v3;
 }
 
+/* The same interpolation on a w x h array (ScalarArray<QC_2D>::interpolate with numX != numY) */
+static double interpolateWH(const double *img, int w, int h, double X, double Y) {
+  int xL = (int)X, yL = (int)Y;
+  if (X >= (double)(w - 1)) xL = w - 2;
+  if (Y >= (double)(h - 1)) yL = h - 2;
+  if (X < 0.) xL = 0;
+  if (Y < 0.) yL = 0;
+  X -= xL;
+  Y -= yL;
+  const int xU = xL + 1, yU = yL + 1;
+  const double v0 = img[xL + (size_t)yL * w], v1 = img[xU + (size_t)yL * w];
+  const double v2 = img[xL + (size_t)yU * w], v3 = img[xU + (size_t)yU * w];
+  if (Y == 0) return (1 - X) * v0 + X * v1;
+  return (1 - X) * (1 - Y) * v0 + X * (1 - Y) * v1 + (1 - X) * Y * v2 + X * Y * v3;
+}
+
+void msor_resample_rect(const double *src, int w, int h, double *dst, int N) {
+  const long double hx = (long double)((double)(w - 1) / (double)(N - 1));
+  const long double hy = (long double)((double)(h - 1) / (double)(N - 1));
+  for (int y = 0; y < N; ++y)
+    for (int x = 0; x < N; ++x) dst[IDX(x, y, N)] = interpolateWH(src, w, h, (double)(x * hx), (double)(y * hy));
+}
+
 /* ScalarArray<QC_2D>::resampleFrom via ResampleCopier (scalarArray.h:308-331,1052-1060): the step
  * (n-1)/(N-1) is held in long double; the scaled coordinate is rounded to double. */
 void msor_resample(const double *src, int n, double *dst, int N) {
-  const long double hx = (long double)(n - 1) / (N - 1);
-  const long double hy = (long double)(n - 1) / (N - 1);
+  /* ResampleCopier: the step divided in double, held in long double, products rounded to double */
+  const long double hx = (long double)((double)(n - 1) / (double)(N - 1));
+  const long double hy = hx;
   for (int y = 0; y < N; ++y)
     for (int x = 0; x < N; ++x) {
       const double sx = (double)(x * hx), sy = (double)(y * hy);

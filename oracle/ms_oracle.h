@@ -119,6 +119,9 @@ void msor_phase_correlation(const double *a, const double *b, int N, int normali
  * x then y. */
 void msor_register_translation(const double *reference, const double *templ, int N, int max_shift, int *shift);
 
+/* ScalarArray<QC_2D>::resampleFrom of a w x h source onto N x N (scalarArray.h:308-331) */
+void msor_resample_rect(const double *src, int w, int h, double *dst, int N);
+
 /* qc::approxInverseDeformation (quoc/deformations.h:341-355): the identity pushed through phi by
  * TransformFunction<QC_2D>::transform (quoc/auxiliary.cpp:33-176: two triangles per element, x-fastest
  * element order, the first triangle covering a node wins), minus the identity; nodes no triangle

@@ -257,6 +257,14 @@ def register_translation(reference, templ, max_shift=0):
     return int(sh[0]), int(sh[1])
 
 
+def resample_rect(src, N):
+    """ScalarArray<QC_2D>::resampleFrom of an (h, w) image onto (N, N)."""
+    src = _f64(src)
+    out = np.zeros((N, N))
+    lib().msor_resample_rect(_p(src), src.shape[1], src.shape[0], _p(out), N)
+    return out
+
+
 def approx_inverse(phi):
     """qc::approxInverseDeformation (quoc/deformations.h:341-355) of a (2, N, N) displacement."""
     phi = _f64(phi)

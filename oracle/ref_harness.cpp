@@ -221,6 +221,13 @@ int main(int argc, char **argv) {
       toVec(&in[0], p1[0]); toVec(&in[NN], p1[1]); toVec(&in[2 * NN], p2[0]); toVec(&in[3 * NN], p2[1]);
       qc::ConcatAndSmoothlyExtendDeformations<ConfType>(p1, p2, grid, res);
       append(out, res[0]); append(out, res[1]);
+    } else if (cmd == "resize") {
+      // params: w h ; in: a w x h image ; out: its resampleFrom onto N x N
+      const int w = atoi(argv[3]), h = atoi(argv[4]);
+      qc::ScalarArray<RType, qc::QC_2D> src(w, h), dst(N, N);
+      for (int i = 0; i < w * h; ++i) src[i] = in[i];
+      dst.resampleFrom(src);
+      append(out, dst);
     } else if (cmd == "inverse") {
       // in: phi (2N^2) ; out: qc::approxInverseDeformation of phi
       aol::MultiVector<RType> phi(2, NN), inv(2, NN);

@@ -5,7 +5,9 @@
     "calcInverseDeformation 1" (its <i>-r-inv/ solves among the per-level records);
   * e2e64_named_<a|b>.npz: the same series with saveNamedDeformedTemplates (plain; extended with the
     mean and nearest-neighbour with reduceDeformations), saveNamedDeformations and
-    averageSaveIncrement 2 / 1: every extra file of both stages.
+    averageSaveIncrement 2 / 1: every extra file of both stages;
+  * e2e128_to64 / e2e_crop64 / e2e_resize64.npz: the program on frames larger than the grid, cropped
+    (cropInput) and resampled (resizeInput); resize.npz: ScalarArray<QC_2D>::resampleFrom alone.
 Runs in the build container only (needs oracle/_ref). Usage: python tests/golden/make_options.py
 """
 import os
@@ -50,6 +52,19 @@ def main():
         np.savez_compressed(os.path.join(HERE, "e2e_crop64.npz"), **G.run_e2e(
             crop, G.E2E_PAR + "cropInput 1\ncropStartX 5\ncropStartY 3\ndontResizeOrCropReference 0\n",
             os.path.join(tmp, "e2e_crop")))
+        # resizeInput: ScalarArray<QC_2D>::resampleFrom through the harness, and the program on
+        # 150x130 frames (resampled to 128^2, normalised there, restricted to 64^2)
+        rz = {}
+        rng = np.random.default_rng(77)
+        for k, (w, h, N) in enumerate(((80, 72, 64), (100, 60, 128), (33, 47, 32))):
+            src = rng.random((h, w))
+            rz[f"src{k}"] = src
+            rz[f"out{k}"] = G.harness("resize", N, [w, h], [src], tmp).reshape(N, N)
+        np.savez_compressed(os.path.join(HERE, "resize.npz"), **rz)
+        rs = np.ascontiguousarray(G.synth.stem_series(160, 4, workload=0, spacing=9.0, sigma=2.2,
+                                                      jitter_step=0.2)[:, :130, :150])
+        np.savez_compressed(os.path.join(HERE, "e2e_resize64.npz"), **G.run_e2e(
+            rs, G.E2E_PAR + "resizeInput 1\ndontResizeOrCropReference 0\n", os.path.join(tmp, "e2e_resize")))
         for tag, make_par in NAMED.items():
             np.savez_compressed(os.path.join(HERE, f"e2e64_named_{tag}.npz"),
                                 **run_named(frames, make_par(G.E2E_PAR), os.path.join(tmp, f"named_{tag}")))

@@ -111,7 +111,7 @@ def check_stage(out, g, st, precision, tol, med_tol):
 
 @pytest.mark.parametrize("fixture,precision,tol", [("e2e64", 64, 1e-6), ("e2e64", 32, 5e-3), ("e2e64_ncc", 64, 1e-6),
                                                    ("e2e64_reduce", 64, 1e-6), ("e2e128_to64", 64, 1e-6),
-                                                   ("e2e_crop64", 64, 1e-6)])
+                                                   ("e2e_crop64", 64, 1e-6), ("e2e_resize64", 64, 1e-6)])
 def test_matchseries_stage1(tmp_path, fixture, precision, tol):
     """Stage 1 from the frames; stage 2 of the same run (lambda 1: its step decisions can part from
     the reference's on inputs that differ in the last bits) is only checked for completeness.
@@ -123,7 +123,8 @@ def test_matchseries_stage1(tmp_path, fixture, precision, tol):
     reference with energies within 1.5e-5, every other solve within 1e-6 px. Hence 5e-3 px.
 
     e2e128_to64: frames of 128^2 at precisionLevel 6 (normalised at 128^2, then restricted);
-    e2e_crop64: frames of 80x72 with cropInput at (5, 3)."""
+    e2e_crop64: frames of 80x72 with cropInput at (5, 3); e2e_resize64: frames of 150x130 with
+    resizeInput (resampled to 128^2, normalised, restricted)."""
     if not os.path.exists(EXE):
         pytest.skip("bin/matchSeries not built")
     with np.load(os.path.join(GOLD, f"{fixture}.npz")) as z:

@@ -3,6 +3,8 @@
 MSG_F64 (FP64 storage) must agree with the oracle to rounding (1e-12 relative); MSG_F32 (FP32
 storage, FP64 arithmetic) to the storage precision.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -203,3 +205,13 @@ def test_normalize(gpu, pct, code, tol):
     out = host(gpu.normalize(t, pct, True))
     for k in range(2):
         assert rel(out[k], O.normalize(src[k], pct, True)) < tol
+
+
+def test_resample_rect(gpu):
+    """resizeInput's ScalarArray<QC_2D>::resampleFrom of non-square sources (tests/golden/resize.npz,
+    written by the reference): bit for bit in FP64."""
+    with np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "resize.npz")) as z:
+        for k in range(3):
+            ref = z[f"out{k}"]
+            src = torch.tensor(z[f"src{k}"], dtype=torch.float64, device="cuda")
+            np.testing.assert_array_equal(gpu.resample_rect(src, ref.shape[0]).cpu().numpy(), ref)
