@@ -90,7 +90,10 @@ void rejectUnsupported(const ParameterParser &p) {
     if (p.checkAndGetBool(f)) MSH_THROW(strprintf("\"%s 1\" is not supported by this build", f));
   if (p.hasVariable("altStartLevel") && p.getInt("altStartLevel") != p.getInt("startLevel"))
     MSH_THROW("\"altStartLevel\" is not supported by this build");
-  if (p.hasVariable("initial-deformation")) MSH_THROW("\"initial-deformation\" is not supported by this build");
+  // "initial-deformation": the registration object loads it (registration.h:495-500), and the series
+  // loop starts every pairing from zero regardless (matchSeries.h:111, 137): only the loading shows
+  if (p.hasVariable("initial-deformation"))
+    for (int c = 0; c < 2; ++c) (void)loadImage(p.getString("initial-deformation", c));
   // the reference itself does not implement a positive threshold (registration.h:604-610)
   if (p.getDoubleOrDefault("averageXDerivNormThreshold", 0) > 0) MSH_THROW("averageXDerivNormThreshold: not implemented");
   if (p.hasVariable("minimizationAlgo") && p.getString("minimizationAlgo") != "GradientDescent")

@@ -130,3 +130,12 @@ def test_png_preview_matches_the_reference(tmp_path, name):
     assert r.returncode == 0, r.stderr
     np.testing.assert_array_equal(read_png_gray8(out), png)
 
+
+
+def test_cli_loads_initial_deformation(tmp_path):
+    # "initial-deformation" is read by the registration object and then unused by the series loop
+    # (registration.h:495-500, matchSeries.h:111): a missing file is an error, an existing one is not
+    with np.load(os.path.join(GOLD, "e2e64.npz")) as z:
+        par = bytes(z["par"]).decode()
+    r = cli(tmp_path, par + "initial-deformation { nothere_0.dat.bz2 nothere_1.dat.bz2 }\n", strict=True)
+    assert r.returncode == 1 and "nothere_0.dat.bz2" in r.stderr
