@@ -223,6 +223,13 @@ typedef struct {
                                       /* two images' roles swapped, started from its          */
                                       /* approximate inverse (matchSeries.h:247-259)          */
   void *inverse_out;                  /* NULL or n device fields: those inverse solves' result*/
+  int alt_start_level;                /* "altStartLevel" (0, or start_level: none): every     */
+  int energy_as_alt_criterion;        /* pairwise solve is repeated from zero at that level;  */
+                                      /* the first result is kept when it leaves fewer        */
+                                      /* template pixels outside the domain (or, with         */
+                                      /* "energyAsAltCriterion", has the lower energy)        */
+                                      /* (matchSeries.h:158-203)                              */
+  void *first_try_out;                /* NULL or n device fields: the start_level solves      */
   const void *raw_templates;          /* NULL, or n device images the reconstruction warps    */
                                       /* instead of `templates` (when the caller prepared the */
                                       /* matching inputs itself: normalised at their own      */
@@ -236,7 +243,8 @@ typedef struct {
  * Records are ordered by solve — the pairwise solves i = 0..npw-1 (npw = 1 with pairwise_in,
  * else n) over levels start..stop, then the refinements i = 1..n-1 over
  * refine_start..refine_stop (none with dont_accumulate), then with calc_inverse the inverse solves
- * i = 0..n-1 over refine_start..refine_stop — and by ascending level within a solve.
+ * i = 0..n-1 over refine_start..refine_stop, then with alt_start_level the repeated pairwise solves
+ * i = 0..npw-1 over alt_start_level..stop — and by ascending level within a solve.
  * Every pointer may be NULL. All host memory. */
 typedef struct {
   double *fields;         /* 2*N_l^2 doubles per record (x component, then y), back to back    */

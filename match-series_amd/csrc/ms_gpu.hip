@@ -1818,7 +1818,8 @@ int inverse_impl(const S *phi, S *out, int n, hipStream_t s) {
 // (matchSeries.h:136-176). Writes the fields to pw and the energies.
 template <typename S>
 int pairwise_batch(const msg_series_params *p, const msg_solve_params &sp, const S *tpl, const S *ref, int i0,
-                   int i1, S *pw, double *energies, const msg_series_levels *lv, const S *init0) {
+                   int i1, S *pw, double *energies, const msg_series_levels *lv, const S *init0, S *first,
+                   long long alt_rec0, size_t alt_field0) {
   const int L = p->precision_level;
   const int N = 1 << L;
   const size_t nn = (size_t)N * N;
@@ -1876,6 +1877,50 @@ int pairwise_batch(const msg_series_params *p, const msg_solve_params &sp, const
   lo.rec0 = (long long)i0 * (p->stop_level - p->start_level + 1);
   lo.field0 = (size_t)i0 * level_block(p->start_level, p->stop_level);
   CHK(solve_impl<S>(pc, p->start_level, p->stop_level, &sp, energies + i0, &lo));
+  if (first) HIPCHK(hipMemcpyAsync(first + (size_t)i0 * 2 * nn, pc->phi[L], sizeof(S) * 2 * nn * nb,
+                                   hipMemcpyDeviceToDevice, s));
+  if (p->alt_start_level > 0 && p->alt_start_level != p->start_level) {
+    // altStartLevel (matchSeries.h:158-203): the solve repeated from zero at alt_start_level; per pair
+    // the first result is kept if it leaves fewer template pixels outside the domain (DeformImage with
+    // +Inf outside), or with energy_as_alt_criterion if its energy is lower
+    auto outside = [&](std::vector<long long> &cnt) -> int {
+      DevBuf w;
+      HIPCHK(hipMalloc(&w.p, sizeof(S) * nn * nb));
+      const size_t c = nn * nb;
+      hipLaunchKernelGGL((msg::k_warp<S>), dim3((unsigned)((c + 255) / 256)), dim3(256), 0, s, (const S *)pc->T[L],
+                         (const S *)pc->phi[L], (S *)w.p, N, nb, 0, 0, (double)INFINITY, 0);
+      HIPCHK(hipGetLastError());
+      std::vector<S> h(c);
+      HIPCHK(hipMemcpyAsync(h.data(), w.p, sizeof(S) * c, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      cnt.assign(nb, 0);
+      for (size_t k = 0; k < c; ++k) cnt[k / nn] += std::isinf(h[k]) && h[k] > 0;
+      return MSG_OK;
+    };
+    std::vector<long long> out1, out2;
+    CHK(outside(out1));
+    const std::vector<double> e1(energies + i0, energies + i1);
+    DevBuf keep;
+    HIPCHK(hipMalloc(&keep.p, sizeof(S) * 2 * nn * nb));
+    HIPCHK(hipMemcpyAsync(keep.p, pc->phi[L], sizeof(S) * 2 * nn * nb, hipMemcpyDeviceToDevice, s));
+    for (int l = 0; l <= L; ++l)
+      HIPCHK(hipMemsetAsync(pc->phi[l], 0, sizeof(S) * ((size_t)2 << (2 * l)) * nb, s));
+    const int na = p->stop_level - p->alt_start_level + 1;
+    LevelOut la;
+    la.lv = lv;
+    la.rec0 = alt_rec0 + (long long)i0 * na;
+    la.field0 = alt_field0 + (size_t)i0 * level_block(p->alt_start_level, p->stop_level);
+    std::vector<double> e2(nb);
+    CHK(solve_impl<S>(pc, p->alt_start_level, p->stop_level, &sp, e2.data(), &la));
+    CHK(outside(out2));
+    for (int k = 0; k < nb; ++k) {
+      const bool keepFirst = p->energy_as_alt_criterion ? e1[k] < e2[k] : out1[k] < out2[k];
+      if (keepFirst)
+        HIPCHK(hipMemcpyAsync((S *)pc->phi[L] + (size_t)k * 2 * nn, (const S *)keep.p + (size_t)k * 2 * nn,
+                              sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, s));
+      energies[i0 + k] = keepFirst ? e1[k] : e2[k];
+    }
+  }
   HIPCHK(hipMemcpyAsync(pw + (size_t)i0 * 2 * nn, pc->phi[L], sizeof(S) * 2 * nn * nb, hipMemcpyDeviceToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
   return MSG_OK;
@@ -1884,14 +1929,16 @@ int pairwise_batch(const msg_series_params *p, const msg_solve_params &sp, const
 // Runs the batches [k*b, min((k+1)*b, npw)) in order, publishing progress after each.
 template <typename S>
 void pairwise_all(const msg_series_params *p, const msg_solve_params &sp, const S *tpl, const S *ref, int npw,
-                  int b, S *pw, double *energies, const msg_series_levels *lv, const S *init0, Progress *pr) {
+                  int b, S *pw, double *energies, const msg_series_levels *lv, const S *init0, Progress *pr, S *first,
+                  long long alt_rec0, size_t alt_field0) {
   int rc = MSG_OK;
   for (int i0 = 0; i0 < npw && rc == MSG_OK; i0 += b) {
     {
       std::lock_guard<std::mutex> g(pr->mu);
       if (pr->abort) break;
     }
-    rc = pairwise_batch<S>(p, sp, tpl, ref, i0, std::min(i0 + b, npw), pw, energies, lv, init0);
+    rc = pairwise_batch<S>(p, sp, tpl, ref, i0, std::min(i0 + b, npw), pw, energies, lv, init0, first, alt_rec0,
+                           alt_field0);
     std::lock_guard<std::mutex> g(pr->mu);
     if (rc == MSG_OK) pr->done = std::min(i0 + b, npw);
     else { pr->rc = rc; pr->err = g_err; }
@@ -1944,6 +1991,24 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
                      hipMemcpyDeviceToDevice));
   }
 
+  // the start_level results of the pairwise solves (altStartLevel replaces some of them); record
+  // blocks of the repeated solves come after the pairwise, refine and inverse ones
+  const bool alt = p->alt_start_level > 0 && p->alt_start_level != p->start_level;
+  DevBuf firstBuf;
+  S *firstp = (S *)p->first_try_out;
+  if (!firstp && alt) {
+    HIPCHK(hipMalloc(&firstBuf.p, sizeof(S) * 2 * nn * n));
+    firstp = (S *)firstBuf.p;
+  }
+  const int nlp = p->stop_level - p->start_level + 1, nlr = p->refine_stop_level - p->refine_start_level + 1;
+  const long long nref = chain ? (long long)(n - 1) + (p->calc_inverse ? n : 0) : 0;
+  const long long alt_rec0 = (long long)npw * nlp + nref * nlr;
+  const size_t alt_field0 = (size_t)npw * level_block(p->start_level, p->stop_level) +
+                            (size_t)nref * (chain ? level_block(p->refine_start_level, p->refine_stop_level) : 0);
+  if (firstp && reuse && n > 1)  // frames taken from the previous stage were not solved again
+    HIPCHK(hipMemcpy(firstp + 2 * nn, (const S *)pairwise_in + 2 * nn, sizeof(S) * 2 * nn * (n - 1),
+                     hipMemcpyDeviceToDevice));
+
   Progress pr;
   if (overlap) {
     int dev = 0;
@@ -1958,10 +2023,10 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
         pr.cv.notify_all();
         return;
       }
-      pairwise_all<S>(p, sp, tpl, ref, npw, b, pwp, energies, lv, init0, &pr);
+      pairwise_all<S>(p, sp, tpl, ref, npw, b, pwp, energies, lv, init0, &pr, firstp, alt_rec0, alt_field0);
     });
   } else {
-    pairwise_all<S>(p, sp, tpl, ref, npw, b, pwp, energies, lv, init0, &pr);
+    pairwise_all<S>(p, sp, tpl, ref, npw, b, pwp, energies, lv, init0, &pr, firstp, alt_rec0, alt_field0);
     if (pr.rc != MSG_OK) return fail(pr.rc, pr.err);
   }
   // Waits (on the calling thread) until pairwise solve i is done; false after a batch failed.
@@ -1983,6 +2048,10 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
   S *defs = (S *)deformations;
   if (!chain) {
     HIPCHK(hipMemcpy(defs, pwp, sizeof(S) * 2 * nn * n, hipMemcpyDeviceToDevice));
+    // with altStartLevel the reference averages frames i > 0 with the start_level results
+    // (createDeformationBaseFileName, matchSeries.h:55-62)
+    if (alt && n > 1)
+      HIPCHK(hipMemcpy(defs + 2 * nn, firstp + 2 * nn, sizeof(S) * 2 * nn * (n - 1), hipMemcpyDeviceToDevice));
   } else {
     // --- sequential chain: accumulate by composition, refine against the reference ---
     hipStream_t rs = rc->stream;

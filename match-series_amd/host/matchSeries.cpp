@@ -88,8 +88,6 @@ void rejectUnsupported(const ParameterParser &p) {
                          "performNoiseAnalysis", "saveNamedDeformedDMXTemplatesAsDMX"};
   for (const char *f : flags)
     if (p.checkAndGetBool(f)) MSH_THROW(strprintf("\"%s 1\" is not supported by this build", f));
-  if (p.hasVariable("altStartLevel") && p.getInt("altStartLevel") != p.getInt("startLevel"))
-    MSH_THROW("\"altStartLevel\" is not supported by this build");
   // "initial-deformation": the registration object loads it (registration.h:495-500), and the series
   // loop starts every pairing from zero regardless (matchSeries.h:111, 137): only the loading shows
   if (p.hasVariable("initial-deformation"))
@@ -98,6 +96,10 @@ void rejectUnsupported(const ParameterParser &p) {
   if (p.getDoubleOrDefault("averageXDerivNormThreshold", 0) > 0) MSH_THROW("averageXDerivNormThreshold: not implemented");
   if (p.hasVariable("minimizationAlgo") && p.getString("minimizationAlgo") != "GradientDescent")
     MSH_THROW("only \"minimizationAlgo GradientDescent\" is supported by this build");
+}
+
+bool useAltStartLevel(const ParameterParser &p) {  // SeriesMatching::_useAltStartLevel (matchSeries.h:45)
+  return p.hasVariable("altStartLevel") && p.getInt("altStartLevel") != p.getInt("startLevel");
 }
 
 msg_series_params seriesParams(const ParameterParser &p) {
@@ -134,6 +136,10 @@ msg_series_params seriesParams(const ParameterParser &p) {
   // after each frame, the inverse problem solved from the approximate inverse (matchSeries.h:247-259)
   s.calc_inverse = p.checkAndGetBool("calcInverseDeformation");
   s.inverse_out = nullptr;
+  // a second pairwise solve from altStartLevel, the better one kept (matchSeries.h:158-203)
+  s.alt_start_level = useAltStartLevel(p) ? p.getInt("altStartLevel") : 0;
+  s.energy_as_alt_criterion = p.checkAndGetBool("energyAsAltCriterion");
+  s.first_try_out = nullptr;
   return s;
 }
 
@@ -210,6 +216,7 @@ void loadRaw(const ParameterParser &p, const std::string &f, int L, int dt, void
 
 // <dir><i>[-r]/deformation_<LL> (SeriesMatching::createDeformationBaseFileName, matchSeries.h:58-67)
 std::string deformationBase(const ParameterParser &p, const std::string &dir, int i, int maxDepth) {
+  if (useAltStartLevel(p) && i == 0) return dir + "deformation-000";
   const bool refined = !p.checkAndGetBool("dontAccumulateDeformation") && i > 0;
   return strprintf("%s%d%s/deformation_%02d", dir.c_str(), i, refined ? "-r" : "",
                    refined ? maxDepth : p.getInt("stopLevel"));
@@ -449,7 +456,11 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
     pin.reset(new DeviceBuffer(es * 2 * nn * n));
     for (int i = 1; i < n; ++i) {
       std::vector<double> phi;
-      loadField(strprintf("%s../stage1/%d/deformation_%02d_%%d.dat.bz2", dir.c_str(), i - 1, L), N, phi);
+      // with altStartLevel the kept result, else the start-level solve's (matchSeries.h:206-211)
+      if (useAltStartLevel(p))
+        loadField(strprintf("%s../stage1/deformation-%03d_%%d.dat.bz2", dir.c_str(), i - 1), N, phi);
+      else
+        loadField(strprintf("%s../stage1/%d/deformation_%02d_%%d.dat.bz2", dir.c_str(), i - 1, L), N, phi);
       upload((char *)pin->p + (size_t)i * es * 2 * nn, phi.data(), 2 * nn, dt);
     }
   }
@@ -459,12 +470,19 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
   const int npw = pin ? 1 : n;
   const int nlp = sp.stop_level - sp.start_level + 1, nlr = sp.refine_stop_level - sp.refine_start_level + 1;
   const bool inverse = accumulate && sp.calc_inverse;
-  const int nrec = npw * nlp + (accumulate ? (n - 1) * nlr : 0) + (inverse ? n * nlr : 0);
+  const int nla = sp.alt_start_level > 0 ? sp.stop_level - sp.alt_start_level + 1 : 0;
+  const int nrec = npw * nlp + (accumulate ? (n - 1) * nlr : 0) + (inverse ? n * nlr : 0) + npw * nla;
   size_t nfld = 0;
   for (int l = sp.start_level; l <= sp.stop_level; ++l) nfld += (size_t)npw * (2u << (2 * l));
   if (accumulate)
     for (int l = sp.refine_start_level; l <= sp.refine_stop_level; ++l)
       nfld += (size_t)(n - 1 + (inverse ? n : 0)) * (2u << (2 * l));
+  for (int l = sp.alt_start_level; nla > 0 && l <= sp.stop_level; ++l) nfld += (size_t)npw * (2u << (2 * l));
+  std::unique_ptr<DeviceBuffer> first;
+  if (nla > 0) {
+    first.reset(new DeviceBuffer(es * 2 * nn * n));
+    sp.first_try_out = first->p;
+  }
   const int maxIt = std::max(sp.max_iter, 0);
   std::vector<double> lvFields(nfld), lvStart(2 * nrec);
   std::vector<msg_iter_trace> lvIters((size_t)std::max(nrec * maxIt, 1));
@@ -477,9 +495,10 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::cerr << "Registered " << n << " templates on the GPU in " << secs << " s\n";
 
-  std::vector<double> hdefs(2 * nn * n), hpw(2 * nn * n);
+  std::vector<double> hdefs(2 * nn * n), hpw(2 * nn * n), hfirst(first ? 2 * nn * n : 0);
   download(hdefs.data(), defs.p, 2 * nn * n, dt);
   download(hpw.data(), pw.p, 2 * nn * n, dt);
+  if (first) download(hfirst.data(), first->p, 2 * nn * n, dt);
   // Writes the level records r0 .. r0+nl-1 of one solve into `sub` (deformation_<LL>_<c>.dat.bz2 and
   // energy_<LL>.txt as GradientDescentBase::writeEnergy prints them: "k lambda*E_reg E", k from -1).
   size_t fldOff = 0;
@@ -515,13 +534,15 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
   for (int i = 0; i < n; ++i) {
     const double *phw = hpw.data() + (size_t)i * 2 * nn;
     if (!loadStageOneResults || i == 0) {
+      // the start-level solve's result (with altStartLevel the kept one may be the repeated solve's)
+      const double *phs = first ? hfirst.data() + (size_t)i * 2 * nn : phw;
       const std::string sub = strprintf("%s%d/", dir.c_str(), i);
-      if (sp.stop_level < L) saveField(sub + strprintf("deformation_%02d_%%d.dat.bz2", L), phw, N);
+      if (sp.stop_level < L) saveField(sub + strprintf("deformation_%02d_%%d.dat.bz2", L), phs, N);
       if (!loadStageOneResults) {
         double norm = 0;
         for (int c = 0; c < 2; ++c) {
           double s = 0;
-          for (size_t k = 0; k < nn; ++k) s += phw[c * nn + k] * phw[c * nn + k];
+          for (size_t k = 0; k < nn; ++k) s += phs[c * nn + k] * phs[c * nn + k];
           norm += std::sqrt(s);
         }
         defNorms << i << " " << norm << " # " << files[i] << "\n";
@@ -536,10 +557,16 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
     }
     energiesFile << i << " " << energies[i] << "\n";
   }
-  for (int i = 0; inverse && i < n; ++i) {  // the inverse solves' records come last
+  for (int i = 0; inverse && i < n; ++i) {  // then the inverse solves' records
     const std::string sub = strprintf("%s%d-r-inv/", dir.c_str(), i);
     makeDirectory(sub);
     writeLevels(sub, sp.refine_start_level, refineStop);
+  }
+  for (int i = 0; nla > 0 && i < npw; ++i) {  // the repeated solves' records and the kept results
+    const std::string sub = strprintf("%s%d-alt/", dir.c_str(), i);
+    makeDirectory(sub);
+    writeLevels(sub, sp.alt_start_level, sp.stop_level);
+    saveField(strprintf("%sdeformation-%03d_%%d.dat.bz2", dir.c_str(), i), hpw.data() + (size_t)i * 2 * nn, N);
   }
   if (sp.reduce_deformations) {  // "<saveDirectory>reduceDef_<c>.dat.bz2" (matchSeries.h:701)
     std::vector<double> hr(2 * nn);

@@ -44,7 +44,9 @@ class SeriesParams(ctypes.Structure):
         ("average_ext_mean", ctypes.c_int), ("average_nearest", ctypes.c_int),
         ("reduce_deformations", ctypes.c_int), ("reduce_max_iter", ctypes.c_int), ("reduce_stop_eps", ctypes.c_double),
         ("pair0_init", ctypes.c_void_p), ("corr_init", ctypes.c_int), ("max_corr_shift", ctypes.c_int),
-        ("calc_inverse", ctypes.c_int), ("inverse_out", ctypes.c_void_p), ("raw_templates", ctypes.c_void_p),
+        ("calc_inverse", ctypes.c_int), ("inverse_out", ctypes.c_void_p),
+        ("alt_start_level", ctypes.c_int), ("energy_as_alt_criterion", ctypes.c_int), ("first_try_out", ctypes.c_void_p),
+        ("raw_templates", ctypes.c_void_p),
     ]
 
 
@@ -391,6 +393,8 @@ def series_level_layout(params, n, reuse=False):
         if params.calc_inverse:
             recs += [("inverse", i, l) for i in range(n)
                      for l in range(params.refine_start_level, params.refine_stop_level + 1)]
+    if params.alt_start_level > 0 and params.alt_start_level != params.start_level:
+        recs += [("alt", i, l) for i in range(npw) for l in range(params.alt_start_level, params.stop_level + 1)]
     return recs
 
 
@@ -447,13 +451,16 @@ def match_series(templates, reference, params, pairwise_in=None, want_pairwise=F
     red = torch.zeros((2, N, N), dtype=templates.dtype, device=templates.device) if params.reduce_deformations else None
     inv = torch.zeros_like(defs) if params.calc_inverse and not params.dont_accumulate else None
     params.inverse_out = ctypes.c_void_p(inv.data_ptr()) if inv is not None else None
+    first = torch.zeros_like(defs) if params.alt_start_level > 0 else None
+    params.first_try_out = ctypes.c_void_p(first.data_ptr()) if first is not None else None
     check(lib().msg_match_series(ctypes.byref(params), dtype_code(templates), _ptr(templates), n, _ptr(reference),
                                  _ptr(pairwise_in) if pairwise_in is not None else None, _ptr(defs),
                                  _ptr(pw) if pw is not None else None, _dptr(en), _ptr(mean), _ptr(cnt), _ptr(med),
                                  ctypes.byref(lv) if lv is not None else None, _ptr(red) if red is not None else None))
     params.inverse_out = None
+    params.first_try_out = None
     out = dict(deformations=defs, pairwise=pw, energies=en, mean=mean, count=cnt, median=med, reduced=red,
-               inverses=inv)
+               inverses=inv, first_try=first)
     if lv is not None:
         levels, off = [], 0
         tr = np.ctypeslib.as_array(iters).view(np.float64).reshape(-1, 8) if mi > 0 else np.zeros((0, 8))

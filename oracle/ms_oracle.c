@@ -1394,6 +1394,29 @@ int msor_match_series(const double *templates, int n, const double *reference, i
     }
     if (pairwise_in == NULL || i == 0) {
       energies[i] = solveML(&ref, &tpl, &def, p->start_level, p->stop_level, p);
+      if (p->first_try_out) memcpy(p->first_try_out + i * 2 * NN, def.lev[L], sizeof(double) * 2 * NN);
+      if (p->alt_start_level > 0 && p->alt_start_level != p->start_level) {
+        /* altStartLevel (matchSeries.h:158-203): a second solve from zero at altStartLevel; the first
+         * is kept when it leaves fewer template pixels outside the domain (or, with
+         * energyAsAltCriterion, has the lower energy) */
+        double *first = (double *)malloc(sizeof(double) * 2 * NN), *warped = (double *)malloc(sizeof(double) * NN);
+        memcpy(first, def.lev[L], sizeof(double) * 2 * NN);
+        const double e1 = energies[i];
+        msor_deform_image(tpl.lev[L], first, N, INFINITY, 0, warped);
+        long out1 = 0, out2 = 0;
+        for (size_t k = 0; k < NN; ++k) out1 += isinf(warped[k]) && warped[k] > 0;
+        double *zero = (double *)calloc(2 * NN, sizeof(double));
+        defSet(&def, zero);
+        free(zero);
+        const double e2 = solveML(&ref, &tpl, &def, p->alt_start_level, p->stop_level, p);
+        msor_deform_image(tpl.lev[L], def.lev[L], N, INFINITY, 0, warped);
+        for (size_t k = 0; k < NN; ++k) out2 += isinf(warped[k]) && warped[k] > 0;
+        const int keepFirst = p->energy_as_alt_criterion ? (e1 < e2) : (out1 < out2);
+        if (keepFirst) defSet(&def, first);
+        energies[i] = keepFirst ? e1 : e2;
+        free(first);
+        free(warped);
+      }
     } else {
       defSet(&def, pairwise_in + i * 2 * NN); /* loadTransformation of the previous stage */
     }

@@ -110,6 +110,9 @@ typedef struct {
   int max_corr_shift;                 /* maxCorrShift                                         */
   int calc_inverse;                   /* calcInverseDeformation                               */
   double *inverse_out;                /* NULL or n fields: the inverse solves' deformations   */
+  int alt_start_level;                /* altStartLevel (0: none, or equal to start_level)     */
+  int energy_as_alt_criterion;        /* energyAsAltCriterion                                 */
+  double *first_try_out;              /* NULL or n fields: the start_level solves' results    */
 } msor_series_params;
 
 /* Convolution::phaseCorrelation (quoc/convolution.h:1049-1114): the inverse FFT of

@@ -37,6 +37,8 @@ class SeriesParams(ctypes.Structure):
         ("reduce_deformations", ctypes.c_int), ("reduce_max_iter", ctypes.c_int), ("reduce_stop_eps", ctypes.c_double),
         ("pair0_init", ctypes.POINTER(ctypes.c_double)), ("corr_init", ctypes.c_int), ("max_corr_shift", ctypes.c_int),
         ("calc_inverse", ctypes.c_int), ("inverse_out", ctypes.POINTER(ctypes.c_double)),
+        ("alt_start_level", ctypes.c_int), ("energy_as_alt_criterion", ctypes.c_int),
+        ("first_try_out", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -291,9 +293,12 @@ def match_series(templates, reference, params, pairwise_in=None, pair0_init=None
     red = np.zeros((2, N, N))
     inv = np.zeros((n, 2, N, N)) if params.calc_inverse and not params.dont_accumulate else None
     params.inverse_out = _p(inv) if inv is not None else None
+    first = np.zeros((n, 2, N, N))
+    params.first_try_out = _p(first)
     lib().msor_match_series(_p(T), n, _p(ref), N, ctypes.byref(params), pin, _p(defs), _p(pw), _p(en),
                             _p(mean), _p(cnt), _p(med), _p(red))
     params.pair0_init = None
     params.inverse_out = None
+    params.first_try_out = None
     return dict(deformations=defs, pairwise=pw, energies=en, mean=mean, count=cnt, median=med,
-                reduced=red if params.reduce_deformations else None, inverses=inv)
+                reduced=red if params.reduce_deformations else None, inverses=inv, first_try=first)

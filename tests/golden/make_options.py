@@ -7,7 +7,9 @@
     mean and nearest-neighbour with reduceDeformations), saveNamedDeformations and
     averageSaveIncrement 2 / 1: every extra file of both stages;
   * e2e128_to64 / e2e_crop64 / e2e_resize64.npz: the program on frames larger than the grid, cropped
-    (cropInput) and resampled (resizeInput); resize.npz: ScalarArray<QC_2D>::resampleFrom alone.
+    (cropInput) and resampled (resizeInput); resize.npz: ScalarArray<QC_2D>::resampleFrom alone;
+  * e2e64_alt / e2e64_alt_energy.npz: altStartLevel 5 (chosen by overlap / energyAsAltCriterion), with
+    the chosen deformation-<iii> files and the <i>-alt/ records.
 Runs in the build container only (needs oracle/_ref). Usage: python tests/golden/make_options.py
 """
 import os
@@ -65,6 +67,19 @@ def main():
                                                       jitter_step=0.2)[:, :130, :150])
         np.savez_compressed(os.path.join(HERE, "e2e_resize64.npz"), **G.run_e2e(
             rs, G.E2E_PAR + "resizeInput 1\ndontResizeOrCropReference 0\n", os.path.join(tmp, "e2e_resize")))
+        # altStartLevel: a second pairwise solve from level 5, chosen by overlap (and by energy)
+        for tag, extra in (("alt", "altStartLevel 5\n"), ("alt_energy", "altStartLevel 5\nenergyAsAltCriterion 1\n")):
+            wd = os.path.join(tmp, tag)
+            e2e = G.run_e2e(frames, G.E2E_PAR + extra, wd)
+            for st in (1, 2):
+                sd = os.path.join(wd, "out", f"stage{st}")
+                for fn in sorted(os.listdir(sd)):
+                    if fn.startswith("deformation-") and fn.endswith("_0.dat.bz2"):
+                        e2e[f"s{st}_chosen_{fn[len('deformation-'):len('deformation-') + 3]}"] = np.stack(
+                            [G.read_p9(os.path.join(sd, fn.replace("_0.dat", f"_{c}.dat"))) for c in range(2)])
+                e2e[f"s{st}_defNorms"] = np.loadtxt(os.path.join(sd, "defNorms.txt"), usecols=(0, 1)) \
+                    if os.path.exists(os.path.join(sd, "defNorms.txt")) else np.zeros((0, 2))
+            np.savez_compressed(os.path.join(HERE, f"e2e64_{tag}.npz"), **e2e)
         for tag, make_par in NAMED.items():
             np.savez_compressed(os.path.join(HERE, f"e2e64_named_{tag}.npz"),
                                 **run_named(frames, make_par(G.E2E_PAR), os.path.join(tmp, f"named_{tag}")))

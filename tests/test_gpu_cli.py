@@ -193,6 +193,41 @@ def test_matchseries_named_outputs_and_increments(tmp_path, tag):
                 assert np.median(d) < 1e-3 * scale and np.mean(d < 0.05 * scale) > 0.99, k
 
 
+@pytest.mark.parametrize("fixture", ["e2e64_alt", "e2e64_alt_energy"])
+def test_matchseries_alt_start_level(tmp_path, fixture):
+    """altStartLevel 5 (kept by overlap, or by energy with energyAsAltCriterion): stage 1's kept
+    deformation-<iii> files, the <i>/ and <i>-alt/ per-level records, the refined deformations and
+    energies against the reference program's; stage 2 reuses the kept files."""
+    if not os.path.exists(EXE):
+        pytest.skip("bin/matchSeries not built")
+    with np.load(os.path.join(GOLD, f"{fixture}.npz")) as z:
+        g = {k: z[k] for k in z.files}
+    out = run_cli(tmp_path, g, 64)
+    s1 = out / "stage1"
+    for i in range(3):
+        d = np.stack([read_p9(str(s1 / f"deformation-{i:03d}_{c}.dat.bz2")) for c in range(2)])
+        assert np.abs(d - g[f"s1_chosen_{i:03d}"]).max() * 63 < 1e-6, i
+    for i in (1, 2):
+        d = np.stack([read_p9(str(s1 / f"{i}-r" / f"deformation_06_{c}.dat.bz2")) for c in range(2)])
+        assert np.abs(d - g["s1_deformations"][i]).max() * 63 < 1e-6, i
+    subs = [k for k in g if k.startswith("s1_") and "_energy_" in k]
+    assert any("_alt_" in k for k in subs)
+    for k in subs:
+        sub, lv = k[len("s1_"):].split("_energy_")
+        sd = s1 / sub.replace("_", "-")
+        ref = g[k]
+        mine = np.loadtxt(str(sd / f"energy_{lv}.txt")).reshape(-1, ref.shape[1])
+        np.testing.assert_array_equal(mine[:, 0], ref[:, 0])
+        np.testing.assert_allclose(mine[:, -1], ref[:, -1], rtol=2e-5, err_msg=k)
+        m = 1 << int(lv)
+        d = np.stack([read_p9(str(sd / f"deformation_{lv}_{c}.dat.bz2")) for c in range(2)])
+        assert np.abs(d - g[k.replace("_energy_", "_deformation_")]).max() * (m - 1) < 1e-6, k
+    np.testing.assert_allclose(np.loadtxt(str(s1 / "energies.txt"))[:, 1], g["s1_energies"], rtol=1e-4)
+    np.testing.assert_allclose(np.loadtxt(str(s1 / "defNorms.txt"), usecols=(0, 1)), g["s1_defNorms"], rtol=1e-5)
+    assert (out / "stage2" / "deformation-000_1.dat.bz2").exists()
+    assert (out / "stage2" / "median.q2bz").exists()
+
+
 def test_matchseries_inverse_deformation(tmp_path):
     """calcInverseDeformation 1: the <i>-r-inv/ solves of stage 1 (per-level deformations and energy
     logs) against the reference program's (tests/golden/e2e64_inv.npz), FP64 fields."""

@@ -253,3 +253,25 @@ def test_match_series_reduce_f64(gpu):
     assert np.abs(out["reduced"].cpu().numpy() - ref["reduced"]).max() * (N - 1) < 1e-6
     np.testing.assert_array_equal(out["count"].cpu().numpy(), ref["count"])
     np.testing.assert_allclose(out["median"].cpu().numpy(), ref["median"], rtol=1e-9, atol=1e-10)
+
+
+@pytest.mark.parametrize("energy_criterion", [0, 1])
+def test_match_series_alt_start_level(gpu, energy_criterion):
+    """altStartLevel: each pairwise solve repeated from zero at level 5 and the better one kept (by the
+    template's overlap with the domain, or by energy), against the oracle's series loop."""
+    from msamd import synth
+    N = 64
+    frames = synth.stem_series(N, 4, workload=0, spacing=6.0, sigma=1.5, jitter_step=0.15)
+    kw = dict(start_level=4, stop_level=6, refine_start_level=5, refine_stop_level=6, precision_level=6,
+              lambda_=10.0, lambda_factor=-1, max_iter=15, stop_eps=-1.0, saturation_pct=0.15,
+              normalize_min_to_zero=1, dont_normalize=0, resample_prolong=1, dont_accumulate=0, data_term=0,
+              alt_start_level=5, energy_as_alt_criterion=energy_criterion)
+    ref = O.match_series(frames[1:], frames[0], O.SeriesParams(**kw))
+    t = torch.tensor(frames, dtype=torch.float64, device="cuda")
+    out = gpu.match_series(t[1:].contiguous(), t[0].contiguous(), gpu.SeriesParams(spec=0, **kw), want_pairwise=True,
+                           want_levels=True)
+    assert np.abs(out["pairwise"].cpu().numpy() - ref["pairwise"]).max() * (N - 1) < 1e-6
+    assert np.abs(out["first_try"].cpu().numpy() - ref["first_try"]).max() * (N - 1) < 1e-6
+    assert np.abs(out["deformations"].cpu().numpy() - ref["deformations"]).max() * (N - 1) < 1e-6
+    np.testing.assert_allclose(out["energies"], ref["energies"], rtol=1e-9)
+    assert sum(lv["kind"] == "alt" for lv in out["levels"]) == 3 * 2
