@@ -269,6 +269,27 @@ def test_matchseries_all_options_fp32_vs_fp64(tmp_path):
     assert np.median(np.abs(m64 - m32)) < 1e-3
 
 
+@pytest.mark.parametrize("extra,msg", [
+    ("cropInput 1\ncropStartX 10\ncropStartY 0\ndontResizeOrCropReference 0\n", "cropInput"),
+    ("averageSaveIncrement 0\n", "averageSaveIncrement"),
+])
+def test_matchseries_option_errors(tmp_path, extra, msg):
+    """A crop window outside the frames and a non-positive averageSaveIncrement are errors, as in the
+    reference (scalarArray.h:1066-1092, matchSeries.h:376-378)."""
+    if not os.path.exists(EXE):
+        pytest.skip("bin/matchSeries not built")
+    with np.load(os.path.join(GOLD, "e2e64.npz")) as z:
+        g = {k: z[k] for k in z.files}
+    g["par"] = np.frombuffer((bytes(g["par"]).decode() + extra).encode(), dtype=np.uint8)
+    (tmp_path / "in").mkdir()
+    for i, f in enumerate(g["frames"]):
+        write_p9(str(tmp_path / "in" / f"frame_{i:02d}.dat.bz2"), f)
+    (tmp_path / "run.par").write_text(bytes(g["par"]).decode() + "gpuFieldPrecision 64\n")
+    r = subprocess.run([EXE, "run.par"], cwd=tmp_path, capture_output=True, text=True,
+                       env=dict(os.environ, MSG_STRICT="1"), timeout=300)
+    assert r.returncode == 1 and msg in r.stderr, r.stderr[-500:]
+
+
 def test_matchseries_inverse_deformation(tmp_path):
     """calcInverseDeformation 1: the <i>-r-inv/ solves of stage 1 (per-level deformations and energy
     logs) against the reference program's (tests/golden/e2e64_inv.npz), FP64 fields."""

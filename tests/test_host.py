@@ -139,3 +139,11 @@ def test_cli_loads_initial_deformation(tmp_path):
         par = bytes(z["par"]).decode()
     r = cli(tmp_path, par + "initial-deformation { nothere_0.dat.bz2 nothere_1.dat.bz2 }\n", strict=True)
     assert r.returncode == 1 and "nothere_0.dat.bz2" in r.stderr
+
+
+def test_cli_translation_start_needs_max_corr_shift(tmp_path):
+    # useCorrelationToInitTranslation reads "maxCorrShift" with getInt, which fails when it is absent
+    with np.load(os.path.join(GOLD, "e2e64.npz")) as z:
+        par = bytes(z["par"]).decode()
+    r = cli(tmp_path, par + "useCorrelationToInitTranslation 1\n", strict=True)
+    assert r.returncode == 1 and "maxCorrShift" in r.stderr
