@@ -801,6 +801,56 @@ Image decodeDM(const std::string &bytes, const std::string &name) {
   return im;
 }
 
+void saveInDMContainer(const std::string &dmFile, const double *data, int nx, int ny, const std::string &outBase) {
+  std::ifstream in(dmFile, std::ios::binary);
+  if (!in) MSH_THROW(strprintf("Cannot open file %s for reading", dmFile.c_str()));
+  std::stringstream ss;
+  ss << in.rdbuf();
+  std::string bytes = ss.str();
+  DMWalker w;
+  w.b = reinterpret_cast<const uint8_t *>(bytes.data());
+  w.n = bytes.size();
+  w.name = dmFile;
+  w.run();
+  if (w.images.empty()) MSH_THROW("No data entry found.");
+  const DMWalker::ImageEntry e = w.images.back();
+  int size;
+  switch (e.type) {
+    case 4: size = 2; break;
+    case 3: case 5: case 6: size = 4; break;
+    case 10: size = 1; break;
+    default: MSH_THROW(strprintf("type %d not implemented ", e.type));
+  }
+  if (e.nx < 1 || e.ny < 1 || e.offset < 0 || e.count < 0 || e.count > e.nx * e.ny ||
+      (size_t)e.offset + (size_t)e.count * size > bytes.size())
+    MSH_THROW(strprintf("%s: DM image data do not fit the image size", dmFile.c_str()));
+  // padFrom: the array centred into e.nx x e.ny (an odd difference shifts it by one), zeros around
+  const int PX = (int)e.nx, PY = (int)e.ny;
+  std::vector<double> pad((size_t)PX * PY, 0.0);
+  const int dx = PX - nx, dy = PY - ny, px = dx / 2, py = dy / 2;
+  const int tx0 = std::max(0, px), tx1 = PX - std::max(0, px + dx % 2);
+  const int ty0 = std::max(0, py), ty1 = PY - std::max(0, py + dy % 2);
+  const int sx0 = -std::min(0, px + dx % 2), sy0 = -std::min(0, py + dy % 2);
+  for (int y = 0; y < ty1 - ty0; ++y)
+    for (int x = 0; x < tx1 - tx0; ++x) pad[(size_t)(ty0 + y) * PX + tx0 + x] = data[(size_t)(sy0 + y) * nx + sx0 + x];
+  char *dst = &bytes[(size_t)e.offset];
+  for (int64_t i = 0; i < e.count; ++i, dst += size) {
+    const double v = pad[(size_t)i];
+    switch (e.type) {
+      case 3: { const int32_t x = (int32_t)v; memcpy(dst, &x, 4); break; }
+      case 4: { const uint16_t x = (uint16_t)v; memcpy(dst, &x, 2); break; }
+      case 5: { const uint32_t x = (uint32_t)v; memcpy(dst, &x, 4); break; }
+      case 6: { const float x = (float)v; memcpy(dst, &x, 4); break; }
+      default: { const uint8_t x = (uint8_t)(v < 0 ? 0.0 : v > 255 ? 255.0 : v); memcpy(dst, &x, 1); break; }
+    }
+  }
+  const std::string out = outBase + (w.version == 3 ? ".dm3" : ".dm4");
+  std::ofstream os(out, std::ios::binary);
+  if (!os)
+    MSH_THROW(strprintf("qc::DM3Reader::saveQuocDataInDM3Container: Cannot open \"%s\" for writing", out.c_str()));
+  os.write(bytes.data(), (std::streamsize)bytes.size());
+}
+
 // ------------------------------------------------------------------------------------------------
 // MetaImage
 // ------------------------------------------------------------------------------------------------

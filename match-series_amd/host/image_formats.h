@@ -25,6 +25,13 @@ Image decodeTIFF(const std::string &bytes, const std::string &name);
 // Dimensions, element types int16/int32/uint16/uint32/float32/uint8 (quoc/dm3Import.h:16-436).
 Image decodeDM(const std::string &bytes, const std::string &name);
 
+// qc::DM3Reader::saveQuocDataInDM3Container (quoc/dm3Import.h:139-162): a copy of the DM file dmFile
+// named outBase + ".dm3" / ".dm4" whose image data (the last ImageList entry) is replaced by the nx x ny
+// array, centred into the file's image size (ScalarArray<QC_2D>::padFrom, scalarArray.h:1110-1133,
+// zeros around) and converted to the file's element type (int32, uint16, uint32, float32, or uint8
+// clamped to [0, 255]).
+void saveInDMContainer(const std::string &dmFile, const double *data, int nx, int ny, const std::string &outBase);
+
 // MetaImage header (.mhd) in the fixed layout the reference writes, MET_FLOAT or MET_USHORT data in
 // the raw file it names next to the header (quoc/auxiliary.h:51-125, quoc/scalarArray.cpp:664-674).
 Image loadMetaImage(const std::string &fileName);

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "array_io.h"
+#include "image_formats.h"
 #include "ms_gpu.h"
 #include "params.h"
 
@@ -85,7 +86,7 @@ std::string dirOf(const ParameterParser &p) { return p.getStringExpandTilde("sav
 // Options of the reference that change the computation but are not implemented here.
 void rejectUnsupported(const ParameterParser &p) {
   const char *flags[] = {"reverseRolesInSeriesMatching", "calcIQ", "calcPrecision",
-                         "performNoiseAnalysis", "saveNamedDeformedDMXTemplatesAsDMX"};
+                         "performNoiseAnalysis"};
   for (const char *f : flags)
     if (p.checkAndGetBool(f)) MSH_THROW(strprintf("\"%s 1\" is not supported by this build", f));
   // "initial-deformation": the registration object loads it (registration.h:495-500), and the series
@@ -310,6 +311,16 @@ void seriesExtras(const ParameterParser &p, int stage, const std::vector<std::st
   const bool namedDef = p.checkAndGetBool("saveNamedDeformations");
   const int inc = p.hasVariable("averageSaveIncrement") ? p.getInt("averageSaveIncrement") : n;
   if (inc <= 0) MSH_THROW("'averageSaveIncrement' must be bigger than zero.");
+  // saveNamedDeformedTemplateHelper (matchSeries.h:336-344): DM3/DM4 inputs optionally back into a copy
+  // of their own container, everything else as <basename>.q2bz
+  auto saveNamed = [&](const std::string &file, const double *a) {
+    const bool dm = file.size() >= 3 && (file.compare(file.size() - 3, 3, "dm3") == 0 ||
+                                         file.compare(file.size() - 3, 3, "dm4") == 0);
+    if (dm && p.checkAndGetBool("saveNamedDeformedDMXTemplatesAsDMX"))
+      saveInDMContainer(file, a, N, N, dir + baseFileName(file));
+    else
+      saveP9(dir + baseFileName(file) + ".q2bz", a, N, N);
+  };
   if (namedTpl && stage == 1) {
     const std::string ref = p.getStringExpandTilde("reference");
     int L = 0;
@@ -318,7 +329,7 @@ void seriesExtras(const ParameterParser &p, int stage, const std::vector<std::st
     loadRaw(p, ref, L, dt, r.p);
     std::vector<double> h(nn);
     download(h.data(), r.p, nn, dt);
-    saveP9(dir + baseFileName(ref) + ".q2bz", h.data(), N, N);
+    saveNamed(ref, h.data());
   }
   if (namedTpl) {
     const bool extMean = p.checkAndGetBool("saveNamedDeformedTemplatesExtendedWithMean");
@@ -334,7 +345,7 @@ void seriesExtras(const ParameterParser &p, int stage, const std::vector<std::st
       }
       msgCheck(msg_warp(dt, im, (const char *)defs + (size_t)i * es * 2 * nn, out.p, N, 1, 0, 0, ext, nearest, nullptr));
       download(h.data(), out.p, nn, dt);
-      saveP9(dir + baseFileName(files[i]) + ".q2bz", h.data(), N, N);
+      saveNamed(files[i], h.data());
     }
   }
   if (namedDef) {

@@ -8,6 +8,7 @@
     averageSaveIncrement 2 / 1: every extra file of both stages;
   * e2e128_to64 / e2e_crop64 / e2e_resize64.npz: the program on frames larger than the grid, cropped
     (cropInput) and resampled (resizeInput); resize.npz: ScalarArray<QC_2D>::resampleFrom alone;
+  * e2e64_dm3.npz: the series as DM3 files, deformed templates written back into DM3 containers;
   * e2e64_alt / e2e64_alt_energy.npz: altStartLevel 5 (chosen by overlap / energyAsAltCriterion), with
     the chosen deformation-<iii> files and the <i>-alt/ records.
 Runs in the build container only (needs oracle/_ref). Usage: python tests/golden/make_options.py
@@ -80,6 +81,8 @@ def main():
                 e2e[f"s{st}_defNorms"] = np.loadtxt(os.path.join(sd, "defNorms.txt"), usecols=(0, 1)) \
                     if os.path.exists(os.path.join(sd, "defNorms.txt")) else np.zeros((0, 2))
             np.savez_compressed(os.path.join(HERE, f"e2e64_{tag}.npz"), **e2e)
+        # DM3 frames (float32 data) with the deformed templates written back into DM3 containers
+        np.savez_compressed(os.path.join(HERE, "e2e64_dm3.npz"), **run_dm(frames, os.path.join(tmp, "dm3")))
         for tag, make_par in NAMED.items():
             np.savez_compressed(os.path.join(HERE, f"e2e64_named_{tag}.npz"),
                                 **run_named(frames, make_par(G.E2E_PAR), os.path.join(tmp, f"named_{tag}")))
@@ -93,6 +96,38 @@ NAMED = {
         "saveNamedDeformedTemplatesUsingNearestNeighborInterpolation 1\ndeformForAverageExtendedWithMean 1\n"
         "averageSaveIncrement 1\n"),
 }
+
+
+def run_dm(frames, wd):
+    """The reference program on the frames stored as DM3 files (float32 image data, as
+    tests/golden/make_formats.py writes them), with saveNamedDeformedTemplates and
+    saveNamedDeformedDMXTemplatesAsDMX: the input files, every DM3 file it writes and the
+    stage-1 deformations."""
+    import make_formats as F
+    os.makedirs(os.path.join(wd, "in"))
+    out = {}
+    for i, f in enumerate(frames):
+        b = F.dm_file(3, [("ImageList", [("", [("ImageData", F.dm_image(f.astype(np.float32), 6))])])])
+        with open(os.path.join(wd, "in", f"frame_{i:02d}.dm3"), "wb") as fh:
+            fh.write(b)
+        out[f"in/frame_{i:02d}.dm3"] = np.frombuffer(b, dtype=np.uint8)
+    par = (G.E2E_PAR.replace("in/frame_%02d.dat.bz2", "in/frame_%02d.dm3")
+           + "saveNamedDeformedTemplates 1\nsaveNamedDeformedDMXTemplatesAsDMX 1\n")
+    with open(os.path.join(wd, "run.par"), "w") as fh:
+        fh.write(par)
+    env = dict(os.environ, QUOC_NO_SYSTEM_PAUSE="ON")
+    subprocess.check_call([G.MATCHSERIES, "run.par"], cwd=wd, env=env, stdout=subprocess.DEVNULL,
+                          stderr=subprocess.DEVNULL)
+    out["par"] = np.frombuffer(par.encode(), dtype=np.uint8)
+    for st in (1, 2):
+        sd = os.path.join(wd, "out", f"stage{st}")
+        for fn in sorted(os.listdir(sd)):
+            if fn.endswith(".dm3"):
+                out[f"s{st}/{fn}"] = np.fromfile(os.path.join(sd, fn), dtype=np.uint8)
+    out["s1_deformations"] = np.stack([np.stack([G.read_p9(os.path.join(
+        wd, "out", "stage1", f"{i}" if i == 0 else f"{i}-r", f"deformation_06_{c}.dat.bz2")) for c in range(2)])
+        for i in range(3)])
+    return out
 
 
 def run_named(frames, par, wd):

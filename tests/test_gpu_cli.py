@@ -2,6 +2,7 @@
 the same parameter file and frames (tests/golden/e2e64.npz: 4 STEM frames of 64x64, two stages)."""
 import bz2
 import os
+import struct
 import subprocess
 
 import sys
@@ -288,6 +289,50 @@ def test_matchseries_option_errors(tmp_path, extra, msg):
     r = subprocess.run([EXE, "run.par"], cwd=tmp_path, capture_output=True, text=True,
                        env=dict(os.environ, MSG_STRICT="1"), timeout=300)
     assert r.returncode == 1 and msg in r.stderr, r.stderr[-500:]
+
+
+def test_matchseries_dm3_frames_and_containers(tmp_path):
+    """DM3 frames in, saveNamedDeformedTemplates with saveNamedDeformedDMXTemplatesAsDMX: each deformed
+    template comes back as a copy of its DM3 file with the image data replaced, as the reference's
+    (tests/golden/e2e64_dm3.npz): the same bytes outside the image data, the data within 1e-5 of the
+    value range in stage 1; stage 2 (lambda 1, on this run's own stage-1 median) to 1 % in the median."""
+    if not os.path.exists(EXE):
+        pytest.skip("bin/matchSeries not built")
+    with np.load(os.path.join(GOLD, "e2e64_dm3.npz")) as z:
+        g = {k: z[k] for k in z.files}
+    (tmp_path / "in").mkdir()
+    for k in g:
+        if k.startswith("in/"):
+            (tmp_path / k).write_bytes(g[k].tobytes())
+    (tmp_path / "run.par").write_text(bytes(g["par"]).decode() + "gpuFieldPrecision 64\n")
+    r = subprocess.run([EXE, "run.par"], cwd=tmp_path, capture_output=True, text=True,
+                       env=dict(os.environ, MSG_STRICT="1"), timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = tmp_path / "out"
+    for i in range(3):
+        sub = f"{i}" if i == 0 else f"{i}-r"
+        d = np.stack([read_p9(str(out / "stage1" / sub / f"deformation_06_{c}.dat.bz2")) for c in range(2)])
+        assert np.abs(d - g["s1_deformations"][i]).max() * 63 < 1e-6, i
+    keys = [k for k in g if k.startswith(("s1/", "s2/"))]
+    assert len(keys) == 8
+    for k in keys:
+        st, fn = k.split("/")
+        mine = (out / f"stage{st[1:]}" / fn).read_bytes()
+        ref = g[k].tobytes()
+        src = g[f"in/{fn}"].tobytes()
+        assert len(mine) == len(ref) == len(src), k
+        # the image data: the float32 array of the ImageData "Data" tag (64 x 64 values)
+        n = 64 * 64 * 4
+        tag = b"Data%%%%" + struct.pack(">iiii", 3, 20, 6, 64 * 64)
+        off = src.rindex(tag) + len(tag)
+        assert mine[:off] == ref[:off] and mine[off + n:] == ref[off + n:], k
+        a = np.frombuffer(mine[off:off + n], "<f4").astype(float)
+        b = np.frombuffer(ref[off:off + n], "<f4").astype(float)
+        scale = max(1.0, np.abs(b).max())
+        if st == "s1":
+            assert np.abs(a - b).max() < 1e-5 * scale, k
+        else:  # stage 2 runs at lambda 1 on this run's own stage-1 median (see above): ~1 % here
+            assert np.median(np.abs(a - b)) < 1e-2 * scale, k
 
 
 def test_matchseries_inverse_deformation(tmp_path):
