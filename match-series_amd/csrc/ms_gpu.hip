@@ -1480,9 +1480,15 @@ int compose_impl(const void *phi1, const void *phi2, void *result, int n, hipStr
 template <typename S>
 int stats_impl(const void *stack, int nimg, int n, double fill, void *mean, void *count, void *median, hipStream_t s) {
   const int nn = n * n;
-  const size_t lds = sizeof(double) * 64 * nimg;
-  hipLaunchKernelGGL((msg::k_series_stats<S>), dim3(cdiv(nn, 64)), dim3(64), lds, s, (const S *)stack, nimg, nn, fill,
-                     (S *)mean, (S *)count, (S *)median);
+  const int P = msg::stats_pixels_per_wave(nimg);
+  if (P == 0) return fail(MSG_ERR_ARG, "series longer than 8192 frames are not supported by the median kernel");
+  const size_t lds = sizeof(double) * P * std::max(nimg, 1);
+  const dim3 g(cdiv(nn, P)), b(64);
+  const S *st = (const S *)stack;
+  if (P == 64) hipLaunchKernelGGL((msg::k_series_stats<S, 64>), g, b, lds, s, st, nimg, nn, fill, (S *)mean, (S *)count, (S *)median);
+  else if (P == 16) hipLaunchKernelGGL((msg::k_series_stats<S, 16>), g, b, lds, s, st, nimg, nn, fill, (S *)mean, (S *)count, (S *)median);
+  else if (P == 4) hipLaunchKernelGGL((msg::k_series_stats<S, 4>), g, b, lds, s, st, nimg, nn, fill, (S *)mean, (S *)count, (S *)median);
+  else hipLaunchKernelGGL((msg::k_series_stats<S, 1>), g, b, lds, s, st, nimg, nn, fill, (S *)mean, (S *)count, (S *)median);
   HIPCHK(hipGetLastError());
   return MSG_OK;
 }
@@ -1674,7 +1680,21 @@ struct Progress {
 // following pairs of the same size.
 struct TranslationWork {
   int n = 0;
-  DevBuf w, x, y, z, corr, part;
+  DevBuf w, x, y, z, corr, part, win;
+  size_t win_cap = 0;  // doubles held by `win`
+  // The (2m+1)^2 correlation values of a maxCorrShift window (m < n, so up to (2n-1)^2 > 2 n^2:
+  // a buffer of its own, grown on demand).
+  int ensure_window(size_t cnt) {
+    if (cnt <= win_cap) return MSG_OK;
+    if (win.p) {
+      HIPCHK(hipFree(win.p));
+      win.p = nullptr;
+    }
+    win_cap = 0;
+    HIPCHK(hipMalloc(&win.p, sizeof(double) * cnt));
+    win_cap = cnt;
+    return MSG_OK;
+  }
   int ensure(int nn_side) {
     if (n == nn_side) return MSG_OK;
     const size_t nn = (size_t)nn_side * nn_side;
@@ -1754,7 +1774,8 @@ int translation_impl(const S *refimg, const S *tplimg, int n, int max_shift, int
   } else {
     const int side = 2 * max_shift + 1;
     const size_t cnt = (size_t)side * side;
-    double *win = (double *)wk.x.p;  // free by now; holds 2 n^2 >= (2 max_shift + 1)^2 doubles
+    CHK(wk.ensure_window(cnt));
+    double *win = (double *)wk.win.p;
     hipLaunchKernelGGL(msg::k_gather_window, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, corr, n, max_shift,
                        win);
     HIPCHK(hipGetLastError());
@@ -2142,7 +2163,6 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
   if (pairwise_out) HIPCHK(hipMemcpy(pairwise_out, pwp, sizeof(S) * 2 * nn * n, hipMemcpyDeviceToDevice));
 
   // --- reconstruction: warp the raw templates (Inf outside), then mean / count / median ---
-  if (n > msg::ST_MAXF) return fail(MSG_ERR_ARG, "series longer than 128 frames are not supported by the median kernel");
   DevBuf stack;
   HIPCHK(hipMalloc(&stack.p, sizeof(S) * nn * std::max(n, 1)));
   // reduceDeformations (matchSeries.h:565-570, 678-704): the reduced deformation is applied to every
@@ -2210,7 +2230,7 @@ int msg_series_stats(int dtype, const void *stack, int nimg, int n, double fill,
   CHK(check_dtype(dtype));
   CHK(check_n(n));
   if (!stack || !mean || !count || !median || nimg < 0 || nimg > msg::ST_MAXF)
-    return fail(MSG_ERR_ARG, "invalid argument (at most 128 images)");
+    return fail(MSG_ERR_ARG, "invalid argument (at most 8192 images)");
   if (dtype == MSG_F64) CHK(stats_impl<double>(stack, nimg, n, fill, mean, count, median, strm(stream)));
   else CHK(stats_impl<float>(stack, nimg, n, fill, mean, count, median, strm(stream)));
   HIPCHK(hipStreamSynchronize(strm(stream)));
@@ -2294,6 +2314,8 @@ int msg_match_series(const msg_series_params *p, int dtype, const void *template
   if (!p->dont_accumulate && n > 1 &&
       (p->refine_start_level < 1 || p->refine_stop_level > L || p->refine_start_level > p->refine_stop_level))
     return fail(MSG_ERR_ARG, "invalid refineStartLevel/refineStopLevel");
+  // checked before any solve: the reconstruction's median holds a pixel's samples in LDS
+  if (n > msg::ST_MAXF) return fail(MSG_ERR_ARG, "series longer than 8192 frames are not supported by the median kernel");
   if (dtype == MSG_F64)
     return match_series_impl<double>(p, templates, n, reference, pairwise_in, deformations, pairwise_out, energies,
                                      mean, count, median, levels, reduced);

@@ -145,8 +145,62 @@ def dtype_code(t):
 
 
 def _ptr(t):
-    assert t.is_contiguous() and t.is_cuda, "expected a contiguous device tensor"
+    if not (t.is_cuda and t.is_contiguous()):
+        raise ValueError("expected a contiguous device tensor")
     return ctypes.c_void_p(t.data_ptr())
+
+
+def _arg(t, name, code=None, shape=None):
+    """Checks one tensor argument before it reaches the library (which trusts sizes and types):
+    float32/float64, the dtype `code` and `shape` when given, contiguous. Returns its dtype code.
+    (The device is checked afterwards for all arguments together, _on_device.)"""
+    torch = _torch()
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if t.dtype not in (torch.float32, torch.float64):
+        raise TypeError(f"{name}: tensors must be float32 or float64, got {t.dtype}")
+    c = dtype_code(t)
+    if code is not None and c != code:
+        raise TypeError(f"{name}: dtype {t.dtype} differs from the other arguments' {torch_dtype(code)}")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: expected a contiguous tensor")
+    return c
+
+
+def _on_device(**named):
+    """After the type and shape checks of _arg: every tensor on one ROCm device."""
+    first = None
+    for name, t in named.items():
+        if t is None:
+            continue
+        if not t.is_cuda:
+            raise ValueError(f"{name}: expected a device tensor, got one on {t.device}")
+        if first is not None and t.device != first[1]:
+            raise ValueError(f"{name}: on {t.device}, {first[0]} on {first[1]}")
+        first = first or (name, t.device)
+
+
+def _square(t, name, lead):
+    """(lead..., n, n) with n >= 2: returns n."""
+    if t.dim() != len(lead) + 2 or t.shape[-1] != t.shape[-2] or t.shape[-1] < 2:
+        raise ValueError(f"{name}: expected shape {tuple(lead) + ('n', 'n')}, got {tuple(t.shape)}")
+    for k, (got, want) in enumerate(zip(t.shape, lead)):
+        if want is not None and got != want:
+            raise ValueError(f"{name}: dimension {k} is {got}, expected {want}")
+    return t.shape[-1]
+
+
+def _pair_args(R, T, phi):
+    """R, T: (B, n, n); phi: (B, 2, n, n); one dtype and device. Returns (code, B, n)."""
+    n = _square(R, "R", (None,))
+    B = R.shape[0]
+    code = _arg(R, "R")
+    _arg(T, "T", code, (B, n, n))
+    _arg(phi, "phi", code, (B, 2, n, n))
+    _on_device(R=R, T=T, phi=phi)
+    return code, B, n
 
 
 def _dptr(a):
@@ -174,100 +228,134 @@ def torch_dtype(code):
 # ------------------------------------------------------------------------------------------------
 def energy(R, T, phi):
     """Per-pair (E_SSD, 1/2 sum_c phi_c K phi_c). R, T: (B, n, n); phi: (B, 2, n, n)."""
-    B, n = R.shape[0], R.shape[-1]
+    code, B, n = _pair_args(R, T, phi)
     es = np.zeros(B)
     er = np.zeros(B)
-    check(lib().msg_energy(dtype_code(R), _ptr(R), _ptr(T), _ptr(phi), n, B, _dptr(es), _dptr(er), _stream()))
+    check(lib().msg_energy(code, _ptr(R), _ptr(T), _ptr(phi), n, B, _dptr(es), _dptr(er), _stream()))
     return es, er
 
 
 def ncc_energy(R, T, phi):
     """Per-pair (E_NCC, 1/2 sum_c phi_c K phi_c) of the deformationModel 0 data term."""
-    B, n = R.shape[0], R.shape[-1]
+    code, B, n = _pair_args(R, T, phi)
     es = np.zeros(B)
     er = np.zeros(B)
-    check(lib().msg_ncc_energy(dtype_code(R), _ptr(R), _ptr(T), _ptr(phi), n, B, _dptr(es), _dptr(er), _stream()))
+    check(lib().msg_ncc_energy(code, _ptr(R), _ptr(T), _ptr(phi), n, B, _dptr(es), _dptr(er), _stream()))
     return es, er
 
 
 def ncc_gradient(R, T, phi, lam):
-    B, n = R.shape[0], R.shape[-1]
+    code, B, n = _pair_args(R, T, phi)
     out = _torch().empty_like(phi)
-    check(lib().msg_ncc_gradient(dtype_code(R), _ptr(R), _ptr(T), _ptr(phi), _ptr(out), n, B, lam, _stream()))
+    check(lib().msg_ncc_gradient(code, _ptr(R), _ptr(T), _ptr(phi), _ptr(out), n, B, lam, _stream()))
     return out
 
 
 def gradient(R, T, phi, lam):
-    B, n = R.shape[0], R.shape[-1]
+    code, B, n = _pair_args(R, T, phi)
     out = _torch().empty_like(phi)
-    check(lib().msg_gradient(dtype_code(R), _ptr(R), _ptr(T), _ptr(phi), _ptr(out), n, B, lam, _stream()))
+    check(lib().msg_gradient(code, _ptr(R), _ptr(T), _ptr(phi), _ptr(out), n, B, lam, _stream()))
     return out
 
 
 def h1_smooth(d, sigma):
-    """d: (k, n, n) fields smoothed independently."""
-    n = d.shape[-1]
+    """d: (..., n, n) fields smoothed independently."""
+    if d.dim() < 2:
+        raise ValueError("d: expected (..., n, n)")
+    n = _square(d, "d", (None,) * (d.dim() - 2))
+    code = _arg(d, "d")
+    _on_device(d=d)
     k = d.numel() // (n * n)
     out = _torch().empty_like(d)
-    check(lib().msg_h1_smooth(dtype_code(d), _ptr(d), _ptr(out), n, k, sigma, _stream()))
+    check(lib().msg_h1_smooth(code, _ptr(d), _ptr(out), n, k, sigma, _stream()))
     return out
 
 
 def warp(img, phi, mode=0, ext=float("inf"), nearest=False):
     """img: (k, n, n); phi: (k, 2, n, n) or (2, n, n) shared. mode 0: constant outside, 1: clamp, 2: NaN."""
-    n = img.shape[-1]
+    if img.dim() < 2:
+        raise ValueError("img: expected (..., n, n)")
+    n = _square(img, "img", (None,) * (img.dim() - 2))
+    code = _arg(img, "img")
     k = img.numel() // (n * n)
     shared = 1 if phi.dim() == 3 else 0
+    _arg(phi, "phi", code, (2, n, n) if shared else (k, 2, n, n))
+    _on_device(img=img, phi=phi)
     out = _torch().empty_like(img)
-    check(lib().msg_warp(dtype_code(img), _ptr(img), _ptr(phi), _ptr(out), n, k, shared, mode, ext, int(nearest), _stream()))
+    check(lib().msg_warp(code, _ptr(img), _ptr(phi), _ptr(out), n, k, shared, mode, ext, int(nearest), _stream()))
     return out
 
 
 def restrict(fine):
-    n = fine.shape[-1]
+    if fine.dim() < 2:
+        raise ValueError("fine: expected (..., n, n)")
+    n = _square(fine, "fine", (None,) * (fine.dim() - 2))
+    code = _arg(fine, "fine")
+    _on_device(fine=fine)
     k = fine.numel() // (n * n)
     out = _torch().empty(fine.shape[:-2] + (n // 2, n // 2), dtype=fine.dtype, device=fine.device)
-    check(lib().msg_restrict(dtype_code(fine), _ptr(fine), _ptr(out), n, k, _stream()))
+    check(lib().msg_restrict(code, _ptr(fine), _ptr(out), n, k, _stream()))
     return out
 
 
 def prolong(coarse):
-    nc = coarse.shape[-1]
+    if coarse.dim() < 2:
+        raise ValueError("coarse: expected (..., n, n)")
+    nc = _square(coarse, "coarse", (None,) * (coarse.dim() - 2))
+    code = _arg(coarse, "coarse")
+    _on_device(coarse=coarse)
     k = coarse.numel() // (nc * nc)
     out = _torch().empty(coarse.shape[:-2] + (2 * nc, 2 * nc), dtype=coarse.dtype, device=coarse.device)
-    check(lib().msg_prolong(dtype_code(coarse), _ptr(coarse), _ptr(out), nc, k, _stream()))
+    check(lib().msg_prolong(code, _ptr(coarse), _ptr(out), nc, k, _stream()))
     return out
 
 
 def resample(src, n):
-    nc = src.shape[-1]
+    if src.dim() < 2:
+        raise ValueError("src: expected (..., n, n)")
+    nc = _square(src, "src", (None,) * (src.dim() - 2))
+    code = _arg(src, "src")
+    _on_device(src=src)
     k = src.numel() // (nc * nc)
     out = _torch().empty(src.shape[:-2] + (n, n), dtype=src.dtype, device=src.device)
-    check(lib().msg_resample(dtype_code(src), _ptr(src), _ptr(out), nc, n, k, _stream()))
+    check(lib().msg_resample(code, _ptr(src), _ptr(out), nc, n, k, _stream()))
     return out
 
 
 def compose(phi1, phi2, start=None):
+    n = _square(phi1, "phi1", (2,))
+    code = _arg(phi1, "phi1")
+    _arg(phi2, "phi2", code, (2, n, n))
+    if start is not None:
+        _arg(start, "start", code, (2, n, n))
+    _on_device(phi1=phi1, phi2=phi2, start=start)
     out = _torch().zeros_like(phi1) if start is None else start.clone()
-    check(lib().msg_compose(dtype_code(phi1), _ptr(phi1), _ptr(phi2), _ptr(out), phi1.shape[-1], _stream()))
+    check(lib().msg_compose(code, _ptr(phi1), _ptr(phi2), _ptr(out), phi1.shape[-1], _stream()))
     return out
 
 
 def series_stats(stack, fill):
-    k, n = stack.shape[0], stack.shape[-1]
+    n = _square(stack, "stack", (None,))
+    code = _arg(stack, "stack")
+    _on_device(stack=stack)
+    k = stack.shape[0]
     torch = _torch()
     mean = torch.empty((n, n), dtype=stack.dtype, device=stack.device)
     cnt = torch.empty_like(mean)
     med = torch.empty_like(mean)
-    check(lib().msg_series_stats(dtype_code(stack), _ptr(stack), k, n, fill, _ptr(mean), _ptr(cnt), _ptr(med), _stream()))
+    check(lib().msg_series_stats(code, _ptr(stack), k, n, fill, _ptr(mean), _ptr(cnt), _ptr(med), _stream()))
     return mean, cnt, med
 
 
 def normalize(img, pct=-1.0, min_to_zero=True):
     """In place on (k, n, n) images."""
-    n = img.shape[-1]
+    if img.dim() < 2:
+        raise ValueError("img: expected (..., n, n)")
+    n = _square(img, "img", (None,) * (img.dim() - 2))
+    code = _arg(img, "img")
+    _on_device(img=img)
     k = img.numel() // (n * n)
-    check(lib().msg_normalize(dtype_code(img), _ptr(img), n, k, pct, int(min_to_zero), _stream()))
+    check(lib().msg_normalize(code, _ptr(img), n, k, pct, int(min_to_zero), _stream()))
     return img
 
 
@@ -275,10 +363,13 @@ def register_translation(reference, templ, max_shift=0, want_corr=False):
     """Integer translation (bx, by) in pixels with reference(. + b) ~ templ, by phase correlation
     (max_shift > 0: plain correlation within |b|_inf <= max_shift); with want_corr also the
     correlation surface as an (n, n) float64 device tensor."""
-    n = reference.shape[-1]
+    n = _square(reference, "reference", ())
+    code = _arg(reference, "reference")
+    _arg(templ, "templ", code, (n, n))
+    _on_device(reference=reference, templ=templ)
     sh = (ctypes.c_int * 2)()
     corr = _torch().empty((n, n), dtype=_torch().float64, device=reference.device) if want_corr else None
-    check(lib().msg_register_translation(dtype_code(reference), _ptr(reference), _ptr(templ), n, int(max_shift), sh,
+    check(lib().msg_register_translation(code, _ptr(reference), _ptr(templ), n, int(max_shift), sh,
                                          _ptr(corr) if want_corr else None, _stream()))
     b = (int(sh[0]), int(sh[1]))
     return (b, corr) if want_corr else b
@@ -323,6 +414,13 @@ class Context:
         check(lib().msg_set_template(self.h, pair, _dptr(a)))
 
     def set_images_device(self, ref, tpl):
+        shape = (self.npairs, self.n, self.n)
+        if ref is None and tpl is None:
+            raise ValueError("set_images_device: no image given")
+        code = _arg(ref, "ref", None, shape) if ref is not None else _arg(tpl, "tpl", None, shape)
+        if ref is not None and tpl is not None:
+            _arg(tpl, "tpl", code, shape)
+        _on_device(ref=ref, tpl=tpl)
         _ready()
         check(lib().msg_set_images_device(self.h, _ptr(ref) if ref is not None else None,
                                           _ptr(tpl) if tpl is not None else None,
@@ -341,10 +439,14 @@ class Context:
         check(lib().msg_set_deformation_zero(self.h, pair))
 
     def set_deformations_device(self, phi):
+        _arg(phi, "phi", self.dtype, (self.npairs, 2, self.n, self.n))
+        _on_device(phi=phi)
         _ready()
         check(lib().msg_set_deformations_device(self.h, _ptr(phi)))
 
     def get_deformations_device(self, phi):
+        _arg(phi, "phi", self.dtype, (self.npairs, 2, self.n, self.n))
+        _on_device(phi=phi)
         _ready()
         check(lib().msg_get_deformations_device(self.h, _ptr(phi)))
 
@@ -400,28 +502,40 @@ def series_level_layout(params, n, reuse=False):
 
 def reduce_deformations(defs, max_iter=1000, stop_eps=5e-7):
     """reduceDeformations on device fields defs (n, 2, N, N): (phi, energy, iterations)."""
-    n, N = defs.shape[0], defs.shape[-1]
+    N = _square(defs, "defs", (None, 2))
+    code = _arg(defs, "defs")
+    _on_device(defs=defs)
+    if N & (N - 1):
+        raise ValueError("defs: the grid side must be a power of two")
+    n = defs.shape[0]
     phi = _torch().zeros_like(defs[0])
     e = np.zeros(1)
     it = ctypes.c_int(0)
     _ready()
-    check(lib().msg_reduce_deformations(dtype_code(defs), _ptr(defs), n, int(np.log2(N)), max_iter, stop_eps, _ptr(phi),
+    check(lib().msg_reduce_deformations(code, _ptr(defs), n, int(np.log2(N)), max_iter, stop_eps, _ptr(phi),
                                         _dptr(e), ctypes.byref(it)))
     return phi, float(e[0]), it.value
 
 
 def resample_rect(src, n):
     """ScalarArray<QC_2D>::resampleFrom of an (h, w) device image onto (n, n)."""
+    if src.dim() != 2:
+        raise ValueError(f"src: expected (h, w), got {tuple(src.shape)}")
+    code = _arg(src, "src")
+    _on_device(src=src)
     h, w = src.shape[-2], src.shape[-1]
     out = _torch().empty((n, n), dtype=src.dtype, device=src.device)
-    check(lib().msg_resample_rect(dtype_code(src), _ptr(src), w, h, _ptr(out), n, _stream()))
+    check(lib().msg_resample_rect(code, _ptr(src), w, h, _ptr(out), n, _stream()))
     return out
 
 
 def approx_inverse(phi):
     """qc::approxInverseDeformation of a (2, n, n) device displacement field."""
+    _square(phi, "phi", (2,))
+    code = _arg(phi, "phi")
+    _on_device(phi=phi)
     out = _torch().empty_like(phi)
-    check(lib().msg_approx_inverse(dtype_code(phi), _ptr(phi), _ptr(out), phi.shape[-1], _stream()))
+    check(lib().msg_approx_inverse(code, _ptr(phi), _ptr(out), phi.shape[-1], _stream()))
     return out
 
 
@@ -430,7 +544,13 @@ def match_series(templates, reference, params, pairwise_in=None, want_pairwise=F
     want_levels: also return the per-level deformations and iteration records of every solve; with
     params.calc_inverse the inverse solves' deformations come back as "inverses"."""
     torch = _torch()
-    n, N = templates.shape[0], templates.shape[-1]
+    N = _square(templates, "templates", (None,))
+    code = _arg(templates, "templates")
+    n = templates.shape[0]
+    _arg(reference, "reference", code, (N, N))
+    if pairwise_in is not None:
+        _arg(pairwise_in, "pairwise_in", code, (n, 2, N, N))
+    _on_device(templates=templates, reference=reference, pairwise_in=pairwise_in)
     lv = None
     if want_levels:
         recs = series_level_layout(params, n, pairwise_in is not None)
@@ -447,13 +567,14 @@ def match_series(templates, reference, params, pairwise_in=None, want_pairwise=F
     mean = torch.zeros((N, N), dtype=templates.dtype, device=templates.device)
     cnt = torch.zeros_like(mean)
     med = torch.zeros_like(mean)
-    _ready()
     red = torch.zeros((2, N, N), dtype=templates.dtype, device=templates.device) if params.reduce_deformations else None
     inv = torch.zeros_like(defs) if params.calc_inverse and not params.dont_accumulate else None
-    params.inverse_out = ctypes.c_void_p(inv.data_ptr()) if inv is not None else None
     first = torch.zeros_like(defs) if params.alt_start_level > 0 else None
+    # every output's zero fill is on torch's stream: finished before the library's streams write
+    _ready()
+    params.inverse_out = ctypes.c_void_p(inv.data_ptr()) if inv is not None else None
     params.first_try_out = ctypes.c_void_p(first.data_ptr()) if first is not None else None
-    check(lib().msg_match_series(ctypes.byref(params), dtype_code(templates), _ptr(templates), n, _ptr(reference),
+    check(lib().msg_match_series(ctypes.byref(params), code, _ptr(templates), n, _ptr(reference),
                                  _ptr(pairwise_in) if pairwise_in is not None else None, _ptr(defs),
                                  _ptr(pw) if pw is not None else None, _dptr(en), _ptr(mean), _ptr(cnt), _ptr(med),
                                  ctypes.byref(lv) if lv is not None else None, _ptr(red) if red is not None else None))

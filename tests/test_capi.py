@@ -44,3 +44,39 @@ def test_error_reporting_without_device_work():
     assert L.msg_h1_smooth(0, None, None, 48, 1, 1.0, None) == 1
     assert b"power of two" in L.msg_last_error()
     assert L.msg_version().startswith(b"msgpu")
+
+
+def test_wrappers_reject_mismatched_tensors_before_the_library():
+    """The ctypes wrappers check dtype, shape, contiguity and device of every tensor argument with
+    explicit raises (the library trusts the sizes it is given). CPU tensors reach the device check
+    last, so every mismatch below is reported without a GPU and without a library call."""
+    torch = pytest.importorskip("torch")
+    from msamd import gpu
+    n = 8
+    R = torch.zeros((2, n, n), dtype=torch.float64)
+    T = torch.zeros((2, n, n), dtype=torch.float64)
+    phi = torch.zeros((2, 2, n, n), dtype=torch.float64)
+    with pytest.raises(TypeError, match="phi: dtype"):
+        gpu.energy(R, T, phi.float())
+    with pytest.raises(TypeError, match="T: dtype"):
+        gpu.gradient(R, T.float(), phi, 1.0)
+    with pytest.raises(ValueError, match="phi: shape"):
+        gpu.energy(R, T, phi[:1].contiguous())
+    with pytest.raises(ValueError, match="T: shape"):
+        gpu.ncc_energy(R, T[:, :4, :4].contiguous(), phi)
+    with pytest.raises(ValueError, match="contiguous"):
+        gpu.energy(R, T.transpose(1, 2), phi)
+    with pytest.raises(ValueError, match="device tensor"):
+        gpu.energy(R, T, phi)
+    with pytest.raises(TypeError, match="float32 or float64"):
+        gpu.h1_smooth(torch.zeros((1, n, n), dtype=torch.float16), 1.0)
+    with pytest.raises(ValueError, match="phi: shape"):
+        gpu.warp(R, torch.zeros((3, 2, n, n), dtype=torch.float64))
+    with pytest.raises(TypeError, match="phi2: dtype"):
+        gpu.compose(phi[0], phi[1].float())
+    with pytest.raises(ValueError, match="templ: shape"):
+        gpu.register_translation(R[0], torch.zeros((n, 2 * n), dtype=torch.float64))
+    with pytest.raises(ValueError, match="reference: shape"):
+        gpu.match_series(R, torch.zeros((n + 1, n + 1), dtype=torch.float64), gpu.SeriesParams())
+    with pytest.raises(ValueError, match="expected shape"):
+        gpu.series_stats(torch.zeros((3, n, n + 1), dtype=torch.float64), 0.0)

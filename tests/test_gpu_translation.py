@@ -58,6 +58,23 @@ def test_windowed_correlation_matches_oracle(gpu, N, bx, by, m):
         assert got == (bx, by)
 
 
+@pytest.mark.parametrize("N,m", [(64, 63), (64, 46), (8, 7), (16, 15)])
+def test_window_wider_than_half_the_frame(gpu, N, m):
+    """maxCorrShift up to n - 1: the (2m+1)^2 window wraps around the periodic correlation surface
+    and holds up to (2n-1)^2 values, more than any n x n work buffer; the scan order (x fastest from
+    (-m, -m), strict maximum, start at (0, 0)) decides between the repeated values as the oracle does."""
+    for seed, (bx, by) in enumerate(((3, -5), (N // 2 - 1, 2), (-N // 2 + 1, N // 2 - 1))):
+        if N < 16:
+            rng = np.random.default_rng(seed)
+            a = rng.random((N, N))
+            b = np.roll(a, (-by, -bx), axis=(0, 1))
+        else:
+            a, b = shifted_pair(N, bx % 8, by % 8, 11 + seed)
+        ta = torch.tensor(a, dtype=torch.float64, device="cuda")
+        tb = torch.tensor(b, dtype=torch.float64, device="cuda")
+        assert gpu.register_translation(ta, tb, max_shift=m) == O.register_translation(a, b, m)
+
+
 def test_float_frames_and_periodic_shifts(gpu):
     rng = np.random.default_rng(5)
     N = 128
