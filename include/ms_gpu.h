@@ -9,9 +9,10 @@
  *   - an image is n*n values; a displacement field is 2*n*n values (x component, then y), in
  *     domain units (pixel offset = value / h);
  *   - a batch of k images/fields is stored back to back.
- * Host arrays are float64 (the reference's RealType). Device arrays carry a dtype: MSG_F32 fields
- * are stored in FP32 and computed in FP64 registers; MSG_F64 stores FP64. Energies, slopes and
- * Armijo decisions are FP64 in both.
+ * Host arrays are float64 (the reference's RealType). Device arrays carry a dtype: MSG_F64 stores and
+ * computes in FP64 in the reference's operation order. MSG_F32 stores FP32; with the SSD data term the
+ * warp, the stencils and the V-cycle compute in FP32 (energy sums, slopes, norms and every step
+ * decision stay FP64), the NCC and consistency energies in FP64 on the FP32 values.
  *
  * Each entry point names the reference interface it replaces (file:line, QuocMesh sources under
  * quocmesh/modules/ and quocmesh/projects/electronMicroscopy/).
@@ -96,14 +97,22 @@ int msg_solve(msg_ctx *ctx, int start_level, int stop_level, const msg_solve_par
 
 /* How a level solve is driven (no counterpart in the reference; results are identical):
  * 0: the host reads the per-pair flags after every Armijo energy pass;
- * 1: device-driven — control runs in the last workgroup of each kernel, the host checks flags
- *    every 8 iterations; 2: as 1 with the 8 iterations replayed as one hipGraph. Default 0.
+ * 1: device-driven — each control phase is its own one-workgroup-per-pair kernel after the pass it
+ *    reduces, the host checks flags every 8 iterations; 2: as 1 with the 8 iterations replayed as
+ *    one hipGraph. Default 0.
  * msg_set_default_mode applies to contexts created later and to msg_match_series. */
 int msg_ctx_set_mode(msg_ctx *ctx, int mode);
 /* The registration configurator of the context's solves (StandardRegistrationMultilevelDescent's
  * RegistrationConfiguratorType, matchSeries.cpp:17-21 and 194-197): MSG_SSD (default) or MSG_NCC. */
 int msg_ctx_set_data_term(msg_ctx *ctx, int data_term);
 int msg_set_default_mode(int mode);
+/* FP32 fields with the SSD data term compute the energy passes in FP32 arithmetic. A step decision
+ * of the Armijo search (G >= 1/2, f >= E; ArmijoSearch.h:148-190) whose margin lies within the
+ * passes' error bound is then taken on both values evaluated again in FP64 arithmetic at the stored
+ * positions (on: 1, the default), so that FP32 rounding alone does not flip it; 0 decides on the
+ * FP32-arithmetic values. No counterpart in the reference; applies to contexts created later and to
+ * msg_match_series. */
+int msg_set_default_exact_decisions(int on);
 
 /* Totals of the last msg_solve / msg_solve_level: iterations, reference-equivalent energy
  * evaluations and energy passes summed over pairs and levels. */
@@ -274,9 +283,41 @@ int msg_reduce_deformations(int dtype, const void *deformations, int n, int leve
 int msg_release_cache(void);
 
 /* ------------------------------------------------------------------------------------------------
+ * Learned warm start (no counterpart in the reference): a BF16 U-Net on MFMA that predicts a
+ * deformation from a (template, reference) pair; the gradient flow then refines from it
+ * (msg_set_deformations_device / msg_solve_level). Forward, backward and Adam run on the GPU; FP32
+ * master weights, BF16 activations and weights in the products, FP32 accumulation.
+ * Network: `scales` scales of two 3x3 convolutions + ReLU each (base_channels * 2^s channels), 2x2
+ * average pooling down, nearest-neighbour upsampling + concatenation with the skip tensor up, a 1x1
+ * head to 2 channels. Inputs: device FP32 [batch][2][H][W] (template, reference images); outputs and
+ * targets: device FP32 [batch][2][H][W] displacements in pixels (x, then y). H and W must be multiples
+ * of 32 * 2^(scales-1). Parameters: one flat FP32 vector, per convolution its weights
+ * [9][Cout][Cin] (tap = 3 (dy + 1) + dx + 1; the input's 2 channels padded to 16 with zero weights)
+ * and its biases, in network order, then the head's [2][C] weights and 2 biases.
+ * ---------------------------------------------------------------------------------------------- */
+typedef struct msg_unet msg_unet;
+/* He-normal weights from `seed`, zero biases. */
+int msg_unet_create(msg_unet **net, int scales, int base_channels, unsigned long long seed);
+int msg_unet_destroy(msg_unet *net);
+int msg_unet_param_count(msg_unet *net, long long *count);
+/* Host copies of the flat parameter vector (setting also resets the Adam state). */
+int msg_unet_get_params(msg_unet *net, float *host);
+int msg_unet_set_params(msg_unet *net, const float *host);
+/* out = net(in). `stream` (or NULL) is synchronised before the network reads `in`; the call returns
+ * when `out` is written. */
+int msg_unet_forward(msg_unet *net, const float *in, float *out, int batch, int H, int W, void *stream);
+/* Gradient of the loss 1/(2 n) sum (net(in) - target)^2 (n = 2 batch H W) with respect to every
+ * parameter, kept for msg_unet_adam_step; loss and grad_host (param_count floats) may be NULL. */
+int msg_unet_gradient(msg_unet *net, const float *in, const float *target, int batch, int H, int W, double *loss,
+                      float *grad_host, void *stream);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) with learning rate lr on the last gradient. */
+int msg_unet_adam_step(msg_unet *net, float lr);
+
+/* ------------------------------------------------------------------------------------------------
  * Kernel timing (no counterpart in the reference; used by the benchmark). With class bit k set in
  * `mask`, every launch of kernel class k is bracketed by HIP events on its own stream. Classes:
- * 0 energy, 1 variation, 2 H1 V-cycle, 3 CG direction, 4 update, 5 control, 6 composition.
+ * 0 energy, 1 variation, 2 H1 V-cycle, 3 (none: the CG direction is formed inside the energy and
+ * update kernels), 4 update, 5 control, 6 composition.
  * msg_profile_read returns the summed event time, launch count and algorithmic bytes of a class.
  * ---------------------------------------------------------------------------------------------- */
 int msg_profile_enable(unsigned mask);

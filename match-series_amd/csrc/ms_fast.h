@@ -89,14 +89,17 @@ __device__ __forceinline__ float bilin_f(__amdgpu_buffer_rsrc_t T, const HitF &p
   return p.ly * (b - a) + a;
 }
 
-// a^T K_loc b of the Q1 stiffness matrix in the difference form of stiff_form
-__device__ __forceinline__ float stiff_bi_f(const float a[4], const float b[4]) {
+// a^T K_loc b of the Q1 stiffness matrix in the difference form of stiff_form: the edge and diagonal
+// sums in FP32, their weights 1/6 and 2/6 applied in FP64 (FP32's 1/6 is 3e-8 too large, a bias
+// that every element would share)
+__device__ __forceinline__ double stiff_bi_f(const float a[4], const float b[4]) {
 #pragma clang fp contract(fast)
   const float ea0 = a[0] - a[1], ea1 = a[0] - a[2], ea2 = a[1] - a[3], ea3 = a[2] - a[3];
   const float da0 = a[0] - a[3], da1 = a[1] - a[2];
   const float eb0 = b[0] - b[1], eb1 = b[0] - b[2], eb2 = b[1] - b[3], eb3 = b[2] - b[3];
   const float db0 = b[0] - b[3], db1 = b[1] - b[2];
-  return (ea0 * eb0 + ea1 * eb1 + ea2 * eb2 + ea3 * eb3) * (1.0f / 6.0f) + (da0 * db0 + da1 * db1) * (2.0f / 6.0f);
+  const float e = ea0 * eb0 + ea1 * eb1 + ea2 * eb2 + ea3 * eb3, d = da0 * db0 + da1 * db1;
+  return (double)e * (1.0 / 6.0) + (double)d * (2.0 / 6.0);
 }
 
 // ================================================================================================
@@ -111,7 +114,7 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
   const int wg = xcd_remap(blockIdx.x, tiles * npairs);
   const int pair = wg / tiles, tile = wg - pair * tiles;
   const int nc = st[pair].ncand;
-  if (nc == 0) return;  // uniform across the workgroup
+  if (nc == 0 || st[pair].exact) return;  // uniform across the workgroup (k_energy_exact_f32 takes exact pairs)
   const int dm = st[pair].dirmode;
   __shared__ double red[4 * 2 * MAX_CAND];
 
@@ -161,9 +164,9 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
       udx[q] *= nm1;
       udy[q] *= nm1;
     }
-    const double sff = (double)stiff_bi_f(f0, f0) + (double)stiff_bi_f(f1, f1);
-    const double sfd = (double)stiff_bi_f(f0, d0) + (double)stiff_bi_f(f1, d1);
-    const double sdd = (double)stiff_bi_f(d0, d0) + (double)stiff_bi_f(d1, d1);
+    const double sff = stiff_bi_f(f0, f0) + stiff_bi_f(f1, f1);
+    const double sfd = stiff_bi_f(f0, d0) + stiff_bi_f(f1, d1);
+    const double sdd = stiff_bi_f(d0, d0) + stiff_bi_f(d1, d1);
 #pragma unroll
     for (int k = 0; k < MAX_CAND; ++k) {
       if (k < nc) {
@@ -194,6 +197,22 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
   }
 }
 
+// The same pass in FP64 arithmetic at the stored positions (energy_tile<float>, the pass of FP64 data
+// terms on FP32 fields) for the pairs whose control flagged a near-tie step decision (`exact`). Its own
+// kernel so that the FP32 pass keeps its register budget; a grid-stride loop over the tiles of the
+// flagged pairs, so that a pass without any costs one check per pair.
+__global__ void __launch_bounds__(256)
+k_energy_exact_f32(const float *__restrict__ R, const float *__restrict__ T, const float *__restrict__ phi,
+                   const float *__restrict__ z, const float *__restrict__ dold, int n, double h, int width,
+                   const PairState *st, double *__restrict__ part, int ntx, int nty, int npairs) {
+  const int tiles = ntx * nty;
+  for (int pair = 0; pair < npairs; ++pair) {
+    if (!st[pair].exact || st[pair].ncand == 0) continue;  // uniform across the workgroup
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+      energy_tile<float>(R, T, phi, z, dold, n, h, width, st, part, ntx, nty, pair, tile);
+  }
+}
+
 // ================================================================================================
 // SSD variation DE = F_SSD + lambda K phi (k_grad<S, DT_SSD> in FP32 arithmetic).
 // ================================================================================================
@@ -206,7 +225,7 @@ k_grad_f32(const float *__restrict__ R, const float *__restrict__ T, const float
   const int wg = xcd_remap(blockIdx.x, tiles * npairs);
   const int pair = wg / tiles, tile = wg - pair * tiles;
   if (st && !participates(st[pair])) return;
-  const float lambda = (float)(st ? st[pair].lambda : lambda_default);
+  const double lambda6 = (st ? st[pair].lambda : lambda_default) * (1.0 / 6.0);
 
   __shared__ float cf[GR_EY * GR_EX][4][2];  // force contribution per element, dof, component
   __shared__ float ck[GR_EY * GR_EX][4][2];  // stiffness contribution
@@ -258,13 +277,12 @@ k_grad_f32(const float *__restrict__ R, const float *__restrict__ T, const float
       cf[e][dof][0] = wv * a0;
       cf[e][dof][1] = wv * a1;
     }
-    // K_loc (1/6)[4 -1 -1 -2; -1 4 -2 -1; -1 -2 4 -1; -2 -1 -1 4] applied to the element's dofs. Its
+    // 6 K_loc = [4 -1 -1 -2; -1 4 -2 -1; -1 -2 4 -1; -2 -1 -1 4] applied to the element's dofs. Its
     // rows sum to zero, so row i is sum over j != i of K_ij (f_j - f_i): differences of close values
-    // instead of a sum that cancels to the FP32 ulp of f for smooth fields.
-    const float kl[4][4] = {{4.f / 6, -1.f / 6, -1.f / 6, -2.f / 6},
-                            {-1.f / 6, 4.f / 6, -2.f / 6, -1.f / 6},
-                            {-1.f / 6, -2.f / 6, 4.f / 6, -1.f / 6},
-                            {-2.f / 6, -1.f / 6, -1.f / 6, 4.f / 6}};
+    // instead of a sum that cancels to the FP32 ulp of f for smooth fields. The integer weights are
+    // exact; the factor lambda / 6 is applied in FP64 (FP32's 1/6 carries a 3e-8 bias).
+    const float kl[4][4] = {{4.f, -1.f, -1.f, -2.f}, {-1.f, 4.f, -2.f, -1.f}, {-1.f, -2.f, 4.f, -1.f},
+                            {-2.f, -1.f, -1.f, 4.f}};
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       float s0 = 0.f, s1 = 0.f;
@@ -296,8 +314,8 @@ k_grad_f32(const float *__restrict__ R, const float *__restrict__ T, const float
     }
   float *o = DE + pair * 2 * nn;
   const int i = x + y * n;
-  o[i] = lambda * K0 + F0;
-  o[nn + i] = lambda * K1 + F1;
+  o[i] = (float)(lambda6 * (double)K0 + (double)F0);
+  o[nn + i] = (float)(lambda6 * (double)K1 + (double)F1);
 }
 
 // ================================================================================================

@@ -24,10 +24,18 @@
 #include "ms_inverse.h"
 
 using msg::PairState;
+namespace {
+thread_local std::string g_err;
+}
+namespace msg {
+// Records an error of another translation unit of the library for msg_last_error().
+int set_error(int code, const std::string &m) {
+  g_err = m;
+  return code;
+}
+}  // namespace msg
 
 namespace {
-
-thread_local std::string g_err;
 
 int fail(int code, const std::string &m) {
   g_err = m;
@@ -115,6 +123,7 @@ struct Profiler {
 Profiler g_prof;
 thread_local bool g_capturing = false;  // no event timing inside a stream capture
 int g_default_mode = 0;
+int g_default_exact = 1;
 
 hipEvent_t prof_event() {
   if (!g_prof.pool.empty()) {
@@ -233,7 +242,8 @@ size_t energy_part_size(int n, int npairs) {
 // The images of the data term: R, T (SSD, NCC) or, for DT_CONS, the npsi deformations in T.
 template <typename S>
 int launch_energy(const S *R, const S *T, const S *phi, const S *z, const S *dold, int n, const PairState *st,
-                  double *part, int npairs, hipStream_t s, int data_term, double bytes = 0, int npsi = 0) {
+                  double *part, int npairs, hipStream_t s, int data_term, double bytes = 0, int npsi = 0,
+                  bool exact_pass = true) {
   ProfScope ps(KC_ENERGY, s, bytes);
   const EnergyGeom g = energy_geom(n);
   const double h = 1.0 / (double)(n - 1);
@@ -244,9 +254,15 @@ int launch_energy(const S *R, const S *T, const S *phi, const S *z, const S *dol
   else if (data_term == msg::DT_NCC)
     hipLaunchKernelGGL((msg::k_energy_ncc<S>), dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, z, dold, n, h,
                        st, part, g.ntx, g.nty, npairs);
-  else if constexpr (std::is_same<S, float>::value)
+  else if constexpr (std::is_same<S, float>::value) {
     hipLaunchKernelGGL(msg::k_energy_f32, dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, z, dold, n, h * h, st,
                        part, g.ntx, g.nty, npairs);
+    if (exact_pass) {  // the pairs flagged for an FP64-arithmetic evaluation (none: a check per pair)
+      HIPCHK(hipGetLastError());
+      hipLaunchKernelGGL(msg::k_energy_exact_f32, dim3(std::min(g.nblk, 1024)), dim3(256), 0, s, R, T, phi, z, dold,
+                         n, h, width, st, part, g.ntx, g.nty, npairs);
+    }
+  }
   else
     hipLaunchKernelGGL((msg::k_energy<S>), dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, z, dold, n, h, width,
                        st, part, g.ntx, g.nty, npairs);
@@ -462,6 +478,7 @@ struct msg_ctx {
   std::vector<GraphEntry> graphs;
   std::vector<double> level_start;  // per pair: E and lambda*E_reg at the start of the last level solve
   int data_term = msg::DT_SSD;
+  int exact_decisions = 1;      // FP32 SSD: near-tie step decisions on FP64-arithmetic values
   // reduceDeformations: the deformations (device, npsi fields at level L) the consistency energy reads,
   // and the descent configuration of plain H1GradientDescent
   const void *psi = nullptr;
@@ -537,6 +554,17 @@ int set_image(msg_ctx *c, void **lev, int pair, const double *img) {
   return rebuild_pyramid<S>(c, lev, pair, 1);
 }
 
+// Relative error bound of one FP32-arithmetic SSD energy evaluation against FP64 arithmetic on the
+// same FP32 fields (with a margin over the largest difference measured on STEM-like frames of
+// 64^2 to 1024^2 and displacements of up to 6 pixels). Step decisions whose margin lies within
+// it are taken on FP64-arithmetic values (needs_exact). FP64 fields and the FP64-arithmetic data
+// terms never need it.
+constexpr double F32_PASS_TOL = 2e-8;
+template <typename S>
+double exact_tol_for(const msg_ctx *c) {
+  return (std::is_same<S, float>::value && c->data_term == msg::DT_SSD && c->exact_decisions) ? F32_PASS_TOL : 0.0;
+}
+
 // ------------------------------------------------------------------------------------------------
 // Batched level solve (GradientDescentBase::applySingle for every pair at once)
 // ------------------------------------------------------------------------------------------------
@@ -583,6 +611,7 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
   a.sigma_fixed = c->sigma_fixed;
   a.sigma0 = c->sigma0;
   a.grad_stop = c->grad_stop;
+  a.exact_tol = exact_tol_for<S>(c);
 
   hipLaunchKernelGGL(k_state_init, dim3(cdiv(B, 64)), dim3(64), 0, s, c->st, B, lambda);
   HIPCHK(hipGetLastError());
@@ -714,6 +743,7 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
   a.sigma_fixed = c->sigma_fixed;
   a.sigma0 = c->sigma0;
   a.grad_stop = c->grad_stop;
+  a.exact_tol = exact_tol_for<S>(c);
   a.slot = 0;
   // Each control phase is its own one-workgroup-per-pair kernel after the pass whose partial sums it
   // reduces: kernel boundaries make the partials visible across the XCDs (a last-workgroup tail would
@@ -929,6 +959,7 @@ __global__ void k_state_bench(PairState *st, int npairs, int nc, double tau0) {
   if (p >= npairs) return;
   st[p].dirmode = msg::DIR_CG;
   st[p].ncand = nc;
+  st[p].exact = 0;
   for (int k = 0; k < nc; ++k) st[p].cand[k] = tau0 / (double)(1 << k);
 }
 
@@ -947,7 +978,7 @@ int time_energy_pass(msg_ctx *c, int l, int nc, int reps, double *ms, double *by
   int rc = MSG_OK;
   for (int r = 0; r < reps && rc == MSG_OK; ++r)
     rc = launch_energy<S>(R, T, (const S *)c->phi[l], (const S *)c->z, (const S *)c->d, n, c->st, c->part_e, B, s,
-                          c->data_term, 0.0, c->npsi);
+                          c->data_term, 0.0, c->npsi, false);
   HIPCHK(hipEventRecord(e1, s));
   HIPCHK(hipEventSynchronize(e1));
   float t = 0;
@@ -973,6 +1004,7 @@ int msg_ctx_create(msg_ctx **out, int max_level, int npairs, int dtype) {
   if (npairs < 1) return fail(MSG_ERR_ARG, "npairs must be positive");
   msg_ctx *c = new msg_ctx();
   c->mode = g_default_mode;
+  c->exact_decisions = g_default_exact;
   if (hipGetDevice(&c->device) != hipSuccess) {
     delete c;
     return fail(MSG_ERR_HIP, "no HIP device");
@@ -2402,6 +2434,19 @@ int msg_set_default_mode(int mode) {
   g_default_mode = mode;
   std::lock_guard<std::mutex> g(g_ctx_cache.mu);
   for (msg_ctx *c : g_ctx_cache.free) c->mode = mode;
+  return MSG_OK;
+}
+
+int msg_set_default_exact_decisions(int on) {
+  if (on != 0 && on != 1) return fail(MSG_ERR_ARG, "on must be 0 or 1");
+  g_default_exact = on;
+  std::lock_guard<std::mutex> g(g_ctx_cache.mu);
+  for (msg_ctx *c : g_ctx_cache.free) {
+    if (c->exact_decisions != on)
+      for (auto &gr : c->graphs) (void)hipGraphExecDestroy(gr.exec);  // the captured controls carry it
+    if (c->exact_decisions != on) c->graphs.clear();
+    c->exact_decisions = on;
+  }
   return MSG_OK;
 }
 

@@ -113,6 +113,8 @@ struct CtrlArgs {
   double sigma0;
   int grad_stop;     // 1: USE_GRADIENT_BASED_STOPPING (gradientDescent.h:416)
   int slot;          // step slot of the current iteration (0 or 1)
+  double exact_tol;  // > 0 (FP32 SSD passes): relative error bound of an evaluation; decisions whose
+                     // margin lies within it are taken on FP64-arithmetic values (0: never)
 };
 // Partial sums of the V-cycle's last pass, per field and block: <z,z>, <DE,z>, <DE,d_old>, <DE,DE>.
 constexpr int NORM_TERMS = 4;
@@ -195,17 +197,18 @@ __device__ void armijo_plan(PairState &s, int spec) {
 // template's mean / deviation.
 struct EvalRec {
   double E, R, D, M, V;
+  int ex;  // from an FP64-arithmetic pass
 };
 
 __device__ bool armijo_lookup(const PairState &s, double t, EvalRec &ev) {
   for (int j = 0; j < s.ncand; ++j)
     if (s.cand[j] == t) {
-      ev = {s.cand_E[j], s.cand_R[j], s.cand_D[j], s.cand_M[j], s.cand_V[j]};
+      ev = {s.cand_E[j], s.cand_R[j], s.cand_D[j], s.cand_M[j], s.cand_V[j], s.cand_ex};
       return true;
     }
   for (int j = 0; j < 2; ++j)
     if (s.last_tau[j] == t) {
-      ev = {s.last_E[j], s.last_R[j], s.last_D[j], s.last_M[j], s.last_V[j]};
+      ev = {s.last_E[j], s.last_R[j], s.last_D[j], s.last_M[j], s.last_V[j], s.last_ex[j]};
       return true;
     }
   return false;
@@ -218,13 +221,34 @@ __device__ void armijo_consume(PairState &s, double t, const EvalRec &ev) {
   s.last_D[1] = s.last_D[0];
   s.last_M[1] = s.last_M[0];
   s.last_V[1] = s.last_V[0];
+  s.last_ex[1] = s.last_ex[0];
   s.last_tau[0] = t;
   s.last_E[0] = ev.E;
   s.last_R[0] = ev.R;
   s.last_D[0] = ev.D;
   s.last_M[0] = ev.M;
   s.last_V[0] = ev.V;
+  s.last_ex[0] = ev.ex;
   s.evals++;
+}
+
+// A step decision compares E = E(phi + t d) with f = E(phi): Armijo's G >= 1/2, i.e.
+// E - f <= 1/2 Df t, and f >= E (ArmijoSearch.h:148-190). With FP32 energy passes (tol > 0) a margin
+// within tol (|f| + |E|) of either test could go either way on rounding alone; then both values are
+// evaluated again in FP64 arithmetic at the stored positions before deciding.
+__device__ bool needs_exact(const PairState &s, double f, double t, const EvalRec &ev, double tol) {
+  if (tol <= 0 || (ev.ex && s.f_ex) || !isfinite(ev.E) || !isfinite(f)) return false;
+  const double b = tol * (fabs(f) + fabs(ev.E));
+  return fabs(f - ev.E) <= b || fabs((ev.E - f) - ARMIJO_SIGMA * s.a_Df * t) <= b;
+}
+// Next pass: FP64 arithmetic at step t and at 0 (the current position: f).
+__device__ void plan_exact(PairState &s, double t) {
+  s.cand[0] = t;
+  s.cand[1] = 0.0;
+  s.ncand = 2;
+  s.exact = 1;
+  s.searching = 1;
+  s.passes++;
 }
 
 __device__ void armijo_start(PairState &s, int dirmode, double Df, double oldTau) {
@@ -245,6 +269,7 @@ __device__ void finish_iteration(PairState &s, double t, IterTrace *trace, int t
     EvalRec ev;
     if (armijo_lookup(s, t, ev)) {
       En = ev.E;
+      s.f_ex = ev.ex;
       Rn = ev.R;
       s.ncc_d = ev.D;  // the NCC variation at the new position needs these
       s.ncc_m = ev.M;
@@ -289,7 +314,7 @@ __device__ void finish_iteration(PairState &s, double t, IterTrace *trace, int t
 
 // Replays getTimestepWidthWithArmijoLineSearch (ArmijoSearch.h:118-200) over the evaluations
 // available, including the fall-back to the smoothed gradient (gradientDescent.h:231-235).
-__device__ void armijo_run(PairState &s, int spec, IterTrace *trace, int trace_cap, int pair, int slot) {
+__device__ void armijo_run(PairState &s, int spec, double tol, IterTrace *trace, int trace_cap, int pair, int slot) {
   const double f = s.energy;
   for (int guard = 0; guard < 256; ++guard) {
     double t = s.a_tau;
@@ -304,6 +329,10 @@ __device__ void armijo_run(PairState &s, int spec, IterTrace *trace, int trace_c
           armijo_plan(s, spec);
           return;
         }
+        if (needs_exact(s, f, t, ev, tol)) {
+          plan_exact(s, t);
+          return;
+        }
         armijo_consume(s, t, ev);
         const double E = ev.E;
         const double G = (E - f) / (s.a_Df * t);
@@ -314,6 +343,10 @@ __device__ void armijo_run(PairState &s, int spec, IterTrace *trace, int trace_c
       const double tn = t * 2.;
       if (!armijo_lookup(s, tn, ev)) {
         armijo_plan(s, spec);
+        return;
+      }
+      if (needs_exact(s, f, tn, ev, tol)) {
+        plan_exact(s, tn);
         return;
       }
       armijo_consume(s, tn, ev);
@@ -327,6 +360,10 @@ __device__ void armijo_run(PairState &s, int spec, IterTrace *trace, int trace_c
       const double tn = t > TAU_MIN ? t * 0.5 : t;
       if (!armijo_lookup(s, tn, ev)) {
         armijo_plan(s, spec);
+        return;
+      }
+      if (needs_exact(s, f, tn, ev, tol)) {
+        plan_exact(s, tn);
         return;
       }
       armijo_consume(s, tn, ev);
@@ -403,6 +440,13 @@ __device__ void control_pair(int pair, PairState &s, const double *__restrict__ 
         s.cand_R[k] = sums[2 * k + 1];
         s.cand_D[k] = s.cand_M[k] = s.cand_V[k] = 0;
       }
+      s.cand_ex = s.exact;
+      if (s.exact) {  // the value at 0 is the FP64-arithmetic energy of the current position
+        for (int k = 0; k < nc; ++k)
+          if (s.cand[k] == 0.0) s.energy = s.cand_E[k];
+        s.f_ex = 1;
+        s.exact = 0;
+      }
     }
   }
   if (a.phase == PH_LEVEL_INIT || a.phase == PH_ARMIJO_STEP) {
@@ -429,7 +473,7 @@ __device__ void control_pair(int pair, PairState &s, const double *__restrict__ 
         s.total_evals = s.total_passes = 0;
         s.step_tau[0] = s.step_tau[1] = 0;
       } else if (s.searching) {
-        armijo_run(s, a.spec, trace, a.trace_cap, pair, a.slot);
+        armijo_run(s, a.spec, a.exact_tol, trace, a.trace_cap, pair, a.slot);
       }
     }
   } else if (a.phase == PH_ARMIJO_INIT) {
@@ -466,7 +510,7 @@ __device__ void control_pair(int pair, PairState &s, const double *__restrict__ 
       s.tau_first = 0;
       armijo_start(s, DIR_CG, Df, s.tau);
       s.searching = 1;
-      armijo_run(s, a.spec, trace, a.trace_cap, pair, a.slot);
+      armijo_run(s, a.spec, a.exact_tol, trace, a.trace_cap, pair, a.slot);
     }
   }
   if (threadIdx.x == 0 && flags) flags[pair] = (s.searching ? 1 : 0) | (s.active ? 2 : 0);
@@ -508,16 +552,16 @@ __device__ __forceinline__ bool participates(const PairState &s) {
 // ================================================================================================
 constexpr int EN_TX = 64, EN_TY = 4;  // elements per workgroup tile (one per thread)
 
+// One workgroup's tile of the pass for one pair (also the FP64-arithmetic re-evaluation of the FP32
+// pass, which calls it with S = float).
 template <typename S>
-__global__ void __launch_bounds__(256)
-k_energy(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__ phi, const S *__restrict__ z,
-         const S *__restrict__ dold, int n, double h, int width, const PairState *st,
-         double *__restrict__ part, int ntx, int nty, int npairs) {
+__device__ __forceinline__ void energy_tile(const S *__restrict__ R, const S *__restrict__ T,
+                                            const S *__restrict__ phi, const S *__restrict__ z,
+                                            const S *__restrict__ dold, int n, double h, int width,
+                                            const PairState *st, double *__restrict__ part, int ntx, int nty,
+                                            int pair, int tile) {
   const int tiles = ntx * nty;
-  const int wg = xcd_remap(blockIdx.x, tiles * npairs);
-  const int pair = wg / tiles, tile = wg - pair * tiles;
   const int nc = st[pair].ncand;
-  if (nc == 0) return;  // uniform across the workgroup
   const int dm = st[pair].dirmode;
   __shared__ double red[4];
 
@@ -595,6 +639,18 @@ k_energy(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__
       }
     }
   }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_energy(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__ phi, const S *__restrict__ z,
+         const S *__restrict__ dold, int n, double h, int width, const PairState *st,
+         double *__restrict__ part, int ntx, int nty, int npairs) {
+  const int tiles = ntx * nty;
+  const int wg = xcd_remap(blockIdx.x, tiles * npairs);
+  const int pair = wg / tiles, tile = wg - pair * tiles;
+  if (st[pair].ncand == 0) return;  // uniform across the workgroup
+  energy_tile<S>(R, T, phi, z, dold, n, h, width, st, part, ntx, nty, pair, tile);
 }
 
 // ================================================================================================

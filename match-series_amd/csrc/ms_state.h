@@ -54,7 +54,12 @@ struct PairState {
   int evals;          // evaluations consumed by this iteration's searches (reference count)
   int passes;         // energy passes launched for this iteration
   int total_evals, total_passes;
-  int pad;
+  // FP32 energy passes: a step decision whose margin lies within the pass's error bound is taken on
+  // values evaluated again in FP64 arithmetic at the stored positions
+  int exact;          // the planned pass evaluates in FP64 arithmetic (candidates: the step, and 0 for f)
+  int cand_ex;        // the candidates' values come from such a pass
+  int last_ex[2];     // the same for the last two consumed evaluations
+  int f_ex;           // `energy` (the Armijo f) comes from such a pass
 };
 
 // Per-iteration trace record (optional).

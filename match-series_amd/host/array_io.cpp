@@ -221,6 +221,12 @@ void savePNGScaled(const std::string &fileName, const double *data, int nx, int 
     mn = std::min(mn, data[i]);
     mx = std::max(mx, data[i]);
   }
+  savePNGClipScaled(fileName, data, nx, ny, mn, mx);
+}
+
+void savePNGClipScaled(const std::string &fileName, const double *data, int nx, int ny, double mn, double mx) {
+  if (nx <= 0 || ny <= 0) MSH_THROW("Cannot save ScalarArray of size 0 as PNG.");
+  const size_t cnt = (size_t)nx * ny;
   const double range = mx - mn;
   std::string raw;
   raw.reserve(cnt + ny);

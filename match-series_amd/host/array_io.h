@@ -23,6 +23,10 @@ void saveP9(const std::string &fileName, const double *data, int nx, int ny, con
 // setOverflowHandlingToCurrentValueRange; CLIP_THEN_SCALE in Vector::createOverflowHandledData,
 // vec.cpp:240-252; PNGSaveInterface::loadDataFromScalarArray2d, pngInterface.cpp:136-159).
 void savePNGScaled(const std::string &fileName, const double *data, int nx, int ny);
+// The same with the range [mn, mx] given (values clipped to it first): setOverflowHandling
+// (aol::CLIP_THEN_SCALE, mn, mx), e.g. the saturated range of setOverflowHandlingToEnhanceContrast
+// (aol/vec.h:342-345).
+void savePNGClipScaled(const std::string &fileName, const double *data, int nx, int ny, double mn, double mx);
 
 // Component-wise load/save of a displacement field from a "%d" file-name mask
 // (qc::MultiArray::load / save, quoc/multiArray.h:272-306).

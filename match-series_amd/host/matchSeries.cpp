@@ -2,17 +2,22 @@
 // matchSeries.cpp:11-209, matchSeries.h:1249-1320) over the gfx950 library's C ABI.
 //
 // deformationModel 22 (SSD data term) and 0 (normalised cross-correlation), each with the Dirichlet
-// regulariser and the H1 gradient flow, are computed on the GPU; ACTION 0 (match and average) and 1 (only average) are supported. The output tree is the
+// regulariser and the H1 gradient flow, are computed on the GPU. ACTION 0 (match and average), 1 (only
+// average), 2 (analyse the saved deformations) and 3 (apply their sum to the templates) as the
+// reference's SeriesMatching::doAction (matchSeries.h:782-799). The output tree is the
 // reference's: <saveDirectory>/{log.txt, parameter-dump.txt, stage<k>/...}. Errors are reported as
 // the reference does (message on stderr, exit code 0); MSG_STRICT=1 makes them exit with 1.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <memory>
 #include <sstream>
@@ -295,6 +300,19 @@ void reconstructAndWrite(const std::string &dir, const void *stack, int k, int N
   }
 }
 
+// saveNamedDeformedTemplate / saveNamedDeformedTemplateHelper (matchSeries.h:332-351): DM3/DM4 inputs
+// optionally back into a copy of their own container, everything else as <basename>.q2bz in the
+// save directory.
+void saveNamedDeformedTemplate(const ParameterParser &p, const std::string &file, const double *a, int N) {
+  const std::string dir = dirOf(p);
+  const bool dm = file.size() >= 3 && (file.compare(file.size() - 3, 3, "dm3") == 0 ||
+                                       file.compare(file.size() - 3, 3, "dm4") == 0);
+  if (dm && p.checkAndGetBool("saveNamedDeformedDMXTemplatesAsDMX"))
+    saveInDMContainer(file, a, N, N, dir + baseFileName(file));
+  else
+    saveP9(dir + baseFileName(file) + ".q2bz", a, N, N);
+}
+
 // The optional outputs of SeriesMatching::averageSeries (matchSeries.h:365-423) besides the final
 // reconstruction: <basename>.q2bz, each template deformed by its own deformation with 0 (or its mean)
 // outside (saveNamedDeformedTemplates; in stage 1 also the undeformed reference);
@@ -311,16 +329,7 @@ void seriesExtras(const ParameterParser &p, int stage, const std::vector<std::st
   const bool namedDef = p.checkAndGetBool("saveNamedDeformations");
   const int inc = p.hasVariable("averageSaveIncrement") ? p.getInt("averageSaveIncrement") : n;
   if (inc <= 0) MSH_THROW("'averageSaveIncrement' must be bigger than zero.");
-  // saveNamedDeformedTemplateHelper (matchSeries.h:336-344): DM3/DM4 inputs optionally back into a copy
-  // of their own container, everything else as <basename>.q2bz
-  auto saveNamed = [&](const std::string &file, const double *a) {
-    const bool dm = file.size() >= 3 && (file.compare(file.size() - 3, 3, "dm3") == 0 ||
-                                         file.compare(file.size() - 3, 3, "dm4") == 0);
-    if (dm && p.checkAndGetBool("saveNamedDeformedDMXTemplatesAsDMX"))
-      saveInDMContainer(file, a, N, N, dir + baseFileName(file));
-    else
-      saveP9(dir + baseFileName(file) + ".q2bz", a, N, N);
-  };
+  auto saveNamed = [&](const std::string &file, const double *a) { saveNamedDeformedTemplate(p, file, a, N); };
   if (namedTpl && stage == 1) {
     const std::string ref = p.getStringExpandTilde("reference");
     int L = 0;
@@ -593,6 +602,207 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
                dt);
 }
 
+// aol::MixedFormat(3, 6), the format aol::Vec prints its entries with (aol.cpp:167-216): fixed
+// "%10.6Lf", or "%.2e" after a blank (two for non-negative values) when the value rounded to 1e-6
+// has an integer part of 100 or more, or none and a fraction below 1e-3.
+std::string mixedFormat(double v) {
+  const int before = 3, after = 6;
+  const long double val = v;
+  const long double mx = powl(10.0L, (long double)(before - 1)), eps = powl(10.0L, (long double)(-after));
+  const long double rval = eps * rintl(val / eps);
+  long double l;
+  const long double r = fabsl(modfl(rval, &l));
+  std::ostringstream oss;
+  if (std::isinf(v)) {
+    oss << std::string(before - 2, ' ') << std::setw(2 + 1 + after) << std::left << (v > 0 ? " INF" : "-INF");
+  } else if (std::isnan(v)) {
+    oss << std::string(before - 1, ' ') << std::setw(1 + 1 + after) << std::left << "NAN";
+  } else if (v == 0.0) {
+    oss << std::setw(before) << std::right << 0 << std::string(1 + after, ' ');
+  } else if (fabsl(l) >= mx || (l == 0.0L && r < sqrtl(eps))) {
+    oss << std::setprecision(after - 4) << std::left << std::scientific << std::string(before - 2, ' ')
+        << (val >= 0.0L ? " " : "") << val;
+  } else {
+    oss << strprintf("%*.*Lf", before + after + 1, after, val);
+  }
+  return oss.str();
+}
+
+// aol::Vector::getSaturatedMinMaxValue (aol/vec.h:864-870) as setOverflowHandlingToEnhanceContrast
+// uses it (aol/vec.h:342-345): the sorted values at int(p/200 (n-1)) and int((1 - p/200)(n-1)),
+// the upper one raised to min + 1 unless above the lower.
+void enhancedRange(const std::vector<double> &v, double pct, double &mn, double &mx) {
+  std::vector<double> t(v);
+  std::sort(t.begin(), t.end());
+  const int m1 = (int)t.size() - 1;
+  mn = t[std::min(std::max((int)(pct / 200 * m1), 0), m1)];
+  const double hi = t[std::min(std::max((int)((1 - pct / 200) * m1), 0), m1)];
+  mx = hi > mn ? hi : mn + 1;
+}
+
+// ACTION 2: SeriesMatching::analyzeDeformations (matchSeries.h:578-676) on the stage's saved
+// deformations: per frame the mean displacement (the translation) and the largest and mean pointwise
+// norm of the rest (the non-rigid component), with "saveDeformedTemplates" each template deformed by
+// its deformation (defTempl_<iii>.dat.bz2, +Inf outside, and a PNG preview with the template's mean
+// outside) and with "saveXDerivativeOfDef" the pointwise norm of the x derivative of the non-rigid
+// component (def_<iii>.q2bz). The translation plots go through gnuplot as in the reference.
+void analyzeDeformations(const ParameterParser &p) {
+  const std::string dir = dirOf(p);
+  const int L = p.getInt("precisionLevel"), N = 1 << L;
+  const size_t nn = (size_t)N * N;
+  const int dt = fieldType(p), es = dt == MSG_F64 ? 8 : 4;
+  const double H = 1.0 / (N - 1);
+  const std::vector<std::string> files = createTemplateFileNameList(p);
+  const int n = (int)files.size();
+  const bool saveDef = p.checkAndGetBool("saveDeformedTemplates");
+  const int nearest = p.checkAndGetBool("saveDeformedTemplatesUsingNearestNeighborInterpolation") ? 1 : 0;
+  std::vector<std::array<double, 2>> tr(n);
+  double avgMax = 0, avgMean = 0;
+  std::vector<double> phi, temp(nn), norms(nn);
+  for (int i = 0; i < n; ++i) {
+    loadField(deformationBase(p, dir, i, L) + "_%d.dat.bz2", N, phi);
+    if (saveDef) {
+      // loadAndPrepareImage(file, curTemplate, NoScaling = true): unnormalised, pre-smoothed
+      DeviceBuffer tpl((size_t)es * nn), d((size_t)es * 2 * nn), out((size_t)es * nn);
+      prepareFrame(p, loadImage(files[i]), files[i], L, dt, false, true, false, tpl.p);
+      upload(d.p, phi.data(), 2 * nn, dt);
+      msgCheck(msg_warp(dt, tpl.p, d.p, out.p, N, 1, 0, 0, INFINITY, nearest, nullptr));
+      download(temp.data(), out.p, nn, dt);
+      saveP9(strprintf("%sdefTempl_%03d.dat.bz2", dir.c_str(), i), temp.data(), N, N);
+      std::vector<double> t(nn);
+      download(t.data(), tpl.p, nn, dt);
+      const double m = meanOf(t);
+      for (size_t k = 0; k < nn; ++k)
+        if (std::isinf(temp[k])) temp[k] = m;
+      double mn, mx;
+      if (p.hasVariable("enhanceContrastSaturationPercentage")) {
+        enhancedRange(temp, p.getDouble("enhanceContrastSaturationPercentage"), mn, mx);
+      } else {
+        mn = *std::min_element(temp.begin(), temp.end());
+        mx = *std::max_element(temp.begin(), temp.end());
+      }
+      savePNGClipScaled(strprintf("%sdefTempl_%03d-%f_%f.png", dir.c_str(), i, mn, mx), temp.data(), N, N, mn, mx);
+    }
+    for (int c = 0; c < 2; ++c) {  // qc::MultiArray::getMeanValue: sum in order / size
+      double s = 0;
+      for (size_t k = 0; k < nn; ++k) s += phi[c * nn + k];
+      tr[i][c] = s / (double)nn;
+      for (size_t k = 0; k < nn; ++k) phi[c * nn + k] += -tr[i][c];
+    }
+    for (size_t k = 0; k < nn; ++k) {
+      double s2 = 0;
+      s2 += phi[k] * phi[k];
+      s2 += phi[nn + k] * phi[nn + k];
+      norms[k] = std::sqrt(s2);
+    }
+    const double nmax = *std::max_element(norms.begin(), norms.end());
+    double s = 0;
+    for (size_t k = 0; k < nn; ++k) s += norms[k];
+    avgMax += nmax;
+    avgMean += s / (double)nn;
+    double t2 = 0;
+    t2 += tr[i][0] * tr[i][0];
+    t2 += tr[i][1] * tr[i][1];
+    std::cerr << "Translation " << mixedFormat(tr[i][0]) << " " << mixedFormat(tr[i][1])
+              << ", non-rigid component max " << nmax / std::sqrt(t2) * 100 << "%, " << nmax / H << " pixels"
+              << std::endl;
+    if (p.checkAndGetBool("saveXDerivativeOfDef")) {  // ScalarArrayHelper::getPointWiseXDerivNorm
+      for (int y = 0; y < N; ++y)
+        for (int x = 0; x < N; ++x) {
+          double v2 = 0;
+          for (int c = 0; c < 2; ++c) {
+            const double *a = phi.data() + c * nn + (size_t)y * N;
+            const double dx = x == 0 ? a[1] - a[0] : (x == N - 1 ? a[x] - a[x - 1] : 0.5 * (a[x + 1] - a[x - 1]));
+            const double g = dx / H;
+            v2 += g * g;
+          }
+          temp[(size_t)y * N + x] = std::sqrt(v2);
+        }
+      saveP9(strprintf("%sdef_%03d.q2bz", dir.c_str(), i), temp.data(), N, N);
+    }
+  }
+  avgMax /= n;
+  avgMean /= n;
+  double avgTr = 0;
+  for (int i = 1; i < n; ++i) {
+    const double a = tr[i][0] - tr[i - 1][0], b = tr[i][1] - tr[i - 1][1];
+    double q = 0;
+    q += a * a;
+    q += b * b;
+    avgTr += std::sqrt(q);
+  }
+  avgTr /= (n - 1);
+  std::cerr << "average translation = " << avgTr << ", " << avgTr / H << " pixels" << std::endl;
+  std::cerr << "average non-rigid component = " << avgMean << ", " << avgMean / H << " pixels" << std::endl;
+  std::cerr << "ratio  = " << avgMean / avgTr << std::endl;
+  std::cerr << "average non-rigid component max = " << avgMax << ", " << avgMax / H << " pixels" << std::endl;
+  std::cerr << "ratio  = " << avgMax / avgTr << std::endl;
+  // <dir>trans.png (the translations as a curve) and <dir>norms.png (their norms over the frame
+  // number), drawn by gnuplot as aol::Plotter does (aol/gnuplotter.h)
+  auto plot = [&](const std::string &base, bool curve) {
+    const std::string dat = base + ".dat", gp = base + ".gnuplot";
+    {
+      std::ofstream f(dat);
+      f.precision(17);
+      for (int i = 0; i < n; ++i) {
+        if (curve) f << tr[i][0] << " " << tr[i][1] << "\n";
+        else f << (i + 1) << " " << std::sqrt(tr[i][0] * tr[i][0] + tr[i][1] * tr[i][1]) << "\n";
+      }
+    }
+    {
+      std::ofstream f(gp);
+      f << "set terminal png\nset output \"" << base << ".png\"\nplot \"" << dat << "\" with lines notitle\n";
+    }
+    if (std::system(("gnuplot \"" + gp + "\"").c_str()) != 0)
+      std::cerr << "aol::runGnuplot: Calling gnuplot returned an error." << std::endl;
+    std::remove(dat.c_str());
+    std::remove(gp.c_str());
+  };
+  plot(dir + "trans", true);
+  plot(dir + "norms", false);
+}
+
+// ACTION 3: SeriesMatching::applyDeformation (matchSeries.h:706-734): the sum of the saved
+// deformations <deformationBaseNamePattern with i, i> for i = numDeformations-1 down to 0
+// (setTransformation + addTransformationTo: a plain sum), applied to every template (0 or the
+// template's mean outside, optionally nearest neighbour) and written as <basename>.q2bz (or into a
+// copy of the DM3/DM4 container). Stage 1 puts the reference in front of the templates; the loop
+// keeps the count of templates without it, as the reference does.
+void applyDeformation(const ParameterParser &p, int stage) {
+  const std::string dir = dirOf(p);
+  const int L = p.getInt("precisionLevel"), N = 1 << L;
+  const size_t nn = (size_t)N * N;
+  const int dt = fieldType(p), es = dt == MSG_F64 ? 8 : 4;
+  std::vector<std::string> files = createTemplateFileNameList(p);
+  const int count = (int)files.size();
+  if (stage == 1) files.insert(files.begin(), p.getStringExpandTilde("reference"));
+  const int numDef = p.getInt("numDeformations");
+  const std::string pat = p.getString("deformationBaseNamePattern");
+  std::vector<double> sum, phi;
+  loadField(strprintf(pat.c_str(), numDef - 1, numDef - 1) + "_%d.dat.bz2", N, sum);
+  for (int i = numDef - 2; i >= 0; --i) {
+    loadField(strprintf(pat.c_str(), i, i) + "_%d.dat.bz2", N, phi);
+    for (size_t k = 0; k < 2 * nn; ++k) sum[k] += phi[k];
+  }
+  DeviceBuffer d((size_t)es * 2 * nn), tpl((size_t)es * nn), out((size_t)es * nn);
+  upload(d.p, sum.data(), 2 * nn, dt);
+  const bool extMean = p.checkAndGetBool("saveNamedDeformedTemplatesExtendedWithMean");
+  const int nearest = p.checkAndGetBool("saveNamedDeformedTemplatesUsingNearestNeighborInterpolation") ? 1 : 0;
+  std::vector<double> h(nn);
+  for (int i = 0; i < count; ++i) {
+    // loadAndPrepareImage(file, curTemplate, NoScaling, NoResizeOrCrop = false, NoSmoothing)
+    loadRaw(p, files[i], L, dt, tpl.p);
+    double ext = 0;
+    if (extMean) {
+      download(h.data(), tpl.p, nn, dt);
+      ext = meanOf(h);
+    }
+    msgCheck(msg_warp(dt, tpl.p, d.p, out.p, N, 1, 0, 0, ext, nearest, nullptr));
+    download(h.data(), out.p, nn, dt);
+    saveNamedDeformedTemplate(p, files[i], h.data(), N);
+  }
+}
+
 void doAction(ParameterParser &p, int action, bool loadStageOneResults, int stage) {
   // the registration object's constructor prepares the stage directory (registration.h:263-274)
   makeDirectory(dirOf(p));
@@ -600,7 +810,9 @@ void doAction(ParameterParser &p, int action, bool loadStageOneResults, int stag
   switch (action) {
     case 0: matchAndAverage(p, loadStageOneResults, stage); break;
     case 1: averageOnly(p, stage); break;
-    default: MSH_THROW(strprintf("ACTION %d is not supported by this build (0: match and average, 1: average)", action));
+    case 2: analyzeDeformations(p); break;
+    case 3: applyDeformation(p, stage); break;
+    default: MSH_THROW("SeriesMatching::doAction: Invalid ActionType");
   }
 }
 

@@ -27,7 +27,8 @@ def build_lib():
     out = os.path.join(PKG, "lib", "libmsgpu.so")
     tmp = out + ".tmp"
     run([HIPCC] + HIP_FLAGS + ["-shared", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(PKG, "csrc"),
-                               os.path.join(PKG, "csrc", "ms_gpu.hip"), "-o", tmp])
+                               os.path.join(PKG, "csrc", "ms_gpu.hip"), os.path.join(PKG, "csrc", "ms_unet.hip"),
+                               "-o", tmp])
     os.replace(tmp, out)
     return out
 

@@ -94,6 +94,7 @@ SIGNATURES = {
     "msg_ncc_gradient": ([ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp],
                          ctypes.c_int),
     "msg_set_default_mode": ([ctypes.c_int], ctypes.c_int),
+    "msg_set_default_exact_decisions": ([ctypes.c_int], ctypes.c_int),
     "msg_profile_enable": ([ctypes.c_uint], ctypes.c_int),
     "msg_profile_reset": ([], ctypes.c_int),
     "msg_profile_read": ([ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong), _dp], ctypes.c_int),
@@ -103,6 +104,15 @@ SIGNATURES = {
     "msg_resample_rect": ([ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "msg_approx_inverse": ([ctypes.c_int, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "msg_register_translation": ([ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _ip, _vp, _vp], ctypes.c_int),
+    "msg_unet_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong], ctypes.c_int),
+    "msg_unet_destroy": ([_vp], ctypes.c_int),
+    "msg_unet_param_count": ([_vp, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+    "msg_unet_get_params": ([_vp, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "msg_unet_set_params": ([_vp, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "msg_unet_forward": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+    "msg_unet_gradient": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_float),
+                           _vp], ctypes.c_int),
+    "msg_unet_adam_step": ([_vp, ctypes.c_float], ctypes.c_int),
     "msg_reduce_deformations": ([ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _dp,
                                  ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
 }
@@ -625,3 +635,77 @@ def set_default_mode(mode):
 def release_cache():
     """Frees the contexts msg_match_series keeps for reuse."""
     check(lib().msg_release_cache())
+
+
+# ------------------------------------------------------------------------------------------------
+# Learned warm start (msg_unet_*)
+# ------------------------------------------------------------------------------------------------
+class UNet:
+    """The warm-start network on the GPU: forward, gradient and Adam steps through the C ABI.
+    Inputs (B, 2, H, W) float32 device tensors (template, reference); outputs and targets
+    (B, 2, H, W) float32 displacements in pixels."""
+
+    def __init__(self, scales=4, base=32, seed=0):
+        self.h = ctypes.c_void_p()
+        check(lib().msg_unet_create(ctypes.byref(self.h), scales, base, seed))
+        self.scales, self.base = scales, base
+        n = ctypes.c_longlong()
+        check(lib().msg_unet_param_count(self.h, ctypes.byref(n)))
+        self.nparams = n.value
+
+    def close(self):
+        if self.h:
+            check(lib().msg_unet_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        a = np.zeros(self.nparams, dtype=np.float32)
+        check(lib().msg_unet_get_params(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return a
+
+    def set_params(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        if a.size != self.nparams:
+            raise ValueError(f"expected {self.nparams} parameters, got {a.size}")
+        check(lib().msg_unet_set_params(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+
+    def _io(self, x, name):
+        torch = _torch()
+        if not isinstance(x, torch.Tensor) or x.dtype != torch.float32 or x.dim() != 4 or x.shape[1] != 2:
+            raise ValueError(f"{name}: expected a float32 tensor of shape (B, 2, H, W)")
+        if not (x.is_cuda and x.is_contiguous()):
+            raise ValueError(f"{name}: expected a contiguous device tensor")
+        return x.shape[0], x.shape[2], x.shape[3]
+
+    def forward(self, x):
+        B, H, W = self._io(x, "x")
+        out = _torch().empty_like(x)
+        check(lib().msg_unet_forward(self.h, _ptr(x), _ptr(out), B, H, W, _stream()))
+        return out
+
+    def gradient(self, x, target, want_grad=False):
+        B, H, W = self._io(x, "x")
+        if tuple(target.shape) != tuple(x.shape):
+            raise ValueError("target: shape must match x")
+        self._io(target, "target")
+        loss = ctypes.c_double()
+        g = np.zeros(self.nparams, dtype=np.float32) if want_grad else None
+        check(lib().msg_unet_gradient(self.h, _ptr(x), _ptr(target), B, H, W, ctypes.byref(loss),
+                                      g.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if want_grad else None,
+                                      _stream()))
+        return (loss.value, g) if want_grad else loss.value
+
+    def adam_step(self, lr):
+        check(lib().msg_unet_adam_step(self.h, float(lr)))
+
+    def train_step(self, x, target, lr):
+        loss = self.gradient(x, target)
+        self.adam_step(lr)
+        return loss
+
