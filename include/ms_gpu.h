@@ -129,6 +129,16 @@ int msg_ctx_buffer(msg_ctx *ctx, int which /* 0 ref, 1 tpl, 2 phi */, int level,
  * per pair E_SSD(phi) and 1/2 sum_c phi_c^T K phi_c (host outputs, batch values each). */
 int msg_energy(int dtype, const void *R, const void *T, const void *phi, int n, int batch, double *e_ssd,
                double *e_reg, void *stream);
+/* msg_energy without any allocation (no counterpart in the reference). For callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc. Same kernels and summation order as msg_energy, so the results are bit-identical.
+ * Its pair states and partial sums live in a fixed 32 MiB device array that each GPU holds until the
+ * process ends; the call fails with MSG_ERR_ARG when batch x n^2 needs more (about
+ * (8 batch - 6) ceil((n-1)/64) ceil((n-1)/4) doubles: e.g. 32 pairs of 2048^2). Every call on one
+ * GPU shares that array, so calls that overlap on one GPU (several threads or streams) race;
+ * serialise them. The call synchronises `stream` before returning. */
+int msg_energy_noalloc(int dtype, const void *R, const void *T, const void *phi, int n, int batch, double *e_ssd,
+                       double *e_reg, void *stream);
 
 /* SSDForce (registration.h:1622-1667) + lambda * K phi (DirichletRegularizationConfigurator,
  * registration.h:677-710): the variation DE of the registration energy. */

@@ -1337,6 +1337,112 @@ int msg_gradient(int dtype, const void *R, const void *T, const void *phi, void 
 
 }  // extern "C"
 
+// ------------------------------------------------------------------------------------------------
+// msg_energy without allocation: the pair states, the energy pass's partial sums and the reduced
+// sums live in one file-scope device array. Each device holds its own copy of the array at its own
+// address; host code uses that address (hipGetSymbolAddress, taken once per device), never the
+// array's name, which on the host is only a stand-in.
+// ------------------------------------------------------------------------------------------------
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEV = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[NOALLOC_MAX_DEV] = {};
+
+// The current device's scratch address (looked up on the first call on that device).
+int noalloc_scratch(void **p) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEV) return fail(MSG_ERR_ARG, "device index beyond the no-allocation table");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void *q = nullptr;
+    HIPCHK(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+    if (!q) return fail(MSG_ERR_HIP, "hipGetSymbolAddress(scratch) returned no address");
+    g_noalloc_ptr[dev] = q;
+  }
+  *p = g_noalloc_ptr[dev];
+  return MSG_OK;
+}
+
+// Bytes of scratch the no-allocation energy pass needs: pair states, the partials of candidate 0
+// (terms E_SSD and E_reg of every pair, laid out as the level solve's partials) and 2 sums per pair.
+size_t noalloc_energy_bytes(int n, int batch, size_t *off_part, size_t *off_sum) {
+  const int nblk = energy_geom(n).nblk;
+  const size_t st = ((sizeof(PairState) * (size_t)batch + 255) / 256) * 256;
+  const size_t part_doubles = ((size_t)(batch - 1) * msg::MAX_CAND * 2 + 2) * (size_t)nblk;
+  const size_t part = ((sizeof(double) * part_doubles + 255) / 256) * 256;
+  *off_part = st;
+  *off_sum = st + part;
+  return st + part + sizeof(double) * 2 * (size_t)batch;
+}
+
+// Sums of the partials in block order (the order msg_energy sums them on the host): out[p] = E_SSD,
+// out[batch + p] = E_reg of pair p.
+__global__ void k_energy_sums(const double *__restrict__ part, int nblk, int batch, double *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 2 * batch) return;
+  const int p = i % batch, term = i / batch;
+  const double *src = part + msg::part_index(msg::DT_SSD, p, 0, term, nblk);
+  double a = 0;
+  for (int b = 0; b < nblk; ++b) a += src[b];
+  out[i] = a;
+}
+
+template <typename S>
+int energy_noalloc_impl(const void *R, const void *T, const void *phi, int n, int batch, double *e_ssd,
+                        double *e_reg, hipStream_t s) {
+  size_t off_part = 0, off_sum = 0;
+  const size_t need = noalloc_energy_bytes(n, batch, &off_part, &off_sum);
+  if (need > NOALLOC_SCRATCH_BYTES)
+    return fail(MSG_ERR_ARG, "msg_energy_noalloc: " + std::to_string(batch) + " pairs of " + std::to_string(n) + "^2 need " +
+                                 std::to_string(need) + " bytes of scratch, beyond its " +
+                                 std::to_string(NOALLOC_SCRATCH_BYTES) + "; use msg_energy");
+  void *base = nullptr;
+  CHK(noalloc_scratch(&base));
+  unsigned char *b = static_cast<unsigned char *>(base);
+  PairState *st = reinterpret_cast<PairState *>(b);
+  double *part = reinterpret_cast<double *>(b + off_part);
+  double *sums = reinterpret_cast<double *>(b + off_sum);
+  const EnergyGeom g = energy_geom(n);
+  const double h = 1.0 / (double)(n - 1);
+  // every word the pass reads is written first: states by k_state_init, partials by the pass
+  hipLaunchKernelGGL(k_state_init, dim3(cdiv(batch, 64)), dim3(64), 0, s, st, batch, 0.0);
+  HIPCHK(hipGetLastError());
+  if constexpr (std::is_same<S, float>::value)
+    hipLaunchKernelGGL(msg::k_energy_f32, dim3(g.nblk * batch), dim3(256), 0, s, (const float *)R, (const float *)T,
+                       (const float *)phi, (const float *)phi, (const float *)phi, n, h * h, st, part, g.ntx, g.nty,
+                       batch);
+  else
+    hipLaunchKernelGGL((msg::k_energy<S>), dim3(g.nblk * batch), dim3(256), 0, s, (const S *)R, (const S *)T,
+                       (const S *)phi, (const S *)phi, (const S *)phi, n, h, (int)((n - 1) * h), st, part, g.ntx,
+                       g.nty, batch);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_energy_sums, dim3(cdiv(2 * batch, 64)), dim3(64), 0, s, (const double *)part, g.nblk, batch,
+                     sums);
+  HIPCHK(hipGetLastError());
+  if (e_ssd) HIPCHK(hipMemcpyAsync(e_ssd, sums, sizeof(double) * batch, hipMemcpyDeviceToHost, s));
+  if (e_reg) HIPCHK(hipMemcpyAsync(e_reg, sums + batch, sizeof(double) * batch, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return MSG_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int msg_energy_noalloc(int dtype, const void *R, const void *T, const void *phi, int n, int batch, double *e_ssd,
+                       double *e_reg, void *stream) {
+  CHK(check_dtype(dtype));
+  CHK(check_n(n));
+  if (!R || !T || !phi || batch < 1) return fail(MSG_ERR_ARG, "invalid argument");
+  return dtype == MSG_F64 ? energy_noalloc_impl<double>(R, T, phi, n, batch, e_ssd, e_reg, strm(stream))
+                          : energy_noalloc_impl<float>(R, T, phi, n, batch, e_ssd, e_reg, strm(stream));
+}
+
+}  // extern "C"
+
 namespace {
 template <typename S>
 int h1_impl(const void *in, void *out, int n, int nfields, double sigma, hipStream_t s) {

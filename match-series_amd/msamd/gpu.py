@@ -76,6 +76,7 @@ SIGNATURES = {
                          ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
     "msg_ctx_buffer": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
     "msg_energy": ([ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _vp], ctypes.c_int),
+    "msg_energy_noalloc": ([ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _vp], ctypes.c_int),
     "msg_gradient": ([ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp], ctypes.c_int),
     "msg_h1_smooth": ([ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp], ctypes.c_int),
     "msg_warp": ([ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -242,6 +243,15 @@ def energy(R, T, phi):
     es = np.zeros(B)
     er = np.zeros(B)
     check(lib().msg_energy(code, _ptr(R), _ptr(T), _ptr(phi), n, B, _dptr(es), _dptr(er), _stream()))
+    return es, er
+
+
+def energy_noalloc(R, T, phi):
+    """energy() through msg_energy_noalloc: no device allocation, fixed-size scratch array."""
+    code, B, n = _pair_args(R, T, phi)
+    es = np.zeros(B)
+    er = np.zeros(B)
+    check(lib().msg_energy_noalloc(code, _ptr(R), _ptr(T), _ptr(phi), n, B, _dptr(es), _dptr(er), _stream()))
     return es, er
 
 
