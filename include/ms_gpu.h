@@ -26,9 +26,12 @@ extern "C" {
 
 enum { MSG_OK = 0, MSG_ERR_ARG = 1, MSG_ERR_HIP = 2, MSG_ERR_STATE = 3, MSG_ERR_ALLOC = 4 };
 enum { MSG_F32 = 0, MSG_F64 = 1 };
-/* Data terms: SSD (matchSeries deformationModel 22, SSDRegistrationConfigurator) and normalised
- * cross-correlation (deformationModel 0, NCCRegistrationConfigurator; registration.h:1766-1781). */
-enum { MSG_SSD = 0, MSG_NCC = 1 };
+/* Data terms: SSD (matchSeries deformationModel 22, SSDRegistrationConfigurator), normalised
+ * cross-correlation (deformationModel 0, NCCRegistrationConfigurator; registration.h:1766-1781) and the
+ * seismic row-wise x-shift (matchSeismicSeries: FELineShiftRegisEnergy, registration.h:2137-2193, with
+ * findXShift's regulariser K_xx + yRegFactor K_yy, matchSeries.h:805-868; the y component of the
+ * deformation stays zero). */
+enum { MSG_SSD = 0, MSG_NCC = 1, MSG_XSHIFT = 3 };
 
 /* Description of the last error raised on the calling thread ("" if none). */
 const char *msg_last_error(void);
@@ -63,6 +66,11 @@ int msg_set_images_device(msg_ctx *ctx, const void *d_ref, const void *d_tpl, in
 int msg_set_deformation(msg_ctx *ctx, int pair, const double *phi);
 int msg_get_deformation(msg_ctx *ctx, int pair, double *phi);
 int msg_set_deformation_zero(msg_ctx *ctx, int pair);
+/* The field of one level alone (2 x 2^level x 2^level float64, x then y), the other levels left as
+ * they are: the deformation of MultiDimMultilevelArray::setCurLevel(level) that descentOnCurrentGrid
+ * works on (registration.h:1092-1112). */
+int msg_set_deformation_level(msg_ctx *ctx, int pair, int level, const double *phi);
+int msg_get_deformation_level(msg_ctx *ctx, int pair, int level, double *phi);
 /* Device copy of all pairs' max_level fields (dtype of the context), in or out. */
 int msg_set_deformations_device(msg_ctx *ctx, const void *d_phi);
 int msg_get_deformations_device(msg_ctx *ctx, void *d_phi);
@@ -103,8 +111,12 @@ int msg_solve(msg_ctx *ctx, int start_level, int stop_level, const msg_solve_par
  * msg_set_default_mode applies to contexts created later and to msg_match_series. */
 int msg_ctx_set_mode(msg_ctx *ctx, int mode);
 /* The registration configurator of the context's solves (StandardRegistrationMultilevelDescent's
- * RegistrationConfiguratorType, matchSeries.cpp:17-21 and 194-197): MSG_SSD (default) or MSG_NCC. */
+ * RegistrationConfiguratorType, matchSeries.cpp:17-21 and 194-197): MSG_SSD (default) or MSG_NCC; or
+ * the x-shift registration of XShiftRegistrationMultilevelDescent (MSG_XSHIFT, rows of at most 2048
+ * nodes; matchSeries.h:925-982). */
 int msg_ctx_set_data_term(msg_ctx *ctx, int data_term);
+/* "yRegFactor" of the x-shift regulariser (default 1). */
+int msg_ctx_set_y_reg_factor(msg_ctx *ctx, double y_reg_factor);
 int msg_set_default_mode(int mode);
 /* FP32 fields with the SSD data term compute the energy passes in FP32 arithmetic. A step decision
  * of the Armijo search (G >= 1/2, f >= E; ArmijoSearch.h:148-190) whose margin lies within the
@@ -151,6 +163,16 @@ int msg_ncc_energy(int dtype, const void *R, const void *T, const void *phi, int
                    double *e_reg, void *stream);
 int msg_ncc_gradient(int dtype, const void *R, const void *T, const void *phi, void *DE, int n, int batch,
                      double lambda, void *stream);
+/* The x-shift energy of findXShift (projects/electronMicroscopy/matchSeries.h:805-868) per pair, phi
+ * laid out as for msg_energy with only its x component read: e_ncc = FELineShiftRegisEnergy::apply
+ * (registration.h:2146-2156, the 1-D NCC of every row, rows starting with NaN left out, divided by
+ * the number of rows) and e_reg = 1/2 phi_x^T (K_xx + y_reg_factor K_yy) phi_x (matchSeries.h:821-828).
+ * The variation: DE_x = FELineShiftRegisEnergy::applyDerivative (registration.h:2161-2175) + lambda
+ * (K_xx + y_reg_factor K_yy) phi_x, DE_y = 0. Rows of at most 2048 nodes. */
+int msg_xshift_energy(int dtype, const void *R, const void *T, const void *phi, int n, int batch, double y_reg_factor,
+                      double *e_ncc, double *e_reg, void *stream);
+int msg_xshift_gradient(int dtype, const void *R, const void *T, const void *phi, void *DE, int n, int batch,
+                        double lambda, double y_reg_factor, void *stream);
 
 /* LinearSmoothOp::apply (quoc/linearSmoothOp.cpp:591-612): one FD heat multigrid V-cycle with
  * tau = sigma^2/2 on each of nfields fields (in and out may alias). */
@@ -253,6 +275,17 @@ typedef struct {
                                       /* instead of `templates` (when the caller prepared the */
                                       /* matching inputs itself: normalised at their own      */
                                       /* resolution, then restricted, registration.h:369-399) */
+  double y_reg_factor;                /* MSG_XSHIFT: "yRegFactor"                             */
+  int keep_start_transformation;      /* the series loop sees, after each solve, the          */
+                                      /* deformation the solve started from: matchSeismic-    */
+                                      /* Series' dyadic path, whose registration object hands */
+                                      /* back only what was set (getTransformation returns    */
+                                      /* _phi, which the full-resolution step alone updates,  */
+                                      /* matchSeries.h:1078-1096, 1177-1213). A pairing       */
+                                      /* solved here composes as zero, a loaded one as loaded;*/
+                                      /* the refinement's start, not its result, stays the    */
+                                      /* accumulated deformation; each frame is averaged with */
+                                      /* its refined deformation (the files it was saved to)  */
 } msg_series_params;
 
 /* Optional per-level records of msg_match_series: what the reference writes after every level of
@@ -281,6 +314,13 @@ typedef struct {
 int msg_match_series(const msg_series_params *p, int dtype, const void *templates, int n, const void *reference,
                      const void *pairwise_in, void *deformations, void *pairwise_out, double *energies,
                      void *mean, void *count, void *median, const msg_series_levels *levels, void *reduced);
+/* Totals of the solves of the last msg_match_series in the process (no counterpart in the
+ * reference): gradient-flow iterations, reference-equivalent energy evaluations and energy passes
+ * over every pairwise, refinement and inverse solve, as msg_last_counts reports them per context;
+ * node_iterations and node_evals weight each iteration and evaluation by its level's node count
+ * (4^level), the unit of the path's byte model. Any output may be NULL. */
+int msg_series_last_counts(long long *iterations, long long *evals, long long *passes, double *node_iterations,
+                           double *node_evals);
 /* SeriesMatching::reduceDeformations (matchSeries.h:678-704): from zero, the displacement phi
  * minimising sum_i 1/2 int |psi_i(x + phi(x)) + phi(x)|^2 (ConsistencyEnergyOp and its variation,
  * AmbrosioTortorelli.h:687-800) by H1GradientDescent with filter width sqrt(10 h), nonlinear CG and

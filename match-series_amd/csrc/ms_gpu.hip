@@ -22,6 +22,7 @@
 #include "ms_stats.h"
 #include "ms_fft.h"
 #include "ms_inverse.h"
+#include "ms_xshift.h"
 
 using msg::PairState;
 namespace {
@@ -236,19 +237,25 @@ EnergyGeom energy_geom(int n) {
 
 // doubles of the energy pass's partial sums for npairs pairs at grid size n (either data term)
 size_t energy_part_size(int n, int npairs) {
-  return (size_t)npairs * (msg::MAX_CAND * 4 + 2) * energy_geom(n).nblk;
+  // the x-shift pass writes one partial per image row
+  return (size_t)npairs * (msg::MAX_CAND * 4 + 2) * std::max(energy_geom(n).nblk, n);
 }
+// partial sums per (pair, candidate, term) of the energy pass of a data term at grid size n
+int energy_blocks(int n, int data_term) { return data_term == msg::DT_XSHIFT ? n : energy_geom(n).nblk; }
 
 // The images of the data term: R, T (SSD, NCC) or, for DT_CONS, the npsi deformations in T.
 template <typename S>
 int launch_energy(const S *R, const S *T, const S *phi, const S *z, const S *dold, int n, const PairState *st,
                   double *part, int npairs, hipStream_t s, int data_term, double bytes = 0, int npsi = 0,
-                  bool exact_pass = true) {
+                  bool exact_pass = true, double yreg = 0) {
   ProfScope ps(KC_ENERGY, s, bytes);
   const EnergyGeom g = energy_geom(n);
   const double h = 1.0 / (double)(n - 1);
   const int width = (int)((n - 1) * h);
-  if (data_term == msg::DT_CONS)
+  if (data_term == msg::DT_XSHIFT)
+    hipLaunchKernelGGL((msg::k_energy_xshift<S>), dim3(n * npairs), dim3(msg::XS_THREADS), 3 * n * sizeof(double), s,
+                       R, T, phi, z, dold, n, h, yreg, st, part, npairs);
+  else if (data_term == msg::DT_CONS)
     hipLaunchKernelGGL((msg::k_energy_cons<S>), dim3(g.nblk * npairs), dim3(256), 0, s, T, npsi, phi, z, dold, n, h,
                        width, st, part, g.ntx, g.nty, npairs);
   else if (data_term == msg::DT_NCC)
@@ -272,12 +279,15 @@ int launch_energy(const S *R, const S *T, const S *phi, const S *z, const S *dol
 
 template <typename S>
 int launch_grad(const S *R, const S *T, const S *phi, S *DE, int n, const PairState *st, double lambda, int npairs,
-                hipStream_t s, int data_term, int npsi = 0) {
+                hipStream_t s, int data_term, int npsi = 0, double yreg = 0) {
   ProfScope ps(KC_GRAD, s, 6.0 * n * n * sizeof(S) * npairs);
   const int ntx = cdiv(n, msg::GR_TX), nty = cdiv(n, msg::GR_TY);
   const double h = 1.0 / (double)(n - 1);
   const int width = (int)((n - 1) * h);
-  if (data_term == msg::DT_CONS)
+  if (data_term == msg::DT_XSHIFT)
+    hipLaunchKernelGGL((msg::k_grad_xshift<S>), dim3(n * npairs), dim3(msg::XS_THREADS), 3 * n * sizeof(double), s, R,
+                       T, phi, DE, n, h, yreg, st, lambda, npairs);
+  else if (data_term == msg::DT_CONS)
     hipLaunchKernelGGL((msg::k_grad<S, msg::DT_CONS>), dim3(ntx * nty * npairs), dim3(256), 0, s, R, T, phi, DE, n, h,
                        width, st, lambda, ntx, nty, npairs, npsi);
   else if (data_term == msg::DT_NCC)
@@ -466,6 +476,7 @@ struct msg_ctx {
   int trace_cap = 0;
   hipStream_t stream = nullptr;
   long long cnt_iter = 0, cnt_evals = 0, cnt_passes = 0;
+  double cnt_node_iter = 0, cnt_node_evals = 0;  // the same weighted by the level's node count
   std::vector<void *> allocs;
   int mode = 0;                 // 0 host-synchronised rounds, 1 device-driven, 2 device-driven + hipGraph
   struct GraphEntry {
@@ -479,6 +490,7 @@ struct msg_ctx {
   std::vector<double> level_start;  // per pair: E and lambda*E_reg at the start of the last level solve
   int data_term = msg::DT_SSD;
   int exact_decisions = 1;      // FP32 SSD: near-tie step decisions on FP64-arithmetic values
+  double yreg = 1.0;            // DT_XSHIFT: "yRegFactor"
   // reduceDeformations: the deformations (device, npsi fields at level L) the consistency energy reads,
   // and the descent configuration of plain H1GradientDescent
   const void *psi = nullptr;
@@ -599,7 +611,7 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
   }
 
   msg::CtrlArgs a;
-  a.nblk_energy = eg.nblk;
+  a.nblk_energy = energy_blocks(n, c->data_term);
   a.nblk_norm = nblk_norm;
   a.lambda = lambda;
   a.stop_eps = stop_eps;
@@ -617,7 +629,7 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
   HIPCHK(hipGetLastError());
   if (c->data_term == msg::DT_NCC)
     CHK(ncc_reference_stats<S>(R, n, c->st, c->part_e, c->part_n, B, a, c->flags_d, tr, s));
-  CHK(launch_energy<S>(R, T, phi, z, d, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi));
+  CHK(launch_energy<S>(R, T, phi, z, d, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi, true, c->yreg));
   a.phase = msg::PH_LEVEL_INIT;
   a.slot = 0;
   hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, c->st, c->part_e, c->part_n, a, c->flags_d, tr);
@@ -627,7 +639,7 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
   int nactive = max_iter > 0 ? B : 0;
   for (int it = 0; any_active; ++it) {
     a.slot = it & 1;
-    CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s, c->data_term, npsi));
+    CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s, c->data_term, npsi, c->yreg));
     CHK(launch_vcycle<S>(DE, 1, z, l, mg, c->st, 1.0, 2 * B, c->part_n, d, s));
     a.phase = msg::PH_ARMIJO_INIT;
     hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, c->st, c->part_e, c->part_n, a, c->flags_d, tr);
@@ -636,7 +648,7 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
     int rounds = 0;
     int nsearch = nactive;  // pairs evaluating in this round (upper bound for the first round)
     while (searching) {
-      CHK(launch_energy<S>(R, T, phi, z, d, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi));
+      CHK(launch_energy<S>(R, T, phi, z, d, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi, true, c->yreg));
       a.phase = msg::PH_ARMIJO_STEP;
       hipLaunchKernelGGL(msg::k_control, dim3(B), dim3(msg::CTRL_THREADS), 0, s, c->st, c->part_e, c->part_n, a, c->flags_d,
                          tr);
@@ -669,6 +681,8 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
     c->cnt_iter += hs[p].iter;
     c->cnt_evals += hs[p].total_evals;
     c->cnt_passes += hs[p].total_passes;
+    c->cnt_node_iter += (double)hs[p].iter * n * n;
+    c->cnt_node_evals += (double)hs[p].total_evals * n * n;
   }
   // algorithmic bytes of the level's energy passes: the first reads R, T and phi, every later one
   // also the direction (the device counts the passes)
@@ -731,7 +745,7 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
     HIPCHK(hipMemsetAsync(tr, 0, sizeof(msg::IterTrace) * trace_cap * B, s));
   }
   msg::CtrlArgs a;
-  a.nblk_energy = eg.nblk;
+  a.nblk_energy = energy_blocks(n, c->data_term);
   a.nblk_norm = mg_norm_blocks(l, mgc_maxl<S>());
   a.lambda = lambda;
   a.stop_eps = stop_eps;
@@ -764,17 +778,17 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
   HIPCHK(hipGetLastError());
   if (c->data_term == msg::DT_NCC)
     CHK(ncc_reference_stats<S>(R, n, c->st, c->part_e, c->part_n, B, a, c->flags_d, tr, s));
-  CHK(launch_energy<S>(R, T, phi, z, d, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi));
+  CHK(launch_energy<S>(R, T, phi, z, d, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi, true, c->yreg));
   CHK(control(msg::PH_LEVEL_INIT, 0));
 
   // iteration k of a replay uses step slot k & 1 (K is even, so the slots keep alternating)
   auto iteration = [&](int k) -> int {
     const int slot = k & 1;
-    CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s, c->data_term, npsi));
+    CHK(launch_grad<S>(R, T, phi, DE, n, c->st, lambda, B, s, c->data_term, npsi, c->yreg));
     CHK(launch_vcycle<S>(DE, 1, z, l, mg, c->st, 1.0, 2 * B, c->part_n, d, s));
     CHK(control(msg::PH_ARMIJO_INIT, slot));
     for (int r = 0; r < Rr; ++r) {
-      CHK(launch_energy<S>(R, T, phi, z, d, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi));
+      CHK(launch_energy<S>(R, T, phi, z, d, n, c->st, c->part_e, B, s, c->data_term, 0.0, npsi, true, c->yreg));
       CHK(control(msg::PH_ARMIJO_STEP, slot));
     }
     {
@@ -840,6 +854,8 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
     c->cnt_iter += hs[p].iter;
     c->cnt_evals += hs[p].total_evals;
     c->cnt_passes += hs[p].total_passes;
+    c->cnt_node_iter += (double)hs[p].iter * n * n;
+    c->cnt_node_evals += (double)hs[p].total_evals * n * n;
   }
   // algorithmic bytes of the level's energy passes: the first reads R, T and phi, every later one
   // also the direction (the device counts the passes)
@@ -978,7 +994,7 @@ int time_energy_pass(msg_ctx *c, int l, int nc, int reps, double *ms, double *by
   int rc = MSG_OK;
   for (int r = 0; r < reps && rc == MSG_OK; ++r)
     rc = launch_energy<S>(R, T, (const S *)c->phi[l], (const S *)c->z, (const S *)c->d, n, c->st, c->part_e, B, s,
-                          c->data_term, 0.0, c->npsi, false);
+                          c->data_term, 0.0, c->npsi, false, c->yreg);
   HIPCHK(hipEventRecord(e1, s));
   HIPCHK(hipEventSynchronize(e1));
   float t = 0;
@@ -1133,6 +1149,24 @@ int msg_get_deformation(msg_ctx *c, int pair, double *phi) {
   return download_host(c, phi, (char *)c->phi[c->L] + pair * 2 * nn * c->es, 2 * nn);
 }
 
+int msg_set_deformation_level(msg_ctx *c, int pair, int level, const double *phi) {
+  CHK(check_pair(c, pair));
+  if (!phi) return fail(MSG_ERR_ARG, "null field");
+  if (level < 0 || level > c->L) return fail(MSG_ERR_ARG, "level outside 0..max_level");
+  const size_t nn = (size_t)1 << (2 * level);
+  CHK(upload_host(c, (char *)c->phi[level] + pair * 2 * nn * c->es, phi, 2 * nn));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MSG_OK;
+}
+
+int msg_get_deformation_level(msg_ctx *c, int pair, int level, double *phi) {
+  CHK(check_pair(c, pair));
+  if (!phi) return fail(MSG_ERR_ARG, "null field");
+  if (level < 0 || level > c->L) return fail(MSG_ERR_ARG, "level outside 0..max_level");
+  const size_t nn = (size_t)1 << (2 * level);
+  return download_host(c, phi, (char *)c->phi[level] + pair * 2 * nn * c->es, 2 * nn);
+}
+
 int msg_set_deformation_zero(msg_ctx *c, int pair) {
   CHK(check_pair(c, pair));
   for (int l = 0; l <= c->L; ++l) {
@@ -1213,26 +1247,28 @@ template <typename S>
 // if st_out is given, the pair states the NCC variation reads (reference and template statistics,
 // E_NCC at phi).
 int energy_impl(const void *R, const void *T, const void *phi, int n, int batch, int data_term, double *e_data,
-                       double *e_reg, hipStream_t s) {
-  const EnergyGeom g = energy_geom(n);
+                       double *e_reg, hipStream_t s, double yreg = 0) {
+  const int nb = energy_blocks(n, data_term);
   void *stp = nullptr, *part = nullptr;
   CHK(scratch_get(0, sizeof(PairState) * batch, &stp));
   CHK(scratch_get(1, sizeof(double) * energy_part_size(n, batch), &part));
   hipLaunchKernelGGL(k_state_init, dim3(cdiv(batch, 64)), dim3(64), 0, s, (PairState *)stp, batch, 0.0);
   HIPCHK(hipGetLastError());
   CHK(launch_energy<S>((const S *)R, (const S *)T, (const S *)phi, (const S *)phi, (const S *)phi, n,
-                       (const PairState *)stp, (double *)part, batch, s, data_term));
+                       (const PairState *)stp, (double *)part, batch, s, data_term, 0, 0, true, yreg));
   std::vector<double> hp(energy_part_size(n, batch));
   HIPCHK(hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   auto sum = [&](size_t off) {
     double a = 0;
-    for (int i = 0; i < g.nblk; ++i) a += hp[off + i];
+    for (int i = 0; i < nb; ++i) a += hp[off + i];
     return a;
   };
   for (int p = 0; p < batch; ++p) {
-    if (e_data) e_data[p] = sum(msg::part_index(msg::DT_SSD, p, 0, 0, g.nblk));
-    if (e_reg) e_reg[p] = sum(msg::part_index(msg::DT_SSD, p, 0, 1, g.nblk));
+    // x-shift: the rows' NCC energies averaged over the rows (FELineShiftRegisEnergy::apply)
+    const double dscale = data_term == msg::DT_XSHIFT ? 1.0 / n : 1.0;
+    if (e_data) e_data[p] = sum(msg::part_index(msg::DT_SSD, p, 0, 0, nb)) * dscale;
+    if (e_reg) e_reg[p] = sum(msg::part_index(msg::DT_SSD, p, 0, 1, nb));
   }
   return MSG_OK;
 }
@@ -1331,6 +1367,33 @@ int msg_gradient(int dtype, const void *R, const void *T, const void *phi, void 
   else
     CHK(launch_grad<float>((const float *)R, (const float *)T, (const float *)phi, (float *)DE, n, nullptr, lambda,
                            batch, strm(stream), msg::DT_SSD));
+  HIPCHK(hipStreamSynchronize(strm(stream)));
+  return MSG_OK;
+}
+
+int msg_xshift_energy(int dtype, const void *R, const void *T, const void *phi, int n, int batch, double y_reg_factor,
+                      double *e_ncc, double *e_reg, void *stream) {
+  CHK(check_dtype(dtype));
+  CHK(check_n(n));
+  if (n > (1 << 11)) return fail(MSG_ERR_ARG, "x-shift rows are staged in LDS: n <= 2048");
+  if (!R || !T || !phi || batch < 1) return fail(MSG_ERR_ARG, "invalid argument");
+  return dtype == MSG_F64
+             ? energy_impl<double>(R, T, phi, n, batch, msg::DT_XSHIFT, e_ncc, e_reg, strm(stream), y_reg_factor)
+             : energy_impl<float>(R, T, phi, n, batch, msg::DT_XSHIFT, e_ncc, e_reg, strm(stream), y_reg_factor);
+}
+
+int msg_xshift_gradient(int dtype, const void *R, const void *T, const void *phi, void *DE, int n, int batch,
+                        double lambda, double y_reg_factor, void *stream) {
+  CHK(check_dtype(dtype));
+  CHK(check_n(n));
+  if (n > (1 << 11)) return fail(MSG_ERR_ARG, "x-shift rows are staged in LDS: n <= 2048");
+  if (!R || !T || !phi || !DE || batch < 1) return fail(MSG_ERR_ARG, "invalid argument");
+  if (dtype == MSG_F64)
+    CHK(launch_grad<double>((const double *)R, (const double *)T, (const double *)phi, (double *)DE, n, nullptr,
+                            lambda, batch, strm(stream), msg::DT_XSHIFT, 0, y_reg_factor));
+  else
+    CHK(launch_grad<float>((const float *)R, (const float *)T, (const float *)phi, (float *)DE, n, nullptr, lambda,
+                           batch, strm(stream), msg::DT_XSHIFT, 0, y_reg_factor));
   HIPCHK(hipStreamSynchronize(strm(stream)));
   return MSG_OK;
 }
@@ -1697,6 +1760,15 @@ struct CtxCache {
 };
 CtxCache g_ctx_cache;
 
+// Iterations, reference-equivalent energy evaluations and energy passes of every solve run on a
+// cached context since the last reset (the work of the last msg_match_series call).
+struct SeriesCounts {
+  std::mutex mu;
+  long long v[3] = {0, 0, 0};
+  double w[2] = {0, 0};
+};
+SeriesCounts g_series_counts;
+
 int ctx_acquire(int L, int B, int dtype, msg_ctx **out) {
   {
     std::lock_guard<std::mutex> g(g_ctx_cache.mu);
@@ -1706,6 +1778,8 @@ int ctx_acquire(int L, int B, int dtype, msg_ctx **out) {
       msg_ctx *c = g_ctx_cache.free[i];
       if (c->L == L && c->B == B && c->dtype == dtype && c->device == dev) {
         g_ctx_cache.free.erase(g_ctx_cache.free.begin() + i);
+        c->cnt_iter = c->cnt_evals = c->cnt_passes = 0;
+        c->cnt_node_iter = c->cnt_node_evals = 0;
         *out = c;
         return MSG_OK;
       }
@@ -1716,6 +1790,14 @@ int ctx_acquire(int L, int B, int dtype, msg_ctx **out) {
 
 void ctx_release(msg_ctx *c) {
   if (!c) return;
+  {
+    std::lock_guard<std::mutex> g(g_series_counts.mu);
+    g_series_counts.v[0] += c->cnt_iter;
+    g_series_counts.v[1] += c->cnt_evals;
+    g_series_counts.v[2] += c->cnt_passes;
+    g_series_counts.w[0] += c->cnt_node_iter;
+    g_series_counts.w[1] += c->cnt_node_evals;
+  }
   std::lock_guard<std::mutex> g(g_ctx_cache.mu);
   g_ctx_cache.free.push_back(c);
 }
@@ -1988,6 +2070,7 @@ int pairwise_batch(const msg_series_params *p, const msg_solve_params &sp, const
   CHK(ctx_acquire(L, nb, dt, &pc));
   CtxGuard gpc{pc};
   CHK(msg_ctx_set_data_term(pc, p->data_term));
+  if (p->data_term == MSG_XSHIFT) CHK(msg_ctx_set_y_reg_factor(pc, p->y_reg_factor));
   hipStream_t s = pc->stream;
   for (int i = i0; i < i1; ++i) {
     const S *r = (i == 0 || p->dont_accumulate) ? ref : tpl + (size_t)(i - 1) * nn;
@@ -2136,10 +2219,16 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
   const bool overlap = chain && b < npw;
 
   hipStream_t s = nullptr;  // the caller thread's stream for the reconstruction
+  {
+    std::lock_guard<std::mutex> g(g_series_counts.mu);
+    g_series_counts.v[0] = g_series_counts.v[1] = g_series_counts.v[2] = 0;
+    g_series_counts.w[0] = g_series_counts.w[1] = 0;
+  }
   msg_ctx *rc = nullptr;
   if (chain) CHK(ctx_acquire(L, 1, dt, &rc));
   CtxGuard grc{rc};
   if (chain) CHK(msg_ctx_set_data_term(rc, p->data_term));
+  if (chain && p->data_term == MSG_XSHIFT) CHK(msg_ctx_set_y_reg_factor(rc, p->y_reg_factor));
 
   DevBuf pw;
   HIPCHK(hipMalloc(&pw.p, sizeof(S) * 2 * nn * n));
@@ -2223,13 +2312,22 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
       DevBuf acc;
       HIPCHK(hipMalloc(&acc.p, sizeof(S) * 2 * nn));
       HIPCHK(hipMemsetAsync(acc.p, 0, sizeof(S) * 2 * nn, rs));
-      DevBuf prev, swp;
+      DevBuf prev, swp, zero;
       HIPCHK(hipMalloc(&prev.p, sizeof(S) * 2 * nn));
+      if (p->keep_start_transformation) {
+        HIPCHK(hipMalloc(&zero.p, sizeof(S) * 2 * nn));
+        HIPCHK(hipMemsetAsync(zero.p, 0, sizeof(S) * 2 * nn, rs));
+      }
       if (p->calc_inverse) HIPCHK(hipMalloc(&swp.p, sizeof(S) * nn));
       for (int i = 0; i < n; ++i) {
         if (i < npw && !wait_pairwise(i)) return pr.rc;
-        HIPCHK(hipMemcpyAsync(prev.p, acc.p, sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, rs));
-        CHK(compose_impl<S>(pwp + (size_t)i * 2 * nn, prev.p, acc.p, N, rs));
+        {
+          // keep_start_transformation: a pairing solved here hands back the zero it started from, a
+          // loaded one (reuseStage1Results) the loaded deformation
+          const S *cur = (p->keep_start_transformation && i < npw) ? (const S *)zero.p : pwp + (size_t)i * 2 * nn;
+          HIPCHK(hipMemcpyAsync(prev.p, acc.p, sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, rs));
+          CHK(compose_impl<S>(cur, prev.p, acc.p, N, rs));
+        }
         if (i > 0) {
           HIPCHK(hipMemcpyAsync(rc->T[L], tpl + (size_t)i * nn, nn * sizeof(S), hipMemcpyDeviceToDevice, rs));
           if (!p->dont_normalize)
@@ -2248,7 +2346,10 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
                       (size_t)(i - 1) * level_block(p->refine_start_level, p->refine_stop_level);
           CHK(solve_impl<S>(rc, p->refine_start_level, p->refine_stop_level, &sp, &e, &lo));
           energies[i] = e;
-          HIPCHK(hipMemcpyAsync(acc.p, rc->phi[L], sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, rs));
+          // keep_start_transformation: the refinement's start stays the accumulated deformation; its
+          // result only goes to the frame's average
+          HIPCHK(hipMemcpyAsync(p->keep_start_transformation ? (void *)(defs + (size_t)i * 2 * nn) : acc.p,
+                                rc->phi[L], sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, rs));
         }
         if (p->calc_inverse) {
           // calcInverseDeformation (matchSeries.h:247-259): reference and template swapped, the
@@ -2288,8 +2389,9 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
                                   hipMemcpyDeviceToDevice, rs));
           CHK(swap_images());
         }
-        HIPCHK(hipMemcpyAsync(defs + (size_t)i * 2 * nn, (i > 0) ? acc.p : (const void *)pwp, sizeof(S) * 2 * nn,
-                              hipMemcpyDeviceToDevice, rs));
+        if (!(p->keep_start_transformation && i > 0))  // frame i > 0 was stored by its refinement
+          HIPCHK(hipMemcpyAsync(defs + (size_t)i * 2 * nn, (i > 0) ? acc.p : (const void *)pwp, sizeof(S) * 2 * nn,
+                                hipMemcpyDeviceToDevice, rs));
       }
       HIPCHK(hipStreamSynchronize(rs));
       return MSG_OK;
@@ -2439,6 +2541,17 @@ int msg_normalize(int dtype, void *img, int n, int nimg, double pct, int min_to_
   return MSG_OK;
 }
 
+int msg_series_last_counts(long long *iterations, long long *evals, long long *passes, double *node_iterations,
+                           double *node_evals) {
+  std::lock_guard<std::mutex> g(g_series_counts.mu);
+  if (iterations) *iterations = g_series_counts.v[0];
+  if (evals) *evals = g_series_counts.v[1];
+  if (passes) *passes = g_series_counts.v[2];
+  if (node_iterations) *node_iterations = g_series_counts.w[0];
+  if (node_evals) *node_evals = g_series_counts.w[1];
+  return MSG_OK;
+}
+
 int msg_match_series(const msg_series_params *p, int dtype, const void *templates, int n, const void *reference,
                      const void *pairwise_in, void *deformations, void *pairwise_out, double *energies, void *mean,
                      void *count, void *median, const msg_series_levels *levels, void *reduced) {
@@ -2526,12 +2639,26 @@ int msg_ctx_set_mode(msg_ctx *c, int mode) {
 
 int msg_ctx_set_data_term(msg_ctx *c, int data_term) {
   if (!c) return fail(MSG_ERR_ARG, "null context");
-  if (data_term != MSG_SSD && data_term != MSG_NCC) return fail(MSG_ERR_ARG, "data term must be MSG_SSD or MSG_NCC");
+  if (data_term != MSG_SSD && data_term != MSG_NCC && data_term != MSG_XSHIFT)
+    return fail(MSG_ERR_ARG, "data term must be MSG_SSD, MSG_NCC or MSG_XSHIFT");
+  if (data_term == MSG_XSHIFT && (1 << c->L) > 2048)
+    return fail(MSG_ERR_ARG, "MSG_XSHIFT: rows of at most 2048 nodes (level 11)");
   if (c->data_term != data_term) {
     for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);  // the captured kernels depend on it
     c->graphs.clear();
   }
   c->data_term = data_term;
+  return MSG_OK;
+}
+
+int msg_ctx_set_y_reg_factor(msg_ctx *c, double y) {
+  if (!c) return fail(MSG_ERR_ARG, "null context");
+  if (!(y >= 0) || !std::isfinite(y)) return fail(MSG_ERR_ARG, "yRegFactor must be finite and >= 0");
+  if (c->yreg != y) {
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);  // the captured kernels carry it
+    c->graphs.clear();
+  }
+  c->yreg = y;
   return MSG_OK;
 }
 

@@ -428,6 +428,24 @@ __device__ void control_pair(int pair, PairState &s, const double *__restrict__ 
         s.cand_V[k] = vt;
       }
     }
+  } else if ((a.phase == PH_LEVEL_INIT || a.phase == PH_ARMIJO_STEP) && a.data_term == DT_XSHIFT) {
+    // rows' NCC energies summed, then divided by the number of rows (FELineShiftRegisEnergy::apply)
+    const int nc = s.ncand;
+    reduce_arrays(part_e + part_index(DT_SSD, pair, 0, 0, a.nblk_energy), 2 * nc, a.nblk_energy, sums);
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < nc; ++k) {
+        double D = sums[2 * k];
+        D /= a.nblk_energy;
+        double E = 0;
+        E += D;
+        E += s.lambda * sums[2 * k + 1];
+        s.cand_E[k] = E;
+        s.cand_R[k] = sums[2 * k + 1];
+        s.cand_D[k] = D;
+        s.cand_M[k] = s.cand_V[k] = 0;
+      }
+      s.cand_ex = 0;
+    }
   } else if (a.phase == PH_LEVEL_INIT || a.phase == PH_ARMIJO_STEP) {
     const int nc = s.ncand;
     reduce_arrays(part_e + part_index(DT_SSD, pair, 0, 0, a.nblk_energy), 2 * nc, a.nblk_energy, sums);

@@ -7,8 +7,9 @@ namespace msg {
 enum ArmijoPhase { AP_IDLE = 0, AP_FIRST = 1, AP_GROW = 2, AP_SHRINK = 3 };
 enum DirMode { DIR_CG = 0, DIR_GRAD = 1, DIR_NONE = 2 };
 enum { MAX_CAND = 4 };
-// deformationModel 22 / deformationModel 0 / reduceDeformations' consistency energy
-enum DataTerm { DT_SSD = 0, DT_NCC = 1, DT_CONS = 2 };
+// deformationModel 22 / deformationModel 0 / reduceDeformations' consistency energy / the seismic
+// row-wise x-shift NCC (matchSeismicSeries)
+enum DataTerm { DT_SSD = 0, DT_NCC = 1, DT_CONS = 2, DT_XSHIFT = 3 };
 
 struct PairState {
   // gradient-flow state (GradientDescentBase::applySingle)

@@ -34,6 +34,14 @@ using namespace mshost;
 
 namespace {
 
+// matchSeismicSeries (built from this source with MS_SEISMIC): the same series loop over the x-shift
+// registration of XShiftNonDyadicRegistrationMultilevelDescent (matchSeismicSeries.cpp:10-86).
+#ifdef MS_SEISMIC
+constexpr bool kSeismic = true;
+#else
+constexpr bool kSeismic = false;
+#endif
+
 // Duplicates everything written to a stream into log.txt (aol::AdditionalOutputToFile,
 // multiStreambuf.h:49-63).
 class TeeBuf : public std::streambuf {
@@ -146,6 +154,20 @@ msg_series_params seriesParams(const ParameterParser &p) {
   s.alt_start_level = useAltStartLevel(p) ? p.getInt("altStartLevel") : 0;
   s.energy_as_alt_criterion = p.checkAndGetBool("energyAsAltCriterion");
   s.first_try_out = nullptr;
+  if (kSeismic) {
+    // XShiftRegistrationMultilevelDescent::descentOnCurrentGrid (matchSeries.h:953-972): findXShift with
+    // "lambda" on every level, "yRegFactor", maxGDIterations (default 500), stopEpsilon 0; the frames
+    // scaled to [0, 1] (NonDyadicRegistrationMultilevelDescent::loadAndPrepareImage, matchSeries.h:1040-1053)
+    s.data_term = MSG_XSHIFT;
+    s.y_reg_factor = p.getDouble("yRegFactor");
+    s.lambda_factor = -1.0;
+    s.max_iter = p.getIntOrDefault("maxGDIterations", 500);
+    s.stop_eps = 0.0;
+    s.saturation_pct = -1.0;
+    s.normalize_min_to_zero = 1;
+    s.pre_smooth_sigma = 0;
+    s.keep_start_transformation = 1;
+  }
   return s;
 }
 
@@ -533,7 +555,7 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
       std::ofstream e(sub + strprintf("energy_%02d.txt", l));
       auto line = [&](int k, double regw, double E) {
         e << k << " ";
-        if (sp.data_term == MSG_NCC) e << E - regw << " ";
+        if (sp.data_term == MSG_NCC || sp.data_term == MSG_XSHIFT) e << E - regw << " ";
         e << regw << " " << E << "\n";
       };
       line(-1, lvStart[2 * r + 1], lvStart[2 * r]);
@@ -565,7 +587,9 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
           for (size_t k = 0; k < nn; ++k) s += phs[c * nn + k] * phs[c * nn + k];
           norm += std::sqrt(s);
         }
-        defNorms << i << " " << norm << " # " << files[i] << "\n";
+        // matchSeismicSeries: the registration object's own deformation is never set on its dyadic
+        // path, so its norm is 0 (NonDyadicRegistrationMultilevelDescent::getTransformationNorm)
+        defNorms << i << " " << (kSeismic ? 0.0 : norm) << " # " << files[i] << "\n";
       }
     }
     if (accumulate && i > 0) {
@@ -575,7 +599,8 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
       writeLevels(sub, sp.refine_start_level, refineStop);
       if (refineStop < L) saveField(sub + strprintf("deformation_%02d_%%d.dat.bz2", L), phd, N);
     }
-    energiesFile << i << " " << energies[i] << "\n";
+    // matchSeismicSeries: getEnergyOfLastSolution is set by the full-resolution step only (NaN here)
+    energiesFile << i << " " << (kSeismic ? (double)NAN : energies[i]) << "\n";
   }
   for (int i = 0; inverse && i < n; ++i) {  // then the inverse solves' records
     const std::string sub = strprintf("%s%d-r-inv/", dir.c_str(), i);
@@ -884,15 +909,117 @@ void runMatchSeries(ParameterParser &parser, int argc, char **argv) {
 
 }  // namespace
 
+// matchSeismicSeries' front end (matchSeismicSeries.cpp:24-82): the save directory optionally named
+// after the parameters, the initial average made by interleaving the reference's rows with the
+// templates' (without alignment: "justStackIniAverage" or "justCopyLastRefLineForIniAverage"), and
+// the dyadic path only: the full-resolution step of NonDyadicRegistrationMultilevelDescent and the
+// aligned interleaving need a Cholesky factorisation of the H1 metric, which is not part of this build.
+void seismicFrontEnd(ParameterParser &parser) {
+  if (parser.checkAndGetBool("addParamsToSafeDirName")) {
+    std::string d = parser.getStringExpandTilde("saveDirectory");
+    if (!d.empty() && d.back() == '/') d.pop_back();
+    std::ostringstream o;
+    o << d;
+    if (parser.checkAndGetBool("saveNamedDeformedTemplatesUsingNearestNeighborInterpolation")) o << "-nointerp";
+    o << "-" << parser.getDouble("lambda") << "-" << parser.getDouble("yRegFactor") << "-"
+      << parser.getDouble("laplaceWeight") << "-" << parser.getDouble("averageXDerivNormThreshold") << "-"
+      << parser.getDouble("extraStagesLambdaFactor");
+    if (parser.checkAndGetBool("forceNonDyadicXShiftYConst")) o << "-yconst";
+    o << "/";
+    parser.changeVariableValue("saveDirectory", o.str());
+  }
+  if (parser.checkAndGetBool("forceNonDyadicXShiftYConst"))
+    MSH_THROW("\"forceNonDyadicXShiftYConst 1\" selects the full-resolution step, which this build does not run");
+  if (parser.getDouble("laplaceWeight") > 0) MSH_THROW("\"laplaceWeight\" > 0 is not supported by this build");
+  (void)parser.getInt("keepEmptyTopBottomBlocks");  // read by the registration object's constructor
+  const int L = parser.getInt("precisionLevel");
+  if (!parser.hasVariable("dyadicDepth") || parser.getInt("dyadicDepth") != L || parser.getInt("stopLevel") != L)
+    MSH_THROW("matchSeismicSeries: this build runs the dyadic path only: \"dyadicDepth\", \"stopLevel\" and "
+              "\"precisionLevel\" equal and the frames of 2^precisionLevel x 2^precisionLevel (the full-resolution "
+              "step needs a Cholesky factorisation of the H1 metric)");
+  const std::vector<std::string> files = createTemplateFileNameList(parser);
+  for (const std::string &f : files) {
+    const Image im = loadImage(f);
+    if (im.nx != (1 << L) || im.ny != (1 << L))
+      MSH_THROW(strprintf("matchSeismicSeries: %s is %dx%d; this build runs the dyadic path only (frames of "
+                          "2^precisionLevel x 2^precisionLevel)", f.c_str(), im.nx, im.ny));
+    for (int y = 0; y < im.ny; ++y) {
+      double s2 = 0;
+      for (int x = 0; x < im.nx; ++x) s2 += im.data[(size_t)y * im.nx + x] * im.data[(size_t)y * im.nx + x];
+      if (s2 == 0) MSH_THROW(strprintf("matchSeismicSeries: %s has an empty row (%d); rows are dropped from the "
+                                       "dyadic grid only by the full-resolution path", f.c_str(), y));
+    }
+  }
+  if (!parser.hasVariable("initialAverage")) {
+    // interleaveImages (matchSeismicSeries.h:9-96) without alignment
+    const bool align = parser.getInt("justStackIniAverage") == 0;
+    const bool copyLast = parser.hasVariable("justCopyLastRefLineForIniAverage") &&
+                          parser.checkAndGetBool("justCopyLastRefLineForIniAverage");
+    if (align && !copyLast)
+      MSH_THROW("matchSeismicSeries: the aligned interleaving needs a Cholesky factorisation of the 1-D H1 metric; "
+                "set \"justStackIniAverage 1\" or give \"initialAverage\"");
+    const int numTemplates = parser.getInt("refLineDistance") - 1;
+    if (numTemplates >= (int)files.size())
+      MSH_THROW("refLineDistance too big for the amount of supplied template images");
+    const bool clamp = parser.hasVariable("clampRange");
+    auto load = [&](const std::string &f) {
+      Image im = loadImage(f);
+      if (clamp)
+        for (double &v : im.data) v = std::min(std::max(v, parser.getDouble("clampRange", 0)), parser.getDouble("clampRange", 1));
+      return im;
+    };
+    const Image ref = load(files[0]);
+    std::vector<Image> tpls;
+    for (int i = 0; i < numTemplates; ++i) tpls.push_back(load(files[i + 1]));
+    std::vector<double> out(ref.data);
+    const int nx = ref.nx, ny = ref.ny, dist = numTemplates + 1;
+    auto rowEmpty = [&](const Image &im, int r) {
+      double s2 = 0;
+      for (int x = 0; x < nx; ++x) s2 += im.data[(size_t)r * nx + x] * im.data[(size_t)r * nx + x];
+      return s2 == 0;
+    };
+    int lastNonEmpty = -1;
+    for (int line = 0; line < ny; ++line) {
+      if (rowEmpty(ref, line)) {
+        if (line == 0 && numTemplates > 0)
+          std::copy(tpls[0].data.begin(), tpls[0].data.begin() + nx, out.begin());
+        else if (copyLast && lastNonEmpty >= 0)
+          std::copy(ref.data.begin() + (size_t)lastNonEmpty * nx, ref.data.begin() + (size_t)(lastNonEmpty + 1) * nx,
+                    out.begin() + (size_t)line * nx);
+        continue;
+      }
+      lastNonEmpty = line;
+      if (copyLast || line + dist >= ny) continue;
+      if (rowEmpty(ref, line + dist))
+        MSH_THROW(strprintf("Unexpected empty line in reference (current line index is %d)", line));
+      for (int i = 0; i < numTemplates; ++i) {
+        if (rowEmpty(tpls[i], line + i + 1))
+          MSH_THROW(strprintf("Unexpected empty line in template (current line index is %d)", line));
+        std::copy(tpls[i].data.begin() + (size_t)(line + i + 1) * nx, tpls[i].data.begin() + (size_t)(line + i + 2) * nx,
+                  out.begin() + (size_t)(line + 1 + i) * nx);
+      }
+    }
+    std::string d = parser.getStringExpandTilde("saveDirectory");
+    makeDirectory(d);
+    const std::string name = d + "initialAverage.q2bz";
+    saveP9(name, out.data(), nx, ny);
+    parser.addVariable("initialAverage", name);
+  }
+}
+
 int main(int argc, char **argv) {
   const char *strictEnv = getenv("MSG_STRICT");
   const bool strict = strictEnv && atoi(strictEnv) == 1;
   try {
     ParameterParser parser(argc > 1 ? argv[1] : "matchSeries.par");
-    const int model = parser.getIntOrDefault("deformationModel", 0);
-    if (model != 22 && model != 0)
-      MSH_THROW(strprintf("deformationModel %d is not supported by this build (0: NCC, 22: SSD; both with the "
-                          "Dirichlet regulariser)", model));
+    if (kSeismic) {
+      seismicFrontEnd(parser);
+    } else {
+      const int model = parser.getIntOrDefault("deformationModel", 0);
+      if (model != 22 && model != 0)
+        MSH_THROW(strprintf("deformationModel %d is not supported by this build (0: NCC, 22: SSD; both with the "
+                            "Dirichlet regulariser)", model));
+    }
     runMatchSeries(parser, argc, argv);
   } catch (Exception &e) {
     e.dump(std::cerr);

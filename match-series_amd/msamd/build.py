@@ -50,6 +50,12 @@ def build_cli():
          "-L", libdir, "-lmsgpu", "-Wl,-rpath,$ORIGIN/../lib", "-L/opt/rocm/lib", "-lamdhip64",
          "-Wl,-rpath,/opt/rocm/lib", bz2, zlib])
     outs.append(out)
+    # matchSeismicSeries: the same driver with the row-wise x-shift registration (dyadic path)
+    out = os.path.join(PKG, "bin", "matchSeismicSeries")
+    run(["g++", "-std=c++17", "-O2", "-Wall", "-DMS_SEISMIC"] + inc + common +
+        [os.path.join(host, "matchSeries.cpp"), "-o", out, "-L", libdir, "-lmsgpu", "-Wl,-rpath,$ORIGIN/../lib",
+         "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib", bz2, zlib])
+    outs.append(out)
     out = os.path.join(PKG, "bin", "msgio")
     run(["g++", "-std=c++17", "-O2", "-Wall"] + inc + common + [os.path.join(host, "msgio.cpp"), "-o", out, bz2, zlib])
     outs.append(out)
@@ -60,6 +66,8 @@ def build_oracle(with_reference=True):
     run(["make", "-C", os.path.join(ROOT, "oracle"), "-B", "lib"])
     if with_reference and os.path.isdir("/root/reference/quocmesh"):
         run(["make", "-C", os.path.join(ROOT, "oracle"), "-j8", "ref"])
+        # the reference's driver over the GPU registration object (build check of the adapter)
+        run(["make", "-C", os.path.join(ROOT, "oracle"), "-B", "adapter"])
 
 
 def build_all(with_reference=True):

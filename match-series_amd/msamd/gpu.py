@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libmsgpu.so")
 
 F32, F64 = 0, 1
-SSD, NCC = 0, 1  # data terms: deformationModel 22 / deformationModel 0
+SSD, NCC, XSHIFT = 0, 1, 3  # data terms: deformationModel 22 / deformationModel 0 / seismic row x-shift
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -46,7 +46,7 @@ class SeriesParams(ctypes.Structure):
         ("pair0_init", ctypes.c_void_p), ("corr_init", ctypes.c_int), ("max_corr_shift", ctypes.c_int),
         ("calc_inverse", ctypes.c_int), ("inverse_out", ctypes.c_void_p),
         ("alt_start_level", ctypes.c_int), ("energy_as_alt_criterion", ctypes.c_int), ("first_try_out", ctypes.c_void_p),
-        ("raw_templates", ctypes.c_void_p),
+        ("raw_templates", ctypes.c_void_p), ("y_reg_factor", ctypes.c_double), ("keep_start_transformation", ctypes.c_int),
     ]
 
 
@@ -67,6 +67,8 @@ SIGNATURES = {
     "msg_set_deformation": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
     "msg_get_deformation": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
     "msg_set_deformation_zero": ([_vp, ctypes.c_int], ctypes.c_int),
+    "msg_set_deformation_level": ([_vp, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
+    "msg_get_deformation_level": ([_vp, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
     "msg_set_deformations_device": ([_vp, _vp], ctypes.c_int),
     "msg_get_deformations_device": ([_vp, _vp], ctypes.c_int),
     "msg_solve_level": ([_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _ip,
@@ -91,9 +93,15 @@ SIGNATURES = {
     "msg_release_cache": ([], ctypes.c_int),
     "msg_ctx_set_mode": ([_vp, ctypes.c_int], ctypes.c_int),
     "msg_ctx_set_data_term": ([_vp, ctypes.c_int], ctypes.c_int),
+    "msg_ctx_set_y_reg_factor": ([_vp, ctypes.c_double], ctypes.c_int),
     "msg_ncc_energy": ([ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _vp], ctypes.c_int),
     "msg_ncc_gradient": ([ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp],
                          ctypes.c_int),
+    "msg_xshift_energy": ([ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, _vp],
+                          ctypes.c_int),
+    "msg_xshift_gradient": ([ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                            ctypes.c_double, _vp], ctypes.c_int),
+    "msg_series_last_counts": ([ctypes.POINTER(ctypes.c_longlong)] * 3 + [_dp, _dp], ctypes.c_int),
     "msg_set_default_mode": ([ctypes.c_int], ctypes.c_int),
     "msg_set_default_exact_decisions": ([ctypes.c_int], ctypes.c_int),
     "msg_profile_enable": ([ctypes.c_uint], ctypes.c_int),
@@ -271,6 +279,25 @@ def ncc_gradient(R, T, phi, lam):
     return out
 
 
+def xshift_energy(R, T, phi, y_reg_factor):
+    """Per-pair (row-averaged 1-D NCC, 1/2 phi_x (K_xx + y K_yy) phi_x) of the x-shift registration;
+    phi: (B, 2, n, n), its x component read."""
+    code, B, n = _pair_args(R, T, phi)
+    es = np.zeros(B)
+    er = np.zeros(B)
+    check(lib().msg_xshift_energy(code, _ptr(R), _ptr(T), _ptr(phi), n, B, float(y_reg_factor), _dptr(es), _dptr(er),
+                                  _stream()))
+    return es, er
+
+
+def xshift_gradient(R, T, phi, lam, y_reg_factor):
+    code, B, n = _pair_args(R, T, phi)
+    out = _torch().empty_like(phi)
+    check(lib().msg_xshift_gradient(code, _ptr(R), _ptr(T), _ptr(phi), _ptr(out), n, B, float(lam),
+                                    float(y_reg_factor), _stream()))
+    return out
+
+
 def gradient(R, T, phi, lam):
     code, B, n = _pair_args(R, T, phi)
     out = _torch().empty_like(phi)
@@ -413,6 +440,9 @@ class Context:
 
     def set_mode(self, mode):
         check(lib().msg_ctx_set_mode(self.h, mode))
+
+    def set_y_reg_factor(self, y):
+        check(lib().msg_ctx_set_y_reg_factor(self.h, y))
 
     def close(self):
         if self.h:
@@ -613,6 +643,17 @@ def match_series(templates, reference, params, pairwise_in=None, want_pairwise=F
             off += 2 * m * m
         out["levels"] = levels
     return out
+
+
+def series_last_counts():
+    """Counts of the last match_series call: dict of iterations, evals, passes (totals) and
+    node_iterations, node_evals (weighted by each level's node count)."""
+    v = [ctypes.c_longlong() for _ in range(3)]
+    w = np.zeros(2)
+    check(lib().msg_series_last_counts(*[ctypes.byref(x) for x in v], w[0:1].ctypes.data_as(_dp),
+                                       w[1:2].ctypes.data_as(_dp)))
+    return dict(iterations=v[0].value, evals=v[1].value, passes=v[2].value, node_iterations=float(w[0]),
+                node_evals=float(w[1]))
 
 
 # ------------------------------------------------------------------------------------------------

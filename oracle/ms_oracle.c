@@ -715,6 +715,220 @@ void msor_consistency_force(const double *psi, const double *phi, int N, double 
 }
 
 /* ---------------------------------------------------------------------------------------------
+ * Seismic x-shift registration (matchSeismicSeries): a displacement along x only, phi_x, whose data
+ * term is the mean over the image rows ("lines") of the 1-D normalised cross-correlation of the
+ * template row shifted by the row of phi_x against the reference row (FELineShiftRegisEnergy,
+ * registration.h:2137-2193, with NCCRegistrationConfigurator on the 1-D grid of the row), and whose
+ * regulariser is lambda/2 phi_x^T (K_xx + yRegFactor K_yy) phi_x (findXShift, matchSeries.h:805-868).
+ * The 1-D configurator integrates with 2 Gauss points of weight 1/2 (baseFunctionSet.h:1535-1554).
+ * ------------------------------------------------------------------------------------------- */
+
+/* aol::Vector::interpolate (vec.h:461-471) */
+static double lineInterp(const double *v, int n, double pos) {
+  const int p = (int)pos;
+  const double pf = pos - p;
+  if (p == n - 1) return v[p];
+  return (1 - pf) * v[p] + pf * v[p + 1];
+}
+
+/* The assembled 1-D mass matrix (MassOp ASSEMBLED): element local entries by the 2-point rule, added
+ * in element order; diag[i] = M_ii, off[i] = M_i,i+1 = M_i+1,i. */
+static void mass1D(int n, double *diag, double *off) {
+  const double h = gridH(n);
+  double L[2][2] = {{0, 0}, {0, 0}};
+  for (int q = 0; q < 2; ++q) {
+    const double b[2] = {1.0 - QP[q], QP[q]};
+    for (int a = 0; a < 2; ++a)
+      for (int c = 0; c < 2; ++c) L[a][c] += 0.5 * b[a] * b[c];
+  }
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 2; ++c) L[a][c] *= h;
+  for (int i = 0; i < n; ++i) diag[i] = 0;
+  for (int e = 0; e < n - 1; ++e) {
+    diag[e] += L[0][0];
+    off[e] = L[0][1];
+    diag[e + 1] += L[1][1];
+  }
+}
+
+/* y = M x, each row summed in ascending column order */
+static void massApply1D(const double *x, int n, const double *diag, const double *off, double *y) {
+  for (int i = 0; i < n; ++i) {
+    double s = 0;
+    if (i > 0) s += off[i - 1] * x[i - 1];
+    s += diag[i] * x[i];
+    if (i < n - 1) s += off[i] * x[i + 1];
+    y[i] = s;
+  }
+}
+
+/* normalizeImageForNCC (registration.h:1934-1947) on one row; returns sqrt(u.Mu) */
+static double nccNormalize1D(const double *img, int n, const double *diag, const double *off, double *out,
+                             double *tmp) {
+  massApply1D(img, n, diag, off, tmp);
+  double mean = 0;
+  for (int i = 0; i < n; ++i) mean += tmp[i];
+  for (int i = 0; i < n; ++i) out[i] = img[i];
+  for (int i = 0; i < n; ++i) out[i] += -mean;
+  massApply1D(out, n, diag, off, tmp);
+  double dot = 0;
+  for (int i = 0; i < n; ++i) dot += out[i] * tmp[i];
+  const double var = sqrt(dot);
+  if (!(fabs(var - 0.) < 1e-15))
+    for (int i = 0; i < n; ++i) out[i] /= var;
+  return var;
+}
+
+/* One row: FastDeformImage<1-D> (deformations.h:111-137: t at i + y_i/h clamped to [0, n-1]),
+ * normalisation, E = -(M tn) . rn. Writes tn and the deformed row's deviation. */
+static double lineNcc(const double *rn, const double *t, const double *y, int n, const double *diag,
+                      const double *off, double *tn, double *var_t, double *tmp, double *def) {
+  const double h = gridH(n);
+  for (int i = 0; i < n; ++i) {
+    double ds = i + y[i] / h;
+    ds = ds < 0 ? 0 : (ds > (double)(n - 1) ? (double)(n - 1) : ds);
+    def[i] = lineInterp(t, n, ds);
+  }
+  *var_t = nccNormalize1D(def, n, diag, off, tn, tmp);
+  massApply1D(tn, n, diag, off, def);
+  double dot = 0;
+  for (int i = 0; i < n; ++i) dot += def[i] * rn[i];
+  return -dot;
+}
+
+/* The assembled K_xx + yRegFactor K_yy of findXShift (FEOpMixedDerivative(0,0) and (1,1) added into one
+ * matrix, the second scaled, matchSeries.h:820-826) applied to u: element local matrices by the 2x2
+ * rule, added per element in element order, each row summed over its 3x3 neighbours. */
+static void xshiftStiffApply(const double *u, int N, double yreg, double *out) {
+  initBQ();
+  double Kx[4][4] = {{0}}, Ky[4][4] = {{0}};
+  for (int q = 0; q < 4; ++q) {
+    const double qx = QP[q / 2], qy = QP[q % 2];
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) {
+        Kx[a][b] += dxBasis(a, qx, qy) * dxBasis(b, qx, qy) * QW;
+        Ky[a][b] += dyBasis(a, qx, qy) * dyBasis(b, qx, qy) * QW;
+      }
+  }
+  for (int y = 0; y < N; ++y)
+    for (int x = 0; x < N; ++x) {
+      double c[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int ey = y - 1; ey <= y; ++ey)
+        for (int ex = x - 1; ex <= x; ++ex) {
+          if (ex < 0 || ey < 0 || ex >= N - 1 || ey >= N - 1) continue;
+          const int bi = (x - ex) + 2 * (y - ey);
+          for (int bj = 0; bj < 4; ++bj) {
+            const int jx = ex + (bj & 1), jy = ey + ((bj & 2) >> 1);
+            c[jy - y + 1][jx - x + 1] += Kx[bi][bj];
+          }
+        }
+      for (int ey = y - 1; ey <= y; ++ey)
+        for (int ex = x - 1; ex <= x; ++ex) {
+          if (ex < 0 || ey < 0 || ex >= N - 1 || ey >= N - 1) continue;
+          const int bi = (x - ex) + 2 * (y - ey);
+          for (int bj = 0; bj < 4; ++bj) {
+            const int jx = ex + (bj & 1), jy = ey + ((bj & 2) >> 1);
+            c[jy - y + 1][jx - x + 1] += yreg * Ky[bi][bj];
+          }
+        }
+      double s = 0;
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int jx = x + dx, jy = y + dy;
+          if (jx < 0 || jy < 0 || jx >= N || jy >= N) continue;
+          s += c[dy + 1][dx + 1] * u[IDX(jx, jy, N)];
+        }
+      out[IDX(x, y, N)] = s;
+    }
+}
+
+/* E_data = (sum over rows whose reference and template both start with a number of E_row) / N, rows
+ * summed in order (FELineShiftRegisEnergy::apply); E_reg = 1/2 phi_x . A phi_x (QuadraticFormOp,
+ * op.h:1417-1421). */
+void msor_xshift_energy(const double *R, const double *T, const double *phix, int N, double yreg, double *e_data,
+                        double *e_reg) {
+  double *diag = (double *)malloc(sizeof(double) * N), *off = (double *)malloc(sizeof(double) * N);
+  double *rn = (double *)malloc(sizeof(double) * N), *tn = (double *)malloc(sizeof(double) * N);
+  double *tmp = (double *)malloc(sizeof(double) * N), *def = (double *)malloc(sizeof(double) * N);
+  double *au = (double *)malloc(sizeof(double) * N * N);
+  mass1D(N, diag, off);
+  double E = 0;
+  for (int line = 0; line < N; ++line) {
+    const double *r = R + (size_t)line * N, *t = T + (size_t)line * N;
+    if (isnan(r[0]) || isnan(t[0])) continue;
+    nccNormalize1D(r, N, diag, off, rn, tmp);
+    double var_t;
+    E += lineNcc(rn, t, phix + (size_t)line * N, N, diag, off, tn, &var_t, tmp, def);
+  }
+  E /= N;
+  if (e_data) *e_data = E;
+  if (e_reg) {
+    xshiftStiffApply(phix, N, yreg, au);
+    double dot = 0;
+    for (int i = 0; i < N * N; ++i) dot += au[i] * phix[i];
+    *e_reg = 0.5 * dot;
+  }
+  free(diag); free(off); free(rn); free(tn); free(tmp); free(def); free(au);
+}
+
+/* DE_data: per row the derivative of its NCC energy (VariationOfNormalizedCrossCorrelationEnergy:
+ * w = (rn + E tn) / var_t, then CrossCorrelationForce on the row: -w(x_q) T'(x_q + y(x_q)) at each
+ * quad point whose shifted position stays on the row, zero elsewhere), divided by N; plus
+ * lambda A phi_x. out: N*N values (the x component). */
+void msor_xshift_gradient(const double *R, const double *T, const double *phix, int N, double yreg, double lambda,
+                          double *out) {
+  const double h = gridH(N);
+  double *diag = (double *)malloc(sizeof(double) * N), *off = (double *)malloc(sizeof(double) * N);
+  double *rn = (double *)malloc(sizeof(double) * N), *tn = (double *)malloc(sizeof(double) * N);
+  double *tmp = (double *)malloc(sizeof(double) * N), *def = (double *)malloc(sizeof(double) * N);
+  double *w = (double *)malloc(sizeof(double) * N), *au = (double *)malloc(sizeof(double) * N * N);
+  mass1D(N, diag, off);
+  memset(out, 0, sizeof(double) * N * N);
+  for (int line = 0; line < N; ++line) {
+    const double *r = R + (size_t)line * N, *t = T + (size_t)line * N, *y = phix + (size_t)line * N;
+    if (isnan(r[0]) || isnan(t[0])) continue;
+    nccNormalize1D(r, N, diag, off, rn, tmp);
+    double var_t;
+    const double e = lineNcc(rn, t, y, N, diag, off, tn, &var_t, tmp, def);
+    for (int i = 0; i < N; ++i) w[i] = rn[i];
+    for (int i = 0; i < N; ++i) w[i] += e * tn[i];
+    if (!(fabs(var_t - 0.) < 1e-15))
+      for (int i = 0; i < N; ++i) w[i] /= var_t;
+    double *o = out + (size_t)line * N;
+    for (int el = 0; el < N - 1; ++el) {
+      double nl[2];
+      for (int q = 0; q < 2; ++q) {
+        const double xi = QP[q];
+        const double off_q = (1 - xi) * y[el] + xi * y[el + 1];       /* evaluateAtQuadPoint */
+        const double tc = (double)el + xi + off_q / h;                  /* transformCoord<.., false> */
+        int tel = (short)tc;
+        double tloc = tc - tel;
+        if (tel == N - 1 && tloc == 0) {
+          tel = N - 2;
+          tloc = 1;
+        }
+        if (tc < 0. || tel < 0 || tel >= N - 1) {
+          nl[q] = 0;
+          continue;
+        }
+        const double g = (t[tel + 1] - t[tel]) / h;                     /* evaluateGradient */
+        const double wq = (1 - xi) * w[el] + xi * w[el + 1];
+        nl[q] = g * -wq;
+      }
+      for (int dof = 0; dof < 2; ++dof) {
+        double a = 0;
+        for (int q = 0; q < 2; ++q) a += (dof ? QP[q] : 1 - QP[q]) * 0.5 * nl[q];
+        o[el + dof] += h * a;
+      }
+    }
+    for (int i = 0; i < N; ++i) o[i] /= N;
+  }
+  xshiftStiffApply(phix, N, yreg, au);
+  for (int i = 0; i < N * N; ++i) out[i] += lambda * au[i];
+  free(diag); free(off); free(rn); free(tn); free(tmp); free(def); free(w); free(au);
+}
+
+/* ---------------------------------------------------------------------------------------------
  * One level of the H1 gradient flow with nonlinear CG and Armijo step control
  * ------------------------------------------------------------------------------------------- */
 
@@ -729,6 +943,7 @@ typedef struct {
                       energy object does in its constructor) and a scratch image */
   const double *psi; /* consistency: the n deformations, 2*N*N each */
   int npsi;
+  double yreg; /* x-shift: yRegFactor */
 } Energy;
 
 enum { DT_CONSISTENCY = 100 }; /* internal: reduceDeformations' energy */
@@ -738,6 +953,13 @@ static double energyAt(Energy *e, const double *phi) {
   double E = 0;
   if (e->data_term == DT_CONSISTENCY) {  /* LinCombOp of the n consistency energies, coefficient 1 */
     for (int i = 0; i < e->npsi; ++i) E += msor_consistency_energy(e->psi + (size_t)i * 2 * e->N * e->N, phi, e->N);
+    return E;
+  }
+  if (e->data_term == MSOR_XSHIFT) {  /* (0 + E_data) + lambda E_reg (LinCombOp, matchSeries.h:844-846) */
+    double ed, er;
+    msor_xshift_energy(e->R, e->T, phi, e->N, e->yreg, &ed, &er);
+    E += ed;
+    E += e->lambda * er;
     return E;
   }
   if (e->data_term == MSOR_NCC) {
@@ -761,6 +983,11 @@ static void gradientAt(Energy *e, const double *phi, double *out) {
       for (int k = 0; k < 2 * NN; ++k) out[k] += tmp[k];
     }
     free(tmp);
+    return;
+  }
+  if (e->data_term == MSOR_XSHIFT) {  /* the y component stays zero */
+    msor_xshift_gradient(e->R, e->T, phi, e->N, e->yreg, e->lambda, out);
+    memset(out + NN, 0, sizeof(double) * NN);
     return;
   }
   double *k = (double *)malloc(sizeof(double) * NN);
@@ -916,7 +1143,8 @@ int msor_level_solve(const double *R, const double *T, double *phi, int N, const
                      double *energy_trace, double *tau_trace, double *sigma_trace, int *evals_trace,
                      int trace_cap, int *n_armijo_calls, double *final_energy) {
   const int NN = N * N, n2 = 2 * NN;
-  Energy e = {R, T, N, p->lambda, (double *)malloc(sizeof(double) * n2), 0, p->data_term, NULL, NULL, NULL, 0};
+  Energy e = {R, T, N, p->lambda, (double *)malloc(sizeof(double) * n2), 0, p->data_term, NULL, NULL, NULL, 0,
+              p->y_reg};
   if (p->data_term == MSOR_NCC) {
     e.Rn = (double *)malloc(sizeof(double) * NN);
     e.Tn = (double *)malloc(sizeof(double) * NN);
@@ -934,7 +1162,7 @@ int msor_level_solve(const double *R, const double *T, double *phi, int N, const
 int msor_reduce_deformations(const double *psi, int n, int N, int max_iter, double stop_eps, double *phi,
                              double *final_energy) {
   const int n2 = 2 * N * N;
-  Energy e = {NULL, NULL, N, 0.0, (double *)malloc(sizeof(double) * n2), 0, DT_CONSISTENCY, NULL, NULL, psi, n};
+  Energy e = {NULL, NULL, N, 0.0, (double *)malloc(sizeof(double) * n2), 0, DT_CONSISTENCY, NULL, NULL, psi, n, 0.0};
   memset(phi, 0, sizeof(double) * n2);
   const int iter = descent(&e, phi, max_iter, stop_eps, sqrt(10 * gridH(N)), 0, 1, NULL, NULL, NULL, NULL, 0, NULL,
                            final_energy);
@@ -1177,7 +1405,7 @@ static double solveML(const Pyramid *R, const Pyramid *T, DefPyramid *d, int sta
                       const msor_series_params *p) {
   double energy = NAN;
   for (int level = start; level <= stop; ++level) {
-    msor_level_params lp = {levelLambda(p, level), p->max_iter, p->stop_eps, p->data_term};
+    msor_level_params lp = {levelLambda(p, level), p->max_iter, p->stop_eps, p->data_term, 0.0};
     msor_level_solve(R->lev[level], T->lev[level], d->lev[level], 1 << level, &lp, NULL, NULL, NULL, NULL, 0,
                      NULL, &energy);
     if (level < stop) defProlong(d, level, p->resample_prolong);

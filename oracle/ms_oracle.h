@@ -47,14 +47,24 @@ void msor_vcycle(const double *d, int N, double sigma, double *out);
  * phi (2*N*N) is updated in place. Traces (may be NULL) are written per Armijo call
  * (tau, sigma at the call, energy evaluations of the call) and per iteration (energy after the
  * step, energy_trace[0] = initial energy); trace_cap bounds every trace. Returns iterations. */
-enum { MSOR_SSD = 0, MSOR_NCC = 1 }; /* data term: deformationModel 22 / deformationModel 0 */
+/* data term: deformationModel 22 / deformationModel 0 / the seismic line-wise x-shift NCC */
+enum { MSOR_SSD = 0, MSOR_NCC = 1, MSOR_XSHIFT = 2 };
 
 typedef struct {
   double lambda;      /* regularisation weight of this level            */
   int max_iter;       /* maxGDIterations                                */
   double stop_eps;    /* stopEpsilon                                    */
-  int data_term;      /* MSOR_SSD or MSOR_NCC                           */
+  int data_term;      /* MSOR_SSD, MSOR_NCC or MSOR_XSHIFT              */
+  double y_reg;       /* MSOR_XSHIFT: yRegFactor                        */
 } msor_level_params;
+
+/* --- seismic x-shift (FELineShiftRegisEnergy, registration.h:2137-2193; findXShift,
+ * matchSeries.h:805-868): phix is the N*N x component; the y component of the solver's field stays
+ * zero. Energies unweighted (E = e_data + lambda e_reg); the gradient includes lambda A phix. --- */
+void msor_xshift_energy(const double *R, const double *T, const double *phix, int N, double yreg, double *e_data,
+                        double *e_reg);
+void msor_xshift_gradient(const double *R, const double *T, const double *phix, int N, double yreg, double lambda,
+                          double *out);
 
 int msor_level_solve(const double *R, const double *T, double *phi, int N, const msor_level_params *p,
                      double *energy_trace, double *tau_trace, double *sigma_trace, int *evals_trace,

@@ -16,12 +16,12 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 
 
-SSD, NCC = 0, 1  # data terms: deformationModel 22 / deformationModel 0
+SSD, NCC, XSHIFT = 0, 1, 2  # data terms: deformationModel 22 / deformationModel 0 / seismic line shift
 
 
 class LevelParams(ctypes.Structure):
     _fields_ = [("lambda_", ctypes.c_double), ("max_iter", ctypes.c_int), ("stop_eps", ctypes.c_double),
-                ("data_term", ctypes.c_int)]
+                ("data_term", ctypes.c_int), ("y_reg", ctypes.c_double)]
 
 
 class SeriesParams(ctypes.Structure):
@@ -174,7 +174,7 @@ def vcycle(d, sigma):
     return out
 
 
-def level_solve(R, T, phi0, lam, max_iter, stop_eps, trace_cap=None, data_term=SSD):
+def level_solve(R, T, phi0, lam, max_iter, stop_eps, trace_cap=None, data_term=SSD, y_reg=0.0):
     """Returns (phi, info) with info: iterations, final energy, energy trace per iteration and
     tau / sigma / evaluations per Armijo call."""
     R, T = _f64(R), _f64(T)
@@ -187,12 +187,29 @@ def level_solve(R, T, phi0, lam, max_iter, stop_eps, trace_cap=None, data_term=S
     ev = np.zeros(cap, dtype=np.int32)
     calls = ctypes.c_int(0)
     fe = ctypes.c_double(0)
-    p = LevelParams(lam, max_iter, stop_eps, data_term)
+    p = LevelParams(lam, max_iter, stop_eps, data_term, y_reg)
     it = lib().msor_level_solve(_p(R), _p(T), _p(phi), N, ctypes.byref(p), _p(et), _p(tt), _p(st),
                                 ev.ctypes.data_as(_ip), cap, ctypes.byref(calls), ctypes.byref(fe))
     nc = calls.value
     return phi, dict(iterations=it, energy=fe.value, energies=et[: it + 1].copy(), taus=tt[:nc].copy(),
                      sigmas=st[:nc].copy(), evals=ev[:nc].copy())
+
+
+def xshift_energy(R, T, phix, y_reg):
+    """(E_data, E_reg) of the seismic x-shift registration (the mean row NCC and 1/2 phix.A phix)."""
+    R, T, phix = _f64(R), _f64(T), _f64(phix)
+    ed, er = ctypes.c_double(), ctypes.c_double()
+    lib().msor_xshift_energy(_p(R), _p(T), _p(phix), R.shape[-1], ctypes.c_double(y_reg), ctypes.byref(ed),
+                             ctypes.byref(er))
+    return ed.value, er.value
+
+
+def xshift_gradient(R, T, phix, y_reg, lam):
+    R, T, phix = _f64(R), _f64(T), _f64(phix)
+    out = np.zeros_like(phix)
+    lib().msor_xshift_gradient(_p(R), _p(T), _p(phix), R.shape[-1], ctypes.c_double(y_reg), ctypes.c_double(lam),
+                               _p(out))
+    return out
 
 
 def normalize(img, pct=-1.0, min_to_zero=True):

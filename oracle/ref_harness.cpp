@@ -17,6 +17,7 @@
 #include <gradientDescent.h>
 #include <multilevelArray.h>
 #include <AmbrosioTortorelli.h>
+#include <matchSeries.h>
 
 #include <cstdio>
 #include <cstdlib>
@@ -89,6 +90,38 @@ protected:
                                               const RType tw) const {
     if (tw != 0) ++evals;
     return GDType::ArmijoLineSearchHelpFunction_evaluate(Dir, Pos, tw);
+  }
+};
+
+// The seismic x-shift solve of findXShift (matchSeries.h:805-868) on the dyadic grid: a scalar field
+// with the V-cycle metric and the automatic filter width, traced like TracingGD.
+typedef qc::RectangularGridConfigurator<RType, qc::QC_1D, aol::GaussQuadrature<RType, qc::QC_1D, 3> > Conf1D;
+typedef qc::FELineShiftRegisEnergy<ConfType, Conf1D, qc::NCCRegistrationConfigurator<Conf1D> > LineEnergy;
+typedef aol::H1GradientDescent<ConfType, aol::Vector<RType>, SmoothType> H1VType;
+typedef aol::GradientDescentWithAutomaticFilterWidth<ConfType, aol::Vector<RType>, H1VType> GDVType;
+
+class TracingGDV : public GDVType {
+public:
+  mutable std::vector<double> taus, sigmas;
+  mutable int evals;
+  mutable std::vector<int> evalsPerIter;
+  TracingGDV(const GridType &g, const aol::Op<aol::Vector<RType>, aol::Scalar<RType> > &E,
+             const aol::Op<aol::Vector<RType> > &DE, int maxIt)
+      : GDVType(g, E, DE, maxIt), evals(0) {}
+protected:
+  RType getStepSizeWithSelectedController(aol::Vector<RType> &Dir, const aol::Vector<RType> &Pos,
+                                          const RType OldTau) const {
+    const int before = evals;
+    RType t = GDVType::getStepSizeWithSelectedController(Dir, Pos, OldTau);
+    taus.push_back(t);
+    sigmas.push_back(this->_filterWidth);
+    evalsPerIter.push_back(evals - before);
+    return t;
+  }
+  RType ArmijoLineSearchHelpFunction_evaluate(const aol::Vector<RType> &Dir, const aol::Vector<RType> &Pos,
+                                              const RType tw) const {
+    if (tw != 0) ++evals;
+    return GDVType::ArmijoLineSearchHelpFunction_evaluate(Dir, Pos, tw);
   }
 };
 
@@ -330,6 +363,63 @@ int main(int argc, char **argv) {
       for (size_t k = 0; k < gd.taus.size(); ++k) out.push_back(gd.taus[k]);
       for (size_t k = 0; k < gd.sigmas.size(); ++k) out.push_back(gd.sigmas[k]);
       for (size_t k = 0; k < gd.evalsPerIter.size(); ++k) out.push_back(gd.evalsPerIter[k]);
+    } else if (cmd == "xshift" || cmd == "xshift_solve") {
+      // params: lambda yRegFactor [maxIt] ; in: R, T, phix
+      // xshift: out E_data, E_reg (1/2 phix.A phix), DE (N^2) of findXShift's E and DE
+      // xshift_solve: findXShift's descent from phix (laplaceWeight 0); out phix, iterations, E_final,
+      // E_total per iteration (from 0), number of Armijo calls, tau / sigma / evals per call
+      const double lambda = atof(argv[3]), yreg = atof(argv[4]);
+      const int maxIt = cmd == "xshift_solve" ? atoi(argv[5]) : 0;
+      qc::ScalarArray<RType, qc::QC_2D> R(grid), T(grid), phix(grid);
+      toVec(&in[0], R); toVec(&in[NN], T); toVec(&in[2 * NN], phix);
+      LineEnergy eData(grid, R, T);
+      const aol::DerivativeWrapper<RType, LineEnergy, aol::Vector<RType> > deData(eData);
+      const aol::FEOpMixedDerivative<ConfType> stiffX(grid, 0, 0, aol::ONTHEFLY);
+      const aol::FEOpMixedDerivative<ConfType> stiffY(grid, 1, 1, aol::ONTHEFLY);
+      ConfType::MatrixType mat(grid);
+      stiffX.assembleAddMatrix(mat);
+      stiffY.assembleAddMatrix(mat, yreg);
+      const aol::QuadraticFormOp<aol::Vector<RType> > eReg(mat);
+      aol::LinCombOp<aol::Vector<RType>, aol::Scalar<RType> > E;
+      E.appendReference(eData);
+      E.appendReference(eReg, lambda);
+      aol::LinCombOp<aol::Vector<RType> > DE;
+      DE.appendReference(deData);
+      DE.appendReference(mat, lambda);
+      if (cmd == "xshift") {
+        aol::Scalar<RType> ed, er;
+        eData.apply(phix, ed);
+        eReg.apply(phix, er);
+        out.push_back(ed[0]);
+        out.push_back(er[0]);
+        aol::Vector<RType> de(NN);
+        DE.apply(phix, de);
+        append(out, de);
+      } else {
+        TracingGDV gd(grid, E, DE, maxIt);
+        gd.setConfigurationFlags(GDVType::USE_NONLINEAR_CG | GDVType::DO_NOT_WRITE_CONSOLE_OUTPUT);
+        std::stringstream energyLog;
+        energyLog.precision(17);
+        gd.setOutStream(energyLog);
+        gd.applySingle(phix);
+        append(out, phix);
+        std::vector<double> etot;
+        std::string line;
+        while (std::getline(energyLog, line)) {
+          std::istringstream ls(line);
+          std::vector<double> vals; double v;
+          while (ls >> v) vals.push_back(v);
+          if (!vals.empty()) etot.push_back(vals.back());
+        }
+        const int iters = static_cast<int>(etot.size()) - 1;
+        out.push_back(iters);
+        out.push_back(gd.getEnergyAtLastPosition());
+        for (int k = 0; k <= iters; ++k) out.push_back(etot[k]);
+        out.push_back(static_cast<double>(gd.taus.size()));
+        for (size_t k = 0; k < gd.taus.size(); ++k) out.push_back(gd.taus[k]);
+        for (size_t k = 0; k < gd.sigmas.size(); ++k) out.push_back(gd.sigmas[k]);
+        for (size_t k = 0; k < gd.evalsPerIter.size(); ++k) out.push_back(gd.evalsPerIter[k]);
+      }
     } else {
       fprintf(stderr, "unknown command %s\n", cmd.c_str());
       return 2;
