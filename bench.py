@@ -17,6 +17,14 @@ reference and template, 6 FP32 fields of N^2; one launch performs 3 evaluations 
 which the measured `traffic` shows (rocprofv3 FETCH_SIZE + WRITE_SIZE of the same launch, calibrated
 on a plain copy, from profiles/k_energy_f32_8pairs_1024_traffic.json). `--roofline-only` runs just this measurement (for rocprofv3).
 
+Whole step: the iterations and energy evaluations every solve of the step executed (counted by the
+library, weighted by each level's node count) priced with the path's byte model, per node and
+iteration 4 B x (73 1/3 + 6 n_E) (DE 6 field passes, H1 V-cycle 49 1/3, NCG 12, update 6, and 6 per
+energy evaluation), plus the series average (6 passes per frame); the fraction of 8 TB/s it reaches.
+
+FP64 line: one more step of the same series with FP64 fields (the reference's arithmetic), its time
+and its gap to the FP32 result (largest deformation difference, largest relative energy difference).
+
 CPU baseline: the reference's own level solve (oracle/_ref/ref_harness, compiled from the reference
 sources) timed for a few iterations at 1024^2 on this host, extrapolated to the same schedule (cost
 proportional to N^2).
@@ -151,6 +159,7 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="pairwise solves per batch (0: automatic)")
     ap.add_argument("--classes", action="store_true", help="also time every kernel class in one extra step")
     ap.add_argument("--roofline-only", action="store_true", help="only the energy-pass roofline measurement")
+    ap.add_argument("--no-f64", action="store_true", help="skip the FP64 comparison step")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -196,6 +205,32 @@ def main():
     dt_s = t1 - t0
     ms_step = 1000.0 * dt_s / args.steps
     value = F * args.steps / dt_s
+    cnt = gpu.series_last_counts()  # the last timed step
+    step_bytes = 4.0 * ((73.0 + 1.0 / 3.0) * cnt["node_iterations"] + 6.0 * cnt["node_evals"]) + 4.0 * N * N * 6 * F
+    whole_step = {"iterations": cnt["iterations"], "energy_evaluations": cnt["evals"], "energy_passes": cnt["passes"],
+                  "n_E": round(cnt["evals"] / max(cnt["iterations"], 1), 3),
+                  "node_iterations": cnt["node_iterations"], "algorithmic_bytes": step_bytes,
+                  "achieved_gbs": round(step_bytes / (ms_step * 1e-3) / 1e9, 1),
+                  "frac": round(step_bytes / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
+
+    f64 = None
+    if not args.no_f64 and args.precision == "f32":
+        d64 = torch.tensor(frames, dtype=torch.float64, device="cuda")
+        t64, r64 = d64[1:].contiguous(), d64[0].contiguous()
+        gpu.match_series(t64, r64, p)  # warm-up (contexts, graphs)
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        o64 = gpu.match_series(t64, r64, p)
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        dd = (out["deformations"].double() - o64["deformations"]).abs().reshape(F - 1, -1).amax(dim=1).cpu().numpy()
+        de = np.abs(out["energies"] - o64["energies"]) / np.abs(o64["energies"])
+        f64 = {"ms_per_step": round(1000.0 * (tb - ta), 2), "value": round(F / (tb - ta), 3), "unit": "frames/s",
+               "fp32_vs_fp64_max_deformation_px": round(float(dd.max()) * (N - 1), 6),
+               "fp32_vs_fp64_median_deformation_px": round(float(np.median(dd)) * (N - 1), 6),
+               "fp32_vs_fp64_max_rel_energy": float(de.max()),
+               "frames_over_1e-3_px": int((dd * (N - 1) > 1e-3).sum())}
+        del d64, t64, r64, o64
 
     roofline = energy_roofline(gpu, frames, out["deformations"][1:9].double().cpu().numpy())
 
@@ -237,6 +272,8 @@ def main():
                    "frames": F, "size": N, "iterations_per_level": args.iters, "method": "plain",
                    "iteration_mode": args.mode, "armijo_candidates": args.spec or 3},
         "roofline": roofline,
+        "whole_step": whole_step,
+        "fp64": f64,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
         "kernel_classes": classes,
