@@ -1,0 +1,238 @@
+"""The benchmark configurations on the GPU against the reference program and the oracle.
+
+Fixtures (tests/golden/make_configs.py): cfg1 (8 frames of 256^2, levels 6-8, 50 it/level) and the
+1024^2 generator's 256^2 (4 frames) and 512^2 (3 frames) crops (levels L-3..L, 100 it/level); frames
+are regenerated from the seeded generator here. Each fixture also holds the oracle's own sensitivity
+to rounding-level input changes (ulp_spread_*: FP64 frames times 1 +- 2^-52; f32_spread_*: FP32
+inputs times 1 +- 2^-24); the bounds below add a multiple of it, so a configuration whose trajectory
+is decided by rounding-level step decisions is held to the spread the reference itself shows.
+
+  * FP64 fields reproduce the reference program's stage 1: deformations within 2e-6 px (the fixture
+    stores pixels as float32, 5e-7 px) plus 10 ulp spreads; energies within 1e-5 relative (the
+    reference writes energies.txt with 6 significant digits) plus 10 ulp spreads (the GPU's summation
+    order is one more rounding-level perturbation; the spread is the largest of a few samples);
+  * FP32 fields given the FP32-rounded normalised frames the FP32 mode computes on, against the oracle
+    run on those same values: 1e-3 px and 1e-6 relative energy plus 20 f32 spreads (the spread is the
+    largest of a few sampled input perturbations; the FP32 run is one more such perturbation, drawn
+    from the same distribution, whose tail the few samples underestimate);
+  * FP32 fields on the FP64 frames (the benchmark's path, normalised on the GPU in FP32): the gap to
+    the reference program is printed and bounded;
+  * the full 32 x 1024^2 benchmark series: finite outputs, per-level energies that never increase by
+    more than the FP32 pass's error bound, and the FP32 run's gap to the FP64 run of the same series.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from msamd import gpu as G
+    G.lib()
+    return G
+
+
+def config(name):
+    from msamd import synth
+    if name == "cfg1":
+        s = synth.CPU_SERIES
+        fr = synth.stem_series(s["N"], s["frames"], workload=s["workload"], spacing=s["spacing"], sigma=s["sigma"],
+                               jitter_step=s["jitter_step"])
+        return fr, "cfg1_256.npz"
+    S = int(name.split("_")[1])
+    g = synth.GPU_SERIES
+    fr = synth.stem_series(g["N"], 4 if S == 256 else 3, workload=g["workload"], spacing=g["spacing"],
+                           sigma=g["sigma"], jitter_step=g["jitter_step"])[:, :S, :S]
+    return np.ascontiguousarray(fr), f"{name}.npz"
+
+
+def params(gpu, lv, **kw):
+    l0, L, r0, its = (int(v) for v in lv)
+    base = dict(start_level=l0, stop_level=L, refine_start_level=r0, refine_stop_level=L, precision_level=L,
+                lambda_=10.0, lambda_factor=-1.0, max_iter=its, stop_eps=-1.0, saturation_pct=0.15,
+                normalize_min_to_zero=1, dont_normalize=0, resample_prolong=1, dont_accumulate=0, spec=0)
+    base.update(kw)
+    return gpu.SeriesParams(**base)
+
+
+def load(name):
+    frames, fn = config(name)
+    path = os.path.join(GOLD, fn)
+    if not os.path.exists(path):
+        pytest.skip(f"{fn} not generated")
+    with np.load(path) as z:
+        g = {k: z[k] for k in z.files}
+    return frames, g
+
+
+NAMES = ["cfg1", "cfg2_256", "cfg2_512"]
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_config_f64_matches_reference(gpu, name):
+    frames, g = load(name)
+    N = frames.shape[-1]
+    print(f"{name}: {len(frames)} frames of {N}^2, FP64", flush=True)
+    t = torch.tensor(frames, dtype=torch.float64, device="cuda")
+    out = gpu.match_series(t[1:].contiguous(), t[0].contiguous(), params(gpu, g["levels"]))
+    d = out["deformations"].cpu().numpy() * (N - 1)
+    err = np.abs(d - g["ref_defs_px"]).max()
+    egap = np.max(np.abs(out["energies"] - g["ref_energies"]) / np.abs(g["ref_energies"]))
+    spread, espread = float(g["ulp_spread_px"]), float(g["ulp_spread_energy"])
+    print(f"{name} FP64: max deformation difference {err:.2e} px, energies {egap:.2e} (the oracle's own 1-ulp "
+          f"spread: {spread:.2e} px, {espread:.2e})", flush=True)
+    assert err < 2e-6 + 10 * spread
+    assert egap < 1e-5 + 10 * espread
+    if spread < 1e-6:  # a well-conditioned configuration: the reconstruction too
+        np.testing.assert_array_equal(out["count"].cpu().numpy(), g["ref_count"])
+        assert np.abs(out["median"].cpu().numpy() - g["ref_median"]).max() < 1e-6
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_config_f32_matches_oracle_on_its_inputs(gpu, name):
+    frames, g = load(name)
+    N = frames.shape[-1]
+    norm = np.stack([O.normalize(f, 0.15, True) for f in frames]).astype(np.float32)
+    print(f"{name}: {len(frames)} frames of {N}^2, FP32 on FP32-rounded normalised frames", flush=True)
+    t = torch.tensor(norm, dtype=torch.float32, device="cuda")
+    out = gpu.match_series(t[1:].contiguous(), t[0].contiguous(),
+                           params(gpu, g["levels"], dont_normalize=1, saturation_pct=-1.0))
+    d = out["deformations"].cpu().numpy().astype(np.float64) * (N - 1)
+    err = np.abs(d - g["o32_defs_px"]).max()
+    egap = np.max(np.abs(out["energies"] - g["o32_energies"]) / np.abs(g["o32_energies"]))
+    spread, espread = float(g["f32_spread_px"]), float(g["f32_spread_energy"])
+    print(f"{name} FP32 vs oracle on the same inputs: max deformation difference {err:.2e} px, "
+          f"max relative energy difference {egap:.2e} (the oracle's own spread under 2^-24 input changes: "
+          f"{spread:.2e} px, {espread:.2e})", flush=True)
+    assert err < 1e-3 + 20 * spread
+    assert egap < 1e-6 + 20 * espread
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_config_f32_gap_to_reference(gpu, name):
+    """The benchmark's FP32 path from the FP64 frames: input rounding (frames, normalisation in FP32)
+    moves the start of every solve by ~1e-7 relative; the gap to the FP64 reference is reported."""
+    frames, g = load(name)
+    N = frames.shape[-1]
+    t = torch.tensor(frames, dtype=torch.float32, device="cuda")
+    out = gpu.match_series(t[1:].contiguous(), t[0].contiguous(), params(gpu, g["levels"]))
+    d = out["deformations"].cpu().numpy().astype(np.float64) * (N - 1)
+    err = np.abs(d - g["ref_defs_px"]).max()
+    egap = np.max(np.abs(out["energies"] - g["ref_energies"]) / np.abs(g["ref_energies"]))
+    print(f"{name} FP32 (FP32 inputs) vs reference: max deformation difference {err:.2e} px, "
+          f"max relative energy difference {egap:.2e}", flush=True)
+    assert np.isfinite(d).all()
+    assert err < 0.25 and egap < 1e-3
+
+
+def test_f32_energy_pass_error_bound(gpu):
+    """The FP32-arithmetic energy pass against FP64 arithmetic on the same FP32 values (STEM frames,
+    smooth displacements of up to 6 px plus noise): the relative error stays well inside the bound
+    the step decisions use (2e-8 of |E|, checked here at a quarter of it)."""
+    from msamd import synth
+    worst = 0.0
+    for N, amp in ((64, 2.0), (256, 4.0), (1024, 6.0)):
+        s = synth.stem_series(N, 3, workload=1, spacing=max(16.0 * N / 1024, 3.0), sigma=max(3.0 * N / 1024, 0.8),
+                              jitter_step=0.4 * N / 1024)
+        R = np.stack([O.normalize(s[0], 0.15, True)] * 2)
+        T = np.stack([O.normalize(s[1], 0.15, True), O.normalize(s[2], 0.15, True)])
+        h = 1.0 / (N - 1)
+        y, x = np.mgrid[0:N, 0:N] * h
+        rng = np.random.default_rng(N)
+        phi = np.stack([np.stack([amp * h * np.sin(3 * y + k), amp * h * np.cos(2 * x - k)]) for k in range(2)])
+        phi += 0.2 * h * rng.standard_normal(phi.shape)
+        t32 = [torch.tensor(a, dtype=torch.float32, device="cuda") for a in (R, T, phi)]
+        t64 = [a.double() for a in t32]
+        e32, r32 = gpu.energy(*t32)
+        e64, r64 = gpu.energy(*t64)
+        tot = np.abs(e64 + 10.0 * r64)
+        rel = np.maximum(np.abs(e32 - e64), 10.0 * np.abs(r32 - r64)) / tot
+        print(f"N={N}: FP32 energy pass relative error {rel.max():.2e}", flush=True)
+        worst = max(worst, rel.max())
+    assert worst < 0.5e-8
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_level_solve_256_same_steps_as_oracle(gpu, precision):
+    """Level solves of the 256^2 cfg1 pair in FP32 with near-tie decisions re-evaluated in FP64: the
+    same step size as the oracle at every Armijo call on the same (FP32-rounded) inputs."""
+    from msamd import synth
+    s = synth.CPU_SERIES
+    fr = synth.stem_series(s["N"], 2, workload=s["workload"], spacing=s["spacing"], sigma=s["sigma"],
+                           jitter_step=s["jitter_step"])
+    R, T = (O.normalize(f, 0.15, True).astype(np.float32).astype(np.float64) for f in fr)
+    N, L = 256, 8
+    dt = gpu.F32 if precision == "f32" else gpu.F64
+    ctx = gpu.Context(L, 1, dt)
+    ctx.set_reference(0, R)
+    ctx.set_template(0, T)
+    e, it, tr = ctx.solve_level(L, 10.0, 50, -1.0, trace_cap=50)
+    phi = ctx.get_deformation(0)
+    ctx.close()
+    phi_o, info = O.level_solve(R, T, np.zeros((2, N, N)), 10.0, 50, -1.0)
+    calls = []
+    for row in tr[0]:
+        calls.append(row[3])
+        if row[3] == 0:
+            calls.append(row[2])
+    np.testing.assert_array_equal(np.array(calls), info["taus"])
+    assert abs(e[0] - info["energy"]) <= 1e-6 * abs(info["energy"])
+    assert np.abs(phi - phi_o).max() * (N - 1) < 1e-3
+
+
+def test_cfg2_full_size_properties(gpu):
+    """The benchmark series (32 frames of 1024^2, levels 7-10, refinement 9-10, 100 it/level) in FP32:
+    finite outputs, energies that never increase within a level solve, and the gap to the FP64 run of
+    the same series.
+
+    The series is a square lattice of 16 px period whose frames move against each other by up to
+    about half a period (row jitter clipped at 4 px plus 2 px sine modes in each frame), so the
+    accumulated chain can settle one lattice period apart from frame to frame: which alignment a
+    chained solve lands on is decided by rounding-level differences. FP32 and FP64 runs therefore
+    agree frame by frame only up to whole lattice periods; away from the border (64 px) the difference
+    of their deformations, less the nearest multiple of the period, stays within 1 px."""
+    from msamd import synth
+    g = synth.GPU_SERIES
+    fr = synth.stem_series(g["N"], g["frames"], workload=g["workload"], spacing=g["spacing"], sigma=g["sigma"],
+                           jitter_step=g["jitter_step"])
+    res = {}
+    for dt in (torch.float32, torch.float64):
+        print(f"cfg2 full size, {dt}", flush=True)
+        t = torch.tensor(fr, dtype=dt, device="cuda")
+        res[dt] = gpu.match_series(t[1:].contiguous(), t[0].contiguous(), params(gpu, (7, 10, 9, 100)),
+                                   want_levels=(dt == torch.float32))
+        del t
+    o32, o64 = res[torch.float32], res[torch.float64]
+    d32 = o32["deformations"].double().cpu().numpy()
+    d64 = o64["deformations"].cpu().numpy()
+    assert np.isfinite(d32).all() and torch.isfinite(o32["median"]).all()
+    # an accepted step never raises the energy; a near-tie decision replaces the FP32 value of the
+    # current energy by its FP64-arithmetic evaluation, which may sit above it by the pass's error bound
+    for lv in o32["levels"]:
+        e = np.concatenate([[lv["start"][0]], lv["trace"][:, 0]])
+        assert np.all(np.diff(e) <= 5e-8 * np.abs(e[:-1])), (lv["kind"], lv["frame"], lv["level"])
+    P, b = float(g["spacing"]), 64
+    worst_rest = 0.0
+    for i in range(d32.shape[0]):
+        D = (d32[i] - d64[i])[:, b:-b, b:-b] * 1023
+        m = D.reshape(2, -1).mean(axis=1)
+        k = np.round(m / P)
+        rest = np.abs(D - (k * P)[:, None, None]).max()
+        worst_rest = max(worst_rest, rest)
+        print(f"  template {i + 1}: FP32 energy {o32['energies'][i]:.6e}, FP64 {o64['energies'][i]:.6e}, "
+              f"mean difference ({m[0]:+.3f}, {m[1]:+.3f}) px = periods ({k[0]:+.0f}, {k[1]:+.0f}) + "
+              f"at most {rest:.3f} px", flush=True)
+    full = np.abs(d32 - d64).max() * 1023
+    print(f"cfg2 32x1024^2: FP32 vs FP64 max deformation difference {full:.3e} px; less whole lattice periods, "
+          f"away from the border: {worst_rest:.3e} px", flush=True)
+    assert worst_rest < 1.0
