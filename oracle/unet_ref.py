@@ -71,18 +71,21 @@ def _virtual(acts_y, c):
     return x1 if c["in2"] < 0 else np.concatenate([x1, acts_y[c["in2"]]], axis=3)
 
 
-def conv3x3(x, w):
-    """y[b, y, x, o] = sum_tap sum_i w[tap, o, i] x[b, y + dy, x + dx, i] (zero outside)."""
+def conv3x3(x, w, acc32=False):
+    """y[b, y, x, o] = sum_tap sum_i w[tap, o, i] x[b, y + dy, x + dx, i] (zero outside), accumulated
+    in FP64, or in FP32 with acc32 (another summation that is as valid for BF16 operands: the two
+    differ by accumulation rounding only, which sets the noise floor of any comparison)."""
     B, H, W, _ = x.shape
-    xp = np.pad(x, ((0, 0), (1, 1), (1, 1), (0, 0)))
-    y = np.zeros((B, H, W, w.shape[1]))
+    ft = np.float32 if acc32 else np.float64
+    xp = np.pad(x, ((0, 0), (1, 1), (1, 1), (0, 0))).astype(ft)
+    y = np.zeros((B, H, W, w.shape[1]), dtype=ft)
     for t in range(9):
         dy, dx = t // 3 - 1, t % 3 - 1
-        y += xp[:, 1 + dy:1 + dy + H, 1 + dx:1 + dx + W, :] @ w[t].T
-    return y
+        y += xp[:, 1 + dy:1 + dy + H, 1 + dx:1 + dx + W, :] @ w[t].T.astype(ft)
+    return y.astype(np.float64)
 
 
-def forward(params, x_in, scales, base, bf16=True):
+def forward(params, x_in, scales, base, bf16=True, acc32=False):
     """x_in: (B, 2, H, W) pairs. Returns (out (B, 2, H, W), cache)."""
     acts, convs, pools, head, _ = graph(scales, base)
     rnd = round_bf16 if bf16 else (lambda a: a)
@@ -101,7 +104,7 @@ def forward(params, x_in, scales, base, bf16=True):
         cin = c["cin1"] + c["cin2"]
         w = rnd(params[c["woff"]:c["woff"] + 9 * c["cout"] * cin].reshape(9, c["cout"], cin))
         b = params[c["boff"]:c["boff"] + c["cout"]]
-        ys[c["out"]] = rnd(np.maximum(conv3x3(_virtual(ys, c), w) + b, 0.0))
+        ys[c["out"]] = rnd(np.maximum(conv3x3(_virtual(ys, c), w, acc32) + b, 0.0))
     xh = ys[head["input"]]
     wh = params[head["woff"]:head["woff"] + 2 * base].reshape(2, base)
     bh = params[head["boff"]:head["boff"] + 2]
@@ -109,11 +112,12 @@ def forward(params, x_in, scales, base, bf16=True):
     return out, ys
 
 
-def loss_and_grad(params, x_in, target, scales, base, bf16=True):
-    """Loss 1/(2 n) sum (out - target)^2 and its gradient with respect to the flat parameters."""
+def loss_and_grad(params, x_in, target, scales, base, bf16=True, acc32=False):
+    """Loss 1/(2 n) sum (out - target)^2 and its gradient with respect to the flat parameters
+    (acc32: the forward convolutions accumulated in FP32)."""
     acts, convs, pools, head, npar = graph(scales, base)
     rnd = round_bf16 if bf16 else (lambda a: a)
-    out, ys = forward(params, x_in, scales, base, bf16)
+    out, ys = forward(params, x_in, scales, base, bf16, acc32)
     n = out.size
     e = out - target
     loss = 0.5 * np.sum(e * e) / n
