@@ -25,6 +25,13 @@ energy evaluation), plus the series average (6 passes per frame); the fraction o
 FP64 line: one more step of the same series with FP64 fields (the reference's arithmetic), its time
 and its gap to the FP32 result (largest deformation difference, largest relative energy difference).
 
+Seismic line (matchSeismicSeries, the x-shift registration): 24 synthetic gathers of 2048 time
+samples x 512 traces (msamd.synth.seismic_gather: 60 interfaces, 40 Hz Ricker at 0.5 ms, hyperbolic
+moveout, static shifts), resampled onto the dyadic 1024^2 grid the reference's registration object
+solves on (its default dyadicDepth for 2048 samples), stage 1 at levels 7-10, refinement 9-10, at
+most 100 iterations per level, lambda 5, yRegFactor 1, FP32 fields. The reference's
+full-resolution step (Cholesky H1 metric) is not part of this build.
+
 CPU baseline: the reference's own level solve (oracle/_ref/ref_harness, compiled from the reference
 sources) timed for a few iterations at 1024^2 on this host, extrapolated to the same schedule (cost
 proportional to N^2).
@@ -143,6 +150,45 @@ def energy_roofline(gpu, frames, defs=None, batch=8, reps=100):
             "shape": f"{batch} pairs x {N}^2, {evals} step sizes, FP32 fields"}
 
 
+def seismic_line(gpu, torch, count=24, size=1024, its=100):
+    """Stage 1 of matchSeismicSeries' dyadic registration on `count` gathers resampled to size^2, with
+    FP32 fields (timed) and FP64 fields (timed, and the FP32 result's gap to it)."""
+    from msamd import synth
+    L = int(np.log2(size))
+    g = synth.seismic_series(count, workload=synth.SEISMIC_SERIES["workload"])
+    p = series_params(gpu, (L - 3, L), (L - 1, L), its, lam=5.0)
+    p.data_term = gpu.XSHIFT
+    p.y_reg_factor = 1.0
+    p.keep_start_transformation = 1
+    p.stop_eps = 0.0
+    p.saturation_pct = -1.0
+    res = {}
+    for name, dt in (("f32", torch.float32), ("f64", torch.float64)):
+        raw = torch.tensor(g, dtype=dt, device="cuda")
+        dev = torch.stack([gpu.resample_rect(raw[k].contiguous(), size) for k in range(count)]).contiguous()
+        tpl, ref = dev[1:].contiguous(), dev[0].contiguous()
+        gpu.match_series(tpl, ref, p)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = gpu.match_series(tpl, ref, p)
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0, out, gpu.series_last_counts())
+        del raw, dev, tpl, ref
+    t, out, cnt = res["f32"]
+    t64, out64, _ = res["f64"]
+    dd = (out["deformations"].double() - out64["deformations"]).abs().reshape(count - 1, -1).amax(dim=1).cpu().numpy()
+    de = np.abs(out["energies"] - out64["energies"]) / np.abs(out64["energies"])
+    return {"workload": f"matchSeismicSeries stage 1 (dyadic path): {count} gathers of 2048 samples x 512 traces "
+                        f"resampled to {size}^2, levels {L - 3}-{L}, refine {L - 1}-{L}, <= {its} it/level, "
+                        f"lambda 5, yRegFactor 1",
+            "value": round(count / t, 3), "unit": "gathers/s", "ms_per_step": round(1000.0 * t, 2), "dtype": "f32",
+            "iterations": cnt["iterations"], "energy_evaluations": cnt["evals"],
+            "f64_ms_per_step": round(1000.0 * t64, 2),
+            "fp32_vs_fp64_max_deformation_px": round(float(dd.max()) * (size - 1), 6),
+            "fp32_vs_fp64_max_rel_energy": float(de.max()),
+            "energies_finite": bool(np.isfinite(out["energies"]).all())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +206,7 @@ def main():
     ap.add_argument("--classes", action="store_true", help="also time every kernel class in one extra step")
     ap.add_argument("--roofline-only", action="store_true", help="only the energy-pass roofline measurement")
     ap.add_argument("--no-f64", action="store_true", help="skip the FP64 comparison step")
+    ap.add_argument("--no-seismic", action="store_true", help="skip the seismic (x-shift) line")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -256,6 +303,10 @@ def main():
     roofline["copy_ceiling_gbs"] = round(10 * 2 * a.numel() * 4 / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
     del a, b
 
+    seismic = None
+    if not args.no_seismic:
+        seismic = seismic_line(gpu, torch)
+
     cpu = cpu_all = None
     if not args.no_cpu:
         ncores = min(16, len(os.sched_getaffinity(0)))
@@ -274,6 +325,7 @@ def main():
         "roofline": roofline,
         "whole_step": whole_step,
         "fp64": f64,
+        "seismic": seismic,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
         "kernel_classes": classes,

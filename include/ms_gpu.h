@@ -321,6 +321,26 @@ int msg_match_series(const msg_series_params *p, int dtype, const void *template
  * (4^level), the unit of the path's byte model. Any output may be NULL. */
 int msg_series_last_counts(long long *iterations, long long *evals, long long *passes, double *node_iterations,
                            double *node_evals);
+
+/* Several independent series of possibly different frame sizes in one call (no counterpart in the
+ * reference, whose program matches one series per run). Each job is one msg_match_series call's
+ * inputs and outputs (device arrays of `dtype`, energies on the host); each series' results are
+ * bitwise those of msg_match_series on it alone. Series with the same frame size and solve
+ * parameters share contexts: their pairwise solves are batched together and their accumulate-and-
+ * refine chains advance in lockstep, refinement i of every such series in one batched solve; groups
+ * of different sizes run concurrently on their own streams. Not taken here (msg_match_series has
+ * them): reduce_deformations, corr_init, calc_inverse, alt_start_level, pair0_init, raw_templates,
+ * keep_start_transformation, MSG_XSHIFT, per-level records, reuse of earlier pairwise results. */
+typedef struct {
+  const msg_series_params *params;  /* the series' parameters (precision_level: its frame size) */
+  const void *templates;            /* n images, 2^L x 2^L                                     */
+  int n;
+  const void *reference;            /* one image                                               */
+  void *deformations;               /* n fields (output)                                       */
+  double *energies;                 /* n host values (output)                                  */
+  void *mean, *count, *median;       /* one image each (output)                                 */
+} msg_series_job;
+int msg_match_series_batch(int dtype, int njobs, const msg_series_job *jobs);
 /* SeriesMatching::reduceDeformations (matchSeries.h:678-704): from zero, the displacement phi
  * minimising sum_i 1/2 int |psi_i(x + phi(x)) + phi(x)|^2 (ConsistencyEnergyOp and its variation,
  * AmbrosioTortorelli.h:687-800) by H1GradientDescent with filter width sqrt(10 h), nonlinear CG and

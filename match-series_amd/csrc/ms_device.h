@@ -62,8 +62,13 @@ __device__ __forceinline__ double rnd(double v) {
 }
 
 // phi + tau * d as stored in S: FP64 fields in FP64 arithmetic, FP32 fields by one fused multiply-add
-// (the step sizes are powers of two, exact in FP32). The update kernel and every energy pass use this
-// one definition, so an accepted candidate is bitwise the position the update stores.
+// (the step sizes are powers of two, exact in FP32). The update kernel, the FP64-arithmetic energy
+// passes (k_energy and its FP64 re-evaluation of FP32 pairs, k_energy_exact_f32), the NCC,
+// consistency and x-shift passes use this one definition, so a candidate they evaluate is bitwise
+// the position the update stores. The FP32-arithmetic SSD pass (k_energy_f32) forms the quad points
+// of phi + tau d linearly in tau from one interpolation of phi and d; they differ from those of the
+// stored position by FP32 rounding, and a step decision whose margin lies within that pass's error
+// bound is taken again on the re-evaluation at the stored positions.
 template <typename S>
 __device__ __forceinline__ S step_at(S f, double tau, S d);
 template <>

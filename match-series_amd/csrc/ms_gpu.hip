@@ -2451,6 +2451,8 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
   return MSG_OK;
 }
 
+#include "ms_batch.h"
+
 }  // namespace
 
 extern "C" {
@@ -2539,6 +2541,20 @@ int msg_normalize(int dtype, void *img, int n, int nimg, double pct, int min_to_
   else CHK(normalize_impl<float>(img, n, nimg, pct, min_to_zero, strm(stream)));
   HIPCHK(hipStreamSynchronize(strm(stream)));
   return MSG_OK;
+}
+
+int msg_match_series_batch(int dtype, int njobs, const msg_series_job *jobs) {
+  CHK(check_dtype(dtype));
+  if (njobs < 1 || !jobs) return fail(MSG_ERR_ARG, "invalid argument");
+  for (int k = 0; k < njobs; ++k) {
+    const msg_series_job &j = jobs[k];
+    if (!j.params || !j.templates || !j.reference || !j.deformations || !j.energies || !j.mean || !j.count ||
+        !j.median || j.n < 1)
+      return fail(MSG_ERR_ARG, "job " + std::to_string(k) + ": invalid argument");
+    CHK(check_batch_params(j.params));
+    if (j.n > msg::ST_MAXF) return fail(MSG_ERR_ARG, "job " + std::to_string(k) + ": at most 8192 templates");
+  }
+  return dtype == MSG_F64 ? match_series_batch_impl<double>(njobs, jobs) : match_series_batch_impl<float>(njobs, jobs);
 }
 
 int msg_series_last_counts(long long *iterations, long long *evals, long long *passes, double *node_iterations,

@@ -171,7 +171,11 @@ msg_series_params seriesParams(const ParameterParser &p) {
   return s;
 }
 
-int fieldType(const ParameterParser &p) { return p.getIntOrDefault("gpuFieldPrecision", 32) == 64 ? MSG_F64 : MSG_F32; }
+// Field storage: FP32 by default; matchSeismicSeries defaults to FP64 (its row kernels compute in
+// FP64 whatever the storage, so FP32 fields save little there and move its short solves).
+int fieldType(const ParameterParser &p) {
+  return p.getIntOrDefault("gpuFieldPrecision", kSeismic ? 64 : 32) == 64 ? MSG_F64 : MSG_F32;
+}
 
 // An input image as RegistrationMultilevelDescentInterface::initMultilevelArrayFromImage prepares it
 // (registration.h:330-399): resampled onto the grid of level max(L, log2 width) with resizeInput, cut

@@ -50,6 +50,12 @@ class SeriesParams(ctypes.Structure):
     ]
 
 
+class SeriesJob(ctypes.Structure):
+    _fields_ = [("params", ctypes.POINTER(SeriesParams)), ("templates", ctypes.c_void_p), ("n", ctypes.c_int),
+                ("reference", ctypes.c_void_p), ("deformations", ctypes.c_void_p), ("energies", _dp),
+                ("mean", ctypes.c_void_p), ("count", ctypes.c_void_p), ("median", ctypes.c_void_p)]
+
+
 class SeriesLevels(ctypes.Structure):
     _fields_ = [("fields", _dp), ("iters", ctypes.POINTER(IterTrace)), ("start", _dp),
                 ("iterations", ctypes.POINTER(ctypes.c_int))]
@@ -101,6 +107,7 @@ SIGNATURES = {
                           ctypes.c_int),
     "msg_xshift_gradient": ([ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                             ctypes.c_double, _vp], ctypes.c_int),
+    "msg_match_series_batch": ([ctypes.c_int, ctypes.c_int, ctypes.POINTER(SeriesJob)], ctypes.c_int),
     "msg_series_last_counts": ([ctypes.POINTER(ctypes.c_longlong)] * 3 + [_dp, _dp], ctypes.c_int),
     "msg_set_default_mode": ([ctypes.c_int], ctypes.c_int),
     "msg_set_default_exact_decisions": ([ctypes.c_int], ctypes.c_int),
@@ -643,6 +650,36 @@ def match_series(templates, reference, params, pairwise_in=None, want_pairwise=F
             off += 2 * m * m
         out["levels"] = levels
     return out
+
+
+def match_series_batch(series):
+    """Several series in one call (msg_match_series_batch). series: list of (templates (n, N, N),
+    reference (N, N), SeriesParams), device tensors of one dtype. Returns one dict per series as
+    match_series does (deformations, energies, mean, count, median)."""
+    torch = _torch()
+    code = None
+    jobs = (SeriesJob * len(series))()
+    outs, keep = [], []
+    for k, (templates, reference, params) in enumerate(series):
+        N = _square(templates, "templates", (None,))
+        c = _arg(templates, "templates")
+        code = c if code is None else code
+        if c != code:
+            raise ValueError("all series must have the same dtype")
+        _arg(reference, "reference", code, (N, N))
+        _on_device(templates=templates, reference=reference)
+        n = templates.shape[0]
+        o = dict(deformations=torch.zeros((n, 2, N, N), dtype=templates.dtype, device=templates.device),
+                 energies=np.zeros(n), mean=torch.zeros((N, N), dtype=templates.dtype, device=templates.device))
+        o["count"] = torch.zeros_like(o["mean"])
+        o["median"] = torch.zeros_like(o["mean"])
+        outs.append(o)
+        keep.append(params)
+        jobs[k] = SeriesJob(ctypes.pointer(params), _ptr(templates), n, _ptr(reference), _ptr(o["deformations"]),
+                            _dptr(o["energies"]), _ptr(o["mean"]), _ptr(o["count"]), _ptr(o["median"]))
+    _ready()
+    check(lib().msg_match_series_batch(code, len(series), jobs))
+    return outs
 
 
 def series_last_counts():
