@@ -32,11 +32,19 @@ solves on (its default dyadicDepth for 2048 samples), stage 1 at levels 7-10, re
 most 100 iterations per level, lambda 5, yRegFactor 1, FP32 fields. The reference's
 full-resolution step (Cholesky H1 metric) is not part of this build.
 
+Batched uneven series (msg_match_series_batch): 16 independent series, frame sizes drawn from
+{256, 512, 1024, 2048}^2, 10-40 templates each, lattice spacing 10-20 px (the STEM model, frames made
+on the GPU by msamd.warmstart.stem_frames_gpu), stage 1 at levels L-3..L, refinement L-1..L, 100
+iterations per level, lambda 10, FP32 fields; the Armijo candidates per energy pass for each frame
+size from msamd/tuning.json (msamd.autotune). One call runs all 16: series of one size share contexts
+(pairwise solves batched across series, chains in lockstep), sizes run concurrently.
+
 CPU baseline: the reference's own level solve (oracle/_ref/ref_harness, compiled from the reference
 sources) timed for a few iterations at 1024^2 on this host, extrapolated to the same schedule (cost
 proportional to N^2).
 """
 import argparse
+import collections
 import json
 import os
 import subprocess
@@ -189,6 +197,52 @@ def seismic_line(gpu, torch, count=24, size=1024, its=100):
             "energies_finite": bool(np.isfinite(out["energies"]).all())}
 
 
+def batch_line(gpu, torch, sequential=False, nseries=16, its=100):
+    """One msg_match_series_batch call over nseries synthetic series of uneven sizes and lengths."""
+    from msamd import autotune, synth
+    from msamd import warmstart as W
+    rng = np.random.default_rng(synth.BASE_SEED + 3)
+    sizes = rng.choice([256, 512, 1024, 2048], nseries)
+    counts = rng.integers(10, 41, nseries)
+    spacing = rng.uniform(10.0, 20.0, nseries)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(synth.BASE_SEED + 3)
+    jobs = []
+    for k in range(nseries):
+        N = int(sizes[k])
+        L = int(np.log2(N))
+        fr = W.stem_frames_gpu(N, int(counts[k]) + 1, gen, spacing=float(spacing[k]), sigma=2.5 * spacing[k] / 12.0,
+                               jitter_step=0.3)
+        spec, _ = autotune.tuned(N)  # the batch API sizes its own pairwise contexts
+        p = series_params(gpu, (L - 3, L), (L - 1, L), its, spec=spec)
+        jobs.append((fr[1:].contiguous(), fr[0].contiguous(), p))
+    torch.cuda.synchronize()
+    gpu.match_series_batch(jobs)  # warm-up: contexts and graphs
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    outs = gpu.match_series_batch(jobs)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    nt = int(counts.sum())
+    line = {"workload": f"{nseries} series, sizes {sorted(collections.Counter(int(x) for x in sizes).items())}, "
+                        f"{nt} templates in all (10-40 per series), levels L-3..L, refine L-1..L, {its} it/level, "
+                        f"lambda 10, SSD, FP32 fields, one msg_match_series_batch call",
+            "value": round(nt / t, 3), "unit": "frames/s", "ms_per_step": round(1000.0 * t, 2),
+            "pixels_per_s": round(float(sum(int(counts[k]) * int(sizes[k]) ** 2 for k in range(nseries))) / t, 1),
+            "tuning": {str(N): autotune.tuned(N) for N in (256, 512, 1024, 2048)},
+            "energies_finite": bool(all(np.isfinite(o["energies"]).all() for o in outs))}
+    if sequential:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for tpl, ref, p in jobs:
+            gpu.match_series(tpl, ref, p)
+        torch.cuda.synchronize()
+        line["sequential_ms"] = round(1000.0 * (time.perf_counter() - t0), 2)
+    del jobs, outs
+    gpu.release_cache()
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,7 +260,13 @@ def main():
     ap.add_argument("--classes", action="store_true", help="also time every kernel class in one extra step")
     ap.add_argument("--roofline-only", action="store_true", help="only the energy-pass roofline measurement")
     ap.add_argument("--no-f64", action="store_true", help="skip the FP64 comparison step")
+    ap.add_argument("--chain-shape", action="store_true",
+                    help="only the chain's shape: one pair of the series at levels 9 and 10, 100 iterations each, "
+                         "three times (for rocprofv3)")
     ap.add_argument("--no-seismic", action="store_true", help="skip the seismic (x-shift) line")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-series line")
+    ap.add_argument("--batch-vs-sequential", action="store_true",
+                    help="also time the batched series one msg_match_series call at a time")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -224,6 +284,25 @@ def main():
     levels, refine = (L - 3, L), (L - 1, L)
     dt = torch.float32 if args.precision == "f32" else torch.float64
     frames = synth.stem_series(N, F, workload=1, spacing=16.0, sigma=3.0, jitter_step=0.4)
+    if args.chain_shape:
+        ctx = gpu.Context(L, 1, gpu.F32)
+        try:
+            ctx.set_reference(0, unit_range(frames[0]))
+            ctx.set_template(0, unit_range(frames[1]))
+            res = []
+            for rep_ in range(3):
+                ctx.set_deformation_zero(0)
+                for lv in (L - 1, L):
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    e, it, _ = ctx.solve_level(lv, 10.0, args.iters, -1.0)
+                    torch.cuda.synchronize()
+                    res.append({"repetition": rep_, "level": lv, "iterations": int(it[0]),
+                                "us_per_iteration": round(1e6 * (time.perf_counter() - t0) / max(int(it[0]), 1), 1)})
+        finally:
+            ctx.close()
+        print(json.dumps({"chain_shape": res}), flush=True)
+        return
     if args.roofline_only:
         r = energy_roofline(gpu, frames)
         # calibration for HBM byte counters: plain copies of 256 MiB (read 256 MiB, write 256 MiB each)
@@ -306,6 +385,9 @@ def main():
     seismic = None
     if not args.no_seismic:
         seismic = seismic_line(gpu, torch)
+    batch = None
+    if not args.no_batch:
+        batch = batch_line(gpu, torch, args.batch_vs_sequential)
 
     cpu = cpu_all = None
     if not args.no_cpu:
@@ -326,6 +408,7 @@ def main():
         "whole_step": whole_step,
         "fp64": f64,
         "seismic": seismic,
+        "batched_series": batch,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
         "kernel_classes": classes,

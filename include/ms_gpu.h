@@ -366,7 +366,7 @@ int msg_release_cache(void);
  * and its biases, in network order, then the head's [2][C] weights and 2 biases.
  * ---------------------------------------------------------------------------------------------- */
 typedef struct msg_unet msg_unet;
-/* He-normal weights from `seed`, zero biases. */
+/* He-normal weights from `seed`, zero biases. scales 1..6; base_channels a multiple of 32, at most 256. */
 int msg_unet_create(msg_unet **net, int scales, int base_channels, unsigned long long seed);
 int msg_unet_destroy(msg_unet *net);
 int msg_unet_param_count(msg_unet *net, long long *count);

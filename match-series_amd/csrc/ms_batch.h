@@ -6,9 +6,11 @@
 // Scheduling. Series are grouped by frame size and solve parameters (one group = one context shape
 // and one set of level-solve parameters). Within a group:
 //   * the pairwise solves of all its series are batched together, across series, in contexts of up
-//     to `batch` pairs (every pair's arithmetic is independent of the batch it sits in);
-//   * the accumulate-and-refine chains advance in lockstep: refinement i of every series that has a
-//     template i is one batched solve, so a group of k series pays one chain's launches for k chains;
+//     to `batch` pairs (every pair's arithmetic is independent of the batch it sits in), in the order
+//     of their template index, on a second host thread;
+//   * the accumulate-and-refine chains advance in lockstep, refinement i of every series that has a
+//     template i in one batched solve (a group of k series pays one chain's launches for k chains),
+//     each step as soon as the pairwise solves up to template i are done;
 //   * the series' reconstructions follow.
 // Groups run concurrently, each on its own host thread and streams, so groups of different frame
 // sizes share the GPU (a latency-bound chain of one size beside the batches of another).
@@ -17,8 +19,8 @@
 // A persistent host thread per concurrent group (thread_local scratch buffers stay with their
 // thread, so a worker is reused rather than recreated for every call).
 constexpr int BATCH_WORKERS = 8;
-Worker *batch_worker(int k) {
-  static Worker *w[BATCH_WORKERS] = {};
+Worker *batch_worker(int k) {  // k < 2 * BATCH_WORKERS: a group's chain thread, then its pairwise thread
+  static Worker *w[2 * BATCH_WORKERS] = {};
   static std::mutex mu;
   std::lock_guard<std::mutex> g(mu);
   if (!w[k]) w[k] = new Worker;
@@ -66,7 +68,7 @@ int batch_pairs_per_context(const msg_series_params *p, int total) {
 }
 
 template <typename S>
-int batch_group(const std::vector<const msg_series_job *> &jobs) {
+int batch_group(const std::vector<const msg_series_job *> &jobs, int slot) {
   const msg_series_params *p = jobs[0]->params;
   const int L = p->precision_level, N = 1 << L;
   const size_t nn = (size_t)N * N;
@@ -104,18 +106,21 @@ int batch_group(const std::vector<const msg_series_job *> &jobs) {
     return MSG_OK;
   };
 
-  // --- pairwise solves of every series, batched across series ---
+  // --- pairwise solves of every series, batched across series, in template order ---
   std::vector<DevBuf> pw(J);
-  std::vector<std::pair<int, int>> pairs;  // (series, template)
+  std::vector<std::pair<int, int>> pairs;  // (series, template), ordered by template
+  int maxn = 0;
   for (int j = 0; j < J; ++j) {
     HIPCHK(hipMalloc(&pw[j].p, sizeof(S) * 2 * nn * jobs[j]->n));
-    for (int i = 0; i < jobs[j]->n; ++i) pairs.push_back({j, i});
     for (int i = 0; i < jobs[j]->n; ++i) jobs[j]->energies[i] = NAN;
+    maxn = std::max(maxn, jobs[j]->n);
   }
+  for (int i = 0; i < maxn; ++i)
+    for (int j = 0; j < J; ++j)
+      if (jobs[j]->n > i) pairs.push_back({j, i});
   const int total = (int)pairs.size();
   const int b = batch_pairs_per_context(p, total);
-  for (int k0 = 0; k0 < total; k0 += b) {
-    const int nb = std::min(b, total - k0);
+  auto pairwise = [&](int k0, int nb) -> int {
     msg_ctx *pc = nullptr;
     CHK(ctx_acquire(L, nb, dt, &pc));
     CtxGuard gpc{pc};
@@ -134,25 +139,64 @@ int batch_group(const std::vector<const msg_series_job *> &jobs) {
     CHK(solve_impl<S>(pc, p->start_level, p->stop_level, &sp, e.data()));
     for (int k = 0; k < nb; ++k) {
       const int j = pairs[k0 + k].first, i = pairs[k0 + k].second;
-      jobs[j]->energies[i] = e[k];
+      if (i == 0 || p->dont_accumulate) jobs[j]->energies[i] = e[k];  // refinements replace the others
       HIPCHK(hipMemcpyAsync((S *)pw[j].p + (size_t)i * 2 * nn, (const S *)pc->phi[L] + (size_t)k * 2 * nn,
                             sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, cs));
     }
     HIPCHK(hipStreamSynchronize(cs));
-  }
+    return MSG_OK;
+  };
+  Progress pr;  // pr.done: pairs finished, a prefix of `pairs`
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  batch_worker(BATCH_WORKERS + slot)->post([&, dev] {
+    int rc = hipSetDevice(dev) == hipSuccess ? MSG_OK : fail(MSG_ERR_HIP, "hipSetDevice failed");
+    for (int k0 = 0; k0 < total && rc == MSG_OK; k0 += b) {
+      {
+        std::lock_guard<std::mutex> g(pr.mu);
+        if (pr.abort) break;
+      }
+      const int nb = std::min(b, total - k0);
+      rc = pairwise(k0, nb);
+      std::lock_guard<std::mutex> g(pr.mu);
+      if (rc == MSG_OK) pr.done = k0 + nb;
+      pr.cv.notify_all();
+    }
+    std::lock_guard<std::mutex> g(pr.mu);
+    if (rc != MSG_OK) {
+      pr.rc = rc;
+      pr.err = g_err;
+    }
+    pr.finished = true;
+    pr.cv.notify_all();
+  });
+  // Waits until the first `count` pairs are solved; false after a failure.
+  auto wait_pairs = [&](int count) {
+    std::unique_lock<std::mutex> l(pr.mu);
+    pr.cv.wait(l, [&] { return pr.done >= count || pr.rc != MSG_OK || pr.finished; });
+    return pr.done >= count;
+  };
+  // Stops the pairwise thread (after an error) and waits for it to let go of the buffers.
+  auto join = [&](int code) {
+    std::unique_lock<std::mutex> l(pr.mu);
+    if (code != MSG_OK) pr.abort = true;
+    pr.cv.wait(l, [&] { return pr.finished; });
+    return code;
+  };
 
   // --- the chains in lockstep: refinement i of every series with a template i in one solve ---
-  if (p->dont_accumulate) {
-    for (int j = 0; j < J; ++j)
-      HIPCHK(hipMemcpyAsync(jobs[j]->deformations, pw[j].p, sizeof(S) * 2 * nn * jobs[j]->n,
-                            hipMemcpyDeviceToDevice, s));
-  } else {
+  const int chain_rc = [&]() -> int {
+    if (p->dont_accumulate) {
+      if (!wait_pairs(total)) return pr.rc != MSG_OK ? pr.rc : MSG_ERR_STATE;
+      for (int j = 0; j < J; ++j)
+        HIPCHK(hipMemcpyAsync(jobs[j]->deformations, pw[j].p, sizeof(S) * 2 * nn * jobs[j]->n,
+                              hipMemcpyDeviceToDevice, s));
+      return MSG_OK;
+    }
     std::vector<DevBuf> acc(J), refn(J);
     DevBuf prev;
     HIPCHK(hipMalloc(&prev.p, sizeof(S) * 2 * nn));
-    int maxn = 0;
     for (int j = 0; j < J; ++j) {
-      maxn = std::max(maxn, jobs[j]->n);
       HIPCHK(hipMalloc(&acc[j].p, sizeof(S) * 2 * nn));
       HIPCHK(hipMemsetAsync(acc[j].p, 0, sizeof(S) * 2 * nn, s));
       // the reference as the chain's context prepares it
@@ -160,12 +204,16 @@ int batch_group(const std::vector<const msg_series_job *> &jobs) {
       HIPCHK(hipMemcpyAsync(refn[j].p, jobs[j]->reference, nn * sizeof(S), hipMemcpyDeviceToDevice, s));
       if (!p->dont_normalize)
         CHK(normalize_impl<S>(refn[j].p, N, 1, p->saturation_pct, p->normalize_min_to_zero, s));
-      if (p->pre_smooth_sigma > 0) CHK(h1_impl<S>(refn[j].p, refn[j].p, N, 1, p->pre_smooth_sigma / (double)(N - 1), s));
+      if (p->pre_smooth_sigma > 0)
+        CHK(h1_impl<S>(refn[j].p, refn[j].p, N, 1, p->pre_smooth_sigma / (double)(N - 1), s));
     }
+    int upto = 0;  // pairs with template index <= i (a prefix of `pairs`)
     for (int i = 0; i < maxn; ++i) {
       std::vector<int> act;
       for (int j = 0; j < J; ++j)
         if (jobs[j]->n > i) act.push_back(j);
+      upto += (int)act.size();
+      if (!wait_pairs(upto)) return pr.rc != MSG_OK ? pr.rc : MSG_ERR_STATE;
       for (int j : act) {
         HIPCHK(hipMemcpyAsync(prev.p, acc[j].p, sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, s));
         CHK(compose_impl<S>((const S *)pw[j].p + (size_t)i * 2 * nn, prev.p, acc[j].p, N, s));
@@ -188,7 +236,8 @@ int batch_group(const std::vector<const msg_series_job *> &jobs) {
         // the reference is already prepared; only the templates are normalised here
         if (!p->dont_normalize)
           CHK(normalize_impl<S>(rc->T[L], N, nb, p->saturation_pct, p->normalize_min_to_zero, cs));
-        if (p->pre_smooth_sigma > 0) CHK(h1_impl<S>(rc->T[L], rc->T[L], N, nb, p->pre_smooth_sigma / (double)(N - 1), cs));
+        if (p->pre_smooth_sigma > 0)
+          CHK(h1_impl<S>(rc->T[L], rc->T[L], N, nb, p->pre_smooth_sigma / (double)(N - 1), cs));
         CHK(rebuild_pyramid<S>(rc, rc->R, -1, 1));
         CHK(rebuild_pyramid<S>(rc, rc->T, -1, 1));
         CHK(rebuild_pyramid<S>(rc, rc->phi, -1, 2));
@@ -206,7 +255,11 @@ int batch_group(const std::vector<const msg_series_job *> &jobs) {
         HIPCHK(hipMemcpyAsync((S *)jobs[j]->deformations + (size_t)i * 2 * nn, i > 0 ? acc[j].p : pw[j].p,
                               sizeof(S) * 2 * nn, hipMemcpyDeviceToDevice, s));
     }
-  }
+    return MSG_OK;
+  }();
+  join(chain_rc);
+  if (chain_rc != MSG_OK) return chain_rc == pr.rc && !pr.err.empty() ? fail(chain_rc, pr.err) : chain_rc;
+  if (pr.rc != MSG_OK) return fail(pr.rc, pr.err);
 
   // --- reconstructions (averageSeries, matchSeries.h:353-451) ---
   for (int j = 0; j < J; ++j) {
@@ -263,9 +316,10 @@ int match_series_batch_impl(int njobs, const msg_series_job *jobs) {
   HIPCHK(hipGetDevice(&dev));
   for (size_t g = 0; g < groups.size(); ++g) {
     const std::vector<const msg_series_job *> *grp = &groups[g];
-    batch_worker((int)(g % BATCH_WORKERS))->post([grp, dev, &done] {
+    const int slot = (int)(g % BATCH_WORKERS);
+    batch_worker(slot)->post([grp, dev, slot, &done] {
       int rc = hipSetDevice(dev) == hipSuccess ? MSG_OK : fail(MSG_ERR_HIP, "hipSetDevice failed");
-      if (rc == MSG_OK) rc = batch_group<S>(*grp);
+      if (rc == MSG_OK) rc = batch_group<S>(*grp, slot);
       std::lock_guard<std::mutex> l(done.mu);
       if (rc != MSG_OK && done.rc == MSG_OK) {
         done.rc = rc;

@@ -228,14 +228,20 @@ __global__ void k_sum_chunks(const float *__restrict__ part, int nchunks, long n
 }
 
 // dz = bf16(dy * [y > 0]) and per-block partial sums of dz (in FP32) per channel for the bias gradient.
-// One thread per (pixel, channel); blocks of 256 cover 256 / C pixels.
+// With C dividing 256 a block of 256 threads covers RELU_CHUNKS x 256 consecutive values and every
+// thread keeps its channel; with C a multiple of 256 a block covers 256 channels of one pixel.
+constexpr int RELU_CHUNKS = 16;
 __global__ void k_relu_grad(const float *__restrict__ dy, const bf16 *__restrict__ y, bf16 *__restrict__ dz, long n,
                             int C, float *__restrict__ bpart) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int chunks = C <= 256 ? RELU_CHUNKS : 1;
   float v = 0.f;
-  if (i < n) {
-    v = bf2f(y[i]) > 0.f ? dy[i] : 0.f;
-    dz[i] = f2bf(v);
+  for (int k = 0; k < chunks; ++k) {
+    const long i = ((long)blockIdx.x * chunks + k) * 256 + threadIdx.x;
+    if (i < n) {
+      const float u = bf2f(y[i]) > 0.f ? dy[i] : 0.f;
+      dz[i] = f2bf(u);
+      v += u;
+    }
   }
   __shared__ float s[256];
   s[threadIdx.x] = v;
@@ -252,18 +258,28 @@ __global__ void k_relu_grad(const float *__restrict__ dy, const bf16 *__restrict
   }
 }
 
-// db[c] = sum over blocks of bpart (fixed order)
-__global__ void k_bias_sum(const float *__restrict__ bpart, int nblk, int C, float *__restrict__ db) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= C) return;
+// out[c] = sum over rows r of part[r * stride + off(c)], one workgroup per column c: each thread sums
+// the rows r = t, t + 256, ... in order, then a fixed tree over the 256 threads (deterministic).
+// Bias gradients (rows: the blocks of k_relu_grad; C > 256: a pixel's channels span C / 256 blocks)
+// and the head's partials use it.
+__global__ void __launch_bounds__(256)
+k_col_sum(const float *__restrict__ part, int nrows, long stride, int C, float *__restrict__ out) {
+  const int c = blockIdx.x;
   float a = 0.f;
-  if (C <= 256) {
-    for (int b = 0; b < nblk; ++b) a += bpart[(long)b * C + c];
+  if (C <= 256 || stride != 256) {
+    for (int r = threadIdx.x; r < nrows; r += 256) a += part[(long)r * stride + c];
   } else {
     const int per = C / 256;  // blocks per pixel
-    for (int b = c / 256; b < nblk; b += per) a += bpart[(long)b * 256 + (c % 256)];
+    for (int r = c / 256 + per * (int)threadIdx.x; r < nrows; r += per * 256) a += part[(long)r * 256 + (c % 256)];
   }
-  db[c] = a;
+  __shared__ float s[256];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = s[0];
 }
 
 // 2x2 average pooling, NHWC bf16: y[b][yy][x][c] = mean of x[b][2yy..][2x..][c]
@@ -377,33 +393,56 @@ __global__ void k_mse(const float *__restrict__ out, const float *__restrict__ t
   if (threadIdx.x == 0) lpart[blockIdx.x] = s[0];
 }
 
-// Head backward: dx[p][c] += sum_o wh[o][c] dout[o][p]; per-block partials of dwh (2C) and dbh (2).
+// Head backward over 256 pixels per workgroup: dx[p][c] += sum_o wh[o][c] dout[o][p], and per-block
+// partials of dwh[o][c] = sum_p dout[o][p] x[p][c] (2C) and dbh[o] = sum_p dout[o][p] (2). Thread t
+// takes channel t % C of the pixels t / C, t / C + 256 / C, ... (C divides 256), so the loads and
+// stores of a wave are consecutive channels; the 256 / C partial sums of a channel are added in a
+// fixed order.
+constexpr int HEAD_CMAX = 256;
 __global__ void __launch_bounds__(256)
 k_head_grad(const bf16 *__restrict__ x, const float *__restrict__ wh, const float *__restrict__ dout,
             float *__restrict__ dx, int B, int H, int W, int C, float *__restrict__ part) {
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long hw = (long)H * W, n = (long)B * hw;
-  __shared__ float s[256];
-  float d0 = 0.f, d1 = 0.f;
-  const bool ok = p < n;
-  if (ok) {
-    const long b = p / hw, q = p % hw;
-    d0 = dout[(b * 2) * hw + q];
-    d1 = dout[(b * 2 + 1) * hw + q];
+  const long p0 = (long)blockIdx.x * 256;
+  const int t = threadIdx.x, c = t % C, G = 256 / C, g = t / C;
+  const float w0 = wh[c], w1 = wh[C + c];
+  float a0 = 0.f, a1 = 0.f, s0 = 0.f, s1 = 0.f;
+  for (int q = g; q < 256; q += G) {
+    const long p = p0 + q;
+    if (p >= n) break;
+    const long b = p / hw, r = p % hw;
+    const float d0 = dout[(b * 2) * hw + r], d1 = dout[(b * 2 + 1) * hw + r];
+    const float v = bf2f(x[p * C + c]);
+    dx[p * C + c] += w0 * d0 + w1 * d1;
+    a0 += d0 * v;
+    a1 += d1 * v;
+    s0 += d0;
+    s1 += d1;
   }
-  for (int c = 0; c < C + 1; ++c) {
-    const float v = (c < C) ? (ok ? bf2f(x[p * C + c]) : 0.f) : 1.f;
-    if (ok && c < C) dx[p * C + c] += wh[c] * d0 + wh[C + c] * d1;
-    for (int o = 0; o < 2; ++o) {
-      s[threadIdx.x] = (o == 0 ? d0 : d1) * v;
-      __syncthreads();
-      for (int k = 128; k > 0; k >>= 1) {
-        if ((int)threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
-        __syncthreads();
-      }
-      if (threadIdx.x == 0) part[(long)blockIdx.x * (2 * C + 2) + (c < C ? o * C + c : 2 * C + o)] = s[0];
-      __syncthreads();
+  __shared__ float r0[256], r1[256], q0[256], q1[256];
+  r0[t] = a0;
+  r1[t] = a1;
+  q0[t] = s0;
+  q1[t] = s1;
+  __syncthreads();
+  float *dst = part + (long)blockIdx.x * (2 * C + 2);
+  if (t < C) {
+    float u0 = 0.f, u1 = 0.f;
+    for (int k = 0; k < G; ++k) {
+      u0 += r0[k * C + t];
+      u1 += r1[k * C + t];
     }
+    dst[t] = u0;
+    dst[C + t] = u1;
+  }
+  if (t == 0) {  // every pixel once: the threads of channel 0
+    float u0 = 0.f, u1 = 0.f;
+    for (int k = 0; k < G; ++k) {
+      u0 += q0[k * C];
+      u1 += q1[k * C];
+    }
+    dst[2 * C] = u0;
+    dst[2 * C + 1] = u1;
   }
 }
 
@@ -677,8 +716,8 @@ int backward(msg_unet *u, const float *target, double *loss) {
                      u->w.as<float>() + u->whoff, u->dout.as<float>(), u->acts[u->head_in].dy.as<float>(), B, H, W,
                      C0, u->hpart.as<float>());
   UHIP(hipGetLastError());
-  hipLaunchKernelGGL(k_sum_chunks, dim3(cdivu(2 * C0 + 2, 256)), dim3(256), 0, u->s, u->hpart.as<float>(), nhb,
-                     (long)(2 * C0 + 2), g + u->whoff);  // dwh then dbh are contiguous in the flat vector
+  hipLaunchKernelGGL(k_col_sum, dim3(2 * C0 + 2), dim3(256), 0, u->s, u->hpart.as<float>(), nhb,
+                     (long)(2 * C0 + 2), 2 * C0 + 2, g + u->whoff);  // dwh then dbh are contiguous in the flat vector
   UHIP(hipGetLastError());
   size_t pi = u->pools.size();
   for (int k = (int)u->convs.size() - 1; k >= 0; --k) {
@@ -687,12 +726,12 @@ int backward(msg_unet *u, const float *target, double *loss) {
     const int Hs = H >> o.scale, Ws = W >> o.scale;
     const long px = (long)B * Hs * Ws;
     const long n = px * c.cout;
-    const int nb = cdivu(n, 256);
+    const int nb = cdivu(n, 256L * (c.cout <= 256 ? RELU_CHUNKS : 1));
     hipLaunchKernelGGL(k_relu_grad, dim3(nb), dim3(256), 0, u->s, o.dy.as<float>(), o.y.as<bf16>(), u->dz.as<bf16>(),
                        n, c.cout, u->bpart.as<float>());
     UHIP(hipGetLastError());
-    hipLaunchKernelGGL(k_bias_sum, dim3(cdivu(c.cout, 64)), dim3(64), 0, u->s, u->bpart.as<float>(), nb, c.cout,
-                       g + c.boff);
+    hipLaunchKernelGGL(k_col_sum, dim3(c.cout), dim3(256), 0, u->s, u->bpart.as<float>(), nb,
+                       (long)(c.cout <= 256 ? c.cout : 256), c.cout, g + c.boff);
     UHIP(hipGetLastError());
     const ConvIn in = conv_input(u, c);
     const int cin = c.cin1 + c.cin2;
@@ -751,8 +790,11 @@ int backward(msg_unet *u, const float *target, double *loss) {
 extern "C" {
 
 int msg_unet_create(msg_unet **net, int scales, int base_channels, unsigned long long seed) {
-  if (!net || scales < 1 || scales > 6 || base_channels < 32 || base_channels % 32 != 0)
-    return ufail(MSG_ERR_ARG, "msg_unet_create: scales in 1..6, base_channels a positive multiple of 32");
+  // powers of two: every layer's channel count then divides 256 or is a multiple of it, which the
+  // per-channel reductions assume
+  if (!net || scales < 1 || scales > 6 || base_channels < 32 || base_channels > HEAD_CMAX ||
+      (base_channels & (base_channels - 1)) != 0)
+    return ufail(MSG_ERR_ARG, "msg_unet_create: scales in 1..6, base_channels 32, 64, 128 or 256");
   msg_unet *u = new msg_unet();
   u->scales = scales;
   u->base = base_channels;
