@@ -39,6 +39,12 @@ iterations per level, lambda 10, FP32 fields; the Armijo candidates per energy p
 size from msamd/tuning.json (msamd.autotune). One call runs all 16: series of one size share contexts
 (pairwise solves batched across series, chains in lockstep), sizes run concurrently.
 
+Warm-start network line: the BF16 U-Net (4 scales, 32 base channels, 1.95 M parameters) of the
+learned warm start, training steps (forward, loss, backward, Adam) on 8 pairs of 256^2 and forward
+passes, timed over 20 each; FLOPs counted from the architecture (2 per multiply-add of the 3x3
+convolutions; a training step is the forward, the data gradient of every layer but the first and the
+weight gradient), against MI355X's dense BF16 peak of 2.5 PFLOP/s. `--unet-only` runs just this.
+
 CPU baseline: the reference's own level solve (oracle/_ref/ref_harness, compiled from the reference
 sources) timed for a few iterations at 1024^2 on this host, extrapolated to the same schedule (cost
 proportional to N^2).
@@ -243,6 +249,44 @@ def batch_line(gpu, torch, sequential=False, nseries=16, its=100):
     return line
 
 
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+
+
+def unet_line(gpu, torch, B=8, H=256, steps=20):
+    """Training-step and forward throughput of the warm-start network."""
+    from msamd import warmstart as W
+    net = gpu.UNet(4, 32, seed=1)
+    try:
+        g = torch.Generator(device="cuda")
+        g.manual_seed(5)
+        x = torch.rand((B, 2, H, H), device="cuda", generator=g)
+        t = torch.randn((B, 2, H, H), device="cuda", generator=g)
+        net.train_step(x, t, 1e-3)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            net.train_step(x, t, 1e-3)
+        torch.cuda.synchronize()
+        ms_train = 1000.0 * (time.perf_counter() - t0) / steps
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            net.forward(x)
+        torch.cuda.synchronize()
+        ms_fwd = 1000.0 * (time.perf_counter() - t0) / steps
+        params = net.nparams
+    finally:
+        net.close()
+    macs, first = W.unet_macs(4, 32, B, H, H)
+    train_flops = 2.0 * (3 * macs - first)
+    return {"workload": f"BF16 U-Net, 4 scales, 32 base channels, {params} parameters, batch {B} x 2 x {H}^2",
+            "train_ms_per_step": round(ms_train, 3), "forward_ms": round(ms_fwd, 3),
+            "train_tflops": round(train_flops / (ms_train * 1e-3) / 1e12, 1),
+            "forward_tflops": round(2.0 * macs / (ms_fwd * 1e-3) / 1e12, 1),
+            "peak_tflops": BF16_PEAK_TFLOPS,
+            "train_frac": round(train_flops / (ms_train * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),
+            "forward_frac": round(2.0 * macs / (ms_fwd * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -265,6 +309,8 @@ def main():
                          "three times (for rocprofv3)")
     ap.add_argument("--no-seismic", action="store_true", help="skip the seismic (x-shift) line")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-series line")
+    ap.add_argument("--no-unet", action="store_true", help="skip the warm-start network line")
+    ap.add_argument("--unet-only", action="store_true", help="only the warm-start network line (for rocprofv3)")
     ap.add_argument("--batch-vs-sequential", action="store_true",
                     help="also time the batched series one msg_match_series call at a time")
     args = ap.parse_args()
@@ -284,6 +330,9 @@ def main():
     levels, refine = (L - 3, L), (L - 1, L)
     dt = torch.float32 if args.precision == "f32" else torch.float64
     frames = synth.stem_series(N, F, workload=1, spacing=16.0, sigma=3.0, jitter_step=0.4)
+    if args.unet_only:
+        print(json.dumps({"unet": unet_line(gpu, torch)}), flush=True)
+        return
     if args.chain_shape:
         ctx = gpu.Context(L, 1, gpu.F32)
         try:
@@ -388,6 +437,9 @@ def main():
     batch = None
     if not args.no_batch:
         batch = batch_line(gpu, torch, args.batch_vs_sequential)
+    unet = None
+    if not args.no_unet:
+        unet = unet_line(gpu, torch)
 
     cpu = cpu_all = None
     if not args.no_cpu:
@@ -409,6 +461,7 @@ def main():
         "fp64": f64,
         "seismic": seismic,
         "batched_series": batch,
+        "warm_start_network": unet,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
         "kernel_classes": classes,

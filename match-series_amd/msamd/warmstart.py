@@ -24,6 +24,25 @@ import numpy as np
 from . import gpu
 
 
+def unet_macs(scales, base, B, H, W):
+    """Multiply-adds of one forward pass of the network (its 3x3 convolutions; the 1x1 head, pooling
+    and the elementwise work are left out): per scale s two convolutions at (H >> s) x (W >> s), the
+    encoder's from 16 (the padded input) or base * 2^(s-1) channels to base * 2^s, the decoder's from
+    the upsampled deeper tensor plus the skip tensor."""
+    total, first = 0, None
+    ch = [base << s for s in range(scales)]
+    for s in range(scales):
+        px = B * (H >> s) * (W >> s)
+        cin = 16 if s == 0 else ch[s - 1]
+        m1 = px * 9 * cin * ch[s]
+        first = m1 if s == 0 else first
+        total += m1 + px * 9 * ch[s] * ch[s]
+    for s in range(scales - 2, -1, -1):
+        px = B * (H >> s) * (W >> s)
+        total += px * 9 * (ch[s + 1] + ch[s]) * ch[s] + px * 9 * ch[s] * ch[s]
+    return total, first
+
+
 def _torch():
     import torch
     return torch
