@@ -81,76 +81,98 @@ struct ConvIn {
 // ------------------------------------------------------------------------------------------------
 // 3x3 convolution, forward (or data gradient with flip = 1 and the transposed weights):
 // y[p][co] = act(bias[co] + sum_tap sum_ci w[tap'][co][ci] in[p + off(tap)][ci]), tap' = tap or 8 - tap.
-// One wave: 32 output channels x NT times and 32 consecutive pixels (of one image row: W % 32 == 0);
-// four waves per workgroup along the pixels. Output FP32 or BF16, NHWC with `cout` channels.
+// One wave: 32 output channels x NT times and 32 x NP consecutive pixels of one image row (W % (32 NP)
+// == 0); four waves per workgroup along the pixels. A weight operand loaded for an MFMA serves the NP
+// pixel tiles, an input operand the NT channel tiles. Output FP32 or BF16, NHWC with `cout` channels.
 // ------------------------------------------------------------------------------------------------
-template <int NT, typename OutT>
+template <int NT, int NP, typename OutT>
 __global__ void __launch_bounds__(256)
 k_conv3x3(ConvIn in, const bf16 *__restrict__ w, const float *__restrict__ bias, OutT *__restrict__ y, int B, int H,
           int W, int cout, int relu, int flip) {
   const int cin = in.c1 + in.c2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const long M = (long)B * H * W;
-  const long m0 = ((long)blockIdx.x * 4 + wave) * 32;
-  if (m0 >= M) return;  // whole waves only (M % 32 == 0)
+  const long m0 = ((long)blockIdx.x * 4 + wave) * 32 * NP;
+  if (m0 >= M) return;  // whole waves only (M % (32 NP) == 0)
   const int n0 = blockIdx.y * 32 * NT;
   const int px = lane & 31, kh = lane >> 5;
-  const long m = m0 + px;
-  const int x = (int)(m % W);
-  const long t0 = m / W;
+  // the NP tiles lie in one image row: same row and batch, x advancing by 32
+  const int x0 = (int)((m0 + px) % W);
+  const long t0 = (m0 + px) / W;
   const int yy = (int)(t0 % H), b = (int)(t0 / H);
-  f32x16 acc[NT];
+  f32x16 acc[NT][NP];
 #pragma unroll
   for (int t = 0; t < NT; ++t)
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[t][r] = 0.f;
+    for (int q = 0; q < NP; ++q)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[t][q][r] = 0.f;
   const int Hh = H >> 1, Wh = W >> 1;
   for (int tap = 0; tap < 9; ++tap) {
-    const int sy = yy + tap / 3 - 1, sx = x + tap % 3 - 1;
-    const bool inb = sy >= 0 && sy < H && sx >= 0 && sx < W;
+    const int sy = yy + tap / 3 - 1;
     const int wt = flip ? 8 - tap : tap;
-    const bf16 *p1 = in.x1, *p2 = in.x2;
-    if (inb) {
-      p1 += (in.up1 ? (((long)b * Hh + (sy >> 1)) * Wh + (sx >> 1)) : (((long)b * H + sy) * W + sx)) * in.c1;
-      if (in.c2) p2 += (((long)b * H + sy) * W + sx) * in.c2;
+    const bf16 *p1[NP], *p2[NP];
+    bool inb[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      const int sx = x0 + 32 * q + tap % 3 - 1;
+      inb[q] = sy >= 0 && sy < H && sx >= 0 && sx < W;
+      p1[q] = in.x1;
+      p2[q] = in.x2;
+      if (inb[q]) {
+        p1[q] += (in.up1 ? (((long)b * Hh + (sy >> 1)) * Wh + (sx >> 1)) : (((long)b * H + sy) * W + sx)) * in.c1;
+        if (in.c2) p2[q] += (((long)b * H + sy) * W + sx) * in.c2;
+      }
     }
     const bf16 *wrow = w + ((long)wt * cout + n0 + px) * cin + 8 * kh;
     for (int c0 = 0; c0 < cin; c0 += 16) {
       const int c = c0 + 8 * kh;
-      u16x8 bv = (u16x8)0;
-      if (inb) bv = *(const u16x8 *)(c < in.c1 ? p1 + c : p2 + (c - in.c1));
+      u16x8 bv[NP];
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        bv[q] = (u16x8)0;
+        if (inb[q]) bv[q] = *(const u16x8 *)(c < in.c1 ? p1[q] + c : p2[q] + (c - in.c1));
+      }
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const u16x8 av = *(const u16x8 *)(wrow + (long)t * 32 * cin + c0);
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(av), as_bf16x8(bv), acc[t], 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < NP; ++q)
+          acc[t][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(av), as_bf16x8(bv[q]), acc[t][q], 0, 0, 0);
       }
     }
   }
-  // lane: pixel m (column px); rows = output channels (r/4)*8 + kh*4 + r%4
+  // lane: pixel m0 + 32 q + px (column px of tile q); rows = output channels (r/4)*8 + kh*4 + r%4
 #pragma unroll
-  for (int t = 0; t < NT; ++t)
+  for (int q = 0; q < NP; ++q) {
+    const long m = m0 + 32 * q + px;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int co = n0 + t * 32 + j * 8 + kh * 4;
-      OutT *dst = y + m * cout + co;
+    for (int t = 0; t < NT; ++t)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        float v = acc[t][j * 4 + q] + (bias ? bias[co + q] : 0.f);
-        if (relu) v = v > 0.f ? v : 0.f;
-        if constexpr (sizeof(OutT) == 2)
-          dst[q] = f2bf(v);
-        else
-          dst[q] = v;
+      for (int j = 0; j < 4; ++j) {
+        const int co = n0 + t * 32 + j * 8 + kh * 4;
+        OutT *dst = y + m * cout + co;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          float v = acc[t][q][j * 4 + e] + (bias ? bias[co + e] : 0.f);
+          if (relu) v = v > 0.f ? v : 0.f;
+          if constexpr (sizeof(OutT) == 2)
+            dst[e] = f2bf(v);
+          else
+            dst[e] = v;
+        }
       }
-    }
+  }
 }
 
 // ------------------------------------------------------------------------------------------------
 // Weight gradient of a 3x3 convolution over one chunk of pixels:
 // part[chunk][tap][co][ci] = sum_{p in chunk} dz[p][co] in[p + off(tap)][ci].
 // Grid: x = chunk, y = tap * (cout / 32) + co block, z = ci block of 32. Each wave reduces a quarter of
-// the chunk with MFMA (rows co, columns ci, reduction over 16 pixels per instruction); the four waves
-// are summed in LDS in a fixed order.
+// the chunk with MFMA (rows co, columns ci, reduction over 16 pixels per instruction). Both operands
+// are channel-innermost while the reduction runs over pixels, so a wave stages each step's 16 pixels
+// x 32 channels of dz and of the input through its own LDS block with one 16-byte load per lane and
+// operand, then reads them across pixels; the four waves are summed in LDS in a fixed order.
 // ------------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_conv3x3_wgrad(ConvIn in, const bf16 *__restrict__ dz, float *__restrict__ part, int B, int H, int W, int cout,
@@ -163,57 +185,66 @@ k_conv3x3_wgrad(ConvIn in, const bf16 *__restrict__ dz, float *__restrict__ part
   const long M = (long)B * H * W;
   const long pbeg = (long)blockIdx.x * chunk, pend = pbeg + chunk < M ? pbeg + chunk : M;
   const int px = lane & 31, kh = lane >> 5;
-  const int co = co0 + px, ci = ci0 + px;
-  const bool ci_ok = ci < cin;
-  const bool from1 = ci < in.c1;
+  // staging: lane -> pixel q = lane / 4 of the step, channels 8 g .. 8 g + 7 of the 32-block
+  const int q = lane >> 2, grp = lane & 3;
+  const int cs = ci0 + 8 * grp;  // first input channel this lane stages
+  const bool cs_ok = cs < cin;
+  const bool cs1 = cs < in.c1;
   const int Hh = H >> 1, Wh = W >> 1;
+  __shared__ bf16 sa[4][16 * 32], sb[4][16 * 32];  // [wave][pixel][channel]
+  bf16 *la = sa[wave], *lb = sb[wave];
   f32x16 acc;
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.f;
   const long quarter = (chunk / 4 + 15) / 16 * 16;
   const long wbeg = pbeg + wave * quarter;
   const long wend = wbeg + quarter < pend ? wbeg + quarter : pend;
+  auto wave_sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
   for (long p0 = wbeg; p0 < wend; p0 += 16) {
     // 16 consecutive pixels from a multiple of 16 lie in one image row (W % 32 == 0)
     const int row = (int)(p0 / W), x0 = (int)(p0 - (long)row * W);
     const int yy = row % H, b = row / H;
     const int sy = yy + dy;
-    const bool rowok = ci_ok && sy >= 0 && sy < H;
-    const bf16 *src = nullptr;
-    int cstride = 0, xs = 0;
-    if (rowok) {
-      if (from1) {
-        src = in.x1 + ((in.up1 ? ((long)b * Hh + (sy >> 1)) * Wh : ((long)b * H + sy) * W)) * in.c1 + ci;
-        cstride = in.c1;
-        xs = in.up1;
-      } else {
-        src = in.x2 + ((long)b * H + sy) * W * in.c2 + (ci - in.c1);
-        cstride = in.c2;
-      }
+    const bool pin = p0 + q < wend;
+    u16x8 av = (u16x8)0, bv = (u16x8)0;
+    if (pin) av = *(const u16x8 *)(dz + (p0 + q) * cout + co0 + 8 * grp);
+    const int sx = x0 + q + dx;
+    if (pin && cs_ok && sy >= 0 && sy < H && sx >= 0 && sx < W) {
+      const bf16 *src;
+      if (cs1)
+        src = in.x1 + (in.up1 ? (((long)b * Hh + (sy >> 1)) * Wh + (sx >> 1)) : (((long)b * H + sy) * W + sx)) * in.c1 + cs;
+      else
+        src = in.x2 + (((long)b * H + sy) * W + sx) * in.c2 + (cs - in.c1);
+      bv = *(const u16x8 *)src;
     }
-    const bf16 *dzp = dz + p0 * cout + co;
-    u16x8 av, bv;
+    wave_sync();  // the previous step's reads are done
+    *(u16x8 *)(la + q * 32 + 8 * grp) = av;
+    *(u16x8 *)(lb + q * 32 + 8 * grp) = bv;
+    wave_sync();
+    u16x8 a, bb;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int q = 8 * kh + j;
-      const bool pin = p0 + q < wend;
-      av[j] = pin ? dzp[(long)q * cout] : (unsigned short)0;
-      const int sx = x0 + q + dx;
-      bv[j] = (pin && rowok && sx >= 0 && sx < W) ? src[(long)(sx >> xs) * cstride] : (unsigned short)0;
+    for (int k = 0; k < 8; ++k) {
+      a[k] = la[(8 * kh + k) * 32 + px];
+      bb[k] = lb[(8 * kh + k) * 32 + px];
     }
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(av), as_bf16x8(bv), acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(a), as_bf16x8(bb), acc, 0, 0, 0);
   }
   __shared__ float red[4][16][64];
 #pragma unroll
   for (int r = 0; r < 16; ++r) red[wave][r][lane] = acc[r];
   __syncthreads();
   if (wave == 0) {
+    const int ci = ci0 + px;
     float *dst = part + (((long)blockIdx.x * 9 + tap) * cout) * cin;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const float v = ((red[0][r][lane] + red[1][r][lane]) + red[2][r][lane]) + red[3][r][lane];
-      const int row = co0 + (r >> 2) * 8 + kh * 4 + (r & 3);
-      if (ci_ok) dst[(long)row * cin + ci] = v;
+      const int orow = co0 + (r >> 2) * 8 + kh * 4 + (r & 3);
+      if (ci < cin) dst[(long)orow * cin + ci] = v;
     }
   }
 }
@@ -638,21 +669,28 @@ int launch_conv(msg_unet *u, const Conv &c, const bf16 *w, const float *bias, vo
                 int relu, int flip, ConvIn in, int H, int W) {
   const long M = (long)u->B * H * W;
   const int nt = (cout % 64 == 0) ? 2 : 1;
-  dim3 grid(cdivu(M, 128), cout / (32 * nt));
-  if (nt == 2) {
-    if (out_f32)
-      hipLaunchKernelGGL((k_conv3x3<2, float>), grid, dim3(256), 0, u->s, in, w, bias, (float *)y, u->B, H, W, cout,
-                         relu, flip);
+  const int np = (W % 64 == 0) ? 2 : 1;
+  dim3 grid(cdivu(M, 128 * np), cout / (32 * nt));
+  if (out_f32) {
+    float *yo = (float *)y;
+    if (nt == 2 && np == 2)
+      hipLaunchKernelGGL((k_conv3x3<2, 2, float>), grid, dim3(256), 0, u->s, in, w, bias, yo, u->B, H, W, cout, relu, flip);
+    else if (nt == 2)
+      hipLaunchKernelGGL((k_conv3x3<2, 1, float>), grid, dim3(256), 0, u->s, in, w, bias, yo, u->B, H, W, cout, relu, flip);
+    else if (np == 2)
+      hipLaunchKernelGGL((k_conv3x3<1, 2, float>), grid, dim3(256), 0, u->s, in, w, bias, yo, u->B, H, W, cout, relu, flip);
     else
-      hipLaunchKernelGGL((k_conv3x3<2, bf16>), grid, dim3(256), 0, u->s, in, w, bias, (bf16 *)y, u->B, H, W, cout,
-                         relu, flip);
+      hipLaunchKernelGGL((k_conv3x3<1, 1, float>), grid, dim3(256), 0, u->s, in, w, bias, yo, u->B, H, W, cout, relu, flip);
   } else {
-    if (out_f32)
-      hipLaunchKernelGGL((k_conv3x3<1, float>), grid, dim3(256), 0, u->s, in, w, bias, (float *)y, u->B, H, W, cout,
-                         relu, flip);
+    bf16 *yo = (bf16 *)y;
+    if (nt == 2 && np == 2)
+      hipLaunchKernelGGL((k_conv3x3<2, 2, bf16>), grid, dim3(256), 0, u->s, in, w, bias, yo, u->B, H, W, cout, relu, flip);
+    else if (nt == 2)
+      hipLaunchKernelGGL((k_conv3x3<2, 1, bf16>), grid, dim3(256), 0, u->s, in, w, bias, yo, u->B, H, W, cout, relu, flip);
+    else if (np == 2)
+      hipLaunchKernelGGL((k_conv3x3<1, 2, bf16>), grid, dim3(256), 0, u->s, in, w, bias, yo, u->B, H, W, cout, relu, flip);
     else
-      hipLaunchKernelGGL((k_conv3x3<1, bf16>), grid, dim3(256), 0, u->s, in, w, bias, (bf16 *)y, u->B, H, W, cout,
-                         relu, flip);
+      hipLaunchKernelGGL((k_conv3x3<1, 1, bf16>), grid, dim3(256), 0, u->s, in, w, bias, yo, u->B, H, W, cout, relu, flip);
   }
   UHIP(hipGetLastError());
   (void)c;

@@ -71,10 +71,14 @@ def check_stage(out, g, st, precision, tol, med_tol):
     deformations, med_tol for the median where the sample counts agree."""
     N = 64
     nt = 3 if st == 1 else 4
+    worst = 0.0
     for i in range(nt):
         sub = f"{i}" if i == 0 else f"{i}-r"
         d = np.stack([read_p9(str(out / f"stage{st}" / sub / f"deformation_06_{c}.dat.bz2")) for c in range(2)])
-        assert np.abs(d - g[f"s{st}_deformations"][i]).max() * (N - 1) < tol, (st, i)
+        err = np.abs(d - g[f"s{st}_deformations"][i]).max() * (N - 1)
+        worst = max(worst, err)
+        assert err < tol, (st, i)
+    print(f"stage {st}, precision {precision}: largest deformation difference {worst:.3e} px", flush=True)
     cnt = read_p9(str(out / f"stage{st}" / "numSamples.q2bz"))
     assert np.abs(cnt - g[f"s{st}_numSamples"]).sum() <= (0 if precision == 64 else 8)
     # pixels with the same sample count (FP32 fields can move a sample point across the image edge)
@@ -110,18 +114,18 @@ def check_stage(out, g, st, precision, tol, med_tol):
         assert np.abs(red - g[f"s{st}_reduceDef"]).max() * (N - 1) < tol, st
 
 
-@pytest.mark.parametrize("fixture,precision,tol", [("e2e64", 64, 1e-6), ("e2e64", 32, 5e-3), ("e2e64_ncc", 64, 1e-6),
+@pytest.mark.parametrize("fixture,precision,tol", [("e2e64", 64, 1e-6), ("e2e64", 32, 1e-3), ("e2e64_ncc", 64, 1e-6),
                                                    ("e2e64_reduce", 64, 1e-6), ("e2e128_to64", 64, 1e-6),
                                                    ("e2e_crop64", 64, 1e-6), ("e2e_resize64", 64, 1e-6)])
 def test_matchseries_stage1(tmp_path, fixture, precision, tol):
     """Stage 1 from the frames; stage 2 of the same run (lambda 1: its step decisions can part from
     the reference's on inputs that differ in the last bits) is only checked for completeness.
 
-    FP32 fields with FP32 warp and stencil arithmetic: an Armijo decision between two step sizes whose
-    energies agree to the FP32 noise (a flat direction of the energy) can go the other way; the
-    deformation then moves along that direction while every logged energy stays within 2e-3 of the
-    reference's. Measured on this fixture: the chained solve of template 2 ends 1.8e-3 px from the
-    reference with energies within 1.5e-5, every other solve within 1e-6 px. Hence 5e-3 px.
+    FP32 fields with FP32 warp and stencil arithmetic: an Armijo decision whose margin lies within the
+    FP32 energy pass's error bound is taken on an FP64-arithmetic evaluation of the same candidates,
+    so FP32 rounding alone does not flip it (without that, one decision along a flat direction moved
+    the chained solve of template 2 by 1.8e-3 px). Measured: 9e-7 px from the reference; held to
+    1e-3 px.
 
     e2e128_to64: frames of 128^2 at precisionLevel 6 (normalised at 128^2, then restricted);
     e2e_crop64: frames of 80x72 with cropInput at (5, 3); e2e_resize64: frames of 150x130 with
@@ -416,10 +420,15 @@ def test_matchseries_stage2_from_reference_stage1(tmp_path, fixture, precision, 
     if precision == 64:
         check_stage(out, g, 2, precision, tol, 1e-4)
         return
+    worst = 0.0
     for i in range(4):
         sub = f"{i}" if i == 0 else f"{i}-r"
         d = np.stack([read_p9(str(out / "stage2" / sub / f"deformation_06_{c}.dat.bz2")) for c in range(2)])
-        assert np.abs(d - g["s2_deformations"][i]).max() * 63 < tol, i
+        err = np.abs(d - g["s2_deformations"][i]).max() * 63
+        worst = max(worst, err)
+        assert err < tol, i
     en = np.loadtxt(str(out / "stage2" / "energies.txt"))[:, 1]
+    print(f"stage 2, precision 32: largest deformation difference {worst:.3e} px, energies "
+          f"{np.max(np.abs(en - g['s2_energies']) / np.abs(g['s2_energies'])):.2e}", flush=True)
     np.testing.assert_allclose(en, g["s2_energies"], rtol=5e-2)
     assert np.isfinite(read_p9(str(out / "stage2" / "median.q2bz"))).all()

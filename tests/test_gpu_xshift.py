@@ -163,7 +163,7 @@ def test_match_seismic_series_program(tmp_path, gpu, g, precision):
     2e-9 px. FP32 fields round the frames and every step's deformation and V-cycle to FP32, which
     moves these short, unconverged solves (at most 40 iterations per level) and where they stop:
     measured 0.05 px and 1.3e-4 relative in the final energies on stage 1 (held to 0.1 px and 3e-4),
-    0.32 px on stage 2 (held to 0.5 px)."""
+    0.32 px on stage 2 (held to 0.5 px, its average and median to 5 % of the largest value)."""
     if not os.path.exists(EXE):
         pytest.skip("bin/matchSeismicSeries not built")
     od = run_series(tmp_path, g, precision)
@@ -196,7 +196,7 @@ def test_match_seismic_series_program(tmp_path, gpu, g, precision):
             mine = read_p9(str(sd / f"{name}.q2bz"))
             r = np.abs(mine - ref).max() / np.abs(ref).max()
             print(f"stage {st} {name}: largest difference {r:.2e} of the largest value", flush=True)
-            assert r < (1e-9 if precision == 64 else 2e-2), (st, name)
+            assert r < (1e-9 if precision == 64 else 5e-2), (st, name)
         assert (sd / "energies.txt").read_bytes() == bytes(g[f"s{st}_energies_txt"])
     assert (od / "stage1" / "defNorms.txt").read_bytes() == bytes(g["s1_defnorms_txt"])
     assert worst[1] < (2e-9 if precision == 64 else 0.1)
