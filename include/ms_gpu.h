@@ -303,6 +303,14 @@ typedef struct {
   msg_iter_trace *iters;  /* max_iter records per record slot (the first `iterations` are set) */
   double *start;          /* 2 per record: E and lambda_l*E_reg at the start of the level     */
   int *iterations;        /* 1 per record                                                       */
+  /* Called as each record completes (the reference writes a level's files when it finishes,
+   * registration.h:1110-1111), with the level's field (2*N_l^2 doubles), its iteration records, the
+   * count and the 2 start values; the pointers are valid during the call only. Called from the
+   * library's threads (the pairwise solves and the chain run on different ones): it must be
+   * thread-safe and should return quickly. NULL: none. */
+  void (*on_level)(void *user, long long record, int level, const double *field, const msg_iter_trace *iters,
+                   int iterations, const double *start);
+  void *user;
 } msg_series_levels;
 
 /* templates: n device images (dtype) at precision_level; reference: one device image.

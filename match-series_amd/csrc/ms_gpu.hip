@@ -919,16 +919,23 @@ int record_level(msg_ctx *c, int level, int start, int stop, int max_iter, const
   const size_t per_solve = level_block(start, stop);
   const size_t lev_off = level_block(start, level) - nn2;
   std::vector<S> h;
-  if (lv->fields) {
+  if (lv->fields || lv->on_level) {
     h.resize(nn2 * c->B);
     HIPCHK(hipMemcpyAsync(h.data(), c->phi[level], sizeof(S) * h.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
+  std::vector<double> one(lv->on_level ? nn2 : 0);
   for (int k = 0; k < c->B; ++k) {
     const long long r = o.rec0 + (long long)k * nl + (level - start);
     if (lv->fields) {
       double *dst = lv->fields + o.field0 + k * per_solve + lev_off;
       for (size_t i = 0; i < nn2; ++i) dst[i] = (double)h[k * nn2 + i];
+    }
+    if (lv->on_level) {
+      for (size_t i = 0; i < nn2; ++i) one[i] = (double)h[k * nn2 + i];
+      const double st2[2] = {c->level_start[2 * k], c->level_start[2 * k + 1]};
+      lv->on_level(lv->user, r, level, one.data(), max_iter > 0 ? tr.data() + (size_t)k * max_iter : nullptr,
+                   std::min(its[k], std::max(max_iter, 0)), st2);
     }
     if (lv->iters && max_iter > 0)
       std::memcpy(lv->iters + r * max_iter, tr.data() + (size_t)k * max_iter, sizeof(msg_iter_trace) * max_iter);
@@ -946,7 +953,7 @@ int solve_impl(msg_ctx *c, int start, int stop, const msg_solve_params *p, doubl
                const LevelOut *out = nullptr) {
   std::vector<double> e(c->B, NAN);
   const bool rec = out && out->lv;
-  const int cap = rec && out->lv->iters ? p->max_iter : 0;
+  const int cap = rec && (out->lv->iters || out->lv->on_level) ? p->max_iter : 0;
   std::vector<msg_iter_trace> tr((size_t)c->B * std::max(cap, 0));
   std::vector<int> its(c->B, 0);
   for (int level = start; level <= stop; ++level) {

@@ -12,6 +12,11 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -437,6 +442,104 @@ void averageOnly(const ParameterParser &p, int stage) {
 }
 
 // ACTION 0 of one stage: SeriesMatching::matchAndAverageSeries (matchSeries.h:565-570).
+// Writes the per-level files of the solves while the GPU runs: the library hands each finished level
+// to onLevel (from its own threads), which queues the file writing (bzip2 of the field) for a few
+// worker threads. Holding nothing beyond the queue, the stage needs no host copy of all its levels.
+class LevelWriter {
+ public:
+  struct Slot {
+    std::string sub;  // directory of the solve ("<i>/", "<i>-r/", ...)
+    int level;
+  };
+  LevelWriter(std::vector<Slot> slots, const msg_series_params &sp, int nthreads)
+      : slots_(std::move(slots)), sp_(sp) {
+    for (int k = 0; k < nthreads; ++k) threads_.emplace_back([this] { loop(); });
+  }
+  ~LevelWriter() { join(); }
+  // Waits until every queued file is written; rethrows the first failure.
+  void finish() {
+    join();
+    if (!err_.empty()) MSH_THROW(err_);
+  }
+  static void onLevel(void *user, long long record, int level, const double *field, const msg_iter_trace *iters,
+                      int iterations, const double *start) {
+    auto *w = static_cast<LevelWriter *>(user);
+    const int m = 1 << level;
+    auto job = std::make_shared<Job>();
+    job->record = record;
+    job->field.assign(field, field + (size_t)2 * m * m);
+    if (iters) job->iters.assign(iters, iters + std::max(iterations, 0));
+    job->start[0] = start[0];
+    job->start[1] = start[1];
+    std::lock_guard<std::mutex> g(w->mu_);
+    w->q_.push_back(job);
+    w->cv_.notify_one();
+  }
+
+ private:
+  void join() {
+    {
+      std::unique_lock<std::mutex> l(mu_);
+      done_ = true;
+      cv_.notify_all();
+    }
+    for (auto &t : threads_)
+      if (t.joinable()) t.join();
+    threads_.clear();
+  }
+  struct Job {
+    long long record;
+    std::vector<double> field;
+    std::vector<msg_iter_trace> iters;
+    double start[2];
+  };
+  void loop() {
+    for (;;) {
+      std::shared_ptr<Job> j;
+      {
+        std::unique_lock<std::mutex> l(mu_);
+        cv_.wait(l, [this] { return done_ || !q_.empty(); });
+        if (q_.empty()) return;
+        j = q_.front();
+        q_.pop_front();
+      }
+      try {
+        write(*j);
+      } catch (std::exception &e) {
+        std::lock_guard<std::mutex> g(mu_);
+        if (err_.empty()) err_ = e.what();
+      }
+    }
+  }
+  // deformation_<LL>_<c>.dat.bz2 and energy_<LL>.txt as GradientDescentBase::writeEnergy prints them:
+  // "k lambda*E_reg E" from k = -1; NCC (a GenEnergyOp too) puts its last value first, E - lambda*E_reg
+  void write(const Job &j) const {
+    if (j.record < 0 || j.record >= (long long)slots_.size()) return;
+    const Slot &sl = slots_[j.record];
+    const int l = sl.level, m = 1 << l;
+    saveField(sl.sub + strprintf("deformation_%02d_%%d.dat.bz2", l), j.field.data(), m);
+    const double lam =
+        sp_.lambda_factor > 0 ? sp_.lambda * std::pow(sp_.lambda_factor, sp_.stop_level - l) : sp_.lambda;
+    std::ofstream e(sl.sub + strprintf("energy_%02d.txt", l));
+    auto line = [&](int k, double regw, double E) {
+      e << k << " ";
+      if (sp_.data_term == MSG_NCC || sp_.data_term == MSG_XSHIFT) e << E - regw << " ";
+      e << regw << " " << E << "\n";
+    };
+    line(-1, j.start[1], j.start[0]);
+    for (size_t k = 0; k < j.iters.size(); ++k) line((int)k, lam * j.iters[k].ereg, j.iters[k].energy);
+  }
+
+  std::vector<Slot> slots_;
+  msg_series_params sp_;
+  std::vector<std::thread> threads_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::deque<std::shared_ptr<Job>> q_;
+  bool done_ = false;
+  std::string err_;
+};
+
 void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int stage) {
   const std::string dir = dirOf(p);
   const int L = p.getInt("precisionLevel"), N = 1 << L;
@@ -518,65 +621,42 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
   const bool inverse = accumulate && sp.calc_inverse;
   const int nla = sp.alt_start_level > 0 ? sp.stop_level - sp.alt_start_level + 1 : 0;
   const int nrec = npw * nlp + (accumulate ? (n - 1) * nlr : 0) + (inverse ? n * nlr : 0) + npw * nla;
-  size_t nfld = 0;
-  for (int l = sp.start_level; l <= sp.stop_level; ++l) nfld += (size_t)npw * (2u << (2 * l));
-  if (accumulate)
-    for (int l = sp.refine_start_level; l <= sp.refine_stop_level; ++l)
-      nfld += (size_t)(n - 1 + (inverse ? n : 0)) * (2u << (2 * l));
-  for (int l = sp.alt_start_level; nla > 0 && l <= sp.stop_level; ++l) nfld += (size_t)npw * (2u << (2 * l));
   std::unique_ptr<DeviceBuffer> first;
   if (nla > 0) {
     first.reset(new DeviceBuffer(es * 2 * nn * n));
     sp.first_try_out = first->p;
   }
-  const int maxIt = std::max(sp.max_iter, 0);
-  std::vector<double> lvFields(nfld), lvStart(2 * nrec);
-  std::vector<msg_iter_trace> lvIters((size_t)std::max(nrec * maxIt, 1));
-  std::vector<int> lvIts(nrec);
-  msg_series_levels lv = {lvFields.data(), lvIters.data(), lvStart.data(), lvIts.data()};
+  // every level record's directory and level, in the library's record order (msg_series_levels)
+  std::vector<LevelWriter::Slot> slots;
+  slots.reserve(nrec);
+  auto addSolve = [&](const std::string &sub, int l0, int l1) {
+    makeDirectory(sub);
+    for (int l = l0; l <= l1; ++l) slots.push_back({sub, l});
+  };
+  for (int i = 0; i < npw; ++i) addSolve(strprintf("%s%d/", dir.c_str(), i), sp.start_level, sp.stop_level);
+  for (int i = 1; accumulate && i < n; ++i)
+    addSolve(strprintf("%s%d-r/", dir.c_str(), i), sp.refine_start_level, sp.refine_stop_level);
+  for (int i = 0; inverse && i < n; ++i)
+    addSolve(strprintf("%s%d-r-inv/", dir.c_str(), i), sp.refine_start_level, sp.refine_stop_level);
+  for (int i = 0; i < npw && nla > 0; ++i) addSolve(strprintf("%s%d-alt/", dir.c_str(), i), sp.alt_start_level, sp.stop_level);
+  if ((int)slots.size() != nrec) MSH_THROW("internal error: level record count");
+  LevelWriter writer(std::move(slots), sp, 8);
+  msg_series_levels lv = {nullptr, nullptr, nullptr, nullptr, &LevelWriter::onLevel, &writer};
   const auto t0 = std::chrono::steady_clock::now();
   DeviceBuffer reduced(es * 2 * nn);
   msgCheck(msg_match_series(&sp, dt, tpl.p, n, ref.p, pin ? pin->p : nullptr, defs.p, pw.p, energies.data(), mean.p,
                             cnt.p, med.p, &lv, sp.reduce_deformations ? reduced.p : nullptr));
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::cerr << "Registered " << n << " templates on the GPU in " << secs << " s\n";
+  writer.finish();
 
   std::vector<double> hdefs(2 * nn * n), hpw(2 * nn * n), hfirst(first ? 2 * nn * n : 0);
   download(hdefs.data(), defs.p, 2 * nn * n, dt);
   download(hpw.data(), pw.p, 2 * nn * n, dt);
   if (first) download(hfirst.data(), first->p, 2 * nn * n, dt);
-  // Writes the level records r0 .. r0+nl-1 of one solve into `sub` (deformation_<LL>_<c>.dat.bz2 and
-  // energy_<LL>.txt as GradientDescentBase::writeEnergy prints them: "k lambda*E_reg E", k from -1).
-  size_t fldOff = 0;
-  int recNext = 0;
-  auto writeLevels = [&](const std::string &sub, int l0, int l1) {
-    for (int l = l0; l <= l1; ++l, ++recNext) {
-      const int r = recNext, m = 1 << l;
-      saveField(sub + strprintf("deformation_%02d_%%d.dat.bz2", l), lvFields.data() + fldOff, m);
-      fldOff += (size_t)2 * m * m;
-      const double lam = sp.lambda_factor > 0 ? sp.lambda * std::pow(sp.lambda_factor, sp.stop_level - l) : sp.lambda;
-      // NCC is a GenEnergyOp too: its last value comes first (E - lambda*E_reg)
-      std::ofstream e(sub + strprintf("energy_%02d.txt", l));
-      auto line = [&](int k, double regw, double E) {
-        e << k << " ";
-        if (sp.data_term == MSG_NCC || sp.data_term == MSG_XSHIFT) e << E - regw << " ";
-        e << regw << " " << E << "\n";
-      };
-      line(-1, lvStart[2 * r + 1], lvStart[2 * r]);
-      for (int k = 0; k < lvIts[r] && k < maxIt; ++k) {
-        const msg_iter_trace &t = lvIters[(size_t)r * maxIt + k];
-        line(k, lam * t.ereg, t.energy);
-      }
-    }
-  };
   std::ofstream defNorms, energiesFile(dir + "/energies.txt");
   if (!loadStageOneResults) defNorms.open(dir + "/defNorms.txt");
   const int refineStop = sp.refine_stop_level;
-  for (int i = 0; i < npw; ++i) {  // pairwise records come first
-    const std::string sub = strprintf("%s%d/", dir.c_str(), i);
-    makeDirectory(sub);
-    writeLevels(sub, sp.start_level, sp.stop_level);
-  }
   for (int i = 0; i < n; ++i) {
     const double *phw = hpw.data() + (size_t)i * 2 * nn;
     if (!loadStageOneResults || i == 0) {
@@ -598,25 +678,14 @@ void matchAndAverage(const ParameterParser &p, bool loadStageOneResults, int sta
     }
     if (accumulate && i > 0) {
       const std::string sub = strprintf("%s%d-r/", dir.c_str(), i);
-      makeDirectory(sub);
       const double *phd = hdefs.data() + (size_t)i * 2 * nn;
-      writeLevels(sub, sp.refine_start_level, refineStop);
       if (refineStop < L) saveField(sub + strprintf("deformation_%02d_%%d.dat.bz2", L), phd, N);
     }
     // matchSeismicSeries: getEnergyOfLastSolution is set by the full-resolution step only (NaN here)
     energiesFile << i << " " << (kSeismic ? (double)NAN : energies[i]) << "\n";
   }
-  for (int i = 0; inverse && i < n; ++i) {  // then the inverse solves' records
-    const std::string sub = strprintf("%s%d-r-inv/", dir.c_str(), i);
-    makeDirectory(sub);
-    writeLevels(sub, sp.refine_start_level, refineStop);
-  }
-  for (int i = 0; nla > 0 && i < npw; ++i) {  // the repeated solves' records and the kept results
-    const std::string sub = strprintf("%s%d-alt/", dir.c_str(), i);
-    makeDirectory(sub);
-    writeLevels(sub, sp.alt_start_level, sp.stop_level);
+  for (int i = 0; nla > 0 && i < npw; ++i)  // the kept results of the repeated solves
     saveField(strprintf("%sdeformation-%03d_%%d.dat.bz2", dir.c_str(), i), hpw.data() + (size_t)i * 2 * nn, N);
-  }
   if (sp.reduce_deformations) {  // "<saveDirectory>reduceDef_<c>.dat.bz2" (matchSeries.h:701)
     std::vector<double> hr(2 * nn);
     download(hr.data(), reduced.p, 2 * nn, dt);

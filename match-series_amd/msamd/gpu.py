@@ -56,9 +56,13 @@ class SeriesJob(ctypes.Structure):
                 ("mean", ctypes.c_void_p), ("count", ctypes.c_void_p), ("median", ctypes.c_void_p)]
 
 
+LEVEL_CALLBACK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, _dp, ctypes.c_void_p,
+                                  ctypes.c_int, _dp)
+
+
 class SeriesLevels(ctypes.Structure):
     _fields_ = [("fields", _dp), ("iters", ctypes.POINTER(IterTrace)), ("start", _dp),
-                ("iterations", ctypes.POINTER(ctypes.c_int))]
+                ("iterations", ctypes.POINTER(ctypes.c_int)), ("on_level", LEVEL_CALLBACK), ("user", ctypes.c_void_p)]
 
 
 # exported symbols with their argument types (also the list the C-ABI test checks)
