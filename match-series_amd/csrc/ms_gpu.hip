@@ -2237,9 +2237,11 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
   if (chain) CHK(msg_ctx_set_data_term(rc, p->data_term));
   if (chain && p->data_term == MSG_XSHIFT) CHK(msg_ctx_set_y_reg_factor(rc, p->y_reg_factor));
 
-  DevBuf pw;
-  HIPCHK(hipMalloc(&pw.p, sizeof(S) * 2 * nn * n));
-  S *pwp = (S *)pw.p;
+  // Buffers of the stage come from the calling thread's scratch slots: allocated by the first call,
+  // grown when a larger series comes, kept for the next call (no allocation in a repeated stage).
+  void *pwv = nullptr;
+  CHK(scratch_get(100, sizeof(S) * 2 * nn * n, &pwv));
+  S *pwp = (S *)pwv;
   for (int i = 0; i < n; ++i) energies[i] = NAN;
   if (reuse && n > 1) {
     HIPCHK(hipMemcpy(pwp + 2 * nn, (const S *)pairwise_in + 2 * nn, sizeof(S) * 2 * nn * (n - 1),
@@ -2316,11 +2318,13 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
       if (!p->dont_normalize) CHK(normalize_impl<S>(rc->R[L], N, 1, p->saturation_pct, p->normalize_min_to_zero, rs));
       if (p->pre_smooth_sigma > 0) CHK(h1_impl<S>(rc->R[L], rc->R[L], N, 1, p->pre_smooth_sigma / (double)(N - 1), rs));
       CHK(rebuild_pyramid<S>(rc, rc->R, -1, 1));
-      DevBuf acc;
-      HIPCHK(hipMalloc(&acc.p, sizeof(S) * 2 * nn));
+      DevBuf swp, zero;
+      struct {
+        void *p = nullptr;  // a scratch slot of this thread: not freed here
+      } acc, prev;
+      CHK(scratch_get(101, sizeof(S) * 2 * nn, &acc.p));
+      CHK(scratch_get(102, sizeof(S) * 2 * nn, &prev.p));
       HIPCHK(hipMemsetAsync(acc.p, 0, sizeof(S) * 2 * nn, rs));
-      DevBuf prev, swp, zero;
-      HIPCHK(hipMalloc(&prev.p, sizeof(S) * 2 * nn));
       if (p->keep_start_transformation) {
         HIPCHK(hipMalloc(&zero.p, sizeof(S) * 2 * nn));
         HIPCHK(hipMemsetAsync(zero.p, 0, sizeof(S) * 2 * nn, rs));
@@ -2410,8 +2414,10 @@ int match_series_impl(const msg_series_params *p, const void *templates, int n, 
   if (pairwise_out) HIPCHK(hipMemcpy(pairwise_out, pwp, sizeof(S) * 2 * nn * n, hipMemcpyDeviceToDevice));
 
   // --- reconstruction: warp the raw templates (Inf outside), then mean / count / median ---
-  DevBuf stack;
-  HIPCHK(hipMalloc(&stack.p, sizeof(S) * nn * std::max(n, 1)));
+  struct {
+    void *p = nullptr;  // a scratch slot of this thread: not freed here
+  } stack;
+  CHK(scratch_get(103, sizeof(S) * nn * std::max(n, 1), &stack.p));
   // reduceDeformations (matchSeries.h:565-570, 678-704): the reduced deformation is applied to every
   // template first (DeformImage, +Inf outside), then the template's own deformation (L386-392)
   DevBuf red, pre;
