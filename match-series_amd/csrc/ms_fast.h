@@ -102,9 +102,41 @@ __device__ __forceinline__ double stiff_bi_f(const float a[4], const float b[4])
   return (double)e * (1.0 / 6.0) + (double)d * (2.0 / 6.0);
 }
 
+// One axis of a deformed quad point for the energy's value: the position e + u clamped to the grid,
+// [0, n-1] in grid units (lo = -e, hi = n-1-e), as element offset `off` from e and local coordinate
+// `loc`. At the upper bound the element is the last node's (offset hi, loc 0) and the bilinear form
+// returns that node's value exactly -- the interpolant's value there, which the reference's clip to
+// width - eps reaches to rounding; the variation keeps clip_rel, which also reports a clipped axis.
+// NaN clamps to the lower bound, so the node index stays in [0, n-1].
+__device__ __forceinline__ void clamp_rel(float u, float lo, float hi, int &off, float &loc) {
+  const float uc = fminf(fmaxf(u, lo), hi);
+  const float fl = floorf(uc);
+  off = (int)fl;
+  loc = uc - fl;
+}
+
+// bilinear value of T at byte offset vo + (lx, ly) (reads past the image return 0 and meet a weight of
+// 0); the upper row's loads take the row pitch as their scalar offset
+__device__ __forceinline__ float bilin_at(__amdgpu_buffer_rsrc_t T, unsigned vo, int n, float lx, float ly) {
+#pragma clang fp contract(fast)
+  const unsigned pitch = (unsigned)n * 4u;
+  auto ld = [&](unsigned extra, unsigned so) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(T, vo + extra, so, 0));
+  };
+  const float t0 = ld(0u, 0u), t1 = ld(4u, 0u), t2 = ld(0u, pitch), t3 = ld(4u, pitch);
+  const float a = lx * (t1 - t0) + t0;
+  const float b = lx * (t3 - t2) + t2;
+  return ly * (b - a) + a;
+}
+
 // ================================================================================================
-// SSD energy pass for up to MAX_CAND step sizes (k_energy in FP32 arithmetic).
+// SSD energy pass for up to MAX_CAND step sizes (k_energy in FP32 arithmetic). Tiles of 64 x 16
+// elements; each thread takes a vertical run of EN32_RUN elements, reusing the upper node row of one
+// element as the lower row of the next.
 // ================================================================================================
+// FIDX: byte offsets of the gathers formed in FP32 arithmetic, exact while n^2 <= 2^24 (n <= 4096):
+// two fused multiply-adds and one conversion instead of two conversions and integer arithmetic.
+template <bool FIDX>
 __global__ void __launch_bounds__(256)
 k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const float *__restrict__ phi,
              const float *__restrict__ z, const float *__restrict__ dold, int n, double vol,
@@ -116,7 +148,8 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
   const int nc = st[pair].ncand;
   if (nc == 0 || st[pair].exact) return;  // uniform across the workgroup (k_energy_exact_f32 takes exact pairs)
   const int dm = st[pair].dirmode;
-  __shared__ double red[4 * 2 * MAX_CAND];
+  constexpr int NACC = MAX_CAND + 3;  // E_SSD per candidate; sums of f'Kf, f'Kd, d'Kd
+  __shared__ double red[4 * NACC];
 
   const size_t nn = (size_t)n * n;
   const float *Rp = R + pair * nn, *Tp = T + pair * nn;
@@ -127,89 +160,144 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
   const double beta = st[pair].beta;
   const bool use_dir = (dm != DIR_NONE);
   const float nm1 = (float)(n - 1);
+  float tau[MAX_CAND];
+#pragma unroll
+  for (int k = 0; k < MAX_CAND; ++k) tau[k] = (k < nc) ? (float)st[pair].cand[k] : 0.f;  // powers of two: exact
 
   const int ex = (tile % ntx) * EN_TX + (threadIdx.x % EN_TX);
-  const int ey = (tile / ntx) * EN_TY + (threadIdx.x / EN_TX);
-  const bool valid = (ex < n - 1) && (ey < n - 1);
+  const int ey0 = (tile / ntx) * EN32_TY + (threadIdx.x / EN_TX) * EN32_RUN;
 
-  double acc[2 * MAX_CAND];
+  double acc[NACC];
 #pragma unroll
-  for (int k = 0; k < 2 * MAX_CAND; ++k) acc[k] = 0.0;
+  for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
 
-  if (valid) {
-    // Positions are linear in the step size: the quad points of phi + tau d sit at u_f + tau u_d
-    // (grid units relative to the element), and psi^T K psi of phi + tau d is a quadratic in tau. One
-    // interpolation of phi and d per element serves every candidate; positions differ from those of
-    // the stored phi + tau d (rounded per node) by that rounding, about 3e-7 px at 10 px.
+  if (ex < n - 1) {
+    // node values of the element, b = x + 2 y: slots 0, 1 the lower row, 2, 3 the upper row
     float f0[4], f1[4], d0[4], d1[4], rn[4];
+    auto load_row = [&](int y, int slot) {
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int g = l2g(ex, ey, b, n);
-      f0[b] = px[g];
-      f1[b] = py[g];
-      d0[b] = use_dir ? search_dir<float>(zx, ox_, g, cg, use_old, beta) : 0.f;
-      d1[b] = use_dir ? search_dir<float>(zy, oy_, g, cg, use_old, beta) : 0.f;
-      rn[b] = Rp[g];
-    }
-    float rq[4], ufx[4], ufy[4], udx[4], udy[4];
-    quad_values(rn, rq);
-    quad_values(f0, ufx);
-    quad_values(f1, ufy);
-    quad_values(d0, udx);
-    quad_values(d1, udy);
+      for (int c = 0; c < 2; ++c) {
+        const int g = ex + c + y * n;
+        f0[slot + c] = px[g];
+        f1[slot + c] = py[g];
+        d0[slot + c] = use_dir ? search_dir<float>(zx, ox_, g, cg, use_old, beta) : 0.f;
+        d1[slot + c] = use_dir ? search_dir<float>(zy, oy_, g, cg, use_old, beta) : 0.f;
+        rn[slot + c] = Rp[g];
+      }
+    };
+    if (ey0 < n - 1) load_row(ey0, 0);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      ufx[q] = ufx[q] * nm1 + qpxf(q);
-      ufy[q] = ufy[q] * nm1 + qpyf(q);
-      udx[q] *= nm1;
-      udy[q] *= nm1;
-    }
-    const double sff = stiff_bi_f(f0, f0) + stiff_bi_f(f1, f1);
-    const double sfd = stiff_bi_f(f0, d0) + stiff_bi_f(f1, d1);
-    const double sdd = stiff_bi_f(d0, d0) + stiff_bi_f(d1, d1);
+    for (int j = 0; j < EN32_RUN; ++j) {
+      const int ey = ey0 + j;
+      if (ey >= n - 1) break;  // uniform across a wave (a wave is one row of 64 elements)
+      if (j > 0) {
 #pragma unroll
-    for (int k = 0; k < MAX_CAND; ++k) {
-      if (k < nc) {
-        const double taud = st[pair].cand[k];
-        const float tau = (float)taud;
-        float a = 0.f;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const HitF hp = hit_rel(ex, ey, tau * udx[q] + ufx[q], tau * udy[q] + ufy[q], n);
-          float t4[4];
-          const float t = bilin_f(Trs, hp, n, t4) - rq[q];
-          a += t * t;
+        for (int c = 0; c < 2; ++c) {
+          f0[c] = f0[2 + c];
+          f1[c] = f1[2 + c];
+          d0[c] = d0[2 + c];
+          d1[c] = d1[2 + c];
+          rn[c] = rn[2 + c];
         }
-        acc[2 * k] = (double)a * (0.125 * vol);  // 1/2 t^2 * quad weight 1/4 * vol
-        acc[2 * k + 1] = 0.5 * (sff + taud * (2.0 * sfd + taud * sdd));
+      }
+      load_row(ey + 1, 2);
+      // Positions are linear in the step size: the quad points of phi + tau d sit at u_f + tau u_d
+      // (grid units relative to the element), and psi^T K psi of phi + tau d is a quadratic in tau. One
+      // interpolation of phi and d per element serves every candidate; positions differ from those of
+      // the stored phi + tau d (rounded per node) by that rounding, about 3e-7 px at 10 px.
+      float rq[4], ufx[4], ufy[4], udx[4], udy[4];
+      quad_values(rn, rq);
+      quad_values(f0, ufx);
+      quad_values(f1, ufy);
+      quad_values(d0, udx);
+      quad_values(d1, udy);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        ufx[q] = ufx[q] * nm1 + qpxf(q);
+        ufy[q] = ufy[q] * nm1 + qpyf(q);
+        udx[q] *= nm1;
+        udy[q] *= nm1;
+      }
+      acc[MAX_CAND] += stiff_bi_f(f0, f0) + stiff_bi_f(f1, f1);
+      acc[MAX_CAND + 1] += stiff_bi_f(f0, d0) + stiff_bi_f(f1, d1);
+      acc[MAX_CAND + 2] += stiff_bi_f(d0, d0) + stiff_bi_f(d1, d1);
+      // the quad point clamped to [0, n-1] in grid units: lo = -e, hi = n-1-e relative to the element
+      const float lox = -(float)ex, hix = (float)(n - 1 - ex), loy = -(float)ey, hiy = (float)(n - 1 - ey);
+      const int gbase = ex + ey * n;
+      const float base4 = 4.f * (float)gbase, pitch4 = 4.f * (float)n;
+#pragma unroll
+      for (int k = 0; k < MAX_CAND; ++k) {
+        if (k < nc) {
+          float a = 0.f;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            float lx, ly;
+            unsigned vo;
+            if constexpr (FIDX) {
+              // med3 clamps; a NaN position gives offset 0 (the conversion's NaN result), in bounds
+              const float cx = __builtin_amdgcn_fmed3f(tau[k] * udx[q] + ufx[q], lox, hix);
+              const float cy = __builtin_amdgcn_fmed3f(tau[k] * udy[q] + ufy[q], loy, hiy);
+              const float fx = floorf(cx), fy = floorf(cy);
+              lx = cx - fx;
+              ly = cy - fy;
+              // 4 (gbase + fx + fy n): integers below 2^26 that are multiples of 4, exact in FP32
+              vo = (unsigned)(fy * pitch4 + (fx * 4.f + base4));
+            } else {
+              int ox, oy;
+              clamp_rel(tau[k] * udx[q] + ufx[q], lox, hix, ox, lx);
+              clamp_rel(tau[k] * udy[q] + ufy[q], loy, hiy, oy, ly);
+              vo = (unsigned)(gbase + ox + oy * n) * 4u;
+            }
+            const float t = bilin_at(Trs, vo, n, lx, ly) - rq[q];
+            a += t * t;
+          }
+          acc[k] += (double)a * (0.125 * vol);  // 1/2 t^2 * quad weight 1/4 * vol
+        }
       }
     }
   }
   const int nblk = tiles;
-  block_sum_n<2 * MAX_CAND>(acc, red);
+  block_sum_n<NACC>(acc, red);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < MAX_CAND; ++k)
       if (k < nc) {
-        part[((size_t)(pair * MAX_CAND + k) * 2 + 0) * nblk + tile] = acc[2 * k];
-        part[((size_t)(pair * MAX_CAND + k) * 2 + 1) * nblk + tile] = acc[2 * k + 1];
+        const double taud = st[pair].cand[k];
+        part[((size_t)(pair * MAX_CAND + k) * 2 + 0) * nblk + tile] = acc[k];
+        part[((size_t)(pair * MAX_CAND + k) * 2 + 1) * nblk + tile] =
+            0.5 * (acc[MAX_CAND] + taud * (2.0 * acc[MAX_CAND + 1] + taud * acc[MAX_CAND + 2]));
       }
   }
 }
 
-// The same pass in FP64 arithmetic at the stored positions (energy_tile<float>, the pass of FP64 data
-// terms on FP32 fields) for the pairs whose control flagged a near-tie step decision (`exact`). Its own
-// kernel so that the FP32 pass keeps its register budget; a grid-stride loop over the tiles of the
-// flagged pairs, so that a pass without any costs one check per pair.
+// The same pass in FP64 arithmetic at the stored positions (energy_elem<float>, the pass of FP64 data
+// terms on FP32 fields) for the pairs whose control flagged a near-tie step decision (`exact`), on the
+// tiles and partial layout of k_energy_f32. Its own kernel so that the FP32 pass keeps its register
+// budget; a grid-stride loop over the tiles of the flagged pairs, so that a pass without any costs one
+// check per pair.
 __global__ void __launch_bounds__(256)
 k_energy_exact_f32(const float *__restrict__ R, const float *__restrict__ T, const float *__restrict__ phi,
                    const float *__restrict__ z, const float *__restrict__ dold, int n, double h, int width,
                    const PairState *st, double *__restrict__ part, int ntx, int nty, int npairs) {
   const int tiles = ntx * nty;
+  const size_t nn = (size_t)n * n;
   for (int pair = 0; pair < npairs; ++pair) {
     if (!st[pair].exact || st[pair].ncand == 0) continue;  // uniform across the workgroup
-    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x)
-      energy_tile<float>(R, T, phi, z, dold, n, h, width, st, part, ntx, nty, pair, tile);
+    const float *Rp = R + pair * nn, *Tp = T + pair * nn;
+    const float *px = phi + pair * 2 * nn, *py = px + nn;
+    const float *zx = z + pair * 2 * nn, *zy = zx + nn, *ox_ = dold + pair * 2 * nn, *oy_ = ox_ + nn;
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+      const int ex = (tile % ntx) * EN_TX + (threadIdx.x % EN_TX);
+      const int ey0 = (tile / ntx) * EN32_TY + (threadIdx.x / EN_TX) * EN32_RUN;
+      double accS[MAX_CAND], accR[MAX_CAND];
+#pragma unroll
+      for (int k = 0; k < MAX_CAND; ++k) accS[k] = accR[k] = 0.0;
+      if (ex < n - 1)
+        for (int j = 0; j < EN32_RUN; ++j)
+          if (ey0 + j < n - 1)
+            energy_elem<float>(Rp, Tp, px, py, zx, zy, ox_, oy_, n, h, width, st[pair], ex, ey0 + j, accS, accR);
+      energy_write_partials(accS, accR, st[pair].ncand, part, pair, tile, tiles);
+    }
   }
 }
 

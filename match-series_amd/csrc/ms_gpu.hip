@@ -227,12 +227,18 @@ int convert(const void *in, int din, void *out, int dout, size_t cnt, hipStream_
 struct EnergyGeom {
   int ntx, nty, nblk;
 };
-EnergyGeom energy_geom(int n) {
+// Tiles of an energy pass at grid size n: 64 x 4 elements, or 64 x 16 for the FP32 SSD pass
+// (k_energy_f32 and its FP64 re-evaluation).
+EnergyGeom energy_geom(int n, bool f32_ssd = false) {
   EnergyGeom g;
   g.ntx = cdiv(n - 1, msg::EN_TX);
-  g.nty = cdiv(n - 1, msg::EN_TY);
+  g.nty = cdiv(n - 1, f32_ssd ? msg::EN32_TY : msg::EN_TY);
   g.nblk = g.ntx * g.nty;
   return g;
+}
+template <typename S>
+bool is_f32_ssd(int data_term) {
+  return std::is_same<S, float>::value && data_term == msg::DT_SSD;
 }
 
 // doubles of the energy pass's partial sums for npairs pairs at grid size n (either data term)
@@ -241,7 +247,10 @@ size_t energy_part_size(int n, int npairs) {
   return (size_t)npairs * (msg::MAX_CAND * 4 + 2) * std::max(energy_geom(n).nblk, n);
 }
 // partial sums per (pair, candidate, term) of the energy pass of a data term at grid size n
-int energy_blocks(int n, int data_term) { return data_term == msg::DT_XSHIFT ? n : energy_geom(n).nblk; }
+template <typename S>
+int energy_blocks(int n, int data_term) {
+  return data_term == msg::DT_XSHIFT ? n : energy_geom(n, is_f32_ssd<S>(data_term)).nblk;
+}
 
 // The images of the data term: R, T (SSD, NCC) or, for DT_CONS, the npsi deformations in T.
 template <typename S>
@@ -249,7 +258,7 @@ int launch_energy(const S *R, const S *T, const S *phi, const S *z, const S *dol
                   double *part, int npairs, hipStream_t s, int data_term, double bytes = 0, int npsi = 0,
                   bool exact_pass = true, double yreg = 0) {
   ProfScope ps(KC_ENERGY, s, bytes);
-  const EnergyGeom g = energy_geom(n);
+  const EnergyGeom g = energy_geom(n, is_f32_ssd<S>(data_term));
   const double h = 1.0 / (double)(n - 1);
   const int width = (int)((n - 1) * h);
   if (data_term == msg::DT_XSHIFT)
@@ -262,8 +271,13 @@ int launch_energy(const S *R, const S *T, const S *phi, const S *z, const S *dol
     hipLaunchKernelGGL((msg::k_energy_ncc<S>), dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, z, dold, n, h,
                        st, part, g.ntx, g.nty, npairs);
   else if constexpr (std::is_same<S, float>::value) {
-    hipLaunchKernelGGL(msg::k_energy_f32, dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, z, dold, n, h * h, st,
-                       part, g.ntx, g.nty, npairs);
+    if (n > 16384) return fail(MSG_ERR_ARG, "FP32 fields: the SSD energy pass addresses grids up to 16384^2");
+    if (n <= 4096)
+      hipLaunchKernelGGL(msg::k_energy_f32<true>, dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, z, dold, n,
+                         h * h, st, part, g.ntx, g.nty, npairs);
+    else
+      hipLaunchKernelGGL(msg::k_energy_f32<false>, dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, z, dold, n,
+                         h * h, st, part, g.ntx, g.nty, npairs);
     if (exact_pass) {  // the pairs flagged for an FP64-arithmetic evaluation (none: a check per pair)
       HIPCHK(hipGetLastError());
       hipLaunchKernelGGL(msg::k_energy_exact_f32, dim3(std::min(g.nblk, 1024)), dim3(256), 0, s, R, T, phi, z, dold,
@@ -611,7 +625,7 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
   }
 
   msg::CtrlArgs a;
-  a.nblk_energy = energy_blocks(n, c->data_term);
+  a.nblk_energy = energy_blocks<S>(n, c->data_term);
   a.nblk_norm = nblk_norm;
   a.lambda = lambda;
   a.stop_eps = stop_eps;
@@ -745,7 +759,7 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
     HIPCHK(hipMemsetAsync(tr, 0, sizeof(msg::IterTrace) * trace_cap * B, s));
   }
   msg::CtrlArgs a;
-  a.nblk_energy = energy_blocks(n, c->data_term);
+  a.nblk_energy = energy_blocks<S>(n, c->data_term);
   a.nblk_norm = mg_norm_blocks(l, mgc_maxl<S>());
   a.lambda = lambda;
   a.stop_eps = stop_eps;
@@ -1255,7 +1269,7 @@ template <typename S>
 // E_NCC at phi).
 int energy_impl(const void *R, const void *T, const void *phi, int n, int batch, int data_term, double *e_data,
                        double *e_reg, hipStream_t s, double yreg = 0) {
-  const int nb = energy_blocks(n, data_term);
+  const int nb = energy_blocks<S>(n, data_term);
   void *stp = nullptr, *part = nullptr;
   CHK(scratch_get(0, sizeof(PairState) * batch, &stp));
   CHK(scratch_get(1, sizeof(double) * energy_part_size(n, batch), &part));
@@ -1439,8 +1453,9 @@ int noalloc_scratch(void **p) {
 
 // Bytes of scratch the no-allocation energy pass needs: pair states, the partials of candidate 0
 // (terms E_SSD and E_reg of every pair, laid out as the level solve's partials) and 2 sums per pair.
+template <typename S>
 size_t noalloc_energy_bytes(int n, int batch, size_t *off_part, size_t *off_sum) {
-  const int nblk = energy_geom(n).nblk;
+  const int nblk = energy_geom(n, is_f32_ssd<S>(msg::DT_SSD)).nblk;
   const size_t st = ((sizeof(PairState) * (size_t)batch + 255) / 256) * 256;
   const size_t part_doubles = ((size_t)(batch - 1) * msg::MAX_CAND * 2 + 2) * (size_t)nblk;
   const size_t part = ((sizeof(double) * part_doubles + 255) / 256) * 256;
@@ -1465,7 +1480,7 @@ template <typename S>
 int energy_noalloc_impl(const void *R, const void *T, const void *phi, int n, int batch, double *e_ssd,
                         double *e_reg, hipStream_t s) {
   size_t off_part = 0, off_sum = 0;
-  const size_t need = noalloc_energy_bytes(n, batch, &off_part, &off_sum);
+  const size_t need = noalloc_energy_bytes<S>(n, batch, &off_part, &off_sum);
   if (need > NOALLOC_SCRATCH_BYTES)
     return fail(MSG_ERR_ARG, "msg_energy_noalloc: " + std::to_string(batch) + " pairs of " + std::to_string(n) + "^2 need " +
                                  std::to_string(need) + " bytes of scratch, beyond its " +
@@ -1476,15 +1491,22 @@ int energy_noalloc_impl(const void *R, const void *T, const void *phi, int n, in
   PairState *st = reinterpret_cast<PairState *>(b);
   double *part = reinterpret_cast<double *>(b + off_part);
   double *sums = reinterpret_cast<double *>(b + off_sum);
-  const EnergyGeom g = energy_geom(n);
+  const EnergyGeom g = energy_geom(n, is_f32_ssd<S>(msg::DT_SSD));
   const double h = 1.0 / (double)(n - 1);
   // every word the pass reads is written first: states by k_state_init, partials by the pass
   hipLaunchKernelGGL(k_state_init, dim3(cdiv(batch, 64)), dim3(64), 0, s, st, batch, 0.0);
   HIPCHK(hipGetLastError());
-  if constexpr (std::is_same<S, float>::value)
-    hipLaunchKernelGGL(msg::k_energy_f32, dim3(g.nblk * batch), dim3(256), 0, s, (const float *)R, (const float *)T,
-                       (const float *)phi, (const float *)phi, (const float *)phi, n, h * h, st, part, g.ntx, g.nty,
-                       batch);
+  if constexpr (std::is_same<S, float>::value) {
+    if (n > 16384) return fail(MSG_ERR_ARG, "FP32 fields: the SSD energy pass addresses grids up to 16384^2");
+    if (n <= 4096)
+      hipLaunchKernelGGL(msg::k_energy_f32<true>, dim3(g.nblk * batch), dim3(256), 0, s, (const float *)R,
+                         (const float *)T, (const float *)phi, (const float *)phi, (const float *)phi, n, h * h, st,
+                         part, g.ntx, g.nty, batch);
+    else
+      hipLaunchKernelGGL(msg::k_energy_f32<false>, dim3(g.nblk * batch), dim3(256), 0, s, (const float *)R,
+                         (const float *)T, (const float *)phi, (const float *)phi, (const float *)phi, n, h * h, st,
+                         part, g.ntx, g.nty, batch);
+  }
   else
     hipLaunchKernelGGL((msg::k_energy<S>), dim3(g.nblk * batch), dim3(256), 0, s, (const S *)R, (const S *)T,
                        (const S *)phi, (const S *)phi, (const S *)phi, n, h, (int)((n - 1) * h), st, part, g.ntx,

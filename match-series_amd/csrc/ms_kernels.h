@@ -569,83 +569,77 @@ __device__ __forceinline__ bool participates(const PairState &s) {
 // assembled element by element. Output: per block partial sums (ssd, reg) per candidate.
 // ================================================================================================
 constexpr int EN_TX = 64, EN_TY = 4;  // elements per workgroup tile (one per thread)
+// The FP32 SSD pass (k_energy_f32, and its FP64 re-evaluation) takes tiles of 64 x 16 elements, a
+// vertical run of 4 elements per thread: the workgroup's reduction and partial sums are shared by 4
+// elements, and a run reads 5 node rows instead of 8.
+constexpr int EN32_TY = 16, EN32_RUN = EN32_TY / 4;
 
-// One workgroup's tile of the pass for one pair (also the FP64-arithmetic re-evaluation of the FP32
-// pass, which calls it with S = float).
+// The terms of element (ex, ey) of one pair for each candidate step size, added to accS (SSD) and
+// accR (regulariser): FP64 arithmetic at the stored positions (step_at). Used by the FP64 pass and by
+// the FP64-arithmetic re-evaluation of the FP32 pass (S = float).
 template <typename S>
-__device__ __forceinline__ void energy_tile(const S *__restrict__ R, const S *__restrict__ T,
-                                            const S *__restrict__ phi, const S *__restrict__ z,
-                                            const S *__restrict__ dold, int n, double h, int width,
-                                            const PairState *st, double *__restrict__ part, int ntx, int nty,
-                                            int pair, int tile) {
-  const int tiles = ntx * nty;
-  const int nc = st[pair].ncand;
-  const int dm = st[pair].dirmode;
-  __shared__ double red[4];
-
-  const size_t nn = (size_t)n * n;
-  const S *Rp = R + pair * nn, *Tp = T + pair * nn;
-  const S *px = phi + pair * 2 * nn, *py = px + nn;
-  const S *zx = z + pair * 2 * nn, *zy = zx + nn, *ox_ = dold + pair * 2 * nn, *oy_ = ox_ + nn;
-  const bool cg = (dm == DIR_CG), use_old = st[pair].cg_old != 0;
-  const double beta = st[pair].beta;
+__device__ __forceinline__ void energy_elem(const S *__restrict__ Rp, const S *__restrict__ Tp,
+                                            const S *__restrict__ px, const S *__restrict__ py,
+                                            const S *__restrict__ zx, const S *__restrict__ zy,
+                                            const S *__restrict__ ox_, const S *__restrict__ oy_, int n, double h,
+                                            int width, const PairState &sp, int ex, int ey, double (&accS)[MAX_CAND],
+                                            double (&accR)[MAX_CAND]) {
+  const int nc = sp.ncand;
+  const int dm = sp.dirmode;
+  const bool cg = (dm == DIR_CG), use_old = sp.cg_old != 0;
+  const double beta = sp.beta;
   const bool use_dir = (dm != DIR_NONE);
-
-  const int ex = (tile % ntx) * EN_TX + (threadIdx.x % EN_TX);
-  const int ey = (tile / ntx) * EN_TY + (threadIdx.x / EN_TX);
-  const bool valid = (ex < n - 1) && (ey < n - 1);
-
-  double accS[MAX_CAND], accR[MAX_CAND];
+  double f0[4], f1[4], d0[4], d1[4], rq[4];
+  double rn[4];
 #pragma unroll
-  for (int k = 0; k < MAX_CAND; ++k) accS[k] = accR[k] = 0.0;
-
-  if (valid) {
-    double f0[4], f1[4], d0[4], d1[4], rq[4];
-    double rn[4];
+  for (int b = 0; b < 4; ++b) {
+    const int g = l2g(ex, ey, b, n);
+    f0[b] = ld(px, g);
+    f1[b] = ld(py, g);
+    d0[b] = use_dir ? (double)search_dir<S>(zx, ox_, g, cg, use_old, beta) : 0.0;
+    d1[b] = use_dir ? (double)search_dir<S>(zy, oy_, g, cg, use_old, beta) : 0.0;
+    rn[b] = ld(Rp, g);
+  }
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int g = l2g(ex, ey, b, n);
-      f0[b] = ld(px, g);
-      f1[b] = ld(py, g);
-      d0[b] = use_dir ? (double)search_dir<S>(zx, ox_, g, cg, use_old, beta) : 0.0;
-      d1[b] = use_dir ? (double)search_dir<S>(zy, oy_, g, cg, use_old, beta) : 0.0;
-      rn[b] = ld(Rp, g);
-    }
+  for (int q = 0; q < 4; ++q) {
+    double w = 0.;
+    for (int b = 0; b < 4; ++b) w += rn[b] * bq(b, q);
+    rq[q] = w;
+  }
+  const double vol = h * h;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      double w = 0.;
-      for (int b = 0; b < 4; ++b) w += rn[b] * bq(b, q);
-      rq[q] = w;
-    }
-    const double vol = h * h;
-#pragma unroll
-    for (int k = 0; k < MAX_CAND; ++k) {
-      if (k < nc) {
-        const double tau = st[pair].cand[k];
-        double p0[4], p1[4];
-        for (int b = 0; b < 4; ++b) {
-          p0[b] = use_dir ? (double)step_at<S>((S)f0[b], tau, (S)d0[b]) : f0[b];
-          p1[b] = use_dir ? (double)step_at<S>((S)f1[b], tau, (S)d1[b]) : f1[b];
-        }
-        double a = 0.;
-        for (int q = 0; q < 4; ++q) {
-          double ox = 0., oy = 0.;
-          for (int b = 0; b < 4; ++b) {
-            ox += p0[b] * bq(b, q);
-            oy += p1[b] * bq(b, q);
-          }
-          const Hit hp = deform_qp(ex, ey, q, ox, oy, n, h, width);
-          double t4[4];
-          const double t = eval_img(Tp, hp, n, t4) - rq[q];
-          a += 0.5 * (t * t) * 0.25;
-        }
-        a *= vol;
-        accS[k] = a;
-        accR[k] = 0.5 * (stiff_form(p0) + stiff_form(p1));
+  for (int k = 0; k < MAX_CAND; ++k) {
+    if (k < nc) {
+      const double tau = sp.cand[k];
+      double p0[4], p1[4];
+      for (int b = 0; b < 4; ++b) {
+        p0[b] = use_dir ? (double)step_at<S>((S)f0[b], tau, (S)d0[b]) : f0[b];
+        p1[b] = use_dir ? (double)step_at<S>((S)f1[b], tau, (S)d1[b]) : f1[b];
       }
+      double a = 0.;
+      for (int q = 0; q < 4; ++q) {
+        double ox = 0., oy = 0.;
+        for (int b = 0; b < 4; ++b) {
+          ox += p0[b] * bq(b, q);
+          oy += p1[b] * bq(b, q);
+        }
+        const Hit hp = deform_qp(ex, ey, q, ox, oy, n, h, width);
+        double t4[4];
+        const double t = eval_img(Tp, hp, n, t4) - rq[q];
+        a += 0.5 * (t * t) * 0.25;
+      }
+      a *= vol;
+      accS[k] += a;
+      accR[k] += 0.5 * (stiff_form(p0) + stiff_form(p1));
     }
   }
-  const int nblk = tiles;
+}
+
+// Sums of accS / accR over the workgroup written as the partials of block `tile` (nblk per
+// candidate and term) of one pair.
+__device__ __forceinline__ void energy_write_partials(double (&accS)[MAX_CAND], double (&accR)[MAX_CAND], int nc,
+                                                      double *__restrict__ part, int pair, int tile, int nblk) {
+  __shared__ double red[4];
 #pragma unroll
   for (int k = 0; k < MAX_CAND; ++k) {  // static indices keep acc* in registers; nc is uniform
     if (k < nc) {
@@ -657,6 +651,27 @@ __device__ __forceinline__ void energy_tile(const S *__restrict__ R, const S *__
       }
     }
   }
+}
+
+// One workgroup's tile of the FP64 pass for one pair: one element per thread.
+template <typename S>
+__device__ __forceinline__ void energy_tile(const S *__restrict__ R, const S *__restrict__ T,
+                                            const S *__restrict__ phi, const S *__restrict__ z,
+                                            const S *__restrict__ dold, int n, double h, int width,
+                                            const PairState *st, double *__restrict__ part, int ntx, int nty,
+                                            int pair, int tile) {
+  const size_t nn = (size_t)n * n;
+  const S *Rp = R + pair * nn, *Tp = T + pair * nn;
+  const S *px = phi + pair * 2 * nn, *py = px + nn;
+  const S *zx = z + pair * 2 * nn, *zy = zx + nn, *ox_ = dold + pair * 2 * nn, *oy_ = ox_ + nn;
+  const int ex = (tile % ntx) * EN_TX + (threadIdx.x % EN_TX);
+  const int ey = (tile / ntx) * EN_TY + (threadIdx.x / EN_TX);
+  double accS[MAX_CAND], accR[MAX_CAND];
+#pragma unroll
+  for (int k = 0; k < MAX_CAND; ++k) accS[k] = accR[k] = 0.0;
+  if ((ex < n - 1) && (ey < n - 1))
+    energy_elem<S>(Rp, Tp, px, py, zx, zy, ox_, oy_, n, h, width, st[pair], ex, ey, accS, accR);
+  energy_write_partials(accS, accR, st[pair].ncand, part, pair, tile, ntx * nty);
 }
 
 template <typename S>

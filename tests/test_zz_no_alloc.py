@@ -45,10 +45,10 @@ def test_noalloc_energy_refuses_oversized_batches():
     n = 2048
     R = torch.zeros((1, n, n), dtype=torch.float32, device="cuda:0")
     phi = torch.zeros((1, 2, n, n), dtype=torch.float32, device="cuda:0")
-    es = np.zeros(64)
-    # 64 pairs of 2048^2 exceed the 32 MiB array; nothing is launched (the pointers are only checked
-    # for NULL, and the size check comes first)
-    rc = gpu.lib().msg_energy_noalloc(gpu.F32, gpu._ptr(R), gpu._ptr(R), gpu._ptr(phi), n, 64,
+    es = np.zeros(256)
+    # 256 pairs of 2048^2 exceed the 32 MiB array (their partial sums alone are 64 MiB); nothing is
+    # launched (the pointers are only checked for NULL, and the size check comes first)
+    rc = gpu.lib().msg_energy_noalloc(gpu.F32, gpu._ptr(R), gpu._ptr(R), gpu._ptr(phi), n, 256,
                                       es.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), None, None)
     assert rc == 1
     assert b"scratch" in gpu.lib().msg_last_error()
