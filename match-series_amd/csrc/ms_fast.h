@@ -428,24 +428,23 @@ __device__ __forceinline__ float heat_apply_f(const Get &get, int x, int y, int 
   if (n == 1) return get(0, 0);
   const float u = get(x, y);
   const int e = n - 1;
-  const bool bx0 = (x == 0), bx1 = (x == e), by0 = (y == 0), by1 = (y == e);
-  float lap;
-  if (!(bx0 || bx1 || by0 || by1)) {
-    lap = ((u - get(x + 1, y)) + (u - get(x - 1, y))) + ((u - get(x, y + 1)) + (u - get(x, y - 1)));
-  } else if ((bx0 || bx1) && (by0 || by1)) {  // corner: the inward neighbours count twice
-    lap = 2.f * ((u - get(x + (bx0 ? 1 : -1), y)) + (u - get(x, y + (by0 ? 1 : -1))));
-  } else if (by0 || by1) {  // edge parallel to the x axis
-    lap = 2.f * (u - get(x, y + (by0 ? 1 : -1))) + ((u - get(x + 1, y)) + (u - get(x - 1, y)));
-  } else {  // edge parallel to the y axis
-    lap = 2.f * (u - get(x + (bx0 ? 1 : -1), y)) + ((u - get(x, y + 1)) + (u - get(x, y - 1)));
-  }
+  // Neighbours outside the grid are mirrored inward, so on an edge the inward difference appears
+  // twice (u - v) + (u - v) = 2 (u - v) exactly, and every node takes the same branch-free form:
+  // interior ((u - r) + (u - l)) + ((u - t) + (u - b)); edge, e.g. y = 0, 2 (u - t) + ((u - r) + (u - l))
+  // summed in the other order (FP addition commutes); corner 2 (u - r) + 2 (u - t) = 2 ((u - r) +
+  // (u - t)) exactly. A wave covering a grid edge then runs one path instead of three.
+  const int xl = x > 0 ? x - 1 : x + 1, xr = x < e ? x + 1 : x - 1;
+  const int yb = y > 0 ? y - 1 : y + 1, yt = y < e ? y + 1 : y - 1;
+  const float lap = ((u - get(xr, y)) + (u - get(xl, y))) + ((u - get(x, yt)) + (u - get(x, yb)));
   return hs * lap + u;
 }
 
+template <int TX>
 __global__ void __launch_bounds__(256)
 k_mg_down_f32(const float *__restrict__ rhs, int negate, float *__restrict__ x2out, float *__restrict__ rhsc, int n,
               const PairState *st, double sigma_default, int ntx, int nty, int nfields) {
 #pragma clang fp contract(fast)
+  constexpr int HX = TX + 4;
   const int tiles = ntx * nty;
   const int wg = xcd_remap(blockIdx.x, tiles * nfields);
   const int field = wg / tiles, tile = wg - field * tiles;
@@ -454,18 +453,18 @@ k_mg_down_f32(const float *__restrict__ rhs, int negate, float *__restrict__ x2o
   float hs, idg;
   level_consts_f(st, pair, sigma_default, n, hs, idg);
 
-  __shared__ float sr[MG_HY][MG_HX];    // rhs on tile+2
-  __shared__ float s1[MG_HY][MG_HX];    // x1 on tile+2
-  __shared__ float s2[MG_HY][MG_HX];    // x2 on tile+1
-  __shared__ float sres[MG_TY][MG_TX];  // residual on tile
+  __shared__ float sr[MG32_HY][HX];    // rhs on tile+2
+  __shared__ float s1[MG32_HY][HX];    // x1 on tile+2
+  __shared__ float s2[MG32_HY][HX];    // x2 on tile+1
+  __shared__ float sres[MG32_TY][TX];  // residual on tile
   const size_t nn = (size_t)n * n;
   const float *rp = rhs + field * nn;
-  const int x0 = (tile % ntx) * MG_TX, y0 = (tile / ntx) * MG_TY;
+  const int x0 = (tile % ntx) * TX, y0 = (tile / ntx) * MG32_TY;
   const int bx = x0 - 2, by = y0 - 2;
   const float sg = negate ? -1.f : 1.f;
 
-  for (int i = threadIdx.x; i < MG_HX * MG_HY; i += 256) {
-    const int lx = i % MG_HX, ly = i / MG_HX;
+  for (int i = threadIdx.x; i < HX * MG32_HY; i += 256) {
+    const int lx = i % HX, ly = i / HX;
     const int gx = bx + lx, gy = by + ly;
     if (gx >= 0 && gy >= 0 && gx < n && gy < n) {
       const float v = sg * rp[gx + gy * n];
@@ -475,17 +474,17 @@ k_mg_down_f32(const float *__restrict__ rhs, int negate, float *__restrict__ x2o
   }
   __syncthreads();
   auto get1 = [&](int gx, int gy) { return s1[gy - by][gx - bx]; };
-  for (int i = threadIdx.x; i < MG_HX * MG_HY; i += 256) {
-    const int lx = i % MG_HX, ly = i / MG_HX;
+  for (int i = threadIdx.x; i < HX * MG32_HY; i += 256) {
+    const int lx = i % HX, ly = i / HX;
     const int gx = bx + lx, gy = by + ly;
-    if (lx >= 1 && ly >= 1 && lx < MG_HX - 1 && ly < MG_HY - 1 && gx >= 0 && gy >= 0 && gx < n && gy < n)
+    if (lx >= 1 && ly >= 1 && lx < HX - 1 && ly < MG32_HY - 1 && gx >= 0 && gy >= 0 && gx < n && gy < n)
       s2[ly][lx] = (sr[ly][lx] - heat_apply_f(get1, gx, gy, n, hs)) * idg + s1[ly][lx];
   }
   __syncthreads();
   auto get2 = [&](int gx, int gy) { return s2[gy - by][gx - bx]; };
   float *xo = x2out + field * nn;
-  for (int i = threadIdx.x; i < MG_TX * MG_TY; i += 256) {
-    const int tx = i % MG_TX, ty = i / MG_TX;
+  for (int i = threadIdx.x; i < TX * MG32_TY; i += 256) {
+    const int tx = i % TX, ty = i / TX;
     const int gx = x0 + tx, gy = y0 + ty;
     if (gx < n && gy < n) {
       sres[ty][tx] = sr[ty + 2][tx + 2] - heat_apply_f(get2, gx, gy, n, hs);
@@ -495,8 +494,8 @@ k_mg_down_f32(const float *__restrict__ rhs, int negate, float *__restrict__ x2o
   __syncthreads();
   const int nc = n / 2;
   float *rc = rhsc + field * (size_t)nc * nc;
-  for (int i = threadIdx.x; i < (MG_TX / 2) * (MG_TY / 2); i += 256) {
-    const int cx = i % (MG_TX / 2), cy = i / (MG_TX / 2);
+  for (int i = threadIdx.x; i < (TX / 2) * (MG32_TY / 2); i += 256) {
+    const int cx = i % (TX / 2), cy = i / (TX / 2);
     const int gcx = x0 / 2 + cx, gcy = y0 / 2 + cy;
     if (gcx < nc && gcy < nc)
       rc[gcx + gcy * nc] =
@@ -504,11 +503,13 @@ k_mg_down_f32(const float *__restrict__ rhs, int negate, float *__restrict__ x2o
   }
 }
 
+template <int TX>
 __global__ void __launch_bounds__(256)
 k_mg_up_f32(const float *__restrict__ x2in, const float *__restrict__ xc, const float *__restrict__ rhs, int negate,
             float *__restrict__ xout, int n, const PairState *st, double sigma_default, int ntx, int nty, int nfields,
             const float *__restrict__ dold, double *__restrict__ nrm_part) {
 #pragma clang fp contract(fast)
+  constexpr int HX = TX + 4;
   const int tiles = ntx * nty;
   const int wg = xcd_remap(blockIdx.x, tiles * nfields);
   const int field = wg / tiles, tile = wg - field * tiles;
@@ -517,28 +518,28 @@ k_mg_up_f32(const float *__restrict__ x2in, const float *__restrict__ xc, const 
   float hs, idg;
   level_consts_f(st, pair, sigma_default, n, hs, idg);
 
-  __shared__ float sx[MG_HY][MG_HX];  // x on tile+2
-  __shared__ float sr[MG_HY][MG_HX];  // rhs on tile+1
-  __shared__ float s3[MG_HY][MG_HX];  // x3 on tile+1
+  __shared__ float sx[MG32_HY][HX];  // x on tile+2
+  __shared__ float sr[MG32_HY][HX];  // rhs on tile+1
+  __shared__ float s3[MG32_HY][HX];  // x3 on tile+1
   __shared__ double red[4 * NORM_TERMS];
   const size_t nn = (size_t)n * n;
   const int nc = n / 2;
   const float *x2p = x2in + field * nn, *rp = rhs + field * nn, *xcp = xc + field * (size_t)nc * nc;
-  const int x0 = (tile % ntx) * MG_TX, y0 = (tile / ntx) * MG_TY;
+  const int x0 = (tile % ntx) * TX, y0 = (tile / ntx) * MG32_TY;
   const int bx = x0 - 2, by = y0 - 2;
   const float sg = negate ? -1.f : 1.f;
   // the previous direction at this thread's output nodes, loaded now so the load overlaps the sweeps
-  constexpr int OUT_PER_THREAD = MG_TX * MG_TY / 256;
+  constexpr int OUT_PER_THREAD = TX * MG32_TY / 256;
   float dprev[OUT_PER_THREAD];
 #pragma unroll
   for (int k = 0; k < OUT_PER_THREAD; ++k) {
     const int i = threadIdx.x + 256 * k;
-    const int gx = x0 + i % MG_TX, gy = y0 + i / MG_TX;
+    const int gx = x0 + i % TX, gy = y0 + i / TX;
     dprev[k] = (nrm_part && gx < n && gy < n) ? dold[field * nn + gx + gy * n] : 0.f;
   }
 
-  for (int i = threadIdx.x; i < MG_HX * MG_HY; i += 256) {
-    const int lx = i % MG_HX, ly = i / MG_HX;
+  for (int i = threadIdx.x; i < HX * MG32_HY; i += 256) {
+    const int lx = i % HX, ly = i / HX;
     const int gx = bx + lx, gy = by + ly;
     if (gx >= 0 && gy >= 0 && gx < n && gy < n) {
       sx[ly][lx] = x2p[gx + gy * n] + xcp[(gx >> 1) + (gy >> 1) * nc];
@@ -547,10 +548,10 @@ k_mg_up_f32(const float *__restrict__ x2in, const float *__restrict__ xc, const 
   }
   __syncthreads();
   auto getx = [&](int gx, int gy) { return sx[gy - by][gx - bx]; };
-  for (int i = threadIdx.x; i < MG_HX * MG_HY; i += 256) {
-    const int lx = i % MG_HX, ly = i / MG_HX;
+  for (int i = threadIdx.x; i < HX * MG32_HY; i += 256) {
+    const int lx = i % HX, ly = i / HX;
     const int gx = bx + lx, gy = by + ly;
-    if (lx >= 1 && ly >= 1 && lx < MG_HX - 1 && ly < MG_HY - 1 && gx >= 0 && gy >= 0 && gx < n && gy < n)
+    if (lx >= 1 && ly >= 1 && lx < HX - 1 && ly < MG32_HY - 1 && gx >= 0 && gy >= 0 && gx < n && gy < n)
       s3[ly][lx] = (sr[ly][lx] - heat_apply_f(getx, gx, gy, n, hs)) * idg + sx[ly][lx];
   }
   __syncthreads();
@@ -560,7 +561,7 @@ k_mg_up_f32(const float *__restrict__ x2in, const float *__restrict__ xc, const 
 #pragma unroll
   for (int k = 0; k < OUT_PER_THREAD; ++k) {
     const int i = threadIdx.x + 256 * k;
-    const int tx = i % MG_TX, ty = i / MG_TX;
+    const int tx = i % TX, ty = i / TX;
     const int gx = x0 + tx, gy = y0 + ty;
     if (gx < n && gy < n) {
       const float o = (sr[ty + 2][tx + 2] - heat_apply_f(get3, gx, gy, n, hs)) * idg + s3[ty + 2][tx + 2];
@@ -585,7 +586,8 @@ k_mg_up_f32(const float *__restrict__ x2in, const float *__restrict__ xc, const 
 // MGC_THREADS threads (k_mg_coarse, one level deeper: FP32 arrays of level 6 and below fit in LDS).
 // The Jacobi sweeps write a second buffer instead of updating in place, the residual is restricted as
 // it is formed and the prolongation is read on the fly by the first sweep of the up leg, so a level
-// costs 3 barriers down and 2 up.
+// costs 3 barriers down and 2 up. Node coordinates come from shifts and masks (a division by the
+// runtime level size would cost an integer-division sequence per node).
 constexpr int MGC_THREADS = 1024;
 constexpr int MGC_MAXL_F32 = 6;
 
@@ -625,11 +627,11 @@ k_mg_coarse_f32(const float *__restrict__ rhs, int negate, float *__restrict__ x
     for (int i = tid; i < mm; i += nthr) Y[i] = b[i] * idg;  // first sweep from zero, into Y
     sync();
     auto gy = [&](int gx, int gyy) { return Y[gx + gyy * m]; };
-    for (int i = tid; i < mm; i += nthr) x[i] = (b[i] - heat_apply_f(gy, i % m, i / m, m, hs)) * idg + Y[i];
+    for (int i = tid; i < mm; i += nthr) x[i] = (b[i] - heat_apply_f(gy, i & (m - 1), i >> l, m, hs)) * idg + Y[i];
     sync();
     auto gx2 = [&](int px, int py) { return x[px + py * m]; };
     for (int i = tid; i < mc * mc; i += nthr) {  // residual at the 4 children of a coarse node, averaged
-      const int cx = i % mc, cy = i / mc;
+      const int cx = i & (mc - 1), cy = i >> (l - 1);
       float r[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
@@ -651,12 +653,12 @@ k_mg_coarse_f32(const float *__restrict__ rhs, int negate, float *__restrict__ x
     const float *b = B + MGC_OFF[l];
     auto gp = [&](int px, int py) { return x[px + py * m] + xc[(px >> 1) + (py >> 1) * mc]; };
     for (int i = tid; i < mm; i += nthr) {
-      const int px = i % m, py = i / m;
+      const int px = i & (m - 1), py = i >> l;
       Y[i] = (b[i] - heat_apply_f(gp, px, py, m, hs)) * idg + gp(px, py);
     }
     sync();
     auto gy = [&](int gx, int gyy) { return Y[gx + gyy * m]; };
-    for (int i = tid; i < mm; i += nthr) x[i] = (b[i] - heat_apply_f(gy, i % m, i / m, m, hs)) * idg + Y[i];
+    for (int i = tid; i < mm; i += nthr) x[i] = (b[i] - heat_apply_f(gy, i & (m - 1), i >> l, m, hs)) * idg + Y[i];
     sync();
   };
   auto block_sync = [] { __syncthreads(); };

@@ -353,10 +353,20 @@ constexpr int mgc_maxl() {
 }
 
 // Partial sums per field and term of the V-cycle's last pass at level l.
+// Tiles of the V-cycle's fine passes at grid size n for a field type
+template <typename S>
+constexpr int mg_tx(int n) {
+  return !std::is_same<S, float>::value ? msg::MG_TX : (n >= msg::MG32_WIDE_FROM ? msg::MG32_TX_WIDE : 32);
+}
+template <typename S>
+constexpr int mg_ty() {
+  return std::is_same<S, float>::value ? msg::MG32_TY : msg::MG_TY;
+}
+template <typename S = double>
 int mg_norm_blocks(int l, int coarse_max) {
   if (l <= coarse_max) return 1;
   const int n = 1 << l;
-  return cdiv(n, msg::MG_TX) * cdiv(n, msg::MG_TY);
+  return cdiv(n, mg_tx<S>(n)) * cdiv(n, mg_ty<S>());
 }
 
 // The V-cycle kernels of a field type: FP32 arithmetic for FP32 fields, FP64 for FP64 fields.
@@ -373,9 +383,14 @@ void mg_coarse(dim3 g, hipStream_t s, const S *rhs, int negate, S *xout, int L, 
 template <typename S>
 void mg_down(dim3 g, hipStream_t s, const S *rhs, int negate, S *x2out, S *rhsc, int n, const PairState *st,
              double sigma, int ntx, int nty, int nfields) {
-  if constexpr (std::is_same<S, float>::value)
-    hipLaunchKernelGGL(msg::k_mg_down_f32, g, dim3(256), 0, s, rhs, negate, x2out, rhsc, n, st, sigma, ntx, nty,
-                       nfields);
+  if constexpr (std::is_same<S, float>::value) {
+    if (mg_tx<S>(n) == msg::MG32_TX_WIDE)
+      hipLaunchKernelGGL(msg::k_mg_down_f32<msg::MG32_TX_WIDE>, g, dim3(256), 0, s, rhs, negate, x2out, rhsc, n, st,
+                         sigma, ntx, nty, nfields);
+    else
+      hipLaunchKernelGGL(msg::k_mg_down_f32<32>, g, dim3(256), 0, s, rhs, negate, x2out, rhsc, n, st, sigma, ntx, nty,
+                         nfields);
+  }
   else
     hipLaunchKernelGGL((msg::k_mg_down<S>), g, dim3(256), 0, s, rhs, negate, x2out, rhsc, n, st, sigma, ntx, nty,
                        nfields);
@@ -383,9 +398,14 @@ void mg_down(dim3 g, hipStream_t s, const S *rhs, int negate, S *x2out, S *rhsc,
 template <typename S>
 void mg_up(dim3 g, hipStream_t s, const S *x2in, const S *xc, const S *rhs, int negate, S *xout, int n,
            const PairState *st, double sigma, int ntx, int nty, int nfields, const S *dold, double *nrm_part) {
-  if constexpr (std::is_same<S, float>::value)
-    hipLaunchKernelGGL(msg::k_mg_up_f32, g, dim3(256), 0, s, x2in, xc, rhs, negate, xout, n, st, sigma, ntx, nty,
-                       nfields, dold, nrm_part);
+  if constexpr (std::is_same<S, float>::value) {
+    if (mg_tx<S>(n) == msg::MG32_TX_WIDE)
+      hipLaunchKernelGGL(msg::k_mg_up_f32<msg::MG32_TX_WIDE>, g, dim3(256), 0, s, x2in, xc, rhs, negate, xout, n, st,
+                         sigma, ntx, nty, nfields, dold, nrm_part);
+    else
+      hipLaunchKernelGGL(msg::k_mg_up_f32<32>, g, dim3(256), 0, s, x2in, xc, rhs, negate, xout, n, st, sigma, ntx,
+                         nty, nfields, dold, nrm_part);
+  }
   else
     hipLaunchKernelGGL((msg::k_mg_up<S>), g, dim3(256), 0, s, x2in, xc, rhs, negate, xout, n, st, sigma, ntx, nty,
                        nfields, dold, nrm_part);
@@ -406,7 +426,7 @@ int launch_vcycle(const S *in, int negate, S *out, int l, const MGBufs<S> &b, co
   // down leg: levels l .. CM+1
   for (int m = l; m > CM; --m) {
     const int n = 1 << m;
-    const int ntx = cdiv(n, msg::MG_TX), nty = cdiv(n, msg::MG_TY);
+    const int ntx = cdiv(n, mg_tx<S>(n)), nty = cdiv(n, mg_ty<S>());
     const S *src = (m == l) ? in : b.rhs[m];
     const int neg = (m == l) ? negate : 0;
     S *x2 = (m == l) ? b.x2_top : b.x2[m];
@@ -418,7 +438,7 @@ int launch_vcycle(const S *in, int negate, S *out, int l, const MGBufs<S> &b, co
   HIPCHK(hipGetLastError());
   for (int m = CM + 1; m <= l; ++m) {
     const int n = 1 << m;
-    const int ntx = cdiv(n, msg::MG_TX), nty = cdiv(n, msg::MG_TY);
+    const int ntx = cdiv(n, mg_tx<S>(n)), nty = cdiv(n, mg_ty<S>());
     const S *x2 = (m == l) ? b.x2_top : b.x2[m];
     const S *rhs = (m == l) ? in : b.rhs[m];
     const int neg = (m == l) ? negate : 0;
@@ -606,7 +626,7 @@ int solve_level_impl(msg_ctx *c, int l, double lambda, int max_iter, double stop
   S *DE = (S *)c->DE, *z = (S *)c->z, *d = (S *)c->d;
   const MGBufs<S> mg = ctx_mg<S>(c);
   const EnergyGeom eg = energy_geom(n);
-  const int nblk_norm = mg_norm_blocks(l, mgc_maxl<S>());
+  const int nblk_norm = mg_norm_blocks<S>(l, mgc_maxl<S>());
   const int nblk_upd = cdiv(2 * n * n, 256 * msg::UPD_PER_THREAD);
   if (spec <= 0) spec = 3;
   if (spec > msg::MAX_CAND) spec = msg::MAX_CAND;
@@ -760,7 +780,7 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
   }
   msg::CtrlArgs a;
   a.nblk_energy = energy_blocks<S>(n, c->data_term);
-  a.nblk_norm = mg_norm_blocks(l, mgc_maxl<S>());
+  a.nblk_norm = mg_norm_blocks<S>(l, mgc_maxl<S>());
   a.lambda = lambda;
   a.stop_eps = stop_eps;
   a.max_iter = max_iter;

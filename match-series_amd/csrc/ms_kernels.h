@@ -1093,6 +1093,11 @@ k_grad(const S *__restrict__ R, const S *__restrict__ T, const S *__restrict__ p
 // ================================================================================================
 constexpr int MG_TX = 32, MG_TY = 16;
 constexpr int MG_HX = MG_TX + 4, MG_HY = MG_TY + 4;
+// FP32 fields: tiles of 64 x 16 nodes (4 per thread) on levels of 1024^2 and above -- 512 workgroups
+// per field at 1024^2, resident at once, and a halo of 1.33x the tile instead of 1.41x -- and of
+// 32 x 16 below, where a workgroup's latency, not the level's traffic, sets the pass's time.
+constexpr int MG32_TY = 16, MG32_HY = MG32_TY + 4;
+constexpr int MG32_TX_WIDE = 64, MG32_WIDE_FROM = 1024;
 
 __device__ __forceinline__ void level_consts(const PairState *st, int pair, double sigma_default, int n, double &hs,
                                              double &diag) {
@@ -1279,7 +1284,7 @@ k_mg_coarse(const S *__restrict__ rhs, int negate, S *__restrict__ xout, int L, 
     double *x = X + MGC_OFF[l];
     const double *b = B + MGC_OFF[l];
     auto get = [&](int gx, int gy) { return x[gx + gy * m]; };
-    for (int i = threadIdx.x; i < m * m; i += 256) D[i] = heat_apply(get, i % m, i / m, m, hs);
+    for (int i = threadIdx.x; i < m * m; i += 256) D[i] = heat_apply(get, i & (m - 1), i >> l, m, hs);
     __syncthreads();
     for (int i = threadIdx.x; i < m * m; i += 256) {
       const double v = b[i] - D[i];
@@ -1298,11 +1303,11 @@ k_mg_coarse(const S *__restrict__ rhs, int negate, S *__restrict__ xout, int L, 
     const double s = (double)((m - 1) * (m - 1));
     const double hs = tau * s;
     auto get = [&](int gx, int gy) { return x[gx + gy * m]; };
-    for (int i = threadIdx.x; i < m * m; i += 256) D[i] = B[MGC_OFF[l] + i] - heat_apply(get, i % m, i / m, m, hs);
+    for (int i = threadIdx.x; i < m * m; i += 256) D[i] = B[MGC_OFF[l] + i] - heat_apply(get, i & (m - 1), i >> l, m, hs);
     __syncthreads();
     const int mc = m / 2;
     for (int i = threadIdx.x; i < mc * mc; i += 256) {
-      const int cx = i % mc, cy = i / mc;
+      const int cx = i & (mc - 1), cy = i >> (l - 1);
       double v = 0;
       v += D[2 * cx + 2 * cy * m];
       v += D[2 * cx + 1 + 2 * cy * m];
@@ -1326,7 +1331,7 @@ k_mg_coarse(const S *__restrict__ rhs, int negate, S *__restrict__ xout, int L, 
     const int m = 1 << l, mc = m / 2;
     double *x = X + MGC_OFF[l];
     const double *xcs = X + MGC_OFF[l - 1];
-    for (int i = threadIdx.x; i < m * m; i += 256) x[i] += xcs[((i % m) >> 1) + ((i / m) >> 1) * mc];
+    for (int i = threadIdx.x; i < m * m; i += 256) x[i] += xcs[((i & (m - 1)) >> 1) + ((i >> l) >> 1) * mc];
     __syncthreads();
     sweep(l);
     sweep(l);
