@@ -28,73 +28,80 @@ enum CtrlPhase { PH_LEVEL_INIT = 0, PH_ARMIJO_INIT = 2, PH_ARMIJO_STEP = 3, PH_N
 constexpr int CTRL_THREADS = 1024;  // threads of a control workgroup
 constexpr int CTRL_MAXT = 16;       // partial arrays one control sweep can reduce
 
-// Sums of NT (<= 8) consecutive partial arrays base[t * len .. t * len + len) in one sweep of the
-// control workgroup. The partials were just written by a kernel on other XCDs, so a load is a memory
-// round trip: the loads of the sweep (CTRL_RUN per thread and array) are unconditional, from clamped
-// indices, with out-of-range values dropped by a select, so none waits for another. Each wave sums
-// its lanes; wave 0 then sums the 16 wave results of every array with lane-group shuffles. Fixed
-// order; the sums land in out[0 .. NT) (shared memory), valid for every thread after the call.
-constexpr int CTRL_RUN = 4;  // one sweep covers the 4096 partials of a 1024^2 level
-constexpr int CTRL_WAVES = CTRL_THREADS / 64;
-static_assert(CTRL_WAVES == 16, "the final sum assumes 16 waves");
+// Sums of NT (<= 8) consecutive partial arrays base[t * len .. t * len + len) in the control
+// workgroup, in a fixed order. The partials were just written by a kernel on other XCDs, so a load is
+// a memory round trip: 256 threads issue all their loads (CTRL_RUN per array and sweep) before
+// summing, from clamped indices with out-of-range values dropped by a select, so none waits for
+// another. Each thread's sums go to LDS; 16 threads per array then sum 16 of them each, and one
+// thread per array the 16 results -- plain LDS reads and adds (cross-lane shuffles of doubles by 16
+// waves at once cost more than the loads). The sums land in out[0 .. NT) (shared memory), valid for
+// every thread after the call; red / red2 are the caller's LDS blocks of RED_THREADS and 16 doubles
+// per array.
+constexpr int CTRL_RUN = 4;
+constexpr int RED_THREADS = 256;
 template <int NT>
-__device__ __forceinline__ void reduce_arrays_n(const double *base, int len, double *out) {
-  __shared__ double red[NT][CTRL_WAVES];
-  double acc[NT];
+__device__ __forceinline__ void reduce_arrays_n(const double *base, int len, double *out, double (*red)[RED_THREADS],
+                                                double (*red2)[16]) {
+  const int tid = threadIdx.x;
+  if (tid < RED_THREADS) {
+    double acc[NT];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = 0.;
-  for (int i0 = 0; i0 < len; i0 += CTRL_THREADS * CTRL_RUN) {
-    double v[NT][CTRL_RUN];
+    for (int t = 0; t < NT; ++t) acc[t] = 0.;
+    for (int i0 = 0; i0 < len; i0 += RED_THREADS * CTRL_RUN) {
+      double v[NT][CTRL_RUN];
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
+      for (int t = 0; t < NT; ++t)
 #pragma unroll
-      for (int r = 0; r < CTRL_RUN; ++r) {
-        const int k = i0 + r * CTRL_THREADS + (int)threadIdx.x;
-        v[t][r] = base[(size_t)t * len + (k < len ? k : len - 1)];
-      }
+        for (int r = 0; r < CTRL_RUN; ++r) {
+          const int k = i0 + r * RED_THREADS + tid;
+          v[t][r] = base[(size_t)t * len + (k < len ? k : len - 1)];
+        }
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
+      for (int t = 0; t < NT; ++t)
 #pragma unroll
-      for (int r = 0; r < CTRL_RUN; ++r) {
-        const int k = i0 + r * CTRL_THREADS + (int)threadIdx.x;
-        acc[t] += (k < len) ? v[t][r] : 0.;
-      }
-  }
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = wave_sum(acc[t]);
-  if (lane == 0)
-#pragma unroll
-    for (int t = 0; t < NT; ++t) red[t][wid] = acc[t];
-  __syncthreads();
-  if (wid == 0) {
-    // lane = 16 g + w holds wave w's sum of array t = 4 h + g (h: the lane's pass over the arrays)
-#pragma unroll
-    for (int h = 0; h < (NT + 3) / 4; ++h) {
-      const int t = 4 * h + (lane >> 4), w = lane & 15;
-      double x = (t < NT) ? red[t < NT ? t : 0][w] : 0.;
-#pragma unroll
-      for (int off = 8; off > 0; off >>= 1) x += __shfl_down(x, off, 16);
-      if (w == 0 && t < NT) out[t] = x;
+        for (int r = 0; r < CTRL_RUN; ++r) {
+          const int k = i0 + r * RED_THREADS + tid;
+          acc[t] += (k < len) ? v[t][r] : 0.;
+        }
     }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) red[t][tid] = acc[t];
+  }
+  __syncthreads();
+  if (tid < NT * 16) {
+    const int t = tid >> 4, g = tid & 15;
+    double a = 0.;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) a += red[t][16 * g + j];
+    red2[t][g] = a;
+  }
+  __syncthreads();
+  if (tid < NT) {
+    double a = 0.;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) a += red2[tid][j];
+    out[tid] = a;
   }
   __syncthreads();
 }
 
 // nt (<= CTRL_MAXT) arrays: sweeps of at most 8 arrays
 __device__ __forceinline__ void reduce_arrays(const double *base, int nt, int len, double *out) {
+  __shared__ double red[8][RED_THREADS];
+  __shared__ double red2[8][16];
+  static_assert(CTRL_THREADS >= RED_THREADS, "the reduction's threads are part of the control workgroup");
   for (int t0 = 0; t0 < nt; t0 += 8) {
     const int m = nt - t0 < 8 ? nt - t0 : 8;
     const double *b = base + (size_t)t0 * len;
     switch (m) {
-      case 8: reduce_arrays_n<8>(b, len, out + t0); break;
-      case 7: reduce_arrays_n<7>(b, len, out + t0); break;
-      case 6: reduce_arrays_n<6>(b, len, out + t0); break;
-      case 5: reduce_arrays_n<5>(b, len, out + t0); break;
-      case 4: reduce_arrays_n<4>(b, len, out + t0); break;
-      case 3: reduce_arrays_n<3>(b, len, out + t0); break;
-      case 2: reduce_arrays_n<2>(b, len, out + t0); break;
-      default: reduce_arrays_n<1>(b, len, out + t0); break;
+      case 8: reduce_arrays_n<8>(b, len, out + t0, red, red2); break;
+      case 7: reduce_arrays_n<7>(b, len, out + t0, red, red2); break;
+      case 6: reduce_arrays_n<6>(b, len, out + t0, red, red2); break;
+      case 5: reduce_arrays_n<5>(b, len, out + t0, red, red2); break;
+      case 4: reduce_arrays_n<4>(b, len, out + t0, red, red2); break;
+      case 3: reduce_arrays_n<3>(b, len, out + t0, red, red2); break;
+      case 2: reduce_arrays_n<2>(b, len, out + t0, red, red2); break;
+      default: reduce_arrays_n<1>(b, len, out + t0, red, red2); break;
     }
   }
 }
@@ -167,7 +174,7 @@ __device__ __host__ __forceinline__ double ncc_value(double s1, double s2, doubl
 __device__ __forceinline__ double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // Plan the next energy pass of the Armijo machine: evaluations the sequential algorithm may ask for next.
-__device__ void armijo_plan(PairState &s, int spec) {
+__device__ __forceinline__ void armijo_plan(PairState &s, int spec) {
   double c[MAX_CAND];
   int k = 0;
   const double t = s.a_tau;
@@ -200,7 +207,7 @@ struct EvalRec {
   int ex;  // from an FP64-arithmetic pass
 };
 
-__device__ bool armijo_lookup(const PairState &s, double t, EvalRec &ev) {
+__device__ __forceinline__ bool armijo_lookup(const PairState &s, double t, EvalRec &ev) {
   for (int j = 0; j < s.ncand; ++j)
     if (s.cand[j] == t) {
       ev = {s.cand_E[j], s.cand_R[j], s.cand_D[j], s.cand_M[j], s.cand_V[j], s.cand_ex};
@@ -214,7 +221,7 @@ __device__ bool armijo_lookup(const PairState &s, double t, EvalRec &ev) {
   return false;
 }
 
-__device__ void armijo_consume(PairState &s, double t, const EvalRec &ev) {
+__device__ __forceinline__ void armijo_consume(PairState &s, double t, const EvalRec &ev) {
   s.last_tau[1] = s.last_tau[0];
   s.last_E[1] = s.last_E[0];
   s.last_R[1] = s.last_R[0];
@@ -236,13 +243,13 @@ __device__ void armijo_consume(PairState &s, double t, const EvalRec &ev) {
 // E - f <= 1/2 Df t, and f >= E (ArmijoSearch.h:148-190). With FP32 energy passes (tol > 0) a margin
 // within tol (|f| + |E|) of either test could go either way on rounding alone; then both values are
 // evaluated again in FP64 arithmetic at the stored positions before deciding.
-__device__ bool needs_exact(const PairState &s, double f, double t, const EvalRec &ev, double tol) {
+__device__ __forceinline__ bool needs_exact(const PairState &s, double f, double t, const EvalRec &ev, double tol) {
   if (tol <= 0 || (ev.ex && s.f_ex) || !isfinite(ev.E) || !isfinite(f)) return false;
   const double b = tol * (fabs(f) + fabs(ev.E));
   return fabs(f - ev.E) <= b || fabs((ev.E - f) - ARMIJO_SIGMA * s.a_Df * t) <= b;
 }
 // Next pass: FP64 arithmetic at step t and at 0 (the current position: f).
-__device__ void plan_exact(PairState &s, double t) {
+__device__ __forceinline__ void plan_exact(PairState &s, double t) {
   s.cand[0] = t;
   s.cand[1] = 0.0;
   s.ncand = 2;
@@ -251,7 +258,7 @@ __device__ void plan_exact(PairState &s, double t) {
   s.passes++;
 }
 
-__device__ void armijo_start(PairState &s, int dirmode, double Df, double oldTau) {
+__device__ __forceinline__ void armijo_start(PairState &s, int dirmode, double Df, double oldTau) {
   s.dirmode = dirmode;
   s.a_Df = Df;
   s.a_tau = clampd(oldTau, TAU_MIN, TAU_MAX);
@@ -261,7 +268,7 @@ __device__ void armijo_start(PairState &s, int dirmode, double Df, double oldTau
 }
 
 // End of an iteration (GradientDescentBase::applySingle L361-439).
-__device__ void finish_iteration(PairState &s, double t, IterTrace *trace, int trace_cap, int pair, int slot) {
+__device__ __forceinline__ void finish_iteration(PairState &s, double t, IterTrace *trace, int trace_cap, int pair, int slot) {
   s.step_tau[slot] = t;
   s.step_dir[slot] = s.dirmode;
   double En = s.energy, Rn = s.ereg_new;
@@ -314,7 +321,7 @@ __device__ void finish_iteration(PairState &s, double t, IterTrace *trace, int t
 
 // Replays getTimestepWidthWithArmijoLineSearch (ArmijoSearch.h:118-200) over the evaluations
 // available, including the fall-back to the smoothed gradient (gradientDescent.h:231-235).
-__device__ void armijo_run(PairState &s, int spec, double tol, IterTrace *trace, int trace_cap, int pair, int slot) {
+__device__ __forceinline__ void armijo_run(PairState &s, int spec, double tol, IterTrace *trace, int trace_cap, int pair, int slot) {
   const double f = s.energy;
   for (int guard = 0; guard < 256; ++guard) {
     double t = s.a_tau;
@@ -391,7 +398,7 @@ __device__ void armijo_run(PairState &s, int spec, double tol, IterTrace *trace,
 
 // Runs one control phase for one pair with the whole workgroup (the reductions need every thread)
 // on the workgroup's copy s of the pair state.
-__device__ void control_pair(int pair, PairState &s, const double *__restrict__ part_e,
+__device__ __forceinline__ void control_pair(int pair, PairState &s, const double *__restrict__ part_e,
                              const double *__restrict__ part_n, const CtrlArgs &a, int *__restrict__ flags,
                              IterTrace *__restrict__ trace) {
   __shared__ double sums[CTRL_MAXT];
