@@ -25,7 +25,7 @@ constexpr double ARMIJO_SIGMA = 0.5;
 // the fly from z and the previous direction, and the update stores it for the next iteration.
 enum CtrlPhase { PH_LEVEL_INIT = 0, PH_ARMIJO_INIT = 2, PH_ARMIJO_STEP = 3, PH_NCC_REF0 = 5, PH_NCC_REF1 = 6 };
 
-constexpr int CTRL_THREADS = 1024;  // threads of a control workgroup
+constexpr int CTRL_THREADS = 256;  // threads of a control workgroup (a small workgroup finds a free slot sooner)
 constexpr int CTRL_MAXT = 16;       // partial arrays one control sweep can reduce
 
 // Sums of NT (<= 8) consecutive partial arrays base[t * len .. t * len + len) in the control
