@@ -130,13 +130,13 @@ __device__ __forceinline__ float bilin_at(__amdgpu_buffer_rsrc_t T, unsigned vo,
 }
 
 // ================================================================================================
-// SSD energy pass for up to MAX_CAND step sizes (k_energy in FP32 arithmetic). Tiles of 64 x 16
-// elements; each thread takes a vertical run of EN32_RUN elements, reusing the upper node row of one
+// SSD energy pass for up to MAX_CAND step sizes (k_energy in FP32 arithmetic). Tiles of 64 x 4 RUN
+// elements; each thread takes a vertical run of RUN elements, reusing the upper node row of one
 // element as the lower row of the next.
 // ================================================================================================
 // FIDX: byte offsets of the gathers formed in FP32 arithmetic, exact while n^2 <= 2^24 (n <= 4096):
 // two fused multiply-adds and one conversion instead of two conversions and integer arithmetic.
-template <bool FIDX>
+template <bool FIDX, int RUN>
 __global__ void __launch_bounds__(256)
 k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const float *__restrict__ phi,
              const float *__restrict__ z, const float *__restrict__ dold, int n, double vol,
@@ -165,7 +165,7 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
   for (int k = 0; k < MAX_CAND; ++k) tau[k] = (k < nc) ? (float)st[pair].cand[k] : 0.f;  // powers of two: exact
 
   const int ex = (tile % ntx) * EN_TX + (threadIdx.x % EN_TX);
-  const int ey0 = (tile / ntx) * EN32_TY + (threadIdx.x / EN_TX) * EN32_RUN;
+  const int ey0 = (tile / ntx) * (4 * RUN) + (threadIdx.x / EN_TX) * RUN;
 
   double acc[NACC];
 #pragma unroll
@@ -187,7 +187,7 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
     };
     if (ey0 < n - 1) load_row(ey0, 0);
 #pragma unroll
-    for (int j = 0; j < EN32_RUN; ++j) {
+    for (int j = 0; j < RUN; ++j) {
       const int ey = ey0 + j;
       if (ey >= n - 1) break;  // uniform across a wave (a wave is one row of 64 elements)
       if (j > 0) {
@@ -281,6 +281,7 @@ k_energy_exact_f32(const float *__restrict__ R, const float *__restrict__ T, con
                    const PairState *st, double *__restrict__ part, int ntx, int nty, int npairs) {
   const int tiles = ntx * nty;
   const size_t nn = (size_t)n * n;
+  const int run = en32_run(n);
   for (int pair = 0; pair < npairs; ++pair) {
     if (!st[pair].exact || st[pair].ncand == 0) continue;  // uniform across the workgroup
     const float *Rp = R + pair * nn, *Tp = T + pair * nn;
@@ -288,12 +289,12 @@ k_energy_exact_f32(const float *__restrict__ R, const float *__restrict__ T, con
     const float *zx = z + pair * 2 * nn, *zy = zx + nn, *ox_ = dold + pair * 2 * nn, *oy_ = ox_ + nn;
     for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
       const int ex = (tile % ntx) * EN_TX + (threadIdx.x % EN_TX);
-      const int ey0 = (tile / ntx) * EN32_TY + (threadIdx.x / EN_TX) * EN32_RUN;
+      const int ey0 = (tile / ntx) * (4 * run) + (threadIdx.x / EN_TX) * run;
       double accS[MAX_CAND], accR[MAX_CAND];
 #pragma unroll
       for (int k = 0; k < MAX_CAND; ++k) accS[k] = accR[k] = 0.0;
       if (ex < n - 1)
-        for (int j = 0; j < EN32_RUN; ++j)
+        for (int j = 0; j < run; ++j)
           if (ey0 + j < n - 1)
             energy_elem<float>(Rp, Tp, px, py, zx, zy, ox_, oy_, n, h, width, st[pair], ex, ey0 + j, accS, accR);
       energy_write_partials(accS, accR, st[pair].ncand, part, pair, tile, tiles);

@@ -227,12 +227,12 @@ int convert(const void *in, int din, void *out, int dout, size_t cnt, hipStream_
 struct EnergyGeom {
   int ntx, nty, nblk;
 };
-// Tiles of an energy pass at grid size n: 64 x 4 elements, or 64 x 16 for the FP32 SSD pass
-// (k_energy_f32 and its FP64 re-evaluation).
+// Tiles of an energy pass at grid size n: 64 x 4 elements, or 64 x 4 run for the FP32 SSD pass
+// (k_energy_f32 and its FP64 re-evaluation; the run is en32_run(n)).
 EnergyGeom energy_geom(int n, bool f32_ssd = false) {
   EnergyGeom g;
   g.ntx = cdiv(n - 1, msg::EN_TX);
-  g.nty = cdiv(n - 1, f32_ssd ? msg::EN32_TY : msg::EN_TY);
+  g.nty = cdiv(n - 1, f32_ssd ? 4 * msg::en32_run(n) : msg::EN_TY);
   g.nblk = g.ntx * g.nty;
   return g;
 }
@@ -272,12 +272,15 @@ int launch_energy(const S *R, const S *T, const S *phi, const S *z, const S *dol
                        st, part, g.ntx, g.nty, npairs);
   else if constexpr (std::is_same<S, float>::value) {
     if (n > 16384) return fail(MSG_ERR_ARG, "FP32 fields: the SSD energy pass addresses grids up to 16384^2");
-    if (n <= 4096)
-      hipLaunchKernelGGL(msg::k_energy_f32<true>, dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, z, dold, n,
+    if (n <= 4096 && msg::en32_run(n) == 2)
+      hipLaunchKernelGGL((msg::k_energy_f32<true, 2>), dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, z, dold, n,
+                         h * h, st, part, g.ntx, g.nty, npairs);
+    else if (n <= 4096)
+      hipLaunchKernelGGL((msg::k_energy_f32<true, 4>), dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, z, dold, n,
                          h * h, st, part, g.ntx, g.nty, npairs);
     else
-      hipLaunchKernelGGL(msg::k_energy_f32<false>, dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, z, dold, n,
-                         h * h, st, part, g.ntx, g.nty, npairs);
+      hipLaunchKernelGGL((msg::k_energy_f32<false, 4>), dim3(g.nblk * npairs), dim3(256), 0, s, R, T, phi, z, dold,
+                         n, h * h, st, part, g.ntx, g.nty, npairs);
     if (exact_pass) {  // the pairs flagged for an FP64-arithmetic evaluation (none: a check per pair)
       HIPCHK(hipGetLastError());
       hipLaunchKernelGGL(msg::k_energy_exact_f32, dim3(std::min(g.nblk, 1024)), dim3(256), 0, s, R, T, phi, z, dold,
@@ -1518,14 +1521,16 @@ int energy_noalloc_impl(const void *R, const void *T, const void *phi, int n, in
   HIPCHK(hipGetLastError());
   if constexpr (std::is_same<S, float>::value) {
     if (n > 16384) return fail(MSG_ERR_ARG, "FP32 fields: the SSD energy pass addresses grids up to 16384^2");
-    if (n <= 4096)
-      hipLaunchKernelGGL(msg::k_energy_f32<true>, dim3(g.nblk * batch), dim3(256), 0, s, (const float *)R,
-                         (const float *)T, (const float *)phi, (const float *)phi, (const float *)phi, n, h * h, st,
-                         part, g.ntx, g.nty, batch);
+    const float *Rf = (const float *)R, *Tf = (const float *)T, *pf = (const float *)phi;
+    if (n <= 4096 && msg::en32_run(n) == 2)
+      hipLaunchKernelGGL((msg::k_energy_f32<true, 2>), dim3(g.nblk * batch), dim3(256), 0, s, Rf, Tf, pf, pf, pf, n,
+                         h * h, st, part, g.ntx, g.nty, batch);
+    else if (n <= 4096)
+      hipLaunchKernelGGL((msg::k_energy_f32<true, 4>), dim3(g.nblk * batch), dim3(256), 0, s, Rf, Tf, pf, pf, pf, n,
+                         h * h, st, part, g.ntx, g.nty, batch);
     else
-      hipLaunchKernelGGL(msg::k_energy_f32<false>, dim3(g.nblk * batch), dim3(256), 0, s, (const float *)R,
-                         (const float *)T, (const float *)phi, (const float *)phi, (const float *)phi, n, h * h, st,
-                         part, g.ntx, g.nty, batch);
+      hipLaunchKernelGGL((msg::k_energy_f32<false, 4>), dim3(g.nblk * batch), dim3(256), 0, s, Rf, Tf, pf, pf, pf, n,
+                         h * h, st, part, g.ntx, g.nty, batch);
   }
   else
     hipLaunchKernelGGL((msg::k_energy<S>), dim3(g.nblk * batch), dim3(256), 0, s, (const S *)R, (const S *)T,

@@ -576,10 +576,15 @@ __device__ __forceinline__ bool participates(const PairState &s) {
 // assembled element by element. Output: per block partial sums (ssd, reg) per candidate.
 // ================================================================================================
 constexpr int EN_TX = 64, EN_TY = 4;  // elements per workgroup tile (one per thread)
-// The FP32 SSD pass (k_energy_f32, and its FP64 re-evaluation) takes tiles of 64 x 16 elements, a
-// vertical run of 4 elements per thread: the workgroup's reduction and partial sums are shared by 4
-// elements, and a run reads 5 node rows instead of 8.
-constexpr int EN32_TY = 16, EN32_RUN = EN32_TY / 4;
+// The FP32 SSD pass (k_energy_f32, and its FP64 re-evaluation) takes tiles of 64 x 4 * run elements,
+// a vertical run of `run` elements per thread: the workgroup's reduction and partial sums are shared by
+// the run, and a run reads run + 1 node rows instead of 2 run. Runs of 4 from 1024^2 up; runs of 2
+// below, where a single pair would otherwise give too few workgroups to hide the gathers' latency
+// (one pair at 512^2: 8.5 us per pass instead of 11.2 us; at 1024^2 runs of 4 are faster, 21.3 us
+// against 22.6). The run depends on the grid size only, so a pair's partial sums do not depend on the
+// batch it sits in.
+constexpr int EN32_RUN_FROM_1024 = 4, EN32_RUN_BELOW = 2;
+__host__ __device__ constexpr int en32_run(int n) { return n >= 1024 ? EN32_RUN_FROM_1024 : EN32_RUN_BELOW; }
 
 // The terms of element (ex, ey) of one pair for each candidate step size, added to accS (SSD) and
 // accR (regulariser): FP64 arithmetic at the stored positions (step_at). Used by the FP64 pass and by
