@@ -48,7 +48,8 @@ def random_problem(N, seed, amp_px=3.0):
 DT = [(1, 1e-12), (0, 2e-6)]  # (dtype code, tolerance)
 
 
-@pytest.mark.parametrize("N", [4, 8, 32, 64, 128])
+# 1024: the FP32 pass's runs of 4 elements per thread (runs of 2 below)
+@pytest.mark.parametrize("N", [4, 8, 32, 64, 128, 1024])
 @pytest.mark.parametrize("code,tol", DT)
 def test_energy(gpu, N, code, tol):
     probs = [random_problem(N, s) for s in range(3)]
@@ -64,7 +65,7 @@ def test_energy(gpu, N, code, tol):
         assert abs(er[b] - eo) <= max(tol, 1e-12) * eo
 
 
-@pytest.mark.parametrize("N", [4, 8, 32, 64, 128])
+@pytest.mark.parametrize("N", [4, 8, 32, 64, 128, 1024])
 @pytest.mark.parametrize("code,tol", DT)
 def test_gradient(gpu, N, code, tol):
     probs = [random_problem(N, 10 + s) for s in range(2)]
@@ -77,7 +78,16 @@ def test_gradient(gpu, N, code, tol):
     for b in range(len(probs)):
         r, t, ph = host(R[b]), host(T[b]), host(phi[b])
         ref = O.ssd_force(r, t, ph) + lam * np.stack([O.stiff_apply(ph[0]), O.stiff_apply(ph[1])])
-        assert rel(out[b], ref) < max(tol, 1e-12) * 10
+        err = np.abs(out[b] - ref) / max(np.max(np.abs(ref)), 1e-300)
+        bad = err >= max(tol, 1e-12) * 10
+        if code == 0 and N >= 1024:
+            # FP32 quadrature points are exact to ~6e-8 px relative to their element; among the 8M
+            # of a 1024^2 pair a few land on the other side of a node than the FP64 oracle's, where
+            # the bilinear template's gradient jumps (these images are white noise). Those nodes,
+            # at most 1e-5 of them, may differ; every other node holds the bar.
+            assert bad.sum() <= 1e-5 * bad.size, (int(bad.sum()), float(err.max()))
+        else:
+            assert not bad.any(), float(err.max())
 
 
 @pytest.mark.parametrize("N", [4, 8, 32, 64, 128])
@@ -116,7 +126,8 @@ def test_ncc_golden(gpu):
         assert rel(de, g["de_ncc"]) < 1e-10
 
 
-@pytest.mark.parametrize("N", [2, 8, 32, 64, 128, 256])
+# 1024 and 2048: the FP32 V-cycle's 64 x 16 tiles (32 x 16 below)
+@pytest.mark.parametrize("N", [2, 8, 32, 64, 128, 256, 1024, 2048])
 @pytest.mark.parametrize("sigma", [1.0, 2.0 ** -5])
 @pytest.mark.parametrize("code,tol", DT)
 def test_h1_smooth(gpu, N, sigma, code, tol):
