@@ -146,7 +146,8 @@ int msg_energy(int dtype, const void *R, const void *T, const void *phi, int n, 
  * hipMalloc. Same kernels and summation order as msg_energy, so the results are bit-identical.
  * Its pair states and partial sums live in a fixed 32 MiB device array that each GPU holds until the
  * process ends; the call fails with MSG_ERR_ARG when batch x n^2 needs more (about
- * (8 batch - 6) ceil((n-1)/64) ceil((n-1)/4) doubles: e.g. 32 pairs of 2048^2). Every call on one
+ * (8 batch - 6) ceil((n-1)/64) ceil((n-1)/r) doubles, r = 4 for FP64 fields, 16 for FP32 fields of
+ * 1024^2 and above, 8 below: e.g. 32 pairs of 2048^2 in FP64, 128 in FP32). Every call on one
  * GPU shares that array, so calls that overlap on one GPU (several threads or streams) race;
  * serialise them. The call synchronises `stream` before returning. */
 int msg_energy_noalloc(int dtype, const void *R, const void *T, const void *phi, int n, int batch, double *e_ssd,
