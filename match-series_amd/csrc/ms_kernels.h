@@ -248,11 +248,12 @@ __device__ __forceinline__ bool needs_exact(const PairState &s, double f, double
   const double b = tol * (fabs(f) + fabs(ev.E));
   return fabs(f - ev.E) <= b || fabs((ev.E - f) - ARMIJO_SIGMA * s.a_Df * t) <= b;
 }
-// Next pass: FP64 arithmetic at step t and at 0 (the current position: f).
+// Next pass: FP64 arithmetic at step t and, unless f already comes from such a pass at the current
+// position (f_ex: consecutive near ties late in a level solve), at 0 (the current position: f).
 __device__ __forceinline__ void plan_exact(PairState &s, double t) {
   s.cand[0] = t;
   s.cand[1] = 0.0;
-  s.ncand = 2;
+  s.ncand = s.f_ex ? 1 : 2;
   s.exact = 1;
   s.searching = 1;
   s.passes++;
