@@ -230,3 +230,20 @@ def test_resample_rect(gpu):
             ref = z[f"out{k}"]
             src = torch.tensor(z[f"src{k}"], dtype=torch.float64, device="cuda")
             np.testing.assert_array_equal(gpu.resample_rect(src, ref.shape[0]).cpu().numpy(), ref)
+
+
+def test_energy_f32_8192(gpu):
+    """Grids above 4096^2: the FP32 pass forms its gather offsets in integer arithmetic (the FP32
+    form is exact only below 2^26 bytes). FP32 fields against FP64 fields holding the same values,
+    which the FP64 pass evaluates in the reference's arithmetic."""
+    N = 8192
+    g = torch.Generator(device="cuda")
+    g.manual_seed(8192)
+    h = 1.0 / (N - 1)
+    R = torch.rand((1, N, N), generator=g, device="cuda", dtype=torch.float32)
+    T = (torch.roll(R, 1, dims=2) + 0.05 * torch.rand((1, N, N), generator=g, device="cuda")).contiguous()
+    phi = ((torch.rand((1, 2, N, N), generator=g, device="cuda") - 0.5) * (6.0 * h)).float().contiguous()
+    es32, er32 = gpu.energy(R, T, phi)
+    es64, er64 = gpu.energy(R.double(), T.double(), phi.double())
+    assert abs(es32[0] - es64[0]) <= 2e-6 * abs(es64[0])
+    assert abs(er32[0] - er64[0]) <= 2e-6 * abs(er64[0])
