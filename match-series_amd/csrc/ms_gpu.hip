@@ -508,7 +508,8 @@ struct msg_ctx {
   double *part_e = nullptr, *part_n = nullptr;
   double *coord = nullptr;
   PairState *st = nullptr;
-  int *flags_d = nullptr, *flags_h = nullptr;
+  int *flags_d = nullptr, *flags_h = nullptr;  // flags_h: two snapshots of B flags (pipelined replays)
+  hipEvent_t flag_ev[2] = {nullptr, nullptr};   // recorded after each snapshot's copy
   msg::IterTrace *trace_d = nullptr;
   int trace_cap = 0;
   hipStream_t stream = nullptr;
@@ -866,20 +867,39 @@ int solve_level_dd(msg_ctx *c, int l, double lambda, int max_iter, double stop_e
 
   // Every pair needs at most max_iter iterations; a search that spills over costs extra launches
   // only. The bound below is far above any legal run and only guards against a logic error.
+  // Replays are issued one ahead while a pair can still need the next one (fewer than max_iter
+  // iterations replayed), so the stream does not wait for the host to read the flags between
+  // replays; each replay's flags are copied to their own snapshot. A replay that turns out not to
+  // be needed finds every pair inactive and changes nothing (its kernels return at once).
   const long long max_batches = (long long)(max_iter + 4) * 8 / K + 64;
-  for (long long batch = 0;; ++batch) {
-    if (batch > max_batches) return fail(MSG_ERR_STATE, "device-driven level solve did not terminate");
-    if (max_iter <= 0) break;
+  auto replay = [&](long long b) -> int {
     if (use_graph) {
       HIPCHK(hipGraphLaunch(exec, s));
     } else {
       for (int k = 0; k < K; ++k) CHK(iteration(k));
     }
-    HIPCHK(hipMemcpyAsync(c->flags_h, c->flags_d, sizeof(int) * B, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    bool busy = false;
-    for (int p = 0; p < B; ++p) busy |= (c->flags_h[p] & 3) != 0;
-    if (!busy) break;
+    HIPCHK(hipMemcpyAsync(c->flags_h + (b & 1) * B, c->flags_d, sizeof(int) * B, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(c->flag_ev[b & 1], s));
+    return MSG_OK;
+  };
+  if (max_iter > 0) {
+    CHK(replay(0));
+    long long issued = 1;
+    for (long long batch = 0;; ++batch) {
+      if (batch > max_batches) return fail(MSG_ERR_STATE, "device-driven level solve did not terminate");
+      if (issued == batch + 1 && (batch + 1) * K < max_iter) {
+        CHK(replay(batch + 1));
+        ++issued;
+      }
+      HIPCHK(hipEventSynchronize(c->flag_ev[batch & 1]));
+      bool busy = false;
+      for (int p = 0; p < B; ++p) busy |= (c->flags_h[(batch & 1) * B + p] & 3) != 0;
+      if (!busy) break;
+      if (issued == batch + 1) {
+        CHK(replay(batch + 1));
+        ++issued;
+      }
+    }
   }
 
   std::vector<PairState> hs(B);
@@ -1105,10 +1125,15 @@ int msg_ctx_create(msg_ctx **out, int max_level, int npairs, int dtype) {
     msg_ctx_destroy(c);
     return rc;
   }
-  if (hipHostMalloc((void **)&c->flags_h, sizeof(int) * B, hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc((void **)&c->flags_h, sizeof(int) * 2 * B, hipHostMallocDefault) != hipSuccess) {
     msg_ctx_destroy(c);
     return fail(MSG_ERR_ALLOC, "pinned allocation failed");
   }
+  for (auto &e : c->flag_ev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      msg_ctx_destroy(c);
+      return fail(MSG_ERR_HIP, "event creation failed");
+    }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     msg_ctx_destroy(c);
     return fail(MSG_ERR_HIP, "stream creation failed");
@@ -1135,6 +1160,8 @@ int msg_ctx_destroy(msg_ctx *c) {
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->trace_d) (void)hipFree(c->trace_d);
   if (c->flags_h) (void)hipHostFree(c->flags_h);
+  for (auto &e : c->flag_ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return MSG_OK;
