@@ -102,6 +102,26 @@ int scratch_get(int slot, size_t bytes, void **p) {
   return MSG_OK;
 }
 
+// Pinned snapshots of small device flags with an event per snapshot, per thread and device, for host
+// checks that run one batch of launches behind the stream (made on first use, kept).
+struct FlagSnap {
+  int *h = nullptr;  // 2 snapshots of up to 4 ints
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+thread_local FlagSnap g_flagsnap[16];
+
+int flag_snap(FlagSnap **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  FlagSnap &f = g_flagsnap[dev & 15];
+  if (!f.h) {
+    HIPCHK(hipHostMalloc((void **)&f.h, sizeof(int) * 8, hipHostMallocDefault));
+    for (auto &e : f.ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  *out = &f;
+  return MSG_OK;
+}
+
 // ------------------------------------------------------------------------------------------------
 // Kernel timing with HIP events on the stream each kernel is launched on (opt-in, per class mask).
 // ------------------------------------------------------------------------------------------------
@@ -1736,11 +1756,12 @@ int compose_impl(const void *phi1, const void *phi2, void *result, int n, hipStr
   HIPCHK(hipStreamSynchronize(s));
   const int nbu = std::max(1, cdiv(nu, 256));
   double *dbuf[2] = {(double *)d0, (double *)d1};
-  int hdone[2] = {0, 0};
-  for (int k = 0; k <= 1000;) {
-    HIPCHK(hipMemcpyAsync(hdone, done, sizeof(hdone), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (hdone[0] && hdone[1]) break;
+  // Blocks of 8 iterations, each followed by a snapshot of the done flags; the host checks a block's
+  // snapshot while the next block runs (iterations past convergence change nothing).
+  FlagSnap *fs = nullptr;
+  CHK(flag_snap(&fs));
+  int k = 0;
+  auto block = [&](int blk) -> int {
     for (int r = 0; r < 8 && k <= 1000; ++r, ++k) {
       const double *dp = dbuf[k > 0 ? (k - 1) & 1 : 0];
       double *dc = dbuf[k & 1];
@@ -1752,6 +1773,22 @@ int compose_impl(const void *phi1, const void *phi2, void *result, int n, hipStr
                          (double *)partb, nbu, (const msg::CgState *)sa, sb, (const int *)done, n);
     }
     HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(fs->h + 2 * (blk & 1), done, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(fs->ev[blk & 1], s));
+    return MSG_OK;
+  };
+  // the state after the start (k_cg_init may already have met the stopping rule)
+  HIPCHK(hipMemcpyAsync(fs->h + 2, done, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(fs->ev[1], s));
+  HIPCHK(hipEventSynchronize(fs->ev[1]));
+  if (!(fs->h[2] && fs->h[3])) {
+    CHK(block(0));
+    for (int blk = 0;; ++blk) {
+      const bool more = k <= 1000;
+      if (more) CHK(block(blk + 1));
+      HIPCHK(hipEventSynchronize(fs->ev[blk & 1]));
+      if ((fs->h[2 * (blk & 1)] && fs->h[2 * (blk & 1) + 1]) || !more) break;
+    }
   }
   hipLaunchKernelGGL((msg::k_cg_finish<S>), dim3(cdiv(2 * nn, 256)), b, 0, s, (const double *)x, (const double *)rhs,
                      (const unsigned char *)def, (const msg::CgState *)sb, (S *)result, n);
