@@ -84,7 +84,8 @@ typedef struct {
   int max_iter;           /* "maxGDIterations" (reference default 1000)                      */
   double stop_eps;        /* "stopEpsilon" (reference default 0)                             */
   int resample_prolong;   /* "resampleInsteadOfProlongateDeformation"                         */
-  int spec;               /* Armijo candidates per energy pass, 1..4 (0: default 3)          */
+  int spec;               /* Armijo candidates per energy pass, 1..4 (0: default 3; a pass   */
+                          /* never plans more than 3, so 4 acts as 3)                          */
 } msg_solve_params;
 
 /* Per-iteration record of a level solve (the content of energy_<LL>.txt plus step data). */
@@ -403,7 +404,7 @@ int msg_profile_enable(unsigned mask);
 int msg_profile_reset(void);
 int msg_profile_read(int cls, double *ms, long long *launches, double *bytes);
 /* Times the energy pass of a context's level solve (no counterpart in the reference; used by the
- * benchmark's roofline): every pair evaluates `ncand` step sizes (1, 1/2, ...) along the direction the
+ * benchmark's roofline): every pair evaluates `ncand` (1..3) step sizes (1, 1/2, ...) along the direction the
  * last solve at this level left in the context, `reps` launches back to back on the context's stream
  * between two HIP events. Outputs the mean launch time (ms) and the algorithmic bytes of one launch
  * (reference, template, deformation and direction: 6 fields per pair). Leaves the pair state unusable

@@ -130,7 +130,7 @@ __device__ __forceinline__ float bilin_at(__amdgpu_buffer_rsrc_t T, unsigned vo,
 }
 
 // ================================================================================================
-// SSD energy pass for up to MAX_CAND step sizes (k_energy in FP32 arithmetic). Tiles of 64 x 4 RUN
+// SSD energy pass for up to PLAN_CAND step sizes (k_energy in FP32 arithmetic). Tiles of 64 x 4 RUN
 // elements; each thread takes a vertical run of RUN elements, reusing the upper node row of one
 // element as the lower row of the next.
 // ================================================================================================
@@ -148,7 +148,10 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
   const int nc = st[pair].ncand;
   if (nc == 0 || st[pair].exact) return;  // uniform across the workgroup (k_energy_exact_f32 takes exact pairs)
   const int dm = st[pair].dirmode;
-  constexpr int NACC = MAX_CAND + 3;  // E_SSD per candidate; sums of f'Kf, f'Kd, d'Kd
+  // Passes evaluate at most PLAN_CAND step sizes (armijo_plan; 2 for an FP64 re-evaluation, 1 for a
+  // plain energy): candidate slots beyond it are never filled.
+  constexpr int NC = PLAN_CAND;
+  constexpr int NACC = NC + 3;  // E_SSD per candidate; sums of f'Kf, f'Kd, d'Kd
   __shared__ double red[4 * NACC];
 
   const size_t nn = (size_t)n * n;
@@ -160,9 +163,9 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
   const double beta = st[pair].beta;
   const bool use_dir = (dm != DIR_NONE);
   const float nm1 = (float)(n - 1);
-  float tau[MAX_CAND];
+  float tau[NC];
 #pragma unroll
-  for (int k = 0; k < MAX_CAND; ++k) tau[k] = (k < nc) ? (float)st[pair].cand[k] : 0.f;  // powers of two: exact
+  for (int k = 0; k < NC; ++k) tau[k] = (k < nc) ? (float)st[pair].cand[k] : 0.f;  // powers of two: exact
 
   const int ex = (tile % ntx) * EN_TX + (threadIdx.x % EN_TX);
   const int ey0 = (tile / ntx) * (4 * RUN) + (threadIdx.x / EN_TX) * RUN;
@@ -218,15 +221,15 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
         udx[q] *= nm1;
         udy[q] *= nm1;
       }
-      acc[MAX_CAND] += stiff_bi_f(f0, f0) + stiff_bi_f(f1, f1);
-      acc[MAX_CAND + 1] += stiff_bi_f(f0, d0) + stiff_bi_f(f1, d1);
-      acc[MAX_CAND + 2] += stiff_bi_f(d0, d0) + stiff_bi_f(d1, d1);
+      acc[NC] += stiff_bi_f(f0, f0) + stiff_bi_f(f1, f1);
+      acc[NC + 1] += stiff_bi_f(f0, d0) + stiff_bi_f(f1, d1);
+      acc[NC + 2] += stiff_bi_f(d0, d0) + stiff_bi_f(d1, d1);
       // the quad point clamped to [0, n-1] in grid units: lo = -e, hi = n-1-e relative to the element
       const float lox = -(float)ex, hix = (float)(n - 1 - ex), loy = -(float)ey, hiy = (float)(n - 1 - ey);
       const int gbase = ex + ey * n;
       const float base4 = 4.f * (float)gbase, pitch4 = 4.f * (float)n;
 #pragma unroll
-      for (int k = 0; k < MAX_CAND; ++k) {
+      for (int k = 0; k < NC; ++k) {
         if (k < nc) {
           float a = 0.f;
 #pragma unroll
@@ -260,12 +263,12 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
   block_sum_n<NACC>(acc, red);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < MAX_CAND; ++k)
+    for (int k = 0; k < NC; ++k)
       if (k < nc) {
         const double taud = st[pair].cand[k];
         part[((size_t)(pair * MAX_CAND + k) * 2 + 0) * nblk + tile] = acc[k];
         part[((size_t)(pair * MAX_CAND + k) * 2 + 1) * nblk + tile] =
-            0.5 * (acc[MAX_CAND] + taud * (2.0 * acc[MAX_CAND + 1] + taud * acc[MAX_CAND + 2]));
+            0.5 * (acc[NC] + taud * (2.0 * acc[NC + 1] + taud * acc[NC + 2]));
       }
   }
 }

@@ -1313,7 +1313,7 @@ int msg_time_energy_pass(msg_ctx *c, int level, int ncand, int reps, double *ms_
                          double *bytes_per_launch) {
   if (!c) return fail(MSG_ERR_ARG, "null context");
   if (level < 1 || level > c->L) return fail(MSG_ERR_ARG, "level out of range");
-  if (ncand < 1 || ncand > msg::MAX_CAND || reps < 1) return fail(MSG_ERR_ARG, "ncand must be 1..4 and reps >= 1");
+  if (ncand < 1 || ncand > msg::PLAN_CAND || reps < 1) return fail(MSG_ERR_ARG, "ncand must be 1..3 and reps >= 1");
   return DISPATCH(c, time_energy_pass, c, level, ncand, reps, ms_per_launch, bytes_per_launch);
 }
 

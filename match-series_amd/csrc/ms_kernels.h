@@ -173,9 +173,13 @@ __device__ __host__ __forceinline__ double ncc_value(double s1, double s2, doubl
 
 __device__ __forceinline__ double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// Plan the next energy pass of the Armijo machine: evaluations the sequential algorithm may ask for next.
+// Plan the next energy pass of the Armijo machine: evaluations the sequential algorithm may ask for next
+// (at most PLAN_CAND: the first step size with its double and half, or the next three doublings or
+// halvings).
+constexpr int PLAN_CAND = 3;
+static_assert(PLAN_CAND <= MAX_CAND, "a plan fits the pair state's candidate slots");
 __device__ __forceinline__ void armijo_plan(PairState &s, int spec) {
-  double c[MAX_CAND];
+  double c[PLAN_CAND];
   int k = 0;
   const double t = s.a_tau;
   if (s.a_phase == AP_FIRST) {
