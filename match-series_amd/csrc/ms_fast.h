@@ -467,11 +467,25 @@ k_mg_down_f32(const float *__restrict__ rhs, int negate, float *__restrict__ x2o
   const int bx = x0 - 2, by = y0 - 2;
   const float sg = negate ? -1.f : 1.f;
 
-  for (int i = threadIdx.x; i < HX * MG32_HY; i += 256) {
+  // every load of the halo tile issued before the first is used (clamped indices; cells outside
+  // the grid are not stored), so a workgroup waits for one memory round trip, not one per sweep
+  constexpr int NH = HX * MG32_HY, PER = (NH + 255) / 256;
+  float hv[PER];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int i = threadIdx.x + 256 * r;
     const int lx = i % HX, ly = i / HX;
     const int gx = bx + lx, gy = by + ly;
-    if (gx >= 0 && gy >= 0 && gx < n && gy < n) {
-      const float v = sg * rp[gx + gy * n];
+    const bool ok = i < NH && gx >= 0 && gy >= 0 && gx < n && gy < n;
+    hv[r] = rp[ok ? gx + gy * n : 0];
+  }
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int i = threadIdx.x + 256 * r;
+    const int lx = i % HX, ly = i / HX;
+    const int gx = bx + lx, gy = by + ly;
+    if (i < NH && gx >= 0 && gy >= 0 && gx < n && gy < n) {
+      const float v = sg * hv[r];
       sr[ly][lx] = v;
       s1[ly][lx] = v * idg;  // first Jacobi sweep from zero
     }
@@ -535,19 +549,42 @@ k_mg_up_f32(const float *__restrict__ x2in, const float *__restrict__ xc, const 
   // the previous direction at this thread's output nodes, loaded now so the load overlaps the sweeps
   constexpr int OUT_PER_THREAD = TX * MG32_TY / 256;
   float dprev[OUT_PER_THREAD];
+  if (nrm_part) {  // uniform: every load issued at once, from clamped indices
 #pragma unroll
-  for (int k = 0; k < OUT_PER_THREAD; ++k) {
-    const int i = threadIdx.x + 256 * k;
-    const int gx = x0 + i % TX, gy = y0 + i / TX;
-    dprev[k] = (nrm_part && gx < n && gy < n) ? dold[field * nn + gx + gy * n] : 0.f;
+    for (int k = 0; k < OUT_PER_THREAD; ++k) {
+      const int i = threadIdx.x + 256 * k;
+      const int gx = x0 + i % TX, gy = y0 + i / TX;
+      const float v = dold[field * nn + ((gx < n && gy < n) ? gx + gy * n : 0)];
+      dprev[k] = (gx < n && gy < n) ? v : 0.f;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < OUT_PER_THREAD; ++k) dprev[k] = 0.f;
   }
 
-  for (int i = threadIdx.x; i < HX * MG32_HY; i += 256) {
+  // every load of the halo tile issued before the first is used (clamped indices; cells outside
+  // the grid are not stored)
+  constexpr int NH = HX * MG32_HY, PER = (NH + 255) / 256;
+  float hx2[PER], hxc[PER], hr[PER];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int i = threadIdx.x + 256 * r;
     const int lx = i % HX, ly = i / HX;
     const int gx = bx + lx, gy = by + ly;
-    if (gx >= 0 && gy >= 0 && gx < n && gy < n) {
-      sx[ly][lx] = x2p[gx + gy * n] + xcp[(gx >> 1) + (gy >> 1) * nc];
-      sr[ly][lx] = sg * rp[gx + gy * n];
+    const bool ok = i < NH && gx >= 0 && gy >= 0 && gx < n && gy < n;
+    const int gi = ok ? gx + gy * n : 0, ci = ok ? (gx >> 1) + (gy >> 1) * nc : 0;
+    hx2[r] = x2p[gi];
+    hxc[r] = xcp[ci];
+    hr[r] = rp[gi];
+  }
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int i = threadIdx.x + 256 * r;
+    const int lx = i % HX, ly = i / HX;
+    const int gx = bx + lx, gy = by + ly;
+    if (i < NH && gx >= 0 && gy >= 0 && gx < n && gy < n) {
+      sx[ly][lx] = hx2[r] + hxc[r];
+      sr[ly][lx] = sg * hr[r];
     }
   }
   __syncthreads();
@@ -610,7 +647,17 @@ k_mg_coarse_f32(const float *__restrict__ rhs, int negate, float *__restrict__ x
   const size_t nn = (size_t)n * n;
   const float *rp = rhs + field * nn;
   const float sg = negate ? -1.f : 1.f;
-  for (int i = threadIdx.x; i < (int)nn; i += MGC_THREADS) B[MGC_OFF[L] + i] = sg * rp[i];
+  constexpr int PER = (1 << (2 * MGC_MAXL_F32)) / MGC_THREADS;  // nodes per thread at the largest level
+  {
+    float v[PER];  // every load issued before the first LDS store (indices clamped into the field)
+#pragma unroll
+    for (int k = 0; k < PER; ++k) v[k] = rp[min((int)threadIdx.x + MGC_THREADS * k, (int)nn - 1)];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int i = threadIdx.x + MGC_THREADS * k;
+      if (i < (int)nn) B[MGC_OFF[L] + i] = sg * v[k];
+    }
+  }
   __syncthreads();
 
   auto consts = [&](int l, float &hs, float &idg) {
@@ -686,15 +733,28 @@ k_mg_coarse_f32(const float *__restrict__ rhs, int negate, float *__restrict__ x
   for (int l = WAVE_LEVELS + 1; l <= L; ++l) up_level(l, (int)threadIdx.x, MGC_THREADS, block_sync);
   float *xo = xout + field * nn;
   double acc[NORM_TERMS] = {0., 0., 0., 0.};  // <z,z>, <DE,z>, <DE,d_old>, <DE,DE> (DE = the raw input)
-  for (int i = threadIdx.x; i < (int)nn; i += MGC_THREADS) {
-    const float o = X[MGC_OFF[L] + i];
-    xo[i] = o;
-    if (nrm_part) {
-      const double od = (double)o, de = (double)rp[i];
-      acc[0] += od * od;
-      acc[1] += de * od;
-      acc[2] += de * (double)dold[field * nn + i];
-      acc[3] += de * de;
+  float rv[PER], ov[PER];
+  if (nrm_part) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int i = min((int)threadIdx.x + MGC_THREADS * k, (int)nn - 1);
+      rv[k] = rp[i];
+      ov[k] = dold[field * nn + i];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = threadIdx.x + MGC_THREADS * k;
+    if (i < (int)nn) {
+      const float o = X[MGC_OFF[L] + i];
+      xo[i] = o;
+      if (nrm_part) {
+        const double od = (double)o, de = (double)rv[k];
+        acc[0] += od * od;
+        acc[1] += de * od;
+        acc[2] += de * (double)ov[k];
+        acc[3] += de * de;
+      }
     }
   }
   if (nrm_part) {

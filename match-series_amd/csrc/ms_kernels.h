@@ -137,6 +137,18 @@ __device__ __forceinline__ S search_dir(const S *__restrict__ z, const S *__rest
   const double od = use_old ? (double)dold[i] : 0.0;
   return static_cast<S>((double)zv + beta * od);
 }
+// The same from values already loaded (z_i, d_old_i); d_old_i is ignored unless use_old.
+template <typename S>
+__device__ __forceinline__ S search_dir_val(S zv, S ov, bool cg, bool use_old, double beta) {
+  if (!cg) return zv;
+  const double od = use_old ? (double)ov : 0.0;
+  return static_cast<S>((double)zv + beta * od);
+}
+template <>
+__device__ __forceinline__ float search_dir_val<float>(float zv, float ov, bool cg, bool use_old, double beta) {
+  const float dv = __fmaf_rn((float)beta, ov, zv);
+  return (cg && use_old) ? dv : zv;
+}
 // FP32 fields: one fused multiply-add in FP32 (the same expression in every kernel that forms d);
 // both loads unconditional and the choice a select, so the loads of neighbouring nodes overlap.
 template <>
@@ -1396,12 +1408,23 @@ k_update(S *__restrict__ phi, S *__restrict__ d, const S *__restrict__ z, int n,
   const S *zp = z + off;
   const int m = 2 * n * n;
   const int base = blk * 256 * UPD_PER_THREAD;
+  // every load first (indices clamped into the field), then the stores: the loads of all
+  // UPD_PER_THREAD nodes are in flight together instead of one node's at a time
+  S zv[UPD_PER_THREAD], ov[UPD_PER_THREAD], pv[UPD_PER_THREAD];
+#pragma unroll
+  for (int k = 0; k < UPD_PER_THREAD; ++k) {
+    const int i = min(base + k * 256 + (int)threadIdx.x, m - 1);
+    zv[k] = zp[i];
+    ov[k] = dp[i];
+    pv[k] = p[i];
+  }
+#pragma unroll
   for (int k = 0; k < UPD_PER_THREAD; ++k) {
     const int i = base + k * 256 + threadIdx.x;
     if (i < m) {
-      const S dv = search_dir<S>(zp, dp, i, cg, use_old, beta);
+      const S dv = search_dir_val<S>(zv[k], ov[k], cg, use_old, beta);
       if (cg) dp[i] = dv;
-      p[i] = step_at<S>(p[i], tau, dv);
+      p[i] = step_at<S>(pv[k], tau, dv);
     }
   }
 }
