@@ -89,17 +89,17 @@ __device__ __forceinline__ float bilin_f(__amdgpu_buffer_rsrc_t T, const HitF &p
   return p.ly * (b - a) + a;
 }
 
-// a^T K_loc b of the Q1 stiffness matrix in the difference form of stiff_form: the edge and diagonal
-// sums in FP32, their weights 1/6 and 2/6 applied in FP64 (FP32's 1/6 is 3e-8 too large, a bias
-// that every element would share)
-__device__ __forceinline__ double stiff_bi_f(const float a[4], const float b[4]) {
+// a^T K_loc b of the Q1 stiffness matrix in the difference form of stiff_form, as its edge and
+// diagonal sums e and d in FP32; the caller applies their weights 1/6 and 2/6 in FP64 (FP32's 1/6 is
+// 3e-8 too large, a bias that every element would share)
+__device__ __forceinline__ void stiff_parts_f(const float a[4], const float b[4], float &e, float &d) {
 #pragma clang fp contract(fast)
   const float ea0 = a[0] - a[1], ea1 = a[0] - a[2], ea2 = a[1] - a[3], ea3 = a[2] - a[3];
   const float da0 = a[0] - a[3], da1 = a[1] - a[2];
   const float eb0 = b[0] - b[1], eb1 = b[0] - b[2], eb2 = b[1] - b[3], eb3 = b[2] - b[3];
   const float db0 = b[0] - b[3], db1 = b[1] - b[2];
-  const float e = ea0 * eb0 + ea1 * eb1 + ea2 * eb2 + ea3 * eb3, d = da0 * db0 + da1 * db1;
-  return (double)e * (1.0 / 6.0) + (double)d * (2.0 / 6.0);
+  e = ea0 * eb0 + ea1 * eb1 + ea2 * eb2 + ea3 * eb3;
+  d = da0 * db0 + da1 * db1;
 }
 
 // One axis of a deformed quad point for the energy's value: the position e + u clamped to the grid,
@@ -175,6 +175,11 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
   for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
 
   if (ex < n - 1) {
+    // A run's sums in FP32 (a few terms each), converted once: per step size the SSD sum, and the
+    // e and d parts of f'Kf, f'Kd, d'Kd
+    float ssd[NC], se[3] = {0.f, 0.f, 0.f}, sd[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < NC; ++k) ssd[k] = 0.f;
     // node values of the element, b = x + 2 y: slots 0, 1 the lower row, 2, 3 the upper row
     float f0[4], f1[4], d0[4], d1[4], rn[4];
     auto load_row = [&](int y, int slot) {
@@ -221,9 +226,27 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
         udx[q] *= nm1;
         udy[q] *= nm1;
       }
-      acc[NC] += stiff_bi_f(f0, f0) + stiff_bi_f(f1, f1);
-      acc[NC + 1] += stiff_bi_f(f0, d0) + stiff_bi_f(f1, d1);
-      acc[NC + 2] += stiff_bi_f(d0, d0) + stiff_bi_f(d1, d1);
+      {
+        float e, d;
+        stiff_parts_f(f0, f0, e, d);
+        se[0] += e;
+        sd[0] += d;
+        stiff_parts_f(f1, f1, e, d);
+        se[0] += e;
+        sd[0] += d;
+        stiff_parts_f(f0, d0, e, d);
+        se[1] += e;
+        sd[1] += d;
+        stiff_parts_f(f1, d1, e, d);
+        se[1] += e;
+        sd[1] += d;
+        stiff_parts_f(d0, d0, e, d);
+        se[2] += e;
+        sd[2] += d;
+        stiff_parts_f(d1, d1, e, d);
+        se[2] += e;
+        sd[2] += d;
+      }
       // the quad point clamped to [0, n-1] in grid units: lo = -e, hi = n-1-e relative to the element
       const float lox = -(float)ex, hix = (float)(n - 1 - ex), loy = -(float)ey, hiy = (float)(n - 1 - ey);
       const int gbase = ex + ey * n;
@@ -254,10 +277,14 @@ k_energy_f32(const float *__restrict__ R, const float *__restrict__ T, const flo
             const float t = bilin_at(Trs, vo, n, lx, ly) - rq[q];
             a += t * t;
           }
-          acc[k] += (double)a * (0.125 * vol);  // 1/2 t^2 * quad weight 1/4 * vol
+          ssd[k] += a;
         }
       }
     }
+#pragma unroll
+    for (int k = 0; k < NC; ++k) acc[k] = (double)ssd[k] * (0.125 * vol);  // 1/2 t^2 * quad weight 1/4 * vol
+#pragma unroll
+    for (int m = 0; m < 3; ++m) acc[NC + m] = (double)se[m] * (1.0 / 6.0) + (double)sd[m] * (2.0 / 6.0);
   }
   const int nblk = tiles;
   block_sum_n<NACC>(acc, red);
